@@ -1,0 +1,194 @@
+"""Benchmark: batched direct + AO bake of the procedural city on one MI355X.
+
+One step = one full bake pass (direct light from the sun and the 16-vertex area light, then
+256 cosine AO rays per texel) over all 32 lightmaps of the city, through rl_bake_batch (one launch
+per pass). The plain path (one bake call per mesh per pass, the reference's per-mesh call shape)
+is timed beside it. Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, HERE)
+
+METRIC = "Mrays/s + bake p50/p99 latency, batched vs plain GPU at ≤1e-3 texel RMSE"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def load_traffic():
+    path = os.path.join(HERE, "tests", "golden", "city_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def load_pmc():
+    path = os.path.join(HERE, "profiles", "ao_count_city_pmc.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
+
+
+def percentile(x, q):
+    return float(np.percentile(np.asarray(x, np.float64), q))
+
+
+def cpu_baseline(spec, gpu_maps, stride, threads):
+    """Times the oracle on every `stride`-th valid lumel of every lightmap (direct + AO) and
+    returns (baseline dict, relative RMSE of the GPU's first pass on the same texels)."""
+    import oracle as O
+
+    S = O.OracleScene.from_spec(spec)
+    rays = 0
+    elapsed = 0.0
+    num = den = 0.0
+    lumels = 0
+    for li in range(len(spec.lightmaps)):
+        texel, pos, nrm = S.lumels(spec, li)
+        sel = slice(li % stride, None, stride)
+        t, p, n = texel[sel], pos[sel], nrm[sel]
+        col = np.zeros((t.size, 3), np.float32)
+        t0 = time.perf_counter()
+        rays += S.bake_direct(p, n, col, threads)
+        _, r = S.bake_ao(p, n, t, li, spec.ao, col, threads)
+        elapsed += time.perf_counter() - t0
+        rays += r
+        lumels += t.size
+        g = gpu_maps[li].reshape(-1, 3)[t].astype(np.float64)
+        num += float(((g - col) ** 2).sum())
+        den += float((col.astype(np.float64) ** 2).sum())
+    base = {"value": rays / elapsed / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"oracle direct+AO on every {stride}th valid lumel of all 32 lightmaps "
+                      f"({lumels} lumels, {rays} rays, {elapsed:.1f} s)"}
+    rmse = (num / den) ** 0.5 if den > 0 else 0.0
+    return base, rmse, lumels
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--cpu-stride", type=int, default=128)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--ao-samples", type=int, default=256)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the bake path "
+              "would shard lightmaps across GPUs with no exchange; this build runs on one).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from relight_amd import Relight, scenes
+
+    t0 = time.time()
+    spec = scenes.city(ao_samples=args.ao_samples)
+    log(f"city generated: {spec.num_tris} tris, {len(spec.lightmaps)} lightmaps ({time.time()-t0:.1f} s)")
+    relight = Relight()
+    t0 = time.time()
+    scene, lms, ids = scenes.build_gpu(relight, spec)
+    info = scene.info()
+    texels = sum(lm.valid_count() for lm in lms)
+    log(f"scene built: {info}, {texels} valid texels ({time.time()-t0:.1f} s)")
+
+    # Check pass (also the first warmup): one batched pass from zero, kept for the parity check.
+    relight.bake_batch(scene, lms, direct=True, ao=spec.ao)
+    first = [lm.to_hdr() for lm in lms]
+    log(f"check pass done: {relight.stats()['rays']} rays")
+    for _ in range(max(0, args.warmup - 1)):
+        relight.bake_batch(scene, lms, direct=True, ao=spec.ao)
+
+    lat, rays, ao_ms, ao_launch, ao_rays, gpu_ms = [], [], 0.0, 0, 0, []
+    for s in range(args.steps):
+        t = time.perf_counter()
+        relight.bake_batch(scene, lms, direct=True, ao=spec.ao)
+        lat.append((time.perf_counter() - t) * 1e3)
+        st = relight.stats()
+        rays.append(st["rays"])
+        gpu_ms.append(st["gpu_ms"])
+        k = st["kernels"].get("ao_count", {"ms": 0.0, "launches": 0})
+        ao_ms += k["ms"]
+        ao_launch += k["launches"]
+        ao_rays += texels * args.ao_samples
+        log(f"step {s}: {lat[-1]:.1f} ms, {st['rays']} rays, kernels {st['kernels']}")
+    total_ms = float(np.sum(lat))
+    value = float(np.sum(rays)) / (total_ms / 1e3) / 1e6
+
+    plain_lat, plain_rays = [], []
+    for s in range(args.plain_steps):
+        t = time.perf_counter()
+        r = 0
+        for mid in ids:
+            relight.bake_direct_light(scene, mid)
+            r += relight.stats()["rays"]
+        for mid in ids:
+            relight.bake_ambient_occlusion(scene, mid, spec.ao)
+            r += relight.stats()["rays"]
+        plain_lat.append((time.perf_counter() - t) * 1e3)
+        plain_rays.append(r)
+        log(f"plain step {s}: {plain_lat[-1]:.1f} ms, {r} rays")
+    plain_value = (float(np.sum(plain_rays)) / (float(np.sum(plain_lat)) / 1e3) / 1e6
+                   if plain_lat else None)
+
+    # Roofline of the dominant kernel (AO any-hit traversal).
+    traffic = load_traffic()
+    roofline = None
+    if ao_launch and traffic:
+        bpr = traffic["ao"]["bytes_per_ray"]
+        avg_s = ao_ms / ao_launch / 1e3
+        rays_per_launch = ao_rays / ao_launch
+        achieved = rays_per_launch * bpr / avg_s / 1e9
+        pmc = load_pmc()
+        roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBS,
+                    "traffic": pmc["hbm_bytes_per_launch"] if pmc else None,
+                    "kernel": "k_ao_count", "avg_launch_ms": avg_s * 1e3,
+                    "bytes_per_ray": bpr, "rays_per_launch": rays_per_launch}
+
+    cpu = None
+    rmse = None
+    if not args.no_cpu:
+        threads = max(1, min(16, os.cpu_count() or 1))
+        cpu, rmse, n = cpu_baseline(spec, first, args.cpu_stride, threads)
+        log(f"cpu baseline: {cpu}, rel RMSE on {n} texels: {rmse:.3e}")
+
+    out = {
+        "metric": METRIC, "value": value, "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": total_ms / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {
+            "workload": "city: 200x200 m ground + 5000 boxes (seed 7), "
+                        f"{spec.num_tris} tris, 32 lightmaps 256^2-2048^2, {texels} valid texels; "
+                        "sun + 2x2 m area light (16 vertices); AO "
+                        f"{args.ao_samples} cosine rays/texel, max 4 m",
+            "lightmaps": len(lms), "texels": texels, "triangles": spec.num_tris,
+            "bvh_nodes": info["nodes"], "path": "rl_bake_batch (direct + AO)"},
+        "latency_ms": {"p50": percentile(lat, 50), "p99": percentile(lat, 99)},
+        "texels_per_s": texels / (total_ms / args.steps / 1e3),
+        "rays_per_step": float(np.mean(rays)),
+        "gpu_ms_per_step": float(np.mean(gpu_ms)),
+        "plain": {"value": plain_value, "unit": "Mrays/s",
+                  "latency_ms": {"p50": percentile(plain_lat, 50), "p99": percentile(plain_lat, 99)}
+                  if plain_lat else None},
+        "batched_vs_plain": (value / plain_value) if plain_value else None,
+        "rmse_vs_oracle": rmse,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

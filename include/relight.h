@@ -1,0 +1,191 @@
+/*
+ * relight.h — C-ABI of the MI355X lightmap baker (librelight_amd.so).
+ *
+ * Drop-in boundary for the ray-cast hot path of dmsovetov/Relight: the public Relight / Scene /
+ * Lightmap API (src/relight/Relight.h:155-189, scene/Scene.h, Lightmap.h:58-150) and the tracer
+ * plugin seam rt::ITracer (rt/Tracer.h:75-108) that Scene::end hard-wires to Embree
+ * (scene/Scene.cpp:129). Every entry point is extern "C" with plain pointers and sizes.
+ *
+ * Ownership: every *_create returns a handle the caller owns and releases with *_destroy (the
+ * reference leaks its factories' objects; this ABI adds the destroy calls).
+ * Errors: int32 status codes mirroring RelightStatus (Relight.h:82-86) plus codes the GPU path
+ * needs; rl_last_error() returns a message for the calling thread's last failure.
+ * Threading: calls on one handle must not overlap; GPU work is ordered on the context's stream
+ * and every bake call returns after its work has finished.
+ */
+#ifndef RELIGHT_AMD_RELIGHT_H
+#define RELIGHT_AMD_RELIGHT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. The first three are RelightStatus (Relight.h:82-86). */
+#define RL_SUCCESS 0
+#define RL_INVALID_CALL 1
+#define RL_NOT_IMPLEMENTED 2
+#define RL_OUT_OF_MEMORY 3
+#define RL_REJECTED 4
+#define RL_DEVICE_ERROR 5
+#define RL_INVALID_ARGUMENT 6
+
+/* Light kinds: Light::createPointLight / createSpotLight / createDirectionalLight /
+ * createAreaLight (scene/Light.cpp:166-234). */
+#define RL_LIGHT_POINT 0
+#define RL_LIGHT_SPOT 1
+#define RL_LIGHT_DIRECTIONAL 2
+#define RL_LIGHT_AREA 3
+
+/* Hemisphere sampling for AO and final gather. RL_SAMPLE_UNIFORM is the reference's
+ * rejection-in-cube + flip (Types.h:924-949); RL_SAMPLE_COSINE is cosine-weighted (Malley). Both
+ * draw from per-(seed, lightmap seed, texel, sample) PCG32 streams instead of rand(). */
+#define RL_SAMPLE_UNIFORM 0
+#define RL_SAMPLE_COSINE 1
+
+/* Trace modes of rl_trace_segments. */
+#define RL_TRACE_ANY 0     /* occlusion: hit flag only */
+#define RL_TRACE_CLOSEST 1 /* closest hit: t, triangle, barycentrics */
+
+typedef struct rl_relight rl_relight;
+typedef struct rl_scene rl_scene;
+typedef struct rl_lightmap rl_lightmap;
+
+/* A light (scene/Light.h). Attenuation is LinearLightAttenuation (scene/Light.cpp:496-500) with
+ * radius and constant/linear/quadratic terms (defaults 0, 0, 25 at Light.h:125). has_attenuation=0
+ * disables it (the directional light has none). Area lights take their light mesh in local
+ * coordinates; one light vertex per face centroid is generated (FaceLightVertexGenerator with
+ * excludeVertices=true, maxSubdivisions=0, scene/Light.cpp:224,293-323) and offset by position. */
+typedef struct rl_light_desc {
+    int32_t type;
+    int32_t casts_shadow;
+    float position[3];
+    float direction[3];
+    float color[3];
+    float intensity;
+    int32_t has_attenuation;
+    float radius;
+    float att_constant;
+    float att_linear;
+    float att_quadratic;
+    float spot_cutoff;    /* RL_LIGHT_SPOT: cosine cutoff (LightSpotCutoff) */
+    float spot_exponent;  /* RL_LIGHT_SPOT: 1 in createSpotLight */
+    const float* area_positions; /* RL_LIGHT_AREA: xyz per vertex, local to position */
+    const uint32_t* area_indices;
+    uint32_t area_vertex_count;
+    uint32_t area_index_count;
+} rl_light_desc;
+
+/* AmbientOcclusionSettings (Relight.h:135-152) plus the sample-stream seed and sampling mode. */
+typedef struct rl_ao_settings {
+    int32_t samples;
+    float occluded_fraction;
+    float max_distance;
+    float exponent;
+    uint32_t seed;
+    int32_t sampling;
+} rl_ao_settings;
+
+/* Closest-hit record of rl_trace_segments: Embree's ray record fields the reference reads
+ * (rt/Embree.cpp:151-160). triangle = scene-order id (-1 on a miss); mesh/face locate it;
+ * (u, v) are Embree barycentrics: hit = (1-u-v)*v0 + u*v1 + v*v2. */
+typedef struct rl_hit {
+    float t;
+    int32_t triangle;
+    int32_t mesh;
+    int32_t face;
+    float u;
+    float v;
+} rl_hit;
+
+/* Kernel slots of rl_stats.kernel_ms. */
+#define RL_KERNEL_DIRECT 0
+#define RL_KERNEL_AO_COUNT 1
+#define RL_KERNEL_AO_FINALIZE 2
+#define RL_KERNEL_TRACE 3
+#define RL_KERNEL_SLOTS 8
+
+/* Counters of the last bake or trace call on a context. */
+typedef struct rl_stats {
+    uint64_t rays;          /* rays actually cast by the GPU */
+    uint64_t texels;        /* valid lumels processed */
+    double gpu_ms;          /* device time of the call, first to last kernel (HIP events) */
+    double wall_ms;         /* host wall time of the call */
+    int32_t launches;       /* kernel launches issued */
+    int32_t pad;
+    double kernel_ms[RL_KERNEL_SLOTS];        /* per-kernel device time (HIP events on the
+                                                 context's stream around each launch) */
+    int32_t kernel_launches[RL_KERNEL_SLOTS];
+} rl_stats;
+
+/* Last error message of the calling thread ("" if none). */
+const char* rl_last_error(void);
+
+/* Relight::create (Relight.h:183). Creates the context: device, stream, events. */
+int32_t rl_create(rl_relight** out);
+int32_t rl_destroy(rl_relight* ctx);
+
+/* Relight::createScene (Relight.h:165) and the Scene state machine begin -> add* -> end
+ * (scene/Scene.cpp:98-142). add_mesh copies and transforms the mesh like Scene::addMesh +
+ * Mesh::transformed (scene/Mesh.cpp:277-297): transform is 16 floats in the reference's
+ * Matrix4 layout (Types.h:1188-1196) or NULL; albedo is the material colour (rgba) or NULL for
+ * Face::colorAt's (1,1,1,1) default (scene/Mesh.cpp:478-485). Indices are 32-bit; meshes are not
+ * limited to the reference's 16-bit Index (Relight.h:59). end() builds the BVH and uploads the
+ * scene to HBM (the rtcCommit of rt/Embree.cpp:55-58). */
+int32_t rl_scene_create(rl_relight* ctx, rl_scene** out);
+int32_t rl_scene_destroy(rl_scene* scene);
+int32_t rl_scene_begin(rl_scene* scene);
+int32_t rl_scene_add_mesh(rl_scene* scene, const float* positions, const float* normals,
+                          const float* lightmap_uvs, uint32_t vertex_count,
+                          const uint32_t* indices, uint32_t index_count, const float* transform,
+                          const float* albedo, int32_t* out_mesh);
+int32_t rl_scene_add_light(rl_scene* scene, const rl_light_desc* light, int32_t* out_light);
+int32_t rl_scene_end(rl_scene* scene);
+/* Scene statistics after end(): triangle count, BVH node count, traversal stack bound. */
+int32_t rl_scene_info(const rl_scene* scene, uint32_t* triangles, uint32_t* nodes,
+                      int32_t* stack_bound);
+
+/* Relight::createLightmap (Relight.h:159) and Lightmap::addMesh (Lightmap.cpp:93-103), which
+ * rasterises the mesh's texels (Lightmap.cpp:133-164). A mesh may belong to one lightmap. The
+ * lightmap seed keys its texels' sample streams (default: creation order in the context). */
+int32_t rl_lightmap_create(rl_relight* ctx, int32_t width, int32_t height, rl_lightmap** out);
+int32_t rl_lightmap_destroy(rl_lightmap* lm);
+int32_t rl_lightmap_add_mesh(rl_lightmap* lm, rl_scene* scene, int32_t mesh);
+int32_t rl_lightmap_set_seed(rl_lightmap* lm, uint32_t seed);
+/* Number of valid lumels (texels covered by a face). */
+int32_t rl_lightmap_valid_count(const rl_lightmap* lm, uint32_t* count);
+/* Lumel setup of valid texels in texel order: texel index, position xyz, normal xyz. Any output
+ * may be NULL. */
+int32_t rl_lightmap_read_lumels(rl_lightmap* lm, uint32_t* texel, float* position, float* normal);
+/* Lightmap::toHdr (Lightmap.cpp:332-349): width*height*3 floats, invalid texels 0. */
+int32_t rl_lightmap_read_hdr(rl_lightmap* lm, float* out_rgb);
+
+/* Relight::bakeDirectLight (Relight.cpp:216-227) -> DirectLight::bakeLumel over every valid lumel
+ * of the mesh's lightmap (LumelBakeIterator, baker/Baker.cpp:52-77,176-185). One launch. */
+int32_t rl_bake_direct(rl_relight* ctx, rl_scene* scene, int32_t mesh);
+/* Relight::bakeAmbientOcclusion (Relight.cpp:260-271) -> AmbientOcclusion::bakeLumel. */
+int32_t rl_bake_ambient_occlusion(rl_relight* ctx, rl_scene* scene, int32_t mesh,
+                                  const rl_ao_settings* settings);
+/* Batched bake pass: direct light (if do_direct) then AO (if ao != NULL) over every valid lumel of
+ * num_maps lightmaps in one launch per pass. Results equal per-lightmap rl_bake_direct followed by
+ * rl_bake_ambient_occlusion. */
+int32_t rl_bake_batch(rl_relight* ctx, rl_scene* scene, rl_lightmap* const* maps,
+                      int32_t num_maps, int32_t do_direct, const rl_ao_settings* ao);
+
+/* rt::ITracer::traceSegment (rt/Tracer.h:87) for n segments start[i] -> end[i] (xyz each):
+ * direction normalize(end-start), interval (0.01, |end-start|) as Embree::initializeRay
+ * (rt/Embree.cpp:107-130). RL_TRACE_ANY writes hit flags (0/1) to out_flags; RL_TRACE_CLOSEST
+ * writes out_hits. Host buffers in, host buffers out. */
+int32_t rl_trace_segments(rl_relight* ctx, rl_scene* scene, const float* starts,
+                          const float* ends, uint32_t n, int32_t mode, uint8_t* out_flags,
+                          rl_hit* out_hits);
+
+/* Counters of the last bake/trace call on ctx. */
+int32_t rl_get_stats(const rl_relight* ctx, rl_stats* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RELIGHT_AMD_RELIGHT_H */

@@ -1,0 +1,224 @@
+"""ORACLE — test infrastructure only (never shipped, never measured as the product).
+
+ctypes wrapper of liboracle.so, the CPU restatement of the reference's bake math (oracle.cpp
+holds the file:line citations). Used by tests/, by the smoke check and by bench.py's CPU-baseline
+leg, only as the checker.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+class or_light(C.Structure):
+    _fields_ = [
+        ("type", C.c_int32), ("casts_shadow", C.c_int32),
+        ("position", C.c_float * 3), ("direction", C.c_float * 3), ("color", C.c_float * 3),
+        ("intensity", C.c_float), ("has_attenuation", C.c_int32),
+        ("radius", C.c_float), ("att_constant", C.c_float), ("att_linear", C.c_float),
+        ("att_quadratic", C.c_float), ("spot_cutoff", C.c_float), ("spot_exponent", C.c_float),
+        ("area_positions", C.POINTER(C.c_float)), ("area_indices", C.POINTER(C.c_uint32)),
+        ("area_vertex_count", C.c_uint32), ("area_index_count", C.c_uint32),
+    ]
+
+
+class or_ao(C.Structure):
+    _fields_ = [("samples", C.c_int32), ("occluded_fraction", C.c_float),
+                ("max_distance", C.c_float), ("exponent", C.c_float), ("seed", C.c_uint32),
+                ("sampling", C.c_int32)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "-B", "liboracle.so"], check=True)
+
+
+_lib = None
+_F = C.POINTER(C.c_float)
+_U32 = C.POINTER(C.c_uint32)
+_U8 = C.POINTER(C.c_uint8)
+_I32 = C.POINTER(C.c_int32)
+_U64 = C.POINTER(C.c_uint64)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        sig = {
+            "or_scene_create": (C.c_void_p, []),
+            "or_scene_destroy": (None, [C.c_void_p]),
+            "or_scene_add_mesh": (C.c_int32, [C.c_void_p, _F, _F, _F, C.c_uint32, _U32, C.c_uint32, _F, _F]),
+            "or_scene_add_light": (C.c_int32, [C.c_void_p, C.POINTER(or_light)]),
+            "or_scene_build": (C.c_int32, [C.c_void_p, C.c_int32]),
+            "or_raster": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _U8, _U32, _F, _F]),
+            "or_bake_direct": (C.c_int32, [C.c_void_p, C.c_uint32, _F, _F, _F, C.c_int32, _U64]),
+            "or_bake_ao": (C.c_int32, [C.c_void_p, C.c_uint32, _F, _F, _U32, C.c_uint32, C.POINTER(or_ao),
+                                       _F, _U32, C.c_int32, _U64]),
+            "or_trace": (C.c_int32, [C.c_void_p, C.c_uint32, _F, _F, C.c_int32, _U8, _F, _I32, _F, _F,
+                                     _U64, _U64, C.c_int32]),
+            "or_tri_info": (C.c_int32, [C.c_void_p, C.c_int32, _I32, _I32]),
+            "or_pcg32": (None, [C.c_uint64, C.c_uint64, C.c_uint32, _U32]),
+            "or_stream_u32": (None, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _U32]),
+            "or_hemisphere": (None, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, _F, _F]),
+            "or_splitmix64": (C.c_uint64, [C.c_uint64]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _f(a):
+    return None if a is None else a.ctypes.data_as(_F)
+
+
+def _u(a):
+    return None if a is None else a.ctypes.data_as(_U32)
+
+
+def pcg32(initstate, initseq, n):
+    out = np.zeros(n, np.uint32)
+    lib().or_pcg32(initstate, initseq, n, _u(out))
+    return out
+
+
+def stream_u32(seed, lm_seed, texel, tech, sample, n):
+    out = np.zeros(n, np.uint32)
+    lib().or_stream_u32(seed, lm_seed, texel, tech, sample, n, _u(out))
+    return out
+
+
+def hemisphere(seed, lm_seed, texel, tech, count, mode, normal):
+    out = np.zeros((count, 3), np.float32)
+    nrm = np.ascontiguousarray(normal, np.float32)
+    lib().or_hemisphere(seed, lm_seed, texel, tech, count, mode, _f(nrm), _f(out))
+    return out
+
+
+class OracleScene:
+    """The oracle's scene: same meshes and lights as the library, its own binary BVH."""
+
+    def __init__(self, use_bvh=True):
+        self._L = lib()
+        self._h = self._L.or_scene_create()
+        self._keep = []
+        self.use_bvh = use_bvh
+        self.meshes = 0
+
+    def __del__(self):
+        try:
+            if self._h:
+                self._L.or_scene_destroy(self._h)
+        except Exception:
+            pass
+
+    def add_mesh(self, positions, normals, uvs, indices, transform=None, albedo=None):
+        p = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+        n = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+        uv = np.ascontiguousarray(uvs, np.float32).reshape(-1, 2)
+        ix = np.ascontiguousarray(indices, np.uint32).reshape(-1)
+        tr = None if transform is None else np.ascontiguousarray(transform, np.float32).reshape(16)
+        al = None if albedo is None else np.ascontiguousarray(albedo, np.float32).reshape(4)
+        self._keep += [p, n, uv, ix, tr, al]
+        self.meshes += 1
+        return self._L.or_scene_add_mesh(self._h, _f(p), _f(n), _f(uv), p.shape[0], _u(ix), ix.size,
+                                         _f(tr), _f(al))
+
+    def add_light(self, L):
+        d = or_light()
+        d.type = L.type
+        d.casts_shadow = 1 if L.casts_shadow else 0
+        d.position[:] = [float(x) for x in L.position]
+        d.direction[:] = [float(x) for x in L.direction]
+        d.color[:] = [float(x) for x in L.color]
+        d.intensity = L.intensity
+        d.has_attenuation = 1 if L.has_attenuation else 0
+        d.radius, d.att_constant, d.att_linear, d.att_quadratic = (
+            L.radius, L.att_constant, L.att_linear, L.att_quadratic)
+        d.spot_cutoff, d.spot_exponent = L.spot_cutoff, L.spot_exponent
+        if L.area_positions is not None:
+            ap = np.ascontiguousarray(L.area_positions, np.float32)
+            ai = np.ascontiguousarray(L.area_indices, np.uint32)
+            self._keep += [ap, ai]
+            d.area_positions = _f(ap)
+            d.area_indices = _u(ai)
+            d.area_vertex_count = ap.shape[0]
+            d.area_index_count = ai.size
+        return self._L.or_scene_add_light(self._h, C.byref(d))
+
+    def build(self):
+        self._L.or_scene_build(self._h, 1 if self.use_bvh else 0)
+
+    def raster(self, mesh, w, h, grids=None):
+        """Lightmap::initializeLumels of one mesh into (valid, tri, pos, nrm) grids."""
+        if grids is None:
+            grids = (np.zeros(w * h, np.uint8), np.full(w * h, 0xFFFFFFFF, np.uint32),
+                     np.zeros((w * h, 3), np.float32), np.zeros((w * h, 3), np.float32))
+        valid, tri, pos, nrm = grids
+        self._L.or_raster(self._h, mesh, w, h, valid.ctypes.data_as(_U8), _u(tri), _f(pos), _f(nrm))
+        return grids
+
+    def bake_direct(self, pos, nrm, color, threads=0):
+        pos = np.ascontiguousarray(pos, np.float32)
+        nrm = np.ascontiguousarray(nrm, np.float32)
+        assert color.dtype == np.float32 and color.flags.c_contiguous
+        rays = C.c_uint64()
+        self._L.or_bake_direct(self._h, pos.shape[0], _f(pos), _f(nrm), _f(color), threads, C.byref(rays))
+        return rays.value
+
+    def bake_ao(self, pos, nrm, texel, lm_seed, ao, color, threads=0):
+        pos = np.ascontiguousarray(pos, np.float32)
+        nrm = np.ascontiguousarray(nrm, np.float32)
+        texel = np.ascontiguousarray(texel, np.uint32)
+        s = or_ao(int(ao.samples), float(ao.occluded_fraction), float(ao.max_distance), float(ao.exponent),
+                  int(ao.seed), int(ao.sampling))
+        occ = np.zeros(pos.shape[0], np.uint32)
+        rays = C.c_uint64()
+        self._L.or_bake_ao(self._h, pos.shape[0], _f(pos), _f(nrm), _u(texel), lm_seed, C.byref(s), _f(color),
+                           _u(occ), threads, C.byref(rays))
+        return occ, rays.value
+
+    def trace(self, starts, ends, closest=False, threads=0):
+        s = np.ascontiguousarray(starts, np.float32).reshape(-1, 3)
+        e = np.ascontiguousarray(ends, np.float32).reshape(-1, 3)
+        n = s.shape[0]
+        flags = np.zeros(n, np.uint8)
+        t = np.zeros(n, np.float32)
+        tri = np.zeros(n, np.int32)
+        u = np.zeros(n, np.float32)
+        v = np.zeros(n, np.float32)
+        nv, nt = C.c_uint64(), C.c_uint64()
+        self._L.or_trace(self._h, n, _f(s), _f(e), 1 if closest else 0, flags.ctypes.data_as(_U8), _f(t),
+                         tri.ctypes.data_as(_I32), _f(u), _f(v), C.byref(nv), C.byref(nt), threads)
+        return {"flags": flags, "t": t, "triangle": tri, "u": u, "v": v,
+                "node_visits": nv.value, "tri_tests": nt.value}
+
+    @classmethod
+    def from_spec(cls, spec, use_bvh=True):
+        S = cls(use_bvh)
+        for m in spec.meshes:
+            S.add_mesh(m.positions, m.normals, m.uvs, m.indices, albedo=m.albedo)
+        for L in spec.lights:
+            S.add_light(L)
+        S.build()
+        return S
+
+    def lumels(self, spec, lightmap):
+        """Valid lumels of one lightmap in texel order: texel, pos, nrm (rasterising its meshes
+        in mesh order, as Lightmap::addMesh calls would)."""
+        w, h = spec.lightmaps[lightmap]
+        grids = None
+        for mi, m in enumerate(spec.meshes):
+            if m.lightmap == lightmap:
+                grids = self.raster(mi, w, h, grids)
+        valid, tri, pos, nrm = grids
+        texel = np.nonzero(valid)[0].astype(np.uint32)
+        return texel, pos[texel].copy(), nrm[texel].copy()

@@ -1,0 +1,740 @@
+// ORACLE — test infrastructure only. A CPU restatement of dmsovetov/Relight's bake math used to
+// check the GPU library; nothing in the product links or calls it. Only tests/, the smoke check
+// and bench.py's CPU-baseline leg load liboracle.so.
+//
+// What it restates (file:line in the reference):
+//   segment -> ray            rt/Embree.cpp:107-130 (dir = normalize(end-start), (0.01, len))
+//   hit semantics             rt/Embree.cpp:145-177 (closest hit; Embree u,v)
+//   lumel raster              Lightmap.cpp:118-164, scene/Mesh.cpp:397-476 (Face::isUvInside etc.)
+//   direct light              baker/DirectLight.cpp:46-125, scene/Light.cpp:372-500
+//   area light vertices       scene/Light.cpp:293-323, scene/Mesh.cpp:322-327 (centroids)
+//   ambient occlusion         baker/AmbientOcclusion.cpp:46-66
+//   hemisphere sampling       Types.h:924-949 (uniform, with PCG32 instead of rand())
+// Project-defined parts shared by contract with the GPU path (not in the reference): PCG32 streams
+// keyed by (seed, lightmap seed, texel, technique, sample); cosine sampling (Malley + Duff et al.
+// 2017 basis); the watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2013); closest-hit ties
+// broken by the smaller scene-order triangle id.
+//
+// Pinning (tests/test_oracle.py): PCG32 against the published pcg32-demo output and an
+// independent Python integer model; sampler moments; ray/triangle known answers and shared-edge
+// watertightness; BVH == brute force; direct light closed forms; AO closed-box and open-plane
+// cases. Compiled with -ffp-contract=off so every expression rounds as written.
+#include <stdint.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cfloat>
+#include <thread>
+#include <vector>
+
+namespace {
+
+// ------------------------------------------------------------------ vectors
+struct Vec {
+    float x, y, z;
+};
+inline Vec mk(float x, float y, float z) { Vec v = {x, y, z}; return v; }
+inline Vec add(Vec a, Vec b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline Vec sub(Vec a, Vec b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline Vec scale(Vec a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+inline Vec divs(Vec a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
+inline Vec neg(Vec a) { return mk(-a.x, -a.y, -a.z); }
+inline float dot3(Vec a, Vec b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline float len3(Vec a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
+// Vec3::normalize, Types.h:852-859.
+inline float normalize3(Vec& a) {
+    float l = len3(a);
+    if (l) { a.x /= l; a.y /= l; a.z /= l; }
+    return l;
+}
+inline float axis(Vec a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+
+// ------------------------------------------------------------------ PCG32 (O'Neill 2014)
+struct Rng {
+    uint64_t s, inc;
+};
+uint32_t rng_u32(Rng& r) {
+    const uint64_t prev = r.s;
+    r.s = prev * 6364136223846793005ULL + r.inc;
+    const uint32_t x = (uint32_t)(((prev >> 18u) ^ prev) >> 27u);
+    const uint32_t k = (uint32_t)(prev >> 59u);
+    return (x >> k) | (x << ((0u - k) & 31u));
+}
+Rng rng_seed(uint64_t initstate, uint64_t initseq) {
+    Rng r;
+    r.s = 0;
+    r.inc = (initseq << 1u) | 1u;
+    rng_u32(r);
+    r.s += initstate;
+    rng_u32(r);
+    return r;
+}
+float rng_f(Rng& r) { return (float)(rng_u32(r) >> 8) * (1.0f / 16777216.0f); }
+uint64_t mix64(uint64_t z) {
+    z += 0x9e3779b97f4a7c15ULL;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+Rng stream(uint32_t seed, uint32_t lm_seed, uint32_t texel, uint32_t tech, uint32_t sample) {
+    uint64_t k = mix64(((uint64_t)seed << 32) | lm_seed);
+    k = mix64(k ^ (uint64_t)texel);
+    return rng_seed(k, ((uint64_t)tech << 32) | sample);
+}
+
+// Types.h:924-949 with PCG draws (zero-length draws rejected).
+Vec hemi_uniform(Rng& r, Vec n) {
+    Vec d;
+    float l;
+    for (;;) {
+        d.x = rng_f(r) * 2.0f - 1.0f;
+        d.y = rng_f(r) * 2.0f - 1.0f;
+        d.z = rng_f(r) * 2.0f - 1.0f;
+        l = len3(d);
+        if (!(l > 1.0f || l == 0.0f)) break;
+    }
+    d = divs(d, l);
+    if (dot3(d, n) < 0.0f) d = neg(d);
+    return d;
+}
+
+// Cosine-weighted: Malley's method on a rejection-sampled disk, Duff et al. 2017 basis.
+Vec hemi_cosine(Rng& r, Vec n) {
+    float x, y, q;
+    for (;;) {
+        x = rng_f(r) * 2.0f - 1.0f;
+        y = rng_f(r) * 2.0f - 1.0f;
+        q = x * x + y * y;
+        if (q < 1.0f) break;
+    }
+    const float z = sqrtf(1.0f - q);
+    const float sg = n.z >= 0.0f ? 1.0f : -1.0f;
+    const float a = -1.0f / (sg + n.z);
+    const float b = n.x * n.y * a;
+    const Vec e1 = mk(1.0f + sg * n.x * n.x * a, sg * b, -sg * n.x);
+    const Vec e2 = mk(b, sg + n.y * n.y * a, -n.y);
+    return mk(e1.x * x + e2.x * y + n.x * z, e1.y * x + e2.y * y + n.y * z, e1.z * x + e2.z * y + n.z * z);
+}
+
+Vec hemi(Rng& r, Vec n, int mode) { return mode == 1 ? hemi_cosine(r, n) : hemi_uniform(r, n); }
+
+// ------------------------------------------------------------------ rays and triangles
+struct Seg {
+    Vec o, d;
+    float t0, t1;
+    int kx, ky, kz;
+    float sx, sy, sz;
+    Vec inv;
+};
+
+Seg make_seg(Vec start, Vec end) {
+    Seg s;
+    s.o = start;
+    s.d = sub(end, start);
+    s.t1 = normalize3(s.d);
+    s.t0 = 0.01f;
+    const float ax = fabsf(s.d.x), ay = fabsf(s.d.y), az = fabsf(s.d.z);
+    int kz;
+    if (ax > ay) kz = ax > az ? 0 : 2;
+    else kz = ay > az ? 1 : 2;
+    int kx = (kz + 1) % 3, ky = (kz + 2) % 3;
+    if (axis(s.d, kz) < 0.0f) std::swap(kx, ky);
+    s.kx = kx; s.ky = ky; s.kz = kz;
+    s.sx = axis(s.d, kx) / axis(s.d, kz);
+    s.sy = axis(s.d, ky) / axis(s.d, kz);
+    s.sz = 1.0f / axis(s.d, kz);
+    s.inv = mk(1.0f / s.d.x, 1.0f / s.d.y, 1.0f / s.d.z);
+    return s;
+}
+
+// Woop-Benthin-Wald watertight test; on success t in (t0, t1) and the Embree barycentrics.
+bool tri_hit(const Seg& s, const float* v, float& t, float& bu, float& bv) {
+    const Vec A = sub(mk(v[0], v[1], v[2]), s.o);
+    const Vec B = sub(mk(v[3], v[4], v[5]), s.o);
+    const Vec C = sub(mk(v[6], v[7], v[8]), s.o);
+    const float az_ = axis(A, s.kz), bz_ = axis(B, s.kz), cz_ = axis(C, s.kz);
+    const float Ax = axis(A, s.kx) - s.sx * az_, Ay = axis(A, s.ky) - s.sy * az_;
+    const float Bx = axis(B, s.kx) - s.sx * bz_, By = axis(B, s.ky) - s.sy * bz_;
+    const float Cx = axis(C, s.kx) - s.sx * cz_, Cy = axis(C, s.ky) - s.sy * cz_;
+    float U = Cx * By - Cy * Bx;
+    float V = Ax * Cy - Ay * Cx;
+    float W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    const bool anyneg = U < 0.0f || V < 0.0f || W < 0.0f;
+    const bool anypos = U > 0.0f || V > 0.0f || W > 0.0f;
+    if (anyneg && anypos) return false;
+    const float det = U + V + W;
+    if (det == 0.0f) return false;
+    const float T = U * (s.sz * az_) + V * (s.sz * bz_) + W * (s.sz * cz_);
+    t = T / det;
+    if (!(t > s.t0 && t < s.t1)) return false;
+    bu = V / det;
+    bv = W / det;
+    return true;
+}
+
+// ------------------------------------------------------------------ scene + binary BVH
+struct Light {
+    int32_t type, shadow;
+    Vec pos, dir, color;
+    float intensity;
+    int32_t has_att;
+    float radius, c, l, q, cutoff, sexp;
+    std::vector<Vec> verts;
+};
+
+struct BNode {  // binned-SAH binary node: box, then children or a leaf range
+    float lo[3], hi[3];
+    int32_t left, right;
+    uint32_t first, count;
+};
+
+struct Scene {
+    std::vector<float> tri;       // 9 floats per scene-order triangle
+    std::vector<uint32_t> tmesh, tface;
+    std::vector<float> tuv;       // 6 per triangle (lightmap uv of a, b, c)
+    std::vector<float> tnrm;      // 9 per triangle
+    std::vector<float> talb;      // 4 per triangle
+    // per mesh
+    std::vector<std::vector<Vec>> mpos, mnrm;
+    std::vector<std::vector<float>> muv;
+    std::vector<std::vector<uint32_t>> midx;
+    std::vector<uint32_t> mfirst;
+    std::vector<Light> lights;
+    std::vector<BNode> nodes;
+    std::vector<uint32_t> order;
+    bool use_bvh = true;
+};
+
+struct Counters {
+    uint64_t nodes = 0, tris = 0;
+};
+
+void build_bvh(Scene& S) {
+    const uint32_t n = (uint32_t)(S.tri.size() / 9);
+    S.order.resize(n);
+    std::vector<float> bl(3 * n), bh(3 * n), ce(3 * n);
+    for (uint32_t i = 0; i < n; i++) {
+        S.order[i] = i;
+        for (int k = 0; k < 3; k++) {
+            const float a = S.tri[9 * i + k], b = S.tri[9 * i + 3 + k], c = S.tri[9 * i + 6 + k];
+            bl[3 * i + k] = std::min(a, std::min(b, c));
+            bh[3 * i + k] = std::max(a, std::max(b, c));
+            ce[3 * i + k] = 0.5f * (bl[3 * i + k] + bh[3 * i + k]);
+        }
+    }
+    S.nodes.clear();
+    S.nodes.push_back(BNode());
+    struct W { int32_t node; uint32_t first, count; };
+    std::vector<W> work;
+    work.push_back({0, 0, n});
+    auto area = [](const float* lo, const float* hi) {
+        double dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+        if (dx < 0) return 0.0;
+        return 2.0 * (dx * dy + dy * dz + dz * dx);
+    };
+    while (!work.empty()) {
+        W w = work.back();
+        work.pop_back();
+        BNode nd;
+        float clo[3], chi[3];
+        for (int k = 0; k < 3; k++) { nd.lo[k] = clo[k] = FLT_MAX; nd.hi[k] = chi[k] = -FLT_MAX; }
+        for (uint32_t i = w.first; i < w.first + w.count; i++) {
+            const uint32_t t = S.order[i];
+            for (int k = 0; k < 3; k++) {
+                nd.lo[k] = std::min(nd.lo[k], bl[3 * t + k]);
+                nd.hi[k] = std::max(nd.hi[k], bh[3 * t + k]);
+                clo[k] = std::min(clo[k], ce[3 * t + k]);
+                chi[k] = std::max(chi[k], ce[3 * t + k]);
+            }
+        }
+        nd.left = nd.right = -1;
+        nd.first = w.first;
+        nd.count = w.count;
+        uint32_t mid = 0;
+        bool do_split = false;
+        if (w.count > 4) {
+            int ax = 0;
+            for (int k = 1; k < 3; k++) if (chi[k] - clo[k] > chi[ax] - clo[ax]) ax = k;
+            const float ext = chi[ax] - clo[ax];
+            uint32_t* ord = &S.order[w.first];
+            if (ext > 0.0f) {
+                const int NB = 16;
+                float blo[NB][3], bhi[NB][3];
+                uint32_t cnt[NB] = {0};
+                for (int b = 0; b < NB; b++) for (int k = 0; k < 3; k++) { blo[b][k] = FLT_MAX; bhi[b][k] = -FLT_MAX; }
+                auto binof = [&](uint32_t t) { int b = (int)((ce[3 * t + ax] - clo[ax]) * (NB / ext)); return b < 0 ? 0 : (b >= NB ? NB - 1 : b); };
+                for (uint32_t i = 0; i < w.count; i++) {
+                    const uint32_t t = ord[i];
+                    const int b = binof(t);
+                    cnt[b]++;
+                    for (int k = 0; k < 3; k++) { blo[b][k] = std::min(blo[b][k], bl[3 * t + k]); bhi[b][k] = std::max(bhi[b][k], bh[3 * t + k]); }
+                }
+                double best = DBL_MAX;
+                int bestb = -1;
+                for (int s = 1; s < NB; s++) {
+                    float l0[3], l1[3], r0[3], r1[3];
+                    uint32_t nl = 0, nr = 0;
+                    for (int k = 0; k < 3; k++) { l0[k] = r0[k] = FLT_MAX; l1[k] = r1[k] = -FLT_MAX; }
+                    for (int b = 0; b < NB; b++) {
+                        if (!cnt[b]) continue;
+                        float* L0 = b < s ? l0 : r0; float* L1 = b < s ? l1 : r1;
+                        for (int k = 0; k < 3; k++) { L0[k] = std::min(L0[k], blo[b][k]); L1[k] = std::max(L1[k], bhi[b][k]); }
+                        (b < s ? nl : nr) += cnt[b];
+                    }
+                    if (!nl || !nr) continue;
+                    const double c = area(l0, l1) * nl + area(r0, r1) * nr;
+                    if (c < best) { best = c; bestb = s; }
+                }
+                if (bestb > 0) {
+                    uint32_t* p = std::partition(ord, ord + w.count, [&](uint32_t t) { return binof(t) < bestb; });
+                    mid = w.first + (uint32_t)(p - ord);
+                    do_split = mid > w.first && mid < w.first + w.count;
+                }
+            }
+            if (!do_split) {
+                std::nth_element(ord, ord + w.count / 2, ord + w.count, [&](uint32_t a, uint32_t b) { return ce[3 * a + ax] < ce[3 * b + ax]; });
+                mid = w.first + w.count / 2;
+                do_split = true;
+            }
+        }
+        if (do_split) {
+            nd.left = (int32_t)S.nodes.size();
+            nd.right = nd.left + 1;
+            S.nodes.push_back(BNode());
+            S.nodes.push_back(BNode());
+            work.push_back({nd.left, w.first, mid - w.first});
+            work.push_back({nd.right, mid, w.first + w.count - mid});
+        }
+        S.nodes[w.node] = nd;
+    }
+}
+
+// Robust slab test (Ize 2013): exit distances scaled by 1 + 2*gamma(3).
+inline bool box_hit(const Seg& s, const float* lo, const float* hi, float tlim, float& tenter) {
+    const float K = 1.0000004f;
+    float t0x = (lo[0] - s.o.x) * s.inv.x, t1x = (hi[0] - s.o.x) * s.inv.x;
+    float t0y = (lo[1] - s.o.y) * s.inv.y, t1y = (hi[1] - s.o.y) * s.inv.y;
+    float t0z = (lo[2] - s.o.z) * s.inv.z, t1z = (hi[2] - s.o.z) * s.inv.z;
+    float tmin = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), s.t0));
+    float tmax = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * K;
+    tenter = tmin;
+    return tmin <= fminf(tmax, tlim * K);
+}
+
+struct HitRec {
+    float t = 0.0f, u = 0.0f, v = 0.0f;
+    int32_t tri = -1;
+};
+
+// Closest hit (any=false) or any hit. Ties at equal t go to the smaller scene-order id.
+bool trace(const Scene& S, const Seg& s, bool any, HitRec& h, Counters* ctr) {
+    h = HitRec();
+    float best = s.t1;
+    auto test_tri = [&](uint32_t id) -> bool {
+        float t, bu, bv;
+        if (ctr) ctr->tris++;
+        if (!tri_hit(s, &S.tri[9 * (size_t)id], t, bu, bv)) return false;
+        if (h.tri < 0 || t < best || (t == best && (int32_t)id < h.tri)) {
+            best = t; h.t = t; h.u = bu; h.v = bv; h.tri = (int32_t)id;
+        }
+        return true;
+    };
+    if (!S.use_bvh) {
+        const uint32_t n = (uint32_t)(S.tri.size() / 9);
+        for (uint32_t i = 0; i < n; i++)
+            if (test_tri(i) && any) return true;
+        return h.tri >= 0;
+    }
+    // Binary traversal, each step fetches one node and tests both children's boxes.
+    int32_t stack[128];
+    int sp = 0;
+    const BNode& root = S.nodes[0];
+    float te;
+    if (ctr) ctr->nodes++;
+    if (!box_hit(s, root.lo, root.hi, best, te)) return false;
+    int32_t cur = 0;
+    for (;;) {
+        const BNode& nd = S.nodes[cur];
+        if (nd.left < 0) {
+            for (uint32_t i = nd.first; i < nd.first + nd.count; i++)
+                if (test_tri(S.order[i]) && any) return true;
+            if (!sp) break;
+            cur = stack[--sp];
+            continue;
+        }
+        if (ctr) ctr->nodes++;
+        float ta, tb;
+        const bool ha = box_hit(s, S.nodes[nd.left].lo, S.nodes[nd.left].hi, best, ta);
+        const bool hb = box_hit(s, S.nodes[nd.right].lo, S.nodes[nd.right].hi, best, tb);
+        if (ha && hb) {
+            int32_t nearc = ta <= tb ? nd.left : nd.right;
+            int32_t farc = ta <= tb ? nd.right : nd.left;
+            stack[sp++] = farc;
+            cur = nearc;
+        } else if (ha) {
+            cur = nd.left;
+        } else if (hb) {
+            cur = nd.right;
+        } else {
+            if (!sp) break;
+            cur = stack[--sp];
+        }
+    }
+    return h.tri >= 0;
+}
+
+// ------------------------------------------------------------------ bakers
+// DirectLight::influenceFromPoint (baker/DirectLight.cpp:101-125) and the light models of
+// scene/Light.cpp:372-500.
+float influence(const Scene& S, const Light& L, Vec lp, Vec p, Vec n, uint64_t& rays) {
+    float inf = 0.0f, att = 1.0f, cut = 1.0f, dist = 0.0f;
+    HitRec h;
+    if (L.type == 2) {  // directional: lambert(-dir, n), shadow towards p - dir*1000
+        float dp = dot3(neg(L.dir), n);
+        inf = dp < 0.0f ? 0.0f : dp;
+        if (inf <= 0.001f) {
+            inf = 0.0f;
+        } else if (L.shadow) {
+            rays++;
+            if (trace(S, make_seg(p, sub(p, scale(L.dir, 1000.0f))), true, h, nullptr)) inf *= 0.0f;
+            else inf *= 1.0f;
+        }
+    } else {
+        Vec d = sub(lp, p);
+        dist = normalize3(d);
+        float dp = dot3(d, n);
+        inf = dp < 0.0f ? 0.0f : dp;
+        if (inf <= 0.001f) {
+            inf = 0.0f;
+        } else if (L.shadow) {
+            rays++;
+            if (trace(S, make_seg(p, lp), true, h, nullptr)) inf *= 0.0f;
+            else inf *= 1.0f;
+        }
+    }
+    if (L.type == 1) {  // LightSpotCutoff::calculate at the light position
+        Vec sd = sub(p, L.pos);
+        normalize3(sd);
+        float v = dot3(sd, L.dir);
+        if (v <= L.cutoff) cut = 0.0f;
+        else {
+            v = (1.0f - (1.0f - v) * 1.0f / (1.0f - L.cutoff));
+            if (fabsf(1.0f - L.sexp) > 0.01f) v = powf(v, L.sexp);
+            cut = v;
+        }
+    }
+    if (L.has_att && L.type != 2) {
+        const float r = dist / L.radius;
+        const float a = 1.0f / (1.0f + L.c + L.l * r + L.q * r * r);
+        att = a > 0.0f ? a : 0.0f;
+    }
+    return inf * att * cut;
+}
+
+void direct_lumel(const Scene& S, Vec p, Vec n, float* col, uint64_t& rays) {
+    for (const Light& L : S.lights) {
+        float r = 0.0f, g = 0.0f, b = 0.0f;
+        if (!L.verts.empty()) {  // DirectLight::lightFromPointSet
+            for (const Vec& v : L.verts) {
+                const float f = influence(S, L, mk(v.x + L.pos.x, v.y + L.pos.y, v.z + L.pos.z), p, n, rays);
+                if (f > 0.0f) {
+                    r += L.color.x * L.intensity * f;
+                    g += L.color.y * L.intensity * f;
+                    b += L.color.z * L.intensity * f;
+                }
+            }
+            const float cnt = (float)L.verts.size();
+            r = r / cnt; g = g / cnt; b = b / cnt;
+        } else {  // DirectLight::lightFromPoint
+            const float f = influence(S, L, L.pos, p, n, rays);
+            if (f > 0.0f) {
+                r = L.color.x * L.intensity * f;
+                g = L.color.y * L.intensity * f;
+                b = L.color.z * L.intensity * f;
+            }
+        }
+        col[0] += r; col[1] += g; col[2] += b;
+    }
+}
+
+template <class F>
+void parallel_for(uint32_t n, int threads, F f) {
+    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    if (threads == 1 || n < 64) {
+        for (uint32_t i = 0; i < n; i++) f(i, 0);
+        return;
+    }
+    std::atomic<uint32_t> next(0);
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; t++) {
+        pool.emplace_back([&, t]() {
+            for (;;) {
+                const uint32_t b = next.fetch_add(64);
+                if (b >= n) break;
+                const uint32_t e = std::min(n, b + 64);
+                for (uint32_t i = b; i < e; i++) f(i, t);
+            }
+        });
+    }
+    for (auto& th : pool) th.join();
+}
+
+}  // namespace
+
+extern "C" {
+
+struct or_light {
+    int32_t type;
+    int32_t casts_shadow;
+    float position[3];
+    float direction[3];
+    float color[3];
+    float intensity;
+    int32_t has_attenuation;
+    float radius, att_constant, att_linear, att_quadratic;
+    float spot_cutoff, spot_exponent;
+    const float* area_positions;
+    const uint32_t* area_indices;
+    uint32_t area_vertex_count;
+    uint32_t area_index_count;
+};
+
+struct or_ao {
+    int32_t samples;
+    float occluded_fraction, max_distance, exponent;
+    uint32_t seed;
+    int32_t sampling;
+};
+
+void* or_scene_create(void) { return new Scene(); }
+void or_scene_destroy(void* s) { delete (Scene*)s; }
+
+// Scene::addMesh + Mesh::transformed (scene/Scene.cpp:98-107, scene/Mesh.cpp:277-297).
+int32_t or_scene_add_mesh(void* sp, const float* pos, const float* nrm, const float* uv, uint32_t nv,
+                          const uint32_t* idx, uint32_t ni, const float* M, const float* albedo) {
+    Scene& S = *(Scene*)sp;
+    std::vector<Vec> P(nv), N(nv);
+    for (uint32_t i = 0; i < nv; i++) {
+        Vec p = mk(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+        Vec n = mk(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
+        if (M) {
+            Vec tp = mk(p.x * M[0] + p.y * M[4] + p.z * M[8] + M[12], p.x * M[1] + p.y * M[5] + p.z * M[9] + M[13],
+                        p.x * M[2] + p.y * M[6] + p.z * M[10] + M[14]);
+            Vec tn = mk(n.x * M[0] + n.y * M[4] + n.z * M[8], n.x * M[1] + n.y * M[5] + n.z * M[9],
+                        n.x * M[2] + n.y * M[6] + n.z * M[10]);
+            normalize3(tn);
+            p = tp;
+            n = tn;
+        }
+        P[i] = p;
+        N[i] = n;
+    }
+    const uint32_t mesh = (uint32_t)S.mpos.size();
+    const uint32_t first = (uint32_t)(S.tri.size() / 9);
+    for (uint32_t f = 0; f < ni / 3; f++) {
+        for (int k = 0; k < 3; k++) {
+            const uint32_t v = idx[3 * f + k];
+            S.tri.push_back(P[v].x); S.tri.push_back(P[v].y); S.tri.push_back(P[v].z);
+            S.tuv.push_back(uv[2 * v]); S.tuv.push_back(uv[2 * v + 1]);
+            S.tnrm.push_back(N[v].x); S.tnrm.push_back(N[v].y); S.tnrm.push_back(N[v].z);
+        }
+        for (int k = 0; k < 4; k++) S.talb.push_back(albedo ? albedo[k] : 1.0f);
+        S.tmesh.push_back(mesh);
+        S.tface.push_back(f);
+    }
+    S.mpos.push_back(P);
+    S.mnrm.push_back(N);
+    S.muv.push_back(std::vector<float>(uv, uv + 2 * (size_t)nv));
+    S.midx.push_back(std::vector<uint32_t>(idx, idx + ni));
+    S.mfirst.push_back(first);
+    return (int32_t)mesh;
+}
+
+int32_t or_scene_add_light(void* sp, const or_light* d) {
+    Scene& S = *(Scene*)sp;
+    Light L;
+    L.type = d->type;
+    L.shadow = d->casts_shadow;
+    L.pos = mk(d->position[0], d->position[1], d->position[2]);
+    L.dir = mk(d->direction[0], d->direction[1], d->direction[2]);
+    L.color = mk(d->color[0], d->color[1], d->color[2]);
+    L.intensity = d->intensity;
+    L.has_att = d->has_attenuation;
+    L.radius = d->radius; L.c = d->att_constant; L.l = d->att_linear; L.q = d->att_quadratic;
+    L.cutoff = d->spot_cutoff; L.sexp = d->spot_exponent;
+    if (d->type == 3) {
+        // Triangle(face) = (face.vertex(0), vertex(1), vertex(2)) = (a, c, b); centroid = sum / 3.
+        for (uint32_t f = 0; f < d->area_index_count / 3; f++) {
+            const float* P = d->area_positions;
+            const uint32_t ia = d->area_indices[3 * f], ib = d->area_indices[3 * f + 1], ic = d->area_indices[3 * f + 2];
+            Vec a = mk(P[3 * ia], P[3 * ia + 1], P[3 * ia + 2]);
+            Vec b = mk(P[3 * ib], P[3 * ib + 1], P[3 * ib + 2]);
+            Vec c = mk(P[3 * ic], P[3 * ic + 1], P[3 * ic + 2]);
+            L.verts.push_back(divs(add(add(a, c), b), 3.0f));
+        }
+    }
+    S.lights.push_back(L);
+    return (int32_t)S.lights.size() - 1;
+}
+
+int32_t or_scene_build(void* sp, int32_t use_bvh) {
+    Scene& S = *(Scene*)sp;
+    S.use_bvh = use_bvh != 0;
+    if (S.use_bvh) build_bvh(S);
+    return 0;
+}
+
+// Lightmap::initializeLumels for every face of `mesh` into a w*h grid (in/out arrays).
+int32_t or_raster(void* sp, int32_t mesh, int32_t w, int32_t h, uint8_t* valid, uint32_t* tri,
+                  float* pos, float* nrm) {
+    const Scene& S = *(Scene*)sp;
+    const std::vector<Vec>& P = S.mpos[mesh];
+    const std::vector<Vec>& N = S.mnrm[mesh];
+    const std::vector<float>& UV = S.muv[mesh];
+    const std::vector<uint32_t>& I = S.midx[mesh];
+    for (uint32_t f = 0; f < I.size() / 3; f++) {
+        const uint32_t a = I[3 * f], b = I[3 * f + 1], c = I[3 * f + 2];
+        const float ua = UV[2 * a], va = UV[2 * a + 1], ub = UV[2 * b], vb = UV[2 * b + 1], uc = UV[2 * c], vc = UV[2 * c + 1];
+        // Lightmap::rect of Face::uvRect: int(min * size), int(max * size).
+        const int x1 = (int)(std::min(ua, std::min(ub, uc)) * w), x2 = (int)(std::max(ua, std::max(ub, uc)) * w);
+        const int y1 = (int)(std::min(va, std::min(vb, vc)) * h), y2 = (int)(std::max(va, std::max(vb, vc)) * h);
+        for (int y = std::max(0, y1); y <= std::min(y2, h - 1); y++) {
+            for (int x = std::max(0, x1); x <= std::min(x2, w - 1); x++) {
+                const float px = (x + 0.5f) / float(w), py = (y + 0.5f) / float(h);
+                // Face::isUvInside: v0 = c-a, v1 = b-a, v2 = uv-a.
+                const float e0x = uc - ua, e0y = vc - va, e1x = ub - ua, e1y = vb - va, e2x = px - ua, e2y = py - va;
+                const float d00 = e0x * e0x + e0y * e0y, d01 = e0x * e1x + e0y * e1y, d02 = e0x * e2x + e0y * e2y;
+                const float d11 = e1x * e1x + e1y * e1y, d12 = e1x * e2x + e1y * e2y;
+                const float inv = 1.0f / (d00 * d11 - d01 * d01);
+                const float bu = (d11 * d02 - d01 * d12) * inv;
+                const float bv = (d00 * d12 - d01 * d02) * inv;
+                if (!(bu >= 0.0f && bv >= 0.0f && bu + bv < 1.0f)) continue;
+                const size_t t = (size_t)y * w + x;
+                valid[t] = 1;
+                tri[t] = S.mfirst[mesh] + f;
+                // Face::positionAt / normalAt: a + (b-a)*y + (c-a)*x with (x, y) = (bu, bv).
+                const Vec A = P[a], B = P[b], C = P[c];
+                pos[3 * t] = A.x + (B.x - A.x) * bv + (C.x - A.x) * bu;
+                pos[3 * t + 1] = A.y + (B.y - A.y) * bv + (C.y - A.y) * bu;
+                pos[3 * t + 2] = A.z + (B.z - A.z) * bv + (C.z - A.z) * bu;
+                const Vec NA = N[a], NB = N[b], NC = N[c];
+                nrm[3 * t] = NA.x + (NB.x - NA.x) * bv + (NC.x - NA.x) * bu;
+                nrm[3 * t + 1] = NA.y + (NB.y - NA.y) * bv + (NC.y - NA.y) * bu;
+                nrm[3 * t + 2] = NA.z + (NB.z - NA.z) * bv + (NC.z - NA.z) * bu;
+            }
+        }
+    }
+    return 0;
+}
+
+// DirectLight::bakeLumel over n lumels; color is in/out (3 per lumel).
+int32_t or_bake_direct(void* sp, uint32_t n, const float* pos, const float* nrm, float* color,
+                       int32_t threads, uint64_t* rays_out) {
+    const Scene& S = *(Scene*)sp;
+    std::vector<uint64_t> rays(256, 0);
+    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    threads = std::min(threads, 255);
+    parallel_for(n, threads, [&](uint32_t i, int t) {
+        direct_lumel(S, mk(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]), mk(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]),
+                     &color[3 * i], rays[t]);
+    });
+    uint64_t tot = 0;
+    for (uint64_t r : rays) tot += r;
+    if (rays_out) *rays_out = tot;
+    return 0;
+}
+
+// AmbientOcclusion::bakeLumel over n lumels (texel ids key the sample streams).
+int32_t or_bake_ao(void* sp, uint32_t n, const float* pos, const float* nrm, const uint32_t* texel,
+                   uint32_t lm_seed, const or_ao* a, float* color, uint32_t* occluded_out,
+                   int32_t threads, uint64_t* rays_out) {
+    const Scene& S = *(Scene*)sp;
+    parallel_for(n, threads, [&](uint32_t i, int) {
+        const Vec p = mk(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+        const Vec nn = mk(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
+        int occ = 0;
+        HitRec h;
+        for (int k = 0; k < a->samples; k++) {
+            Rng r = stream(a->seed, lm_seed, texel[i], 1u, (uint32_t)k);
+            const Vec d = hemi(r, nn, a->sampling);
+            if (trace(S, make_seg(p, add(p, scale(d, a->max_distance))), true, h, nullptr)) occ++;
+        }
+        float value = 1.0f - occ / (a->samples * a->occluded_fraction);
+        if (fabsf(1.0f - a->exponent) > 0.01f) value = powf(value, a->exponent);
+        color[3 * i] *= value;
+        color[3 * i + 1] *= value;
+        color[3 * i + 2] *= value;
+        if (occluded_out) occluded_out[i] = (uint32_t)occ;
+    });
+    if (rays_out) *rays_out = (uint64_t)n * (uint64_t)a->samples;
+    return 0;
+}
+
+// rt::ITracer::traceSegment over n segments. mode 0 = any hit (flags), 1 = closest hit.
+// Counters (node fetches, triangle tests) feed the per-ray traffic model.
+int32_t or_trace(void* sp, uint32_t n, const float* st, const float* en, int32_t mode, uint8_t* flags,
+                 float* t, int32_t* tri, float* u, float* v, uint64_t* node_visits, uint64_t* tri_tests,
+                 int32_t threads) {
+    const Scene& S = *(Scene*)sp;
+    std::vector<Counters> ctr(256);
+    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    threads = std::min(threads, 255);
+    parallel_for(n, threads, [&](uint32_t i, int th) {
+        HitRec h;
+        const bool hit = trace(S, make_seg(mk(st[3 * i], st[3 * i + 1], st[3 * i + 2]), mk(en[3 * i], en[3 * i + 1], en[3 * i + 2])),
+                               mode == 0, h, &ctr[th]);
+        if (flags) flags[i] = hit ? 1 : 0;
+        if (t) t[i] = hit ? h.t : 0.0f;
+        if (tri) tri[i] = hit ? h.tri : -1;
+        if (u) u[i] = hit ? h.u : 0.0f;
+        if (v) v[i] = hit ? h.v : 0.0f;
+    });
+    uint64_t nv = 0, nt = 0;
+    for (auto& c : ctr) { nv += c.nodes; nt += c.tris; }
+    if (node_visits) *node_visits = nv;
+    if (tri_tests) *tri_tests = nt;
+    return 0;
+}
+
+// Scene-order triangle -> mesh, face.
+int32_t or_tri_info(void* sp, int32_t tri, int32_t* mesh, int32_t* face) {
+    const Scene& S = *(Scene*)sp;
+    if (tri < 0 || (size_t)tri >= S.tmesh.size()) return -1;
+    *mesh = (int32_t)S.tmesh[tri];
+    *face = (int32_t)S.tface[tri];
+    return 0;
+}
+
+// Raw PCG32 outputs for pinning against published vectors.
+void or_pcg32(uint64_t initstate, uint64_t initseq, uint32_t n, uint32_t* out) {
+    Rng r = rng_seed(initstate, initseq);
+    for (uint32_t i = 0; i < n; i++) out[i] = rng_u32(r);
+}
+
+// Sample-stream outputs and hemisphere directions for the sampler checks.
+void or_stream_u32(uint32_t seed, uint32_t lm_seed, uint32_t texel, uint32_t tech, uint32_t sample, uint32_t n,
+                   uint32_t* out) {
+    Rng r = stream(seed, lm_seed, texel, tech, sample);
+    for (uint32_t i = 0; i < n; i++) out[i] = rng_u32(r);
+}
+
+void or_hemisphere(uint32_t seed, uint32_t lm_seed, uint32_t texel, uint32_t tech, uint32_t count, int32_t mode,
+                   const float* normal, float* out) {
+    const Vec n = mk(normal[0], normal[1], normal[2]);
+    for (uint32_t k = 0; k < count; k++) {
+        Rng r = stream(seed, lm_seed, texel, tech, k);
+        const Vec d = hemi(r, n, mode);
+        out[3 * k] = d.x; out[3 * k + 1] = d.y; out[3 * k + 2] = d.z;
+    }
+}
+
+uint64_t or_splitmix64(uint64_t z) { return mix64(z); }
+
+}  // extern "C"

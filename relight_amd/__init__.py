@@ -1,0 +1,16 @@
+"""relight_amd — MI355X lightmap baker for the ray-cast hot path of dmsovetov/Relight.
+
+The product is librelight_amd.so (HIP kernels for gfx950 + a C-ABI, include/relight.h); this
+package mirrors the reference's Relight / Scene / Lightmap API over that ABI and provides the
+seeded scene generators the tests and the benchmark use.
+"""
+from ._capi import (RL_LIGHT_AREA, RL_LIGHT_DIRECTIONAL, RL_LIGHT_POINT, RL_LIGHT_SPOT,
+                    RL_SAMPLE_COSINE, RL_SAMPLE_UNIFORM, RL_TRACE_ANY, RL_TRACE_CLOSEST)
+from .relight import (AmbientOcclusionSettings, LightDesc, Lightmap, Relight, RelightError,
+                      Scene)
+
+__all__ = [
+    "Relight", "Scene", "Lightmap", "LightDesc", "AmbientOcclusionSettings", "RelightError",
+    "RL_LIGHT_POINT", "RL_LIGHT_SPOT", "RL_LIGHT_DIRECTIONAL", "RL_LIGHT_AREA",
+    "RL_SAMPLE_UNIFORM", "RL_SAMPLE_COSINE", "RL_TRACE_ANY", "RL_TRACE_CLOSEST",
+]

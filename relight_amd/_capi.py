@@ -1,0 +1,154 @@
+"""ctypes declarations of include/relight.h (librelight_amd.so).
+
+The library is the product: there is no Python or CPU fallback. Importing this module raises
+if the HIP library has not been built.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librelight_amd.so")
+
+RL_SUCCESS = 0
+RL_INVALID_CALL = 1
+RL_NOT_IMPLEMENTED = 2
+RL_OUT_OF_MEMORY = 3
+RL_REJECTED = 4
+RL_DEVICE_ERROR = 5
+RL_INVALID_ARGUMENT = 6
+
+RL_LIGHT_POINT = 0
+RL_LIGHT_SPOT = 1
+RL_LIGHT_DIRECTIONAL = 2
+RL_LIGHT_AREA = 3
+
+RL_SAMPLE_UNIFORM = 0
+RL_SAMPLE_COSINE = 1
+
+RL_TRACE_ANY = 0
+RL_TRACE_CLOSEST = 1
+
+STATUS_NAMES = {
+    0: "Success", 1: "InvalidCall", 2: "NotImplemented", 3: "OutOfMemory",
+    4: "Rejected", 5: "DeviceError", 6: "InvalidArgument",
+}
+
+
+class rl_light_desc(C.Structure):
+    _fields_ = [
+        ("type", C.c_int32),
+        ("casts_shadow", C.c_int32),
+        ("position", C.c_float * 3),
+        ("direction", C.c_float * 3),
+        ("color", C.c_float * 3),
+        ("intensity", C.c_float),
+        ("has_attenuation", C.c_int32),
+        ("radius", C.c_float),
+        ("att_constant", C.c_float),
+        ("att_linear", C.c_float),
+        ("att_quadratic", C.c_float),
+        ("spot_cutoff", C.c_float),
+        ("spot_exponent", C.c_float),
+        ("area_positions", C.POINTER(C.c_float)),
+        ("area_indices", C.POINTER(C.c_uint32)),
+        ("area_vertex_count", C.c_uint32),
+        ("area_index_count", C.c_uint32),
+    ]
+
+
+class rl_ao_settings(C.Structure):
+    _fields_ = [
+        ("samples", C.c_int32),
+        ("occluded_fraction", C.c_float),
+        ("max_distance", C.c_float),
+        ("exponent", C.c_float),
+        ("seed", C.c_uint32),
+        ("sampling", C.c_int32),
+    ]
+
+
+class rl_hit(C.Structure):
+    _fields_ = [
+        ("t", C.c_float),
+        ("triangle", C.c_int32),
+        ("mesh", C.c_int32),
+        ("face", C.c_int32),
+        ("u", C.c_float),
+        ("v", C.c_float),
+    ]
+
+
+RL_KERNEL_DIRECT = 0
+RL_KERNEL_AO_COUNT = 1
+RL_KERNEL_AO_FINALIZE = 2
+RL_KERNEL_TRACE = 3
+RL_KERNEL_SLOTS = 8
+KERNEL_NAMES = {0: "direct", 1: "ao_count", 2: "ao_finalize", 3: "trace"}
+
+
+class rl_stats(C.Structure):
+    _fields_ = [
+        ("rays", C.c_uint64),
+        ("texels", C.c_uint64),
+        ("gpu_ms", C.c_double),
+        ("wall_ms", C.c_double),
+        ("launches", C.c_int32),
+        ("pad", C.c_int32),
+        ("kernel_ms", C.c_double * RL_KERNEL_SLOTS),
+        ("kernel_launches", C.c_int32 * RL_KERNEL_SLOTS),
+    ]
+
+
+_P = C.c_void_p
+_PP = C.POINTER(C.c_void_p)
+_F = C.POINTER(C.c_float)
+_U32 = C.POINTER(C.c_uint32)
+_I32 = C.POINTER(C.c_int32)
+
+# (name, restype, argtypes) for every entry point of include/relight.h.
+SIGNATURES = [
+    ("rl_last_error", C.c_char_p, []),
+    ("rl_create", C.c_int32, [_PP]),
+    ("rl_destroy", C.c_int32, [_P]),
+    ("rl_scene_create", C.c_int32, [_P, _PP]),
+    ("rl_scene_destroy", C.c_int32, [_P]),
+    ("rl_scene_begin", C.c_int32, [_P]),
+    ("rl_scene_add_mesh", C.c_int32, [_P, _F, _F, _F, C.c_uint32, _U32, C.c_uint32, _F, _F, _I32]),
+    ("rl_scene_add_light", C.c_int32, [_P, C.POINTER(rl_light_desc), _I32]),
+    ("rl_scene_end", C.c_int32, [_P]),
+    ("rl_scene_info", C.c_int32, [_P, _U32, _U32, _I32]),
+    ("rl_lightmap_create", C.c_int32, [_P, C.c_int32, C.c_int32, _PP]),
+    ("rl_lightmap_destroy", C.c_int32, [_P]),
+    ("rl_lightmap_add_mesh", C.c_int32, [_P, _P, C.c_int32]),
+    ("rl_lightmap_set_seed", C.c_int32, [_P, C.c_uint32]),
+    ("rl_lightmap_valid_count", C.c_int32, [_P, _U32]),
+    ("rl_lightmap_read_lumels", C.c_int32, [_P, _U32, _F, _F]),
+    ("rl_lightmap_read_hdr", C.c_int32, [_P, _F]),
+    ("rl_bake_direct", C.c_int32, [_P, _P, C.c_int32]),
+    ("rl_bake_ambient_occlusion", C.c_int32, [_P, _P, C.c_int32, C.POINTER(rl_ao_settings)]),
+    ("rl_bake_batch", C.c_int32, [_P, _P, C.POINTER(C.c_void_p), C.c_int32, C.c_int32,
+                                  C.POINTER(rl_ao_settings)]),
+    ("rl_trace_segments", C.c_int32, [_P, _P, _F, _F, C.c_uint32, C.c_int32,
+                                      C.POINTER(C.c_uint8), C.POINTER(rl_hit)]),
+    ("rl_get_stats", C.c_int32, [_P, C.POINTER(rl_stats)]),
+]
+
+_lib = None
+
+
+def load():
+    """Loads librelight_amd.so; raises RuntimeError when it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"librelight_amd.so is missing at {LIB_PATH}: build the HIP library first "
+            "(python -c 'import __graft_entry__ as g; g.build()')")
+    lib = C.CDLL(LIB_PATH)
+    for name, res, args in SIGNATURES:
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib

@@ -1,0 +1,88 @@
+// Device-side data layout shared by the host runtime and the HIP kernels.
+//
+// HBM layout of a built scene (one allocation each, 128-B aligned by hipMalloc):
+//   nodes : Bvh4Node[num_nodes]       128 B per node, child boxes stored SoA inside the node so
+//                                     one lane reads a node as eight 16-B loads.
+//   tris  : float4[3 * num_tris]      48 B per triangle in leaf order: v0, v1, v2; v0.w carries
+//                                     the scene-order triangle id (bit pattern of a uint32).
+//   tri_attr : TriAttr[num_tris]      scene-order attributes used only after a closest hit.
+// Lumels of a lightmap are stored compacted (valid texels only) as three float4 streams: position
+// (w = texel index y*width+x), normal (w = scene-order triangle id), colour (w unused).
+#pragma once
+
+#include <stdint.h>
+#include <hip/hip_runtime.h>
+#include "rl_math.h"
+
+namespace rl {
+
+#define RL_LEAF_BIT 0x80000000u
+#define RL_EMPTY_CHILD 0xFFFFFFFFu
+#define RL_LEAF_MAX 8
+// Leaf child word: bit 31 set, bits 27..30 = count-1, bits 0..26 = first triangle (leaf order).
+#define RL_LEAF_FIRST_MASK 0x07FFFFFFu
+
+struct alignas(128) Bvh4Node {
+    float lox[4], hix[4];
+    float loy[4], hiy[4];
+    float loz[4], hiz[4];
+    uint32_t child[4];
+    uint32_t pad[4];
+};
+static_assert(sizeof(Bvh4Node) == 128, "node size");
+
+// Per scene-order triangle: owning mesh, face index in the mesh, and lightmap uvs of its three
+// vertices in the reference's vertex order a, b, c (indices 0, 1, 2 of the face).
+struct TriAttr {
+    uint32_t mesh;
+    uint32_t face;
+    float uv[6];
+    float n[9];
+    float albedo[4];
+};
+
+struct SceneDev {
+    const Bvh4Node* nodes;
+    const float4* tris;
+    const TriAttr* attr;
+    uint32_t num_nodes;
+    uint32_t num_tris;
+};
+
+// One light as the bake kernels see it (scene/Light.h models, flattened).
+struct LightDev {
+    int32_t type;
+    int32_t casts_shadow;
+    int32_t has_attenuation;
+    int32_t num_vertices;   // > 0: vertex-generated (area) light
+    int32_t vertex_offset;  // into the light-vertex array (xyz float4, local to position)
+    int32_t has_spot;
+    float pos[3];
+    float dir[3];
+    float color[3];
+    float intensity;
+    float radius, att_c, att_l, att_q;
+    float cutoff, spot_exponent;
+};
+
+// One lightmap inside a bake launch.
+struct BakeSeg {
+    const float4* pos;
+    const float4* nrm;
+    float4* color;
+    uint32_t* counts;     // per-lumel scratch counters (AO occluded count)
+    uint32_t count;       // valid lumels
+    uint32_t lm_seed;
+    uint64_t work_begin;  // first global work item of this segment
+};
+
+struct AoParams {
+    int32_t samples;
+    float occluded_fraction;
+    float max_distance;
+    float exponent;
+    uint32_t seed;
+    int32_t sampling;
+};
+
+}  // namespace rl

@@ -1,0 +1,402 @@
+// CDNA4 (gfx950) kernels of the bake path: BVH4 traversal with an LDS stack, watertight
+// triangle tests, and the per-lumel bake kernels that replace DirectLight::bakeLumel
+// (baker/DirectLight.cpp:46-125) and AmbientOcclusion::bakeLumel
+// (baker/AmbientOcclusion.cpp:46-66) over rt::ITracer::traceSegment (rt/Tracer.h:87).
+//
+// Wave64 throughout: blocks are whole wavefronts, per-texel reductions use ballot + popcount so
+// integer results are exact and independent of scheduling.
+#include <hip/hip_runtime.h>
+
+#include "rl_device.h"
+#include "rl_kernels.h"
+
+namespace rl {
+
+namespace {
+
+constexpr int kBlock = 128;  // two wavefronts
+// Slab-test safety factor 1 + 2*gamma(3) (Ize 2013): scales the exit distance so that the
+// rounded slab distances never reject a box the exact ray passes through.
+constexpr float kBoxScale = 1.0000004f;
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+
+__device__ __forceinline__ float bits_f(uint32_t u) { return __uint_as_float(u); }
+__device__ __forceinline__ uint32_t f_bits(float f) { return __float_as_uint(f); }
+
+struct TraceResult {
+    float t;
+    uint32_t tri;  // scene-order id, 0xFFFFFFFF on a miss
+    float U, V, W, det;
+};
+
+__device__ __forceinline__ void cswap(float& ka, uint32_t& va, float& kb, uint32_t& vb) {
+    if (kb < ka) {
+        float tk = ka; ka = kb; kb = tk;
+        uint32_t tv = va; va = vb; vb = tv;
+    }
+}
+
+// Traverses the BVH4 for one ray. ANY: stop at the first triangle with t in (tnear, tfar).
+// Closest: the hit with the smallest t, ties broken by the smaller scene-order triangle id, so
+// the result does not depend on traversal order. stk points at this lane's first LDS stack slot;
+// consecutive entries are `stride` words apart (one column per lane, conflict-free).
+template <bool ANY, int CAP>
+__device__ bool traverse(const SceneDev& S, const Ray& ray, uint32_t* stk, uint32_t stride,
+                         TraceResult& res) {
+    const WtRay w = wt_setup(ray);
+    const float ix = 1.0f / ray.d.x, iy = 1.0f / ray.d.y, iz = 1.0f / ray.d.z;
+    const float ox = ray.o.x, oy = ray.o.y, oz = ray.o.z;
+    const float tnear = ray.tnear;
+    float tlimit = ray.tfar;  // closest: shrinks to the best hit so far
+    res.t = ray.tfar;
+    res.tri = 0xFFFFFFFFu;
+    res.U = res.V = res.W = res.det = 0.0f;
+
+    uint32_t node = 0;
+    int sp = 0;
+    for (;;) {
+        if (!(node & RL_LEAF_BIT)) {
+            const float4* nd = reinterpret_cast<const float4*>(S.nodes + node);
+            const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
+            const uint4 ch = reinterpret_cast<const uint4*>(nd)[6];
+            const float lim = tlimit * kBoxScale;
+            float key[4];
+            uint32_t cid[4] = {ch.x, ch.y, ch.z, ch.w};
+            const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
+            const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
+            const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                float t0x = (lxa[i] - ox) * ix, t1x = (hxa[i] - ox) * ix;
+                float t0y = (lya[i] - oy) * iy, t1y = (hya[i] - oy) * iy;
+                float t0z = (lza[i] - oz) * iz, t1z = (hza[i] - oz) * iz;
+                float tmin = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tnear));
+                float tmax = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * kBoxScale;
+                bool hit = tmin <= fminf(tmax, lim) && cid[i] != RL_EMPTY_CHILD;
+                key[i] = hit ? tmin : __builtin_inff();
+            }
+            // Sort the four (distance, child) pairs: near child next, the rest pushed far-first.
+            cswap(key[0], cid[0], key[1], cid[1]);
+            cswap(key[2], cid[2], key[3], cid[3]);
+            cswap(key[0], cid[0], key[2], cid[2]);
+            cswap(key[1], cid[1], key[3], cid[3]);
+            cswap(key[1], cid[1], key[2], cid[2]);
+            if (key[0] == __builtin_inff()) {
+                if (sp == 0) break;
+                node = stk[(--sp) * stride];
+                continue;
+            }
+#pragma unroll
+            for (int i = 3; i >= 1; i--) {
+                if (key[i] != __builtin_inff() && sp < CAP) {
+                    stk[(sp++) * stride] = cid[i];
+                }
+            }
+            node = cid[0];
+        } else {
+            const uint32_t first = node & RL_LEAF_FIRST_MASK;
+            const uint32_t count = ((node >> 27) & 15u) + 1u;
+            for (uint32_t j = 0; j < count; j++) {
+                const float4* tp = S.tris + 3u * (first + j);
+                const float4 a = tp[0], b = tp[1], c = tp[2];
+                float t, U, V, W, det;
+                if (wt_intersect(w, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t, U, V, W, det)) {
+                    if (ANY) {
+                        res.t = t;
+                        res.tri = f_bits(a.w);
+                        return true;
+                    }
+                    const uint32_t id = f_bits(a.w);
+                    if (t < res.t || (t == res.t && id < res.tri)) {
+                        res.t = t;
+                        res.tri = id;
+                        res.U = U; res.V = V; res.W = W; res.det = det;
+                        tlimit = t;
+                    }
+                }
+            }
+            if (sp == 0) break;
+            node = stk[(--sp) * stride];
+        }
+    }
+    return res.tri != 0xFFFFFFFFu;
+}
+
+__device__ __forceinline__ int find_seg(const BakeSeg* segs, int nsegs, uint64_t g) {
+    int lo = 0, hi = nsegs - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (segs[mid].work_begin <= g) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void count_rays(uint32_t n, unsigned long long* counter) {
+    // Wave sum of per-lane ray counts, one atomic per wavefront.
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) n += __shfl_xor(n, off, 64);
+    if (lane_id() == 0 && n) atomicAdd(counter, (unsigned long long)n);
+}
+
+// LightInfluence::calculate / DirectionalLightInfluence::calculate with LightCutoff and
+// LinearLightAttenuation, i.e. DirectLight::influenceFromPoint (baker/DirectLight.cpp:101-125,
+// scene/Light.cpp:372-421,452-500).
+template <int CAP>
+__device__ float influence_from_point(const SceneDev& S, const LightDev& L, V3 lp, V3 p, V3 n,
+                                      uint32_t* stk, uint32_t& rays) {
+    float inf = 0.0f, att = 1.0f, cut = 1.0f, distance = 0.0f;
+    if (L.type == LIGHT_DIRECTIONAL) {
+        V3 ld = v3(L.dir[0], L.dir[1], L.dir[2]);
+        inf = lambert(-ld, n);
+        if (inf <= 0.001f) {
+            inf = 0.0f;
+        } else if (L.casts_shadow) {
+            Ray r = ray_from_segment(p, p - ld * 1000.0f);
+            TraceResult tr;
+            rays++;
+            inf *= traverse<true, CAP>(S, r, stk, kBlock, tr) ? 0.0f : 1.0f;
+        }
+    } else {
+        V3 dir = lp - p;
+        distance = normalize(dir);
+        inf = lambert(dir, n);
+        if (inf <= 0.001f) {
+            inf = 0.0f;
+        } else if (L.casts_shadow) {
+            Ray r = ray_from_segment(p, lp);
+            TraceResult tr;
+            rays++;
+            inf *= traverse<true, CAP>(S, r, stk, kBlock, tr) ? 0.0f : 1.0f;
+        }
+    }
+    if (L.has_spot) {
+        V3 sd = p - v3(L.pos[0], L.pos[1], L.pos[2]);
+        normalize(sd);
+        float value = dot(sd, v3(L.dir[0], L.dir[1], L.dir[2]));
+        if (value <= L.cutoff) {
+            cut = 0.0f;
+        } else {
+            value = (1.0f - (1.0f - value) * 1.0f / (1.0f - L.cutoff));
+            if (fabsf(1.0f - L.spot_exponent) > 0.01f) value = powf(value, L.spot_exponent);
+            cut = value;
+        }
+    }
+    if (L.has_attenuation) att = attenuation(distance, L.radius, L.att_c, L.att_l, L.att_q);
+    return inf * att * cut;
+}
+
+template <int CAP>
+__global__ __launch_bounds__(kBlock) void k_bake_direct(SceneDev S, const BakeSeg* __restrict__ segs,
+                                                        int nsegs, uint64_t total,
+                                                        const LightDev* __restrict__ lights, int nlights,
+                                                        const float4* __restrict__ lverts,
+                                                        unsigned long long* rays_out) {
+    __shared__ uint32_t stack[CAP * kBlock];
+    uint32_t* stk = stack + threadIdx.x;
+    const uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    uint32_t rays = 0;
+    if (g < total) {
+        const BakeSeg& sg = segs[find_seg(segs, nsegs, g)];
+        const uint32_t i = (uint32_t)(g - sg.work_begin);
+        const float4 p4 = sg.pos[i], n4 = sg.nrm[i];
+        const V3 p = v3(p4.x, p4.y, p4.z), n = v3(n4.x, n4.y, n4.z);
+        float4 col = sg.color[i];
+        for (int l = 0; l < nlights; l++) {
+            const LightDev L = lights[l];
+            float cr, cg, cb;
+            if (L.num_vertices > 0) {
+                // DirectLight::lightFromPointSet (baker/DirectLight.cpp:75-98).
+                cr = cg = cb = 0.0f;
+                for (int v = 0; v < L.num_vertices; v++) {
+                    const float4 lv = lverts[L.vertex_offset + v];
+                    V3 lp = v3(lv.x + L.pos[0], lv.y + L.pos[1], lv.z + L.pos[2]);
+                    float inf = influence_from_point<CAP>(S, L, lp, p, n, stk, rays);
+                    if (inf > 0.0f) {
+                        cr += L.color[0] * L.intensity * inf;
+                        cg += L.color[1] * L.intensity * inf;
+                        cb += L.color[2] * L.intensity * inf;
+                    }
+                }
+                float nv = (float)L.num_vertices;
+                cr = cr / nv; cg = cg / nv; cb = cb / nv;
+            } else {
+                // DirectLight::lightFromPoint (baker/DirectLight.cpp:63-72).
+                float inf = influence_from_point<CAP>(S, L, v3(L.pos[0], L.pos[1], L.pos[2]), p, n, stk, rays);
+                if (inf > 0.0f) {
+                    cr = L.color[0] * L.intensity * inf;
+                    cg = L.color[1] * L.intensity * inf;
+                    cb = L.color[2] * L.intensity * inf;
+                } else {
+                    cr = cg = cb = 0.0f;
+                }
+            }
+            col.x += cr; col.y += cg; col.z += cb;
+        }
+        sg.color[i] = col;
+    }
+    count_rays(rays, rays_out);
+}
+
+// AO pass 1: one lane per (lumel, sample) pair, pairs lumel-major so a lumel's rays share a
+// wavefront. Occluded rays are counted per lumel with a ballot, one integer atomic per run.
+template <int CAP>
+__global__ __launch_bounds__(kBlock) void k_ao_count(SceneDev S, const BakeSeg* __restrict__ segs,
+                                                     int nsegs, uint64_t total, AoParams P,
+                                                     unsigned long long* rays_out) {
+    __shared__ uint32_t stack[CAP * kBlock];
+    uint32_t* stk = stack + threadIdx.x;
+    const uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    const bool active = g < total;
+    bool occ = false;
+    uint32_t* counter = nullptr;
+    uint64_t key = ~0ull;
+    if (active) {
+        const int s = find_seg(segs, nsegs, g);
+        const BakeSeg& sg = segs[s];
+        const uint64_t local = g - sg.work_begin;
+        const uint32_t li = (uint32_t)(local / (uint64_t)P.samples);
+        const uint32_t si = (uint32_t)(local % (uint64_t)P.samples);
+        const float4 p4 = sg.pos[li], n4 = sg.nrm[li];
+        const V3 p = v3(p4.x, p4.y, p4.z), n = v3(n4.x, n4.y, n4.z);
+        Pcg32 rng = sample_stream(P.seed, sg.lm_seed, f_bits(p4.w), TECH_AO, si);
+        V3 dir = sample_hemisphere(rng, n, P.sampling);
+        Ray r = ray_from_segment(p, p + dir * P.max_distance);
+        TraceResult tr;
+        occ = traverse<true, CAP>(S, r, stk, kBlock, tr);
+        counter = sg.counts + li;
+        key = ((uint64_t)s << 32) | li;
+    }
+    const uint64_t occ_mask = __ballot(occ);
+    const uint64_t act_mask = __ballot(active);
+    const uint32_t lane = lane_id();
+    const uint64_t prev_key = __shfl_up(key, 1, 64);
+    const bool head = active && (lane == 0 || prev_key != key);
+    const uint64_t heads = __ballot(head);
+    if (head) {
+        const uint64_t above = (lane == 63) ? 0ull : (heads & (~0ull << (lane + 1)));
+        const uint64_t end_mask = above ? (above & (~above + 1ull)) : 0ull;  // lowest head above
+        const uint64_t run = (end_mask ? (end_mask - 1ull) : ~0ull) & (~0ull << lane);
+        const uint32_t c = (uint32_t)__popcll(occ_mask & run & act_mask);
+        if (c) atomicAdd(counter, c);
+    }
+    if (lane == 0) {
+        const uint32_t nr = (uint32_t)__popcll(act_mask);
+        if (nr) atomicAdd(rays_out, (unsigned long long)nr);
+    }
+}
+
+// AO pass 2: AmbientOcclusion::bakeLumel's value and multiply (baker/AmbientOcclusion.cpp:59-65).
+__global__ __launch_bounds__(kBlock) void k_ao_finalize(const BakeSeg* __restrict__ segs, int nsegs,
+                                                        uint64_t total, AoParams P) {
+    const uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (g >= total) return;
+    const BakeSeg& sg = segs[find_seg(segs, nsegs, g)];
+    const uint32_t i = (uint32_t)(g - sg.work_begin);
+    const uint32_t occluded = sg.counts[i];
+    float value = 1.0f - (float)occluded / ((float)P.samples * P.occluded_fraction);
+    if (fabsf(1.0f - P.exponent) > 0.01f) value = powf(value, P.exponent);
+    float4 c = sg.color[i];
+    c.x *= value; c.y *= value; c.z *= value;
+    sg.color[i] = c;
+    sg.counts[i] = 0u;
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n) p[g] = v;
+}
+
+template <bool ANY, int CAP>
+__global__ __launch_bounds__(kBlock) void k_trace(SceneDev S, const float4* __restrict__ starts,
+                                                  const float4* __restrict__ ends, uint32_t n,
+                                                  uint8_t* __restrict__ flags, float4* __restrict__ hits,
+                                                  unsigned long long* rays_out) {
+    __shared__ uint32_t stack[CAP * kBlock];
+    const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
+    if (g < n) {
+        const float4 a = starts[g], b = ends[g];
+        Ray r = ray_from_segment(v3(a.x, a.y, a.z), v3(b.x, b.y, b.z));
+        TraceResult tr;
+        bool h = traverse<ANY, CAP>(S, r, stack + threadIdx.x, kBlock, tr);
+        if (ANY) {
+            flags[g] = h ? 1 : 0;
+        } else {
+            // Embree barycentrics: u weights v1 (V/det), v weights v2 (W/det).
+            const float u = h ? tr.V / tr.det : 0.0f;
+            const float v = h ? tr.W / tr.det : 0.0f;
+            hits[g] = make_float4(tr.t, bits_f(tr.tri), u, v);
+        }
+    }
+    count_rays(g < n ? 1u : 0u, rays_out);
+}
+
+uint32_t blocks_for(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
+
+}  // namespace
+
+int stack_capacity_for(int bound) {
+    if (bound <= 32) return 32;
+    if (bound <= 64) return 64;
+    return -1;
+}
+
+#define RL_DISPATCH_CAP(cap, ...)          \
+    do {                                   \
+        if ((cap) == 32) {                 \
+            constexpr int CAP = 32;        \
+            __VA_ARGS__;                   \
+        } else {                           \
+            constexpr int CAP = 64;        \
+            __VA_ARGS__;                   \
+        }                                  \
+    } while (0)
+
+hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+hipError_t launch_bake_direct(hipStream_t st, const SceneDev& S, int cap, const BakeSeg* segs,
+                              int nsegs, uint64_t total, const LightDev* lights, int nlights,
+                              const float4* lverts, unsigned long long* rays) {
+    if (total == 0) return hipSuccess;
+    RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_bake_direct<CAP>, dim3(blocks_for(total, kBlock)), dim3(kBlock), 0, st,
+                                            S, segs, nsegs, total, lights, nlights, lverts, rays));
+    return hipGetLastError();
+}
+
+hipError_t launch_ao_count(hipStream_t st, const SceneDev& S, int cap, const BakeSeg* segs,
+                           int nsegs, uint64_t total_pairs, const AoParams& P,
+                           unsigned long long* rays) {
+    if (total_pairs == 0) return hipSuccess;
+    RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_ao_count<CAP>, dim3(blocks_for(total_pairs, kBlock)), dim3(kBlock), 0,
+                                            st, S, segs, nsegs, total_pairs, P, rays));
+    return hipGetLastError();
+}
+
+hipError_t launch_ao_finalize(hipStream_t st, const BakeSeg* segs, int nsegs, uint64_t total,
+                              const AoParams& P) {
+    if (total == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ao_finalize, dim3(blocks_for(total, kBlock)), dim3(kBlock), 0, st, segs, nsegs,
+                       total, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_trace(hipStream_t st, const SceneDev& S, int cap, bool any, const float4* starts,
+                        const float4* ends, uint32_t n, uint8_t* flags, float4* hits,
+                        unsigned long long* rays) {
+    if (n == 0) return hipSuccess;
+    const dim3 grid(blocks_for(n, kBlock)), block(kBlock);
+    if (any) {
+        RL_DISPATCH_CAP(cap, hipLaunchKernelGGL((k_trace<true, CAP>), grid, block, 0, st, S, starts, ends, n, flags,
+                                                hits, rays));
+    } else {
+        RL_DISPATCH_CAP(cap, hipLaunchKernelGGL((k_trace<false, CAP>), grid, block, 0, st, S, starts, ends, n, flags,
+                                                hits, rays));
+    }
+    return hipGetLastError();
+}
+
+}  // namespace rl

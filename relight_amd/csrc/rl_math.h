@@ -1,0 +1,244 @@
+// Host/device arithmetic shared by the bake kernels and the host-side scene code.
+//
+// Every expression here is evaluated in the order the reference writes it, and the library is
+// compiled with -ffp-contract=off, so the host (lumel setup, light vertices) and the device
+// (ray setup, sampling, shading) produce the same bits for the same inputs. Division and sqrt are
+// IEEE correctly rounded on both sides (HIP's default for fp32 div/sqrt on gfx950).
+#pragma once
+
+#include <stdint.h>
+#include <math.h>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define RL_HD __host__ __device__ __forceinline__
+#else
+#define RL_HD inline
+#endif
+
+namespace rl {
+
+struct V3 {
+    float x, y, z;
+};
+
+RL_HD V3 v3(float x, float y, float z) { V3 r; r.x = x; r.y = y; r.z = z; return r; }
+RL_HD V3 operator+(V3 a, V3 b) { return v3(a.x + b.x, a.y + b.y, a.z + b.z); }
+RL_HD V3 operator-(V3 a, V3 b) { return v3(a.x - b.x, a.y - b.y, a.z - b.z); }
+RL_HD V3 operator-(V3 a) { return v3(-a.x, -a.y, -a.z); }
+RL_HD V3 operator*(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
+RL_HD V3 operator/(V3 a, float s) { return v3(a.x / s, a.y / s, a.z / s); }
+// Vec3::operator*(Vec3) is the dot product, summed left to right (Types.h:792-794).
+RL_HD float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+RL_HD float length(V3 a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
+// Vec3::normalize (Types.h:852-859): divide by the length unless it is zero, return the length.
+RL_HD float normalize(V3& a) {
+    float len = length(a);
+    if (len != 0.0f) { a.x /= len; a.y /= len; a.z /= len; }
+    return len;
+}
+RL_HD float comp(V3 a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
+
+// ---------------------------------------------------------------- random streams
+// PCG32 (M. E. O'Neill, "PCG: A Family of Simple Fast Space-Efficient Statistically Good
+// Algorithms for Random Number Generation", 2014): 64-bit LCG state, XSH-RR 32-bit output.
+// It replaces the reference's unseeded global rand() (Types.h:110-113), so every sample below is
+// a pure function of (seed, lightmap seed, texel, technique, sample).
+struct Pcg32 {
+    uint64_t state, inc;
+};
+
+#define RL_PCG_MULT 6364136223846793005ull
+
+RL_HD uint32_t pcg32_next(Pcg32& r) {
+    uint64_t old = r.state;
+    r.state = old * RL_PCG_MULT + r.inc;
+    uint32_t xs = (uint32_t)(((old >> 18) ^ old) >> 27);
+    uint32_t rot = (uint32_t)(old >> 59);
+    return (xs >> rot) | (xs << ((32u - rot) & 31u));
+}
+
+RL_HD Pcg32 pcg32_seed(uint64_t initstate, uint64_t initseq) {
+    Pcg32 r;
+    r.state = 0;
+    r.inc = (initseq << 1) | 1ull;
+    pcg32_next(r);
+    r.state += initstate;
+    pcg32_next(r);
+    return r;
+}
+
+// Uniform float in [0,1) from the top 24 bits (exact in fp32).
+RL_HD float pcg32_float(Pcg32& r) { return (float)(pcg32_next(r) >> 8) * (1.0f / 16777216.0f); }
+
+// SplitMix64 finaliser (Steele, Lea, Flood, "Fast splittable pseudorandom number generators",
+// OOPSLA 2014) to spread (seed, lightmap, texel) keys over PCG initial states.
+RL_HD uint64_t splitmix64(uint64_t z) {
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+enum Technique : uint32_t {
+    TECH_AO = 1,
+    TECH_FINAL_GATHER = 2,
+    TECH_PHOTON = 3,
+    TECH_FORM_FACTOR = 4,
+};
+
+// One independent stream per (bake seed, lightmap seed, texel, technique, sample).
+RL_HD Pcg32 sample_stream(uint32_t seed, uint32_t lm_seed, uint32_t texel, uint32_t tech,
+                          uint32_t sample) {
+    uint64_t key = splitmix64(((uint64_t)seed << 32) | (uint64_t)lm_seed);
+    key = splitmix64(key ^ (uint64_t)texel);
+    return pcg32_seed(key, ((uint64_t)tech << 32) | (uint64_t)sample);
+}
+
+// ---------------------------------------------------------------- hemisphere sampling
+enum Sampling : int32_t {
+    SAMPLE_UNIFORM = 0,  // the reference's rejection-in-cube + flip (Types.h:924-949)
+    SAMPLE_COSINE = 1,   // Malley's method over a rejection-sampled unit disk
+};
+
+// Vec3::randomDirection + randomHemisphereDirection (Types.h:924-949) with PCG draws. A zero
+// length draw is rejected as well (the reference would divide by zero there).
+RL_HD V3 sample_uniform_hemisphere(Pcg32& r, V3 n) {
+    V3 d;
+    float len;
+    do {
+        d.x = pcg32_float(r) * 2.0f - 1.0f;
+        d.y = pcg32_float(r) * 2.0f - 1.0f;
+        d.z = pcg32_float(r) * 2.0f - 1.0f;
+        len = length(d);
+    } while (len > 1.0f || len == 0.0f);
+    d = d / len;
+    if (dot(d, n) < 0.0f) d = -d;
+    return d;
+}
+
+// Cosine-weighted direction about n: a point (x,y) uniform in the unit disk lifted to
+// z = sqrt(1 - x^2 - y^2) (Malley), rotated into the frame of n with the branchless orthonormal
+// basis of Duff et al., "Building an Orthonormal Basis, Revisited", JCGT 6(1), 2017.
+RL_HD V3 sample_cosine_hemisphere(Pcg32& r, V3 n) {
+    float x, y, r2;
+    do {
+        x = pcg32_float(r) * 2.0f - 1.0f;
+        y = pcg32_float(r) * 2.0f - 1.0f;
+        r2 = x * x + y * y;
+    } while (r2 >= 1.0f);
+    float z = sqrtf(1.0f - r2);
+    float s = n.z >= 0.0f ? 1.0f : -1.0f;
+    float a = -1.0f / (s + n.z);
+    float b = n.x * n.y * a;
+    V3 t1 = v3(1.0f + s * n.x * n.x * a, s * b, -s * n.x);
+    V3 t2 = v3(b, s + n.y * n.y * a, -n.y);
+    return v3(t1.x * x + t2.x * y + n.x * z,
+              t1.y * x + t2.y * y + n.y * z,
+              t1.z * x + t2.z * y + n.z * z);
+}
+
+RL_HD V3 sample_hemisphere(Pcg32& r, V3 n, int mode) {
+    return mode == SAMPLE_COSINE ? sample_cosine_hemisphere(r, n) : sample_uniform_hemisphere(r, n);
+}
+
+// ---------------------------------------------------------------- rays
+// Segment query semantics of Embree::initializeRay (rt/Embree.cpp:107-130): direction is the
+// normalised end-start, the valid interval is (0.01, |end-start|).
+#define RL_TNEAR 0.01f
+
+struct Ray {
+    V3 o, d;
+    float tnear, tfar;
+};
+
+RL_HD Ray ray_from_segment(V3 start, V3 end) {
+    Ray r;
+    r.o = start;
+    r.d = end - start;
+    r.tfar = normalize(r.d);
+    r.tnear = RL_TNEAR;
+    return r;
+}
+
+// Per-ray constants of the watertight ray/triangle test (Woop, Benthin, Wald, "Watertight
+// Ray/Triangle Intersection", JCGT 2(1), 2013): the dominant axis kz, the two others kx, ky
+// (swapped when d[kz] < 0 to keep the winding), and the shear constants.
+struct WtRay {
+    V3 o;
+    int kx, ky, kz;
+    float Sx, Sy, Sz;
+    float tnear, tfar;
+};
+
+RL_HD WtRay wt_setup(const Ray& r) {
+    WtRay w;
+    w.o = r.o;
+    float ax = fabsf(r.d.x), ay = fabsf(r.d.y), az = fabsf(r.d.z);
+    int kz = ax > ay ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+    int kx = kz == 2 ? 0 : kz + 1;
+    int ky = kx == 2 ? 0 : kx + 1;
+    float dz = comp(r.d, kz);
+    if (dz < 0.0f) { int t = kx; kx = ky; ky = t; }
+    w.kx = kx; w.ky = ky; w.kz = kz;
+    w.Sx = comp(r.d, kx) / dz;
+    w.Sy = comp(r.d, ky) / dz;
+    w.Sz = 1.0f / dz;
+    w.tnear = r.tnear;
+    w.tfar = r.tfar;
+    return w;
+}
+
+// Returns true and the hit distance t when the triangle is hit with tnear < t < tfar. U, V, W are
+// the unnormalised barycentrics of v0, v1, v2; an exactly-zero edge function is recomputed in
+// double precision, as the paper prescribes, so shared edges are never missed.
+RL_HD bool wt_intersect(const WtRay& r, V3 v0, V3 v1, V3 v2, float& t, float& U, float& V,
+                        float& W, float& det) {
+    V3 A = v0 - r.o, B = v1 - r.o, C = v2 - r.o;
+    float Akz = comp(A, r.kz), Bkz = comp(B, r.kz), Ckz = comp(C, r.kz);
+    float Ax = comp(A, r.kx) - r.Sx * Akz;
+    float Ay = comp(A, r.ky) - r.Sy * Akz;
+    float Bx = comp(B, r.kx) - r.Sx * Bkz;
+    float By = comp(B, r.ky) - r.Sy * Bkz;
+    float Cx = comp(C, r.kx) - r.Sx * Ckz;
+    float Cy = comp(C, r.ky) - r.Sy * Ckz;
+    U = Cx * By - Cy * Bx;
+    V = Ax * Cy - Ay * Cx;
+    W = Bx * Ay - By * Ax;
+    if (U == 0.0f || V == 0.0f || W == 0.0f) {
+        U = (float)((double)Cx * (double)By - (double)Cy * (double)Bx);
+        V = (float)((double)Ax * (double)Cy - (double)Ay * (double)Cx);
+        W = (float)((double)Bx * (double)Ay - (double)By * (double)Ax);
+    }
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    det = U + V + W;
+    if (det == 0.0f) return false;
+    float Az = r.Sz * Akz, Bz = r.Sz * Bkz, Cz = r.Sz * Ckz;
+    float T = U * Az + V * Bz + W * Cz;
+    t = T / det;
+    return t > r.tnear && t < r.tfar;
+}
+
+// ---------------------------------------------------------------- light models
+// Light kinds of Light::create* (scene/Light.cpp:166-234).
+enum LightType : int32_t {
+    LIGHT_POINT = 0,
+    LIGHT_SPOT = 1,
+    LIGHT_DIRECTIONAL = 2,
+    LIGHT_AREA = 3,
+};
+
+// LinearLightAttenuation::calculate (scene/Light.cpp:496-500).
+RL_HD float attenuation(float distance, float radius, float c, float l, float q) {
+    float r = distance / radius;
+    float v = 1.0f / (1.0f + c + l * r + q * r * r);
+    return v > 0.0f ? v : 0.0f;
+}
+
+// LightInfluence::lambert (scene/Light.cpp:392-396).
+RL_HD float lambert(V3 dir, V3 n) {
+    float dp = dot(dir, n);
+    return dp < 0.0f ? 0.0f : dp;
+}
+
+}  // namespace rl

@@ -1,0 +1,847 @@
+// Host runtime behind include/relight.h: context, scene state machine, lumel rasterisation,
+// device residency and the bake entry points.
+//
+// Mirrors Relight (Relight.cpp:174-271), Scene (scene/Scene.cpp:90-142), Lightmap
+// (Lightmap.cpp:55-164,332-349) and the tracer seam (rt/Tracer.h:75-108). Host arithmetic that
+// feeds the kernels (lumel positions and normals, light vertices) follows the reference's
+// expression order and is compiled with -ffp-contract=off.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/relight.h"
+#include "rl_bvh.h"
+#include "rl_device.h"
+#include "rl_kernels.h"
+#include "rl_math.h"
+
+using namespace rl;
+
+namespace {
+
+thread_local std::string g_error;
+
+int32_t fail(int32_t code, const std::string& msg) {
+    g_error = msg;
+    return code;
+}
+
+#define RL_HIP(call)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (call);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            return fail(e_ == hipErrorOutOfMemory ? RL_OUT_OF_MEMORY : RL_DEVICE_ERROR,  \
+                        std::string(#call) + ": " + hipGetErrorString(e_));              \
+        }                                                                                \
+    } while (0)
+
+template <class T>
+hipError_t dev_alloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) return hipSuccess;
+    return hipMalloc((void**)p, count * sizeof(T));
+}
+
+template <class T>
+void dev_free(T*& p) {
+    if (p) hipFree((void*)p);
+    p = nullptr;
+}
+
+struct MeshHost {
+    std::vector<V3> pos, nrm;
+    std::vector<float> uv;  // lightmap uv, 2 per vertex
+    std::vector<uint32_t> idx;
+    float albedo[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+    uint32_t first_tri = 0;
+    uint32_t num_tris = 0;
+    rl_lightmap* lightmap = nullptr;
+};
+
+struct LightHost {
+    LightDev dev;
+    std::vector<V3> verts;
+};
+
+enum SceneState { kInitial = 0, kAdding = 1, kReady = 2 };
+
+}  // namespace
+
+struct rl_relight {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    unsigned long long* d_rays = nullptr;
+    rl_stats stats{};
+    uint32_t next_lm_seed = 0;
+    // Event pairs around each kernel launch of the current call: (slot, start, end).
+    std::vector<hipEvent_t> pool;
+    size_t pool_used = 0;
+    struct Mark { int slot; hipEvent_t a, b; };
+    std::vector<Mark> marks;
+};
+
+struct rl_scene {
+    rl_relight* ctx = nullptr;
+    int state = kInitial;
+    std::vector<MeshHost> meshes;
+    std::vector<LightHost> lights;
+    uint32_t num_tris = 0;
+    uint32_t num_nodes = 0;
+    int stack_bound = 0;
+    int cap = 0;
+    Bvh4Node* d_nodes = nullptr;
+    float4* d_tris = nullptr;
+    TriAttr* d_attr = nullptr;
+    LightDev* d_lights = nullptr;
+    float4* d_lverts = nullptr;
+    std::vector<uint32_t> tri_mesh, tri_face;
+
+    SceneDev dev() const {
+        SceneDev s;
+        s.nodes = d_nodes;
+        s.tris = d_tris;
+        s.attr = d_attr;
+        s.num_nodes = num_nodes;
+        s.num_tris = num_tris;
+        return s;
+    }
+    ~rl_scene() {
+        dev_free(d_nodes);
+        dev_free(d_tris);
+        dev_free(d_attr);
+        dev_free(d_lights);
+        dev_free(d_lverts);
+    }
+};
+
+struct rl_lightmap {
+    rl_relight* ctx = nullptr;
+    rl_scene* scene = nullptr;
+    int w = 0, h = 0;
+    uint32_t seed = 0;
+    // Host lumel grid (Lightmap::m_lumels).
+    std::vector<uint8_t> valid;
+    std::vector<uint32_t> tri;
+    std::vector<V3> pos, nrm;
+    std::vector<float> color;  // 3 per texel
+    // Device copy of the valid lumels, compacted in texel order.
+    bool dirty = true;
+    bool device_newer = false;
+    std::vector<uint32_t> texels;
+    float4* d_pos = nullptr;
+    float4* d_nrm = nullptr;
+    float4* d_color = nullptr;
+    uint32_t* d_counts = nullptr;
+    uint32_t d_capacity = 0;
+
+    ~rl_lightmap() { release(); }
+    void release() {
+        dev_free(d_pos);
+        dev_free(d_nrm);
+        dev_free(d_color);
+        dev_free(d_counts);
+        d_capacity = 0;
+    }
+};
+
+namespace {
+
+// Copies device colours back into the host grid when a bake has run since the last upload.
+int32_t pull_colors(rl_lightmap* lm) {
+    if (!lm->device_newer) return RL_SUCCESS;
+    const size_t n = lm->texels.size();
+    std::vector<float4> c(n);
+    if (n) RL_HIP(hipMemcpy(c.data(), lm->d_color, n * sizeof(float4), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) {
+        float* dst = &lm->color[3 * (size_t)lm->texels[i]];
+        dst[0] = c[i].x; dst[1] = c[i].y; dst[2] = c[i].z;
+    }
+    lm->device_newer = false;
+    return RL_SUCCESS;
+}
+
+// Uploads the valid lumels (position, normal, colour) when the host grid changed.
+int32_t push_lumels(rl_lightmap* lm) {
+    if (!lm->dirty) return RL_SUCCESS;
+    lm->texels.clear();
+    const uint32_t nt = (uint32_t)lm->w * (uint32_t)lm->h;
+    for (uint32_t t = 0; t < nt; t++)
+        if (lm->valid[t]) lm->texels.push_back(t);
+    const uint32_t n = (uint32_t)lm->texels.size();
+    if (n > lm->d_capacity) {
+        lm->release();
+        RL_HIP(dev_alloc(&lm->d_pos, n));
+        RL_HIP(dev_alloc(&lm->d_nrm, n));
+        RL_HIP(dev_alloc(&lm->d_color, n));
+        RL_HIP(dev_alloc(&lm->d_counts, n));
+        lm->d_capacity = n;
+        RL_HIP(launch_fill_u32(lm->ctx->stream, lm->d_counts, n, 0u));
+    }
+    std::vector<float4> p(n), q(n), c(n);
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t t = lm->texels[i];
+        uint32_t tb = t, fb = lm->tri[t];
+        float tw, fw;
+        std::memcpy(&tw, &tb, 4);
+        std::memcpy(&fw, &fb, 4);
+        p[i] = make_float4(lm->pos[t].x, lm->pos[t].y, lm->pos[t].z, tw);
+        q[i] = make_float4(lm->nrm[t].x, lm->nrm[t].y, lm->nrm[t].z, fw);
+        c[i] = make_float4(lm->color[3 * t], lm->color[3 * t + 1], lm->color[3 * t + 2], 0.0f);
+    }
+    if (n) {
+        RL_HIP(hipMemcpy(lm->d_pos, p.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+        RL_HIP(hipMemcpy(lm->d_nrm, q.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+        RL_HIP(hipMemcpy(lm->d_color, c.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    }
+    lm->dirty = false;
+    return RL_SUCCESS;
+}
+
+// Lightmap::initializeLumels(const Face&) (Lightmap.cpp:118-154) with Face::uvRect /
+// isUvInside / positionAt / normalAt (scene/Mesh.cpp:397-476) for face f of mesh m.
+void raster_face(rl_lightmap* lm, const MeshHost& m, uint32_t f, uint32_t tri_id) {
+    const uint32_t ia = m.idx[3 * f], ib = m.idx[3 * f + 1], ic = m.idx[3 * f + 2];
+    const float ax = m.uv[2 * ia], ay = m.uv[2 * ia + 1];
+    const float bx = m.uv[2 * ib], by = m.uv[2 * ib + 1];
+    const float cx = m.uv[2 * ic], cy = m.uv[2 * ic + 1];
+    const float minx = std::min(ax, std::min(bx, cx)), maxx = std::max(ax, std::max(bx, cx));
+    const float miny = std::min(ay, std::min(by, cy)), maxy = std::max(ay, std::max(by, cy));
+    const int x1 = (int)(minx * lm->w), x2 = (int)(maxx * lm->w);
+    const int y1 = (int)(miny * lm->h), y2 = (int)(maxy * lm->h);
+    const V3 pa = m.pos[ia], pb = m.pos[ib], pc = m.pos[ic];
+    const V3 na = m.nrm[ia], nb = m.nrm[ib], nc = m.nrm[ic];
+    for (int v = std::max(y1, 0); v <= std::min(y2, lm->h - 1); v++) {
+        for (int u = std::max(x1, 0); u <= std::min(x2, lm->w - 1); u++) {
+            const float uvx = (u + 0.5f) / float(lm->w), uvy = (v + 0.5f) / float(lm->h);
+            const float v0x = cx - ax, v0y = cy - ay;
+            const float v1x = bx - ax, v1y = by - ay;
+            const float v2x = uvx - ax, v2y = uvy - ay;
+            const float dot00 = v0x * v0x + v0y * v0y;
+            const float dot01 = v0x * v1x + v0y * v1y;
+            const float dot02 = v0x * v2x + v0y * v2y;
+            const float dot11 = v1x * v1x + v1y * v1y;
+            const float dot12 = v1x * v2x + v1y * v2y;
+            const float inv = 1.0f / (dot00 * dot11 - dot01 * dot01);
+            const float bu = (dot11 * dot02 - dot01 * dot12) * inv;
+            const float bv = (dot00 * dot12 - dot01 * dot02) * inv;
+            if (!((bu >= 0.0f) && (bv >= 0.0f) && (bu + bv < 1.0f))) continue;
+            const size_t t = (size_t)v * lm->w + u;
+            lm->valid[t] = 1;
+            lm->tri[t] = tri_id;
+            lm->pos[t] = v3(pa.x + (pb.x - pa.x) * bv + (pc.x - pa.x) * bu,
+                            pa.y + (pb.y - pa.y) * bv + (pc.y - pa.y) * bu,
+                            pa.z + (pb.z - pa.z) * bv + (pc.z - pa.z) * bu);
+            lm->nrm[t] = v3(na.x + (nb.x - na.x) * bv + (nc.x - na.x) * bu,
+                            na.y + (nb.y - na.y) * bv + (nc.y - na.y) * bu,
+                            na.z + (nb.z - na.z) * bv + (nc.z - na.z) * bu);
+            lm->color[3 * t] = lm->color[3 * t + 1] = lm->color[3 * t + 2] = 0.0f;
+        }
+    }
+}
+
+struct CallTimer {
+    rl_relight* ctx;
+    std::chrono::steady_clock::time_point t0;
+    explicit CallTimer(rl_relight* c) : ctx(c), t0(std::chrono::steady_clock::now()) {}
+};
+
+hipEvent_t pool_event(rl_relight* ctx) {
+    if (ctx->pool_used == ctx->pool.size()) {
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        ctx->pool.push_back(e);
+    }
+    return ctx->pool[ctx->pool_used++];
+}
+
+// Records the start event of a kernel launch in `slot`; kernel_end records its end.
+hipError_t kernel_begin(rl_relight* ctx, int slot) {
+    rl_relight::Mark m{slot, pool_event(ctx), pool_event(ctx)};
+    if (!m.a || !m.b) return hipErrorOutOfMemory;
+    ctx->marks.push_back(m);
+    return hipEventRecord(m.a, ctx->stream);
+}
+
+hipError_t kernel_end(rl_relight* ctx) { return hipEventRecord(ctx->marks.back().b, ctx->stream); }
+
+int32_t begin_call(rl_relight* ctx) {
+    ctx->stats = rl_stats{};
+    ctx->pool_used = 0;
+    ctx->marks.clear();
+    RL_HIP(launch_fill_u32(ctx->stream, reinterpret_cast<uint32_t*>(ctx->d_rays), 2, 0u));
+    RL_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+    return RL_SUCCESS;
+}
+
+int32_t end_call(rl_relight* ctx, const CallTimer& tm) {
+    RL_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+    RL_HIP(hipEventSynchronize(ctx->ev1));
+    float ms = 0.0f;
+    RL_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+    unsigned long long rays = 0;
+    RL_HIP(hipMemcpy(&rays, ctx->d_rays, sizeof(rays), hipMemcpyDeviceToHost));
+    ctx->stats.rays = rays;
+    ctx->stats.gpu_ms = ms;
+    for (const rl_relight::Mark& m : ctx->marks) {
+        float km = 0.0f;
+        RL_HIP(hipEventElapsedTime(&km, m.a, m.b));
+        ctx->stats.kernel_ms[m.slot] += km;
+        ctx->stats.kernel_launches[m.slot] += 1;
+    }
+    ctx->stats.wall_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tm.t0).count();
+    return RL_SUCCESS;
+}
+
+int32_t check_scene_ready(rl_scene* scene) {
+    if (!scene) return fail(RL_INVALID_ARGUMENT, "scene is NULL");
+    if (scene->state != kReady) return fail(RL_INVALID_CALL, "scene has not been ended");
+    return RL_SUCCESS;
+}
+
+// Builds the device segment table for a set of lightmaps; `per_lumel` work items per lumel.
+int32_t make_segments(rl_relight* ctx, rl_scene* scene, rl_lightmap* const* maps, int n,
+                      uint64_t per_lumel, BakeSeg** d_segs, uint64_t* total, uint64_t* texels) {
+    std::vector<BakeSeg> segs;
+    uint64_t acc = 0, tex = 0;
+    for (int i = 0; i < n; i++) {
+        rl_lightmap* lm = maps[i];
+        if (!lm) return fail(RL_INVALID_ARGUMENT, "lightmap is NULL");
+        if (lm->scene != scene) return fail(RL_INVALID_CALL, "lightmap holds no mesh of this scene");
+        int32_t st = push_lumels(lm);
+        if (st) return st;
+        BakeSeg s;
+        s.pos = lm->d_pos;
+        s.nrm = lm->d_nrm;
+        s.color = lm->d_color;
+        s.counts = lm->d_counts;
+        s.count = (uint32_t)lm->texels.size();
+        s.lm_seed = lm->seed;
+        s.work_begin = acc;
+        if (s.count == 0) continue;
+        segs.push_back(s);
+        acc += (uint64_t)s.count * per_lumel;
+        tex += s.count;
+    }
+    *total = acc;
+    *texels = tex;
+    *d_segs = nullptr;
+    if (segs.empty()) return RL_SUCCESS;
+    RL_HIP(dev_alloc(d_segs, segs.size()));
+    RL_HIP(hipMemcpy(*d_segs, segs.data(), segs.size() * sizeof(BakeSeg), hipMemcpyHostToDevice));
+    (void)ctx;
+    return RL_SUCCESS;
+}
+
+int32_t run_direct(rl_relight* ctx, rl_scene* scene, rl_lightmap* const* maps, int n) {
+    BakeSeg* d_segs = nullptr;
+    uint64_t total = 0, tex = 0;
+    int32_t st = make_segments(ctx, scene, maps, n, 1, &d_segs, &total, &tex);
+    if (st) return st;
+    int nsegs = 0;
+    for (int i = 0; i < n; i++) nsegs += maps[i]->texels.empty() ? 0 : 1;
+    hipError_t e = hipSuccess;
+    if (total) {
+        e = kernel_begin(ctx, RL_KERNEL_DIRECT);
+        if (e == hipSuccess)
+            e = launch_bake_direct(ctx->stream, scene->dev(), scene->cap, d_segs, nsegs, total,
+                                   scene->d_lights, (int)scene->lights.size(), scene->d_lverts,
+                                   ctx->d_rays);
+        if (e == hipSuccess) e = kernel_end(ctx);
+    }
+    ctx->stats.texels += tex;
+    ctx->stats.launches += total ? 1 : 0;
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    dev_free(d_segs);
+    if (e != hipSuccess) return fail(RL_DEVICE_ERROR, std::string("direct bake: ") + hipGetErrorString(e));
+    for (int i = 0; i < n; i++) maps[i]->device_newer = true;
+    return RL_SUCCESS;
+}
+
+int32_t run_ao(rl_relight* ctx, rl_scene* scene, rl_lightmap* const* maps, int n,
+               const rl_ao_settings* s) {
+    if (!s) return fail(RL_INVALID_ARGUMENT, "AO settings are NULL");
+    if (s->samples <= 0) return fail(RL_INVALID_ARGUMENT, "AO sample count must be positive");
+    if (s->sampling != RL_SAMPLE_UNIFORM && s->sampling != RL_SAMPLE_COSINE)
+        return fail(RL_INVALID_ARGUMENT, "unknown sampling mode");
+    AoParams P;
+    P.samples = s->samples;
+    P.occluded_fraction = s->occluded_fraction;
+    P.max_distance = s->max_distance;
+    P.exponent = s->exponent;
+    P.seed = s->seed;
+    P.sampling = s->sampling;
+    BakeSeg* d_pairs = nullptr;
+    uint64_t pairs = 0, tex = 0;
+    int32_t st = make_segments(ctx, scene, maps, n, (uint64_t)s->samples, &d_pairs, &pairs, &tex);
+    if (st) return st;
+    BakeSeg* d_lum = nullptr;
+    uint64_t lumels = 0, tex2 = 0;
+    st = make_segments(ctx, scene, maps, n, 1, &d_lum, &lumels, &tex2);
+    if (st) {
+        dev_free(d_pairs);
+        return st;
+    }
+    int nsegs = 0;
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < n; i++) {
+        nsegs += maps[i]->texels.empty() ? 0 : 1;
+        if (e == hipSuccess) e = launch_fill_u32(ctx->stream, maps[i]->d_counts, maps[i]->texels.size(), 0u);
+    }
+    if (pairs) {
+        if (e == hipSuccess) e = kernel_begin(ctx, RL_KERNEL_AO_COUNT);
+        if (e == hipSuccess)
+            e = launch_ao_count(ctx->stream, scene->dev(), scene->cap, d_pairs, nsegs, pairs, P, ctx->d_rays);
+        if (e == hipSuccess) e = kernel_end(ctx);
+        if (e == hipSuccess) e = kernel_begin(ctx, RL_KERNEL_AO_FINALIZE);
+        if (e == hipSuccess) e = launch_ao_finalize(ctx->stream, d_lum, nsegs, lumels, P);
+        if (e == hipSuccess) e = kernel_end(ctx);
+    }
+    ctx->stats.texels += tex;
+    ctx->stats.launches += pairs ? 2 : 0;
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    dev_free(d_pairs);
+    dev_free(d_lum);
+    if (e != hipSuccess) return fail(RL_DEVICE_ERROR, std::string("AO bake: ") + hipGetErrorString(e));
+    for (int i = 0; i < n; i++) maps[i]->device_newer = true;
+    return RL_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* rl_last_error(void) { return g_error.c_str(); }
+
+int32_t rl_create(rl_relight** out) {
+    if (!out) return fail(RL_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+        return fail(RL_DEVICE_ERROR, "no HIP device is available");
+    rl_relight* ctx = new (std::nothrow) rl_relight();
+    if (!ctx) return fail(RL_OUT_OF_MEMORY, "context allocation failed");
+    hipGetDevice(&ctx->device);
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_rays, sizeof(unsigned long long)) != hipSuccess) {
+        delete ctx;
+        return fail(RL_DEVICE_ERROR, "stream/event/counter creation failed");
+    }
+    *out = ctx;
+    return RL_SUCCESS;
+}
+
+int32_t rl_destroy(rl_relight* ctx) {
+    if (!ctx) return RL_SUCCESS;
+    if (ctx->d_rays) hipFree(ctx->d_rays);
+    if (ctx->ev0) hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) hipEventDestroy(ctx->ev1);
+    for (hipEvent_t e : ctx->pool) hipEventDestroy(e);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return RL_SUCCESS;
+}
+
+int32_t rl_scene_create(rl_relight* ctx, rl_scene** out) {
+    if (!ctx || !out) return fail(RL_INVALID_ARGUMENT, "NULL argument");
+    rl_scene* s = new (std::nothrow) rl_scene();
+    if (!s) return fail(RL_OUT_OF_MEMORY, "scene allocation failed");
+    s->ctx = ctx;
+    *out = s;
+    return RL_SUCCESS;
+}
+
+int32_t rl_scene_destroy(rl_scene* scene) {
+    delete scene;
+    return RL_SUCCESS;
+}
+
+int32_t rl_scene_begin(rl_scene* scene) {
+    if (!scene) return fail(RL_INVALID_ARGUMENT, "scene is NULL");
+    if (scene->state != kInitial) return fail(RL_INVALID_CALL, "Scene::begin called twice");
+    scene->state = kAdding;
+    return RL_SUCCESS;
+}
+
+int32_t rl_scene_add_mesh(rl_scene* scene, const float* positions, const float* normals,
+                          const float* lightmap_uvs, uint32_t vertex_count,
+                          const uint32_t* indices, uint32_t index_count, const float* transform,
+                          const float* albedo, int32_t* out_mesh) {
+    if (!scene) return fail(RL_INVALID_ARGUMENT, "scene is NULL");
+    if (scene->state != kAdding) return fail(RL_INVALID_CALL, "add_mesh outside begin/end");
+    if (!positions || !normals || !lightmap_uvs || !indices)
+        return fail(RL_INVALID_ARGUMENT, "mesh arrays must not be NULL");
+    if (index_count % 3 != 0) return fail(RL_INVALID_ARGUMENT, "index count is not a multiple of 3");
+    for (uint32_t i = 0; i < index_count; i++)
+        if (indices[i] >= vertex_count) return fail(RL_INVALID_ARGUMENT, "index out of range");
+    MeshHost m;
+    m.pos.resize(vertex_count);
+    m.nrm.resize(vertex_count);
+    m.uv.assign(lightmap_uvs, lightmap_uvs + 2 * (size_t)vertex_count);
+    m.idx.assign(indices, indices + index_count);
+    for (uint32_t i = 0; i < vertex_count; i++) {
+        V3 p = v3(positions[3 * i], positions[3 * i + 1], positions[3 * i + 2]);
+        V3 n = v3(normals[3 * i], normals[3 * i + 1], normals[3 * i + 2]);
+        if (transform) {
+            // Mesh::transform (scene/Mesh.cpp:277-284): Matrix4 * Vec3 and Matrix4::rotate
+            // (Types.h:1188-1196,1303-1312), then the normal is renormalised.
+            const float* M = transform;
+            V3 tp = v3(p.x * M[0] + p.y * M[4] + p.z * M[8] + M[12],
+                       p.x * M[1] + p.y * M[5] + p.z * M[9] + M[13],
+                       p.x * M[2] + p.y * M[6] + p.z * M[10] + M[14]);
+            V3 tn = v3(n.x * M[0] + n.y * M[4] + n.z * M[8],
+                       n.x * M[1] + n.y * M[5] + n.z * M[9],
+                       n.x * M[2] + n.y * M[6] + n.z * M[10]);
+            normalize(tn);
+            p = tp;
+            n = tn;
+        }
+        m.pos[i] = p;
+        m.nrm[i] = n;
+    }
+    if (albedo) std::memcpy(m.albedo, albedo, sizeof(m.albedo));
+    m.num_tris = index_count / 3;
+    scene->meshes.push_back(std::move(m));
+    if (out_mesh) *out_mesh = (int32_t)scene->meshes.size() - 1;
+    return RL_SUCCESS;
+}
+
+int32_t rl_scene_add_light(rl_scene* scene, const rl_light_desc* d, int32_t* out_light) {
+    if (!scene || !d) return fail(RL_INVALID_ARGUMENT, "NULL argument");
+    if (scene->state == kReady) return fail(RL_INVALID_CALL, "add_light after end");
+    if (d->type < RL_LIGHT_POINT || d->type > RL_LIGHT_AREA) return fail(RL_INVALID_ARGUMENT, "unknown light type");
+    LightHost L;
+    std::memset(&L.dev, 0, sizeof(L.dev));
+    L.dev.type = d->type;
+    L.dev.casts_shadow = d->casts_shadow ? 1 : 0;
+    for (int k = 0; k < 3; k++) {
+        L.dev.pos[k] = d->position[k];
+        L.dev.dir[k] = d->direction[k];
+        L.dev.color[k] = d->color[k];
+    }
+    L.dev.intensity = d->intensity;
+    L.dev.has_attenuation = (d->type != RL_LIGHT_DIRECTIONAL && d->has_attenuation) ? 1 : 0;
+    L.dev.radius = d->radius;
+    L.dev.att_c = d->att_constant;
+    L.dev.att_l = d->att_linear;
+    L.dev.att_q = d->att_quadratic;
+    L.dev.has_spot = d->type == RL_LIGHT_SPOT ? 1 : 0;
+    L.dev.cutoff = d->spot_cutoff;
+    L.dev.spot_exponent = d->spot_exponent;
+    if (d->type == RL_LIGHT_AREA) {
+        if (!d->area_positions || !d->area_indices || d->area_index_count % 3 != 0 || d->area_index_count == 0)
+            return fail(RL_INVALID_ARGUMENT, "area light needs a triangle mesh");
+        for (uint32_t i = 0; i < d->area_index_count; i++)
+            if (d->area_indices[i] >= d->area_vertex_count) return fail(RL_INVALID_ARGUMENT, "area index out of range");
+        // FaceLightVertexGenerator::generateFromTriangle at subdivision 0 pushes
+        // Triangle(face).centroid(); Triangle(face) takes face.vertex(0..2) = a, c, b
+        // (scene/Mesh.cpp:322-327,427-437), so the sum is (a + c) + b.
+        const float* P = d->area_positions;
+        for (uint32_t f = 0; f < d->area_index_count / 3; f++) {
+            const uint32_t ia = d->area_indices[3 * f], ib = d->area_indices[3 * f + 1], ic = d->area_indices[3 * f + 2];
+            V3 a = v3(P[3 * ia], P[3 * ia + 1], P[3 * ia + 2]);
+            V3 b = v3(P[3 * ib], P[3 * ib + 1], P[3 * ib + 2]);
+            V3 c = v3(P[3 * ic], P[3 * ic + 1], P[3 * ic + 2]);
+            L.verts.push_back((a + c + b) / 3.0f);
+        }
+    }
+    scene->lights.push_back(std::move(L));
+    if (out_light) *out_light = (int32_t)scene->lights.size() - 1;
+    return RL_SUCCESS;
+}
+
+int32_t rl_scene_end(rl_scene* scene) {
+    if (!scene) return fail(RL_INVALID_ARGUMENT, "scene is NULL");
+    if (scene->state != kAdding) return fail(RL_INVALID_CALL, "Scene::end without begin");
+    uint64_t ntri = 0;
+    for (MeshHost& m : scene->meshes) {
+        m.first_tri = (uint32_t)ntri;
+        ntri += m.num_tris;
+    }
+    if (ntri == 0) return fail(RL_INVALID_CALL, "scene has no triangles");
+    if (ntri > RL_LEAF_FIRST_MASK) return fail(RL_REJECTED, "too many triangles");
+    std::vector<float> tri9((size_t)ntri * 9);
+    std::vector<TriAttr> attr((size_t)ntri);
+    scene->tri_mesh.resize(ntri);
+    scene->tri_face.resize(ntri);
+    for (uint32_t mi = 0; mi < scene->meshes.size(); mi++) {
+        const MeshHost& m = scene->meshes[mi];
+        for (uint32_t f = 0; f < m.num_tris; f++) {
+            const uint32_t t = m.first_tri + f;
+            for (int k = 0; k < 3; k++) {
+                const uint32_t vi = m.idx[3 * f + k];
+                tri9[9 * (size_t)t + 3 * k + 0] = m.pos[vi].x;
+                tri9[9 * (size_t)t + 3 * k + 1] = m.pos[vi].y;
+                tri9[9 * (size_t)t + 3 * k + 2] = m.pos[vi].z;
+                attr[t].uv[2 * k] = m.uv[2 * vi];
+                attr[t].uv[2 * k + 1] = m.uv[2 * vi + 1];
+                attr[t].n[3 * k] = m.nrm[vi].x;
+                attr[t].n[3 * k + 1] = m.nrm[vi].y;
+                attr[t].n[3 * k + 2] = m.nrm[vi].z;
+            }
+            attr[t].mesh = mi;
+            attr[t].face = f;
+            std::memcpy(attr[t].albedo, m.albedo, sizeof(m.albedo));
+            scene->tri_mesh[t] = mi;
+            scene->tri_face[t] = f;
+        }
+    }
+    Bvh4Build bvh;
+    int rc = build_bvh4(tri9.data(), (uint32_t)ntri, bvh);
+    if (rc) return fail(RL_REJECTED, "BVH build failed");
+    int cap = stack_capacity_for(bvh.stack_bound);
+    if (cap < 0) return fail(RL_REJECTED, "BVH traversal stack bound exceeds the LDS stack");
+    std::vector<float4> tris((size_t)ntri * 3);
+    for (uint32_t i = 0; i < ntri; i++) {
+        const uint32_t t = bvh.order[i];
+        const float* v = &tri9[9 * (size_t)t];
+        float idw;
+        std::memcpy(&idw, &t, 4);
+        tris[3 * (size_t)i + 0] = make_float4(v[0], v[1], v[2], idw);
+        tris[3 * (size_t)i + 1] = make_float4(v[3], v[4], v[5], 0.0f);
+        tris[3 * (size_t)i + 2] = make_float4(v[6], v[7], v[8], 0.0f);
+    }
+    std::vector<LightDev> lights;
+    std::vector<float4> lverts;
+    for (LightHost& L : scene->lights) {
+        L.dev.num_vertices = (int32_t)L.verts.size();
+        L.dev.vertex_offset = (int32_t)lverts.size();
+        for (const V3& v : L.verts) lverts.push_back(make_float4(v.x, v.y, v.z, 0.0f));
+        lights.push_back(L.dev);
+    }
+    RL_HIP(dev_alloc(&scene->d_nodes, bvh.nodes.size()));
+    RL_HIP(dev_alloc(&scene->d_tris, tris.size()));
+    RL_HIP(dev_alloc(&scene->d_attr, attr.size()));
+    RL_HIP(hipMemcpy(scene->d_nodes, bvh.nodes.data(), bvh.nodes.size() * sizeof(Bvh4Node), hipMemcpyHostToDevice));
+    RL_HIP(hipMemcpy(scene->d_tris, tris.data(), tris.size() * sizeof(float4), hipMemcpyHostToDevice));
+    RL_HIP(hipMemcpy(scene->d_attr, attr.data(), attr.size() * sizeof(TriAttr), hipMemcpyHostToDevice));
+    if (!lights.empty()) {
+        RL_HIP(dev_alloc(&scene->d_lights, lights.size()));
+        RL_HIP(hipMemcpy(scene->d_lights, lights.data(), lights.size() * sizeof(LightDev), hipMemcpyHostToDevice));
+    }
+    if (!lverts.empty()) {
+        RL_HIP(dev_alloc(&scene->d_lverts, lverts.size()));
+        RL_HIP(hipMemcpy(scene->d_lverts, lverts.data(), lverts.size() * sizeof(float4), hipMemcpyHostToDevice));
+    }
+    scene->num_tris = (uint32_t)ntri;
+    scene->num_nodes = (uint32_t)bvh.nodes.size();
+    scene->stack_bound = bvh.stack_bound;
+    scene->cap = cap;
+    scene->state = kReady;
+    return RL_SUCCESS;
+}
+
+int32_t rl_scene_info(const rl_scene* scene, uint32_t* triangles, uint32_t* nodes, int32_t* stack_bound) {
+    if (!scene) return fail(RL_INVALID_ARGUMENT, "scene is NULL");
+    if (triangles) *triangles = scene->num_tris;
+    if (nodes) *nodes = scene->num_nodes;
+    if (stack_bound) *stack_bound = scene->stack_bound;
+    return RL_SUCCESS;
+}
+
+int32_t rl_lightmap_create(rl_relight* ctx, int32_t width, int32_t height, rl_lightmap** out) {
+    if (!ctx || !out) return fail(RL_INVALID_ARGUMENT, "NULL argument");
+    if (width <= 0 || height <= 0 || (int64_t)width * height > (1ll << 31))
+        return fail(RL_INVALID_ARGUMENT, "bad lightmap size");
+    rl_lightmap* lm = new (std::nothrow) rl_lightmap();
+    if (!lm) return fail(RL_OUT_OF_MEMORY, "lightmap allocation failed");
+    lm->ctx = ctx;
+    lm->w = width;
+    lm->h = height;
+    lm->seed = ctx->next_lm_seed++;
+    const size_t n = (size_t)width * height;
+    lm->valid.assign(n, 0);
+    lm->tri.assign(n, 0xFFFFFFFFu);
+    lm->pos.assign(n, v3(0, 0, 0));
+    lm->nrm.assign(n, v3(0, 0, 0));
+    lm->color.assign(3 * n, 0.0f);
+    *out = lm;
+    return RL_SUCCESS;
+}
+
+int32_t rl_lightmap_destroy(rl_lightmap* lm) {
+    delete lm;
+    return RL_SUCCESS;
+}
+
+int32_t rl_lightmap_set_seed(rl_lightmap* lm, uint32_t seed) {
+    if (!lm) return fail(RL_INVALID_ARGUMENT, "lightmap is NULL");
+    lm->seed = seed;
+    return RL_SUCCESS;
+}
+
+int32_t rl_lightmap_add_mesh(rl_lightmap* lm, rl_scene* scene, int32_t mesh) {
+    if (!lm || !scene) return fail(RL_INVALID_ARGUMENT, "NULL argument");
+    if (mesh < 0 || mesh >= (int32_t)scene->meshes.size()) return fail(RL_INVALID_ARGUMENT, "bad mesh index");
+    if (lm->scene && lm->scene != scene) return fail(RL_INVALID_CALL, "lightmap already holds another scene's meshes");
+    MeshHost& m = scene->meshes[mesh];
+    // Lightmap::addMesh refuses a mesh that already has a lightmap (Lightmap.cpp:95-97).
+    if (m.lightmap) return fail(RL_INVALID_CALL, "mesh already has a lightmap");
+    int32_t st = pull_colors(lm);
+    if (st) return st;
+    m.lightmap = lm;
+    lm->scene = scene;
+    // The scene-order triangle id is assigned at end(); until then faces are numbered per mesh.
+    uint32_t first = 0;
+    for (int32_t i = 0; i < mesh; i++) first += scene->meshes[i].num_tris;
+    for (uint32_t f = 0; f < m.num_tris; f++) raster_face(lm, m, f, first + f);
+    lm->dirty = true;
+    return RL_SUCCESS;
+}
+
+int32_t rl_lightmap_valid_count(const rl_lightmap* lm, uint32_t* count) {
+    if (!lm || !count) return fail(RL_INVALID_ARGUMENT, "NULL argument");
+    uint32_t c = 0;
+    for (uint8_t v : lm->valid) c += v;
+    *count = c;
+    return RL_SUCCESS;
+}
+
+int32_t rl_lightmap_read_lumels(rl_lightmap* lm, uint32_t* texel, float* position, float* normal) {
+    if (!lm) return fail(RL_INVALID_ARGUMENT, "lightmap is NULL");
+    size_t k = 0;
+    for (size_t t = 0; t < lm->valid.size(); t++) {
+        if (!lm->valid[t]) continue;
+        if (texel) texel[k] = (uint32_t)t;
+        if (position) { position[3 * k] = lm->pos[t].x; position[3 * k + 1] = lm->pos[t].y; position[3 * k + 2] = lm->pos[t].z; }
+        if (normal) { normal[3 * k] = lm->nrm[t].x; normal[3 * k + 1] = lm->nrm[t].y; normal[3 * k + 2] = lm->nrm[t].z; }
+        k++;
+    }
+    return RL_SUCCESS;
+}
+
+int32_t rl_lightmap_read_hdr(rl_lightmap* lm, float* out_rgb) {
+    if (!lm || !out_rgb) return fail(RL_INVALID_ARGUMENT, "NULL argument");
+    int32_t st = pull_colors(lm);
+    if (st) return st;
+    std::memcpy(out_rgb, lm->color.data(), lm->color.size() * sizeof(float));
+    return RL_SUCCESS;
+}
+
+int32_t rl_bake_direct(rl_relight* ctx, rl_scene* scene, int32_t mesh) {
+    if (!ctx) return fail(RL_INVALID_ARGUMENT, "context is NULL");
+    int32_t st = check_scene_ready(scene);
+    if (st) return st;
+    if (mesh < 0 || mesh >= (int32_t)scene->meshes.size()) return fail(RL_INVALID_ARGUMENT, "bad mesh index");
+    // Baker::bakeMesh preconditions (baker/Baker.cpp:54-65).
+    if (scene->meshes[mesh].num_tris == 0) return fail(RL_INVALID_CALL, "mesh has no faces");
+    rl_lightmap* lm = scene->meshes[mesh].lightmap;
+    if (!lm) return fail(RL_INVALID_CALL, "mesh has no lightmap");
+    CallTimer tm(ctx);
+    if ((st = begin_call(ctx))) return st;
+    if ((st = run_direct(ctx, scene, &lm, 1))) return st;
+    return end_call(ctx, tm);
+}
+
+int32_t rl_bake_ambient_occlusion(rl_relight* ctx, rl_scene* scene, int32_t mesh,
+                                  const rl_ao_settings* settings) {
+    if (!ctx) return fail(RL_INVALID_ARGUMENT, "context is NULL");
+    int32_t st = check_scene_ready(scene);
+    if (st) return st;
+    if (mesh < 0 || mesh >= (int32_t)scene->meshes.size()) return fail(RL_INVALID_ARGUMENT, "bad mesh index");
+    if (scene->meshes[mesh].num_tris == 0) return fail(RL_INVALID_CALL, "mesh has no faces");
+    rl_lightmap* lm = scene->meshes[mesh].lightmap;
+    if (!lm) return fail(RL_INVALID_CALL, "mesh has no lightmap");
+    CallTimer tm(ctx);
+    if ((st = begin_call(ctx))) return st;
+    if ((st = run_ao(ctx, scene, &lm, 1, settings))) return st;
+    return end_call(ctx, tm);
+}
+
+int32_t rl_bake_batch(rl_relight* ctx, rl_scene* scene, rl_lightmap* const* maps,
+                      int32_t num_maps, int32_t do_direct, const rl_ao_settings* ao) {
+    if (!ctx || !maps || num_maps <= 0) return fail(RL_INVALID_ARGUMENT, "bad batch arguments");
+    int32_t st = check_scene_ready(scene);
+    if (st) return st;
+    CallTimer tm(ctx);
+    if ((st = begin_call(ctx))) return st;
+    if (do_direct && (st = run_direct(ctx, scene, maps, num_maps))) return st;
+    if (ao && (st = run_ao(ctx, scene, maps, num_maps, ao))) return st;
+    return end_call(ctx, tm);
+}
+
+int32_t rl_trace_segments(rl_relight* ctx, rl_scene* scene, const float* starts, const float* ends,
+                          uint32_t n, int32_t mode, uint8_t* out_flags, rl_hit* out_hits) {
+    if (!ctx) return fail(RL_INVALID_ARGUMENT, "context is NULL");
+    int32_t st = check_scene_ready(scene);
+    if (st) return st;
+    if (mode != RL_TRACE_ANY && mode != RL_TRACE_CLOSEST) return fail(RL_INVALID_ARGUMENT, "bad trace mode");
+    if (n == 0) return RL_SUCCESS;
+    if (!starts || !ends) return fail(RL_INVALID_ARGUMENT, "segment arrays are NULL");
+    if (mode == RL_TRACE_ANY && !out_flags) return fail(RL_INVALID_ARGUMENT, "out_flags is NULL");
+    if (mode == RL_TRACE_CLOSEST && !out_hits) return fail(RL_INVALID_ARGUMENT, "out_hits is NULL");
+    std::vector<float4> a(n), b(n);
+    for (uint32_t i = 0; i < n; i++) {
+        a[i] = make_float4(starts[3 * i], starts[3 * i + 1], starts[3 * i + 2], 0.0f);
+        b[i] = make_float4(ends[3 * i], ends[3 * i + 1], ends[3 * i + 2], 0.0f);
+    }
+    float4 *da = nullptr, *db = nullptr, *dh = nullptr;
+    uint8_t* df = nullptr;
+    auto cleanup = [&]() { dev_free(da); dev_free(db); dev_free(dh); dev_free(df); };
+    hipError_t e = dev_alloc(&da, n);
+    if (e == hipSuccess) e = dev_alloc(&db, n);
+    if (e == hipSuccess) e = mode == RL_TRACE_ANY ? dev_alloc(&df, n) : dev_alloc(&dh, n);
+    if (e != hipSuccess) {
+        cleanup();
+        return fail(RL_OUT_OF_MEMORY, "trace buffers");
+    }
+    CallTimer tm(ctx);
+    if (e == hipSuccess) e = hipMemcpy(da, a.data(), n * sizeof(float4), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(db, b.data(), n * sizeof(float4), hipMemcpyHostToDevice);
+    if (e == hipSuccess && (st = begin_call(ctx))) { cleanup(); return st; }
+    if (e == hipSuccess) e = kernel_begin(ctx, RL_KERNEL_TRACE);
+    if (e == hipSuccess)
+        e = launch_trace(ctx->stream, scene->dev(), scene->cap, mode == RL_TRACE_ANY, da, db, n, df, dh, ctx->d_rays);
+    if (e == hipSuccess) e = kernel_end(ctx);
+    ctx->stats.launches = 1;
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    std::vector<float4> h;
+    if (e == hipSuccess) {
+        if (mode == RL_TRACE_ANY) {
+            e = hipMemcpy(out_flags, df, n, hipMemcpyDeviceToHost);
+        } else {
+            h.resize(n);
+            e = hipMemcpy(h.data(), dh, n * sizeof(float4), hipMemcpyDeviceToHost);
+        }
+    }
+    if (e != hipSuccess) {
+        cleanup();
+        return fail(RL_DEVICE_ERROR, std::string("trace: ") + hipGetErrorString(e));
+    }
+    if (mode == RL_TRACE_CLOSEST) {
+        for (uint32_t i = 0; i < n; i++) {
+            uint32_t tri;
+            std::memcpy(&tri, &h[i].y, 4);
+            rl_hit& o = out_hits[i];
+            o.t = h[i].x;
+            o.u = h[i].z;
+            o.v = h[i].w;
+            if (tri == 0xFFFFFFFFu) {
+                o.triangle = -1;
+                o.mesh = -1;
+                o.face = -1;
+            } else {
+                o.triangle = (int32_t)tri;
+                o.mesh = (int32_t)scene->tri_mesh[tri];
+                o.face = (int32_t)scene->tri_face[tri];
+            }
+        }
+    }
+    cleanup();
+    return end_call(ctx, tm);
+}
+
+int32_t rl_get_stats(const rl_relight* ctx, rl_stats* out) {
+    if (!ctx || !out) return fail(RL_INVALID_ARGUMENT, "NULL argument");
+    *out = ctx->stats;
+    return RL_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,309 @@
+"""Python mirror of the reference's Relight / Scene / Lightmap API over librelight_amd.so.
+
+Names, argument meaning and error behaviour follow src/relight/Relight.h:107-189,
+scene/Scene.h and Lightmap.h; every call goes through the C-ABI of include/relight.h, and every
+bake runs on the GPU. A non-success status raises RelightError carrying the RelightStatus-style
+code (Relight.h:82-86) and the library's message.
+"""
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _capi as A
+
+
+class RelightError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{A.STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+        self.message = message
+
+
+def _check(lib, status: int):
+    if status != A.RL_SUCCESS:
+        raise RelightError(status, lib.rl_last_error().decode())
+
+
+def _fptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _uptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_uint32))
+
+
+@dataclass
+class AmbientOcclusionSettings:
+    """AmbientOcclusionSettings (Relight.h:135-152) + sample-stream seed and sampling mode."""
+    samples: int = 128
+    occluded_fraction: float = 0.8
+    max_distance: float = 0.6
+    exponent: float = 1.0
+    seed: int = 1
+    sampling: int = A.RL_SAMPLE_UNIFORM
+
+    # Presets of Relight.cpp:123-172.
+    @classmethod
+    def draft(cls, occluded_fraction=0.8, max_distance=0.6, exponent=1.0):
+        return cls(128, occluded_fraction, max_distance, exponent)
+
+    @classmethod
+    def fast(cls, occluded_fraction=0.8, max_distance=0.6, exponent=1.0):
+        return cls(256, occluded_fraction, max_distance, exponent)
+
+    @classmethod
+    def best(cls, occluded_fraction=0.8, max_distance=0.6, exponent=1.0):
+        return cls(1024, occluded_fraction, max_distance, exponent)
+
+    @classmethod
+    def production(cls, occluded_fraction=0.8, max_distance=0.6, exponent=1.0):
+        return cls(2048, occluded_fraction, max_distance, exponent)
+
+    def to_c(self) -> A.rl_ao_settings:
+        return A.rl_ao_settings(int(self.samples), float(self.occluded_fraction),
+                                float(self.max_distance), float(self.exponent),
+                                int(self.seed) & 0xFFFFFFFF, int(self.sampling))
+
+
+@dataclass
+class LightDesc:
+    """A light as Light::create* builds it (scene/Light.cpp:166-234)."""
+    type: int
+    position: Sequence[float] = (0.0, 0.0, 0.0)
+    direction: Sequence[float] = (0.0, -1.0, 0.0)
+    color: Sequence[float] = (1.0, 1.0, 1.0)
+    intensity: float = 1.0
+    casts_shadow: bool = True
+    has_attenuation: bool = True
+    radius: float = 1.0
+    att_constant: float = 0.0
+    att_linear: float = 0.0
+    att_quadratic: float = 25.0   # LinearLightAttenuation defaults, scene/Light.h:125
+    spot_cutoff: float = 0.0
+    spot_exponent: float = 1.0
+    area_positions: Optional[np.ndarray] = None   # (V, 3) local to position
+    area_indices: Optional[np.ndarray] = None     # (T*3,)
+
+    @classmethod
+    def point(cls, position, radius, color=(1, 1, 1), intensity=1.0, casts_shadow=True):
+        return cls(A.RL_LIGHT_POINT, position=position, radius=radius, color=color,
+                   intensity=intensity, casts_shadow=casts_shadow)
+
+    @classmethod
+    def spot(cls, position, direction, cutoff, radius, color=(1, 1, 1), intensity=1.0,
+             casts_shadow=True):
+        return cls(A.RL_LIGHT_SPOT, position=position, direction=direction, spot_cutoff=cutoff,
+                   radius=radius, color=color, intensity=intensity, casts_shadow=casts_shadow)
+
+    @classmethod
+    def directional(cls, direction, color=(1, 1, 1), intensity=1.0, casts_shadow=True):
+        return cls(A.RL_LIGHT_DIRECTIONAL, direction=direction, color=color, intensity=intensity,
+                   casts_shadow=casts_shadow, has_attenuation=False)
+
+    @classmethod
+    def area(cls, positions, indices, position, radius, color=(1, 1, 1), intensity=1.0,
+             casts_shadow=True):
+        return cls(A.RL_LIGHT_AREA, position=position, radius=radius, color=color,
+                   intensity=intensity, casts_shadow=casts_shadow,
+                   area_positions=np.ascontiguousarray(positions, np.float32),
+                   area_indices=np.ascontiguousarray(indices, np.uint32))
+
+    def to_c(self) -> A.rl_light_desc:
+        d = A.rl_light_desc()
+        d.type = int(self.type)
+        d.casts_shadow = 1 if self.casts_shadow else 0
+        d.position[:] = [float(x) for x in self.position]
+        d.direction[:] = [float(x) for x in self.direction]
+        d.color[:] = [float(x) for x in self.color]
+        d.intensity = float(self.intensity)
+        d.has_attenuation = 1 if self.has_attenuation else 0
+        d.radius = float(self.radius)
+        d.att_constant = float(self.att_constant)
+        d.att_linear = float(self.att_linear)
+        d.att_quadratic = float(self.att_quadratic)
+        d.spot_cutoff = float(self.spot_cutoff)
+        d.spot_exponent = float(self.spot_exponent)
+        if self.area_positions is not None:
+            d.area_positions = _fptr(self.area_positions)
+            d.area_indices = _uptr(self.area_indices)
+            d.area_vertex_count = int(self.area_positions.shape[0])
+            d.area_index_count = int(self.area_indices.size)
+        return d
+
+
+class Scene:
+    """Scene (scene/Scene.h): begin -> add_mesh/add_light -> end (BVH build + upload)."""
+
+    def __init__(self, relight: "Relight"):
+        self._relight = relight
+        self._lib = relight._lib
+        h = C.c_void_p()
+        _check(self._lib, self._lib.rl_scene_create(relight._h, C.byref(h)))
+        self._h = h
+        self._keep = []
+
+    def close(self):
+        if self._h:
+            self._lib.rl_scene_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def begin(self):
+        _check(self._lib, self._lib.rl_scene_begin(self._h))
+
+    def add_mesh(self, positions, normals, lightmap_uvs, indices, transform=None,
+                 albedo=None) -> int:
+        p = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+        n = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+        uv = np.ascontiguousarray(lightmap_uvs, np.float32).reshape(-1, 2)
+        ix = np.ascontiguousarray(indices, np.uint32).reshape(-1)
+        tr = None if transform is None else np.ascontiguousarray(transform, np.float32).reshape(16)
+        al = None if albedo is None else np.ascontiguousarray(albedo, np.float32).reshape(4)
+        out = C.c_int32()
+        _check(self._lib, self._lib.rl_scene_add_mesh(
+            self._h, _fptr(p), _fptr(n), _fptr(uv), p.shape[0], _uptr(ix), ix.size, _fptr(tr),
+            _fptr(al), C.byref(out)))
+        return out.value
+
+    def add_light(self, light: LightDesc) -> int:
+        d = light.to_c()
+        self._keep.append(light)
+        out = C.c_int32()
+        _check(self._lib, self._lib.rl_scene_add_light(self._h, C.byref(d), C.byref(out)))
+        return out.value
+
+    def end(self):
+        _check(self._lib, self._lib.rl_scene_end(self._h))
+
+    def info(self):
+        t, n, s = C.c_uint32(), C.c_uint32(), C.c_int32()
+        _check(self._lib, self._lib.rl_scene_info(self._h, C.byref(t), C.byref(n), C.byref(s)))
+        return {"triangles": t.value, "nodes": n.value, "stack_bound": s.value}
+
+    def trace_segments(self, starts, ends, mode=A.RL_TRACE_ANY):
+        """rt::ITracer::traceSegment over arrays of segments (N, 3) -> flags or hit records."""
+        s = np.ascontiguousarray(starts, np.float32).reshape(-1, 3)
+        e = np.ascontiguousarray(ends, np.float32).reshape(-1, 3)
+        n = s.shape[0]
+        if mode == A.RL_TRACE_ANY:
+            flags = np.zeros(n, np.uint8)
+            _check(self._lib, self._lib.rl_trace_segments(
+                self._relight._h, self._h, _fptr(s), _fptr(e), n, mode,
+                flags.ctypes.data_as(C.POINTER(C.c_uint8)), None))
+            return flags
+        hits = (A.rl_hit * max(n, 1))()
+        _check(self._lib, self._lib.rl_trace_segments(
+            self._relight._h, self._h, _fptr(s), _fptr(e), n, mode, None, hits))
+        arr = np.ctypeslib.as_array(hits)[:n]
+        return {k: np.array(arr[k]) for k in ("t", "triangle", "mesh", "face", "u", "v")}
+
+
+class Lightmap:
+    """Lightmap (Lightmap.h:58-150): texel raster at add_mesh, HDR readback."""
+
+    def __init__(self, relight: "Relight", width: int, height: int):
+        self._lib = relight._lib
+        self.width, self.height = int(width), int(height)
+        h = C.c_void_p()
+        _check(self._lib, self._lib.rl_lightmap_create(relight._h, self.width, self.height,
+                                                       C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            self._lib.rl_lightmap_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_mesh(self, scene: Scene, mesh: int):
+        _check(self._lib, self._lib.rl_lightmap_add_mesh(self._h, scene._h, int(mesh)))
+
+    def set_seed(self, seed: int):
+        _check(self._lib, self._lib.rl_lightmap_set_seed(self._h, int(seed) & 0xFFFFFFFF))
+
+    def valid_count(self) -> int:
+        c = C.c_uint32()
+        _check(self._lib, self._lib.rl_lightmap_valid_count(self._h, C.byref(c)))
+        return c.value
+
+    def lumels(self):
+        n = self.valid_count()
+        texel = np.zeros(n, np.uint32)
+        pos = np.zeros((n, 3), np.float32)
+        nrm = np.zeros((n, 3), np.float32)
+        _check(self._lib, self._lib.rl_lightmap_read_lumels(self._h, _uptr(texel), _fptr(pos),
+                                                            _fptr(nrm)))
+        return texel, pos, nrm
+
+    def to_hdr(self) -> np.ndarray:
+        """Lightmap::toHdr: (height, width, 3) float32, invalid texels 0."""
+        out = np.zeros((self.height, self.width, 3), np.float32)
+        _check(self._lib, self._lib.rl_lightmap_read_hdr(self._h, _fptr(out)))
+        return out
+
+
+class Relight:
+    """Relight (Relight.h:155-189): factories and bake entry points, all on the GPU."""
+
+    def __init__(self):
+        self._lib = A.load()
+        h = C.c_void_p()
+        _check(self._lib, self._lib.rl_create(C.byref(h)))
+        self._h = h
+
+    @classmethod
+    def create(cls) -> "Relight":
+        return cls()
+
+    def close(self):
+        if self._h:
+            self._lib.rl_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def create_scene(self) -> Scene:
+        return Scene(self)
+
+    def create_lightmap(self, width: int, height: int) -> Lightmap:
+        return Lightmap(self, width, height)
+
+    def bake_direct_light(self, scene: Scene, mesh: int):
+        _check(self._lib, self._lib.rl_bake_direct(self._h, scene._h, int(mesh)))
+
+    def bake_ambient_occlusion(self, scene: Scene, mesh: int, settings: AmbientOcclusionSettings):
+        s = settings.to_c()
+        _check(self._lib, self._lib.rl_bake_ambient_occlusion(self._h, scene._h, int(mesh),
+                                                              C.byref(s)))
+
+    def bake_batch(self, scene: Scene, lightmaps: List[Lightmap], direct: bool = True,
+                   ao: Optional[AmbientOcclusionSettings] = None):
+        arr = (C.c_void_p * len(lightmaps))(*[lm._h.value for lm in lightmaps])
+        s = ao.to_c() if ao is not None else None
+        _check(self._lib, self._lib.rl_bake_batch(self._h, scene._h, arr, len(lightmaps),
+                                                  1 if direct else 0,
+                                                  C.byref(s) if s is not None else None))
+
+    def stats(self) -> dict:
+        st = A.rl_stats()
+        _check(self._lib, self._lib.rl_get_stats(self._h, C.byref(st)))
+        kernels = {name: {"ms": st.kernel_ms[i], "launches": st.kernel_launches[i]}
+                   for i, name in A.KERNEL_NAMES.items() if st.kernel_launches[i]}
+        return {"rays": st.rays, "texels": st.texels, "gpu_ms": st.gpu_ms,
+                "wall_ms": st.wall_ms, "launches": st.launches, "kernels": kernels}

@@ -1,0 +1,307 @@
+"""Pins the CPU oracle (CPU only, no GPU).
+
+Each check compares the oracle with something outside it: a published vector (pcg32-demo), an
+independent integer model, a closed form of the reference's light models, a textbook property
+(watertightness, sampler moments), or brute force.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from relight_amd import scenes
+from relight_amd.relight import AmbientOcclusionSettings, LightDesc
+
+M64 = (1 << 64) - 1
+
+
+def py_pcg32(initstate, initseq, n):
+    """Independent integer model of PCG32 (XSH-RR over a 64-bit LCG)."""
+    state, inc = 0, ((initseq << 1) | 1) & M64
+
+    def step():
+        nonlocal state
+        old = state
+        state = (old * 6364136223846793005 + inc) & M64
+        xs = (((old >> 18) ^ old) >> 27) & 0xFFFFFFFF
+        rot = old >> 59
+        return ((xs >> rot) | (xs << ((32 - rot) & 31))) & 0xFFFFFFFF
+
+    step()
+    state = (state + initstate) & M64
+    step()
+    return [step() for _ in range(n)]
+
+
+def py_splitmix(z):
+    z = (z + 0x9E3779B97F4A7C15) & M64
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
+    return z ^ (z >> 31)
+
+
+def test_pcg32_published_demo_vector():
+    # pcg32-demo (pcg-c-basic) seeded with pcg32_srandom_r(42, 54): first six outputs.
+    expect = [0xA15C02B7, 0x7B47F409, 0xBA1D3330, 0x83D2F293, 0xBFA4784B, 0xCBED606E]
+    assert list(O.pcg32(42, 54, 6)) == expect
+
+
+@pytest.mark.parametrize("state,seq", [(42, 54), (0, 0), (M64, 12345), (0xDEADBEEF, 1 << 40)])
+def test_pcg32_matches_integer_model(state, seq):
+    assert list(O.pcg32(state, seq, 500)) == py_pcg32(state, seq, 500)
+
+
+def test_splitmix64_known_answer_and_model():
+    # SplitMix64 seeded with 0: first output.
+    assert O.lib().or_splitmix64(0) == 0xE220A8397B1DCDAF
+    for z in [1, 7, 0x123456789ABCDEF, M64]:
+        assert O.lib().or_splitmix64(z) == py_splitmix(z)
+
+
+def test_sample_stream_keying():
+    seed, lm, texel, tech, sample = 1, 3, 4242, 1, 17
+    key = py_splitmix(py_splitmix((seed << 32) | lm) ^ texel)
+    expect = py_pcg32(key, (tech << 32) | sample, 8)
+    assert list(O.stream_u32(seed, lm, texel, tech, sample, 8)) == expect
+    # Streams of neighbouring samples / texels are distinct.
+    a = O.stream_u32(seed, lm, texel, tech, sample + 1, 8)
+    b = O.stream_u32(seed, lm, texel + 1, tech, sample, 8)
+    assert list(a) != expect and list(b) != expect
+
+
+@pytest.mark.parametrize("mode,ecos,ecos2", [(0, 0.5, 1.0 / 3.0), (1, 2.0 / 3.0, 0.5)])
+@pytest.mark.parametrize("normal", [(0, 0, 1), (0, 1, 0), (0.6, -0.8, 0.0), (0, 0, -1)])
+def test_hemisphere_sampler_moments(mode, ecos, ecos2, normal):
+    n = np.asarray(normal, np.float64)
+    d = O.hemisphere(1, 0, 99, 1, 200000, mode, normal).astype(np.float64)
+    assert np.allclose(np.linalg.norm(d, axis=1), 1.0, atol=2e-6)
+    c = d @ n
+    assert (c >= -1e-6).all()
+    # Uniform: E[cos] = 1/2, E[cos^2] = 1/3.  Cosine-weighted: 2/3 and 1/2.
+    se = 0.5 / math.sqrt(len(c))
+    assert abs(c.mean() - ecos) < 5 * se
+    assert abs((c * c).mean() - ecos2) < 5 * se
+    # Azimuthal symmetry: the mean direction lies along n.
+    assert np.linalg.norm(d.mean(0) - c.mean() * n) < 0.01
+
+
+def _tri_scene(tris, use_bvh=True):
+    tris = np.asarray(tris, np.float32).reshape(-1, 3, 3)
+    S = O.OracleScene(use_bvh)
+    pos = tris.reshape(-1, 3)
+    nrm = np.tile([0.0, 0.0, 1.0], (pos.shape[0], 1)).astype(np.float32)
+    uv = np.zeros((pos.shape[0], 2), np.float32)
+    S.add_mesh(pos, nrm, uv, np.arange(pos.shape[0], dtype=np.uint32))
+    S.build()
+    return S
+
+
+SQUARE = [[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 0, 0], [1, 1, 0], [0, 1, 0]]
+
+
+def test_ray_triangle_known_answers():
+    S = _tri_scene(SQUARE)
+    r = S.trace([[0.75, 0.25, 3.0]], [[0.75, 0.25, -3.0]], closest=True)
+    assert r["flags"][0] == 1 and r["triangle"][0] == 0
+    assert abs(r["t"][0] - 3.0) < 1e-6
+    # Embree barycentrics: hit = (1-u-v) v0 + u v1 + v v2 -> (0.75, 0.25) = 0.5 v0 + 0.5 v1 + 0.25 v2? solve:
+    u, v = r["u"][0], r["v"][0]
+    p = (1 - u - v) * np.array([0, 0, 0]) + u * np.array([1, 0, 0]) + v * np.array([1, 1, 0])
+    assert np.allclose(p, [0.75, 0.25, 0.0], atol=1e-6)
+    # Interval (0.01, |end-start|), open at both ends (Embree::initializeRay semantics).
+    assert S.trace([[0.5, 0.2, 1.0]], [[0.5, 0.2, 0.0]])["flags"][0] == 0          # t == tfar
+    assert S.trace([[0.5, 0.2, 1.0]], [[0.5, 0.2, -1e-3]])["flags"][0] == 1        # t < tfar
+    assert S.trace([[0.5, 0.2, 0.005]], [[0.5, 0.2, -1.0]])["flags"][0] == 0       # t < tnear
+    assert S.trace([[0.5, 0.2, 0.02]], [[0.5, 0.2, -1.0]])["flags"][0] == 1
+    # Parallel ray in the plane and a miss beside the square.
+    assert S.trace([[-1, 0.5, 0.0]], [[2, 0.5, 0.0]])["flags"][0] == 0
+    assert S.trace([[1.5, 0.5, 1.0]], [[1.5, 0.5, -1.0]])["flags"][0] == 0
+
+
+def test_watertight_shared_edge():
+    S = _tri_scene(SQUARE)
+    rng = np.random.default_rng(5)
+    # Rays through points exactly on the shared diagonal (x == y, exact in fp32).
+    x = rng.integers(1, 1 << 20, 4000).astype(np.float32) / np.float32(1 << 20)
+    tgt = np.stack([x, x, np.zeros_like(x)], 1)
+    org = tgt + rng.uniform(-1, 1, (4000, 3)).astype(np.float32) * np.float32([1, 1, 0]) + np.float32([0, 0, 2])
+    end = org + (tgt - org) * np.float32(2.0)
+    r = S.trace(org, end)
+    assert r["flags"].all()
+    # Closest hit through the edge: the tie goes to the smaller triangle id.
+    rc = S.trace(np.stack([x, x, np.ones_like(x)], 1), np.stack([x, x, -np.ones_like(x)], 1), closest=True)
+    assert (rc["triangle"] == 0).all()
+    # Random rays aimed at the square's interior always hit one of its triangles.
+    tgt = rng.uniform(0.001, 0.999, (20000, 2)).astype(np.float32)
+    tgt = np.concatenate([tgt, np.zeros((20000, 1), np.float32)], 1)
+    org = rng.uniform(-3, 3, (20000, 3)).astype(np.float32)
+    org[:, 2] = np.abs(org[:, 2]) + 0.5
+    end = org + (tgt - org) * np.float32(1.5)
+    assert S.trace(org, end)["flags"].all()
+
+
+def _segments(spec, n, seed):
+    rng = np.random.default_rng(seed)
+    allp = np.concatenate([m.positions for m in spec.meshes])
+    lo, hi = allp.min(0), allp.max(0)
+    a = rng.uniform(lo, hi, (n, 3)).astype(np.float32)
+    b = rng.uniform(lo, hi, (n, 3)).astype(np.float32)
+    return a, b
+
+
+@pytest.mark.parametrize("which", ["box_room", "city_small"])
+def test_bvh_equals_brute_force(which):
+    spec = scenes.box_room() if which == "box_room" else scenes.city(num_boxes=300, tiles=(2, 2),
+                                                                    side_range=(64, 128))
+    a, b = _segments(spec, 20000, 11)
+    bvh = O.OracleScene.from_spec(spec, use_bvh=True)
+    bf = O.OracleScene.from_spec(spec, use_bvh=False)
+    ra, rb = bvh.trace(a, b), bf.trace(a, b)
+    assert np.array_equal(ra["flags"], rb["flags"])
+    ca, cb = bvh.trace(a, b, closest=True), bf.trace(a, b, closest=True)
+    assert np.array_equal(ca["triangle"], cb["triangle"])
+    assert np.array_equal(ca["t"], cb["t"])
+    assert np.array_equal(ca["u"], cb["u"]) and np.array_equal(ca["v"], cb["v"])
+    assert ca["node_visits"] < cb["tri_tests"]
+
+
+def _floor_spec(size=20.0, res=64, extra=None):
+    """One square floor chart (y = 0, normal +y) over a res x res lightmap, plus extra quads."""
+    q = [scenes._Quad(np.array([-size / 2, 0, -size / 2]), np.array([0, 0, size]),
+                      np.array([size, 0, 0]), np.array([0.0, 1.0, 0.0]), 1, 1)]
+    q[0].rect = (0, 0, res, res)
+    p, n, uv, ix = scenes._emit(q, res, res)
+    meshes = [scenes.MeshSpec(p, n, uv, ix, 0)]
+    for e in extra or []:
+        e.rect = (0, 0, 0, 0)
+        e.charted = False
+        ep, en, euv, eix = scenes._emit([e], res, res)
+        meshes.append(scenes.MeshSpec(ep, en, euv, eix, 1))
+    return scenes.SceneSpec("floor", meshes, [], [(res, res), (1, 1)])
+
+
+def test_raster_texel_centres():
+    spec = _floor_spec(size=20.0, res=64)
+    S = O.OracleScene.from_spec(spec)
+    texel, pos, nrm = S.lumels(spec, 0)
+    # Every texel centre of a full-map chart is inside one of its two triangles, except those on
+    # the diagonal that the strict u+v<1 test gives to neither (Face::isUvInside).
+    assert texel.size >= 64 * 64 - 64
+    x, y = texel % 64, texel // 64
+    # The chart maps s (u axis) to world z and t (v axis) to world x: uv = (s, t).
+    ez = -10.0 + 20.0 * (x + 0.5) / 64
+    ex = -10.0 + 20.0 * (y + 0.5) / 64
+    assert np.allclose(pos[:, 2], ez, atol=2e-5) and np.allclose(pos[:, 0], ex, atol=2e-5)
+    assert np.allclose(pos[:, 1], 0.0) and np.allclose(nrm, [0, 1, 0])
+
+
+def _direct(spec, lights):
+    spec.lights = lights
+    S = O.OracleScene.from_spec(spec)
+    texel, pos, nrm = S.lumels(spec, 0)
+    col = np.zeros((texel.size, 3), np.float32)
+    S.bake_direct(pos, nrm, col)
+    return pos.astype(np.float64), col.astype(np.float64)
+
+
+def test_direct_point_light_closed_form():
+    h, I, R = 3.0, 50.0, 10.0
+    pos, col = _direct(_floor_spec(), [LightDesc.point((0.0, h, 0.0), radius=R, intensity=I,
+                                                      color=(1.0, 0.5, 0.25))])
+    d = np.linalg.norm(pos - [0, h, 0], axis=1)
+    expect = I * (h / d) / (1.0 + 25.0 * (d / R) ** 2)
+    assert np.allclose(col[:, 0], expect, rtol=2e-5)
+    assert np.allclose(col[:, 1], 0.5 * expect, rtol=2e-5)
+    assert np.allclose(col[:, 2], 0.25 * expect, rtol=2e-5)
+
+
+def test_direct_point_light_shadow_of_square():
+    h = 3.0
+    occ = scenes._Quad(np.array([-1.0, h / 2, -1.0]), np.array([0, 0, 2.0]), np.array([2.0, 0, 0]),
+                       np.array([0, -1.0, 0]), 1, 1)
+    pos, col = _direct(_floor_spec(extra=[occ]), [LightDesc.point((0.0, h, 0.0), radius=10.0,
+                                                                 intensity=50.0)])
+    ax = np.maximum(np.abs(pos[:, 0]), np.abs(pos[:, 2]))
+    inside, outside = ax < 1.95, ax > 2.05   # the square's shadow is [-2, 2]^2 on the floor
+    assert (col[inside] == 0).all()
+    d = np.linalg.norm(pos[outside] - [0, h, 0], axis=1)
+    assert np.allclose(col[outside, 0], 50.0 * (h / d) / (1 + 25 * (d / 10.0) ** 2), rtol=2e-5)
+
+
+def test_direct_directional_and_spot_closed_forms():
+    L = np.array([0.3, -0.9, 0.2])
+    L /= np.linalg.norm(L)
+    _, col = _direct(_floor_spec(), [LightDesc.directional(tuple(L), intensity=2.0)])
+    assert np.allclose(col[:, 0], 2.0 * (-L[1]), rtol=1e-6)
+    cutoff = 0.9
+    pos, col = _direct(_floor_spec(), [LightDesc.spot((0.0, 4.0, 0.0), (0.0, -1.0, 0.0), cutoff,
+                                                      radius=20.0, intensity=10.0)])
+    dv = pos - [0, 4, 0]
+    dist = np.linalg.norm(dv, axis=1)
+    cosa = -dv[:, 1] / dist
+    cut = np.where(cosa <= cutoff, 0.0, 1 - (1 - cosa) / (1 - cutoff))
+    expect = 10.0 * (4.0 / dist) * cut / (1 + 25 * (dist / 20.0) ** 2)
+    near_edge = np.abs(cosa - cutoff) < 1e-4
+    assert np.allclose(col[~near_edge, 0], expect[~near_edge], rtol=3e-5, atol=1e-6)
+
+
+def test_direct_area_light_vertex_average():
+    apos, aidx = scenes._area_light_rect(2.0, 4, 2)
+    P = np.array([1.0, 5.0, -2.0])
+    pos, col = _direct(_floor_spec(), [LightDesc.area(apos, aidx, position=tuple(P), radius=15.0,
+                                                      intensity=8.0)])
+    tri = apos.astype(np.float64)[aidx.reshape(-1, 3)]
+    cents = tri.mean(1) + P
+    assert cents.shape[0] == 16
+    acc = np.zeros(pos.shape[0])
+    for c in cents:
+        dv = c - pos
+        d = np.linalg.norm(dv, axis=1)
+        acc += 8.0 * np.maximum(dv[:, 1] / d, 0) / (1 + 25 * (d / 15.0) ** 2)
+    assert np.allclose(col[:, 0], acc / 16, rtol=3e-5)
+
+
+def _ao(spec, samples=64, dist=1.5, seed=1, sampling=1):
+    S = O.OracleScene.from_spec(spec)
+    texel, pos, nrm = S.lumels(spec, 0)
+    col = np.ones((texel.size, 3), np.float32)
+    ao = AmbientOcclusionSettings(samples, 0.8, dist, 1.0, seed, sampling)
+    occ, rays = S.bake_ao(pos, nrm, texel, 0, ao, col)
+    return pos, occ, col, rays
+
+
+def test_ao_open_plane_is_unoccluded():
+    pos, occ, col, rays = _ao(_floor_spec())
+    assert rays == 64 * pos.shape[0]
+    assert (occ == 0).all() and (col == 1.0).all()
+
+
+def test_ao_closed_box_reference_quirk():
+    # Inside a closed 1 m box every AO ray with max distance 4 m hits: occ = S, and the
+    # reference's unclamped value is 1 - S/(S*0.8) = -0.25 (baker/AmbientOcclusion.cpp:59).
+    quads = scenes._box_quads((0, 0, 0), (1, 1, 1), 2, inward=True)
+    scenes._pack(quads, 64, 64)
+    p, n, uv, ix = scenes._emit(quads, 64, 64)
+    spec = scenes.SceneSpec("cell", [scenes.MeshSpec(p, n, uv, ix, 0)], [], [(64, 64)])
+    for sampling in (0, 1):
+        pos, occ, col, _ = _ao(spec, samples=64, dist=4.0, sampling=sampling)
+        # Texel centres sit at least half a texel (~0.02 m) from every edge, beyond tnear.
+        assert (occ == 64).all()
+        assert np.allclose(col, 1.0 - 64 / (64 * 0.8), atol=1e-6)
+
+
+def test_ao_corner_binomial():
+    # A lumel on the floor at distance a from one infinite-ish wall: the occluded fraction under
+    # cosine sampling is the wall's projected solid angle fraction; check against quadrature.
+    wall = scenes._Quad(np.array([0.5, 0.0, -50.0]), np.array([0, 0, 100.0]), np.array([0, 10.0, 0]),
+                        np.array([-1.0, 0, 0]), 1, 1)
+    spec = _floor_spec(size=1.0, res=8, extra=[wall])
+    pos, occ, col, _ = _ao(spec, samples=4096, dist=1000.0)
+    # Differential element to a long strip of height H at distance a, perpendicular to the
+    # element's plane: F = (1 - a / sqrt(a^2 + H^2)) / 2; cosine sampling hits it with
+    # probability F (binomial noise sd <= 0.008 at 4096 samples).
+    a = 0.5 - pos[:, 0].astype(np.float64)
+    frac = occ / 4096.0
+    est = 0.5 * (1 - a / np.sqrt(a * a + 10.0 ** 2))
+    assert np.all(np.abs(frac - est) < 0.035)
