@@ -99,11 +99,39 @@ typedef struct rl_hit {
     float v;
 } rl_hit;
 
+/* Photon termination: the reference's depth/energy test (baker/Photons.cpp:98) or Russian
+ * roulette on the hit albedo (survive with probability mean(albedo), power divided by it). */
+#define RL_PHOTON_THRESHOLD 0
+#define RL_PHOTON_RUSSIAN_ROULETTE 1
+
+/* IndirectLightSettings (Relight.h:108-132) plus the sample-stream seed, the final-gather
+ * hemisphere sampling, an optional photon count per light per pass (0: intensity * 25000 as
+ * PhotonEmitter::photonCount, scene/Light.cpp:334-337) and the photon termination rule. */
+typedef struct rl_indirect_settings {
+    int32_t photon_pass_count;
+    int32_t photon_bounce_count;
+    float photon_energy_threshold;
+    float photon_max_distance;
+    int32_t final_gather_samples;
+    float final_gather_distance;
+    int32_t final_gather_radius;
+    float sky_color[3];
+    float ambient_color[3];
+    uint32_t seed;
+    int32_t sampling;
+    int32_t photons_per_light;
+    int32_t termination;
+} rl_indirect_settings;
+
 /* Kernel slots of rl_stats.kernel_ms. */
 #define RL_KERNEL_DIRECT 0
 #define RL_KERNEL_AO_COUNT 1
 #define RL_KERNEL_AO_FINALIZE 2
 #define RL_KERNEL_TRACE 3
+#define RL_KERNEL_PHOTON_TRACE 4
+#define RL_KERNEL_PHOTON_STORE 5
+#define RL_KERNEL_PHOTON_GATHER 6
+#define RL_KERNEL_FINAL_GATHER 7
 #define RL_KERNEL_SLOTS 8
 
 /* Counters of the last bake or trace call on a context. */
@@ -180,6 +208,23 @@ int32_t rl_bake_batch(rl_relight* ctx, rl_scene* scene, rl_lightmap* const* maps
 int32_t rl_trace_segments(rl_relight* ctx, rl_scene* scene, const float* starts,
                           const float* ends, uint32_t n, int32_t mode, uint8_t* out_flags,
                           rl_hit* out_hits);
+
+/* Relight::createPhotonmap (Relight.h:162) and Photonmap::addMesh (Lightmap.cpp:359-373): a
+ * photonmap is a lightmap whose texels receive photon stores. read() returns its full grids:
+ * accumulated photon power (w*h*3), photon counts (w*h) and gathered colour (w*h*3); any output
+ * may be NULL. */
+int32_t rl_photonmap_create(rl_relight* ctx, int32_t width, int32_t height, rl_lightmap** out);
+int32_t rl_photonmap_add_mesh(rl_lightmap* pm, rl_scene* scene, int32_t mesh);
+int32_t rl_photonmap_read(rl_lightmap* pm, float* color, uint32_t* photons, float* gathered);
+
+/* Relight::emitPhotons (Relight.cpp:244-257): Photons::emit over all passes and lights
+ * (baker/Photons.cpp:50-144) storing into the hit meshes' photonmaps, then Photonmap::gather
+ * with final_gather_radius (Lightmap.cpp:376-413) on every photonmap of the scene. */
+int32_t rl_emit_photons(rl_relight* ctx, rl_scene* scene, const rl_indirect_settings* settings);
+/* Relight::bakeIndirectLight (Relight.cpp:230-241) -> IndirectLight::bakeLumel
+ * (baker/IndirectLight.cpp:47-75) over every valid lumel of the mesh's lightmap. */
+int32_t rl_bake_indirect(rl_relight* ctx, rl_scene* scene, int32_t mesh,
+                         const rl_indirect_settings* settings);
 
 /* Counters of the last bake/trace call on ctx. */
 int32_t rl_get_stats(const rl_relight* ctx, rl_stats* out);

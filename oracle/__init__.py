@@ -32,6 +32,40 @@ class or_ao(C.Structure):
                 ("sampling", C.c_int32)]
 
 
+class or_indirect(C.Structure):
+    _fields_ = [("photon_pass_count", C.c_int32), ("photon_bounce_count", C.c_int32),
+                ("photon_energy_threshold", C.c_float), ("photon_max_distance", C.c_float),
+                ("final_gather_samples", C.c_int32), ("final_gather_distance", C.c_float),
+                ("final_gather_radius", C.c_int32), ("sky_color", C.c_float * 3),
+                ("ambient_color", C.c_float * 3), ("seed", C.c_uint32), ("sampling", C.c_int32),
+                ("photons_per_light", C.c_int32), ("termination", C.c_int32)]
+
+
+def _indirect(s):
+    o = or_indirect()
+    for name, _ in or_indirect._fields_:
+        v = getattr(s, name)
+        if name in ("sky_color", "ambient_color"):
+            getattr(o, name)[:] = [float(x) for x in v]
+        else:
+            setattr(o, name, v)
+    return o
+
+
+def photon_layout(spec):
+    """Photonmaps in first-appearance mesh order (the library's order) and mesh -> map index."""
+    order, mesh_pm = [], []
+    for m in spec.meshes:
+        k = getattr(m, "photonmap", -1)
+        if k is not None and k >= 0:
+            if k not in order:
+                order.append(k)
+            mesh_pm.append(order.index(k))
+        else:
+            mesh_pm.append(-1)
+    return order, np.asarray(mesh_pm, np.int32)
+
+
 def build():
     subprocess.run(["make", "-s", "-C", HERE, "-B", "liboracle.so"], check=True)
 
@@ -67,6 +101,10 @@ def lib():
             "or_stream_u32": (None, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _U32]),
             "or_hemisphere": (None, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, _F, _F]),
             "or_splitmix64": (C.c_uint64, [C.c_uint64]),
+            "or_emit_photons": (C.c_int32, [C.c_void_p, C.POINTER(or_indirect), _I32, C.c_int32, _I32, _I32, _U8,
+                                            _F, _U32, _F, C.c_int32, _U64]),
+            "or_final_gather": (C.c_int32, [C.c_void_p, C.c_uint32, _F, _F, _U32, C.c_uint32, C.POINTER(or_indirect),
+                                            _I32, C.c_int32, _I32, _I32, _F, _F, C.c_int32, _U64]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -210,6 +248,57 @@ class OracleScene:
             S.add_light(L)
         S.build()
         return S
+
+    def emit_photons(self, spec, settings, threads=0):
+        """Relight::emitPhotons: returns {spec photonmap index: (color, photons, gathered)} and
+        the number of photon rays."""
+        order, mesh_pm = photon_layout(spec)
+        ws = np.asarray([spec.photonmaps[k][0] for k in order], np.int32)
+        hs = np.asarray([spec.photonmaps[k][1] for k in order], np.int32)
+        valid = []
+        for k in order:
+            w, h = spec.photonmaps[k]
+            grids = None
+            for mi, m in enumerate(spec.meshes):
+                if getattr(m, "photonmap", -1) == k:
+                    grids = self.raster(mi, w, h, grids)
+            valid.append(grids[0])
+        n = int(sum(int(w) * int(h) for w, h in zip(ws, hs)))
+        valid = np.concatenate(valid) if valid else np.zeros(1, np.uint8)
+        color = np.zeros(max(n, 1) * 3, np.float32)
+        photons = np.zeros(max(n, 1), np.uint32)
+        gathered = np.zeros(max(n, 1) * 3, np.float32)
+        s = _indirect(settings)
+        rays = C.c_uint64()
+        self._L.or_emit_photons(self._h, C.byref(s), mesh_pm.ctypes.data_as(_I32), len(order),
+                                ws.ctypes.data_as(_I32), hs.ctypes.data_as(_I32), valid.ctypes.data_as(_U8),
+                                _f(color), _u(photons), _f(gathered), threads, C.byref(rays))
+        out, base = {}, 0
+        for k, w, h in zip(order, ws, hs):
+            sz = int(w) * int(h)
+            out[k] = (color[3 * base:3 * (base + sz)].reshape(h, w, 3),
+                      photons[base:base + sz].reshape(h, w),
+                      gathered[3 * base:3 * (base + sz)].reshape(h, w, 3))
+            base += sz
+        return out, rays.value
+
+    def final_gather(self, spec, pos, nrm, texel, lm_seed, settings, maps, color, threads=0):
+        """IndirectLight::bakeLumel over lumels (color in/out); maps from emit_photons."""
+        order, mesh_pm = photon_layout(spec)
+        ws = np.asarray([spec.photonmaps[k][0] for k in order], np.int32)
+        hs = np.asarray([spec.photonmaps[k][1] for k in order], np.int32)
+        g = (np.concatenate([maps[k][2].reshape(-1) for k in order]).astype(np.float32)
+             if order else np.zeros(3, np.float32))
+        pos = np.ascontiguousarray(pos, np.float32)
+        nrm = np.ascontiguousarray(nrm, np.float32)
+        texel = np.ascontiguousarray(texel, np.uint32)
+        assert color.dtype == np.float32 and color.flags.c_contiguous
+        s = _indirect(settings)
+        rays = C.c_uint64()
+        self._L.or_final_gather(self._h, pos.shape[0], _f(pos), _f(nrm), _u(texel), lm_seed, C.byref(s),
+                                mesh_pm.ctypes.data_as(_I32), len(order), ws.ctypes.data_as(_I32),
+                                hs.ctypes.data_as(_I32), _f(g), _f(color), threads, C.byref(rays))
+        return rays.value
 
     def lumels(self, spec, lightmap):
         """Valid lumels of one lightmap in texel order: texel, pos, nrm (rasterising its meshes

@@ -84,8 +84,8 @@ Rng stream(uint32_t seed, uint32_t lm_seed, uint32_t texel, uint32_t tech, uint3
     return rng_seed(k, ((uint64_t)tech << 32) | sample);
 }
 
-// Types.h:924-949 with PCG draws (zero-length draws rejected).
-Vec hemi_uniform(Rng& r, Vec n) {
+// Vec3::randomDirection, Types.h:924-937, with PCG draws (zero-length draws rejected).
+Vec hemi_uniform_sphere(Rng& r) {
     Vec d;
     float l;
     for (;;) {
@@ -95,7 +95,12 @@ Vec hemi_uniform(Rng& r, Vec n) {
         l = len3(d);
         if (!(l > 1.0f || l == 0.0f)) break;
     }
-    d = divs(d, l);
+    return divs(d, l);
+}
+
+// Vec3::randomHemisphereDirection, Types.h:940-949.
+Vec hemi_uniform(Rng& r, Vec n) {
+    Vec d = hemi_uniform_sphere(r);
     if (dot3(d, n) < 0.0f) d = neg(d);
     return d;
 }
@@ -736,5 +741,227 @@ void or_hemisphere(uint32_t seed, uint32_t lm_seed, uint32_t texel, uint32_t tec
 }
 
 uint64_t or_splitmix64(uint64_t z) { return mix64(z); }
+
+// ------------------------------------------------------------------ photons + final gather
+struct or_indirect {
+    int32_t photon_pass_count, photon_bounce_count;
+    float photon_energy_threshold, photon_max_distance;
+    int32_t final_gather_samples;
+    float final_gather_distance;
+    int32_t final_gather_radius;
+    float sky_color[3], ambient_color[3];
+    uint32_t seed;
+    int32_t sampling, photons_per_light, termination;
+};
+
+// Photonmaps are passed concatenated: map k covers keys [base_k, base_k + w_k*h_k) of the
+// valid/color/photons/gathered arrays (color and gathered hold 3 floats per texel).
+struct PMaps {
+    int32_t n;
+    const int32_t *w, *h;
+    std::vector<uint32_t> base;
+    const uint8_t* valid;
+    float* color;
+    uint32_t* photons;
+    float* gathered;
+};
+
+static PMaps make_pmaps(int32_t n, const int32_t* w, const int32_t* h, const uint8_t* valid, float* color,
+                        uint32_t* photons, float* gathered) {
+    PMaps M{n, w, h, {}, valid, color, photons, gathered};
+    uint32_t acc = 0;
+    for (int k = 0; k < n; k++) {
+        M.base.push_back(acc);
+        acc += (uint32_t)(w[k] * h[k]);
+    }
+    return M;
+}
+
+// Face::normalAt / uvAt at Embree's (u, v): a + (b - a) * v + (c - a) * u.
+static inline float flerp(float a, float b, float c, float u, float v) { return a + (b - a) * v + (c - a) * u; }
+
+static inline int lumel_xy(float uv, int size) {
+    int x = (int)(uv * (float)(size - 1));
+    return x < 0 ? 0 : (x >= size ? size - 1 : x);
+}
+
+struct PRec {
+    uint32_t key;
+    float r, g, b;
+};
+
+// Photons::emit / trace / store (baker/Photons.cpp:50-144) with PhotonEmitter::emit,
+// DirectionalPhotonEmitter::emit (scene/Light.cpp:339-361) and LightSpotCutoff, then
+// Photonmap::gather (Lightmap.cpp:376-413) on every map.
+int32_t or_emit_photons(void* sp, const or_indirect* s, const int32_t* mesh_pm, int32_t npm, const int32_t* pw,
+                        const int32_t* ph, const uint8_t* valid, float* color, uint32_t* photons, float* gathered,
+                        int32_t threads, uint64_t* rays_out) {
+    const Scene& S = *(Scene*)sp;
+    PMaps M = make_pmaps(npm, pw, ph, valid, color, photons, gathered);
+    const int nl = (int)S.lights.size();
+    std::vector<uint64_t> first(nl);
+    uint64_t per_pass = 0;
+    for (int l = 0; l < nl; l++) {
+        first[l] = per_pass;
+        int64_t c = s->photons_per_light > 0 ? s->photons_per_light : (int64_t)(int32_t)(S.lights[l].intensity * 25000.0f);
+        per_pass += (uint64_t)std::max<int64_t>(c, 0);
+    }
+    const uint64_t total = per_pass * (uint64_t)s->photon_pass_count;
+    const int D = s->photon_bounce_count;
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (size_t i = 0; i < S.tri.size(); i += 3)
+        for (int k = 0; k < 3; k++) { lo[k] = std::min(lo[k], S.tri[i + k]); hi[k] = std::max(hi[k], S.tri[i + k]); }
+    std::vector<PRec> rec((size_t)total * (D + 1), PRec{0xFFFFFFFFu, 0, 0, 0});
+    std::vector<uint64_t> rays(256, 0);
+    if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
+    threads = std::min(threads, 255);
+    parallel_for((uint32_t)total, threads, [&](uint32_t p, int th) {
+        const uint64_t r = p % per_pass;
+        int l = 0;
+        while (l + 1 < nl && first[l + 1] <= r) l++;
+        const Light& L = S.lights[l];
+        Rng g = stream(s->seed, 0xFFFFFFFFu, p, 3u, 0u);
+        Vec pos, dir;
+        if (L.type == 2) {
+            const float tx = rng_f(g), ty = rng_f(g), tz = rng_f(g);
+            const Vec q = mk(lo[0] * (1.0f - tx) + hi[0] * tx, lo[1] * (1.0f - ty) + hi[1] * ty, lo[2] * (1.0f - tz) + hi[2] * tz);
+            const float d = -dot3(L.dir, L.pos);
+            const float dist = dot3(L.dir, q) + d;
+            pos = sub(sub(q, scale(L.dir, dist)), scale(L.dir, 5.0f));
+            dir = L.dir;
+        } else {
+            pos = L.pos;
+            dir = hemi_uniform_sphere(g);
+        }
+        float cut = 1.0f;
+        if (L.type == 1) {
+            float v = dot3(dir, L.dir);
+            if (v <= L.cutoff) cut = 0.0f;
+            else {
+                v = (1.0f - (1.0f - v) * 1.0f / (1.0f - L.cutoff));
+                if (fabsf(1.0f - L.sexp) > 0.01f) v = powf(v, L.sexp);
+                cut = v;
+            }
+        }
+        if (!(cut > 0.0f)) return;
+        float cr = L.color.x * L.intensity * cut, cg = L.color.y * L.intensity * cut, cb = L.color.z * L.intensity * cut;
+        for (int depth = 0; depth <= D; depth++) {
+            if (s->termination == 0 && 0.2126f * cr + 0.7152f * cg + 0.0722f * cb < s->photon_energy_threshold) break;
+            const Seg sg = make_seg(pos, add(pos, scale(dir, s->photon_max_distance)));
+            HitRec h;
+            rays[th]++;
+            if (!trace(S, sg, false, h, nullptr)) break;
+            const Vec hp = add(sg.o, scale(sg.d, h.t));
+            const float* N = &S.tnrm[9 * (size_t)h.tri];
+            const float* UV = &S.tuv[6 * (size_t)h.tri];
+            const float* A = &S.talb[4 * (size_t)h.tri];
+            const Vec nh = mk(flerp(N[0], N[3], N[6], h.u, h.v), flerp(N[1], N[4], N[7], h.u, h.v), flerp(N[2], N[5], N[8], h.u, h.v));
+            const float ux = flerp(UV[0], UV[2], UV[4], h.u, h.v), uy = flerp(UV[1], UV[3], UV[5], h.u, h.v);
+            float lam = dot3(neg(dir), nh);
+            lam = lam < 0.0f ? 0.0f : lam;
+            const float inf = lam * 1.0f;
+            const float hr = cr * A[0] * inf, hg = cg * A[1] * inf, hb = cb * A[2] * inf;
+            const int32_t pm = mesh_pm[S.tmesh[h.tri]];
+            if (pm >= 0) {
+                const int x = lumel_xy(ux, M.w[pm]), y = lumel_xy(uy, M.h[pm]);
+                const uint32_t t = (uint32_t)(y * M.w[pm] + x);
+                if (M.valid[M.base[pm] + t]) rec[(size_t)p * (D + 1) + depth] = PRec{M.base[pm] + t, hr, hg, hb};
+            }
+            if (s->termination == 1) {
+                const float q = (A[0] + A[1] + A[2]) / 3.0f;
+                if (!(rng_f(g) < q)) break;
+                cr = hr / q; cg = hg / q; cb = hb / q;
+            } else {
+                cr = hr; cg = hg; cb = hb;
+            }
+            pos = hp;
+            dir = hemi_uniform(g, nh);
+        }
+    });
+    // Sequential store in emission order (Photons::store: m_color += color; m_photons++).
+    for (const PRec& q : rec) {
+        if (q.key == 0xFFFFFFFFu) continue;
+        M.color[3 * (size_t)q.key] += q.r;
+        M.color[3 * (size_t)q.key + 1] += q.g;
+        M.color[3 * (size_t)q.key + 2] += q.b;
+        M.photons[q.key]++;
+    }
+    const int R = s->final_gather_radius;
+    for (int k = 0; k < npm; k++) {
+        const int W = M.w[k], H = M.h[k];
+        for (int y = 0; y < H; y++) {
+            for (int x = 0; x < W; x++) {
+                float sr = 0.0f, sg_ = 0.0f, sb = 0.0f;
+                int cnt = 0;
+                for (int j = y - R; j <= y + R; j++) {
+                    for (int i = x - R; i <= x + R; i++) {
+                        if (i < 0 || j < 0 || i >= W || j >= H) continue;
+                        const float dd = sqrtf((float)((x - i) * (x - i) + (y - j) * (y - j)));
+                        if (dd > (float)R) continue;
+                        const size_t t = M.base[k] + (size_t)j * W + i;
+                        sr += M.color[3 * t]; sg_ += M.color[3 * t + 1]; sb += M.color[3 * t + 2];
+                        cnt += (int)M.photons[t];
+                    }
+                }
+                const size_t o = M.base[k] + (size_t)y * W + x;
+                if (cnt == 0) {
+                    M.gathered[3 * o] = M.gathered[3 * o + 1] = M.gathered[3 * o + 2] = 0.0f;
+                } else {
+                    M.gathered[3 * o] = sr / (float)cnt;
+                    M.gathered[3 * o + 1] = sg_ / (float)cnt;
+                    M.gathered[3 * o + 2] = sb / (float)cnt;
+                }
+            }
+        }
+    }
+    uint64_t tot = 0;
+    for (uint64_t x : rays) tot += x;
+    if (rays_out) *rays_out = tot;
+    return 0;
+}
+
+// IndirectLight::bakeLumel (baker/IndirectLight.cpp:47-75) over n lumels; color is in/out.
+int32_t or_final_gather(void* sp, uint32_t n, const float* pos, const float* nrm, const uint32_t* texel,
+                        uint32_t lm_seed, const or_indirect* s, const int32_t* mesh_pm, int32_t npm, const int32_t* pw,
+                        const int32_t* ph, const float* gathered, float* color, int32_t threads, uint64_t* rays_out) {
+    const Scene& S = *(Scene*)sp;
+    PMaps M = make_pmaps(npm, pw, ph, nullptr, nullptr, nullptr, const_cast<float*>(gathered));
+    parallel_for(n, threads, [&](uint32_t i, int) {
+        const Vec p = mk(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2]);
+        const Vec nn = mk(nrm[3 * i], nrm[3 * i + 1], nrm[3 * i + 2]);
+        float gr = 0.0f, gg = 0.0f, gb = 0.0f;
+        for (int k = 0; k < s->final_gather_samples; k++) {
+            Rng g = stream(s->seed, lm_seed, texel[i], 2u, (uint32_t)k);
+            const Vec dir = hemi(g, nn, s->sampling);
+            float inf = dot3(nn, dir);
+            inf = inf > 0.0f ? inf : 0.0f;
+            HitRec h;
+            if (!trace(S, make_seg(p, add(p, scale(dir, s->final_gather_distance))), false, h, nullptr)) {
+                gr += s->sky_color[0] * inf + s->ambient_color[0];
+                gg += s->sky_color[1] * inf + s->ambient_color[1];
+                gb += s->sky_color[2] * inf + s->ambient_color[2];
+                continue;
+            }
+            const float* N = &S.tnrm[9 * (size_t)h.tri];
+            const Vec nh = mk(flerp(N[0], N[3], N[6], h.u, h.v), flerp(N[1], N[4], N[7], h.u, h.v), flerp(N[2], N[5], N[8], h.u, h.v));
+            if (dot3(dir, nh) >= 0.0f) continue;
+            const int32_t pm = mesh_pm[S.tmesh[h.tri]];
+            if (pm < 0) continue;
+            const float* UV = &S.tuv[6 * (size_t)h.tri];
+            const float ux = flerp(UV[0], UV[2], UV[4], h.u, h.v), uy = flerp(UV[1], UV[3], UV[5], h.u, h.v);
+            const int x = lumel_xy(ux, M.w[pm]), y = lumel_xy(uy, M.h[pm]);
+            const size_t t = M.base[pm] + (size_t)y * M.w[pm] + x;
+            gr += M.gathered[3 * t] * inf + s->ambient_color[0];
+            gg += M.gathered[3 * t + 1] * inf + s->ambient_color[1];
+            gb += M.gathered[3 * t + 2] * inf + s->ambient_color[2];
+        }
+        const float fs = (float)s->final_gather_samples;
+        color[3 * i] += gr / fs;
+        color[3 * i + 1] += gg / fs;
+        color[3 * i + 2] += gb / fs;
+    });
+    if (rays_out) *rays_out = (uint64_t)n * (uint64_t)s->final_gather_samples;
+    return 0;
+}
 
 }  // extern "C"

@@ -78,12 +78,39 @@ class rl_hit(C.Structure):
     ]
 
 
+RL_PHOTON_THRESHOLD = 0
+RL_PHOTON_RUSSIAN_ROULETTE = 1
+
+
+class rl_indirect_settings(C.Structure):
+    _fields_ = [
+        ("photon_pass_count", C.c_int32),
+        ("photon_bounce_count", C.c_int32),
+        ("photon_energy_threshold", C.c_float),
+        ("photon_max_distance", C.c_float),
+        ("final_gather_samples", C.c_int32),
+        ("final_gather_distance", C.c_float),
+        ("final_gather_radius", C.c_int32),
+        ("sky_color", C.c_float * 3),
+        ("ambient_color", C.c_float * 3),
+        ("seed", C.c_uint32),
+        ("sampling", C.c_int32),
+        ("photons_per_light", C.c_int32),
+        ("termination", C.c_int32),
+    ]
+
+
 RL_KERNEL_DIRECT = 0
 RL_KERNEL_AO_COUNT = 1
 RL_KERNEL_AO_FINALIZE = 2
 RL_KERNEL_TRACE = 3
+RL_KERNEL_PHOTON_TRACE = 4
+RL_KERNEL_PHOTON_STORE = 5
+RL_KERNEL_PHOTON_GATHER = 6
+RL_KERNEL_FINAL_GATHER = 7
 RL_KERNEL_SLOTS = 8
-KERNEL_NAMES = {0: "direct", 1: "ao_count", 2: "ao_finalize", 3: "trace"}
+KERNEL_NAMES = {0: "direct", 1: "ao_count", 2: "ao_finalize", 3: "trace", 4: "photon_trace",
+                5: "photon_store", 6: "photon_gather", 7: "final_gather"}
 
 
 class rl_stats(C.Structure):
@@ -130,6 +157,11 @@ SIGNATURES = [
                                   C.POINTER(rl_ao_settings)]),
     ("rl_trace_segments", C.c_int32, [_P, _P, _F, _F, C.c_uint32, C.c_int32,
                                       C.POINTER(C.c_uint8), C.POINTER(rl_hit)]),
+    ("rl_photonmap_create", C.c_int32, [_P, C.c_int32, C.c_int32, _PP]),
+    ("rl_photonmap_add_mesh", C.c_int32, [_P, _P, C.c_int32]),
+    ("rl_photonmap_read", C.c_int32, [_P, _F, _U32, _F]),
+    ("rl_emit_photons", C.c_int32, [_P, _P, C.POINTER(rl_indirect_settings)]),
+    ("rl_bake_indirect", C.c_int32, [_P, _P, C.c_int32, C.POINTER(rl_indirect_settings)]),
     ("rl_get_stats", C.c_int32, [_P, C.POINTER(rl_stats)]),
 ]
 

@@ -85,4 +85,48 @@ struct AoParams {
     int32_t sampling;
 };
 
+// A photonmap (Lightmap.h:153-170) as full w*h grids: accumulated photon power (rgb), photon
+// count, gathered colour, and the lumel validity the photon store checks.
+struct PhotonMapDev {
+    float4* color;
+    uint32_t* photons;
+    float4* gathered;
+    const uint8_t* valid;
+    int32_t w, h;
+    uint32_t key_base;  // first global texel key of this map (maps are concatenated)
+    uint32_t pad;
+};
+
+enum PhotonTermination : int32_t {
+    PHOTON_THRESHOLD = 0,        // depth > maxDepth or luminance < threshold (baker/Photons.cpp:98)
+    PHOTON_RUSSIAN_ROULETTE = 1, // survive with probability mean(albedo), power divided by it
+};
+
+struct PhotonParams {
+    uint64_t per_pass;  // photons per pass over all lights
+    uint64_t total;     // passes * per_pass
+    uint32_t seed;
+    int32_t max_depth;
+    float energy_threshold;
+    float max_distance;
+    int32_t termination;
+    int32_t nlights;
+    float scene_lo[3], scene_hi[3];
+};
+
+// One photon store: global texel key (0xFFFFFFFF = nothing stored) and its power.
+struct PhotonRec {
+    uint32_t key;
+    float r, g, b;
+};
+
+struct FinalGatherParams {
+    int32_t samples;
+    float distance;
+    float sky[3];
+    float ambient[3];
+    uint32_t seed;
+    int32_t sampling;
+};
+
 }  // namespace rl

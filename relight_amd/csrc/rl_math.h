@@ -101,9 +101,9 @@ enum Sampling : int32_t {
     SAMPLE_COSINE = 1,   // Malley's method over a rejection-sampled unit disk
 };
 
-// Vec3::randomDirection + randomHemisphereDirection (Types.h:924-949) with PCG draws. A zero
-// length draw is rejected as well (the reference would divide by zero there).
-RL_HD V3 sample_uniform_hemisphere(Pcg32& r, V3 n) {
+// Vec3::randomDirection (Types.h:924-937) with PCG draws: rejection in [-1,1]^3, normalised.
+// A zero-length draw is rejected as well (the reference would divide by zero there).
+RL_HD V3 sample_sphere(Pcg32& r) {
     V3 d;
     float len;
     do {
@@ -112,7 +112,12 @@ RL_HD V3 sample_uniform_hemisphere(Pcg32& r, V3 n) {
         d.z = pcg32_float(r) * 2.0f - 1.0f;
         len = length(d);
     } while (len > 1.0f || len == 0.0f);
-    d = d / len;
+    return d / len;
+}
+
+// Vec3::randomHemisphereDirection (Types.h:940-949): a sphere direction flipped towards n.
+RL_HD V3 sample_uniform_hemisphere(Pcg32& r, V3 n) {
+    V3 d = sample_sphere(r);
     if (dot(d, n) < 0.0f) d = -d;
     return d;
 }
@@ -233,6 +238,20 @@ RL_HD float attenuation(float distance, float radius, float c, float l, float q)
     float r = distance / radius;
     float v = 1.0f / (1.0f + c + l * r + q * r * r);
     return v > 0.0f ? v : 0.0f;
+}
+
+// Rgb::luminance (Types.h:211-213).
+RL_HD float luminance(float r, float g, float b) { return 0.2126f * r + 0.7152f * g + 0.0722f * b; }
+
+// Face::*At interpolation with Embree's hit barycentrics (rt/Embree.cpp:160,169-171 pass (u, v)
+// straight into Face::normalAt/uvAt, scene/Mesh.cpp:454-495): a + (b - a) * v + (c - a) * u.
+RL_HD float face_lerp(float a, float b, float c, float u, float v) { return a + (b - a) * v + (c - a) * u; }
+
+// Lightmap::lumel(const Uv&) (Lightmap.cpp:61-67): int(uv * (size - 1)); clamped to the map
+// (the reference asserts instead).
+RL_HD int lumel_coord(float uv, int size) {
+    int x = (int)(uv * (float)(size - 1));
+    return x < 0 ? 0 : (x >= size ? size - 1 : x);
 }
 
 // LightInfluence::lambert (scene/Light.cpp:392-396).

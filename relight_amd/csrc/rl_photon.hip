@@ -1,0 +1,356 @@
+// Photon emission and tracing, the deterministic photon store, the photonmap gather and the
+// final gather: Photons::emit/trace/store (baker/Photons.cpp:50-144), Photonmap::gather
+// (Lightmap.cpp:376-413) and IndirectLight::bakeLumel (baker/IndirectLight.cpp:47-75).
+//
+// Photon stores land in fixed record slots (photon-major, bounce-minor), are grouped by texel with
+// a stable radix sort and summed in record order, so the photonmap equals a sequential store in
+// the reference's emission order, bit for bit, on every run.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include "rl_device.h"
+#include "rl_kernels.h"
+#include "rl_traverse.h"
+
+namespace rl {
+
+namespace {
+
+using namespace trav;
+
+constexpr int kBlock = 128;
+
+__device__ __forceinline__ int find_map(const PhotonMapDev* pms, int npm, uint32_t key) {
+    int lo = 0, hi = npm - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (pms[mid].key_base <= key) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Hit attributes through Face::normalAt / uvAt with Embree's (u, v) (rt/Embree.cpp:160-171).
+struct HitAttr {
+    V3 n;
+    float uvx, uvy;
+    uint32_t mesh;
+    float albedo[4];
+};
+
+__device__ __forceinline__ HitAttr hit_attributes(const SceneDev& S, const TraceResult& tr) {
+    const TriAttr& A = S.attr[tr.tri];
+    const float u = tr.V / tr.det, v = tr.W / tr.det;
+    HitAttr h;
+    h.n = v3(face_lerp(A.n[0], A.n[3], A.n[6], u, v), face_lerp(A.n[1], A.n[4], A.n[7], u, v),
+             face_lerp(A.n[2], A.n[5], A.n[8], u, v));
+    h.uvx = face_lerp(A.uv[0], A.uv[2], A.uv[4], u, v);
+    h.uvy = face_lerp(A.uv[1], A.uv[3], A.uv[5], u, v);
+    h.mesh = A.mesh;
+    h.albedo[0] = A.albedo[0]; h.albedo[1] = A.albedo[1]; h.albedo[2] = A.albedo[2]; h.albedo[3] = A.albedo[3];
+    return h;
+}
+
+// One thread per photon: emission (PhotonEmitter::emit / DirectionalPhotonEmitter::emit,
+// scene/Light.cpp:339-361), spot cutoff, then the bounce loop of Photons::trace. All draws of a
+// photon come from one PCG32 stream keyed by its global index.
+template <int CAP>
+__global__ __launch_bounds__(kBlock) void k_photon_trace(SceneDev S, const LightDev* __restrict__ lights,
+                                                         const uint64_t* __restrict__ light_first,
+                                                         PhotonParams P, const int32_t* __restrict__ mesh_pm,
+                                                         const PhotonMapDev* __restrict__ pms,
+                                                         PhotonRec* __restrict__ out,
+                                                         unsigned long long* rays_out) {
+    __shared__ uint32_t stack[CAP * kBlock];
+    uint32_t* stk = stack + threadIdx.x;
+    const uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    uint32_t rays = 0;
+    if (p < P.total) {
+        const uint64_t r = p % P.per_pass;
+        int l = 0;
+        while (l + 1 < P.nlights && light_first[l + 1] <= r) l++;
+        const LightDev L = lights[l];
+        Pcg32 rng = sample_stream(P.seed, 0xFFFFFFFFu, (uint32_t)p, TECH_PHOTON, (uint32_t)(p >> 32));
+        V3 pos, dir;
+        if (L.type == LIGHT_DIRECTIONAL) {
+            // Plane::calculate(dir, light position) applied to Bounds::randomPointInside, minus
+            // dir * 5 (Types.h:1445-1450,1655-1658,1715-1718; lerp = a*(1-t) + b*t, Types.h:75-78).
+            const V3 n = v3(L.dir[0], L.dir[1], L.dir[2]);
+            const float tx = pcg32_float(rng), ty = pcg32_float(rng), tz = pcg32_float(rng);
+            const V3 q = v3(P.scene_lo[0] * (1.0f - tx) + P.scene_hi[0] * tx,
+                            P.scene_lo[1] * (1.0f - ty) + P.scene_hi[1] * ty,
+                            P.scene_lo[2] * (1.0f - tz) + P.scene_hi[2] * tz);
+            const float d = -dot(n, v3(L.pos[0], L.pos[1], L.pos[2]));
+            const float dist = dot(n, q) + d;
+            pos = (q - n * dist) - n * 5.0f;
+            dir = n;
+        } else {
+            pos = v3(L.pos[0], L.pos[1], L.pos[2]);
+            dir = sample_sphere(rng);
+        }
+        float cut = 1.0f;
+        if (L.has_spot) {
+            // LightSpotCutoff::cutoffForDirection (scene/Light.cpp:462-476).
+            float value = dot(dir, v3(L.dir[0], L.dir[1], L.dir[2]));
+            if (value <= L.cutoff) {
+                cut = 0.0f;
+            } else {
+                value = (1.0f - (1.0f - value) * 1.0f / (1.0f - L.cutoff));
+                if (fabsf(1.0f - L.spot_exponent) > 0.01f) value = powf(value, L.spot_exponent);
+                cut = value;
+            }
+        }
+        if (cut > 0.0f) {
+            float cr = L.color[0] * L.intensity * cut;
+            float cg = L.color[1] * L.intensity * cut;
+            float cb = L.color[2] * L.intensity * cut;
+            PhotonRec* slot = out + p * (uint64_t)(P.max_depth + 1);
+            for (int depth = 0; depth <= P.max_depth; depth++) {
+                if (P.termination == PHOTON_THRESHOLD && luminance(cr, cg, cb) < P.energy_threshold) break;
+                const Ray ray = ray_from_segment(pos, pos + dir * P.max_distance);
+                TraceResult tr;
+                rays++;
+                if (!traverse<false, CAP>(S, ray, stk, kBlock, tr)) break;
+                const V3 hp = ray.o + ray.d * tr.t;
+                const HitAttr h = hit_attributes(S, tr);
+                const float influence = lambert(-dir, h.n) * 1.0f;
+                const float hr = cr * h.albedo[0] * influence;
+                const float hg = cg * h.albedo[1] * influence;
+                const float hb = cb * h.albedo[2] * influence;
+                const int32_t pm = mesh_pm[h.mesh];
+                if (pm >= 0) {
+                    const PhotonMapDev M = pms[pm];
+                    const int x = lumel_coord(h.uvx, M.w), y = lumel_coord(h.uvy, M.h);
+                    const uint32_t t = (uint32_t)y * (uint32_t)M.w + (uint32_t)x;
+                    if (M.valid[t]) {
+                        PhotonRec rec;
+                        rec.key = M.key_base + t;
+                        rec.r = hr; rec.g = hg; rec.b = hb;
+                        slot[depth] = rec;
+                    }
+                }
+                if (P.termination == PHOTON_RUSSIAN_ROULETTE) {
+                    const float q = (h.albedo[0] + h.albedo[1] + h.albedo[2]) / 3.0f;
+                    if (!(pcg32_float(rng) < q)) break;
+                    cr = hr / q; cg = hg / q; cb = hb / q;
+                } else {
+                    cr = hr; cg = hg; cb = hb;
+                }
+                pos = hp;
+                dir = sample_uniform_hemisphere(rng, h.n);
+            }
+        }
+    }
+    count_rays(rays, rays_out);
+}
+
+__global__ void k_photon_clear(PhotonRec* out, uint64_t n) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n) {
+        PhotonRec r;
+        r.key = 0xFFFFFFFFu;
+        r.r = r.g = r.b = 0.0f;
+        out[g] = r;
+    }
+}
+
+__global__ void k_keys_iota(const PhotonRec* __restrict__ rec, uint32_t* __restrict__ keys,
+                            uint32_t* __restrict__ idx, uint32_t n) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n) {
+        keys[g] = rec[g].key;
+        idx[g] = g;
+    }
+}
+
+// Photons::store (baker/Photons.cpp:129-144) over records grouped by texel: one thread per run
+// head, summing in record order on top of the texel's current value.
+__global__ void k_photon_accumulate(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ idx,
+                                    const PhotonRec* __restrict__ rec, uint32_t n,
+                                    const PhotonMapDev* __restrict__ pms, int npm) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const uint32_t key = keys[g];
+    if (key == 0xFFFFFFFFu) return;
+    if (g > 0 && keys[g - 1] == key) return;
+    const PhotonMapDev M = pms[find_map(pms, npm, key)];
+    const uint32_t t = key - M.key_base;
+    float4 c = M.color[t];
+    uint32_t cnt = M.photons[t];
+    for (uint32_t j = g; j < n && keys[j] == key; j++) {
+        const PhotonRec r = rec[idx[j]];
+        c.x += r.r; c.y += r.g; c.z += r.b;
+        cnt++;
+    }
+    M.color[t] = c;
+    M.photons[t] = cnt;
+}
+
+// Photonmap::gather(x, y, radius) (Lightmap.cpp:386-413) for every texel of every map.
+__global__ void k_photon_gather(const PhotonMapDev* __restrict__ pms, int npm, uint32_t total, int radius) {
+    const uint32_t key = blockIdx.x * blockDim.x + threadIdx.x;
+    if (key >= total) return;
+    const PhotonMapDev M = pms[find_map(pms, npm, key)];
+    const uint32_t t = key - M.key_base;
+    const int x = (int)(t % (uint32_t)M.w), y = (int)(t / (uint32_t)M.w);
+    float cr = 0.0f, cg = 0.0f, cb = 0.0f;
+    int photons = 0;
+    for (int j = y - radius; j <= y + radius; j++) {
+        for (int i = x - radius; i <= x + radius; i++) {
+            if (i < 0 || j < 0 || i >= M.w || j >= M.h) continue;
+            const float distance = sqrtf((float)((x - i) * (x - i) + (y - j) * (y - j)));
+            if (distance > (float)radius) continue;
+            const uint32_t k = (uint32_t)j * (uint32_t)M.w + (uint32_t)i;
+            const float4 c = M.color[k];
+            cr += c.x; cg += c.y; cb += c.z;
+            photons += (int)M.photons[k];
+        }
+    }
+    float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (photons != 0) {
+        const float fp = (float)photons;
+        out = make_float4(cr / fp, cg / fp, cb / fp, 0.0f);
+    }
+    M.gathered[t] = out;
+}
+
+// IndirectLight::bakeLumel: one wavefront per lumel, 64 samples per step; lane 0 adds the
+// per-sample terms in sample order, as the reference's loop does.
+template <int CAP>
+__global__ __launch_bounds__(kBlock) void k_final_gather(SceneDev S, const BakeSeg* __restrict__ segs, int nsegs,
+                                                         uint64_t total, FinalGatherParams P,
+                                                         const int32_t* __restrict__ mesh_pm,
+                                                         const PhotonMapDev* __restrict__ pms,
+                                                         unsigned long long* rays_out) {
+    __shared__ uint32_t stack[CAP * kBlock];
+    __shared__ float terms[kBlock][4];
+    uint32_t* stk = stack + threadIdx.x;
+    const uint32_t lane = lane_id();
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint64_t g = (uint64_t)blockIdx.x * (kBlock / 64) + wave;  // lumel of this wavefront
+    const bool active = g < total;
+    const BakeSeg& sg = segs[find_seg(segs, nsegs, active ? g : total - 1)];
+    const uint32_t i = (uint32_t)((active ? g : total - 1) - sg.work_begin);
+    const float4 p4 = sg.pos[i], n4 = sg.nrm[i];
+    const V3 p = v3(p4.x, p4.y, p4.z), n = v3(n4.x, n4.y, n4.z);
+    const uint32_t texel = f_bits(p4.w);
+    float ar = 0.0f, ag = 0.0f, ab = 0.0f;
+    uint32_t rays = 0;
+    for (int base = 0; base < P.samples; base += 64) {
+        const int s = base + (int)lane;
+        float tr_ = 0.0f, tg_ = 0.0f, tb_ = 0.0f, use = 0.0f;
+        if (active && s < P.samples) {
+            Pcg32 rng = sample_stream(P.seed, sg.lm_seed, texel, TECH_FINAL_GATHER, (uint32_t)s);
+            const V3 dir = sample_hemisphere(rng, n, P.sampling);
+            const float d0 = dot(n, dir);
+            const float influence = d0 > 0.0f ? d0 : 0.0f;
+            const Ray ray = ray_from_segment(p, p + dir * P.distance);
+            TraceResult tr;
+            rays++;
+            if (!traverse<false, CAP>(S, ray, stk, kBlock, tr)) {
+                tr_ = P.sky[0] * influence + P.ambient[0];
+                tg_ = P.sky[1] * influence + P.ambient[1];
+                tb_ = P.sky[2] * influence + P.ambient[2];
+                use = 1.0f;
+            } else {
+                const HitAttr h = hit_attributes(S, tr);
+                if (dot(dir, h.n) < 0.0f) {
+                    const int32_t pm = mesh_pm[h.mesh];
+                    if (pm >= 0) {
+                        const PhotonMapDev M = pms[pm];
+                        const int x = lumel_coord(h.uvx, M.w), y = lumel_coord(h.uvy, M.h);
+                        const float4 gc = M.gathered[(uint32_t)y * (uint32_t)M.w + (uint32_t)x];
+                        tr_ = gc.x * influence + P.ambient[0];
+                        tg_ = gc.y * influence + P.ambient[1];
+                        tb_ = gc.z * influence + P.ambient[2];
+                        use = 1.0f;
+                    }
+                }
+            }
+        }
+        terms[threadIdx.x][0] = tr_;
+        terms[threadIdx.x][1] = tg_;
+        terms[threadIdx.x][2] = tb_;
+        terms[threadIdx.x][3] = use;
+        __syncthreads();
+        if (lane == 0) {
+            const int nb = min(64, P.samples - base);
+            for (int k = 0; k < nb; k++) {
+                const float* tk = terms[threadIdx.x + k];
+                if (tk[3] != 0.0f) { ar += tk[0]; ag += tk[1]; ab += tk[2]; }
+            }
+        }
+        __syncthreads();
+    }
+    if (active && lane == 0) {
+        float4 c = sg.color[i];
+        const float fs = (float)P.samples;
+        c.x += ar / fs; c.y += ag / fs; c.z += ab / fs;
+        sg.color[i] = c;
+    }
+    count_rays(rays, rays_out);
+}
+
+uint32_t blocks_for(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
+
+}  // namespace
+
+#define RL_DISPATCH_CAP(cap, ...)   \
+    do {                            \
+        if ((cap) == 32) {          \
+            constexpr int CAP = 32; \
+            __VA_ARGS__;            \
+        } else {                    \
+            constexpr int CAP = 64; \
+            __VA_ARGS__;            \
+        }                           \
+    } while (0)
+
+hipError_t launch_photon_trace(hipStream_t st, const SceneDev& S, int cap, const LightDev* lights,
+                               const uint64_t* light_first, const PhotonParams& P, const int32_t* mesh_pm,
+                               const PhotonMapDev* pms, PhotonRec* out, unsigned long long* rays) {
+    const uint64_t nrec = P.total * (uint64_t)(P.max_depth + 1);
+    if (nrec == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_photon_clear, dim3(blocks_for(nrec, 256)), dim3(256), 0, st, out, nrec);
+    RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_photon_trace<CAP>, dim3(blocks_for(P.total, kBlock)), dim3(kBlock), 0,
+                                            st, S, lights, light_first, P, mesh_pm, pms, out, rays));
+    return hipGetLastError();
+}
+
+hipError_t photon_sort_temp_bytes(uint32_t n, size_t* bytes) {
+    *bytes = 0;
+    return hipcub::DeviceRadixSort::SortPairs(nullptr, *bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                              (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n);
+}
+
+hipError_t launch_photon_store(hipStream_t st, const PhotonRec* rec, uint32_t n, uint32_t* keys_a,
+                               uint32_t* keys_b, uint32_t* idx_a, uint32_t* idx_b, void* temp,
+                               size_t temp_bytes, const PhotonMapDev* pms, int npm) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_keys_iota, dim3(blocks_for(n, 256)), dim3(256), 0, st, rec, keys_a, idx_a, n);
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys_a, keys_b, idx_a, idx_b, (int)n, 0,
+                                                      32, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_photon_accumulate, dim3(blocks_for(n, 256)), dim3(256), 0, st, keys_b, idx_b, rec, n,
+                       pms, npm);
+    return hipGetLastError();
+}
+
+hipError_t launch_photon_gather(hipStream_t st, const PhotonMapDev* pms, int npm, uint32_t total_texels,
+                                int radius) {
+    if (total_texels == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_photon_gather, dim3(blocks_for(total_texels, 256)), dim3(256), 0, st, pms, npm,
+                       total_texels, radius);
+    return hipGetLastError();
+}
+
+hipError_t launch_final_gather(hipStream_t st, const SceneDev& S, int cap, const BakeSeg* segs, int nsegs,
+                               uint64_t total_lumels, const FinalGatherParams& P, const int32_t* mesh_pm,
+                               const PhotonMapDev* pms, unsigned long long* rays) {
+    if (total_lumels == 0) return hipSuccess;
+    const uint32_t blocks = blocks_for(total_lumels, kBlock / 64);
+    RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_final_gather<CAP>, dim3(blocks), dim3(kBlock), 0, st, S, segs, nsegs,
+                                            total_lumels, P, mesh_pm, pms, rays));
+    return hipGetLastError();
+}
+
+}  // namespace rl

@@ -61,6 +61,7 @@ struct MeshHost {
     uint32_t first_tri = 0;
     uint32_t num_tris = 0;
     rl_lightmap* lightmap = nullptr;
+    rl_lightmap* photonmap = nullptr;
 };
 
 struct LightHost {
@@ -95,6 +96,7 @@ struct rl_scene {
     uint32_t num_nodes = 0;
     int stack_bound = 0;
     int cap = 0;
+    float bounds_lo[3] = {0, 0, 0}, bounds_hi[3] = {0, 0, 0};
     Bvh4Node* d_nodes = nullptr;
     float4* d_tris = nullptr;
     TriAttr* d_attr = nullptr;
@@ -139,8 +141,21 @@ struct rl_lightmap {
     float4* d_color = nullptr;
     uint32_t* d_counts = nullptr;
     uint32_t d_capacity = 0;
+    // Photonmap role (Photonmap, Lightmap.h:153-170): full-grid photon power, count, gathered.
+    bool is_photonmap = false;
+    float4* d_pm_color = nullptr;
+    uint32_t* d_pm_photons = nullptr;
+    float4* d_pm_gathered = nullptr;
+    uint8_t* d_pm_valid = nullptr;
+    bool pm_valid_dirty = true;
 
-    ~rl_lightmap() { release(); }
+    ~rl_lightmap() {
+        release();
+        dev_free(d_pm_color);
+        dev_free(d_pm_photons);
+        dev_free(d_pm_gathered);
+        dev_free(d_pm_valid);
+    }
     void release() {
         dev_free(d_pos);
         dev_free(d_nrm);
@@ -413,6 +428,88 @@ int32_t run_ao(rl_relight* ctx, rl_scene* scene, rl_lightmap* const* maps, int n
     return RL_SUCCESS;
 }
 
+// Device tables of a scene's photonmaps (distinct maps in mesh order) and the mesh -> map index.
+struct PhotonTables {
+    std::vector<rl_lightmap*> maps;
+    std::vector<int32_t> mesh_pm;
+    std::vector<PhotonMapDev> dev;
+    uint64_t total_texels = 0;
+    int32_t* d_mesh_pm = nullptr;
+    PhotonMapDev* d_pms = nullptr;
+    ~PhotonTables() {
+        dev_free(d_mesh_pm);
+        dev_free(d_pms);
+    }
+};
+
+int32_t ensure_photonmap_device(rl_lightmap* pm) {
+    const size_t n = (size_t)pm->w * (size_t)pm->h;
+    if (!pm->d_pm_color) {
+        RL_HIP(dev_alloc(&pm->d_pm_color, n));
+        RL_HIP(dev_alloc(&pm->d_pm_photons, n));
+        RL_HIP(dev_alloc(&pm->d_pm_gathered, n));
+        RL_HIP(dev_alloc(&pm->d_pm_valid, n));
+        RL_HIP(hipMemset(pm->d_pm_color, 0, n * sizeof(float4)));
+        RL_HIP(hipMemset(pm->d_pm_photons, 0, n * sizeof(uint32_t)));
+        RL_HIP(hipMemset(pm->d_pm_gathered, 0, n * sizeof(float4)));
+        pm->pm_valid_dirty = true;
+    }
+    if (pm->pm_valid_dirty) {
+        RL_HIP(hipMemcpy(pm->d_pm_valid, pm->valid.data(), n, hipMemcpyHostToDevice));
+        pm->pm_valid_dirty = false;
+    }
+    return RL_SUCCESS;
+}
+
+int32_t make_photon_tables(rl_scene* scene, PhotonTables& T) {
+    T.mesh_pm.assign(scene->meshes.size(), -1);
+    for (size_t i = 0; i < scene->meshes.size(); i++) {
+        rl_lightmap* pm = scene->meshes[i].photonmap;
+        if (!pm) continue;
+        auto it = std::find(T.maps.begin(), T.maps.end(), pm);
+        if (it == T.maps.end()) {
+            T.maps.push_back(pm);
+            it = T.maps.end() - 1;
+        }
+        T.mesh_pm[i] = (int32_t)(it - T.maps.begin());
+    }
+    for (rl_lightmap* pm : T.maps) {
+        int32_t st = ensure_photonmap_device(pm);
+        if (st) return st;
+        PhotonMapDev d;
+        d.color = pm->d_pm_color;
+        d.photons = pm->d_pm_photons;
+        d.gathered = pm->d_pm_gathered;
+        d.valid = pm->d_pm_valid;
+        d.w = pm->w;
+        d.h = pm->h;
+        d.key_base = (uint32_t)T.total_texels;
+        d.pad = 0;
+        T.total_texels += (uint64_t)pm->w * (uint64_t)pm->h;
+        if (T.total_texels >= 0xFFFFFFFFull) return fail(RL_REJECTED, "photonmaps exceed 2^32 texels");
+        T.dev.push_back(d);
+    }
+    RL_HIP(dev_alloc(&T.d_mesh_pm, T.mesh_pm.size()));
+    RL_HIP(hipMemcpy(T.d_mesh_pm, T.mesh_pm.data(), T.mesh_pm.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (!T.dev.empty()) {
+        RL_HIP(dev_alloc(&T.d_pms, T.dev.size()));
+        RL_HIP(hipMemcpy(T.d_pms, T.dev.data(), T.dev.size() * sizeof(PhotonMapDev), hipMemcpyHostToDevice));
+    }
+    return RL_SUCCESS;
+}
+
+int32_t check_indirect_settings(const rl_indirect_settings* s) {
+    if (!s) return fail(RL_INVALID_ARGUMENT, "indirect settings are NULL");
+    if (s->photon_pass_count < 0 || s->photon_bounce_count < 0 || s->final_gather_samples <= 0 ||
+        s->final_gather_radius < 0)
+        return fail(RL_INVALID_ARGUMENT, "bad indirect settings");
+    if (s->sampling != RL_SAMPLE_UNIFORM && s->sampling != RL_SAMPLE_COSINE)
+        return fail(RL_INVALID_ARGUMENT, "unknown sampling mode");
+    if (s->termination != RL_PHOTON_THRESHOLD && s->termination != RL_PHOTON_RUSSIAN_ROULETTE)
+        return fail(RL_INVALID_ARGUMENT, "unknown photon termination");
+    return RL_SUCCESS;
+}
+
 }  // namespace
 
 extern "C" {
@@ -591,6 +688,17 @@ int32_t rl_scene_end(rl_scene* scene) {
             std::memcpy(attr[t].albedo, m.albedo, sizeof(m.albedo));
             scene->tri_mesh[t] = mi;
             scene->tri_face[t] = f;
+        }
+    }
+    // Scene::bounds (scene/Scene.cpp:81-88): union of the meshes' vertex bounds.
+    for (int k = 0; k < 3; k++) {
+        scene->bounds_lo[k] = tri9[k];
+        scene->bounds_hi[k] = tri9[k];
+    }
+    for (size_t i = 0; i < tri9.size(); i += 3) {
+        for (int k = 0; k < 3; k++) {
+            scene->bounds_lo[k] = std::min(scene->bounds_lo[k], tri9[i + k]);
+            scene->bounds_hi[k] = std::max(scene->bounds_hi[k], tri9[i + k]);
         }
     }
     Bvh4Build bvh;
@@ -835,6 +943,179 @@ int32_t rl_trace_segments(rl_relight* ctx, rl_scene* scene, const float* starts,
         }
     }
     cleanup();
+    return end_call(ctx, tm);
+}
+
+int32_t rl_photonmap_create(rl_relight* ctx, int32_t width, int32_t height, rl_lightmap** out) {
+    int32_t st = rl_lightmap_create(ctx, width, height, out);
+    if (st) return st;
+    (*out)->is_photonmap = true;
+    return RL_SUCCESS;
+}
+
+int32_t rl_photonmap_add_mesh(rl_lightmap* pm, rl_scene* scene, int32_t mesh) {
+    if (!pm || !scene) return fail(RL_INVALID_ARGUMENT, "NULL argument");
+    if (!pm->is_photonmap) return fail(RL_INVALID_CALL, "not a photonmap");
+    if (mesh < 0 || mesh >= (int32_t)scene->meshes.size()) return fail(RL_INVALID_ARGUMENT, "bad mesh index");
+    if (pm->scene && pm->scene != scene) return fail(RL_INVALID_CALL, "photonmap already holds another scene's meshes");
+    MeshHost& m = scene->meshes[mesh];
+    // Photonmap::addMesh refuses a mesh that already has a photonmap (Lightmap.cpp:362-367).
+    if (m.photonmap) return fail(RL_INVALID_CALL, "mesh already has a photonmap");
+    m.photonmap = pm;
+    pm->scene = scene;
+    uint32_t first = 0;
+    for (int32_t i = 0; i < mesh; i++) first += scene->meshes[i].num_tris;
+    for (uint32_t f = 0; f < m.num_tris; f++) raster_face(pm, m, f, first + f);
+    pm->pm_valid_dirty = true;
+    return RL_SUCCESS;
+}
+
+int32_t rl_photonmap_read(rl_lightmap* pm, float* color, uint32_t* photons, float* gathered) {
+    if (!pm) return fail(RL_INVALID_ARGUMENT, "photonmap is NULL");
+    if (!pm->is_photonmap) return fail(RL_INVALID_CALL, "not a photonmap");
+    const size_t n = (size_t)pm->w * (size_t)pm->h;
+    if (!pm->d_pm_color) {
+        if (color) std::memset(color, 0, 3 * n * sizeof(float));
+        if (photons) std::memset(photons, 0, n * sizeof(uint32_t));
+        if (gathered) std::memset(gathered, 0, 3 * n * sizeof(float));
+        return RL_SUCCESS;
+    }
+    std::vector<float4> tmp(n);
+    for (int which = 0; which < 2; which++) {
+        float* dst = which == 0 ? color : gathered;
+        if (!dst) continue;
+        RL_HIP(hipMemcpy(tmp.data(), which == 0 ? pm->d_pm_color : pm->d_pm_gathered, n * sizeof(float4),
+                         hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < n; i++) {
+            dst[3 * i] = tmp[i].x;
+            dst[3 * i + 1] = tmp[i].y;
+            dst[3 * i + 2] = tmp[i].z;
+        }
+    }
+    if (photons) RL_HIP(hipMemcpy(photons, pm->d_pm_photons, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return RL_SUCCESS;
+}
+
+int32_t rl_emit_photons(rl_relight* ctx, rl_scene* scene, const rl_indirect_settings* s) {
+    if (!ctx) return fail(RL_INVALID_ARGUMENT, "context is NULL");
+    int32_t st = check_scene_ready(scene);
+    if (st) return st;
+    if ((st = check_indirect_settings(s))) return st;
+    PhotonTables T;
+    if ((st = make_photon_tables(scene, T))) return st;
+    CallTimer tm(ctx);
+    if ((st = begin_call(ctx))) return st;
+    if (T.maps.empty() || scene->lights.empty()) return end_call(ctx, tm);
+    // Photons::emit: passes x lights x PhotonEmitter::photonCount() = intensity * 25000
+    // (baker/Photons.cpp:50-92, scene/Light.cpp:334-337) unless photons_per_light overrides it.
+    const int nl = (int)scene->lights.size();
+    std::vector<uint64_t> first(nl);
+    uint64_t per_pass = 0;
+    for (int l = 0; l < nl; l++) {
+        first[l] = per_pass;
+        int64_t c = s->photons_per_light > 0 ? s->photons_per_light
+                                             : (int64_t)(int32_t)(scene->lights[l].dev.intensity * 25000.0f);
+        per_pass += (uint64_t)std::max<int64_t>(c, 0);
+    }
+    PhotonParams P;
+    P.per_pass = per_pass;
+    P.total = per_pass * (uint64_t)s->photon_pass_count;
+    P.seed = s->seed;
+    P.max_depth = s->photon_bounce_count;
+    P.energy_threshold = s->photon_energy_threshold;
+    P.max_distance = s->photon_max_distance;
+    P.termination = s->termination;
+    P.nlights = nl;
+    for (int k = 0; k < 3; k++) {
+        P.scene_lo[k] = scene->bounds_lo[k];
+        P.scene_hi[k] = scene->bounds_hi[k];
+    }
+    const uint64_t nrec = P.total * (uint64_t)(P.max_depth + 1);
+    if (nrec == 0) return end_call(ctx, tm);
+    if (nrec > 0x7FFFFFFFull) return fail(RL_REJECTED, "too many photon records for one emit call");
+    uint64_t* d_first = nullptr;
+    PhotonRec* d_rec = nullptr;
+    uint32_t *ka = nullptr, *kb = nullptr, *ia = nullptr, *ib = nullptr;
+    uint8_t* temp = nullptr;
+    size_t temp_bytes = 0;
+    auto cleanup = [&]() {
+        dev_free(d_first); dev_free(d_rec); dev_free(ka); dev_free(kb); dev_free(ia); dev_free(ib); dev_free(temp);
+    };
+    hipError_t e = dev_alloc(&d_first, (size_t)nl);
+    if (e == hipSuccess) e = dev_alloc(&d_rec, (size_t)nrec);
+    if (e == hipSuccess) e = dev_alloc(&ka, (size_t)nrec);
+    if (e == hipSuccess) e = dev_alloc(&kb, (size_t)nrec);
+    if (e == hipSuccess) e = dev_alloc(&ia, (size_t)nrec);
+    if (e == hipSuccess) e = dev_alloc(&ib, (size_t)nrec);
+    if (e == hipSuccess) e = photon_sort_temp_bytes((uint32_t)nrec, &temp_bytes);
+    if (e == hipSuccess) e = dev_alloc(&temp, std::max<size_t>(temp_bytes, 1));
+    if (e != hipSuccess) {
+        cleanup();
+        return fail(RL_OUT_OF_MEMORY, std::string("photon buffers: ") + hipGetErrorString(e));
+    }
+    e = hipMemcpy(d_first, first.data(), nl * sizeof(uint64_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = kernel_begin(ctx, RL_KERNEL_PHOTON_TRACE);
+    if (e == hipSuccess)
+        e = launch_photon_trace(ctx->stream, scene->dev(), scene->cap, scene->d_lights, d_first, P, T.d_mesh_pm,
+                                T.d_pms, d_rec, ctx->d_rays);
+    if (e == hipSuccess) e = kernel_end(ctx);
+    if (e == hipSuccess) e = kernel_begin(ctx, RL_KERNEL_PHOTON_STORE);
+    if (e == hipSuccess)
+        e = launch_photon_store(ctx->stream, d_rec, (uint32_t)nrec, ka, kb, ia, ib, temp, temp_bytes, T.d_pms,
+                                (int)T.dev.size());
+    if (e == hipSuccess) e = kernel_end(ctx);
+    // Relight::emitPhotons gathers every photonmap after the passes (Relight.cpp:250-254).
+    if (e == hipSuccess) e = kernel_begin(ctx, RL_KERNEL_PHOTON_GATHER);
+    if (e == hipSuccess)
+        e = launch_photon_gather(ctx->stream, T.d_pms, (int)T.dev.size(), (uint32_t)T.total_texels,
+                                 s->final_gather_radius);
+    if (e == hipSuccess) e = kernel_end(ctx);
+    ctx->stats.launches += 4;
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    cleanup();
+    if (e != hipSuccess) return fail(RL_DEVICE_ERROR, std::string("photon pass: ") + hipGetErrorString(e));
+    return end_call(ctx, tm);
+}
+
+int32_t rl_bake_indirect(rl_relight* ctx, rl_scene* scene, int32_t mesh, const rl_indirect_settings* s) {
+    if (!ctx) return fail(RL_INVALID_ARGUMENT, "context is NULL");
+    int32_t st = check_scene_ready(scene);
+    if (st) return st;
+    if ((st = check_indirect_settings(s))) return st;
+    if (mesh < 0 || mesh >= (int32_t)scene->meshes.size()) return fail(RL_INVALID_ARGUMENT, "bad mesh index");
+    if (scene->meshes[mesh].num_tris == 0) return fail(RL_INVALID_CALL, "mesh has no faces");
+    rl_lightmap* lm = scene->meshes[mesh].lightmap;
+    if (!lm) return fail(RL_INVALID_CALL, "mesh has no lightmap");
+    PhotonTables T;
+    if ((st = make_photon_tables(scene, T))) return st;
+    CallTimer tm(ctx);
+    if ((st = begin_call(ctx))) return st;
+    BakeSeg* d_segs = nullptr;
+    uint64_t total = 0, tex = 0;
+    if ((st = make_segments(ctx, scene, &lm, 1, 1, &d_segs, &total, &tex))) return st;
+    FinalGatherParams P;
+    P.samples = s->final_gather_samples;
+    P.distance = s->final_gather_distance;
+    for (int k = 0; k < 3; k++) {
+        P.sky[k] = s->sky_color[k];
+        P.ambient[k] = s->ambient_color[k];
+    }
+    P.seed = s->seed;
+    P.sampling = s->sampling;
+    hipError_t e = hipSuccess;
+    if (total) {
+        e = kernel_begin(ctx, RL_KERNEL_FINAL_GATHER);
+        if (e == hipSuccess)
+            e = launch_final_gather(ctx->stream, scene->dev(), scene->cap, d_segs, 1, total, P, T.d_mesh_pm, T.d_pms,
+                                    ctx->d_rays);
+        if (e == hipSuccess) e = kernel_end(ctx);
+    }
+    ctx->stats.texels += tex;
+    ctx->stats.launches += total ? 1 : 0;
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    dev_free(d_segs);
+    if (e != hipSuccess) return fail(RL_DEVICE_ERROR, std::string("final gather: ") + hipGetErrorString(e));
+    lm->device_newer = true;
     return end_call(ctx, tm);
 }
 

@@ -1,0 +1,141 @@
+// BVH4 traversal shared by every kernel that casts rays (device code only).
+//
+// Near-first traversal with a per-lane LDS stack (CAP entries, one column per lane), the
+// conservative slab test of Ize ("Robust BVH Ray Traversal", JCGT 2(2), 2013) and the watertight
+// triangle test of rl_math.h. Closest-hit ties at equal t go to the smaller scene-order triangle
+// id, so results do not depend on traversal order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "rl_device.h"
+
+namespace rl {
+namespace trav {
+
+// Slab-test safety factor 1 + 2*gamma(3) (Ize 2013): scales the exit distance so that the
+// rounded slab distances never reject a box the exact ray passes through.
+constexpr float kBoxScale = 1.0000004f;
+
+__device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
+
+__device__ __forceinline__ float bits_f(uint32_t u) { return __uint_as_float(u); }
+__device__ __forceinline__ uint32_t f_bits(float f) { return __float_as_uint(f); }
+
+struct TraceResult {
+    float t;
+    uint32_t tri;  // scene-order id, 0xFFFFFFFF on a miss
+    float U, V, W, det;
+};
+
+__device__ __forceinline__ void cswap(float& ka, uint32_t& va, float& kb, uint32_t& vb) {
+    if (kb < ka) {
+        float tk = ka; ka = kb; kb = tk;
+        uint32_t tv = va; va = vb; vb = tv;
+    }
+}
+
+// Traverses the BVH4 for one ray. ANY: stop at the first triangle with t in (tnear, tfar).
+// Closest: the hit with the smallest t, ties broken by the smaller scene-order triangle id, so
+// the result does not depend on traversal order. stk points at this lane's first LDS stack slot;
+// consecutive entries are `stride` words apart (one column per lane, conflict-free).
+template <bool ANY, int CAP>
+__device__ bool traverse(const SceneDev& S, const Ray& ray, uint32_t* stk, uint32_t stride,
+                         TraceResult& res) {
+    const WtRay w = wt_setup(ray);
+    const float ix = 1.0f / ray.d.x, iy = 1.0f / ray.d.y, iz = 1.0f / ray.d.z;
+    const float ox = ray.o.x, oy = ray.o.y, oz = ray.o.z;
+    const float tnear = ray.tnear;
+    float tlimit = ray.tfar;  // closest: shrinks to the best hit so far
+    res.t = ray.tfar;
+    res.tri = 0xFFFFFFFFu;
+    res.U = res.V = res.W = res.det = 0.0f;
+
+    uint32_t node = 0;
+    int sp = 0;
+    for (;;) {
+        if (!(node & RL_LEAF_BIT)) {
+            const float4* nd = reinterpret_cast<const float4*>(S.nodes + node);
+            const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
+            const uint4 ch = reinterpret_cast<const uint4*>(nd)[6];
+            const float lim = tlimit * kBoxScale;
+            float key[4];
+            uint32_t cid[4] = {ch.x, ch.y, ch.z, ch.w};
+            const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
+            const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
+            const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                float t0x = (lxa[i] - ox) * ix, t1x = (hxa[i] - ox) * ix;
+                float t0y = (lya[i] - oy) * iy, t1y = (hya[i] - oy) * iy;
+                float t0z = (lza[i] - oz) * iz, t1z = (hza[i] - oz) * iz;
+                float tmin = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tnear));
+                float tmax = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * kBoxScale;
+                bool hit = tmin <= fminf(tmax, lim) && cid[i] != RL_EMPTY_CHILD;
+                key[i] = hit ? tmin : __builtin_inff();
+            }
+            // Sort the four (distance, child) pairs: near child next, the rest pushed far-first.
+            cswap(key[0], cid[0], key[1], cid[1]);
+            cswap(key[2], cid[2], key[3], cid[3]);
+            cswap(key[0], cid[0], key[2], cid[2]);
+            cswap(key[1], cid[1], key[3], cid[3]);
+            cswap(key[1], cid[1], key[2], cid[2]);
+            if (key[0] == __builtin_inff()) {
+                if (sp == 0) break;
+                node = stk[(--sp) * stride];
+                continue;
+            }
+#pragma unroll
+            for (int i = 3; i >= 1; i--) {
+                if (key[i] != __builtin_inff() && sp < CAP) {
+                    stk[(sp++) * stride] = cid[i];
+                }
+            }
+            node = cid[0];
+        } else {
+            const uint32_t first = node & RL_LEAF_FIRST_MASK;
+            const uint32_t count = ((node >> 27) & 15u) + 1u;
+            for (uint32_t j = 0; j < count; j++) {
+                const float4* tp = S.tris + 3u * (first + j);
+                const float4 a = tp[0], b = tp[1], c = tp[2];
+                float t, U, V, W, det;
+                if (wt_intersect(w, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t, U, V, W, det)) {
+                    if (ANY) {
+                        res.t = t;
+                        res.tri = f_bits(a.w);
+                        return true;
+                    }
+                    const uint32_t id = f_bits(a.w);
+                    if (t < res.t || (t == res.t && id < res.tri)) {
+                        res.t = t;
+                        res.tri = id;
+                        res.U = U; res.V = V; res.W = W; res.det = det;
+                        tlimit = t;
+                    }
+                }
+            }
+            if (sp == 0) break;
+            node = stk[(--sp) * stride];
+        }
+    }
+    return res.tri != 0xFFFFFFFFu;
+}
+
+__device__ __forceinline__ int find_seg(const BakeSeg* segs, int nsegs, uint64_t g) {
+    int lo = 0, hi = nsegs - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (segs[mid].work_begin <= g) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void count_rays(uint32_t n, unsigned long long* counter) {
+    // Wave sum of per-lane ray counts, one atomic per wavefront.
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) n += __shfl_xor(n, off, 64);
+    if (lane_id() == 0 && n) atomicAdd(counter, (unsigned long long)n);
+}
+
+}  // namespace trav
+}  // namespace rl

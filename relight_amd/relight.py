@@ -68,6 +68,69 @@ class AmbientOcclusionSettings:
 
 
 @dataclass
+class IndirectLightSettings:
+    """IndirectLightSettings (Relight.h:108-132) + seed, sampling, photon count override and
+    termination rule."""
+    photon_pass_count: int = 16
+    photon_bounce_count: int = 3
+    photon_energy_threshold: float = 0.05
+    photon_max_distance: float = 10.0
+    final_gather_samples: int = 64
+    final_gather_distance: float = 50.0
+    final_gather_radius: int = 7
+    sky_color: Sequence[float] = (0.0, 0.0, 0.0)
+    ambient_color: Sequence[float] = (0.0, 0.0, 0.0)
+    seed: int = 1
+    sampling: int = A.RL_SAMPLE_UNIFORM
+    photons_per_light: int = 0
+    termination: int = A.RL_PHOTON_THRESHOLD
+
+    # Presets of Relight.cpp:42-120: (passes, bounces, final-gather samples).
+    _PRESETS = {"draft": (8, 1, 32), "fast": (16, 3, 64), "best": (32, 3, 128),
+                "production": (64, 4, 1024)}
+
+    @classmethod
+    def _preset(cls, name, sky, ambient, photon_max_distance, final_gather_distance):
+        p, b, s = cls._PRESETS[name]
+        return cls(p, b, 0.05, photon_max_distance, s, final_gather_distance, 7, tuple(sky),
+                   tuple(ambient))
+
+    @classmethod
+    def draft(cls, sky=(0, 0, 0), ambient=(0, 0, 0), photon_max_distance=10.0, final_gather_distance=50.0):
+        return cls._preset("draft", sky, ambient, photon_max_distance, final_gather_distance)
+
+    @classmethod
+    def fast(cls, sky=(0, 0, 0), ambient=(0, 0, 0), photon_max_distance=10.0, final_gather_distance=50.0):
+        return cls._preset("fast", sky, ambient, photon_max_distance, final_gather_distance)
+
+    @classmethod
+    def best(cls, sky=(0, 0, 0), ambient=(0, 0, 0), photon_max_distance=10.0, final_gather_distance=50.0):
+        return cls._preset("best", sky, ambient, photon_max_distance, final_gather_distance)
+
+    @classmethod
+    def production(cls, sky=(0, 0, 0), ambient=(0, 0, 0), photon_max_distance=10.0,
+                   final_gather_distance=50.0):
+        return cls._preset("production", sky, ambient, photon_max_distance, final_gather_distance)
+
+    def to_c(self) -> A.rl_indirect_settings:
+        s = A.rl_indirect_settings()
+        s.photon_pass_count = int(self.photon_pass_count)
+        s.photon_bounce_count = int(self.photon_bounce_count)
+        s.photon_energy_threshold = float(self.photon_energy_threshold)
+        s.photon_max_distance = float(self.photon_max_distance)
+        s.final_gather_samples = int(self.final_gather_samples)
+        s.final_gather_distance = float(self.final_gather_distance)
+        s.final_gather_radius = int(self.final_gather_radius)
+        s.sky_color[:] = [float(x) for x in self.sky_color]
+        s.ambient_color[:] = [float(x) for x in self.ambient_color]
+        s.seed = int(self.seed) & 0xFFFFFFFF
+        s.sampling = int(self.sampling)
+        s.photons_per_light = int(self.photons_per_light)
+        s.termination = int(self.termination)
+        return s
+
+
+@dataclass
 class LightDesc:
     """A light as Light::create* builds it (scene/Light.cpp:166-234)."""
     type: int
@@ -254,6 +317,29 @@ class Lightmap:
         return out
 
 
+class Photonmap(Lightmap):
+    """Photonmap (Lightmap.h:153-170): receives photon stores; gathered colour per texel."""
+
+    def __init__(self, relight: "Relight", width: int, height: int):
+        self._lib = relight._lib
+        self.width, self.height = int(width), int(height)
+        h = C.c_void_p()
+        _check(self._lib, self._lib.rl_photonmap_create(relight._h, self.width, self.height,
+                                                        C.byref(h)))
+        self._h = h
+
+    def add_mesh(self, scene: Scene, mesh: int):
+        _check(self._lib, self._lib.rl_photonmap_add_mesh(self._h, scene._h, int(mesh)))
+
+    def read(self):
+        """(color (h, w, 3), photons (h, w), gathered (h, w, 3))."""
+        c = np.zeros((self.height, self.width, 3), np.float32)
+        p = np.zeros((self.height, self.width), np.uint32)
+        g = np.zeros((self.height, self.width, 3), np.float32)
+        _check(self._lib, self._lib.rl_photonmap_read(self._h, _fptr(c), _uptr(p), _fptr(g)))
+        return c, p, g
+
+
 class Relight:
     """Relight (Relight.h:155-189): factories and bake entry points, all on the GPU."""
 
@@ -283,6 +369,17 @@ class Relight:
 
     def create_lightmap(self, width: int, height: int) -> Lightmap:
         return Lightmap(self, width, height)
+
+    def create_photonmap(self, width: int, height: int) -> Photonmap:
+        return Photonmap(self, width, height)
+
+    def emit_photons(self, scene: Scene, settings: IndirectLightSettings):
+        s = settings.to_c()
+        _check(self._lib, self._lib.rl_emit_photons(self._h, scene._h, C.byref(s)))
+
+    def bake_indirect_light(self, scene: Scene, mesh: int, settings: IndirectLightSettings):
+        s = settings.to_c()
+        _check(self._lib, self._lib.rl_bake_indirect(self._h, scene._h, int(mesh), C.byref(s)))
 
     def bake_direct_light(self, scene: Scene, mesh: int):
         _check(self._lib, self._lib.rl_bake_direct(self._h, scene._h, int(mesh)))

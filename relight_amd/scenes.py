@@ -18,8 +18,8 @@ from typing import List, Optional, Tuple
 
 import numpy as np
 
-from .relight import AmbientOcclusionSettings, LightDesc
-from ._capi import RL_SAMPLE_COSINE
+from .relight import AmbientOcclusionSettings, IndirectLightSettings, LightDesc
+from ._capi import RL_PHOTON_RUSSIAN_ROULETTE, RL_SAMPLE_COSINE, RL_SAMPLE_UNIFORM
 
 
 @dataclass
@@ -30,6 +30,7 @@ class MeshSpec:
     indices: np.ndarray
     lightmap: int
     albedo: Optional[np.ndarray] = None
+    photonmap: int = -1
 
     @property
     def num_tris(self):
@@ -44,6 +45,8 @@ class SceneSpec:
     lightmaps: List[Tuple[int, int]]
     ao: Optional[AmbientOcclusionSettings] = None
     params: dict = field(default_factory=dict)
+    photonmaps: List[Tuple[int, int]] = field(default_factory=list)
+    indirect: Optional[IndirectLightSettings] = None
 
     @property
     def num_tris(self):
@@ -217,8 +220,26 @@ def city(seed: int = 7, num_boxes: int = 5000, tiles: Tuple[int, int] = (8, 4),
                       "area_light_radius": 60.0})
 
 
+def box_room_gi(seed: int = 1, photonmap_size: int = 64, photons_per_light: int = 20000,
+                fg_samples: int = 64) -> SceneSpec:
+    """Box room with coloured albedo, a photonmap and final-gather settings (photons + FG)."""
+    spec = box_room(seed)
+    m = spec.meshes[0]
+    m.albedo = np.array([0.8, 0.7, 0.6, 1.0], np.float32)
+    m.photonmap = 0
+    spec.photonmaps = [(photonmap_size, photonmap_size)]
+    spec.indirect = IndirectLightSettings(
+        photon_pass_count=2, photon_bounce_count=3, photon_energy_threshold=0.05,
+        photon_max_distance=10.0, final_gather_samples=fg_samples, final_gather_distance=10.0,
+        final_gather_radius=3, sky_color=(0.2, 0.3, 0.4), ambient_color=(0.01, 0.01, 0.01),
+        seed=seed, sampling=RL_SAMPLE_UNIFORM, photons_per_light=photons_per_light)
+    spec.name = "box_room_gi"
+    return spec
+
+
 def build_gpu(relight, spec: SceneSpec):
-    """Builds the scene and its lightmaps in the library; returns (scene, lightmaps)."""
+    """Builds the scene, its lightmaps and photonmaps in the library; returns
+    (scene, lightmaps, mesh ids) and sets scene.photonmaps."""
     scene = relight.create_scene()
     scene.begin()
     ids = []
@@ -229,7 +250,12 @@ def build_gpu(relight, spec: SceneSpec):
     lms = [relight.create_lightmap(w, h) for (w, h) in spec.lightmaps]
     for i, lm in enumerate(lms):
         lm.set_seed(i)
+    pms = [relight.create_photonmap(w, h) for (w, h) in spec.photonmaps]
     for mid, m in zip(ids, spec.meshes):
-        lms[m.lightmap].add_mesh(scene, mid)
+        if m.lightmap is not None and m.lightmap >= 0:
+            lms[m.lightmap].add_mesh(scene, mid)
+        if m.photonmap is not None and m.photonmap >= 0:
+            pms[m.photonmap].add_mesh(scene, mid)
     scene.end()
+    scene.photonmaps = pms
     return scene, lms, ids

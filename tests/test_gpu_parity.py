@@ -169,3 +169,49 @@ def test_city_full_size_properties(relight):
     relight.bake_direct_light(scene2, ids2[5])
     assert np.array_equal(full, lms2[5].to_hdr())
     assert np.isfinite(full).all() and (full >= 0).all()
+
+
+def _gi_compare(relight, spec):
+    scene, lms, ids = scenes.build_gpu(relight, spec)
+    relight.emit_photons(scene, spec.indirect)
+    st = relight.stats()
+    S = O.OracleScene.from_spec(spec)
+    maps, rays = S.emit_photons(spec, spec.indirect)
+    assert st["rays"] == rays
+    for k, pm in enumerate(scene.photonmaps):
+        c, p, g = pm.read()
+        oc, op, og = maps[k]
+        assert np.array_equal(p, op), f"photon counts differ in map {k}: {np.abs(p.astype(int) - op).sum()}"
+        assert np.array_equal(c, oc) and np.array_equal(g, og)
+    for li, mid in enumerate(ids[:len(lms)]):
+        relight.bake_indirect_light(scene, mid, spec.indirect)
+        fst = relight.stats()
+        texel, pos, nrm = S.lumels(spec, li)
+        col = np.zeros((texel.size, 3), np.float32)
+        S.final_gather(spec, pos, nrm, texel, li, spec.indirect, maps, col)
+        gpu = lms[li].to_hdr().reshape(-1, 3)[texel]
+        assert fst["rays"] == texel.size * spec.indirect.final_gather_samples
+        assert rel_rmse(gpu, col) <= 1e-3
+        assert np.array_equal(gpu, col)
+
+
+def test_box_room_photons_and_final_gather_bit_exact(relight):
+    print("[gpu] box_room photons + final gather", flush=True)
+    _gi_compare(relight, scenes.box_room_gi())
+
+
+def test_photons_russian_roulette_directional_and_cosine_fg(relight):
+    print("[gpu] city_small photons (RR, sun + area) + cosine final gather", flush=True)
+    from relight_amd import RL_PHOTON_RUSSIAN_ROULETTE, RL_SAMPLE_COSINE, IndirectLightSettings
+    spec = scenes.city(num_boxes=150, tiles=(2, 1), side_range=(48, 64))
+    rng = np.random.default_rng(3)
+    for i, m in enumerate(spec.meshes):
+        m.albedo = np.array([*rng.uniform(0.3, 0.8, 3), 1.0], np.float32)
+        m.photonmap = i
+    spec.photonmaps = [(32, 32) for _ in spec.meshes]
+    spec.indirect = IndirectLightSettings(
+        photon_pass_count=1, photon_bounce_count=3, photon_energy_threshold=0.0,
+        photon_max_distance=300.0, final_gather_samples=40, final_gather_distance=60.0,
+        final_gather_radius=2, sky_color=(0.3, 0.4, 0.5), ambient_color=(0.0, 0.0, 0.0), seed=9,
+        sampling=RL_SAMPLE_COSINE, photons_per_light=15000, termination=RL_PHOTON_RUSSIAN_ROULETTE)
+    _gi_compare(relight, spec)

@@ -305,3 +305,71 @@ def test_ao_corner_binomial():
     frac = occ / 4096.0
     est = 0.5 * (1 - a / np.sqrt(a * a + 10.0 ** 2))
     assert np.all(np.abs(frac - est) < 0.035)
+
+
+def _closed_cube_spec(albedo, res=64):
+    quads = scenes._box_quads((-1, -1, -1), (1, 1, 1), 2, inward=True)
+    scenes._pack(quads, res, res)
+    p, n, uv, ix = scenes._emit(quads, res, res)
+    m = scenes.MeshSpec(p, n, uv, ix, 0, albedo=np.array([albedo, albedo, albedo, 1.0], np.float32),
+                        photonmap=0)
+    return scenes.SceneSpec("cube", [m], [LightDesc.point((0.0, 0.0, 0.0), radius=10.0, intensity=1.0)],
+                            [(res, res)], photonmaps=[(res, res)])
+
+
+def test_photon_first_bounce_energy():
+    # One bounce (max depth 0) from a point light at the centre of a closed cube: every photon
+    # hits a wall with power I * albedo * cos(theta). The mean stored power per stored photon is
+    # albedo * E[cos theta] with E[cos theta] = E[max(|x|,|y|,|z|)] over uniform directions.
+    from relight_amd import IndirectLightSettings
+    spec = _closed_cube_spec(0.5)
+    ind = IndirectLightSettings(photon_pass_count=1, photon_bounce_count=0, photon_energy_threshold=0.0,
+                                photon_max_distance=10.0, final_gather_radius=0, photons_per_light=200000)
+    S = O.OracleScene.from_spec(spec)
+    maps, rays = S.emit_photons(spec, ind)
+    c, p, g = maps[0]
+    assert rays == 200000
+    stored = int(p.sum())
+    assert stored > 0.9 * 200000   # hits on gutter texels are not stored
+    d = np.random.default_rng(0).normal(size=(2_000_000, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    ecos = np.abs(d).max(1).mean()
+    assert abs(c[..., 0].sum() / stored - 0.5 * ecos) < 0.003
+    # Radius-0 gather is the per-texel mean power.
+    nz = p > 0
+    assert np.allclose(g[nz, 0], c[nz, 0] / p[nz], rtol=1e-6)
+
+
+def test_photon_gather_disc_closed_form():
+    # Two photon-bearing texels; Photonmap::gather averages colour/photons over a texel disc.
+    from relight_amd import IndirectLightSettings
+    spec = _closed_cube_spec(0.5, res=32)
+    ind = IndirectLightSettings(photon_pass_count=1, photon_bounce_count=0, photon_energy_threshold=0.0,
+                                photon_max_distance=10.0, final_gather_radius=2, photons_per_light=5000)
+    S = O.OracleScene.from_spec(spec)
+    c, p, g = S.emit_photons(spec, ind)[0][0]
+    H, W = p.shape
+    yy, xx = np.mgrid[0:H, 0:W]
+    for (y, x) in [(5, 7), (16, 16), (0, 0), (31, 30)]:
+        m = (np.sqrt(((xx - x) ** 2 + (yy - y) ** 2).astype(np.float32)) <= 2) & (p >= 0)
+        n = p[m].sum()
+        expect = c[m, 0].astype(np.float64).sum() / n if n else 0.0
+        assert abs(g[y, x, 0] - expect) <= 1e-5 * max(1.0, abs(expect))
+
+
+def test_final_gather_sky_closed_form():
+    # Open floor, no photonmaps: every final-gather ray misses, so each lumel gains
+    # mean(sky * cos) + ambient = sky * E[cos] + ambient (uniform: 1/2, cosine: 2/3).
+    from relight_amd import IndirectLightSettings
+    spec = _floor_spec()
+    for sampling, ecos in ((0, 0.5), (1, 2.0 / 3.0)):
+        ind = IndirectLightSettings(final_gather_samples=4096, final_gather_distance=5.0,
+                                    sky_color=(1.0, 0.5, 0.25), ambient_color=(0.1, 0.1, 0.1),
+                                    sampling=sampling)
+        S = O.OracleScene.from_spec(spec)
+        texel, pos, nrm = S.lumels(spec, 0)
+        sel = slice(0, None, 97)
+        col = np.zeros((texel[sel].size, 3), np.float32)
+        S.final_gather(spec, pos[sel], nrm[sel], texel[sel], 0, ind, {}, col)
+        assert np.allclose(col[:, 0], ecos + 0.1, atol=0.02)
+        assert np.allclose(col[:, 2], 0.25 * ecos + 0.1, atol=0.01)
