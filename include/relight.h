@@ -30,6 +30,7 @@ extern "C" {
 #define RL_REJECTED 4
 #define RL_DEVICE_ERROR 5
 #define RL_INVALID_ARGUMENT 6
+#define RL_NOT_CONVERGED 7
 
 /* Light kinds: Light::createPointLight / createSpotLight / createDirectionalLight /
  * createAreaLight (scene/Light.cpp:166-234). */
@@ -132,7 +133,10 @@ typedef struct rl_indirect_settings {
 #define RL_KERNEL_PHOTON_STORE 5
 #define RL_KERNEL_PHOTON_GATHER 6
 #define RL_KERNEL_FINAL_GATHER 7
-#define RL_KERNEL_SLOTS 8
+#define RL_KERNEL_FORM_FACTORS 8
+#define RL_KERNEL_FF_GEMM_ACCURATE 9
+#define RL_KERNEL_FF_GEMM_FAST 10
+#define RL_KERNEL_SLOTS 16
 
 /* Counters of the last bake or trace call on a context. */
 typedef struct rl_stats {
@@ -225,6 +229,38 @@ int32_t rl_emit_photons(rl_relight* ctx, rl_scene* scene, const rl_indirect_sett
  * (baker/IndirectLight.cpp:47-75) over every valid lumel of the mesh's lightmap. */
 int32_t rl_bake_indirect(rl_relight* ctx, rl_scene* scene, int32_t mesh,
                          const rl_indirect_settings* settings);
+
+/* Radiosity: patches are quads origin + s*U + t*V with normal n (12 floats each: o, U, V, n);
+ * tri_patch maps every scene-order triangle to its patch or 0xFFFFFFFF. Replaces
+ * RadiosityBuilder::build / computeFormFactors (radiosity/RadiosityBuilder.cpp:44-176) and the empty
+ * RadiositySolver::solve (radiosity/RadiositySolver.cpp:34-36).
+ * build_form_factors: F_ij = (closest hits on patch j) / rays_per_patch over rays_per_patch
+ * cosine-weighted rays from uniform points on patch i (segments of max_distance), kept as exact
+ * 16-bit counts (at most 16,384 patches, 65,535 rays per patch).
+ * solve: B = E + rho o (F B) for num_rhs right-hand sides (emission and reflectance are row-major
+ * num_patches x num_rhs): Jacobi sweeps with F and the correction in BF16 on MFMA (fp32
+ * accumulation) inside fp32 iterative refinement; the residual ||E + rho o FB - B|| / ||E|| uses the
+ * exact F and a three-way BF16 split of B. Returns RL_NOT_CONVERGED if `tolerance` is not reached
+ * within max_outer refinement steps (the best solution is still written). */
+typedef struct rl_radiosity rl_radiosity;
+typedef struct rl_solve_info {
+    int32_t outer_iterations;  /* residual evaluations */
+    int32_t sweeps;            /* BF16 Jacobi sweeps */
+    double residual;           /* final relative residual */
+    double gpu_ms;
+} rl_solve_info;
+int32_t rl_radiosity_create(rl_relight* ctx, rl_scene* scene, const float* patches,
+                            uint32_t num_patches, const uint32_t* tri_patch, rl_radiosity** out);
+int32_t rl_radiosity_destroy(rl_radiosity* r);
+int32_t rl_radiosity_build_form_factors(rl_relight* ctx, rl_radiosity* r, uint32_t rays_per_patch,
+                                        uint32_t seed, float max_distance);
+/* Rows [first_row, first_row+rows) of the hit counts, row-major rows x num_patches. */
+int32_t rl_radiosity_read_form_factors(rl_radiosity* r, uint32_t first_row, uint32_t rows,
+                                       uint16_t* counts);
+int32_t rl_radiosity_solve(rl_relight* ctx, rl_radiosity* r, const float* emission,
+                           const float* reflectance, uint32_t num_rhs, float tolerance,
+                           int32_t max_outer, int32_t inner_sweeps, float* out_radiosity,
+                           rl_solve_info* info);
 
 /* Counters of the last bake/trace call on ctx. */
 int32_t rl_get_stats(const rl_relight* ctx, rl_stats* out);

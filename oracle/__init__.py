@@ -105,6 +105,8 @@ def lib():
                                             _F, _U32, _F, C.c_int32, _U64]),
             "or_final_gather": (C.c_int32, [C.c_void_p, C.c_uint32, _F, _F, _U32, C.c_uint32, C.POINTER(or_indirect),
                                             _I32, C.c_int32, _I32, _I32, _F, _F, C.c_int32, _U64]),
+            "or_form_factors": (C.c_int32, [C.c_void_p, _F, _U32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float,
+                                            C.c_uint32, C.c_uint32, C.POINTER(C.c_uint16), C.c_int32]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -299,6 +301,17 @@ class OracleScene:
                                 mesh_pm.ctypes.data_as(_I32), len(order), ws.ctypes.data_as(_I32),
                                 hs.ctypes.data_as(_I32), _f(g), _f(color), threads, C.byref(rays))
         return rays.value
+
+    def form_factors(self, patches, tri_patch, rays, seed, max_distance, rows=None, row0=0, threads=0):
+        """Hit counts of form-factor rows [row0, row0+rows) (u16, rows x num_patches)."""
+        p = np.ascontiguousarray(patches, np.float32).reshape(-1, 12)
+        t = np.ascontiguousarray(tri_patch, np.uint32)
+        n = p.shape[0]
+        rows = n - row0 if rows is None else rows
+        out = np.zeros((rows, n), np.uint16)
+        self._L.or_form_factors(self._h, _f(p), _u(t), n, rays, seed, max_distance, row0, rows,
+                                out.ctypes.data_as(C.POINTER(C.c_uint16)), threads)
+        return out
 
     def lumels(self, spec, lightmap):
         """Valid lumels of one lightmap in texel order: texel, pos, nrm (rasterising its meshes

@@ -964,4 +964,32 @@ int32_t or_final_gather(void* sp, uint32_t n, const float* pos, const float* nrm
     return 0;
 }
 
+// Form-factor rows by hemisphere rays (the estimator the GPU uses in place of the pairwise
+// RadiosityBuilder::distanceFormFactor, radiosity/RadiosityBuilder.cpp:142-176): for patch i,
+// `rays` segments from uniform points o + s*U + t*V with cosine-weighted directions about n,
+// counting closest hits per patch. patches: 12 floats each (o, U, V, n); out: rows x n counts.
+int32_t or_form_factors(void* sp, const float* patches, const uint32_t* tri_patch, uint32_t n, uint32_t rays,
+                        uint32_t seed, float max_distance, uint32_t row0, uint32_t rows, uint16_t* out,
+                        int32_t threads) {
+    const Scene& S = *(Scene*)sp;
+    parallel_for(rows, threads, [&](uint32_t rr, int) {
+        const uint32_t i = row0 + rr;
+        const float* P = patches + 12 * (size_t)i;
+        const Vec o = mk(P[0], P[1], P[2]), U = mk(P[3], P[4], P[5]), V = mk(P[6], P[7], P[8]), N = mk(P[9], P[10], P[11]);
+        uint16_t* row = out + (size_t)rr * n;
+        for (uint32_t j = 0; j < n; j++) row[j] = 0;
+        HitRec h;
+        for (uint32_t k = 0; k < rays; k++) {
+            Rng g = stream(seed, 0xFFFFFFFEu, i, 4u, k);
+            const float s = rng_f(g), t = rng_f(g);
+            const Vec p = add(add(o, scale(U, s)), scale(V, t));
+            const Vec d = hemi_cosine(g, N);
+            if (!trace(S, make_seg(p, add(p, scale(d, max_distance))), false, h, nullptr)) continue;
+            const uint32_t j = tri_patch[h.tri];
+            if (j < n) row[j]++;
+        }
+    });
+    return 0;
+}
+
 }  // extern "C"

@@ -30,7 +30,7 @@ RL_TRACE_CLOSEST = 1
 
 STATUS_NAMES = {
     0: "Success", 1: "InvalidCall", 2: "NotImplemented", 3: "OutOfMemory",
-    4: "Rejected", 5: "DeviceError", 6: "InvalidArgument",
+    4: "Rejected", 5: "DeviceError", 6: "InvalidArgument", 7: "NotConverged",
 }
 
 
@@ -108,9 +108,19 @@ RL_KERNEL_PHOTON_TRACE = 4
 RL_KERNEL_PHOTON_STORE = 5
 RL_KERNEL_PHOTON_GATHER = 6
 RL_KERNEL_FINAL_GATHER = 7
-RL_KERNEL_SLOTS = 8
+RL_KERNEL_FORM_FACTORS = 8
+RL_KERNEL_FF_GEMM_ACCURATE = 9
+RL_KERNEL_FF_GEMM_FAST = 10
+RL_KERNEL_SLOTS = 16
 KERNEL_NAMES = {0: "direct", 1: "ao_count", 2: "ao_finalize", 3: "trace", 4: "photon_trace",
-                5: "photon_store", 6: "photon_gather", 7: "final_gather"}
+                5: "photon_store", 6: "photon_gather", 7: "final_gather", 8: "form_factors",
+                9: "ff_gemm_accurate", 10: "ff_gemm_fast"}
+RL_NOT_CONVERGED = 7
+
+
+class rl_solve_info(C.Structure):
+    _fields_ = [("outer_iterations", C.c_int32), ("sweeps", C.c_int32), ("residual", C.c_double),
+                ("gpu_ms", C.c_double)]
 
 
 class rl_stats(C.Structure):
@@ -162,6 +172,12 @@ SIGNATURES = [
     ("rl_photonmap_read", C.c_int32, [_P, _F, _U32, _F]),
     ("rl_emit_photons", C.c_int32, [_P, _P, C.POINTER(rl_indirect_settings)]),
     ("rl_bake_indirect", C.c_int32, [_P, _P, C.c_int32, C.POINTER(rl_indirect_settings)]),
+    ("rl_radiosity_create", C.c_int32, [_P, _P, _F, C.c_uint32, _U32, _PP]),
+    ("rl_radiosity_destroy", C.c_int32, [_P]),
+    ("rl_radiosity_build_form_factors", C.c_int32, [_P, _P, C.c_uint32, C.c_uint32, C.c_float]),
+    ("rl_radiosity_read_form_factors", C.c_int32, [_P, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint16)]),
+    ("rl_radiosity_solve", C.c_int32, [_P, _P, _F, _F, C.c_uint32, C.c_float, C.c_int32, C.c_int32, _F,
+                                       C.POINTER(rl_solve_info)]),
     ("rl_get_stats", C.c_int32, [_P, C.POINTER(rl_stats)]),
 ]
 

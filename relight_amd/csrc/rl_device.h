@@ -120,6 +120,22 @@ struct PhotonRec {
     float r, g, b;
 };
 
+// A radiosity patch: the quad origin + s*U + t*V (s, t in [0,1]) with normal n.
+struct PatchDev {
+    float o[3], u[3], v[3], n[3];
+};
+
+struct FormFactorParams {
+    uint32_t num_patches;
+    uint32_t row_stride;  // u16 entries per F row (num_patches padded to a multiple of 64)
+    uint32_t rays;
+    uint32_t seed;
+    float max_distance;
+    uint32_t first_row;
+    uint32_t rows;
+    uint32_t pad;
+};
+
 struct FinalGatherParams {
     int32_t samples;
     float distance;

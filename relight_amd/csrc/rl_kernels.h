@@ -1,4 +1,7 @@
-// Host-callable launchers of the kernels in rl_kernels.hip.
+// Host-callable launchers of the HIP kernels (rl_kernels.hip, rl_photon.hip, rl_radiosity.hip).
+//
+// Ray kernels are persistent: `max_blocks` caps the grid (the runtime sizes it to the device's
+// compute units) and `queue` is a device u64 work counter the launcher clears first.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -13,25 +16,43 @@ int stack_capacity_for(int bound);
 
 hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v);
 
-hipError_t launch_bake_direct(hipStream_t st, const SceneDev& S, int cap, const BakeSeg* segs,
-                              int nsegs, uint64_t total, const LightDev* lights, int nlights,
-                              const float4* lverts, unsigned long long* rays);
+hipError_t launch_bake_direct(hipStream_t st, const SceneDev& S, int cap, uint32_t max_blocks,
+                              const BakeSeg* segs, int nsegs, uint64_t total, const LightDev* lights,
+                              int nlights, const float4* lverts, unsigned long long* queue,
+                              unsigned long long* rays);
 
-hipError_t launch_ao_count(hipStream_t st, const SceneDev& S, int cap, const BakeSeg* segs,
-                           int nsegs, uint64_t total_pairs, const AoParams& P,
-                           unsigned long long* rays);
+hipError_t launch_ao_count(hipStream_t st, const SceneDev& S, int cap, uint32_t max_blocks,
+                           const BakeSeg* segs, int nsegs, uint64_t total_pairs, const AoParams& P,
+                           unsigned long long* queue, unsigned long long* rays);
 
 hipError_t launch_ao_finalize(hipStream_t st, const BakeSeg* segs, int nsegs, uint64_t total,
                               const AoParams& P);
 
-hipError_t launch_trace(hipStream_t st, const SceneDev& S, int cap, bool any, const float4* starts,
-                        const float4* ends, uint32_t n, uint8_t* flags, float4* hits,
-                        unsigned long long* rays);
+hipError_t launch_trace(hipStream_t st, const SceneDev& S, int cap, uint32_t max_blocks, bool any,
+                        const float4* starts, const float4* ends, uint32_t n, uint8_t* flags,
+                        float4* hits, unsigned long long* queue, unsigned long long* rays);
+
+// Radiosity (rl_radiosity.hip).
+hipError_t launch_form_factor_rows(hipStream_t st, const SceneDev& S, int cap, const PatchDev* patches,
+                                   const uint32_t* tri_patch, const FormFactorParams& P, uint16_t* F,
+                                   unsigned long long* rays);
+hipError_t launch_ff_gemm(hipStream_t st, bool accurate, const uint16_t* F, uint32_t stride,
+                          const __bf16* X1, const __bf16* X2, const __bf16* X3, uint32_t npad,
+                          uint32_t mpad, uint32_t nsplit, float* Yp);
+hipError_t launch_ff_residual(hipStream_t st, const float* Yp, uint32_t nsplit, uint32_t npad,
+                              uint32_t mpad, float inv_rays, const float* E, const float* rho,
+                              const float* B, float* R, float* D, __bf16* X1, float* part);
+hipError_t launch_ff_sweep(hipStream_t st, const float* Yp, uint32_t nsplit, uint32_t npad,
+                           uint32_t mpad, float inv_rays, const float* rho, const float* R, float* D,
+                           __bf16* X1);
+hipError_t launch_ff_apply(hipStream_t st, uint32_t npad, uint32_t mpad, float* B, const float* D,
+                           int init, __bf16* X1, __bf16* X2, __bf16* X3);
 
 // Photons (rl_photon.hip).
-hipError_t launch_photon_trace(hipStream_t st, const SceneDev& S, int cap, const LightDev* lights,
-                               const uint64_t* light_first, const PhotonParams& P,
-                               const int32_t* mesh_pm, const PhotonMapDev* pms, PhotonRec* out,
+hipError_t launch_photon_trace(hipStream_t st, const SceneDev& S, int cap, uint32_t max_blocks,
+                               const LightDev* lights, const uint64_t* light_first,
+                               const PhotonParams& P, const int32_t* mesh_pm,
+                               const PhotonMapDev* pms, PhotonRec* out, unsigned long long* queue,
                                unsigned long long* rays);
 hipError_t photon_sort_temp_bytes(uint32_t n, size_t* bytes);
 hipError_t launch_photon_store(hipStream_t st, const PhotonRec* rec, uint32_t n, uint32_t* keys_a,
@@ -39,9 +60,10 @@ hipError_t launch_photon_store(hipStream_t st, const PhotonRec* rec, uint32_t n,
                                size_t temp_bytes, const PhotonMapDev* pms, int npm);
 hipError_t launch_photon_gather(hipStream_t st, const PhotonMapDev* pms, int npm,
                                 uint32_t total_texels, int radius);
-hipError_t launch_final_gather(hipStream_t st, const SceneDev& S, int cap, const BakeSeg* segs,
-                               int nsegs, uint64_t total_lumels, const FinalGatherParams& P,
-                               const int32_t* mesh_pm, const PhotonMapDev* pms,
+hipError_t launch_final_gather(hipStream_t st, const SceneDev& S, int cap, uint32_t max_blocks,
+                               const BakeSeg* segs, int nsegs, uint64_t total_lumels,
+                               const FinalGatherParams& P, const int32_t* mesh_pm,
+                               const PhotonMapDev* pms, unsigned long long* queue,
                                unsigned long long* rays);
 
 }  // namespace rl

@@ -1,11 +1,14 @@
-// CDNA4 (gfx950) kernels of the bake path: BVH4 traversal with an LDS stack, watertight
-// triangle tests, and the per-lumel bake kernels that replace DirectLight::bakeLumel
-// (baker/DirectLight.cpp:46-125) and AmbientOcclusion::bakeLumel
-// (baker/AmbientOcclusion.cpp:46-66) over rt::ITracer::traceSegment (rt/Tracer.h:87).
+// CDNA4 (gfx950) kernels of the direct-light and AO bake passes and of segment tracing:
+// DirectLight::bakeLumel (baker/DirectLight.cpp:46-125), AmbientOcclusion::bakeLumel
+// (baker/AmbientOcclusion.cpp:46-66) and rt::ITracer::traceSegment (rt/Tracer.h:87).
 //
-// Wave64 throughout: blocks are whole wavefronts, per-texel reductions use ballot + popcount so
-// integer results are exact and independent of scheduling.
+// All ray kernels are persistent: a grid sized to the machine takes 64-item chunks from an atomic
+// queue, so one launch covers any number of lightmaps of any size (a single dispatch may not
+// exceed 2^32 work-items) and short chunks refill idle wavefronts. Per-texel reductions use
+// ballot + popcount, so integer results are exact and independent of scheduling.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "rl_device.h"
 #include "rl_kernels.h"
@@ -65,17 +68,23 @@ __device__ float influence_from_point(const SceneDev& S, const LightDev& L, V3 l
     return inf * att * cut;
 }
 
+// One lane per lumel; lights and light vertices in the reference's order.
 template <int CAP>
 __global__ __launch_bounds__(kBlock) void k_bake_direct(SceneDev S, const BakeSeg* __restrict__ segs,
                                                         int nsegs, uint64_t total,
                                                         const LightDev* __restrict__ lights, int nlights,
                                                         const float4* __restrict__ lverts,
+                                                        unsigned long long* queue,
                                                         unsigned long long* rays_out) {
     __shared__ uint32_t stack[CAP * kBlock];
     uint32_t* stk = stack + threadIdx.x;
-    const uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    const uint64_t nchunks = (total + 63) / 64;
     uint32_t rays = 0;
-    if (g < total) {
+    for (;;) {
+        const uint64_t chunk = wave_next_chunk(queue);
+        if (chunk >= nchunks) break;
+        const uint64_t g = chunk * 64 + lane_id();
+        if (g >= total) continue;
         const BakeSeg& sg = segs[find_seg(segs, nsegs, g)];
         const uint32_t i = (uint32_t)(g - sg.work_begin);
         const float4 p4 = sg.pos[i], n4 = sg.nrm[i];
@@ -122,81 +131,92 @@ __global__ __launch_bounds__(kBlock) void k_bake_direct(SceneDev S, const BakeSe
 template <int CAP>
 __global__ __launch_bounds__(kBlock) void k_ao_count(SceneDev S, const BakeSeg* __restrict__ segs,
                                                      int nsegs, uint64_t total, AoParams P,
-                                                     unsigned long long* rays_out) {
+                                                     unsigned long long* queue, unsigned long long* rays_out) {
     __shared__ uint32_t stack[CAP * kBlock];
     uint32_t* stk = stack + threadIdx.x;
-    const uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    const bool active = g < total;
-    bool occ = false;
-    uint32_t* counter = nullptr;
-    uint64_t key = ~0ull;
-    if (active) {
-        const int s = find_seg(segs, nsegs, g);
-        const BakeSeg& sg = segs[s];
-        const uint64_t local = g - sg.work_begin;
-        const uint32_t li = (uint32_t)(local / (uint64_t)P.samples);
-        const uint32_t si = (uint32_t)(local % (uint64_t)P.samples);
-        const float4 p4 = sg.pos[li], n4 = sg.nrm[li];
-        const V3 p = v3(p4.x, p4.y, p4.z), n = v3(n4.x, n4.y, n4.z);
-        Pcg32 rng = sample_stream(P.seed, sg.lm_seed, f_bits(p4.w), TECH_AO, si);
-        V3 dir = sample_hemisphere(rng, n, P.sampling);
-        Ray r = ray_from_segment(p, p + dir * P.max_distance);
-        TraceResult tr;
-        occ = traverse<true, CAP>(S, r, stk, kBlock, tr);
-        counter = sg.counts + li;
-        key = ((uint64_t)s << 32) | li;
-    }
-    const uint64_t occ_mask = __ballot(occ);
-    const uint64_t act_mask = __ballot(active);
     const uint32_t lane = lane_id();
-    const uint64_t prev_key = __shfl_up(key, 1, 64);
-    const bool head = active && (lane == 0 || prev_key != key);
-    const uint64_t heads = __ballot(head);
-    if (head) {
-        const uint64_t above = (lane == 63) ? 0ull : (heads & (~0ull << (lane + 1)));
-        const uint64_t end_mask = above ? (above & (~above + 1ull)) : 0ull;  // lowest head above
-        const uint64_t run = (end_mask ? (end_mask - 1ull) : ~0ull) & (~0ull << lane);
-        const uint32_t c = (uint32_t)__popcll(occ_mask & run & act_mask);
-        if (c) atomicAdd(counter, c);
+    const uint64_t nchunks = (total + 63) / 64;
+    uint64_t rays = 0;
+    for (;;) {
+        const uint64_t chunk = wave_next_chunk(queue);
+        if (chunk >= nchunks) break;
+        const uint64_t g = chunk * 64 + lane;
+        const bool active = g < total;
+        bool occ = false;
+        uint32_t* counter = nullptr;
+        uint64_t key = ~0ull;
+        if (active) {
+            const int s = find_seg(segs, nsegs, g);
+            const BakeSeg& sg = segs[s];
+            const uint64_t local = g - sg.work_begin;
+            const uint32_t li = (uint32_t)(local / (uint64_t)P.samples);
+            const uint32_t si = (uint32_t)(local % (uint64_t)P.samples);
+            const float4 p4 = sg.pos[li], n4 = sg.nrm[li];
+            const V3 p = v3(p4.x, p4.y, p4.z), n = v3(n4.x, n4.y, n4.z);
+            Pcg32 rng = sample_stream(P.seed, sg.lm_seed, f_bits(p4.w), TECH_AO, si);
+            V3 dir = sample_hemisphere(rng, n, P.sampling);
+            Ray r = ray_from_segment(p, p + dir * P.max_distance);
+            TraceResult tr;
+            occ = traverse<true, CAP>(S, r, stk, kBlock, tr);
+            counter = sg.counts + li;
+            key = ((uint64_t)s << 32) | li;
+        }
+        const uint64_t occ_mask = __ballot(occ);
+        const uint64_t act_mask = __ballot(active);
+        const uint64_t prev_key = __shfl_up(key, 1, 64);
+        const bool head = active && (lane == 0 || prev_key != key);
+        const uint64_t heads = __ballot(head);
+        if (head) {
+            const uint64_t above = (lane == 63) ? 0ull : (heads & (~0ull << (lane + 1)));
+            const uint64_t end_mask = above ? (above & (~above + 1ull)) : 0ull;  // lowest head above
+            const uint64_t run = (end_mask ? (end_mask - 1ull) : ~0ull) & (~0ull << lane);
+            const uint32_t c = (uint32_t)__popcll(occ_mask & run & act_mask);
+            if (c) atomicAdd(counter, c);
+        }
+        rays += (uint64_t)__popcll(act_mask);
     }
-    if (lane == 0) {
-        const uint32_t nr = (uint32_t)__popcll(act_mask);
-        if (nr) atomicAdd(rays_out, (unsigned long long)nr);
-    }
+    if (lane == 0 && rays) atomicAdd(rays_out, (unsigned long long)rays);
 }
 
 // AO pass 2: AmbientOcclusion::bakeLumel's value and multiply (baker/AmbientOcclusion.cpp:59-65).
 __global__ __launch_bounds__(kBlock) void k_ao_finalize(const BakeSeg* __restrict__ segs, int nsegs,
                                                         uint64_t total, AoParams P) {
-    const uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (g >= total) return;
-    const BakeSeg& sg = segs[find_seg(segs, nsegs, g)];
-    const uint32_t i = (uint32_t)(g - sg.work_begin);
-    const uint32_t occluded = sg.counts[i];
-    float value = 1.0f - (float)occluded / ((float)P.samples * P.occluded_fraction);
-    if (fabsf(1.0f - P.exponent) > 0.01f) value = powf(value, P.exponent);
-    float4 c = sg.color[i];
-    c.x *= value; c.y *= value; c.z *= value;
-    sg.color[i] = c;
-    sg.counts[i] = 0u;
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x; g < total; g += stride) {
+        const BakeSeg& sg = segs[find_seg(segs, nsegs, g)];
+        const uint32_t i = (uint32_t)(g - sg.work_begin);
+        const uint32_t occluded = sg.counts[i];
+        float value = 1.0f - (float)occluded / ((float)P.samples * P.occluded_fraction);
+        if (fabsf(1.0f - P.exponent) > 0.01f) value = powf(value, P.exponent);
+        float4 c = sg.color[i];
+        c.x *= value; c.y *= value; c.z *= value;
+        sg.color[i] = c;
+        sg.counts[i] = 0u;
+    }
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint64_t n, uint32_t v) {
-    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g < n) p[g] = v;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += stride) p[g] = v;
 }
 
 template <bool ANY, int CAP>
 __global__ __launch_bounds__(kBlock) void k_trace(SceneDev S, const float4* __restrict__ starts,
                                                   const float4* __restrict__ ends, uint32_t n,
                                                   uint8_t* __restrict__ flags, float4* __restrict__ hits,
-                                                  unsigned long long* rays_out) {
+                                                  unsigned long long* queue, unsigned long long* rays_out) {
     __shared__ uint32_t stack[CAP * kBlock];
-    const uint32_t g = blockIdx.x * kBlock + threadIdx.x;
-    if (g < n) {
+    const uint64_t nchunks = ((uint64_t)n + 63) / 64;
+    uint32_t rays = 0;
+    for (;;) {
+        const uint64_t chunk = wave_next_chunk(queue);
+        if (chunk >= nchunks) break;
+        const uint64_t g = chunk * 64 + lane_id();
+        if (g >= n) continue;
         const float4 a = starts[g], b = ends[g];
         Ray r = ray_from_segment(v3(a.x, a.y, a.z), v3(b.x, b.y, b.z));
         TraceResult tr;
+        rays++;
         bool h = traverse<ANY, CAP>(S, r, stack + threadIdx.x, kBlock, tr);
         if (ANY) {
             flags[g] = h ? 1 : 0;
@@ -207,10 +227,16 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDev S, const float4* __re
             hits[g] = make_float4(tr.t, bits_f(tr.tri), u, v);
         }
     }
-    count_rays(g < n ? 1u : 0u, rays_out);
+    count_rays(rays, rays_out);
 }
 
 uint32_t blocks_for(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
+
+// Persistent grid: enough blocks for the work, at most max_blocks.
+uint32_t persistent_blocks(uint64_t items, uint32_t max_blocks) {
+    const uint64_t need = (items + kBlock - 1) / kBlock;
+    return (uint32_t)(need < max_blocks ? (need ? need : 1) : max_blocks);
+}
 
 }  // namespace
 
@@ -233,47 +259,57 @@ int stack_capacity_for(int bound) {
 
 hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fill_u32, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, n, v);
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>(blocks_for(n, 256), 65536);
+    hipLaunchKernelGGL(k_fill_u32, dim3(blocks), dim3(256), 0, st, p, n, v);
     return hipGetLastError();
 }
 
-hipError_t launch_bake_direct(hipStream_t st, const SceneDev& S, int cap, const BakeSeg* segs,
-                              int nsegs, uint64_t total, const LightDev* lights, int nlights,
-                              const float4* lverts, unsigned long long* rays) {
+hipError_t launch_bake_direct(hipStream_t st, const SceneDev& S, int cap, uint32_t max_blocks,
+                              const BakeSeg* segs, int nsegs, uint64_t total, const LightDev* lights,
+                              int nlights, const float4* lverts, unsigned long long* queue,
+                              unsigned long long* rays) {
     if (total == 0) return hipSuccess;
-    RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_bake_direct<CAP>, dim3(blocks_for(total, kBlock)), dim3(kBlock), 0, st,
-                                            S, segs, nsegs, total, lights, nlights, lverts, rays));
+    hipError_t e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(queue), 2, 0u);
+    if (e != hipSuccess) return e;
+    const uint32_t grid = persistent_blocks(total, max_blocks);
+    RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_bake_direct<CAP>, dim3(grid), dim3(kBlock), 0, st, S, segs, nsegs,
+                                            total, lights, nlights, lverts, queue, rays));
     return hipGetLastError();
 }
 
-hipError_t launch_ao_count(hipStream_t st, const SceneDev& S, int cap, const BakeSeg* segs,
-                           int nsegs, uint64_t total_pairs, const AoParams& P,
-                           unsigned long long* rays) {
+hipError_t launch_ao_count(hipStream_t st, const SceneDev& S, int cap, uint32_t max_blocks,
+                           const BakeSeg* segs, int nsegs, uint64_t total_pairs, const AoParams& P,
+                           unsigned long long* queue, unsigned long long* rays) {
     if (total_pairs == 0) return hipSuccess;
-    RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_ao_count<CAP>, dim3(blocks_for(total_pairs, kBlock)), dim3(kBlock), 0,
-                                            st, S, segs, nsegs, total_pairs, P, rays));
+    hipError_t e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(queue), 2, 0u);
+    if (e != hipSuccess) return e;
+    const uint32_t grid = persistent_blocks(total_pairs, max_blocks);
+    RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_ao_count<CAP>, dim3(grid), dim3(kBlock), 0, st, S, segs, nsegs,
+                                            total_pairs, P, queue, rays));
     return hipGetLastError();
 }
 
 hipError_t launch_ao_finalize(hipStream_t st, const BakeSeg* segs, int nsegs, uint64_t total,
                               const AoParams& P) {
     if (total == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_ao_finalize, dim3(blocks_for(total, kBlock)), dim3(kBlock), 0, st, segs, nsegs,
-                       total, P);
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>(blocks_for(total, kBlock), 65536);
+    hipLaunchKernelGGL(k_ao_finalize, dim3(blocks), dim3(kBlock), 0, st, segs, nsegs, total, P);
     return hipGetLastError();
 }
 
-hipError_t launch_trace(hipStream_t st, const SceneDev& S, int cap, bool any, const float4* starts,
-                        const float4* ends, uint32_t n, uint8_t* flags, float4* hits,
-                        unsigned long long* rays) {
+hipError_t launch_trace(hipStream_t st, const SceneDev& S, int cap, uint32_t max_blocks, bool any,
+                        const float4* starts, const float4* ends, uint32_t n, uint8_t* flags,
+                        float4* hits, unsigned long long* queue, unsigned long long* rays) {
     if (n == 0) return hipSuccess;
-    const dim3 grid(blocks_for(n, kBlock)), block(kBlock);
+    hipError_t e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(queue), 2, 0u);
+    if (e != hipSuccess) return e;
+    const dim3 grid(persistent_blocks(n, max_blocks)), block(kBlock);
     if (any) {
         RL_DISPATCH_CAP(cap, hipLaunchKernelGGL((k_trace<true, CAP>), grid, block, 0, st, S, starts, ends, n, flags,
-                                                hits, rays));
+                                                hits, queue, rays));
     } else {
         RL_DISPATCH_CAP(cap, hipLaunchKernelGGL((k_trace<false, CAP>), grid, block, 0, st, S, starts, ends, n, flags,
-                                                hits, rays));
+                                                hits, queue, rays));
     }
     return hipGetLastError();
 }

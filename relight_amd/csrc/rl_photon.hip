@@ -59,12 +59,17 @@ __global__ __launch_bounds__(kBlock) void k_photon_trace(SceneDev S, const Light
                                                          PhotonParams P, const int32_t* __restrict__ mesh_pm,
                                                          const PhotonMapDev* __restrict__ pms,
                                                          PhotonRec* __restrict__ out,
+                                                         unsigned long long* queue,
                                                          unsigned long long* rays_out) {
     __shared__ uint32_t stack[CAP * kBlock];
     uint32_t* stk = stack + threadIdx.x;
-    const uint64_t p = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    const uint64_t nchunks = (P.total + 63) / 64;
     uint32_t rays = 0;
-    if (p < P.total) {
+    for (;;) {
+        const uint64_t chunk = wave_next_chunk(queue);
+        if (chunk >= nchunks) break;
+        const uint64_t p = chunk * 64 + lane_id();
+        if (p >= P.total) continue;
         const uint64_t r = p % P.per_pass;
         int l = 0;
         while (l + 1 < P.nlights && light_first[l + 1] <= r) l++;
@@ -213,32 +218,34 @@ __global__ void k_photon_gather(const PhotonMapDev* __restrict__ pms, int npm, u
     M.gathered[t] = out;
 }
 
-// IndirectLight::bakeLumel: one wavefront per lumel, 64 samples per step; lane 0 adds the
-// per-sample terms in sample order, as the reference's loop does.
+// IndirectLight::bakeLumel: one wavefront per lumel (taken from the persistent queue), 64
+// samples per step; the per-sample terms are added in sample order, as the reference's loop does,
+// by broadcasting them lane by lane.
 template <int CAP>
 __global__ __launch_bounds__(kBlock) void k_final_gather(SceneDev S, const BakeSeg* __restrict__ segs, int nsegs,
                                                          uint64_t total, FinalGatherParams P,
                                                          const int32_t* __restrict__ mesh_pm,
                                                          const PhotonMapDev* __restrict__ pms,
+                                                         unsigned long long* queue,
                                                          unsigned long long* rays_out) {
     __shared__ uint32_t stack[CAP * kBlock];
-    __shared__ float terms[kBlock][4];
     uint32_t* stk = stack + threadIdx.x;
     const uint32_t lane = lane_id();
-    const uint32_t wave = threadIdx.x >> 6;
-    const uint64_t g = (uint64_t)blockIdx.x * (kBlock / 64) + wave;  // lumel of this wavefront
-    const bool active = g < total;
-    const BakeSeg& sg = segs[find_seg(segs, nsegs, active ? g : total - 1)];
-    const uint32_t i = (uint32_t)((active ? g : total - 1) - sg.work_begin);
+    uint32_t rays = 0;
+    for (;;) {
+    const uint64_t g = wave_next_chunk(queue);  // lumel of this wavefront
+    if (g >= total) break;
+    const BakeSeg& sg = segs[find_seg(segs, nsegs, g)];
+    const uint32_t i = (uint32_t)(g - sg.work_begin);
     const float4 p4 = sg.pos[i], n4 = sg.nrm[i];
     const V3 p = v3(p4.x, p4.y, p4.z), n = v3(n4.x, n4.y, n4.z);
     const uint32_t texel = f_bits(p4.w);
     float ar = 0.0f, ag = 0.0f, ab = 0.0f;
-    uint32_t rays = 0;
     for (int base = 0; base < P.samples; base += 64) {
         const int s = base + (int)lane;
-        float tr_ = 0.0f, tg_ = 0.0f, tb_ = 0.0f, use = 0.0f;
-        if (active && s < P.samples) {
+        float tr_ = 0.0f, tg_ = 0.0f, tb_ = 0.0f;
+        int use = 0;
+        if (s < P.samples) {
             Pcg32 rng = sample_stream(P.seed, sg.lm_seed, texel, TECH_FINAL_GATHER, (uint32_t)s);
             const V3 dir = sample_hemisphere(rng, n, P.sampling);
             const float d0 = dot(n, dir);
@@ -250,7 +257,7 @@ __global__ __launch_bounds__(kBlock) void k_final_gather(SceneDev S, const BakeS
                 tr_ = P.sky[0] * influence + P.ambient[0];
                 tg_ = P.sky[1] * influence + P.ambient[1];
                 tb_ = P.sky[2] * influence + P.ambient[2];
-                use = 1.0f;
+                use = 1;
             } else {
                 const HitAttr h = hit_attributes(S, tr);
                 if (dot(dir, h.n) < 0.0f) {
@@ -262,30 +269,23 @@ __global__ __launch_bounds__(kBlock) void k_final_gather(SceneDev S, const BakeS
                         tr_ = gc.x * influence + P.ambient[0];
                         tg_ = gc.y * influence + P.ambient[1];
                         tb_ = gc.z * influence + P.ambient[2];
-                        use = 1.0f;
+                        use = 1;
                     }
                 }
             }
         }
-        terms[threadIdx.x][0] = tr_;
-        terms[threadIdx.x][1] = tg_;
-        terms[threadIdx.x][2] = tb_;
-        terms[threadIdx.x][3] = use;
-        __syncthreads();
-        if (lane == 0) {
-            const int nb = min(64, P.samples - base);
-            for (int k = 0; k < nb; k++) {
-                const float* tk = terms[threadIdx.x + k];
-                if (tk[3] != 0.0f) { ar += tk[0]; ag += tk[1]; ab += tk[2]; }
-            }
+        const int nb = min(64, P.samples - base);
+        for (int k = 0; k < nb; k++) {
+            const float xr = __shfl(tr_, k, 64), xg = __shfl(tg_, k, 64), xb = __shfl(tb_, k, 64);
+            if (__shfl(use, k, 64)) { ar += xr; ag += xg; ab += xb; }
         }
-        __syncthreads();
     }
-    if (active && lane == 0) {
+    if (lane == 0) {
         float4 c = sg.color[i];
         const float fs = (float)P.samples;
         c.x += ar / fs; c.y += ag / fs; c.z += ab / fs;
         sg.color[i] = c;
+    }
     }
     count_rays(rays, rays_out);
 }
@@ -305,14 +305,19 @@ uint32_t blocks_for(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b)
         }                           \
     } while (0)
 
-hipError_t launch_photon_trace(hipStream_t st, const SceneDev& S, int cap, const LightDev* lights,
-                               const uint64_t* light_first, const PhotonParams& P, const int32_t* mesh_pm,
-                               const PhotonMapDev* pms, PhotonRec* out, unsigned long long* rays) {
+hipError_t launch_photon_trace(hipStream_t st, const SceneDev& S, int cap, uint32_t max_blocks,
+                               const LightDev* lights, const uint64_t* light_first, const PhotonParams& P,
+                               const int32_t* mesh_pm, const PhotonMapDev* pms, PhotonRec* out,
+                               unsigned long long* queue, unsigned long long* rays) {
     const uint64_t nrec = P.total * (uint64_t)(P.max_depth + 1);
     if (nrec == 0) return hipSuccess;
     hipLaunchKernelGGL(k_photon_clear, dim3(blocks_for(nrec, 256)), dim3(256), 0, st, out, nrec);
-    RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_photon_trace<CAP>, dim3(blocks_for(P.total, kBlock)), dim3(kBlock), 0,
-                                            st, S, lights, light_first, P, mesh_pm, pms, out, rays));
+    hipError_t e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(queue), 2, 0u);
+    if (e != hipSuccess) return e;
+    const uint64_t need = blocks_for(P.total, kBlock);
+    const uint32_t grid = (uint32_t)(need < max_blocks ? need : max_blocks);
+    RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_photon_trace<CAP>, dim3(grid), dim3(kBlock), 0, st, S, lights,
+                                            light_first, P, mesh_pm, pms, out, queue, rays));
     return hipGetLastError();
 }
 
@@ -338,18 +343,23 @@ hipError_t launch_photon_store(hipStream_t st, const PhotonRec* rec, uint32_t n,
 hipError_t launch_photon_gather(hipStream_t st, const PhotonMapDev* pms, int npm, uint32_t total_texels,
                                 int radius) {
     if (total_texels == 0) return hipSuccess;
+    if (total_texels > 0xFFFFFF00u) return hipErrorInvalidValue;  // one dispatch stays below 2^32 items
     hipLaunchKernelGGL(k_photon_gather, dim3(blocks_for(total_texels, 256)), dim3(256), 0, st, pms, npm,
                        total_texels, radius);
     return hipGetLastError();
 }
 
-hipError_t launch_final_gather(hipStream_t st, const SceneDev& S, int cap, const BakeSeg* segs, int nsegs,
-                               uint64_t total_lumels, const FinalGatherParams& P, const int32_t* mesh_pm,
-                               const PhotonMapDev* pms, unsigned long long* rays) {
+hipError_t launch_final_gather(hipStream_t st, const SceneDev& S, int cap, uint32_t max_blocks,
+                               const BakeSeg* segs, int nsegs, uint64_t total_lumels,
+                               const FinalGatherParams& P, const int32_t* mesh_pm, const PhotonMapDev* pms,
+                               unsigned long long* queue, unsigned long long* rays) {
     if (total_lumels == 0) return hipSuccess;
-    const uint32_t blocks = blocks_for(total_lumels, kBlock / 64);
-    RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_final_gather<CAP>, dim3(blocks), dim3(kBlock), 0, st, S, segs, nsegs,
-                                            total_lumels, P, mesh_pm, pms, rays));
+    hipError_t e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(queue), 2, 0u);
+    if (e != hipSuccess) return e;
+    const uint64_t need = blocks_for(total_lumels, kBlock / 64);
+    const uint32_t grid = (uint32_t)(need < max_blocks ? need : max_blocks);
+    RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_final_gather<CAP>, dim3(grid), dim3(kBlock), 0, st, S, segs, nsegs,
+                                            total_lumels, P, mesh_pm, pms, queue, rays));
     return hipGetLastError();
 }
 

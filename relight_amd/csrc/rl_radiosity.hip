@@ -1,0 +1,259 @@
+// Radiosity on the GPU: form-factor rows by hemisphere rays, and the gather B = E + rho o (F B)
+// solved by Jacobi sweeps on BF16 MFMA (fp32 accumulation) inside fp32 iterative refinement.
+//
+// Reference: RadiosityBuilder (radiosity/RadiosityBuilder.cpp:44-176) builds pairwise
+// point-to-point form factors with one visibility ray per pair; RadiositySolver::solve is empty
+// (radiosity/RadiositySolver.cpp:34-36) and the only gather iteration is the legacy
+// indirect_i = rho_i * sum_j F_ij * injected_j (Radiosity.cpp:130-146). Here F_ij is estimated as
+// the fraction of `rays` cosine-weighted rays from patch i whose closest hit lies on patch j
+// (rows sum to <= 1), stored exactly as 16-bit hit counts.
+//
+// F layout: u16 counts, row-major, rows padded to a multiple of 64 entries. Vectors (E, rho, B,
+// residual, correction) are column-major fp32 [rhs][patch]; MFMA operands X are column-major
+// bf16 splits of a vector (x = x1 + x2 + x3 to ~24 bits).
+#include <hip/hip_runtime.h>
+
+#include "rl_device.h"
+#include "rl_kernels.h"
+#include "rl_traverse.h"
+
+namespace rl {
+
+namespace {
+
+using namespace trav;
+
+constexpr int kFfBlock = 128;
+
+// One workgroup per patch row: `rays` closest-hit rays from uniform points on the patch with
+// cosine-weighted directions; hits are histogrammed per patch in LDS (two u16 counters per word,
+// counts <= 65535) and the row is written out as counts.
+template <int CAP>
+__global__ __launch_bounds__(kFfBlock) void k_form_factor_rows(SceneDev S, const PatchDev* __restrict__ patches,
+                                                               const uint32_t* __restrict__ tri_patch,
+                                                               FormFactorParams P, uint16_t* __restrict__ F,
+                                                               unsigned long long* rays_out) {
+    extern __shared__ uint32_t smem[];
+    uint32_t* stack = smem;
+    uint32_t* hist = smem + CAP * kFfBlock;
+    const uint32_t row = P.first_row + blockIdx.x;
+    const uint32_t words = P.row_stride / 2;
+    for (uint32_t w = threadIdx.x; w < words; w += kFfBlock) hist[w] = 0u;
+    __syncthreads();
+    const PatchDev pd = patches[row];
+    const V3 o = v3(pd.o[0], pd.o[1], pd.o[2]), u = v3(pd.u[0], pd.u[1], pd.u[2]);
+    const V3 v = v3(pd.v[0], pd.v[1], pd.v[2]), n = v3(pd.n[0], pd.n[1], pd.n[2]);
+    uint32_t rays = 0;
+    for (uint32_t k = threadIdx.x; k < P.rays; k += kFfBlock) {
+        Pcg32 rng = sample_stream(P.seed, 0xFFFFFFFEu, row, TECH_FORM_FACTOR, k);
+        const float s = pcg32_float(rng), t = pcg32_float(rng);
+        const V3 p = o + u * s + v * t;
+        const V3 dir = sample_cosine_hemisphere(rng, n);
+        const Ray ray = ray_from_segment(p, p + dir * P.max_distance);
+        TraceResult tr;
+        rays++;
+        if (traverse<false, CAP>(S, ray, stack + threadIdx.x, kFfBlock, tr)) {
+            const uint32_t j = tri_patch[tr.tri];
+            if (j < P.num_patches) atomicAdd(&hist[j >> 1], 1u << ((j & 1u) * 16u));
+        }
+    }
+    __syncthreads();
+    uint32_t* out = reinterpret_cast<uint32_t*>(F + (size_t)row * P.row_stride);
+    for (uint32_t w = threadIdx.x; w < words; w += kFfBlock) out[w] = hist[w];
+    count_rays(rays, rays_out);
+}
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// Y_part[split][c][row] = sum over this split's k of F[row][k] * X[k][c] with
+// v_mfma_f32_16x16x32_bf16: each wavefront owns 16 rows and all NT*16 columns. A operand lane l
+// holds F[row0 + l%16][k + 8*(l/16) .. +7]; B operand lane l holds X[k + 8*(l/16) .. +7][16t + l%16];
+// the accumulator lane l holds rows row0 + 4*(l/16) + i, column 16t + l%16.
+// ACC = false: F_hi * x1 (one BF16 product). ACC = true: (F_hi + F_lo) * (x1 + x2 + x3) without the
+// F_lo * x3 term, where F_hi = bf16(count) and F_lo = count - F_hi are both exact.
+template <int NT, bool ACC>
+__global__ __launch_bounds__(256) void k_ff_gemm(const uint16_t* __restrict__ F, uint32_t stride,
+                                                 const __bf16* __restrict__ X1, const __bf16* __restrict__ X2,
+                                                 const __bf16* __restrict__ X3, uint32_t npad,
+                                                 uint32_t klen, float* __restrict__ Yp) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t row0 = (blockIdx.x * 4u + wave) * 16u;
+    const uint32_t k0 = blockIdx.y * klen;
+    const uint32_t sub = lane >> 4, col = lane & 15u;
+    f32x4 acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    const uint16_t* frow = F + (size_t)(row0 + col) * stride + 8u * sub;
+    for (uint32_t k = k0; k < k0 + klen; k += 32) {
+        const uint4 raw = *reinterpret_cast<const uint4*>(frow + k);
+        const uint32_t w[4] = {raw.x, raw.y, raw.z, raw.w};
+        bf16x8 ahi, alo;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const float fc = (float)((w[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu);
+            const __bf16 h = (__bf16)fc;
+            ahi[j] = h;
+            alo[j] = (__bf16)(fc - (float)h);
+        }
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            const size_t xo = (size_t)(16u * t + col) * npad + k + 8u * sub;
+            const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(X1 + xo);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, b1, acc[t], 0, 0, 0);
+            if (ACC) {
+                const bf16x8 b2 = *reinterpret_cast<const bf16x8*>(X2 + xo);
+                const bf16x8 b3 = *reinterpret_cast<const bf16x8*>(X3 + xo);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, b2, acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, b3, acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, b1, acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, b2, acc[t], 0, 0, 0);
+            }
+        }
+    }
+    const uint32_t mpad = NT * 16;
+    float* out = Yp + (size_t)blockIdx.y * mpad * npad;
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) out[(size_t)(16u * t + col) * npad + row0 + 4u * sub + i] = acc[t][i];
+    }
+}
+
+__device__ __forceinline__ void split3(float x, __bf16& a, __bf16& b, __bf16& c) {
+    a = (__bf16)x;
+    const float r1 = x - (float)a;
+    b = (__bf16)r1;
+    c = (__bf16)(r1 - (float)b);
+}
+
+__device__ __forceinline__ float sum_parts(const float* Yp, uint32_t nsplit, size_t plane, size_t idx) {
+    float y = 0.0f;
+    for (uint32_t s = 0; s < nsplit; s++) y += Yp[s * plane + idx];
+    return y;
+}
+
+// Block-level sum of squares in a fixed order (deterministic), one partial per (column, block).
+__device__ __forceinline__ void block_sq_sum(float v, float* part) {
+    __shared__ float red[256];
+    red[threadIdx.x] = v * v;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *part = red[0];
+}
+
+// r = E + rho o (F B) / rays - B; d = r; x1 = bf16(r). grid (npad/256, mpad).
+__global__ __launch_bounds__(256) void k_ff_residual(const float* __restrict__ Yp, uint32_t nsplit, uint32_t npad,
+                                                     uint32_t mpad, float inv_rays, const float* __restrict__ E,
+                                                     const float* __restrict__ rho, const float* __restrict__ B,
+                                                     float* __restrict__ R, float* __restrict__ D,
+                                                     __bf16* __restrict__ X1, float* __restrict__ part) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x, c = blockIdx.y;
+    const size_t idx = (size_t)c * npad + i;
+    const float y = sum_parts(Yp, nsplit, (size_t)mpad * npad, idx);
+    const float r = E[idx] + rho[idx] * (y * inv_rays) - B[idx];
+    R[idx] = r;
+    D[idx] = r;
+    X1[idx] = (__bf16)r;
+    block_sq_sum(r, part + (size_t)c * gridDim.x + blockIdx.x);
+}
+
+// One Jacobi sweep on the correction: d = r + rho o (F_hi d) / rays; x1 = bf16(d).
+__global__ __launch_bounds__(256) void k_ff_sweep(const float* __restrict__ Yp, uint32_t nsplit, uint32_t npad,
+                                                  uint32_t mpad, float inv_rays, const float* __restrict__ rho,
+                                                  const float* __restrict__ R, float* __restrict__ D,
+                                                  __bf16* __restrict__ X1) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x, c = blockIdx.y;
+    const size_t idx = (size_t)c * npad + i;
+    const float y = sum_parts(Yp, nsplit, (size_t)mpad * npad, idx);
+    const float d = R[idx] + rho[idx] * (y * inv_rays);
+    D[idx] = d;
+    X1[idx] = (__bf16)d;
+}
+
+// B += d (or B = d when init), then the three-way bf16 split of B.
+__global__ __launch_bounds__(256) void k_ff_apply(uint32_t npad, float* __restrict__ B, const float* __restrict__ D,
+                                                  int init, __bf16* __restrict__ X1, __bf16* __restrict__ X2,
+                                                  __bf16* __restrict__ X3) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x, c = blockIdx.y;
+    const size_t idx = (size_t)c * npad + i;
+    const float b = init ? D[idx] : B[idx] + D[idx];
+    B[idx] = b;
+    __bf16 a1, a2, a3;
+    split3(b, a1, a2, a3);
+    X1[idx] = a1;
+    X2[idx] = a2;
+    X3[idx] = a3;
+}
+
+uint32_t blocks_for(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
+
+}  // namespace
+
+hipError_t launch_form_factor_rows(hipStream_t st, const SceneDev& S, int cap, const PatchDev* patches,
+                                   const uint32_t* tri_patch, const FormFactorParams& P, uint16_t* F,
+                                   unsigned long long* rays) {
+    if (P.rows == 0) return hipSuccess;
+    const size_t lds = (size_t)cap * kFfBlock * 4 + (size_t)P.row_stride * 2;
+    if (lds > 65536) return hipErrorInvalidValue;
+    if (cap == 32) {
+        hipLaunchKernelGGL(k_form_factor_rows<32>, dim3(P.rows), dim3(kFfBlock), lds, st, S, patches, tri_patch, P, F,
+                           rays);
+    } else {
+        hipLaunchKernelGGL(k_form_factor_rows<64>, dim3(P.rows), dim3(kFfBlock), lds, st, S, patches, tri_patch, P, F,
+                           rays);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_ff_gemm(hipStream_t st, bool accurate, const uint16_t* F, uint32_t stride, const __bf16* X1,
+                          const __bf16* X2, const __bf16* X3, uint32_t npad, uint32_t mpad, uint32_t nsplit,
+                          float* Yp) {
+    const uint32_t klen = npad / nsplit;
+    const dim3 grid(npad / 64, nsplit), block(256);
+#define RL_GEMM_CASE(NTV)                                                                                     \
+    case NTV:                                                                                                 \
+        if (accurate)                                                                                         \
+            hipLaunchKernelGGL((k_ff_gemm<NTV, true>), grid, block, 0, st, F, stride, X1, X2, X3, npad, klen, Yp); \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_ff_gemm<NTV, false>), grid, block, 0, st, F, stride, X1, X2, X3, npad, klen, Yp); \
+        break;
+    switch (mpad / 16) {
+        RL_GEMM_CASE(1)
+        RL_GEMM_CASE(2)
+        RL_GEMM_CASE(3)
+        RL_GEMM_CASE(4)
+        RL_GEMM_CASE(6)
+        RL_GEMM_CASE(8)
+        default:
+            return hipErrorInvalidValue;
+    }
+#undef RL_GEMM_CASE
+    return hipGetLastError();
+}
+
+hipError_t launch_ff_residual(hipStream_t st, const float* Yp, uint32_t nsplit, uint32_t npad, uint32_t mpad,
+                              float inv_rays, const float* E, const float* rho, const float* B, float* R, float* D,
+                              __bf16* X1, float* part) {
+    hipLaunchKernelGGL(k_ff_residual, dim3(npad / 256, mpad), dim3(256), 0, st, Yp, nsplit, npad, mpad, inv_rays, E,
+                       rho, B, R, D, X1, part);
+    return hipGetLastError();
+}
+
+hipError_t launch_ff_sweep(hipStream_t st, const float* Yp, uint32_t nsplit, uint32_t npad, uint32_t mpad,
+                           float inv_rays, const float* rho, const float* R, float* D, __bf16* X1) {
+    hipLaunchKernelGGL(k_ff_sweep, dim3(npad / 256, mpad), dim3(256), 0, st, Yp, nsplit, npad, mpad, inv_rays, rho, R,
+                       D, X1);
+    return hipGetLastError();
+}
+
+hipError_t launch_ff_apply(hipStream_t st, uint32_t npad, uint32_t mpad, float* B, const float* D, int init,
+                           __bf16* X1, __bf16* X2, __bf16* X3) {
+    hipLaunchKernelGGL(k_ff_apply, dim3(npad / 256, mpad), dim3(256), 0, st, npad, B, D, init, X1, X2, X3);
+    return hipGetLastError();
+}
+
+}  // namespace rl

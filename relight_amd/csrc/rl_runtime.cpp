@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstring>
 #include <new>
 #include <string>
@@ -78,6 +79,8 @@ struct rl_relight {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     unsigned long long* d_rays = nullptr;
+    unsigned long long* d_queue = nullptr;  // persistent-kernel work counter
+    uint32_t max_blocks = 4096;             // persistent grid: 16 blocks per compute unit
     rl_stats stats{};
     uint32_t next_lm_seed = 0;
     // Event pairs around each kernel launch of the current call: (slot, start, end).
@@ -365,9 +368,9 @@ int32_t run_direct(rl_relight* ctx, rl_scene* scene, rl_lightmap* const* maps, i
     if (total) {
         e = kernel_begin(ctx, RL_KERNEL_DIRECT);
         if (e == hipSuccess)
-            e = launch_bake_direct(ctx->stream, scene->dev(), scene->cap, d_segs, nsegs, total,
+            e = launch_bake_direct(ctx->stream, scene->dev(), scene->cap, ctx->max_blocks, d_segs, nsegs, total,
                                    scene->d_lights, (int)scene->lights.size(), scene->d_lverts,
-                                   ctx->d_rays);
+                                   ctx->d_queue, ctx->d_rays);
         if (e == hipSuccess) e = kernel_end(ctx);
     }
     ctx->stats.texels += tex;
@@ -412,7 +415,8 @@ int32_t run_ao(rl_relight* ctx, rl_scene* scene, rl_lightmap* const* maps, int n
     if (pairs) {
         if (e == hipSuccess) e = kernel_begin(ctx, RL_KERNEL_AO_COUNT);
         if (e == hipSuccess)
-            e = launch_ao_count(ctx->stream, scene->dev(), scene->cap, d_pairs, nsegs, pairs, P, ctx->d_rays);
+            e = launch_ao_count(ctx->stream, scene->dev(), scene->cap, ctx->max_blocks, d_pairs, nsegs, pairs, P,
+                                ctx->d_queue, ctx->d_rays);
         if (e == hipSuccess) e = kernel_end(ctx);
         if (e == hipSuccess) e = kernel_begin(ctx, RL_KERNEL_AO_FINALIZE);
         if (e == hipSuccess) e = launch_ao_finalize(ctx->stream, d_lum, nsegs, lumels, P);
@@ -527,9 +531,14 @@ int32_t rl_create(rl_relight** out) {
     hipGetDevice(&ctx->device);
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
-        hipMalloc((void**)&ctx->d_rays, sizeof(unsigned long long)) != hipSuccess) {
+        hipMalloc((void**)&ctx->d_rays, sizeof(unsigned long long)) != hipSuccess ||
+        hipMalloc((void**)&ctx->d_queue, 4 * sizeof(unsigned long long)) != hipSuccess) {
         delete ctx;
         return fail(RL_DEVICE_ERROR, "stream/event/counter creation failed");
+    }
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) == hipSuccess && cus > 0) {
+        ctx->max_blocks = (uint32_t)cus * 16u;
     }
     *out = ctx;
     return RL_SUCCESS;
@@ -538,6 +547,7 @@ int32_t rl_create(rl_relight** out) {
 int32_t rl_destroy(rl_relight* ctx) {
     if (!ctx) return RL_SUCCESS;
     if (ctx->d_rays) hipFree(ctx->d_rays);
+    if (ctx->d_queue) hipFree(ctx->d_queue);
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
     for (hipEvent_t e : ctx->pool) hipEventDestroy(e);
@@ -906,7 +916,8 @@ int32_t rl_trace_segments(rl_relight* ctx, rl_scene* scene, const float* starts,
     if (e == hipSuccess && (st = begin_call(ctx))) { cleanup(); return st; }
     if (e == hipSuccess) e = kernel_begin(ctx, RL_KERNEL_TRACE);
     if (e == hipSuccess)
-        e = launch_trace(ctx->stream, scene->dev(), scene->cap, mode == RL_TRACE_ANY, da, db, n, df, dh, ctx->d_rays);
+        e = launch_trace(ctx->stream, scene->dev(), scene->cap, ctx->max_blocks, mode == RL_TRACE_ANY, da, db, n, df,
+                         dh, ctx->d_queue, ctx->d_rays);
     if (e == hipSuccess) e = kernel_end(ctx);
     ctx->stats.launches = 1;
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
@@ -1056,8 +1067,8 @@ int32_t rl_emit_photons(rl_relight* ctx, rl_scene* scene, const rl_indirect_sett
     e = hipMemcpy(d_first, first.data(), nl * sizeof(uint64_t), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = kernel_begin(ctx, RL_KERNEL_PHOTON_TRACE);
     if (e == hipSuccess)
-        e = launch_photon_trace(ctx->stream, scene->dev(), scene->cap, scene->d_lights, d_first, P, T.d_mesh_pm,
-                                T.d_pms, d_rec, ctx->d_rays);
+        e = launch_photon_trace(ctx->stream, scene->dev(), scene->cap, ctx->max_blocks, scene->d_lights, d_first, P,
+                                T.d_mesh_pm, T.d_pms, d_rec, ctx->d_queue, ctx->d_rays);
     if (e == hipSuccess) e = kernel_end(ctx);
     if (e == hipSuccess) e = kernel_begin(ctx, RL_KERNEL_PHOTON_STORE);
     if (e == hipSuccess)
@@ -1106,8 +1117,8 @@ int32_t rl_bake_indirect(rl_relight* ctx, rl_scene* scene, int32_t mesh, const r
     if (total) {
         e = kernel_begin(ctx, RL_KERNEL_FINAL_GATHER);
         if (e == hipSuccess)
-            e = launch_final_gather(ctx->stream, scene->dev(), scene->cap, d_segs, 1, total, P, T.d_mesh_pm, T.d_pms,
-                                    ctx->d_rays);
+            e = launch_final_gather(ctx->stream, scene->dev(), scene->cap, ctx->max_blocks, d_segs, 1, total, P,
+                                    T.d_mesh_pm, T.d_pms, ctx->d_queue, ctx->d_rays);
         if (e == hipSuccess) e = kernel_end(ctx);
     }
     ctx->stats.texels += tex;
@@ -1117,6 +1128,201 @@ int32_t rl_bake_indirect(rl_relight* ctx, rl_scene* scene, int32_t mesh, const r
     if (e != hipSuccess) return fail(RL_DEVICE_ERROR, std::string("final gather: ") + hipGetErrorString(e));
     lm->device_newer = true;
     return end_call(ctx, tm);
+}
+
+}  // extern "C"
+
+struct rl_radiosity {
+    rl_relight* ctx = nullptr;
+    rl_scene* scene = nullptr;
+    uint32_t n = 0, npad = 0;
+    PatchDev* d_patches = nullptr;
+    uint32_t* d_tri_patch = nullptr;
+    uint16_t* d_F = nullptr;
+    bool built = false;
+    uint32_t rays = 0;
+    ~rl_radiosity() {
+        dev_free(d_patches);
+        dev_free(d_tri_patch);
+        dev_free(d_F);
+    }
+};
+
+extern "C" {
+
+int32_t rl_radiosity_create(rl_relight* ctx, rl_scene* scene, const float* patches, uint32_t num_patches,
+                            const uint32_t* tri_patch, rl_radiosity** out) {
+    if (!ctx || !out || !patches || !tri_patch) return fail(RL_INVALID_ARGUMENT, "NULL argument");
+    int32_t st = check_scene_ready(scene);
+    if (st) return st;
+    if (num_patches == 0 || num_patches > 16384)
+        return fail(RL_REJECTED, "patch count must be in [1, 16384] (one LDS histogram per row)");
+    for (uint32_t t = 0; t < scene->num_tris; t++)
+        if (tri_patch[t] != 0xFFFFFFFFu && tri_patch[t] >= num_patches)
+            return fail(RL_INVALID_ARGUMENT, "triangle patch index out of range");
+    rl_radiosity* r = new (std::nothrow) rl_radiosity();
+    if (!r) return fail(RL_OUT_OF_MEMORY, "radiosity allocation failed");
+    r->ctx = ctx;
+    r->scene = scene;
+    r->n = num_patches;
+    r->npad = (num_patches + 255u) / 256u * 256u;
+    std::vector<PatchDev> pd(num_patches);
+    for (uint32_t i = 0; i < num_patches; i++) std::memcpy(&pd[i], patches + 12 * (size_t)i, sizeof(PatchDev));
+    hipError_t e = dev_alloc(&r->d_patches, num_patches);
+    if (e == hipSuccess) e = dev_alloc(&r->d_tri_patch, scene->num_tris);
+    if (e == hipSuccess) e = dev_alloc(&r->d_F, (size_t)r->npad * r->npad);
+    if (e == hipSuccess) e = hipMemcpy(r->d_patches, pd.data(), pd.size() * sizeof(PatchDev), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(r->d_tri_patch, tri_patch, scene->num_tris * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(r->d_F, 0, (size_t)r->npad * r->npad * sizeof(uint16_t));
+    if (e != hipSuccess) {
+        delete r;
+        return fail(e == hipErrorOutOfMemory ? RL_OUT_OF_MEMORY : RL_DEVICE_ERROR,
+                    std::string("radiosity buffers: ") + hipGetErrorString(e));
+    }
+    *out = r;
+    return RL_SUCCESS;
+}
+
+int32_t rl_radiosity_destroy(rl_radiosity* r) {
+    delete r;
+    return RL_SUCCESS;
+}
+
+int32_t rl_radiosity_build_form_factors(rl_relight* ctx, rl_radiosity* r, uint32_t rays_per_patch, uint32_t seed,
+                                        float max_distance) {
+    if (!ctx || !r) return fail(RL_INVALID_ARGUMENT, "NULL argument");
+    if (rays_per_patch == 0 || rays_per_patch > 65535) return fail(RL_INVALID_ARGUMENT, "rays per patch must be in [1, 65535]");
+    if (!(max_distance > 0.0f)) return fail(RL_INVALID_ARGUMENT, "max distance must be positive");
+    CallTimer tm(ctx);
+    int32_t st = begin_call(ctx);
+    if (st) return st;
+    FormFactorParams P;
+    P.num_patches = r->n;
+    P.row_stride = r->npad;
+    P.rays = rays_per_patch;
+    P.seed = seed;
+    P.max_distance = max_distance;
+    P.first_row = 0;
+    P.rows = r->n;
+    P.pad = 0;
+    hipError_t e = kernel_begin(ctx, RL_KERNEL_FORM_FACTORS);
+    if (e == hipSuccess)
+        e = launch_form_factor_rows(ctx->stream, r->scene->dev(), r->scene->cap, r->d_patches, r->d_tri_patch, P,
+                                    r->d_F, ctx->d_rays);
+    if (e == hipSuccess) e = kernel_end(ctx);
+    ctx->stats.launches = 1;
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return fail(RL_DEVICE_ERROR, std::string("form factors: ") + hipGetErrorString(e));
+    r->built = true;
+    r->rays = rays_per_patch;
+    return end_call(ctx, tm);
+}
+
+int32_t rl_radiosity_read_form_factors(rl_radiosity* r, uint32_t first_row, uint32_t rows, uint16_t* counts) {
+    if (!r || !counts) return fail(RL_INVALID_ARGUMENT, "NULL argument");
+    if ((uint64_t)first_row + rows > r->n) return fail(RL_INVALID_ARGUMENT, "row range out of bounds");
+    RL_HIP(hipMemcpy2D(counts, r->n * sizeof(uint16_t), r->d_F + (size_t)first_row * r->npad,
+                       r->npad * sizeof(uint16_t), r->n * sizeof(uint16_t), rows, hipMemcpyDeviceToHost));
+    return RL_SUCCESS;
+}
+
+int32_t rl_radiosity_solve(rl_relight* ctx, rl_radiosity* r, const float* emission, const float* reflectance,
+                           uint32_t num_rhs, float tolerance, int32_t max_outer, int32_t inner_sweeps,
+                           float* out_radiosity, rl_solve_info* info) {
+    if (!ctx || !r || !emission || !reflectance || !out_radiosity) return fail(RL_INVALID_ARGUMENT, "NULL argument");
+    if (!r->built) return fail(RL_INVALID_CALL, "form factors have not been built");
+    if (num_rhs == 0 || num_rhs > 128) return fail(RL_INVALID_ARGUMENT, "right-hand sides must be in [1, 128]");
+    if (max_outer <= 0 || inner_sweeps < 0) return fail(RL_INVALID_ARGUMENT, "bad iteration limits");
+    const uint32_t tiles[] = {1, 2, 3, 4, 6, 8};
+    uint32_t nt = 8;
+    for (uint32_t t : tiles) {
+        if (t * 16 >= num_rhs) { nt = t; break; }
+    }
+    const uint32_t n = r->n, npad = r->npad, mpad = nt * 16, nsplit = 4;
+    const size_t vec = (size_t)mpad * npad;
+    // Row-major (patch, rhs) -> column-major padded.
+    std::vector<float> hE(vec, 0.0f), hR(vec, 0.0f);
+    double e2 = 0.0;
+    for (uint32_t i = 0; i < n; i++) {
+        for (uint32_t c = 0; c < num_rhs; c++) {
+            const float ev = emission[(size_t)i * num_rhs + c], rv = reflectance[(size_t)i * num_rhs + c];
+            hE[(size_t)c * npad + i] = ev;
+            hR[(size_t)c * npad + i] = rv;
+            e2 += (double)ev * ev;
+        }
+    }
+    float *E = nullptr, *RHO = nullptr, *B = nullptr, *R = nullptr, *D = nullptr, *Yp = nullptr, *part = nullptr;
+    __bf16 *X1 = nullptr, *X2 = nullptr, *X3 = nullptr;
+    auto cleanup = [&]() {
+        dev_free(E); dev_free(RHO); dev_free(B); dev_free(R); dev_free(D); dev_free(Yp); dev_free(part);
+        dev_free(X1); dev_free(X2); dev_free(X3);
+    };
+    const uint32_t nblk = npad / 256;
+    hipError_t e = dev_alloc(&E, vec);
+    if (e == hipSuccess) e = dev_alloc(&RHO, vec);
+    if (e == hipSuccess) e = dev_alloc(&B, vec);
+    if (e == hipSuccess) e = dev_alloc(&R, vec);
+    if (e == hipSuccess) e = dev_alloc(&D, vec);
+    if (e == hipSuccess) e = dev_alloc(&Yp, vec * nsplit);
+    if (e == hipSuccess) e = dev_alloc(&part, (size_t)mpad * nblk);
+    if (e == hipSuccess) e = dev_alloc(&X1, vec);
+    if (e == hipSuccess) e = dev_alloc(&X2, vec);
+    if (e == hipSuccess) e = dev_alloc(&X3, vec);
+    if (e != hipSuccess) {
+        cleanup();
+        return fail(RL_OUT_OF_MEMORY, "radiosity solve buffers");
+    }
+    CallTimer tm(ctx);
+    int32_t st = begin_call(ctx);
+    if (st) { cleanup(); return st; }
+    e = hipMemcpy(E, hE.data(), vec * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(RHO, hR.data(), vec * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(D, hE.data(), vec * sizeof(float), hipMemcpyHostToDevice);
+    // B0 = E.
+    if (e == hipSuccess) e = launch_ff_apply(ctx->stream, npad, mpad, B, D, 1, X1, X2, X3);
+    const float inv_rays = 1.0f / (float)r->rays;
+    const double enorm = std::sqrt(e2);
+    std::vector<float> hp((size_t)mpad * nblk);
+    double rel = 0.0;
+    int outer = 0, sweeps = 0;
+    for (; e == hipSuccess && outer < max_outer; outer++) {
+        e = kernel_begin(ctx, RL_KERNEL_FF_GEMM_ACCURATE);
+        if (e == hipSuccess) e = launch_ff_gemm(ctx->stream, true, r->d_F, npad, X1, X2, X3, npad, mpad, nsplit, Yp);
+        if (e == hipSuccess) e = kernel_end(ctx);
+        if (e == hipSuccess) e = launch_ff_residual(ctx->stream, Yp, nsplit, npad, mpad, inv_rays, E, RHO, B, R, D, X1, part);
+        if (e == hipSuccess) e = hipMemcpyAsync(hp.data(), part, hp.size() * sizeof(float), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) break;
+        double r2 = 0.0;
+        for (float v : hp) r2 += (double)v;
+        rel = enorm > 0.0 ? std::sqrt(r2) / enorm : std::sqrt(r2);
+        if (rel <= (double)tolerance) break;
+        for (int s = 0; e == hipSuccess && s < inner_sweeps; s++) {
+            e = kernel_begin(ctx, RL_KERNEL_FF_GEMM_FAST);
+            if (e == hipSuccess) e = launch_ff_gemm(ctx->stream, false, r->d_F, npad, X1, X2, X3, npad, mpad, nsplit, Yp);
+            if (e == hipSuccess) e = kernel_end(ctx);
+            if (e == hipSuccess) e = launch_ff_sweep(ctx->stream, Yp, nsplit, npad, mpad, inv_rays, RHO, R, D, X1);
+            sweeps++;
+        }
+        if (e == hipSuccess) e = launch_ff_apply(ctx->stream, npad, mpad, B, D, 0, X1, X2, X3);
+    }
+    std::vector<float> hB(vec);
+    if (e == hipSuccess) e = hipMemcpy(hB.data(), B, vec * sizeof(float), hipMemcpyDeviceToHost);
+    cleanup();
+    if (e != hipSuccess) return fail(RL_DEVICE_ERROR, std::string("radiosity solve: ") + hipGetErrorString(e));
+    for (uint32_t i = 0; i < n; i++)
+        for (uint32_t c = 0; c < num_rhs; c++) out_radiosity[(size_t)i * num_rhs + c] = hB[(size_t)c * npad + i];
+    st = end_call(ctx, tm);
+    if (info) {
+        info->outer_iterations = outer + (outer < max_outer ? 1 : 0);
+        info->sweeps = sweeps;
+        info->residual = rel;
+        info->gpu_ms = ctx->stats.gpu_ms;
+    }
+    if (st) return st;
+    if (rel > (double)tolerance) return fail(RL_NOT_CONVERGED, "radiosity solve did not reach the tolerance");
+    return RL_SUCCESS;
 }
 
 int32_t rl_get_stats(const rl_relight* ctx, rl_stats* out) {

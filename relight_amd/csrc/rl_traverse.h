@@ -130,6 +130,15 @@ __device__ __forceinline__ int find_seg(const BakeSeg* segs, int nsegs, uint64_t
     return lo;
 }
 
+// Persistent work queue: lane 0 of a wavefront takes the next chunk index, every lane gets it.
+// A wavefront keeps taking chunks until the index passes the chunk count, so every wavefront
+// reaches its exit and the grid drains.
+__device__ __forceinline__ uint64_t wave_next_chunk(unsigned long long* counter) {
+    unsigned long long c = 0;
+    if (lane_id() == 0) c = atomicAdd(counter, 1ull);
+    return __shfl(c, 0, 64);
+}
+
 __device__ __forceinline__ void count_rays(uint32_t n, unsigned long long* counter) {
     // Wave sum of per-lane ray counts, one atomic per wavefront.
 #pragma unroll

@@ -340,6 +340,61 @@ class Photonmap(Lightmap):
         return c, p, g
 
 
+class Radiosity:
+    """Radiosity patches, GPU form factors and the BF16-MFMA gather solve
+    (radiosity/RadiosityBuilder.cpp:44-176, radiosity/RadiositySolver.cpp:34-36)."""
+
+    def __init__(self, relight: "Relight", scene: Scene, patches, tri_patch):
+        self._relight = relight
+        self._lib = relight._lib
+        p = np.ascontiguousarray(patches, np.float32).reshape(-1, 12)
+        t = np.ascontiguousarray(tri_patch, np.uint32).reshape(-1)
+        self.num_patches = p.shape[0]
+        h = C.c_void_p()
+        _check(self._lib, self._lib.rl_radiosity_create(relight._h, scene._h, _fptr(p), p.shape[0],
+                                                        _uptr(t), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            self._lib.rl_radiosity_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def build_form_factors(self, rays_per_patch=1024, seed=1, max_distance=1000.0):
+        _check(self._lib, self._lib.rl_radiosity_build_form_factors(
+            self._relight._h, self._h, int(rays_per_patch), int(seed) & 0xFFFFFFFF, float(max_distance)))
+
+    def form_factor_counts(self, first=0, rows=None) -> np.ndarray:
+        rows = self.num_patches - first if rows is None else rows
+        out = np.zeros((rows, self.num_patches), np.uint16)
+        _check(self._lib, self._lib.rl_radiosity_read_form_factors(
+            self._h, int(first), int(rows), out.ctypes.data_as(C.POINTER(C.c_uint16))))
+        return out
+
+    def solve(self, emission, reflectance, tolerance=1e-5, max_outer=20, inner_sweeps=16,
+              check=True):
+        """B = E + rho o (F B); emission/reflectance (num_patches, num_rhs). Returns (B, info)."""
+        E = np.ascontiguousarray(emission, np.float32)
+        R = np.ascontiguousarray(reflectance, np.float32)
+        assert E.shape == R.shape and E.shape[0] == self.num_patches
+        out = np.zeros_like(E)
+        info = A.rl_solve_info()
+        st = self._lib.rl_radiosity_solve(self._relight._h, self._h, _fptr(E), _fptr(R), E.shape[1],
+                                          float(tolerance), int(max_outer), int(inner_sweeps),
+                                          _fptr(out), C.byref(info))
+        res = {"outer_iterations": info.outer_iterations, "sweeps": info.sweeps,
+               "residual": info.residual, "gpu_ms": info.gpu_ms, "status": st}
+        if check and st != A.RL_SUCCESS:
+            _check(self._lib, st)
+        return out, res
+
+
 class Relight:
     """Relight (Relight.h:155-189): factories and bake entry points, all on the GPU."""
 
@@ -372,6 +427,9 @@ class Relight:
 
     def create_photonmap(self, width: int, height: int) -> Photonmap:
         return Photonmap(self, width, height)
+
+    def create_radiosity(self, scene: Scene, patches, tri_patch) -> Radiosity:
+        return Radiosity(self, scene, patches, tri_patch)
 
     def emit_photons(self, scene: Scene, settings: IndirectLightSettings):
         s = settings.to_c()

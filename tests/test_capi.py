@@ -46,4 +46,4 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(_capi.rl_light_desc) == 4 * 2 + 36 + 4 + 4 + 24 + 4 + 16 + 8
     assert ctypes.sizeof(_capi.rl_ao_settings) == 24
     assert ctypes.sizeof(_capi.rl_hit) == 24
-    assert ctypes.sizeof(_capi.rl_stats) == 40 + 8 * 8 + 4 * 8
+    assert ctypes.sizeof(_capi.rl_stats) == 40 + 16 * 8 + 4 * 16

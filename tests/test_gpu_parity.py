@@ -76,7 +76,9 @@ def test_box_room_direct_and_ao_bit_exact(relight):
 def test_ao_sampling_modes_bit_exact(relight, sampling):
     print(f"[gpu] box_room AO sampling={sampling}", flush=True)
     spec = scenes.box_room()
-    spec.ao = AmbientOcclusionSettings(48, 0.7, 1.1, 1.5, 77, sampling)
+    # Exponent 2 (not 1): exercises powf; a fractional exponent of the reference's unclamped
+    # negative values would give NaN on both sides.
+    spec.ao = AmbientOcclusionSettings(48, 0.7, 1.1, 2.0, 77, sampling)
     scene, lms, ids = scenes.build_gpu(relight, spec)
     relight.bake_direct_light(scene, ids[0])
     relight.bake_ambient_occlusion(scene, ids[0], spec.ao)
