@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -179,7 +180,7 @@ private:
 
 class Collapser {
 public:
-    explicit Collapser(const std::vector<Node2>& n2) : n2_(n2) {}
+    Collapser(const std::vector<Node2>& n2, float pad) : n2_(n2), pad_(pad) {}
 
     // Builds the 4-wide tree rooted at binary node r; the root is always an internal node.
     void run() {
@@ -231,9 +232,9 @@ private:
                 continue;
             }
             const Node2& k = n2_[kids[i]];
-            node.lox[i] = k.box.lo[0]; node.hix[i] = k.box.hi[0];
-            node.loy[i] = k.box.lo[1]; node.hiy[i] = k.box.hi[1];
-            node.loz[i] = k.box.lo[2]; node.hiz[i] = k.box.hi[2];
+            node.lox[i] = k.box.lo[0] - pad_; node.hix[i] = k.box.hi[0] + pad_;
+            node.loy[i] = k.box.lo[1] - pad_; node.hiy[i] = k.box.hi[1] + pad_;
+            node.loz[i] = k.box.lo[2] - pad_; node.hiz[i] = k.box.hi[2] + pad_;
             if (k.left < 0) {
                 node.child[i] = RL_LEAF_BIT | ((k.count - 1) << 27) | k.first;
             } else {
@@ -257,6 +258,7 @@ private:
     }
 
     const std::vector<Node2>& n2_;
+    float pad_;
     int depth_cur_ = 1;
 };
 
@@ -270,7 +272,12 @@ int build_bvh4(const float* tri9, uint32_t num_tris, Bvh4Build& out) {
     for (const Node2& n : b2.nodes_) {
         if (n.left < 0 && n.count > RL_LEAF_MAX) return -3;
     }
-    Collapser c(b2.nodes_);
+    // Box padding for the fma slab test: more than 2^-21 of the largest coordinate magnitude
+    // (ray origins lie inside the scene, so |o| is bounded by it), plus a tiny absolute term.
+    float maxc = 0.0f;
+    for (size_t i = 0; i < (size_t)num_tris * 9; i++) maxc = std::max(maxc, std::fabs(tri9[i]));
+    const float pad = maxc * (1.0f / 1048576.0f) + 1e-30f;
+    Collapser c(b2.nodes_, pad);
     c.run();
     out.nodes = std::move(c.out_);
     out.order = std::move(b2.order_);

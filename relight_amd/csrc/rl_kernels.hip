@@ -76,7 +76,7 @@ __global__ __launch_bounds__(kBlock) void k_bake_direct(SceneDev S, const BakeSe
                                                         const float4* __restrict__ lverts,
                                                         unsigned long long* queue,
                                                         unsigned long long* rays_out) {
-    __shared__ uint32_t stack[CAP * kBlock];
+    __shared__ uint32_t stack[kLdsStack * kBlock];
     uint32_t* stk = stack + threadIdx.x;
     const uint64_t nchunks = (total + 63) / 64;
     uint32_t rays = 0;
@@ -132,7 +132,7 @@ template <int CAP>
 __global__ __launch_bounds__(kBlock) void k_ao_count(SceneDev S, const BakeSeg* __restrict__ segs,
                                                      int nsegs, uint64_t total, AoParams P,
                                                      unsigned long long* queue, unsigned long long* rays_out) {
-    __shared__ uint32_t stack[CAP * kBlock];
+    __shared__ uint32_t stack[kLdsStack * kBlock];
     uint32_t* stk = stack + threadIdx.x;
     const uint32_t lane = lane_id();
     const uint64_t nchunks = (total + 63) / 64;
@@ -205,7 +205,7 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDev S, const float4* __re
                                                   const float4* __restrict__ ends, uint32_t n,
                                                   uint8_t* __restrict__ flags, float4* __restrict__ hits,
                                                   unsigned long long* queue, unsigned long long* rays_out) {
-    __shared__ uint32_t stack[CAP * kBlock];
+    __shared__ uint32_t stack[kLdsStack * kBlock];
     const uint64_t nchunks = ((uint64_t)n + 63) / 64;
     uint32_t rays = 0;
     for (;;) {

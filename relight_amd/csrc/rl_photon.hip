@@ -61,7 +61,7 @@ __global__ __launch_bounds__(kBlock) void k_photon_trace(SceneDev S, const Light
                                                          PhotonRec* __restrict__ out,
                                                          unsigned long long* queue,
                                                          unsigned long long* rays_out) {
-    __shared__ uint32_t stack[CAP * kBlock];
+    __shared__ uint32_t stack[kLdsStack * kBlock];
     uint32_t* stk = stack + threadIdx.x;
     const uint64_t nchunks = (P.total + 63) / 64;
     uint32_t rays = 0;
@@ -228,7 +228,7 @@ __global__ __launch_bounds__(kBlock) void k_final_gather(SceneDev S, const BakeS
                                                          const PhotonMapDev* __restrict__ pms,
                                                          unsigned long long* queue,
                                                          unsigned long long* rays_out) {
-    __shared__ uint32_t stack[CAP * kBlock];
+    __shared__ uint32_t stack[kLdsStack * kBlock];
     uint32_t* stk = stack + threadIdx.x;
     const uint32_t lane = lane_id();
     uint32_t rays = 0;

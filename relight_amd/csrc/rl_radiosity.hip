@@ -35,7 +35,7 @@ __global__ __launch_bounds__(kFfBlock) void k_form_factor_rows(SceneDev S, const
                                                                unsigned long long* rays_out) {
     extern __shared__ uint32_t smem[];
     uint32_t* stack = smem;
-    uint32_t* hist = smem + CAP * kFfBlock;
+    uint32_t* hist = smem + kLdsStack * kFfBlock;
     const uint32_t row = P.first_row + blockIdx.x;
     const uint32_t words = P.row_stride / 2;
     for (uint32_t w = threadIdx.x; w < words; w += kFfBlock) hist[w] = 0u;
@@ -197,7 +197,7 @@ hipError_t launch_form_factor_rows(hipStream_t st, const SceneDev& S, int cap, c
                                    const uint32_t* tri_patch, const FormFactorParams& P, uint16_t* F,
                                    unsigned long long* rays) {
     if (P.rows == 0) return hipSuccess;
-    const size_t lds = (size_t)cap * kFfBlock * 4 + (size_t)P.row_stride * 2;
+    const size_t lds = (size_t)kLdsStack * kFfBlock * 4 + (size_t)P.row_stride * 2;
     if (lds > 65536) return hipErrorInvalidValue;
     if (cap == 32) {
         hipLaunchKernelGGL(k_form_factor_rows<32>, dim3(P.rows), dim3(kFfBlock), lds, st, S, patches, tri_patch, P, F,

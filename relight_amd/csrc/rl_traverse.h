@@ -35,21 +35,37 @@ __device__ __forceinline__ void cswap(float& ka, uint32_t& va, float& kb, uint32
     }
 }
 
+// Entries of a lane's traversal stack kept in LDS; deeper entries (rare) spill to a private
+// per-lane array. Blocks allocate kLdsStack words of LDS per thread.
+constexpr int kLdsStack = 16;
+
 // Traverses the BVH4 for one ray. ANY: stop at the first triangle with t in (tnear, tfar).
 // Closest: the hit with the smallest t, ties broken by the smaller scene-order triangle id, so
 // the result does not depend on traversal order. stk points at this lane's first LDS stack slot;
-// consecutive entries are `stride` words apart (one column per lane, conflict-free).
+// consecutive entries are `stride` words apart (one column per lane, conflict-free). CAP is the
+// total stack capacity (LDS + spill), at least the BVH's worst-case bound.
+//
+// Slab test: t = fma(b, 1/d, -o/d). Its error is a relative term (covered by scaling the exit
+// distance by kBoxScale) plus an absolute shift of the slab plane of at most |o| * 2^-24, which
+// the builder's box padding (more than 2^-21 of the largest coordinate) covers, so no box the
+// exact ray crosses is rejected.
 template <bool ANY, int CAP>
 __device__ bool traverse(const SceneDev& S, const Ray& ray, uint32_t* stk, uint32_t stride,
                          TraceResult& res) {
     const WtRay w = wt_setup(ray);
-    const float ix = 1.0f / ray.d.x, iy = 1.0f / ray.d.y, iz = 1.0f / ray.d.z;
-    const float ox = ray.o.x, oy = ray.o.y, oz = ray.o.z;
+    // A zero direction component would give fma(b, inf, -inf) = NaN; the slab test uses
+    // +-1e-20 there instead (a displacement below 1e-17 m over any ray, inside the box padding).
+    const float dx = fabsf(ray.d.x) > 1e-20f ? ray.d.x : copysignf(1e-20f, ray.d.x);
+    const float dy = fabsf(ray.d.y) > 1e-20f ? ray.d.y : copysignf(1e-20f, ray.d.y);
+    const float dz = fabsf(ray.d.z) > 1e-20f ? ray.d.z : copysignf(1e-20f, ray.d.z);
+    const float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
+    const float oix = -(ray.o.x * ix), oiy = -(ray.o.y * iy), oiz = -(ray.o.z * iz);
     const float tnear = ray.tnear;
     float tlimit = ray.tfar;  // closest: shrinks to the best hit so far
     res.t = ray.tfar;
     res.tri = 0xFFFFFFFFu;
     res.U = res.V = res.W = res.det = 0.0f;
+    uint32_t spill[CAP - kLdsStack];
 
     uint32_t node = 0;
     int sp = 0;
@@ -66,9 +82,9 @@ __device__ bool traverse(const SceneDev& S, const Ray& ray, uint32_t* stk, uint3
             const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                float t0x = (lxa[i] - ox) * ix, t1x = (hxa[i] - ox) * ix;
-                float t0y = (lya[i] - oy) * iy, t1y = (hya[i] - oy) * iy;
-                float t0z = (lza[i] - oz) * iz, t1z = (hza[i] - oz) * iz;
+                float t0x = fmaf(lxa[i], ix, oix), t1x = fmaf(hxa[i], ix, oix);
+                float t0y = fmaf(lya[i], iy, oiy), t1y = fmaf(hya[i], iy, oiy);
+                float t0z = fmaf(lza[i], iz, oiz), t1z = fmaf(hza[i], iz, oiz);
                 float tmin = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tnear));
                 float tmax = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * kBoxScale;
                 bool hit = tmin <= fminf(tmax, lim) && cid[i] != RL_EMPTY_CHILD;
@@ -82,13 +98,16 @@ __device__ bool traverse(const SceneDev& S, const Ray& ray, uint32_t* stk, uint3
             cswap(key[1], cid[1], key[2], cid[2]);
             if (key[0] == __builtin_inff()) {
                 if (sp == 0) break;
-                node = stk[(--sp) * stride];
+                --sp;
+                node = sp < kLdsStack ? stk[sp * stride] : spill[sp - kLdsStack];
                 continue;
             }
 #pragma unroll
             for (int i = 3; i >= 1; i--) {
                 if (key[i] != __builtin_inff() && sp < CAP) {
-                    stk[(sp++) * stride] = cid[i];
+                    if (sp < kLdsStack) stk[sp * stride] = cid[i];
+                    else spill[sp - kLdsStack] = cid[i];
+                    sp++;
                 }
             }
             node = cid[0];
@@ -115,7 +134,8 @@ __device__ bool traverse(const SceneDev& S, const Ray& ray, uint32_t* stk, uint3
                 }
             }
             if (sp == 0) break;
-            node = stk[(--sp) * stride];
+            --sp;
+            node = sp < kLdsStack ? stk[sp * stride] : spill[sp - kLdsStack];
         }
     }
     return res.tri != 0xFFFFFFFFu;
