@@ -84,7 +84,21 @@ def main():
     ap.add_argument("--cpu-stride", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ao-samples", type=int, default=256)
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device-to-device with a plain copy kernel (torch) and exit; "
+                         "used to scale rocprof FETCH_SIZE/WRITE_SIZE readings")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        n = args.copy_calibration * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": 4 * n}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the bake path "
               "would shard lightmaps across GPUs with no exchange; this build runs on one).",
@@ -103,6 +117,13 @@ def main():
     texels = sum(lm.valid_count() for lm in lms)
     log(f"scene built: {info}, {texels} valid texels ({time.time()-t0:.1f} s)")
 
+    tuning, tune_results = None, None
+    if not args.no_autotune:
+        from relight_amd.autotune import autotune
+        t0 = time.time()
+        tuning, tune_results = autotune(relight, scene, lms, spec.ao)
+        log(f"autotune ({time.time()-t0:.1f} s): {tuning}; "
+            + ", ".join(f"{r[1]:.1f}" for r in tune_results))
     # Check pass (also the first warmup): one batched pass from zero, kept for the parity check.
     relight.bake_batch(scene, lms, direct=True, ao=spec.ao)
     first = [lm.to_hdr() for lm in lms]
@@ -184,6 +205,7 @@ def main():
                   if plain_lat else None},
         "batched_vs_plain": (value / plain_value) if plain_value else None,
         "rmse_vs_oracle": rmse,
+        "tuning": relight.tuning(),
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -262,6 +262,22 @@ int32_t rl_radiosity_solve(rl_relight* ctx, rl_radiosity* r, const float* emissi
                            int32_t max_outer, int32_t inner_sweeps, float* out_radiosity,
                            rl_solve_info* info);
 
+/* Launch tuning of a context (no counterpart in the reference; chosen by the autotuner in
+ * relight_amd/autotune.py). ao_kernel: RL_AO_KERNEL_REFILL (lanes take new rays from a per-
+ * wavefront pool when at least refill_min of them are idle; chunks of ao_chunk_lumels lumels) or
+ * RL_AO_KERNEL_CHUNKED (64 rays per wavefront per chunk). blocks_per_cu sizes the persistent
+ * grids (128-thread blocks). Results do not depend on the tuning. */
+#define RL_AO_KERNEL_CHUNKED 0
+#define RL_AO_KERNEL_REFILL 1
+typedef struct rl_tuning {
+    int32_t ao_kernel;
+    uint32_t refill_min;
+    uint32_t ao_chunk_lumels;
+    uint32_t blocks_per_cu;
+} rl_tuning;
+int32_t rl_set_tuning(rl_relight* ctx, const rl_tuning* tuning);
+int32_t rl_get_tuning(const rl_relight* ctx, rl_tuning* out);
+
 /* Counters of the last bake/trace call on ctx. */
 int32_t rl_get_stats(const rl_relight* ctx, rl_stats* out);
 

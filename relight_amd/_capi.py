@@ -118,6 +118,15 @@ KERNEL_NAMES = {0: "direct", 1: "ao_count", 2: "ao_finalize", 3: "trace", 4: "ph
 RL_NOT_CONVERGED = 7
 
 
+RL_AO_KERNEL_CHUNKED = 0
+RL_AO_KERNEL_REFILL = 1
+
+
+class rl_tuning(C.Structure):
+    _fields_ = [("ao_kernel", C.c_int32), ("refill_min", C.c_uint32), ("ao_chunk_lumels", C.c_uint32),
+                ("blocks_per_cu", C.c_uint32)]
+
+
 class rl_solve_info(C.Structure):
     _fields_ = [("outer_iterations", C.c_int32), ("sweeps", C.c_int32), ("residual", C.c_double),
                 ("gpu_ms", C.c_double)]
@@ -178,6 +187,8 @@ SIGNATURES = [
     ("rl_radiosity_read_form_factors", C.c_int32, [_P, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint16)]),
     ("rl_radiosity_solve", C.c_int32, [_P, _P, _F, _F, C.c_uint32, C.c_float, C.c_int32, C.c_int32, _F,
                                        C.POINTER(rl_solve_info)]),
+    ("rl_set_tuning", C.c_int32, [_P, C.POINTER(rl_tuning)]),
+    ("rl_get_tuning", C.c_int32, [_P, C.POINTER(rl_tuning)]),
     ("rl_get_stats", C.c_int32, [_P, C.POINTER(rl_stats)]),
 ]
 

@@ -25,6 +25,12 @@ hipError_t launch_ao_count(hipStream_t st, const SceneDev& S, int cap, uint32_t 
                            const BakeSeg* segs, int nsegs, uint64_t total_pairs, const AoParams& P,
                            unsigned long long* queue, unsigned long long* rays);
 
+// AO with lane refill (one launch over `total_lumels` lumels; counts written, not accumulated).
+hipError_t launch_ao_refill(hipStream_t st, const SceneDev& S, int cap, uint32_t max_blocks,
+                            const BakeSeg* segs, int nsegs, uint64_t total_lumels, const AoParams& P,
+                            uint32_t chunk_lumels, uint32_t refill_min, unsigned long long* queue,
+                            unsigned long long* rays);
+
 hipError_t launch_ao_finalize(hipStream_t st, const BakeSeg* segs, int nsegs, uint64_t total,
                               const AoParams& P);
 

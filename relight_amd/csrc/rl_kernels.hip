@@ -178,6 +178,119 @@ __global__ __launch_bounds__(kBlock) void k_ao_count(SceneDev S, const BakeSeg* 
     if (lane == 0 && rays) atomicAdd(rays_out, (unsigned long long)rays);
 }
 
+// AO pass 1 with lane refill: each wavefront takes chunks of `chunk_lumels` lumels from the queue
+// and keeps a pool of their (lumel, sample) rays; whenever at least `refill_min` lanes have
+// finished their ray, those lanes take the next rays of the pool (ballot + prefix count), so a
+// wavefront no longer waits for its slowest ray. Occlusion counts per lumel accumulate in an LDS
+// ring of kRing slots per wavefront (lumels in flight are consecutive in the wavefront's stream
+// and at most 64/S + 2 of them); the lane that completes a lumel's last ray writes its count.
+constexpr int kRing = 32;
+
+template <int CAP>
+__global__ __launch_bounds__(kBlock) void k_ao_refill(SceneDev S, const BakeSeg* __restrict__ segs,
+                                                      int nsegs, uint64_t total_lumels, AoParams P,
+                                                      uint32_t chunk_lumels, uint32_t refill_min,
+                                                      unsigned long long* queue, unsigned long long* rays_out) {
+    __shared__ uint32_t stack[kLdsStack * kBlock];
+    __shared__ uint32_t ring_occ[kBlock / 64][kRing];
+    __shared__ uint32_t ring_done[kBlock / 64][kRing];
+    uint32_t* stk = stack + threadIdx.x;
+    uint32_t spill[CAP - kLdsStack];
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    for (uint32_t k = lane; k < (uint32_t)kRing; k += 64) {
+        ring_occ[wave][k] = 0u;
+        ring_done[wave][k] = 0u;
+    }
+    const uint32_t ns = (uint32_t)P.samples;
+    // Pool state, identical in every lane of the wavefront.
+    uint64_t chunk_first = 0;
+    uint32_t chunk_count = 0, pool_next = 0, pool_end = 0, stream_base = 0;
+    bool pool_open = true;
+    // Lane state.
+    bool active = false;
+    TravRay tr;
+    uint32_t node = 0, slot = 0, my_stream = 0;
+    int sp = 0;
+    uint32_t* counter = nullptr;
+    uint32_t rays = 0;
+    for (;;) {
+        uint64_t idle = __ballot(!active);
+        if (pool_open && ((uint32_t)__popcll(idle) >= refill_min || idle == ~0ull)) {
+            // Rays may only start lumels fewer than kRing - 2 places after the oldest lumel still
+            // in flight, so a ring slot is never reused before its lumel completes.
+            uint32_t oldest = active ? my_stream : 0xFFFFFFFFu;
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) oldest = min(oldest, (uint32_t)__shfl_xor((int)oldest, off, 64));
+            const uint64_t allow = oldest == 0xFFFFFFFFu ? ~0ull : (uint64_t)oldest + (uint64_t)(kRing - 2);
+            uint64_t want = idle;
+            while (want && pool_open) {
+                if (pool_next == pool_end) {
+                    const uint64_t c = wave_next_chunk(queue);
+                    const uint64_t first = c * (uint64_t)chunk_lumels;
+                    if (first >= total_lumels) {
+                        pool_open = false;
+                        break;
+                    }
+                    stream_base += chunk_count;
+                    chunk_first = first;
+                    const uint64_t left = total_lumels - first;
+                    chunk_count = (uint32_t)(left < chunk_lumels ? left : chunk_lumels);
+                    pool_next = 0;
+                    pool_end = chunk_count * ns;
+                }
+                uint64_t room = pool_end - pool_next;
+                if (allow != ~0ull) {
+                    const uint64_t lim = allow > stream_base ? (allow - stream_base) * (uint64_t)ns : 0ull;
+                    room = lim > pool_next ? min(room, lim - pool_next) : 0ull;
+                }
+                const uint32_t take = (uint32_t)min(room, (uint64_t)__popcll(want));
+                if (take == 0) break;
+                const uint32_t rank = rank_below(want);
+                const bool mine = ((want >> lane) & 1ull) && rank < take;
+                if (mine) {
+                    const uint32_t rr = pool_next + rank;
+                    const uint32_t lc = rr / ns, si = rr % ns;
+                    const uint64_t g = chunk_first + lc;
+                    const BakeSeg& sg = segs[find_seg(segs, nsegs, g)];
+                    const uint32_t li = (uint32_t)(g - sg.work_begin);
+                    const float4 p4 = sg.pos[li], n4 = sg.nrm[li];
+                    const V3 p = v3(p4.x, p4.y, p4.z), n = v3(n4.x, n4.y, n4.z);
+                    Pcg32 rng = sample_stream(P.seed, sg.lm_seed, f_bits(p4.w), TECH_AO, si);
+                    const V3 dir = sample_hemisphere(rng, n, P.sampling);
+                    tr = trav_ray(ray_from_segment(p, p + dir * P.max_distance));
+                    node = 0;
+                    sp = 0;
+                    my_stream = stream_base + lc;
+                    slot = my_stream % (uint32_t)kRing;
+                    counter = sg.counts + li;
+                    active = true;
+                }
+                pool_next += take;
+                want &= ~__ballot(mine);
+            }
+            if (!pool_open && __ballot(active) == 0ull) break;
+        } else if (!pool_open && idle == ~0ull) {
+            break;
+        }
+        if (active) {
+            const int st = any_hit_step<CAP>(S, tr, node, sp, stk, kBlock, spill);
+            if (st != 0) {
+                rays++;
+                if (st > 0) atomicAdd(&ring_occ[wave][slot], 1u);
+                const uint32_t done = atomicAdd(&ring_done[wave][slot], 1u);
+                if (done == ns - 1) {
+                    // Last ray of the lumel: every earlier increment of this slot precedes this
+                    // one in the wavefront's LDS order; read, publish, clear.
+                    *counter = atomicExch(&ring_occ[wave][slot], 0u);
+                    atomicExch(&ring_done[wave][slot], 0u);
+                }
+                active = false;
+            }
+        }
+    }
+    count_rays(rays, rays_out);
+}
+
 // AO pass 2: AmbientOcclusion::bakeLumel's value and multiply (baker/AmbientOcclusion.cpp:59-65).
 __global__ __launch_bounds__(kBlock) void k_ao_finalize(const BakeSeg* __restrict__ segs, int nsegs,
                                                         uint64_t total, AoParams P) {
@@ -286,6 +399,22 @@ hipError_t launch_ao_count(hipStream_t st, const SceneDev& S, int cap, uint32_t 
     const uint32_t grid = persistent_blocks(total_pairs, max_blocks);
     RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_ao_count<CAP>, dim3(grid), dim3(kBlock), 0, st, S, segs, nsegs,
                                             total_pairs, P, queue, rays));
+    return hipGetLastError();
+}
+
+hipError_t launch_ao_refill(hipStream_t st, const SceneDev& S, int cap, uint32_t max_blocks,
+                            const BakeSeg* segs, int nsegs, uint64_t total_lumels, const AoParams& P,
+                            uint32_t chunk_lumels, uint32_t refill_min, unsigned long long* queue,
+                            unsigned long long* rays) {
+    if (total_lumels == 0) return hipSuccess;
+    if (P.samples < 4 || chunk_lumels == 0 || refill_min == 0 || refill_min > 64) return hipErrorInvalidValue;
+    hipError_t e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(queue), 2, 0u);
+    if (e != hipSuccess) return e;
+    const uint64_t need = (total_lumels + chunk_lumels - 1) / chunk_lumels;  // one wavefront per chunk
+    const uint64_t blocks = (need + 1) / 2;
+    const uint32_t grid = (uint32_t)(blocks < max_blocks ? (blocks ? blocks : 1) : max_blocks);
+    RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_ao_refill<CAP>, dim3(grid), dim3(kBlock), 0, st, S, segs, nsegs,
+                                            total_lumels, P, chunk_lumels, refill_min, queue, rays));
     return hipGetLastError();
 }
 

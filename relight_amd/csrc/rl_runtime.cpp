@@ -80,7 +80,9 @@ struct rl_relight {
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     unsigned long long* d_rays = nullptr;
     unsigned long long* d_queue = nullptr;  // persistent-kernel work counter
-    uint32_t max_blocks = 4096;             // persistent grid: 16 blocks per compute unit
+    uint32_t max_blocks = 4096;             // persistent grid: blocks_per_cu blocks per compute unit
+    int cus = 256;
+    rl_tuning tuning{RL_AO_KERNEL_REFILL, 16u, 8u, 16u};
     rl_stats stats{};
     uint32_t next_lm_seed = 0;
     // Event pairs around each kernel launch of the current call: (slot, start, end).
@@ -412,9 +414,13 @@ int32_t run_ao(rl_relight* ctx, rl_scene* scene, rl_lightmap* const* maps, int n
         nsegs += maps[i]->texels.empty() ? 0 : 1;
         if (e == hipSuccess) e = launch_fill_u32(ctx->stream, maps[i]->d_counts, maps[i]->texels.size(), 0u);
     }
+    const bool refill = ctx->tuning.ao_kernel == RL_AO_KERNEL_REFILL && s->samples >= 4;
     if (pairs) {
         if (e == hipSuccess) e = kernel_begin(ctx, RL_KERNEL_AO_COUNT);
-        if (e == hipSuccess)
+        if (e == hipSuccess && refill)
+            e = launch_ao_refill(ctx->stream, scene->dev(), scene->cap, ctx->max_blocks, d_lum, nsegs, lumels, P,
+                                 ctx->tuning.ao_chunk_lumels, ctx->tuning.refill_min, ctx->d_queue, ctx->d_rays);
+        if (e == hipSuccess && !refill)
             e = launch_ao_count(ctx->stream, scene->dev(), scene->cap, ctx->max_blocks, d_pairs, nsegs, pairs, P,
                                 ctx->d_queue, ctx->d_rays);
         if (e == hipSuccess) e = kernel_end(ctx);
@@ -538,7 +544,8 @@ int32_t rl_create(rl_relight** out) {
     }
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) == hipSuccess && cus > 0) {
-        ctx->max_blocks = (uint32_t)cus * 16u;
+        ctx->cus = cus;
+        ctx->max_blocks = (uint32_t)cus * ctx->tuning.blocks_per_cu;
     }
     *out = ctx;
     return RL_SUCCESS;
@@ -1322,6 +1329,24 @@ int32_t rl_radiosity_solve(rl_relight* ctx, rl_radiosity* r, const float* emissi
     }
     if (st) return st;
     if (rel > (double)tolerance) return fail(RL_NOT_CONVERGED, "radiosity solve did not reach the tolerance");
+    return RL_SUCCESS;
+}
+
+int32_t rl_set_tuning(rl_relight* ctx, const rl_tuning* t) {
+    if (!ctx || !t) return fail(RL_INVALID_ARGUMENT, "NULL argument");
+    if (t->ao_kernel != RL_AO_KERNEL_CHUNKED && t->ao_kernel != RL_AO_KERNEL_REFILL)
+        return fail(RL_INVALID_ARGUMENT, "unknown AO kernel");
+    if (t->refill_min < 1 || t->refill_min > 64) return fail(RL_INVALID_ARGUMENT, "refill_min must be in [1, 64]");
+    if (t->ao_chunk_lumels < 1 || t->ao_chunk_lumels > 4096) return fail(RL_INVALID_ARGUMENT, "bad chunk size");
+    if (t->blocks_per_cu < 1 || t->blocks_per_cu > 64) return fail(RL_INVALID_ARGUMENT, "blocks_per_cu must be in [1, 64]");
+    ctx->tuning = *t;
+    ctx->max_blocks = (uint32_t)ctx->cus * t->blocks_per_cu;
+    return RL_SUCCESS;
+}
+
+int32_t rl_get_tuning(const rl_relight* ctx, rl_tuning* out) {
+    if (!ctx || !out) return fail(RL_INVALID_ARGUMENT, "NULL argument");
+    *out = ctx->tuning;
     return RL_SUCCESS;
 }
 

@@ -39,6 +39,95 @@ __device__ __forceinline__ void cswap(float& ka, uint32_t& va, float& kb, uint32
 // per-lane array. Blocks allocate kLdsStack words of LDS per thread.
 constexpr int kLdsStack = 16;
 
+// Number of set bits of `mask` below this lane (a wavefront-local prefix count).
+__device__ __forceinline__ uint32_t rank_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Per-lane constants of a ray for the step-wise traversal below.
+struct TravRay {
+    WtRay w;
+    float ix, iy, iz, oix, oiy, oiz;
+};
+
+__device__ __forceinline__ TravRay trav_ray(const Ray& ray) {
+    TravRay r;
+    r.w = wt_setup(ray);
+    const float dx = fabsf(ray.d.x) > 1e-20f ? ray.d.x : copysignf(1e-20f, ray.d.x);
+    const float dy = fabsf(ray.d.y) > 1e-20f ? ray.d.y : copysignf(1e-20f, ray.d.y);
+    const float dz = fabsf(ray.d.z) > 1e-20f ? ray.d.z : copysignf(1e-20f, ray.d.z);
+    r.ix = 1.0f / dx; r.iy = 1.0f / dy; r.iz = 1.0f / dz;
+    r.oix = -(ray.o.x * r.ix); r.oiy = -(ray.o.y * r.iy); r.oiz = -(ray.o.z * r.iz);
+    return r;
+}
+
+__device__ __forceinline__ void stack_push(uint32_t v, int& sp, uint32_t* stk, uint32_t stride, uint32_t* spill) {
+    if (sp < kLdsStack) stk[sp * stride] = v;
+    else spill[sp - kLdsStack] = v;
+    sp++;
+}
+
+__device__ __forceinline__ uint32_t stack_pop(int& sp, const uint32_t* stk, uint32_t stride, const uint32_t* spill) {
+    --sp;
+    return sp < kLdsStack ? stk[sp * stride] : spill[sp - kLdsStack];
+}
+
+// One step of any-hit traversal: either one internal node (its four child boxes tested, hit
+// children pushed far-first, the nearest taken next) or one leaf (its triangles tested).
+// Returns 1 when the ray is occluded, -1 when traversal ended without a hit, 0 to continue.
+// Same box and triangle tests as traverse(), so results equal traverse<true>().
+template <int CAP>
+__device__ __forceinline__ int any_hit_step(const SceneDev& S, const TravRay& r, uint32_t& node, int& sp,
+                                            uint32_t* stk, uint32_t stride, uint32_t* spill) {
+    if (!(node & RL_LEAF_BIT)) {
+        const float4* nd = reinterpret_cast<const float4*>(S.nodes + node);
+        const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
+        const uint4 ch = reinterpret_cast<const uint4*>(nd)[6];
+        const float lim = r.w.tfar * kBoxScale;
+        float key[4];
+        uint32_t cid[4] = {ch.x, ch.y, ch.z, ch.w};
+        const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
+        const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
+        const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            float t0x = fmaf(lxa[i], r.ix, r.oix), t1x = fmaf(hxa[i], r.ix, r.oix);
+            float t0y = fmaf(lya[i], r.iy, r.oiy), t1y = fmaf(hya[i], r.iy, r.oiy);
+            float t0z = fmaf(lza[i], r.iz, r.oiz), t1z = fmaf(hza[i], r.iz, r.oiz);
+            float tmin = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), r.w.tnear));
+            float tmax = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * kBoxScale;
+            bool hit = tmin <= fminf(tmax, lim) && cid[i] != RL_EMPTY_CHILD;
+            key[i] = hit ? tmin : __builtin_inff();
+        }
+        cswap(key[0], cid[0], key[1], cid[1]);
+        cswap(key[2], cid[2], key[3], cid[3]);
+        cswap(key[0], cid[0], key[2], cid[2]);
+        cswap(key[1], cid[1], key[3], cid[3]);
+        cswap(key[1], cid[1], key[2], cid[2]);
+        if (key[0] == __builtin_inff()) {
+            if (sp == 0) return -1;
+            node = stack_pop(sp, stk, stride, spill);
+            return 0;
+        }
+#pragma unroll
+        for (int i = 3; i >= 1; i--)
+            if (key[i] != __builtin_inff() && sp < CAP) stack_push(cid[i], sp, stk, stride, spill);
+        node = cid[0];
+        return 0;
+    }
+    const uint32_t first = node & RL_LEAF_FIRST_MASK;
+    const uint32_t count = ((node >> 27) & 15u) + 1u;
+    for (uint32_t j = 0; j < count; j++) {
+        const float4* tp = S.tris + 3u * (first + j);
+        const float4 a = tp[0], b = tp[1], c = tp[2];
+        float t, U, V, W, det;
+        if (wt_intersect(r.w, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t, U, V, W, det)) return 1;
+    }
+    if (sp == 0) return -1;
+    node = stack_pop(sp, stk, stride, spill);
+    return 0;
+}
+
 // Traverses the BVH4 for one ray. ANY: stop at the first triangle with t in (tnear, tfar).
 // Closest: the hit with the smallest t, ties broken by the smaller scene-order triangle id, so
 // the result does not depend on traversal order. stk points at this lane's first LDS stack slot;

@@ -431,6 +431,19 @@ class Relight:
     def create_radiosity(self, scene: Scene, patches, tri_patch) -> Radiosity:
         return Radiosity(self, scene, patches, tri_patch)
 
+    def tuning(self) -> dict:
+        t = A.rl_tuning()
+        _check(self._lib, self._lib.rl_get_tuning(self._h, C.byref(t)))
+        return {name: getattr(t, name) for name, _ in A.rl_tuning._fields_}
+
+    def set_tuning(self, **kw):
+        """Launch tuning (ao_kernel, refill_min, ao_chunk_lumels, blocks_per_cu); results do not
+        depend on it."""
+        cur = self.tuning()
+        cur.update(kw)
+        t = A.rl_tuning(*[int(cur[name]) for name, _ in A.rl_tuning._fields_])
+        _check(self._lib, self._lib.rl_set_tuning(self._h, C.byref(t)))
+
     def emit_photons(self, scene: Scene, settings: IndirectLightSettings):
         s = settings.to_c()
         _check(self._lib, self._lib.rl_emit_photons(self._h, scene._h, C.byref(s)))

@@ -241,3 +241,27 @@ def test_axis_parallel_segments_agree(relight):
     assert np.array_equal(scene.trace_segments(a, b), S.trace(a, b)["flags"])
     h = scene.trace_segments(a, b, RL_TRACE_CLOSEST)
     assert np.array_equal(h["triangle"], S.trace(a, b, closest=True)["triangle"])
+
+
+@pytest.mark.parametrize("samples", [4, 5, 48, 100, 256])
+def test_ao_refill_equals_chunked_and_oracle(relight, samples):
+    """The lane-refill AO kernel against the chunked kernel and the oracle, for sample counts
+    below, between and above multiples of 64, under several refill thresholds and chunk sizes."""
+    print(f"[gpu] AO refill vs chunked, samples={samples}", flush=True)
+    from relight_amd import RL_SAMPLE_COSINE
+    spec = scenes.city(num_boxes=300, tiles=(2, 2), side_range=(48, 96))
+    ao = AmbientOcclusionSettings(samples, 0.8, 4.0, 1.0, 3, RL_SAMPLE_COSINE)
+    outs = []
+    for kernel, rmin, chunk in [(0, 16, 8), (1, 1, 1), (1, 16, 8), (1, 64, 3), (1, 8, 64)]:
+        relight.set_tuning(ao_kernel=kernel, refill_min=rmin, ao_chunk_lumels=chunk)
+        scene, lms, ids = scenes.build_gpu(relight, spec)
+        relight.bake_batch(scene, lms, direct=True, ao=ao)
+        assert relight.stats()["kernels"]["ao_count"]["launches"] == 1
+        outs.append([lm.to_hdr() for lm in lms])
+    relight.set_tuning(ao_kernel=1, refill_min=16, ao_chunk_lumels=8)
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert np.array_equal(a, b)
+    for li in (0, 3):
+        texel, _, _, col, _ = oracle_bake(spec, li, True, ao)
+        assert np.array_equal(outs[2][li].reshape(-1, 3)[texel], col)
