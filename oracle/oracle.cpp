@@ -321,16 +321,24 @@ void build_bvh(Scene& S) {
     }
 }
 
-// Robust slab test (Ize 2013): exit distances scaled by 1 + 2*gamma(3).
+// Robust slab test (Ize 2013): exit distances scaled by 1 + 2*gamma(3). An axis the ray does not
+// move along constrains nothing when the origin lies in the slab (ends included) and rejects the
+// box otherwise; (b - o) * (1/0) would give NaN or a one-sided infinity there.
 inline bool box_hit(const Seg& s, const float* lo, const float* hi, float tlim, float& tenter) {
     const float K = 1.0000004f;
-    float t0x = (lo[0] - s.o.x) * s.inv.x, t1x = (hi[0] - s.o.x) * s.inv.x;
-    float t0y = (lo[1] - s.o.y) * s.inv.y, t1y = (hi[1] - s.o.y) * s.inv.y;
-    float t0z = (lo[2] - s.o.z) * s.inv.z, t1z = (hi[2] - s.o.z) * s.inv.z;
-    float tmin = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), s.t0));
-    float tmax = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * K;
+    const float o[3] = {s.o.x, s.o.y, s.o.z}, d[3] = {s.d.x, s.d.y, s.d.z}, inv[3] = {s.inv.x, s.inv.y, s.inv.z};
+    float tmin = s.t0, tmax = INFINITY;
+    for (int k = 0; k < 3; k++) {
+        if (fabsf(d[k]) < 1e-30f) {
+            if (o[k] < lo[k] || o[k] > hi[k]) return false;
+            continue;
+        }
+        const float t0 = (lo[k] - o[k]) * inv[k], t1 = (hi[k] - o[k]) * inv[k];
+        tmin = fmaxf(tmin, fminf(t0, t1));
+        tmax = fminf(tmax, fmaxf(t0, t1));
+    }
     tenter = tmin;
-    return tmin <= fminf(tmax, tlim * K);
+    return tmin <= fminf(tmax * K, tlim * K);
 }
 
 struct HitRec {

@@ -155,6 +155,18 @@ def test_bvh_equals_brute_force(which):
     spec = scenes.box_room() if which == "box_room" else scenes.city(num_boxes=300, tiles=(2, 2),
                                                                     side_range=(64, 128))
     a, b = _segments(spec, 20000, 11)
+    # Half of them axis-parallel (one or two zero direction components) from mesh vertices, which
+    # lie on box planes: the slab test's 1/0 and on-plane cases.
+    rng = np.random.default_rng(12)
+    allp = np.concatenate([m.positions for m in spec.meshes])
+    k = 10000
+    a[:k] = allp[rng.integers(0, allp.shape[0], k)]
+    b[:k] = a[:k]
+    ax = rng.integers(0, 3, k)
+    stp = rng.uniform(-8, 8, k).astype(np.float32)
+    b[np.arange(k), ax] += stp
+    two = np.nonzero(rng.random(k) < 0.3)[0]
+    b[two, (ax[two] + 1) % 3] += stp[two]
     bvh = O.OracleScene.from_spec(spec, use_bvh=True)
     bf = O.OracleScene.from_spec(spec, use_bvh=False)
     ra, rb = bvh.trace(a, b), bf.trace(a, b)
