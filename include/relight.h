@@ -262,6 +262,90 @@ int32_t rl_radiosity_solve(rl_relight* ctx, rl_radiosity* r, const float* emissi
                            int32_t max_outer, int32_t inner_sweeps, float* out_radiosity,
                            rl_solve_info* info);
 
+/* Multi-scene batched bake (the FullBakeJob fan-out of Worker.cpp:43-83 across scenes): one
+ * direct-light launch over every item with direct != 0, then one AO launch over every item with
+ * ao != NULL. An item is lumels [first_lumel, first_lumel + lumel_count) of a lightmap in valid-
+ * texel order (lumel_count = 0xFFFFFFFF: to the end). Results equal per-lightmap rl_bake_direct
+ * and rl_bake_ambient_occlusion calls. Items must not overlap. */
+typedef struct rl_bake_item {
+    rl_scene* scene;
+    rl_lightmap* lightmap;
+    uint32_t first_lumel;
+    uint32_t lumel_count;
+    int32_t direct;
+    const rl_ao_settings* ao;
+} rl_bake_item;
+int32_t rl_bake_items(rl_relight* ctx, const rl_bake_item* items, int32_t num_items);
+
+/* Bake server (no counterpart in the reference; replaces the Worker/ThreadWorker job loop of
+ * Worker.cpp and src/demo/Lightmapping.cpp:619-649 for serving). A background thread owns the
+ * context's stream; submitted jobs wait in a FIFO queue and are admitted while the sum of admitted
+ * jobs' device footprints stays within memory_budget and at most max_active run. Every step the
+ * loop takes up to quantum_rays rays of direct/AO work from all admitted jobs, fairly shared, into
+ * one multi-scene launch per pass (continuous batching: a job submitted while others run joins at
+ * the next step); photon, final-gather and radiosity phases run as whole steps. Scenes, lightmaps
+ * and radiosity objects are borrowed until the job finishes; while a server runs, make no other
+ * call on its context. Results are identical to the same job run alone. */
+#define RL_JOB_DIRECT_AO 0
+#define RL_JOB_INDIRECT 1
+#define RL_JOB_RADIOSITY 2
+#define RL_JOB_QUEUED 0
+#define RL_JOB_RUNNING 1
+#define RL_JOB_DONE 2
+#define RL_JOB_FAILED 3
+#define RL_JOB_CANCELLED 4
+typedef struct rl_server rl_server;
+typedef struct rl_server_config {
+    uint64_t memory_budget;  /* bytes of per-job device memory admitted at once */
+    int32_t max_active;      /* jobs admitted at once */
+    uint64_t quantum_rays;   /* rays of direct/AO work per step */
+} rl_server_config;
+typedef struct rl_job_desc {
+    int32_t kind;
+    rl_scene* scene;
+    rl_lightmap* const* lightmaps;   /* DIRECT_AO and INDIRECT: lightmaps to bake */
+    int32_t num_lightmaps;
+    int32_t direct;                  /* DIRECT_AO: bake direct light first */
+    const rl_ao_settings* ao;        /* DIRECT_AO: AO after direct (NULL: none) */
+    const rl_indirect_settings* indirect; /* INDIRECT: photons, then final gather per lightmap */
+    rl_radiosity* radiosity;         /* RADIOSITY: form factors then solve */
+    uint32_t rays_per_patch;
+    uint32_t ff_seed;
+    float ff_max_distance;
+    const float* emission;           /* RADIOSITY: num_patches x num_rhs (copied at submit) */
+    const float* reflectance;
+    uint32_t num_rhs;
+    float tolerance;
+    float* out_radiosity;            /* RADIOSITY: written when the job is done */
+} rl_job_desc;
+typedef struct rl_job_status {
+    int32_t state;       /* RL_JOB_* */
+    int32_t status;      /* RL_SUCCESS or the failing call's code */
+    double submit_ms;    /* times since the server started */
+    double admit_ms;
+    double done_ms;
+    uint64_t rays;
+    uint64_t footprint;  /* admitted device bytes */
+} rl_job_status;
+typedef struct rl_server_stats {
+    uint64_t submitted, completed, failed, cancelled, rejected;
+    uint64_t steps;
+    uint64_t rays;
+    double busy_ms;      /* wall time spent in steps */
+    uint64_t peak_memory;
+} rl_server_stats;
+int32_t rl_server_create(rl_relight* ctx, const rl_server_config* config, rl_server** out);
+/* RL_REJECTED if the job's footprint exceeds the budget or the description is invalid. */
+int32_t rl_server_submit(rl_server* server, const rl_job_desc* job, uint64_t* out_id);
+int32_t rl_server_poll(rl_server* server, uint64_t id, rl_job_status* out);
+/* Blocks until the job has finished (timeout_ms < 0: no limit); RL_REJECTED on timeout. */
+int32_t rl_server_wait(rl_server* server, uint64_t id, double timeout_ms, rl_job_status* out);
+/* Cancels a queued job (running jobs finish their current step and stop). */
+int32_t rl_server_cancel(rl_server* server, uint64_t id);
+int32_t rl_server_stats_get(rl_server* server, rl_server_stats* out);
+/* Stops the loop after cancelling queued jobs and finishing running ones. */
+int32_t rl_server_destroy(rl_server* server);
+
 /* Launch tuning of a context (no counterpart in the reference; chosen by the autotuner in
  * relight_amd/autotune.py). ao_kernel: RL_AO_KERNEL_REFILL (lanes take new rays from a per-
  * wavefront pool when at least refill_min of them are idle; chunks of ao_chunk_lumels lumels) or

@@ -7,12 +7,12 @@ seeded scene generators the tests and the benchmark use.
 from ._capi import (RL_LIGHT_AREA, RL_LIGHT_DIRECTIONAL, RL_LIGHT_POINT, RL_LIGHT_SPOT,
                     RL_PHOTON_RUSSIAN_ROULETTE, RL_PHOTON_THRESHOLD, RL_SAMPLE_COSINE,
                     RL_SAMPLE_UNIFORM, RL_TRACE_ANY, RL_TRACE_CLOSEST)
-from .relight import (AmbientOcclusionSettings, IndirectLightSettings, LightDesc, Lightmap,
-                      Photonmap, Relight, RelightError, Scene)
+from .relight import (AmbientOcclusionSettings, BakeJob, IndirectLightSettings, LightDesc,
+                      Lightmap, Photonmap, Radiosity, Relight, RelightError, Scene, Server)
 
 __all__ = [
     "Relight", "Scene", "Lightmap", "Photonmap", "LightDesc", "AmbientOcclusionSettings",
-    "IndirectLightSettings", "RelightError",
+    "IndirectLightSettings", "RelightError", "Radiosity", "Server", "BakeJob",
     "RL_LIGHT_POINT", "RL_LIGHT_SPOT", "RL_LIGHT_DIRECTIONAL", "RL_LIGHT_AREA",
     "RL_SAMPLE_UNIFORM", "RL_SAMPLE_COSINE", "RL_TRACE_ANY", "RL_TRACE_CLOSEST",
     "RL_PHOTON_THRESHOLD", "RL_PHOTON_RUSSIAN_ROULETTE",

@@ -145,6 +145,60 @@ class rl_stats(C.Structure):
     ]
 
 
+RL_JOB_DIRECT_AO = 0
+RL_JOB_INDIRECT = 1
+RL_JOB_RADIOSITY = 2
+RL_JOB_QUEUED = 0
+RL_JOB_RUNNING = 1
+RL_JOB_DONE = 2
+RL_JOB_FAILED = 3
+RL_JOB_CANCELLED = 4
+JOB_STATE_NAMES = {0: "queued", 1: "running", 2: "done", 3: "failed", 4: "cancelled"}
+
+
+class rl_bake_item(C.Structure):
+    _fields_ = [("scene", C.c_void_p), ("lightmap", C.c_void_p), ("first_lumel", C.c_uint32),
+                ("lumel_count", C.c_uint32), ("direct", C.c_int32),
+                ("ao", C.POINTER(rl_ao_settings))]
+
+
+class rl_server_config(C.Structure):
+    _fields_ = [("memory_budget", C.c_uint64), ("max_active", C.c_int32), ("quantum_rays", C.c_uint64)]
+
+
+class rl_job_desc(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32),
+        ("scene", C.c_void_p),
+        ("lightmaps", C.POINTER(C.c_void_p)),
+        ("num_lightmaps", C.c_int32),
+        ("direct", C.c_int32),
+        ("ao", C.POINTER(rl_ao_settings)),
+        ("indirect", C.POINTER(rl_indirect_settings)),
+        ("radiosity", C.c_void_p),
+        ("rays_per_patch", C.c_uint32),
+        ("ff_seed", C.c_uint32),
+        ("ff_max_distance", C.c_float),
+        ("emission", C.POINTER(C.c_float)),
+        ("reflectance", C.POINTER(C.c_float)),
+        ("num_rhs", C.c_uint32),
+        ("tolerance", C.c_float),
+        ("out_radiosity", C.POINTER(C.c_float)),
+    ]
+
+
+class rl_job_status(C.Structure):
+    _fields_ = [("state", C.c_int32), ("status", C.c_int32), ("submit_ms", C.c_double),
+                ("admit_ms", C.c_double), ("done_ms", C.c_double), ("rays", C.c_uint64),
+                ("footprint", C.c_uint64)]
+
+
+class rl_server_stats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("cancelled", C.c_uint64), ("rejected", C.c_uint64), ("steps", C.c_uint64),
+                ("rays", C.c_uint64), ("busy_ms", C.c_double), ("peak_memory", C.c_uint64)]
+
+
 _P = C.c_void_p
 _PP = C.POINTER(C.c_void_p)
 _F = C.POINTER(C.c_float)
@@ -190,6 +244,14 @@ SIGNATURES = [
     ("rl_set_tuning", C.c_int32, [_P, C.POINTER(rl_tuning)]),
     ("rl_get_tuning", C.c_int32, [_P, C.POINTER(rl_tuning)]),
     ("rl_get_stats", C.c_int32, [_P, C.POINTER(rl_stats)]),
+    ("rl_bake_items", C.c_int32, [_P, C.POINTER(rl_bake_item), C.c_int32]),
+    ("rl_server_create", C.c_int32, [_P, C.POINTER(rl_server_config), _PP]),
+    ("rl_server_submit", C.c_int32, [_P, C.POINTER(rl_job_desc), C.POINTER(C.c_uint64)]),
+    ("rl_server_poll", C.c_int32, [_P, C.c_uint64, C.POINTER(rl_job_status)]),
+    ("rl_server_wait", C.c_int32, [_P, C.c_uint64, C.c_double, C.POINTER(rl_job_status)]),
+    ("rl_server_cancel", C.c_int32, [_P, C.c_uint64]),
+    ("rl_server_stats_get", C.c_int32, [_P, C.POINTER(rl_server_stats)]),
+    ("rl_server_destroy", C.c_int32, [_P]),
 ]
 
 _lib = None

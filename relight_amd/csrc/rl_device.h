@@ -65,17 +65,6 @@ struct LightDev {
     float cutoff, spot_exponent;
 };
 
-// One lightmap inside a bake launch.
-struct BakeSeg {
-    const float4* pos;
-    const float4* nrm;
-    float4* color;
-    uint32_t* counts;     // per-lumel scratch counters (AO occluded count)
-    uint32_t count;       // valid lumels
-    uint32_t lm_seed;
-    uint64_t work_begin;  // first global work item of this segment
-};
-
 struct AoParams {
     int32_t samples;
     float occluded_fraction;
@@ -83,6 +72,23 @@ struct AoParams {
     float exponent;
     uint32_t seed;
     int32_t sampling;
+};
+
+// One lightmap (or a lumel range of one) inside a bake launch, with the scene, lights and AO
+// settings it is baked with, so one launch can cover lightmaps of different scenes and jobs.
+struct BakeSeg {
+    const float4* pos;
+    const float4* nrm;
+    float4* color;
+    uint32_t* counts;     // per-lumel AO occluded counts
+    uint32_t count;       // lumels in this segment
+    uint32_t lm_seed;
+    uint64_t work_begin;  // first global work item (or chunk, for chunked launches) of this segment
+    SceneDev scene;
+    const LightDev* lights;
+    const float4* lverts;
+    int32_t nlights;
+    AoParams ao;
 };
 
 // A photonmap (Lightmap.h:153-170) as full w*h grids: accumulated photon power (rgb), photon

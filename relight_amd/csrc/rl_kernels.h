@@ -16,23 +16,24 @@ int stack_capacity_for(int bound);
 
 hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v);
 
-hipError_t launch_bake_direct(hipStream_t st, const SceneDev& S, int cap, uint32_t max_blocks,
-                              const BakeSeg* segs, int nsegs, uint64_t total, const LightDev* lights,
-                              int nlights, const float4* lverts, unsigned long long* queue,
+// Segments carry their scene, lights and AO settings; `cap` is the largest traversal-stack
+// capacity any of their scenes needs. work_begin units: lumels (direct, finalize), (lumel,
+// sample) pairs (AO chunked) or chunks of chunk_lumels lumels (AO refill).
+hipError_t launch_bake_direct(hipStream_t st, int cap, uint32_t max_blocks, const BakeSeg* segs,
+                              int nsegs, uint64_t total, unsigned long long* queue,
                               unsigned long long* rays);
 
-hipError_t launch_ao_count(hipStream_t st, const SceneDev& S, int cap, uint32_t max_blocks,
-                           const BakeSeg* segs, int nsegs, uint64_t total_pairs, const AoParams& P,
-                           unsigned long long* queue, unsigned long long* rays);
+hipError_t launch_ao_count(hipStream_t st, int cap, uint32_t max_blocks, const BakeSeg* segs,
+                           int nsegs, uint64_t total_pairs, unsigned long long* queue,
+                           unsigned long long* rays);
 
-// AO with lane refill (one launch over `total_lumels` lumels; counts written, not accumulated).
-hipError_t launch_ao_refill(hipStream_t st, const SceneDev& S, int cap, uint32_t max_blocks,
-                            const BakeSeg* segs, int nsegs, uint64_t total_lumels, const AoParams& P,
-                            uint32_t chunk_lumels, uint32_t refill_min, unsigned long long* queue,
+// AO with lane refill (counts written, not accumulated).
+hipError_t launch_ao_refill(hipStream_t st, int cap, uint32_t max_blocks, const BakeSeg* segs,
+                            int nsegs, uint64_t total_chunks, uint32_t chunk_lumels,
+                            uint32_t refill_min, unsigned long long* queue,
                             unsigned long long* rays);
 
-hipError_t launch_ao_finalize(hipStream_t st, const BakeSeg* segs, int nsegs, uint64_t total,
-                              const AoParams& P);
+hipError_t launch_ao_finalize(hipStream_t st, const BakeSeg* segs, int nsegs, uint64_t total);
 
 hipError_t launch_trace(hipStream_t st, const SceneDev& S, int cap, uint32_t max_blocks, bool any,
                         const float4* starts, const float4* ends, uint32_t n, uint8_t* flags,

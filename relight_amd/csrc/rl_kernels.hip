@@ -2,10 +2,11 @@
 // DirectLight::bakeLumel (baker/DirectLight.cpp:46-125), AmbientOcclusion::bakeLumel
 // (baker/AmbientOcclusion.cpp:46-66) and rt::ITracer::traceSegment (rt/Tracer.h:87).
 //
-// All ray kernels are persistent: a grid sized to the machine takes 64-item chunks from an atomic
-// queue, so one launch covers any number of lightmaps of any size (a single dispatch may not
-// exceed 2^32 work-items) and short chunks refill idle wavefronts. Per-texel reductions use
-// ballot + popcount, so integer results are exact and independent of scheduling.
+// All ray kernels are persistent: a grid sized to the machine takes chunks from an atomic queue,
+// so one launch covers any number of lightmaps of any size (a single dispatch may not exceed
+// 2^32 work-items). Every segment carries its own scene, lights and AO settings, so one launch
+// also covers lightmaps of different scenes and jobs. Per-texel reductions are integer (ballot +
+// popcount or LDS counters), so results are exact and independent of scheduling.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -70,11 +71,8 @@ __device__ float influence_from_point(const SceneDev& S, const LightDev& L, V3 l
 
 // One lane per lumel; lights and light vertices in the reference's order.
 template <int CAP>
-__global__ __launch_bounds__(kBlock) void k_bake_direct(SceneDev S, const BakeSeg* __restrict__ segs,
-                                                        int nsegs, uint64_t total,
-                                                        const LightDev* __restrict__ lights, int nlights,
-                                                        const float4* __restrict__ lverts,
-                                                        unsigned long long* queue,
+__global__ __launch_bounds__(kBlock) void k_bake_direct(const BakeSeg* __restrict__ segs, int nsegs,
+                                                        uint64_t total, unsigned long long* queue,
                                                         unsigned long long* rays_out) {
     __shared__ uint32_t stack[kLdsStack * kBlock];
     uint32_t* stk = stack + threadIdx.x;
@@ -86,18 +84,19 @@ __global__ __launch_bounds__(kBlock) void k_bake_direct(SceneDev S, const BakeSe
         const uint64_t g = chunk * 64 + lane_id();
         if (g >= total) continue;
         const BakeSeg& sg = segs[find_seg(segs, nsegs, g)];
+        const SceneDev S = sg.scene;
         const uint32_t i = (uint32_t)(g - sg.work_begin);
         const float4 p4 = sg.pos[i], n4 = sg.nrm[i];
         const V3 p = v3(p4.x, p4.y, p4.z), n = v3(n4.x, n4.y, n4.z);
         float4 col = sg.color[i];
-        for (int l = 0; l < nlights; l++) {
-            const LightDev L = lights[l];
+        for (int l = 0; l < sg.nlights; l++) {
+            const LightDev L = sg.lights[l];
             float cr, cg, cb;
             if (L.num_vertices > 0) {
                 // DirectLight::lightFromPointSet (baker/DirectLight.cpp:75-98).
                 cr = cg = cb = 0.0f;
                 for (int v = 0; v < L.num_vertices; v++) {
-                    const float4 lv = lverts[L.vertex_offset + v];
+                    const float4 lv = sg.lverts[L.vertex_offset + v];
                     V3 lp = v3(lv.x + L.pos[0], lv.y + L.pos[1], lv.z + L.pos[2]);
                     float inf = influence_from_point<CAP>(S, L, lp, p, n, stk, rays);
                     if (inf > 0.0f) {
@@ -126,11 +125,11 @@ __global__ __launch_bounds__(kBlock) void k_bake_direct(SceneDev S, const BakeSe
     count_rays(rays, rays_out);
 }
 
-// AO pass 1: one lane per (lumel, sample) pair, pairs lumel-major so a lumel's rays share a
-// wavefront. Occluded rays are counted per lumel with a ballot, one integer atomic per run.
+// AO pass 1 (chunked): one lane per (lumel, sample) pair, pairs lumel-major so a lumel's rays
+// share a wavefront; segments' work_begin counts pairs. Occluded rays are counted per lumel with
+// a ballot, one integer atomic per run.
 template <int CAP>
-__global__ __launch_bounds__(kBlock) void k_ao_count(SceneDev S, const BakeSeg* __restrict__ segs,
-                                                     int nsegs, uint64_t total, AoParams P,
+__global__ __launch_bounds__(kBlock) void k_ao_count(const BakeSeg* __restrict__ segs, int nsegs, uint64_t total,
                                                      unsigned long long* queue, unsigned long long* rays_out) {
     __shared__ uint32_t stack[kLdsStack * kBlock];
     uint32_t* stk = stack + threadIdx.x;
@@ -148,6 +147,7 @@ __global__ __launch_bounds__(kBlock) void k_ao_count(SceneDev S, const BakeSeg* 
         if (active) {
             const int s = find_seg(segs, nsegs, g);
             const BakeSeg& sg = segs[s];
+            const AoParams P = sg.ao;
             const uint64_t local = g - sg.work_begin;
             const uint32_t li = (uint32_t)(local / (uint64_t)P.samples);
             const uint32_t si = (uint32_t)(local % (uint64_t)P.samples);
@@ -157,7 +157,7 @@ __global__ __launch_bounds__(kBlock) void k_ao_count(SceneDev S, const BakeSeg* 
             V3 dir = sample_hemisphere(rng, n, P.sampling);
             Ray r = ray_from_segment(p, p + dir * P.max_distance);
             TraceResult tr;
-            occ = traverse<true, CAP>(S, r, stk, kBlock, tr);
+            occ = traverse<true, CAP>(sg.scene, r, stk, kBlock, tr);
             counter = sg.counts + li;
             key = ((uint64_t)s << 32) | li;
         }
@@ -178,19 +178,20 @@ __global__ __launch_bounds__(kBlock) void k_ao_count(SceneDev S, const BakeSeg* 
     if (lane == 0 && rays) atomicAdd(rays_out, (unsigned long long)rays);
 }
 
-// AO pass 1 with lane refill: each wavefront takes chunks of `chunk_lumels` lumels from the queue
-// and keeps a pool of their (lumel, sample) rays; whenever at least `refill_min` lanes have
-// finished their ray, those lanes take the next rays of the pool (ballot + prefix count), so a
-// wavefront no longer waits for its slowest ray. Occlusion counts per lumel accumulate in an LDS
-// ring of kRing slots per wavefront (lumels in flight are consecutive in the wavefront's stream
-// and at most 64/S + 2 of them); the lane that completes a lumel's last ray writes its count.
+// AO pass 1 with lane refill: each wavefront takes chunks of up to `chunk_lumels` lumels (one
+// segment each; segments' work_begin counts chunks) and keeps a pool of their (lumel, sample)
+// rays; whenever at least `refill_min` lanes have finished their ray, those lanes take the next
+// rays of the pool (ballot + prefix count), so a wavefront no longer waits for its slowest ray.
+// Occlusion counts per lumel accumulate in an LDS ring of kRing slots per wavefront; a ray may
+// only start a lumel fewer than kRing - 2 places after the oldest lumel still in flight, and the
+// lane that completes a lumel's last ray writes its count.
 constexpr int kRing = 32;
 
 template <int CAP>
-__global__ __launch_bounds__(kBlock) void k_ao_refill(SceneDev S, const BakeSeg* __restrict__ segs,
-                                                      int nsegs, uint64_t total_lumels, AoParams P,
-                                                      uint32_t chunk_lumels, uint32_t refill_min,
-                                                      unsigned long long* queue, unsigned long long* rays_out) {
+__global__ __launch_bounds__(kBlock) void k_ao_refill(const BakeSeg* __restrict__ segs, int nsegs,
+                                                      uint64_t total_chunks, uint32_t chunk_lumels,
+                                                      uint32_t refill_min, unsigned long long* queue,
+                                                      unsigned long long* rays_out) {
     __shared__ uint32_t stack[kLdsStack * kBlock];
     __shared__ uint32_t ring_occ[kBlock / 64][kRing];
     __shared__ uint32_t ring_done[kBlock / 64][kRing];
@@ -201,23 +202,22 @@ __global__ __launch_bounds__(kBlock) void k_ao_refill(SceneDev S, const BakeSeg*
         ring_occ[wave][k] = 0u;
         ring_done[wave][k] = 0u;
     }
-    const uint32_t ns = (uint32_t)P.samples;
     // Pool state, identical in every lane of the wavefront.
-    uint64_t chunk_first = 0;
-    uint32_t chunk_count = 0, pool_next = 0, pool_end = 0, stream_base = 0;
+    int pseg = 0;
+    uint32_t chunk_first = 0, chunk_count = 0, chunk_ns = 1;
+    uint32_t pool_next = 0, pool_end = 0, stream_base = 0;
     bool pool_open = true;
     // Lane state.
     bool active = false;
     TravRay tr;
-    uint32_t node = 0, slot = 0, my_stream = 0;
+    SceneDev scene;
+    uint32_t node = 0, slot = 0, my_stream = 0, my_ns = 1;
     int sp = 0;
     uint32_t* counter = nullptr;
     uint32_t rays = 0;
     for (;;) {
         uint64_t idle = __ballot(!active);
         if (pool_open && ((uint32_t)__popcll(idle) >= refill_min || idle == ~0ull)) {
-            // Rays may only start lumels fewer than kRing - 2 places after the oldest lumel still
-            // in flight, so a ring slot is never reused before its lumel completes.
             uint32_t oldest = active ? my_stream : 0xFFFFFFFFu;
 #pragma unroll
             for (int off = 32; off >= 1; off >>= 1) oldest = min(oldest, (uint32_t)__shfl_xor((int)oldest, off, 64));
@@ -226,21 +226,22 @@ __global__ __launch_bounds__(kBlock) void k_ao_refill(SceneDev S, const BakeSeg*
             while (want && pool_open) {
                 if (pool_next == pool_end) {
                     const uint64_t c = wave_next_chunk(queue);
-                    const uint64_t first = c * (uint64_t)chunk_lumels;
-                    if (first >= total_lumels) {
+                    if (c >= total_chunks) {
                         pool_open = false;
                         break;
                     }
                     stream_base += chunk_count;
-                    chunk_first = first;
-                    const uint64_t left = total_lumels - first;
-                    chunk_count = (uint32_t)(left < chunk_lumels ? left : chunk_lumels);
+                    pseg = find_seg(segs, nsegs, c);
+                    const BakeSeg& sg = segs[pseg];
+                    chunk_first = (uint32_t)(c - sg.work_begin) * chunk_lumels;
+                    chunk_count = min(chunk_lumels, sg.count - chunk_first);
+                    chunk_ns = (uint32_t)sg.ao.samples;
                     pool_next = 0;
-                    pool_end = chunk_count * ns;
+                    pool_end = chunk_count * chunk_ns;
                 }
                 uint64_t room = pool_end - pool_next;
                 if (allow != ~0ull) {
-                    const uint64_t lim = allow > stream_base ? (allow - stream_base) * (uint64_t)ns : 0ull;
+                    const uint64_t lim = allow > stream_base ? (allow - stream_base) * (uint64_t)chunk_ns : 0ull;
                     room = lim > pool_next ? min(room, lim - pool_next) : 0ull;
                 }
                 const uint32_t take = (uint32_t)min(room, (uint64_t)__popcll(want));
@@ -248,19 +249,20 @@ __global__ __launch_bounds__(kBlock) void k_ao_refill(SceneDev S, const BakeSeg*
                 const uint32_t rank = rank_below(want);
                 const bool mine = ((want >> lane) & 1ull) && rank < take;
                 if (mine) {
+                    const BakeSeg& sg = segs[pseg];
                     const uint32_t rr = pool_next + rank;
-                    const uint32_t lc = rr / ns, si = rr % ns;
-                    const uint64_t g = chunk_first + lc;
-                    const BakeSeg& sg = segs[find_seg(segs, nsegs, g)];
-                    const uint32_t li = (uint32_t)(g - sg.work_begin);
+                    const uint32_t lc = rr / chunk_ns, si = rr % chunk_ns;
+                    const uint32_t li = chunk_first + lc;
                     const float4 p4 = sg.pos[li], n4 = sg.nrm[li];
                     const V3 p = v3(p4.x, p4.y, p4.z), n = v3(n4.x, n4.y, n4.z);
-                    Pcg32 rng = sample_stream(P.seed, sg.lm_seed, f_bits(p4.w), TECH_AO, si);
-                    const V3 dir = sample_hemisphere(rng, n, P.sampling);
-                    tr = trav_ray(ray_from_segment(p, p + dir * P.max_distance));
+                    Pcg32 rng = sample_stream(sg.ao.seed, sg.lm_seed, f_bits(p4.w), TECH_AO, si);
+                    const V3 dir = sample_hemisphere(rng, n, sg.ao.sampling);
+                    tr = trav_ray(ray_from_segment(p, p + dir * sg.ao.max_distance));
+                    scene = sg.scene;
                     node = 0;
                     sp = 0;
                     my_stream = stream_base + lc;
+                    my_ns = chunk_ns;
                     slot = my_stream % (uint32_t)kRing;
                     counter = sg.counts + li;
                     active = true;
@@ -273,12 +275,12 @@ __global__ __launch_bounds__(kBlock) void k_ao_refill(SceneDev S, const BakeSeg*
             break;
         }
         if (active) {
-            const int st = any_hit_step<CAP>(S, tr, node, sp, stk, kBlock, spill);
+            const int st = any_hit_step<CAP>(scene, tr, node, sp, stk, kBlock, spill);
             if (st != 0) {
                 rays++;
                 if (st > 0) atomicAdd(&ring_occ[wave][slot], 1u);
                 const uint32_t done = atomicAdd(&ring_done[wave][slot], 1u);
-                if (done == ns - 1) {
+                if (done == my_ns - 1) {
                     // Last ray of the lumel: every earlier increment of this slot precedes this
                     // one in the wavefront's LDS order; read, publish, clear.
                     *counter = atomicExch(&ring_occ[wave][slot], 0u);
@@ -292,11 +294,11 @@ __global__ __launch_bounds__(kBlock) void k_ao_refill(SceneDev S, const BakeSeg*
 }
 
 // AO pass 2: AmbientOcclusion::bakeLumel's value and multiply (baker/AmbientOcclusion.cpp:59-65).
-__global__ __launch_bounds__(kBlock) void k_ao_finalize(const BakeSeg* __restrict__ segs, int nsegs,
-                                                        uint64_t total, AoParams P) {
+__global__ __launch_bounds__(kBlock) void k_ao_finalize(const BakeSeg* __restrict__ segs, int nsegs, uint64_t total) {
     const uint64_t stride = (uint64_t)gridDim.x * kBlock;
     for (uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x; g < total; g += stride) {
         const BakeSeg& sg = segs[find_seg(segs, nsegs, g)];
+        const AoParams P = sg.ao;
         const uint32_t i = (uint32_t)(g - sg.work_begin);
         const uint32_t occluded = sg.counts[i];
         float value = 1.0f - (float)occluded / ((float)P.samples * P.occluded_fraction);
@@ -345,9 +347,9 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDev S, const float4* __re
 
 uint32_t blocks_for(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
 
-// Persistent grid: enough blocks for the work, at most max_blocks.
-uint32_t persistent_blocks(uint64_t items, uint32_t max_blocks) {
-    const uint64_t need = (items + kBlock - 1) / kBlock;
+// Persistent grid: enough blocks for `waves` wavefronts of work, at most max_blocks.
+uint32_t grid_for_waves(uint64_t waves, uint32_t max_blocks) {
+    const uint64_t need = (waves + (kBlock / 64) - 1) / (kBlock / 64);
     return (uint32_t)(need < max_blocks ? (need ? need : 1) : max_blocks);
 }
 
@@ -377,52 +379,45 @@ hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v) 
     return hipGetLastError();
 }
 
-hipError_t launch_bake_direct(hipStream_t st, const SceneDev& S, int cap, uint32_t max_blocks,
-                              const BakeSeg* segs, int nsegs, uint64_t total, const LightDev* lights,
-                              int nlights, const float4* lverts, unsigned long long* queue,
-                              unsigned long long* rays) {
+hipError_t launch_bake_direct(hipStream_t st, int cap, uint32_t max_blocks, const BakeSeg* segs, int nsegs,
+                              uint64_t total, unsigned long long* queue, unsigned long long* rays) {
     if (total == 0) return hipSuccess;
     hipError_t e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(queue), 2, 0u);
     if (e != hipSuccess) return e;
-    const uint32_t grid = persistent_blocks(total, max_blocks);
-    RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_bake_direct<CAP>, dim3(grid), dim3(kBlock), 0, st, S, segs, nsegs,
-                                            total, lights, nlights, lverts, queue, rays));
+    const uint32_t grid = grid_for_waves((total + 63) / 64, max_blocks);
+    RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_bake_direct<CAP>, dim3(grid), dim3(kBlock), 0, st, segs, nsegs, total,
+                                            queue, rays));
     return hipGetLastError();
 }
 
-hipError_t launch_ao_count(hipStream_t st, const SceneDev& S, int cap, uint32_t max_blocks,
-                           const BakeSeg* segs, int nsegs, uint64_t total_pairs, const AoParams& P,
-                           unsigned long long* queue, unsigned long long* rays) {
+hipError_t launch_ao_count(hipStream_t st, int cap, uint32_t max_blocks, const BakeSeg* segs, int nsegs,
+                           uint64_t total_pairs, unsigned long long* queue, unsigned long long* rays) {
     if (total_pairs == 0) return hipSuccess;
     hipError_t e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(queue), 2, 0u);
     if (e != hipSuccess) return e;
-    const uint32_t grid = persistent_blocks(total_pairs, max_blocks);
-    RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_ao_count<CAP>, dim3(grid), dim3(kBlock), 0, st, S, segs, nsegs,
-                                            total_pairs, P, queue, rays));
+    const uint32_t grid = grid_for_waves((total_pairs + 63) / 64, max_blocks);
+    RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_ao_count<CAP>, dim3(grid), dim3(kBlock), 0, st, segs, nsegs,
+                                            total_pairs, queue, rays));
     return hipGetLastError();
 }
 
-hipError_t launch_ao_refill(hipStream_t st, const SceneDev& S, int cap, uint32_t max_blocks,
-                            const BakeSeg* segs, int nsegs, uint64_t total_lumels, const AoParams& P,
-                            uint32_t chunk_lumels, uint32_t refill_min, unsigned long long* queue,
-                            unsigned long long* rays) {
-    if (total_lumels == 0) return hipSuccess;
-    if (P.samples < 4 || chunk_lumels == 0 || refill_min == 0 || refill_min > 64) return hipErrorInvalidValue;
+hipError_t launch_ao_refill(hipStream_t st, int cap, uint32_t max_blocks, const BakeSeg* segs, int nsegs,
+                            uint64_t total_chunks, uint32_t chunk_lumels, uint32_t refill_min,
+                            unsigned long long* queue, unsigned long long* rays) {
+    if (total_chunks == 0) return hipSuccess;
+    if (chunk_lumels == 0 || refill_min == 0 || refill_min > 64) return hipErrorInvalidValue;
     hipError_t e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(queue), 2, 0u);
     if (e != hipSuccess) return e;
-    const uint64_t need = (total_lumels + chunk_lumels - 1) / chunk_lumels;  // one wavefront per chunk
-    const uint64_t blocks = (need + 1) / 2;
-    const uint32_t grid = (uint32_t)(blocks < max_blocks ? (blocks ? blocks : 1) : max_blocks);
-    RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_ao_refill<CAP>, dim3(grid), dim3(kBlock), 0, st, S, segs, nsegs,
-                                            total_lumels, P, chunk_lumels, refill_min, queue, rays));
+    const uint32_t grid = grid_for_waves(total_chunks, max_blocks);
+    RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_ao_refill<CAP>, dim3(grid), dim3(kBlock), 0, st, segs, nsegs,
+                                            total_chunks, chunk_lumels, refill_min, queue, rays));
     return hipGetLastError();
 }
 
-hipError_t launch_ao_finalize(hipStream_t st, const BakeSeg* segs, int nsegs, uint64_t total,
-                              const AoParams& P) {
+hipError_t launch_ao_finalize(hipStream_t st, const BakeSeg* segs, int nsegs, uint64_t total) {
     if (total == 0) return hipSuccess;
     const uint32_t blocks = (uint32_t)std::min<uint64_t>(blocks_for(total, kBlock), 65536);
-    hipLaunchKernelGGL(k_ao_finalize, dim3(blocks), dim3(kBlock), 0, st, segs, nsegs, total, P);
+    hipLaunchKernelGGL(k_ao_finalize, dim3(blocks), dim3(kBlock), 0, st, segs, nsegs, total);
     return hipGetLastError();
 }
 
@@ -432,7 +427,7 @@ hipError_t launch_trace(hipStream_t st, const SceneDev& S, int cap, uint32_t max
     if (n == 0) return hipSuccess;
     hipError_t e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(queue), 2, 0u);
     if (e != hipSuccess) return e;
-    const dim3 grid(persistent_blocks(n, max_blocks)), block(kBlock);
+    const dim3 grid(grid_for_waves(((uint64_t)n + 63) / 64, max_blocks)), block(kBlock);
     if (any) {
         RL_DISPATCH_CAP(cap, hipLaunchKernelGGL((k_trace<true, CAP>), grid, block, 0, st, S, starts, ends, n, flags,
                                                 hits, queue, rays));

@@ -10,9 +10,15 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/relight.h"
@@ -326,116 +332,167 @@ int32_t check_scene_ready(rl_scene* scene) {
 }
 
 // Builds the device segment table for a set of lightmaps; `per_lumel` work items per lumel.
-int32_t make_segments(rl_relight* ctx, rl_scene* scene, rl_lightmap* const* maps, int n,
-                      uint64_t per_lumel, BakeSeg** d_segs, uint64_t* total, uint64_t* texels) {
-    std::vector<BakeSeg> segs;
-    uint64_t acc = 0, tex = 0;
+// One unit of bake work: lumels [first, first + count) of a lightmap of a scene, with its AO
+// settings. A whole lightmap is first = 0, count = its valid lumels.
+struct BakeItem {
+    rl_lightmap* lm;
+    rl_scene* scene;
+    AoParams ao;
+    uint32_t first;
+    uint32_t count;
+};
+
+enum SegUnit { kUnitLumel = 0, kUnitPair = 1, kUnitChunk = 2 };
+
+// A device segment table for one launch.
+struct DeviceSegs {
+    BakeSeg* d = nullptr;
+    int n = 0;
+    uint64_t total = 0;   // work items in `unit`
+    uint64_t texels = 0;  // lumels covered
+    int cap = 32;         // traversal-stack capacity for every scene in the table
+    ~DeviceSegs() { dev_free(d); }
+};
+
+AoParams ao_params(const rl_ao_settings& s) {
+    AoParams P;
+    P.samples = s.samples;
+    P.occluded_fraction = s.occluded_fraction;
+    P.max_distance = s.max_distance;
+    P.exponent = s.exponent;
+    P.seed = s.seed;
+    P.sampling = s.sampling;
+    return P;
+}
+
+int32_t check_ao(const rl_ao_settings* s) {
+    if (!s) return fail(RL_INVALID_ARGUMENT, "AO settings are NULL");
+    if (s->samples <= 0) return fail(RL_INVALID_ARGUMENT, "AO sample count must be positive");
+    if (s->sampling != RL_SAMPLE_UNIFORM && s->sampling != RL_SAMPLE_COSINE)
+        return fail(RL_INVALID_ARGUMENT, "unknown sampling mode");
+    return RL_SUCCESS;
+}
+
+// Whole-lightmap items of `maps` in `scene`, uploading lumels where needed.
+int32_t whole_items(rl_scene* scene, rl_lightmap* const* maps, int n, const AoParams& ao,
+                    std::vector<BakeItem>& items) {
     for (int i = 0; i < n; i++) {
         rl_lightmap* lm = maps[i];
         if (!lm) return fail(RL_INVALID_ARGUMENT, "lightmap is NULL");
         if (lm->scene != scene) return fail(RL_INVALID_CALL, "lightmap holds no mesh of this scene");
         int32_t st = push_lumels(lm);
         if (st) return st;
+        items.push_back(BakeItem{lm, scene, ao, 0u, (uint32_t)lm->texels.size()});
+    }
+    return RL_SUCCESS;
+}
+
+int32_t upload_segments(const std::vector<BakeItem>& items, int unit, uint32_t chunk_lumels, DeviceSegs& out) {
+    std::vector<BakeSeg> segs;
+    uint64_t acc = 0;
+    for (const BakeItem& it : items) {
+        if (it.count == 0) continue;
+        rl_lightmap* lm = it.lm;
         BakeSeg s;
-        s.pos = lm->d_pos;
-        s.nrm = lm->d_nrm;
-        s.color = lm->d_color;
-        s.counts = lm->d_counts;
-        s.count = (uint32_t)lm->texels.size();
+        s.pos = lm->d_pos + it.first;
+        s.nrm = lm->d_nrm + it.first;
+        s.color = lm->d_color + it.first;
+        s.counts = lm->d_counts + it.first;
+        s.count = it.count;
         s.lm_seed = lm->seed;
         s.work_begin = acc;
-        if (s.count == 0) continue;
+        s.scene = it.scene->dev();
+        s.lights = it.scene->d_lights;
+        s.lverts = it.scene->d_lverts;
+        s.nlights = (int32_t)it.scene->lights.size();
+        s.ao = it.ao;
         segs.push_back(s);
-        acc += (uint64_t)s.count * per_lumel;
-        tex += s.count;
+        if (unit == kUnitLumel) acc += it.count;
+        else if (unit == kUnitPair) acc += (uint64_t)it.count * (uint64_t)it.ao.samples;
+        else acc += (it.count + chunk_lumels - 1) / chunk_lumels;
+        out.texels += it.count;
+        out.cap = std::max(out.cap, it.scene->cap);
     }
-    *total = acc;
-    *texels = tex;
-    *d_segs = nullptr;
+    out.total = acc;
+    out.n = (int)segs.size();
     if (segs.empty()) return RL_SUCCESS;
-    RL_HIP(dev_alloc(d_segs, segs.size()));
-    RL_HIP(hipMemcpy(*d_segs, segs.data(), segs.size() * sizeof(BakeSeg), hipMemcpyHostToDevice));
-    (void)ctx;
+    RL_HIP(dev_alloc(&out.d, segs.size()));
+    RL_HIP(hipMemcpy(out.d, segs.data(), segs.size() * sizeof(BakeSeg), hipMemcpyHostToDevice));
+    return RL_SUCCESS;
+}
+
+// Direct light over items (one launch); the caller synchronises.
+int32_t run_direct_items(rl_relight* ctx, const std::vector<BakeItem>& items) {
+    DeviceSegs ds;
+    int32_t st = upload_segments(items, kUnitLumel, 1, ds);
+    if (st) return st;
+    hipError_t e = hipSuccess;
+    if (ds.total) {
+        e = kernel_begin(ctx, RL_KERNEL_DIRECT);
+        if (e == hipSuccess) e = launch_bake_direct(ctx->stream, ds.cap, ctx->max_blocks, ds.d, ds.n, ds.total,
+                                                    ctx->d_queue, ctx->d_rays);
+        if (e == hipSuccess) e = kernel_end(ctx);
+    }
+    ctx->stats.texels += ds.texels;
+    ctx->stats.launches += ds.total ? 1 : 0;
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return fail(RL_DEVICE_ERROR, std::string("direct bake: ") + hipGetErrorString(e));
+    for (const BakeItem& it : items) it.lm->device_newer = true;
+    return RL_SUCCESS;
+}
+
+// AO over items: count pass (lane refill when every item has >= 4 samples, else chunked) and
+// the finalize pass.
+int32_t run_ao_items(rl_relight* ctx, const std::vector<BakeItem>& items) {
+    bool refill = ctx->tuning.ao_kernel == RL_AO_KERNEL_REFILL;
+    for (const BakeItem& it : items) refill = refill && it.ao.samples >= 4;
+    const uint32_t chunk = ctx->tuning.ao_chunk_lumels;
+    DeviceSegs work, lum;
+    int32_t st = upload_segments(items, refill ? kUnitChunk : kUnitPair, chunk, work);
+    if (st) return st;
+    if ((st = upload_segments(items, kUnitLumel, 1, lum))) return st;
+    hipError_t e = hipSuccess;
+    if (work.total) {
+        e = kernel_begin(ctx, RL_KERNEL_AO_COUNT);
+        if (e == hipSuccess && refill)
+            e = launch_ao_refill(ctx->stream, work.cap, ctx->max_blocks, work.d, work.n, work.total, chunk,
+                                 ctx->tuning.refill_min, ctx->d_queue, ctx->d_rays);
+        if (e == hipSuccess && !refill) {
+            for (const BakeItem& it : items)
+                if (e == hipSuccess && it.count)
+                    e = launch_fill_u32(ctx->stream, it.lm->d_counts + it.first, it.count, 0u);
+            if (e == hipSuccess)
+                e = launch_ao_count(ctx->stream, work.cap, ctx->max_blocks, work.d, work.n, work.total, ctx->d_queue,
+                                    ctx->d_rays);
+        }
+        if (e == hipSuccess) e = kernel_end(ctx);
+        if (e == hipSuccess) e = kernel_begin(ctx, RL_KERNEL_AO_FINALIZE);
+        if (e == hipSuccess) e = launch_ao_finalize(ctx->stream, lum.d, lum.n, lum.total);
+        if (e == hipSuccess) e = kernel_end(ctx);
+    }
+    ctx->stats.texels += lum.texels;
+    ctx->stats.launches += work.total ? 2 : 0;
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return fail(RL_DEVICE_ERROR, std::string("AO bake: ") + hipGetErrorString(e));
+    for (const BakeItem& it : items) it.lm->device_newer = true;
     return RL_SUCCESS;
 }
 
 int32_t run_direct(rl_relight* ctx, rl_scene* scene, rl_lightmap* const* maps, int n) {
-    BakeSeg* d_segs = nullptr;
-    uint64_t total = 0, tex = 0;
-    int32_t st = make_segments(ctx, scene, maps, n, 1, &d_segs, &total, &tex);
+    std::vector<BakeItem> items;
+    AoParams none{};
+    int32_t st = whole_items(scene, maps, n, none, items);
     if (st) return st;
-    int nsegs = 0;
-    for (int i = 0; i < n; i++) nsegs += maps[i]->texels.empty() ? 0 : 1;
-    hipError_t e = hipSuccess;
-    if (total) {
-        e = kernel_begin(ctx, RL_KERNEL_DIRECT);
-        if (e == hipSuccess)
-            e = launch_bake_direct(ctx->stream, scene->dev(), scene->cap, ctx->max_blocks, d_segs, nsegs, total,
-                                   scene->d_lights, (int)scene->lights.size(), scene->d_lverts,
-                                   ctx->d_queue, ctx->d_rays);
-        if (e == hipSuccess) e = kernel_end(ctx);
-    }
-    ctx->stats.texels += tex;
-    ctx->stats.launches += total ? 1 : 0;
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    dev_free(d_segs);
-    if (e != hipSuccess) return fail(RL_DEVICE_ERROR, std::string("direct bake: ") + hipGetErrorString(e));
-    for (int i = 0; i < n; i++) maps[i]->device_newer = true;
-    return RL_SUCCESS;
+    return run_direct_items(ctx, items);
 }
 
 int32_t run_ao(rl_relight* ctx, rl_scene* scene, rl_lightmap* const* maps, int n,
                const rl_ao_settings* s) {
-    if (!s) return fail(RL_INVALID_ARGUMENT, "AO settings are NULL");
-    if (s->samples <= 0) return fail(RL_INVALID_ARGUMENT, "AO sample count must be positive");
-    if (s->sampling != RL_SAMPLE_UNIFORM && s->sampling != RL_SAMPLE_COSINE)
-        return fail(RL_INVALID_ARGUMENT, "unknown sampling mode");
-    AoParams P;
-    P.samples = s->samples;
-    P.occluded_fraction = s->occluded_fraction;
-    P.max_distance = s->max_distance;
-    P.exponent = s->exponent;
-    P.seed = s->seed;
-    P.sampling = s->sampling;
-    BakeSeg* d_pairs = nullptr;
-    uint64_t pairs = 0, tex = 0;
-    int32_t st = make_segments(ctx, scene, maps, n, (uint64_t)s->samples, &d_pairs, &pairs, &tex);
+    int32_t st = check_ao(s);
     if (st) return st;
-    BakeSeg* d_lum = nullptr;
-    uint64_t lumels = 0, tex2 = 0;
-    st = make_segments(ctx, scene, maps, n, 1, &d_lum, &lumels, &tex2);
-    if (st) {
-        dev_free(d_pairs);
-        return st;
-    }
-    int nsegs = 0;
-    hipError_t e = hipSuccess;
-    for (int i = 0; i < n; i++) {
-        nsegs += maps[i]->texels.empty() ? 0 : 1;
-        if (e == hipSuccess) e = launch_fill_u32(ctx->stream, maps[i]->d_counts, maps[i]->texels.size(), 0u);
-    }
-    const bool refill = ctx->tuning.ao_kernel == RL_AO_KERNEL_REFILL && s->samples >= 4;
-    if (pairs) {
-        if (e == hipSuccess) e = kernel_begin(ctx, RL_KERNEL_AO_COUNT);
-        if (e == hipSuccess && refill)
-            e = launch_ao_refill(ctx->stream, scene->dev(), scene->cap, ctx->max_blocks, d_lum, nsegs, lumels, P,
-                                 ctx->tuning.ao_chunk_lumels, ctx->tuning.refill_min, ctx->d_queue, ctx->d_rays);
-        if (e == hipSuccess && !refill)
-            e = launch_ao_count(ctx->stream, scene->dev(), scene->cap, ctx->max_blocks, d_pairs, nsegs, pairs, P,
-                                ctx->d_queue, ctx->d_rays);
-        if (e == hipSuccess) e = kernel_end(ctx);
-        if (e == hipSuccess) e = kernel_begin(ctx, RL_KERNEL_AO_FINALIZE);
-        if (e == hipSuccess) e = launch_ao_finalize(ctx->stream, d_lum, nsegs, lumels, P);
-        if (e == hipSuccess) e = kernel_end(ctx);
-    }
-    ctx->stats.texels += tex;
-    ctx->stats.launches += pairs ? 2 : 0;
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    dev_free(d_pairs);
-    dev_free(d_lum);
-    if (e != hipSuccess) return fail(RL_DEVICE_ERROR, std::string("AO bake: ") + hipGetErrorString(e));
-    for (int i = 0; i < n; i++) maps[i]->device_newer = true;
-    return RL_SUCCESS;
+    std::vector<BakeItem> items;
+    if ((st = whole_items(scene, maps, n, ao_params(*s), items))) return st;
+    return run_ao_items(ctx, items);
 }
 
 // Device tables of a scene's photonmaps (distinct maps in mesh order) and the mesh -> map index.
@@ -1108,9 +1165,13 @@ int32_t rl_bake_indirect(rl_relight* ctx, rl_scene* scene, int32_t mesh, const r
     if ((st = make_photon_tables(scene, T))) return st;
     CallTimer tm(ctx);
     if ((st = begin_call(ctx))) return st;
-    BakeSeg* d_segs = nullptr;
-    uint64_t total = 0, tex = 0;
-    if ((st = make_segments(ctx, scene, &lm, 1, 1, &d_segs, &total, &tex))) return st;
+    std::vector<BakeItem> items;
+    AoParams none{};
+    if ((st = whole_items(scene, &lm, 1, none, items))) return st;
+    DeviceSegs ds;
+    if ((st = upload_segments(items, kUnitLumel, 1, ds))) return st;
+    const uint64_t total = ds.total, tex = ds.texels;
+    BakeSeg* d_segs = ds.d;
     FinalGatherParams P;
     P.samples = s->final_gather_samples;
     P.distance = s->final_gather_distance;
@@ -1131,7 +1192,6 @@ int32_t rl_bake_indirect(rl_relight* ctx, rl_scene* scene, int32_t mesh, const r
     ctx->stats.texels += tex;
     ctx->stats.launches += total ? 1 : 0;
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    dev_free(d_segs);
     if (e != hipSuccess) return fail(RL_DEVICE_ERROR, std::string("final gather: ") + hipGetErrorString(e));
     lm->device_newer = true;
     return end_call(ctx, tm);
@@ -1353,6 +1413,447 @@ int32_t rl_get_tuning(const rl_relight* ctx, rl_tuning* out) {
 int32_t rl_get_stats(const rl_relight* ctx, rl_stats* out) {
     if (!ctx || !out) return fail(RL_INVALID_ARGUMENT, "NULL argument");
     *out = ctx->stats;
+    return RL_SUCCESS;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- multi-scene batch and server
+
+namespace {
+
+int32_t add_item(const rl_bake_item& d, std::vector<BakeItem>& direct, std::vector<BakeItem>& ao) {
+    if (!d.scene || !d.lightmap) return fail(RL_INVALID_ARGUMENT, "item scene or lightmap is NULL");
+    int32_t st = check_scene_ready(d.scene);
+    if (st) return st;
+    if (d.lightmap->scene != d.scene) return fail(RL_INVALID_CALL, "lightmap holds no mesh of this scene");
+    if ((st = push_lumels(d.lightmap))) return st;
+    const uint32_t n = (uint32_t)d.lightmap->texels.size();
+    if (d.first_lumel > n) return fail(RL_INVALID_ARGUMENT, "lumel range out of bounds");
+    const uint32_t cnt = d.lumel_count == 0xFFFFFFFFu ? n - d.first_lumel : d.lumel_count;
+    if ((uint64_t)d.first_lumel + cnt > n) return fail(RL_INVALID_ARGUMENT, "lumel range out of bounds");
+    if (d.direct) direct.push_back(BakeItem{d.lightmap, d.scene, AoParams{}, d.first_lumel, cnt});
+    if (d.ao) {
+        if ((st = check_ao(d.ao))) return st;
+        ao.push_back(BakeItem{d.lightmap, d.scene, ao_params(*d.ao), d.first_lumel, cnt});
+    }
+    return RL_SUCCESS;
+}
+
+// Shadow rays per lumel at most (one per point light, one per light vertex).
+uint32_t direct_rays_per_lumel(const rl_scene* s) {
+    uint32_t r = 0;
+    for (const LightHost& L : s->lights) r += L.verts.empty() ? 1u : (uint32_t)L.verts.size();
+    return std::max(r, 1u);
+}
+
+uint32_t valid_lumels(const rl_lightmap* lm) {
+    uint32_t c = 0;
+    for (uint8_t v : lm->valid) c += v;
+    return c;
+}
+
+struct Job {
+    uint64_t id = 0;
+    rl_job_desc desc{};
+    std::vector<rl_lightmap*> maps;
+    std::vector<uint32_t> valid;  // valid lumels per map
+    rl_ao_settings ao{};
+    bool has_ao = false;
+    rl_indirect_settings ind{};
+    std::vector<float> E, R;
+    int32_t state = RL_JOB_QUEUED;
+    int32_t status = RL_SUCCESS;
+    double submit_ms = 0, admit_ms = 0, done_ms = 0;
+    uint64_t rays = 0, footprint = 0;
+    int phase = 0;        // DIRECT_AO: 0 direct, 1 AO; INDIRECT: 0 photons, 1 final gather; RADIOSITY: 0 F, 1 solve
+    size_t map_i = 0;     // next lightmap of the phase
+    uint32_t cursor = 0;  // next lumel of that lightmap
+    bool cancel = false;
+    bool prepared = false;  // lumels uploaded
+    std::string error;    // message of the failing call, handed to the caller's rl_last_error
+};
+
+}  // namespace
+
+struct rl_server {
+    rl_relight* ctx = nullptr;
+    rl_server_config cfg{};
+    std::mutex m;
+    std::condition_variable cv_work, cv_done;
+    std::thread worker;
+    bool stop = false;
+    std::map<uint64_t, std::unique_ptr<Job>> jobs;
+    std::deque<Job*> queue;
+    std::vector<Job*> active;
+    uint64_t next_id = 1, mem_used = 0;
+    rl_server_stats stats{};
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+
+    double now_ms() const {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    void loop();
+    void step(std::vector<Job*>& js);
+};
+
+namespace {
+
+bool job_finished(const Job* j) {
+    return j->state == RL_JOB_DONE || j->state == RL_JOB_FAILED || j->state == RL_JOB_CANCELLED;
+}
+
+// Moves a DIRECT_AO job to its next phase with work, or to done.
+void advance_direct_ao(Job* j) {
+    j->map_i = 0;
+    j->cursor = 0;
+    j->phase++;
+    if (j->phase == 1 && !j->has_ao) j->phase = 2;
+    if (j->phase >= 2) j->state = RL_JOB_DONE;
+}
+
+// Takes up to `budget` lumels of the job's current phase into `items`; returns lumels taken.
+uint64_t take_lumels(Job* j, uint64_t budget, const AoParams& ao, std::vector<BakeItem>& items) {
+    uint64_t taken = 0;
+    while (budget > 0 && j->map_i < j->maps.size()) {
+        const uint32_t n = j->valid[j->map_i];
+        const uint32_t take = (uint32_t)std::min<uint64_t>(budget, n - j->cursor);
+        if (take) items.push_back(BakeItem{j->maps[j->map_i], j->desc.scene, ao, j->cursor, take});
+        j->cursor += take;
+        budget -= take;
+        taken += take;
+        if (j->cursor >= n) {
+            j->map_i++;
+            j->cursor = 0;
+        }
+    }
+    return taken;
+}
+
+}  // namespace
+
+void rl_server::step(std::vector<Job*>& js) {
+    // Direct/AO work of every DIRECT_AO job, fairly shared, in one launch per pass.
+    std::vector<BakeItem> ditems, aitems;
+    std::vector<Job*> dj, aj;
+    size_t nda = 0;
+    for (Job* j : js)
+        if (j->desc.kind == RL_JOB_DIRECT_AO) nda++;
+    const uint64_t share = std::max<uint64_t>(cfg.quantum_rays / std::max<size_t>(nda, 1), 4096);
+    for (Job* j : js) {
+        if (j->desc.kind != RL_JOB_DIRECT_AO || job_finished(j) || j->prepared) continue;
+        int32_t st = RL_SUCCESS;
+        for (size_t k = 0; k < j->maps.size() && !st; k++) {
+            st = push_lumels(j->maps[k]);
+            if (!st && j->maps[k]->texels.size() != j->valid[k]) st = fail(RL_INVALID_CALL, "lightmap changed after submit");
+        }
+        if (st) {
+            j->state = RL_JOB_FAILED;
+            j->status = st;
+            j->error = g_error;
+        }
+        j->prepared = true;
+    }
+    for (Job* j : js) {
+        if (j->desc.kind != RL_JOB_DIRECT_AO || job_finished(j)) continue;
+        if (j->phase == 0 && !j->desc.direct) advance_direct_ao(j);
+        if (job_finished(j)) continue;
+        const bool direct = j->phase == 0;
+        const uint64_t rpl = direct ? direct_rays_per_lumel(j->desc.scene) : (uint64_t)j->ao.samples;
+        const uint64_t lumels = std::max<uint64_t>(share / rpl, 64);
+        const uint64_t got = take_lumels(j, lumels, direct ? AoParams{} : ao_params(j->ao), direct ? ditems : aitems);
+        j->rays += got * rpl;
+        (direct ? dj : aj).push_back(j);
+    }
+    CallTimer tm(ctx);
+    int32_t st = begin_call(ctx);
+    if (!st && !ditems.empty()) st = run_direct_items(ctx, ditems);
+    if (!st && !aitems.empty()) st = run_ao_items(ctx, aitems);
+    if (!st) st = end_call(ctx, tm);
+    stats.rays += ctx->stats.rays;
+    for (Job* j : js) {
+        if (j->desc.kind != RL_JOB_DIRECT_AO || job_finished(j)) continue;
+        const bool in_step = std::find(dj.begin(), dj.end(), j) != dj.end() || std::find(aj.begin(), aj.end(), j) != aj.end();
+        if (!in_step) continue;
+        if (st) {
+            j->state = RL_JOB_FAILED;
+            j->status = st;
+            j->error = g_error;
+        } else if (j->map_i >= j->maps.size()) {
+            advance_direct_ao(j);
+        }
+    }
+    // Photon / final-gather / radiosity jobs: one whole phase (or one lightmap) per step.
+    for (Job* j : js) {
+        if (job_finished(j) || j->desc.kind == RL_JOB_DIRECT_AO) continue;
+        int32_t s2 = RL_SUCCESS;
+        if (j->desc.kind == RL_JOB_INDIRECT) {
+            if (j->phase == 0) {
+                s2 = rl_emit_photons(ctx, j->desc.scene, &j->ind);
+                j->rays += ctx->stats.rays;
+                j->phase = 1;
+            } else if (j->map_i < j->maps.size()) {
+                rl_lightmap* lm = j->maps[j->map_i++];
+                int32_t mesh = -1;
+                for (size_t k = 0; k < j->desc.scene->meshes.size() && mesh < 0; k++)
+                    if (j->desc.scene->meshes[k].lightmap == lm) mesh = (int32_t)k;
+                s2 = mesh < 0 ? fail(RL_INVALID_CALL, "lightmap holds no mesh of the job's scene")
+                              : rl_bake_indirect(ctx, j->desc.scene, mesh, &j->ind);
+                j->rays += ctx->stats.rays;
+            }
+            if (!s2 && j->phase == 1 && j->map_i >= j->maps.size()) j->state = RL_JOB_DONE;
+        } else {
+            if (j->phase == 0) {
+                s2 = rl_radiosity_build_form_factors(ctx, j->desc.radiosity, j->desc.rays_per_patch, j->desc.ff_seed,
+                                                     j->desc.ff_max_distance);
+                j->rays += ctx->stats.rays;
+                j->phase = 1;
+            } else {
+                rl_solve_info info{};
+                s2 = rl_radiosity_solve(ctx, j->desc.radiosity, j->E.data(), j->R.data(), j->desc.num_rhs,
+                                        j->desc.tolerance, 40, 16, j->desc.out_radiosity, &info);
+                if (!s2) j->state = RL_JOB_DONE;
+            }
+        }
+        stats.rays += ctx->stats.rays;
+        if (s2) {
+            j->state = RL_JOB_FAILED;
+            j->status = s2;
+            j->error = g_error;
+        }
+    }
+}
+
+void rl_server::loop() {
+    std::unique_lock<std::mutex> lk(m);
+    for (;;) {
+        while (!queue.empty() && (int)active.size() < cfg.max_active) {
+            Job* j = queue.front();
+            if (j->cancel) {
+                queue.pop_front();
+                j->state = RL_JOB_CANCELLED;
+                j->done_ms = now_ms();
+                stats.cancelled++;
+                cv_done.notify_all();
+                continue;
+            }
+            if (mem_used + j->footprint > cfg.memory_budget) break;
+            queue.pop_front();
+            j->state = RL_JOB_RUNNING;
+            j->admit_ms = now_ms();
+            mem_used += j->footprint;
+            stats.peak_memory = std::max(stats.peak_memory, mem_used);
+            active.push_back(j);
+        }
+        if (active.empty()) {
+            if (stop) break;
+            cv_work.wait(lk);
+            continue;
+        }
+        std::vector<Job*> js(active);
+        lk.unlock();
+        const double t = now_ms();
+        step(js);
+        lk.lock();
+        stats.busy_ms += now_ms() - t;
+        stats.steps++;
+        bool any = false;
+        for (size_t k = 0; k < active.size();) {
+            Job* j = active[k];
+            if (j->cancel && !job_finished(j)) j->state = RL_JOB_CANCELLED;
+            if (job_finished(j)) {
+                j->done_ms = now_ms();
+                mem_used -= j->footprint;
+                if (j->state == RL_JOB_DONE) stats.completed++;
+                else if (j->state == RL_JOB_FAILED) stats.failed++;
+                else stats.cancelled++;
+                active.erase(active.begin() + k);
+                any = true;
+            } else {
+                k++;
+            }
+        }
+        if (any) cv_done.notify_all();
+    }
+}
+
+namespace {
+
+// Device bytes a job needs while admitted (resident scene data excluded).
+uint64_t job_footprint(const Job& j) {
+    uint64_t lumels = 0;
+    for (uint32_t v : j.valid) lumels += v;
+    uint64_t bytes = lumels * (3 * sizeof(float4) + sizeof(uint32_t));
+    if (j.desc.kind == RL_JOB_INDIRECT) {
+        uint64_t per_pass = 0;
+        for (const LightHost& L : j.desc.scene->lights)
+            per_pass += j.ind.photons_per_light > 0 ? (uint64_t)j.ind.photons_per_light
+                                                    : (uint64_t)std::max(0, (int32_t)(L.dev.intensity * 25000.0f));
+        const uint64_t recs = per_pass * (uint64_t)std::max(0, j.ind.photon_pass_count) * (uint64_t)(j.ind.photon_bounce_count + 1);
+        bytes += recs * (sizeof(PhotonRec) + 4 * sizeof(uint32_t)) + recs * 2;  // records, keys/indices, sort scratch
+        for (const MeshHost& mh : j.desc.scene->meshes)
+            if (mh.photonmap) bytes += (uint64_t)mh.photonmap->w * mh.photonmap->h * (2 * sizeof(float4) + sizeof(uint32_t) + 1);
+    } else if (j.desc.kind == RL_JOB_RADIOSITY) {
+        const uint64_t npad = j.desc.radiosity->npad, mpad = (j.desc.num_rhs + 15) / 16 * 16;
+        bytes += npad * npad * sizeof(uint16_t) + npad * mpad * (9 * sizeof(float) + 3 * 2);
+    }
+    return bytes;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t rl_bake_items(rl_relight* ctx, const rl_bake_item* items, int32_t num_items) {
+    if (!ctx || !items || num_items <= 0) return fail(RL_INVALID_ARGUMENT, "bad item list");
+    std::vector<BakeItem> direct, ao;
+    for (int32_t i = 0; i < num_items; i++) {
+        int32_t st = add_item(items[i], direct, ao);
+        if (st) return st;
+    }
+    CallTimer tm(ctx);
+    int32_t st = begin_call(ctx);
+    if (st) return st;
+    if (!direct.empty() && (st = run_direct_items(ctx, direct))) return st;
+    if (!ao.empty() && (st = run_ao_items(ctx, ao))) return st;
+    return end_call(ctx, tm);
+}
+
+int32_t rl_server_create(rl_relight* ctx, const rl_server_config* cfg, rl_server** out) {
+    if (!ctx || !cfg || !out) return fail(RL_INVALID_ARGUMENT, "NULL argument");
+    if (cfg->max_active <= 0 || cfg->quantum_rays == 0 || cfg->memory_budget == 0)
+        return fail(RL_INVALID_ARGUMENT, "bad server configuration");
+    rl_server* s = new (std::nothrow) rl_server();
+    if (!s) return fail(RL_OUT_OF_MEMORY, "server allocation failed");
+    s->ctx = ctx;
+    s->cfg = *cfg;
+    s->worker = std::thread([s]() { s->loop(); });
+    *out = s;
+    return RL_SUCCESS;
+}
+
+int32_t rl_server_submit(rl_server* s, const rl_job_desc* d, uint64_t* out_id) {
+    if (!s || !d) return fail(RL_INVALID_ARGUMENT, "NULL argument");
+    std::unique_ptr<Job> j(new (std::nothrow) Job());
+    if (!j) return fail(RL_OUT_OF_MEMORY, "job allocation failed");
+    j->desc = *d;
+    int32_t st = check_scene_ready(d->scene);
+    if (st) return fail(RL_REJECTED, std::string("job rejected: ") + g_error);
+    if (d->kind == RL_JOB_DIRECT_AO || d->kind == RL_JOB_INDIRECT) {
+        if (!d->lightmaps || d->num_lightmaps <= 0) return fail(RL_REJECTED, "job rejected: no lightmaps");
+        for (int32_t i = 0; i < d->num_lightmaps; i++) {
+            rl_lightmap* lm = d->lightmaps[i];
+            if (!lm || lm->scene != d->scene) return fail(RL_REJECTED, "job rejected: lightmap not in the job's scene");
+            j->maps.push_back(lm);
+            j->valid.push_back(valid_lumels(lm));
+        }
+    }
+    if (d->kind == RL_JOB_DIRECT_AO) {
+        if (d->ao) {
+            if (check_ao(d->ao)) return fail(RL_REJECTED, std::string("job rejected: ") + g_error);
+            j->ao = *d->ao;
+            j->has_ao = true;
+        }
+    } else if (d->kind == RL_JOB_INDIRECT) {
+        if (check_indirect_settings(d->indirect)) return fail(RL_REJECTED, std::string("job rejected: ") + g_error);
+        j->ind = *d->indirect;
+    } else if (d->kind == RL_JOB_RADIOSITY) {
+        if (!d->radiosity || !d->emission || !d->reflectance || !d->out_radiosity || d->num_rhs == 0 ||
+            d->num_rhs > 128 || d->radiosity->scene != d->scene)
+            return fail(RL_REJECTED, "job rejected: bad radiosity job");
+        const size_t n = (size_t)d->radiosity->n * d->num_rhs;
+        j->E.assign(d->emission, d->emission + n);
+        j->R.assign(d->reflectance, d->reflectance + n);
+    } else {
+        return fail(RL_REJECTED, "job rejected: unknown kind");
+    }
+    j->footprint = job_footprint(*j);
+    std::lock_guard<std::mutex> lk(s->m);
+    s->stats.submitted++;
+    if (j->footprint > s->cfg.memory_budget) {
+        s->stats.rejected++;
+        return fail(RL_REJECTED, "job rejected: footprint exceeds the memory budget");
+    }
+    if (s->stop) return fail(RL_INVALID_CALL, "server is stopping");
+    j->id = s->next_id++;
+    j->submit_ms = s->now_ms();
+    Job* raw = j.get();
+    s->jobs[raw->id] = std::move(j);
+    s->queue.push_back(raw);
+    if (out_id) *out_id = raw->id;
+    s->cv_work.notify_one();
+    return RL_SUCCESS;
+}
+
+static void fill_status(const Job* j, rl_job_status* out) {
+    if (j->state == RL_JOB_FAILED) g_error = j->error;
+    out->state = j->state;
+    out->status = j->status;
+    out->submit_ms = j->submit_ms;
+    out->admit_ms = j->admit_ms;
+    out->done_ms = j->done_ms;
+    out->rays = j->rays;
+    out->footprint = j->footprint;
+}
+
+int32_t rl_server_poll(rl_server* s, uint64_t id, rl_job_status* out) {
+    if (!s || !out) return fail(RL_INVALID_ARGUMENT, "NULL argument");
+    std::lock_guard<std::mutex> lk(s->m);
+    auto it = s->jobs.find(id);
+    if (it == s->jobs.end()) return fail(RL_INVALID_ARGUMENT, "unknown job id");
+    fill_status(it->second.get(), out);
+    return RL_SUCCESS;
+}
+
+int32_t rl_server_wait(rl_server* s, uint64_t id, double timeout_ms, rl_job_status* out) {
+    if (!s) return fail(RL_INVALID_ARGUMENT, "NULL argument");
+    std::unique_lock<std::mutex> lk(s->m);
+    auto it = s->jobs.find(id);
+    if (it == s->jobs.end()) return fail(RL_INVALID_ARGUMENT, "unknown job id");
+    Job* j = it->second.get();
+    auto done = [j]() { return job_finished(j); };
+    if (timeout_ms < 0) {
+        s->cv_done.wait(lk, done);
+    } else if (!s->cv_done.wait_for(lk, std::chrono::duration<double, std::milli>(timeout_ms), done)) {
+        if (out) fill_status(j, out);
+        return fail(RL_REJECTED, "timed out waiting for the job");
+    }
+    if (out) fill_status(j, out);
+    return RL_SUCCESS;
+}
+
+int32_t rl_server_cancel(rl_server* s, uint64_t id) {
+    if (!s) return fail(RL_INVALID_ARGUMENT, "NULL argument");
+    std::lock_guard<std::mutex> lk(s->m);
+    auto it = s->jobs.find(id);
+    if (it == s->jobs.end()) return fail(RL_INVALID_ARGUMENT, "unknown job id");
+    it->second->cancel = true;
+    s->cv_work.notify_one();
+    return RL_SUCCESS;
+}
+
+int32_t rl_server_stats_get(rl_server* s, rl_server_stats* out) {
+    if (!s || !out) return fail(RL_INVALID_ARGUMENT, "NULL argument");
+    std::lock_guard<std::mutex> lk(s->m);
+    *out = s->stats;
+    return RL_SUCCESS;
+}
+
+int32_t rl_server_destroy(rl_server* s) {
+    if (!s) return RL_SUCCESS;
+    {
+        std::lock_guard<std::mutex> lk(s->m);
+        s->stop = true;
+        for (Job* j : s->queue) j->cancel = true;
+        s->cv_work.notify_all();
+    }
+    s->worker.join();
+    // Jobs still queued when the loop stopped were never admitted.
+    for (Job* j : s->queue) {
+        j->state = RL_JOB_CANCELLED;
+        s->stats.cancelled++;
+    }
+    delete s;
     return RL_SUCCESS;
 }
 

@@ -347,6 +347,7 @@ class Radiosity:
     def __init__(self, relight: "Relight", scene: Scene, patches, tri_patch):
         self._relight = relight
         self._lib = relight._lib
+        self.scene = scene
         p = np.ascontiguousarray(patches, np.float32).reshape(-1, 12)
         t = np.ascontiguousarray(tri_patch, np.uint32).reshape(-1)
         self.num_patches = p.shape[0]
@@ -468,6 +469,22 @@ class Relight:
                                                   1 if direct else 0,
                                                   C.byref(s) if s is not None else None))
 
+    def bake_items(self, items):
+        """One multi-scene launch per pass over (scene, lightmap, first, count, direct, ao) items
+        (count None: to the end of the lightmap); equals the per-lightmap bakes."""
+        keep = []
+        arr = (A.rl_bake_item * len(items))()
+        for k, (scene, lm, first, count, direct, ao) in enumerate(items):
+            s = ao.to_c() if ao is not None else None
+            keep.append(s)
+            arr[k] = A.rl_bake_item(scene._h.value, lm._h.value, int(first),
+                                    0xFFFFFFFF if count is None else int(count), 1 if direct else 0,
+                                    C.pointer(s) if s is not None else None)
+        _check(self._lib, self._lib.rl_bake_items(self._h, arr, len(items)))
+
+    def create_server(self, memory_budget=64 << 30, max_active=16, quantum_rays=1 << 28) -> "Server":
+        return Server(self, memory_budget, max_active, quantum_rays)
+
     def stats(self) -> dict:
         st = A.rl_stats()
         _check(self._lib, self._lib.rl_get_stats(self._h, C.byref(st)))
@@ -475,3 +492,118 @@ class Relight:
                    for i, name in A.KERNEL_NAMES.items() if st.kernel_launches[i]}
         return {"rays": st.rays, "texels": st.texels, "gpu_ms": st.gpu_ms,
                 "wall_ms": st.wall_ms, "launches": st.launches, "kernels": kernels}
+
+
+class BakeJob:
+    """A job submitted to a Server; holds what must outlive it (the radiosity output array)."""
+
+    def __init__(self, server: "Server", job_id: int, kind: int, out=None):
+        self.server = server
+        self.id = job_id
+        self.kind = kind
+        self.result = out
+
+    @staticmethod
+    def _status(st: A.rl_job_status) -> dict:
+        return {"state": A.JOB_STATE_NAMES[st.state], "status": st.status, "submit_ms": st.submit_ms,
+                "admit_ms": st.admit_ms, "done_ms": st.done_ms, "rays": st.rays,
+                "footprint": st.footprint}
+
+    def poll(self) -> dict:
+        st = A.rl_job_status()
+        _check(self.server._lib, self.server._lib.rl_server_poll(self.server._h, self.id, C.byref(st)))
+        return self._status(st)
+
+    def wait(self, timeout: Optional[float] = None) -> dict:
+        """Blocks until the job finishes (timeout in seconds); raises RelightError if it failed
+        and TimeoutError if it is still running at the timeout."""
+        lib = self.server._lib
+        st = A.rl_job_status()
+        rc = lib.rl_server_wait(self.server._h, self.id, -1.0 if timeout is None else 1e3 * timeout,
+                                C.byref(st))
+        if rc == A.RL_REJECTED and st.state in (A.RL_JOB_QUEUED, A.RL_JOB_RUNNING):
+            raise TimeoutError(f"job {self.id} still {A.JOB_STATE_NAMES[st.state]}")
+        _check(lib, rc)
+        if st.state == A.RL_JOB_FAILED:
+            raise RelightError(st.status, lib.rl_last_error().decode())
+        return self._status(st)
+
+    def cancel(self):
+        _check(self.server._lib, self.server._lib.rl_server_cancel(self.server._h, self.id))
+
+
+class Server:
+    """Continuous-batching bake server (no counterpart in the reference; it replaces the
+    Worker/ThreadWorker job loop of src/relight/Worker.cpp for serving). A native thread admits
+    queued jobs under a device-memory budget and bakes the direct/AO work of every admitted job
+    in shared launches of quantum_rays rays; results equal the same job baked alone. While the
+    server runs, make no other call on its Relight context."""
+
+    def __init__(self, relight: Relight, memory_budget=64 << 30, max_active=16, quantum_rays=1 << 28):
+        self._relight = relight
+        self._lib = relight._lib
+        cfg = A.rl_server_config(int(memory_budget), int(max_active), int(quantum_rays))
+        h = C.c_void_p()
+        _check(self._lib, self._lib.rl_server_create(relight._h, C.byref(cfg), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            self._lib.rl_server_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _submit(self, d: A.rl_job_desc, out=None) -> BakeJob:
+        jid = C.c_uint64()
+        _check(self._lib, self._lib.rl_server_submit(self._h, C.byref(d), C.byref(jid)))
+        return BakeJob(self, jid.value, d.kind, out)
+
+    @staticmethod
+    def _maps(lightmaps):
+        return (C.c_void_p * len(lightmaps))(*[lm._h.value for lm in lightmaps])
+
+    def submit_direct_ao(self, scene: Scene, lightmaps: Sequence[Lightmap], direct: bool = True,
+                         ao: Optional[AmbientOcclusionSettings] = None) -> BakeJob:
+        maps = self._maps(lightmaps)
+        s = ao.to_c() if ao is not None else None
+        d = A.rl_job_desc(kind=A.RL_JOB_DIRECT_AO, scene=scene._h.value, lightmaps=maps,
+                          num_lightmaps=len(lightmaps), direct=1 if direct else 0,
+                          ao=C.pointer(s) if s is not None else None)
+        return self._submit(d)
+
+    def submit_indirect(self, scene: Scene, lightmaps: Sequence[Lightmap],
+                        settings: IndirectLightSettings) -> BakeJob:
+        maps = self._maps(lightmaps)
+        s = settings.to_c()
+        d = A.rl_job_desc(kind=A.RL_JOB_INDIRECT, scene=scene._h.value, lightmaps=maps,
+                          num_lightmaps=len(lightmaps), indirect=C.pointer(s))
+        return self._submit(d)
+
+    def submit_radiosity(self, radiosity: Radiosity, emission, reflectance, rays_per_patch=1024,
+                         seed=1, max_distance=1000.0, tolerance=1e-5) -> BakeJob:
+        E = np.ascontiguousarray(emission, np.float32)
+        R = np.ascontiguousarray(reflectance, np.float32)
+        assert E.shape == R.shape and E.shape[0] == radiosity.num_patches
+        out = np.zeros_like(E)
+        d = A.rl_job_desc(kind=A.RL_JOB_RADIOSITY, scene=radiosity.scene._h.value,
+                          radiosity=radiosity._h.value, rays_per_patch=int(rays_per_patch),
+                          ff_seed=int(seed) & 0xFFFFFFFF, ff_max_distance=float(max_distance),
+                          emission=_fptr(E), reflectance=_fptr(R), num_rhs=E.shape[1],
+                          tolerance=float(tolerance), out_radiosity=_fptr(out))
+        return self._submit(d, out)
+
+    def stats(self) -> dict:
+        st = A.rl_server_stats()
+        _check(self._lib, self._lib.rl_server_stats_get(self._h, C.byref(st)))
+        return {name: getattr(st, name) for name, _ in A.rl_server_stats._fields_}

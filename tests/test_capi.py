@@ -41,9 +41,43 @@ def test_bindings_cover_the_header():
     assert {name for name, _, _ in _capi.SIGNATURES} == header_symbols()
 
 
-def test_struct_layouts_match_header():
-    # rl_light_desc: 2 ints, 3 float[3], intensity, has_attenuation, 6 floats, 2 pointers, 2 uints.
-    assert ctypes.sizeof(_capi.rl_light_desc) == 4 * 2 + 36 + 4 + 4 + 24 + 4 + 16 + 8
-    assert ctypes.sizeof(_capi.rl_ao_settings) == 24
-    assert ctypes.sizeof(_capi.rl_hit) == 24
-    assert ctypes.sizeof(_capi.rl_stats) == 40 + 16 * 8 + 4 * 16
+# Every struct of the header with its ctypes mirror; the C compiler is the judge of the layout.
+_STRUCTS = ["rl_light_desc", "rl_ao_settings", "rl_hit", "rl_indirect_settings", "rl_tuning",
+            "rl_solve_info", "rl_stats", "rl_bake_item", "rl_server_config", "rl_job_desc",
+            "rl_job_status", "rl_server_stats"]
+
+
+def test_struct_layouts_match_header(tmp_path):
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "relight.h"', "int main(void) {"]
+    for name in _STRUCTS:
+        py = getattr(_capi, name)
+        lines.append(f'printf("{name} %zu\\n", sizeof({name}));')
+        for field, _ in py._fields_:
+            lines.append(f'printf("{name}.{field} %zu\\n", offsetof({name}, {field}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.dirname(HEADER), str(src), "-o", str(exe)],
+                   check=True)
+    got = dict(l.rsplit(" ", 1) for l in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                         check=True).stdout.splitlines())
+    for name in _STRUCTS:
+        py = getattr(_capi, name)
+        assert int(got[name]) == ctypes.sizeof(py), name
+        for field, _ in py._fields_:
+            assert int(got[f"{name}.{field}"]) == getattr(py, field).offset, f"{name}.{field}"
+
+def test_argument_errors_without_a_device():
+    """Entry points reject NULL handles before any device work (callable without a GPU)."""
+    if not os.path.exists(_capi.LIB_PATH):
+        pytest.skip("librelight_amd.so not built")
+    lib = _capi.load()
+    cfg = _capi.rl_server_config(1 << 30, 4, 1 << 20)
+    out = ctypes.c_void_p()
+    assert lib.rl_server_create(None, ctypes.byref(cfg), ctypes.byref(out)) == _capi.RL_INVALID_ARGUMENT
+    assert b"NULL" in lib.rl_last_error()
+    assert lib.rl_bake_items(None, None, 0) == _capi.RL_INVALID_ARGUMENT
+    assert lib.rl_server_destroy(None) == _capi.RL_SUCCESS
+    st = _capi.rl_job_status()
+    assert lib.rl_server_poll(None, 1, ctypes.byref(st)) == _capi.RL_INVALID_ARGUMENT
