@@ -32,8 +32,8 @@ def load_traffic():
         return json.load(f)
 
 
-def load_pmc():
-    path = os.path.join(HERE, "profiles", "ao_count_city_pmc.json")
+def load_pmc(kernel):
+    path = os.path.join(HERE, "profiles", f"{kernel}_city_pmc.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -81,7 +81,7 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--plain-steps", type=int, default=2)
-    ap.add_argument("--cpu-stride", type=int, default=128)
+    ap.add_argument("--cpu-stride", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ao-samples", type=int, default=256)
     ap.add_argument("--no-autotune", action="store_true")
@@ -171,12 +171,20 @@ def main():
         avg_s = ao_ms / ao_launch / 1e3
         rays_per_launch = ao_rays / ao_launch
         achieved = rays_per_launch * bpr / avg_s / 1e9
-        pmc = load_pmc()
+        kernel = "k_ao_refill" if relight.tuning()["ao_kernel"] == 1 else "k_ao_count"
+        pmc = load_pmc(kernel)
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": achieved / HBM_PEAK_GBS,
                     "traffic": pmc["hbm_bytes_per_launch"] if pmc else None,
-                    "kernel": "k_ao_count", "avg_launch_ms": avg_s * 1e3,
+                    "kernel": kernel, "avg_launch_ms": avg_s * 1e3,
                     "bytes_per_ray": bpr, "rays_per_launch": rays_per_launch}
+        if pmc:
+            # Measured on the same workload: physical HBM rate, L2 hit rate and VALU issue rate
+            # (wave instructions per SIMD-cycle; 0.25 is the wave64 issue ceiling).
+            sq = pmc["sq"]
+            roofline["hbm_gbs_measured"] = pmc["hbm_bytes_per_launch"] / avg_s / 1e9
+            roofline["l2_hit_rate"] = pmc["l2_hit_rate"]
+            roofline["valu_issue_per_simd_cycle"] = sq["SQ_INSTS_VALU"] / 1024 / (sq["GRBM_GUI_ACTIVE"] / 8)
 
     cpu = None
     rmse = None
