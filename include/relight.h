@@ -105,9 +105,17 @@ typedef struct rl_hit {
 #define RL_PHOTON_THRESHOLD 0
 #define RL_PHOTON_RUSSIAN_ROULETTE 1
 
+/* Photonmap gather: the reference's texel disc (Photonmap::gather, Lightmap.cpp:376-413), or the
+ * k nearest stored photons in 3D (hash grid) around each photonmap texel's surface point, within
+ * knn_radius, normalised as the reference does (sum of power / number of photons). */
+#define RL_GATHER_TEXEL_DISC 0
+#define RL_GATHER_KNN 1
+
 /* IndirectLightSettings (Relight.h:108-132) plus the sample-stream seed, the final-gather
  * hemisphere sampling, an optional photon count per light per pass (0: intensity * 25000 as
- * PhotonEmitter::photonCount, scene/Light.cpp:334-337) and the photon termination rule. */
+ * PhotonEmitter::photonCount, scene/Light.cpp:334-337), the photon termination rule and the
+ * gather rule (k-NN: knn_k in 1..64 photons within knn_radius; the photons of one
+ * rl_emit_photons call). */
 typedef struct rl_indirect_settings {
     int32_t photon_pass_count;
     int32_t photon_bounce_count;
@@ -122,6 +130,9 @@ typedef struct rl_indirect_settings {
     int32_t sampling;
     int32_t photons_per_light;
     int32_t termination;
+    int32_t gather_mode;
+    int32_t knn_k;
+    float knn_radius;
 } rl_indirect_settings;
 
 /* Kernel slots of rl_stats.kernel_ms. */
@@ -136,6 +147,7 @@ typedef struct rl_indirect_settings {
 #define RL_KERNEL_FORM_FACTORS 8
 #define RL_KERNEL_FF_GEMM_ACCURATE 9
 #define RL_KERNEL_FF_GEMM_FAST 10
+#define RL_KERNEL_PHOTON_GRID 11
 #define RL_KERNEL_SLOTS 16
 
 /* Counters of the last bake or trace call on a context. */

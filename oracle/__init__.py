@@ -38,13 +38,14 @@ class or_indirect(C.Structure):
                 ("final_gather_samples", C.c_int32), ("final_gather_distance", C.c_float),
                 ("final_gather_radius", C.c_int32), ("sky_color", C.c_float * 3),
                 ("ambient_color", C.c_float * 3), ("seed", C.c_uint32), ("sampling", C.c_int32),
-                ("photons_per_light", C.c_int32), ("termination", C.c_int32)]
+                ("photons_per_light", C.c_int32), ("termination", C.c_int32),
+                ("gather_mode", C.c_int32), ("knn_k", C.c_int32), ("knn_radius", C.c_float)]
 
 
 def _indirect(s):
     o = or_indirect()
     for name, _ in or_indirect._fields_:
-        v = getattr(s, name)
+        v = getattr(s, name, 0)
         if name in ("sky_color", "ambient_color"):
             getattr(o, name)[:] = [float(x) for x in v]
         else:
@@ -102,7 +103,8 @@ def lib():
             "or_hemisphere": (None, [C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, _F, _F]),
             "or_splitmix64": (C.c_uint64, [C.c_uint64]),
             "or_emit_photons": (C.c_int32, [C.c_void_p, C.POINTER(or_indirect), _I32, C.c_int32, _I32, _I32, _U8,
-                                            _F, _U32, _F, C.c_int32, _U64]),
+                                            _F, _F, _U32, _F, C.c_int32, _U64]),
+            "or_photon_records": (C.c_int32, [C.c_void_p, _U64, _U32, _F, _F]),
             "or_final_gather": (C.c_int32, [C.c_void_p, C.c_uint32, _F, _F, _U32, C.c_uint32, C.POINTER(or_indirect),
                                             _I32, C.c_int32, _I32, _I32, _F, _F, C.c_int32, _U64]),
             "or_form_factors": (C.c_int32, [C.c_void_p, _F, _U32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float,
@@ -257,7 +259,7 @@ class OracleScene:
         order, mesh_pm = photon_layout(spec)
         ws = np.asarray([spec.photonmaps[k][0] for k in order], np.int32)
         hs = np.asarray([spec.photonmaps[k][1] for k in order], np.int32)
-        valid = []
+        valid, ppos = [], []
         for k in order:
             w, h = spec.photonmaps[k]
             grids = None
@@ -265,8 +267,10 @@ class OracleScene:
                 if getattr(m, "photonmap", -1) == k:
                     grids = self.raster(mi, w, h, grids)
             valid.append(grids[0])
+            ppos.append(grids[2])
         n = int(sum(int(w) * int(h) for w, h in zip(ws, hs)))
         valid = np.concatenate(valid) if valid else np.zeros(1, np.uint8)
+        ppos = np.ascontiguousarray(np.concatenate(ppos) if ppos else np.zeros((1, 3)), np.float32)
         color = np.zeros(max(n, 1) * 3, np.float32)
         photons = np.zeros(max(n, 1), np.uint32)
         gathered = np.zeros(max(n, 1) * 3, np.float32)
@@ -274,7 +278,7 @@ class OracleScene:
         rays = C.c_uint64()
         self._L.or_emit_photons(self._h, C.byref(s), mesh_pm.ctypes.data_as(_I32), len(order),
                                 ws.ctypes.data_as(_I32), hs.ctypes.data_as(_I32), valid.ctypes.data_as(_U8),
-                                _f(color), _u(photons), _f(gathered), threads, C.byref(rays))
+                                _f(ppos), _f(color), _u(photons), _f(gathered), threads, C.byref(rays))
         out, base = {}, 0
         for k, w, h in zip(order, ws, hs):
             sz = int(w) * int(h)
@@ -283,6 +287,17 @@ class OracleScene:
                       gathered[3 * base:3 * (base + sz)].reshape(h, w, 3))
             base += sz
         return out, rays.value
+
+    def photon_records(self):
+        """Records of the last emit_photons call in slot order: (key, power (n,3), hit point (n,3));
+        key 0xFFFFFFFF marks an empty slot."""
+        n = C.c_uint64()
+        self._L.or_photon_records(self._h, C.byref(n), None, None, None)
+        key = np.zeros(n.value, np.uint32)
+        power = np.zeros((n.value, 3), np.float32)
+        pos = np.zeros((n.value, 3), np.float32)
+        self._L.or_photon_records(self._h, C.byref(n), _u(key), _f(power), _f(pos))
+        return key, power, pos
 
     def final_gather(self, spec, pos, nrm, texel, lm_seed, settings, maps, color, threads=0):
         """IndirectLight::bakeLumel over lumels (color in/out); maps from emit_photons."""

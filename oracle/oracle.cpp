@@ -10,6 +10,11 @@
 //   area light vertices       scene/Light.cpp:293-323, scene/Mesh.cpp:322-327 (centroids)
 //   ambient occlusion         baker/AmbientOcclusion.cpp:46-66
 //   hemisphere sampling       Types.h:924-949 (uniform, with PCG32 instead of rand())
+//   photons                   baker/Photons.cpp:50-144, scene/Light.cpp:334-361, Relight.cpp:244-257
+//   photonmap gather          Lightmap.cpp:376-413 (texel disc); k-nearest photons in 3D with the
+//                             same normalisation (the configuration's hash-grid gather)
+//   final gather              baker/IndirectLight.cpp:47-75
+//   form factors              hemisphere-ray estimator of radiosity/RadiosityBuilder.cpp:142-176
 // Project-defined parts shared by contract with the GPU path (not in the reference): PCG32 streams
 // keyed by (seed, lightmap seed, texel, technique, sample); cosine sampling (Malley + Duff et al.
 // 2017 basis); the watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2013); closest-hit ties
@@ -18,7 +23,10 @@
 // Pinning (tests/test_oracle.py): PCG32 against the published pcg32-demo output and an
 // independent Python integer model; sampler moments; ray/triangle known answers and shared-edge
 // watertightness; BVH == brute force; direct light closed forms; AO closed-box and open-plane
-// cases. Compiled with -ffp-contract=off so every expression rounds as written.
+// cases; first-bounce photon energy; texel-disc gather by hand; k-nearest gather against a
+// numpy brute force; final gather under a constant sky in closed form; form factors against
+// Howell's parallel and perpendicular plate formulas and closed-enclosure row sums.
+// Compiled with -ffp-contract=off so every expression rounds as written.
 #include <stdint.h>
 #include <math.h>
 #include <string.h>
@@ -27,6 +35,7 @@
 #include <atomic>
 #include <cfloat>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 namespace {
@@ -215,6 +224,9 @@ struct Scene {
     std::vector<BNode> nodes;
     std::vector<uint32_t> order;
     bool use_bvh = true;
+    // Records of the last photon emission (slot order): texel key (0xFFFFFFFF: none), power, hit point.
+    std::vector<uint32_t> last_key;
+    std::vector<float> last_power, last_pos;
 };
 
 struct Counters {
@@ -760,6 +772,8 @@ struct or_indirect {
     float sky_color[3], ambient_color[3];
     uint32_t seed;
     int32_t sampling, photons_per_light, termination;
+    int32_t gather_mode, knn_k;
+    float knn_radius;
 };
 
 // Photonmaps are passed concatenated: map k covers keys [base_k, base_k + w_k*h_k) of the
@@ -796,15 +810,81 @@ static inline int lumel_xy(float uv, int size) {
 struct PRec {
     uint32_t key;
     float r, g, b;
+    float x, y, z;
 };
+
+// k-nearest gather (gather_mode 1): for every valid photonmap texel, the k stored photons nearest
+// to its surface point (squared distance ex*ex + ey*ey + ez*ez, e = photon - point) within the
+// radius, ordered by (squared distance, emission slot); their power summed in that order and
+// divided by their number, as Photonmap::gather normalises (Lightmap.cpp:407-412). A uniform grid
+// of cell 1.01 * radius holds every photon within the radius in the 27 cells around a point.
+static void knn_gather(const std::vector<PRec>& rec, PMaps& M, const float* pm_pos, int k, float radius,
+                       int threads) {
+    std::vector<uint32_t> ids;
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX};
+    for (uint32_t i = 0; i < (uint32_t)rec.size(); i++) {
+        if (rec[i].key == 0xFFFFFFFFu) continue;
+        ids.push_back(i);
+        lo[0] = std::min(lo[0], rec[i].x); lo[1] = std::min(lo[1], rec[i].y); lo[2] = std::min(lo[2], rec[i].z);
+    }
+    const float cell = radius * 1.01f, r2 = radius * radius;
+    auto cell_of = [&](float x, float y, float z, int64_t* c) {
+        c[0] = (int64_t)floor((double)(x - lo[0]) / cell);
+        c[1] = (int64_t)floor((double)(y - lo[1]) / cell);
+        c[2] = (int64_t)floor((double)(z - lo[2]) / cell);
+    };
+    auto pack = [](int64_t a, int64_t b, int64_t c) {
+        return ((uint64_t)(a + (1 << 20)) << 42) | ((uint64_t)(b + (1 << 20)) << 21) | (uint64_t)(c + (1 << 20));
+    };
+    std::unordered_map<uint64_t, std::vector<uint32_t>> grid;
+    for (uint32_t i : ids) {
+        int64_t c[3];
+        cell_of(rec[i].x, rec[i].y, rec[i].z, c);
+        grid[pack(c[0], c[1], c[2])].push_back(i);
+    }
+    uint32_t total = 0;
+    for (int m = 0; m < M.n; m++) total += (uint32_t)(M.w[m] * M.h[m]);
+    parallel_for(total, threads, [&](uint32_t t, int) {
+        float* out = &M.gathered[3 * (size_t)t];
+        out[0] = out[1] = out[2] = 0.0f;
+        if (!M.valid[t] || ids.empty()) return;
+        const float qx = pm_pos[3 * (size_t)t], qy = pm_pos[3 * (size_t)t + 1], qz = pm_pos[3 * (size_t)t + 2];
+        int64_t c[3];
+        cell_of(qx, qy, qz, c);
+        std::vector<std::pair<float, uint32_t>> cand;
+        for (int dz = -1; dz <= 1; dz++)
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dx = -1; dx <= 1; dx++) {
+                    auto it = grid.find(pack(c[0] + dx, c[1] + dy, c[2] + dz));
+                    if (it == grid.end()) continue;
+                    for (uint32_t i : it->second) {
+                        const float ex = rec[i].x - qx, ey = rec[i].y - qy, ez = rec[i].z - qz;
+                        const float d2 = ex * ex + ey * ey + ez * ez;
+                        if (d2 <= r2) cand.push_back({d2, i});
+                    }
+                }
+        std::sort(cand.begin(), cand.end());
+        const size_t n = std::min<size_t>(cand.size(), (size_t)k);
+        if (n == 0) return;
+        float sr = 0.0f, sg = 0.0f, sb = 0.0f;
+        for (size_t j = 0; j < n; j++) {
+            sr += rec[cand[j].second].r;
+            sg += rec[cand[j].second].g;
+            sb += rec[cand[j].second].b;
+        }
+        out[0] = sr / (float)n;
+        out[1] = sg / (float)n;
+        out[2] = sb / (float)n;
+    });
+}
 
 // Photons::emit / trace / store (baker/Photons.cpp:50-144) with PhotonEmitter::emit,
 // DirectionalPhotonEmitter::emit (scene/Light.cpp:339-361) and LightSpotCutoff, then
 // Photonmap::gather (Lightmap.cpp:376-413) on every map.
 int32_t or_emit_photons(void* sp, const or_indirect* s, const int32_t* mesh_pm, int32_t npm, const int32_t* pw,
-                        const int32_t* ph, const uint8_t* valid, float* color, uint32_t* photons, float* gathered,
-                        int32_t threads, uint64_t* rays_out) {
-    const Scene& S = *(Scene*)sp;
+                        const int32_t* ph, const uint8_t* valid, const float* pm_pos, float* color,
+                        uint32_t* photons, float* gathered, int32_t threads, uint64_t* rays_out) {
+    Scene& S = *(Scene*)sp;
     PMaps M = make_pmaps(npm, pw, ph, valid, color, photons, gathered);
     const int nl = (int)S.lights.size();
     std::vector<uint64_t> first(nl);
@@ -819,7 +899,7 @@ int32_t or_emit_photons(void* sp, const or_indirect* s, const int32_t* mesh_pm, 
     float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
     for (size_t i = 0; i < S.tri.size(); i += 3)
         for (int k = 0; k < 3; k++) { lo[k] = std::min(lo[k], S.tri[i + k]); hi[k] = std::max(hi[k], S.tri[i + k]); }
-    std::vector<PRec> rec((size_t)total * (D + 1), PRec{0xFFFFFFFFu, 0, 0, 0});
+    std::vector<PRec> rec((size_t)total * (D + 1), PRec{0xFFFFFFFFu, 0, 0, 0, 0, 0, 0});
     std::vector<uint64_t> rays(256, 0);
     if (threads <= 0) threads = (int)std::max(1u, std::thread::hardware_concurrency());
     threads = std::min(threads, 255);
@@ -873,7 +953,8 @@ int32_t or_emit_photons(void* sp, const or_indirect* s, const int32_t* mesh_pm, 
             if (pm >= 0) {
                 const int x = lumel_xy(ux, M.w[pm]), y = lumel_xy(uy, M.h[pm]);
                 const uint32_t t = (uint32_t)(y * M.w[pm] + x);
-                if (M.valid[M.base[pm] + t]) rec[(size_t)p * (D + 1) + depth] = PRec{M.base[pm] + t, hr, hg, hb};
+                if (M.valid[M.base[pm] + t])
+                    rec[(size_t)p * (D + 1) + depth] = PRec{M.base[pm] + t, hr, hg, hb, hp.x, hp.y, hp.z};
             }
             if (s->termination == 1) {
                 const float q = (A[0] + A[1] + A[2]) / 3.0f;
@@ -894,7 +975,22 @@ int32_t or_emit_photons(void* sp, const or_indirect* s, const int32_t* mesh_pm, 
         M.color[3 * (size_t)q.key + 2] += q.b;
         M.photons[q.key]++;
     }
+    S.last_key.resize(rec.size());
+    S.last_power.resize(3 * rec.size());
+    S.last_pos.resize(3 * rec.size());
+    for (size_t i = 0; i < rec.size(); i++) {
+        S.last_key[i] = rec[i].key;
+        S.last_power[3 * i] = rec[i].r; S.last_power[3 * i + 1] = rec[i].g; S.last_power[3 * i + 2] = rec[i].b;
+        S.last_pos[3 * i] = rec[i].x; S.last_pos[3 * i + 1] = rec[i].y; S.last_pos[3 * i + 2] = rec[i].z;
+    }
     const int R = s->final_gather_radius;
+    if (s->gather_mode == 1) {
+        knn_gather(rec, M, pm_pos, s->knn_k, s->knn_radius, threads);
+        uint64_t tot = 0;
+        for (uint64_t x : rays) tot += x;
+        if (rays_out) *rays_out = tot;
+        return 0;
+    }
     for (int k = 0; k < npm; k++) {
         const int W = M.w[k], H = M.h[k];
         for (int y = 0; y < H; y++) {
@@ -925,6 +1021,17 @@ int32_t or_emit_photons(void* sp, const or_indirect* s, const int32_t* mesh_pm, 
     uint64_t tot = 0;
     for (uint64_t x : rays) tot += x;
     if (rays_out) *rays_out = tot;
+    return 0;
+}
+
+// Records of the last or_emit_photons call in slot order (photon-major, bounce-minor): count, then
+// key / power (3) / hit point (3) when the arrays are given.
+int32_t or_photon_records(void* sp, uint64_t* n, uint32_t* key, float* power, float* pos) {
+    const Scene& S = *(Scene*)sp;
+    *n = S.last_key.size();
+    if (key) memcpy(key, S.last_key.data(), S.last_key.size() * 4);
+    if (power) memcpy(power, S.last_power.data(), S.last_power.size() * 4);
+    if (pos) memcpy(pos, S.last_pos.data(), S.last_pos.size() * 4);
     return 0;
 }
 

@@ -97,7 +97,14 @@ class rl_indirect_settings(C.Structure):
         ("sampling", C.c_int32),
         ("photons_per_light", C.c_int32),
         ("termination", C.c_int32),
+        ("gather_mode", C.c_int32),
+        ("knn_k", C.c_int32),
+        ("knn_radius", C.c_float),
     ]
+
+
+RL_GATHER_TEXEL_DISC = 0
+RL_GATHER_KNN = 1
 
 
 RL_KERNEL_DIRECT = 0
@@ -114,7 +121,7 @@ RL_KERNEL_FF_GEMM_FAST = 10
 RL_KERNEL_SLOTS = 16
 KERNEL_NAMES = {0: "direct", 1: "ao_count", 2: "ao_finalize", 3: "trace", 4: "photon_trace",
                 5: "photon_store", 6: "photon_gather", 7: "final_gather", 8: "form_factors",
-                9: "ff_gemm_accurate", 10: "ff_gemm_fast"}
+                9: "ff_gemm_accurate", 10: "ff_gemm_fast", 11: "photon_grid"}
 RL_NOT_CONVERGED = 7
 
 

@@ -126,6 +126,21 @@ struct PhotonRec {
     float r, g, b;
 };
 
+// Hash grid of stored photons for the k-nearest gather: cell = floor((p - lo) * inv_h) per axis,
+// bucket = spatial hash of the cell & mask. Cell size h = radius / 1.98, so every photon within
+// the radius lies in the 5x5x5 cells around a query's cell (max_shell = 2), with a margin for
+// rounding; shell_d2[s] = ((s - 0.01) h)^2 lets a query stop after shell s once its k-th
+// distance is below it (unvisited photons are farther).
+struct KnnGrid {
+    float lo[3];
+    float inv_h;
+    uint32_t mask;
+    int32_t max_shell;
+    int32_t k;
+    float r2;
+    float shell_d2[4];
+};
+
 // A radiosity patch: the quad origin + s*U + t*V (s, t in [0,1]) with normal n.
 struct PatchDev {
     float o[3], u[3], v[3], n[3];

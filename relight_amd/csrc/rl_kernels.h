@@ -59,14 +59,25 @@ hipError_t launch_ff_apply(hipStream_t st, uint32_t npad, uint32_t mpad, float* 
 hipError_t launch_photon_trace(hipStream_t st, const SceneDev& S, int cap, uint32_t max_blocks,
                                const LightDev* lights, const uint64_t* light_first,
                                const PhotonParams& P, const int32_t* mesh_pm,
-                               const PhotonMapDev* pms, PhotonRec* out, unsigned long long* queue,
-                               unsigned long long* rays);
+                               const PhotonMapDev* pms, PhotonRec* out, float4* out_pos,
+                               unsigned long long* queue, unsigned long long* rays);
 hipError_t photon_sort_temp_bytes(uint32_t n, size_t* bytes);
 hipError_t launch_photon_store(hipStream_t st, const PhotonRec* rec, uint32_t n, uint32_t* keys_a,
                                uint32_t* keys_b, uint32_t* idx_a, uint32_t* idx_b, void* temp,
                                size_t temp_bytes, const PhotonMapDev* pms, int npm);
 hipError_t launch_photon_gather(hipStream_t st, const PhotonMapDev* pms, int npm,
                                 uint32_t total_texels, int radius);
+// k-nearest gather: hash grid over the stored records (key_bits = log2(mask + 1) + 1), then one
+// wavefront per query (float4: surface point, w = global texel key bits).
+hipError_t knn_sort_temp_bytes(uint32_t n, int key_bits, size_t* bytes);
+hipError_t launch_knn_build(hipStream_t st, const PhotonRec* rec, const float4* ppos, uint32_t n,
+                            const KnnGrid& G, int key_bits, uint32_t* keys_a, uint32_t* keys_b, uint32_t* idx_a,
+                            uint32_t* idx_b, void* temp, size_t temp_bytes, float4* spos, uint32_t* cstart,
+                            uint32_t* cend);
+hipError_t launch_knn_gather(hipStream_t st, uint32_t max_blocks, const float4* queries, uint32_t nq,
+                             const float4* spos, const PhotonRec* rec, const uint32_t* cstart,
+                             const uint32_t* cend, const KnnGrid& G, const PhotonMapDev* pms, int npm,
+                             unsigned long long* queue);
 hipError_t launch_final_gather(hipStream_t st, const SceneDev& S, int cap, uint32_t max_blocks,
                                const BakeSeg* segs, int nsegs, uint64_t total_lumels,
                                const FinalGatherParams& P, const int32_t* mesh_pm,

@@ -59,6 +59,7 @@ __global__ __launch_bounds__(kBlock) void k_photon_trace(SceneDev S, const Light
                                                          PhotonParams P, const int32_t* __restrict__ mesh_pm,
                                                          const PhotonMapDev* __restrict__ pms,
                                                          PhotonRec* __restrict__ out,
+                                                         float4* __restrict__ out_pos,
                                                          unsigned long long* queue,
                                                          unsigned long long* rays_out) {
     __shared__ uint32_t stack[kLdsStack * kBlock];
@@ -131,6 +132,7 @@ __global__ __launch_bounds__(kBlock) void k_photon_trace(SceneDev S, const Light
                         rec.key = M.key_base + t;
                         rec.r = hr; rec.g = hg; rec.b = hb;
                         slot[depth] = rec;
+                        if (out_pos) out_pos[p * (uint64_t)(P.max_depth + 1) + depth] = make_float4(hp.x, hp.y, hp.z, 0.0f);
                     }
                 }
                 if (P.termination == PHOTON_RUSSIAN_ROULETTE) {
@@ -290,6 +292,158 @@ __global__ __launch_bounds__(kBlock) void k_final_gather(SceneDev S, const BakeS
     count_rays(rays, rays_out);
 }
 
+// ---------------------------------------------------------------- k-nearest photon gather
+
+__device__ __forceinline__ int knn_axis(float x, float lo, float inv_h) { return (int)floorf((x - lo) * inv_h); }
+
+__device__ __forceinline__ uint32_t knn_bucket(const KnnGrid& G, int cx, int cy, int cz) {
+    return (((uint32_t)cx * 73856093u) ^ ((uint32_t)cy * 19349663u) ^ ((uint32_t)cz * 83492791u)) & G.mask;
+}
+
+// Bucket key of every record slot (mask + 1 = empty slot, sorted last) and the slot index.
+__global__ void k_knn_keys(const PhotonRec* __restrict__ rec, const float4* __restrict__ ppos, uint32_t n,
+                           KnnGrid G, uint32_t* __restrict__ keys, uint32_t* __restrict__ idx) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    uint32_t key = G.mask + 1u;
+    if (rec[g].key != 0xFFFFFFFFu) {
+        const float4 p = ppos[g];
+        key = knn_bucket(G, knn_axis(p.x, G.lo[0], G.inv_h), knn_axis(p.y, G.lo[1], G.inv_h),
+                         knn_axis(p.z, G.lo[2], G.inv_h));
+    }
+    keys[g] = key;
+    idx[g] = g;
+}
+
+// Photons in bucket order (position, w = slot bits) and each bucket's [start, end) range.
+__global__ void k_knn_scatter(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ idx, uint32_t n,
+                              const float4* __restrict__ ppos, KnnGrid G, float4* __restrict__ spos,
+                              uint32_t* __restrict__ cstart, uint32_t* __restrict__ cend) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t b = keys[i];
+    if (b > G.mask) return;
+    const uint32_t slot = idx[i];
+    const float4 p = ppos[slot];
+    spos[i] = make_float4(p.x, p.y, p.z, __uint_as_float(slot));
+    if (i == 0 || keys[i - 1] != b) cstart[b] = i;
+    if (i + 1 == n || keys[i + 1] != b) cend[b] = i + 1;
+}
+
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
+    const uint32_t lo = __shfl((uint32_t)v, src, 64), hi = __shfl((uint32_t)(v >> 32), src, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
+    const uint32_t lo = __shfl_xor((uint32_t)v, m, 64), hi = __shfl_xor((uint32_t)(v >> 32), m, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Bitonic sort of one key per lane, ascending by lane.
+__device__ __forceinline__ uint64_t wave_sort(uint64_t v, int lane) {
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const uint64_t o = shfl_xor_u64(v, j);
+            const bool up = (lane & k) == 0, low = (lane & j) == 0;
+            v = (low == up) ? (v < o ? v : o) : (v < o ? o : v);
+        }
+    }
+    return v;
+}
+
+// The 64 smallest of two ascending lane-sorted sequences, ascending: min(a, reverse(b)) is
+// bitonic and holds them; a half-cleaner cascade sorts it.
+__device__ __forceinline__ uint64_t wave_merge_low(uint64_t a, uint64_t b, int lane) {
+    const uint64_t br = shfl_u64(b, 63 - lane);
+    uint64_t v = a < br ? a : br;
+#pragma unroll
+    for (int j = 32; j > 0; j >>= 1) {
+        const uint64_t o = shfl_xor_u64(v, j);
+        v = (lane & j) == 0 ? (v < o ? v : o) : (v < o ? o : v);
+    }
+    return v;
+}
+
+// One wavefront per photonmap texel (persistent queue): the k stored photons nearest to the
+// texel's surface point within the radius, ordered by (squared distance, emission slot), their
+// power summed in that order and divided by their number (Photonmap::gather's normalisation,
+// Lightmap.cpp:407-412). Keys are (float bits of d^2) << 32 | slot: non-negative floats order
+// as their bit patterns.
+__global__ __launch_bounds__(256) void k_knn_gather(const float4* __restrict__ queries, uint32_t nq,
+                                                    const float4* __restrict__ spos,
+                                                    const PhotonRec* __restrict__ rec,
+                                                    const uint32_t* __restrict__ cstart,
+                                                    const uint32_t* __restrict__ cend, KnnGrid G,
+                                                    const PhotonMapDev* __restrict__ pms, int npm,
+                                                    unsigned long long* queue) {
+    const int lane = (int)lane_id();
+    for (;;) {
+        const uint64_t qi = wave_next_chunk(queue);
+        if (qi >= nq) break;
+        const float4 Q = queries[qi];
+        const int qx = knn_axis(Q.x, G.lo[0], G.inv_h), qy = knn_axis(Q.y, G.lo[1], G.inv_h),
+                  qz = knn_axis(Q.z, G.lo[2], G.inv_h);
+        uint64_t best = ~0ull;
+        for (int s = 0; s <= G.max_shell; s++) {
+            for (int dz = -s; dz <= s; dz++) {
+                for (int dy = -s; dy <= s; dy++) {
+                    for (int dx = -s; dx <= s; dx++) {
+                        const int ax = dx < 0 ? -dx : dx, ay = dy < 0 ? -dy : dy, az = dz < 0 ? -dz : dz;
+                        if ((ax > ay ? (ax > az ? ax : az) : (ay > az ? ay : az)) != s) continue;
+                        const int cx = qx + dx, cy = qy + dy, cz = qz + dz;
+                        const uint32_t b = knn_bucket(G, cx, cy, cz);
+                        const uint32_t st = cstart[b], en = cend[b];
+                        for (uint32_t base = st; base < en; base += 64) {
+                            const uint32_t i = base + (uint32_t)lane;
+                            uint64_t key = ~0ull;
+                            if (i < en) {
+                                const float4 P = spos[i];
+                                if (knn_axis(P.x, G.lo[0], G.inv_h) == cx && knn_axis(P.y, G.lo[1], G.inv_h) == cy &&
+                                    knn_axis(P.z, G.lo[2], G.inv_h) == cz) {
+                                    const float ex = P.x - Q.x, ey = P.y - Q.y, ez = P.z - Q.z;
+                                    const float d2 = ex * ex + ey * ey + ez * ez;
+                                    if (d2 <= G.r2) key = ((uint64_t)__float_as_uint(d2) << 32) | __float_as_uint(P.w);
+                                }
+                            }
+                            const uint64_t kth = wave_uniform(shfl_u64(best, G.k - 1));
+                            if (!__any(key < kth)) continue;
+                            best = wave_merge_low(best, wave_sort(key, lane), lane);
+                        }
+                    }
+                }
+            }
+            const uint64_t kth = wave_uniform(shfl_u64(best, G.k - 1));
+            if ((uint32_t)(kth >> 32) < __float_as_uint(G.shell_d2[s])) break;
+        }
+        const bool mine = lane < G.k && best != ~0ull;
+        const int found = __popcll(__ballot(mine));
+        float pr = 0.0f, pg = 0.0f, pb = 0.0f;
+        if (mine) {
+            const PhotonRec r = rec[(uint32_t)best];
+            pr = r.r; pg = r.g; pb = r.b;
+        }
+        float sr = 0.0f, sg = 0.0f, sb = 0.0f;
+        for (int j = 0; j < found; j++) {
+            sr += __shfl(pr, j, 64);
+            sg += __shfl(pg, j, 64);
+            sb += __shfl(pb, j, 64);
+        }
+        if (lane == 0) {
+            const uint32_t key = __float_as_uint(Q.w);
+            const PhotonMapDev M = pms[find_map(pms, npm, key)];
+            float4 out = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            if (found) {
+                const float fn = (float)found;
+                out = make_float4(sr / fn, sg / fn, sb / fn, 0.0f);
+            }
+            M.gathered[key - M.key_base] = out;
+        }
+    }
+}
+
 uint32_t blocks_for(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
 
 }  // namespace
@@ -308,7 +462,7 @@ uint32_t blocks_for(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b)
 hipError_t launch_photon_trace(hipStream_t st, const SceneDev& S, int cap, uint32_t max_blocks,
                                const LightDev* lights, const uint64_t* light_first, const PhotonParams& P,
                                const int32_t* mesh_pm, const PhotonMapDev* pms, PhotonRec* out,
-                               unsigned long long* queue, unsigned long long* rays) {
+                               float4* out_pos, unsigned long long* queue, unsigned long long* rays) {
     const uint64_t nrec = P.total * (uint64_t)(P.max_depth + 1);
     if (nrec == 0) return hipSuccess;
     hipLaunchKernelGGL(k_photon_clear, dim3(blocks_for(nrec, 256)), dim3(256), 0, st, out, nrec);
@@ -317,7 +471,7 @@ hipError_t launch_photon_trace(hipStream_t st, const SceneDev& S, int cap, uint3
     const uint64_t need = blocks_for(P.total, kBlock);
     const uint32_t grid = (uint32_t)(need < max_blocks ? need : max_blocks);
     RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_photon_trace<CAP>, dim3(grid), dim3(kBlock), 0, st, S, lights,
-                                            light_first, P, mesh_pm, pms, out, queue, rays));
+                                            light_first, P, mesh_pm, pms, out, out_pos, queue, rays));
     return hipGetLastError();
 }
 
@@ -346,6 +500,43 @@ hipError_t launch_photon_gather(hipStream_t st, const PhotonMapDev* pms, int npm
     if (total_texels > 0xFFFFFF00u) return hipErrorInvalidValue;  // one dispatch stays below 2^32 items
     hipLaunchKernelGGL(k_photon_gather, dim3(blocks_for(total_texels, 256)), dim3(256), 0, st, pms, npm,
                        total_texels, radius);
+    return hipGetLastError();
+}
+
+hipError_t knn_sort_temp_bytes(uint32_t n, int key_bits, size_t* bytes) {
+    *bytes = 0;
+    return hipcub::DeviceRadixSort::SortPairs(nullptr, *bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
+                                              (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, key_bits);
+}
+
+hipError_t launch_knn_build(hipStream_t st, const PhotonRec* rec, const float4* ppos, uint32_t n,
+                            const KnnGrid& G, int key_bits, uint32_t* keys_a, uint32_t* keys_b, uint32_t* idx_a,
+                            uint32_t* idx_b, void* temp, size_t temp_bytes, float4* spos, uint32_t* cstart,
+                            uint32_t* cend) {
+    if (n == 0) return hipSuccess;
+    hipError_t e = launch_fill_u32(st, cstart, (uint64_t)G.mask + 1u, 0u);
+    if (e == hipSuccess) e = launch_fill_u32(st, cend, (uint64_t)G.mask + 1u, 0u);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_knn_keys, dim3(blocks_for(n, 256)), dim3(256), 0, st, rec, ppos, n, G, keys_a, idx_a);
+    e = hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys_a, keys_b, idx_a, idx_b, (int)n, 0, key_bits, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_knn_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, st, keys_b, idx_b, n, ppos, G, spos,
+                       cstart, cend);
+    return hipGetLastError();
+}
+
+hipError_t launch_knn_gather(hipStream_t st, uint32_t max_blocks, const float4* queries, uint32_t nq,
+                             const float4* spos, const PhotonRec* rec, const uint32_t* cstart,
+                             const uint32_t* cend, const KnnGrid& G, const PhotonMapDev* pms, int npm,
+                             unsigned long long* queue) {
+    if (nq == 0) return hipSuccess;
+    if (G.k < 1 || G.k > 64 || G.max_shell < 0 || G.max_shell > 3) return hipErrorInvalidValue;
+    hipError_t e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(queue), 2, 0u);
+    if (e != hipSuccess) return e;
+    const uint64_t need = blocks_for(nq, 256 / 64);
+    const uint32_t grid = (uint32_t)(need < max_blocks ? need : max_blocks);
+    hipLaunchKernelGGL(k_knn_gather, dim3(grid), dim3(256), 0, st, queries, nq, spos, rec, cstart, cend, G, pms,
+                       npm, queue);
     return hipGetLastError();
 }
 

@@ -574,6 +574,11 @@ int32_t check_indirect_settings(const rl_indirect_settings* s) {
         return fail(RL_INVALID_ARGUMENT, "unknown sampling mode");
     if (s->termination != RL_PHOTON_THRESHOLD && s->termination != RL_PHOTON_RUSSIAN_ROULETTE)
         return fail(RL_INVALID_ARGUMENT, "unknown photon termination");
+    if (s->gather_mode != RL_GATHER_TEXEL_DISC && s->gather_mode != RL_GATHER_KNN)
+        return fail(RL_INVALID_ARGUMENT, "unknown gather mode");
+    if (s->gather_mode == RL_GATHER_KNN &&
+        (s->knn_k < 1 || s->knn_k > 64 || !(s->knn_radius > 0.0f) || !std::isfinite(s->knn_radius)))
+        return fail(RL_INVALID_ARGUMENT, "k-NN gather needs knn_k in 1..64 and a positive finite knn_radius");
     return RL_SUCCESS;
 }
 
@@ -1108,13 +1113,50 @@ int32_t rl_emit_photons(rl_relight* ctx, rl_scene* scene, const rl_indirect_sett
     const uint64_t nrec = P.total * (uint64_t)(P.max_depth + 1);
     if (nrec == 0) return end_call(ctx, tm);
     if (nrec > 0x7FFFFFFFull) return fail(RL_REJECTED, "too many photon records for one emit call");
+    const bool knn = s->gather_mode == RL_GATHER_KNN;
+    // k-NN grid: cell = radius / 1.98 (all photons within the radius lie in the 5x5x5 cells
+    // around a query's cell), a power-of-two bucket table of at least the record count.
+    KnnGrid G{};
+    int key_bits = 0;
+    std::vector<float4> queries;
+    if (knn) {
+        const float h = s->knn_radius / 1.98f;
+        for (int k = 0; k < 3; k++) G.lo[k] = scene->bounds_lo[k];
+        G.inv_h = 1.0f / h;
+        uint64_t buckets = 1024;
+        while (buckets < nrec && buckets < (1ull << 28)) buckets <<= 1;
+        G.mask = (uint32_t)(buckets - 1);
+        for (key_bits = 0; (1ull << key_bits) <= buckets; key_bits++) {}
+        G.max_shell = 2;
+        G.k = s->knn_k;
+        G.r2 = s->knn_radius * s->knn_radius;
+        G.shell_d2[0] = 0.0f;
+        for (int k = 1; k < 4; k++) {
+            const float e = ((float)k - 0.01f) * h;
+            G.shell_d2[k] = e * e;
+        }
+        for (size_t m = 0; m < T.maps.size(); m++) {
+            const rl_lightmap* pm = T.maps[m];
+            const uint32_t n = (uint32_t)pm->w * (uint32_t)pm->h;
+            for (uint32_t t = 0; t < n; t++) {
+                if (!pm->valid[t]) continue;
+                const uint32_t key = T.dev[m].key_base + t;
+                float kw;
+                std::memcpy(&kw, &key, 4);
+                queries.push_back(make_float4(pm->pos[t].x, pm->pos[t].y, pm->pos[t].z, kw));
+            }
+        }
+    }
     uint64_t* d_first = nullptr;
     PhotonRec* d_rec = nullptr;
     uint32_t *ka = nullptr, *kb = nullptr, *ia = nullptr, *ib = nullptr;
     uint8_t* temp = nullptr;
-    size_t temp_bytes = 0;
+    size_t temp_bytes = 0, knn_temp = 0;
+    float4 *d_ppos = nullptr, *d_spos = nullptr, *d_queries = nullptr;
+    uint32_t *d_cstart = nullptr, *d_cend = nullptr;
     auto cleanup = [&]() {
         dev_free(d_first); dev_free(d_rec); dev_free(ka); dev_free(kb); dev_free(ia); dev_free(ib); dev_free(temp);
+        dev_free(d_ppos); dev_free(d_spos); dev_free(d_queries); dev_free(d_cstart); dev_free(d_cend);
     };
     hipError_t e = dev_alloc(&d_first, (size_t)nl);
     if (e == hipSuccess) e = dev_alloc(&d_rec, (size_t)nrec);
@@ -1123,16 +1165,24 @@ int32_t rl_emit_photons(rl_relight* ctx, rl_scene* scene, const rl_indirect_sett
     if (e == hipSuccess) e = dev_alloc(&ia, (size_t)nrec);
     if (e == hipSuccess) e = dev_alloc(&ib, (size_t)nrec);
     if (e == hipSuccess) e = photon_sort_temp_bytes((uint32_t)nrec, &temp_bytes);
-    if (e == hipSuccess) e = dev_alloc(&temp, std::max<size_t>(temp_bytes, 1));
+    if (e == hipSuccess && knn) e = knn_sort_temp_bytes((uint32_t)nrec, key_bits, &knn_temp);
+    if (e == hipSuccess) e = dev_alloc(&temp, std::max<size_t>(std::max(temp_bytes, knn_temp), 1));
+    if (e == hipSuccess && knn) e = dev_alloc(&d_ppos, (size_t)nrec);
+    if (e == hipSuccess && knn) e = dev_alloc(&d_spos, (size_t)nrec);
+    if (e == hipSuccess && knn) e = dev_alloc(&d_cstart, (size_t)G.mask + 1);
+    if (e == hipSuccess && knn) e = dev_alloc(&d_cend, (size_t)G.mask + 1);
+    if (e == hipSuccess && knn && !queries.empty()) e = dev_alloc(&d_queries, queries.size());
     if (e != hipSuccess) {
         cleanup();
         return fail(RL_OUT_OF_MEMORY, std::string("photon buffers: ") + hipGetErrorString(e));
     }
     e = hipMemcpy(d_first, first.data(), nl * sizeof(uint64_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess && d_queries)
+        e = hipMemcpy(d_queries, queries.data(), queries.size() * sizeof(float4), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = kernel_begin(ctx, RL_KERNEL_PHOTON_TRACE);
     if (e == hipSuccess)
         e = launch_photon_trace(ctx->stream, scene->dev(), scene->cap, ctx->max_blocks, scene->d_lights, d_first, P,
-                                T.d_mesh_pm, T.d_pms, d_rec, ctx->d_queue, ctx->d_rays);
+                                T.d_mesh_pm, T.d_pms, d_rec, d_ppos, ctx->d_queue, ctx->d_rays);
     if (e == hipSuccess) e = kernel_end(ctx);
     if (e == hipSuccess) e = kernel_begin(ctx, RL_KERNEL_PHOTON_STORE);
     if (e == hipSuccess)
@@ -1140,11 +1190,29 @@ int32_t rl_emit_photons(rl_relight* ctx, rl_scene* scene, const rl_indirect_sett
                                 (int)T.dev.size());
     if (e == hipSuccess) e = kernel_end(ctx);
     // Relight::emitPhotons gathers every photonmap after the passes (Relight.cpp:250-254).
-    if (e == hipSuccess) e = kernel_begin(ctx, RL_KERNEL_PHOTON_GATHER);
-    if (e == hipSuccess)
-        e = launch_photon_gather(ctx->stream, T.d_pms, (int)T.dev.size(), (uint32_t)T.total_texels,
-                                 s->final_gather_radius);
-    if (e == hipSuccess) e = kernel_end(ctx);
+    if (knn) {
+        // Texels without a surface point get 0; the rest the k-nearest estimate of this call's photons.
+        for (size_t m = 0; m < T.maps.size() && e == hipSuccess; m++)
+            e = launch_fill_u32(ctx->stream, reinterpret_cast<uint32_t*>(T.dev[m].gathered),
+                                4ull * (uint64_t)T.dev[m].w * (uint64_t)T.dev[m].h, 0u);
+        if (e == hipSuccess) e = kernel_begin(ctx, RL_KERNEL_PHOTON_GRID);
+        if (e == hipSuccess)
+            e = launch_knn_build(ctx->stream, d_rec, d_ppos, (uint32_t)nrec, G, key_bits, ka, kb, ia, ib, temp,
+                                 std::max(temp_bytes, knn_temp), d_spos, d_cstart, d_cend);
+        if (e == hipSuccess) e = kernel_end(ctx);
+        if (e == hipSuccess) e = kernel_begin(ctx, RL_KERNEL_PHOTON_GATHER);
+        if (e == hipSuccess)
+            e = launch_knn_gather(ctx->stream, ctx->max_blocks, d_queries, (uint32_t)queries.size(), d_spos, d_rec,
+                                  d_cstart, d_cend, G, T.d_pms, (int)T.dev.size(), ctx->d_queue);
+        if (e == hipSuccess) e = kernel_end(ctx);
+        ctx->stats.launches += 3;
+    } else {
+        if (e == hipSuccess) e = kernel_begin(ctx, RL_KERNEL_PHOTON_GATHER);
+        if (e == hipSuccess)
+            e = launch_photon_gather(ctx->stream, T.d_pms, (int)T.dev.size(), (uint32_t)T.total_texels,
+                                     s->final_gather_radius);
+        if (e == hipSuccess) e = kernel_end(ctx);
+    }
     ctx->stats.launches += 4;
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     cleanup();

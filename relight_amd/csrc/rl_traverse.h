@@ -239,13 +239,23 @@ __device__ __forceinline__ int find_seg(const BakeSeg* segs, int nsegs, uint64_t
     return lo;
 }
 
-// Persistent work queue: lane 0 of a wavefront takes the next chunk index, every lane gets it.
-// A wavefront keeps taking chunks until the index passes the chunk count, so every wavefront
-// reaches its exit and the grid drains.
+// A 64-bit value equal in every active lane, moved to scalar registers so the compiler treats
+// branches on it as wave-uniform.
+__device__ __forceinline__ uint64_t wave_uniform(uint64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Persistent work queue: a wavefront takes the next chunk index and every lane gets it. A
+// wavefront keeps taking chunks until the index passes the chunk count, so every wavefront reaches
+// its exit and the grid drains. Every lane executes the atomic (lane 0 adds 1, the others 0) and
+// the index is returned wave-uniform: with a lane-0-only atomic at the loop head and a lane-0-only
+// store at the loop end, the compiler may rotate the loop so that the two regions merge and the
+// other lanes re-read the index from lane 0 after it has left their loop, never reaching the exit.
 __device__ __forceinline__ uint64_t wave_next_chunk(unsigned long long* counter) {
-    unsigned long long c = 0;
-    if (lane_id() == 0) c = atomicAdd(counter, 1ull);
-    return __shfl(c, 0, 64);
+    const unsigned long long c = atomicAdd(counter, lane_id() == 0 ? 1ull : 0ull);
+    return wave_uniform(__shfl(c, 0, 64));
 }
 
 __device__ __forceinline__ void count_rays(uint32_t n, unsigned long long* counter) {

@@ -69,8 +69,8 @@ class AmbientOcclusionSettings:
 
 @dataclass
 class IndirectLightSettings:
-    """IndirectLightSettings (Relight.h:108-132) + seed, sampling, photon count override and
-    termination rule."""
+    """IndirectLightSettings (Relight.h:108-132) + seed, sampling, photon count override,
+    termination rule and gather rule (texel disc, or k nearest photons in 3D)."""
     photon_pass_count: int = 16
     photon_bounce_count: int = 3
     photon_energy_threshold: float = 0.05
@@ -84,6 +84,9 @@ class IndirectLightSettings:
     sampling: int = A.RL_SAMPLE_UNIFORM
     photons_per_light: int = 0
     termination: int = A.RL_PHOTON_THRESHOLD
+    gather_mode: int = A.RL_GATHER_TEXEL_DISC
+    knn_k: int = 64
+    knn_radius: float = 1.0
 
     # Presets of Relight.cpp:42-120: (passes, bounces, final-gather samples).
     _PRESETS = {"draft": (8, 1, 32), "fast": (16, 3, 64), "best": (32, 3, 128),
@@ -127,6 +130,9 @@ class IndirectLightSettings:
         s.sampling = int(self.sampling)
         s.photons_per_light = int(self.photons_per_light)
         s.termination = int(self.termination)
+        s.gather_mode = int(self.gather_mode)
+        s.knn_k = int(self.knn_k)
+        s.knn_radius = float(self.knn_radius)
         return s
 
 

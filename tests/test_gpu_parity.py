@@ -220,6 +220,34 @@ def test_photons_russian_roulette_directional_and_cosine_fg(relight):
     _gi_compare(relight, spec)
 
 
+@pytest.mark.parametrize("k,radius", [(64, 0.6), (16, 0.25), (1, 0.3)])
+def test_knn_photon_gather_bit_exact(relight, k, radius):
+    """Hash-grid k-nearest gather (one wavefront per photonmap texel) against the oracle's grid
+    search, then the final gather that reads it."""
+    print(f"[gpu] box_room k-NN gather k={k} r={radius}", flush=True)
+    from relight_amd import RL_GATHER_KNN
+    spec = scenes.box_room_gi()
+    spec.indirect.gather_mode, spec.indirect.knn_k, spec.indirect.knn_radius = RL_GATHER_KNN, k, radius
+    _gi_compare(relight, spec)
+
+
+def test_knn_gather_city_multi_map(relight):
+    print("[gpu] city k-NN gather, 8 photonmaps, RR", flush=True)
+    from relight_amd import RL_GATHER_KNN, RL_PHOTON_RUSSIAN_ROULETTE, IndirectLightSettings
+    spec = scenes.city(num_boxes=200, tiles=(4, 2), side_range=(32, 64))
+    rng = np.random.default_rng(5)
+    for i, m in enumerate(spec.meshes):
+        m.albedo = np.array([*rng.uniform(0.3, 0.8, 3), 1.0], np.float32)
+        m.photonmap = i
+    spec.photonmaps = [(48, 48) for _ in spec.meshes]
+    spec.indirect = IndirectLightSettings(
+        photon_pass_count=1, photon_bounce_count=3, photon_energy_threshold=0.0,
+        photon_max_distance=300.0, final_gather_samples=24, final_gather_distance=60.0,
+        final_gather_radius=2, sky_color=(0.3, 0.4, 0.5), seed=4, photons_per_light=40000,
+        termination=RL_PHOTON_RUSSIAN_ROULETTE, gather_mode=RL_GATHER_KNN, knn_k=64, knn_radius=3.0)
+    _gi_compare(relight, spec)
+
+
 def test_axis_parallel_segments_agree(relight):
     """Segments with one or two direction components exactly zero (the slab test's 1/0 case),
     from origins on box faces and inside the city, against the oracle."""

@@ -385,3 +385,45 @@ def test_final_gather_sky_closed_form():
         S.final_gather(spec, pos[sel], nrm[sel], texel[sel], 0, ind, {}, col)
         assert np.allclose(col[:, 0], ecos + 0.1, atol=0.02)
         assert np.allclose(col[:, 2], 0.25 * ecos + 0.1, atol=0.01)
+
+
+def _knn_brute_force(key, power, pos, qpos, k, radius):
+    """k-nearest gather by exhaustive search in float32 (each operation rounded as written):
+    order by (squared distance, slot), sum power in that order, divide by the count."""
+    stored = np.nonzero(key != 0xFFFFFFFF)[0]
+    P = pos[stored]
+    r2 = np.float32(radius) * np.float32(radius)
+    out = np.zeros((qpos.shape[0], 3), np.float32)
+    for i, q in enumerate(qpos):
+        e = P - q
+        d2 = (e[:, 0] * e[:, 0] + e[:, 1] * e[:, 1]) + e[:, 2] * e[:, 2]
+        m = np.nonzero(d2 <= r2)[0]
+        if m.size == 0:
+            continue
+        order = np.lexsort((stored[m], d2[m]))[:k]
+        sel = stored[m][order]
+        acc = np.add.accumulate(power[sel], axis=0, dtype=np.float32)[-1]
+        out[i] = acc / np.float32(sel.size)
+    return out
+
+
+@pytest.mark.parametrize("k,radius", [(16, 0.35), (64, 0.6), (1, 0.2)])
+def test_knn_gather_equals_brute_force(k, radius):
+    """The oracle's k-nearest photon gather (grid search) against an exhaustive float32 search on
+    the box room's photons; k = 1 returns the nearest photon's own power."""
+    from relight_amd import RL_GATHER_KNN
+    spec = scenes.box_room_gi(photonmap_size=32, photons_per_light=4000)
+    ind = spec.indirect
+    ind.gather_mode, ind.knn_k, ind.knn_radius = RL_GATHER_KNN, k, radius
+    S = O.OracleScene.from_spec(spec)
+    maps, _ = S.emit_photons(spec, ind)
+    c, p, g = maps[0]
+    key, power, pos = S.photon_records()
+    assert (key != 0xFFFFFFFF).sum() == int(p.sum())
+    valid, _, qpos, _ = S.raster(0, 32, 32)
+    sel = np.nonzero(valid)[0]
+    expect = _knn_brute_force(key, power, pos, qpos[sel], k, radius)
+    got = g.reshape(-1, 3)
+    assert np.array_equal(got[sel], expect)
+    assert (got[valid == 0] == 0).all()
+    assert (np.abs(expect).sum(1) > 0).mean() > 0.9
