@@ -293,14 +293,15 @@ int32_t rl_bake_items(rl_relight* ctx, const rl_bake_item* items, int32_t num_it
  * Worker.cpp and src/demo/Lightmapping.cpp:619-649 for serving). A background thread owns the
  * context's stream; submitted jobs wait in a FIFO queue and are admitted while the sum of admitted
  * jobs' device footprints stays within memory_budget and at most max_active run. Every step the
- * loop takes up to quantum_rays rays of direct/AO work from all admitted jobs, fairly shared, into
- * one multi-scene launch per pass (continuous batching: a job submitted while others run joins at
- * the next step); photon, final-gather and radiosity phases run as whole steps. Scenes, lightmaps
+ * loop takes up to quantum_rays rays of direct, AO and final-gather work from all admitted jobs,
+ * fairly shared, into one multi-scene launch per pass (continuous batching: a job submitted while
+ * others run joins at the next step); photon emission and radiosity phases run as whole steps. Scenes, lightmaps
  * and radiosity objects are borrowed until the job finishes; while a server runs, make no other
  * call on its context. Results are identical to the same job run alone. */
 #define RL_JOB_DIRECT_AO 0
-#define RL_JOB_INDIRECT 1
+#define RL_JOB_INDIRECT 1      /* photons, then final gather per lightmap */
 #define RL_JOB_RADIOSITY 2
+#define RL_JOB_FINAL_GATHER 3  /* final gather from the scene's resident photonmaps */
 #define RL_JOB_QUEUED 0
 #define RL_JOB_RUNNING 1
 #define RL_JOB_DONE 2
@@ -319,7 +320,7 @@ typedef struct rl_job_desc {
     int32_t num_lightmaps;
     int32_t direct;                  /* DIRECT_AO: bake direct light first */
     const rl_ao_settings* ao;        /* DIRECT_AO: AO after direct (NULL: none) */
-    const rl_indirect_settings* indirect; /* INDIRECT: photons, then final gather per lightmap */
+    const rl_indirect_settings* indirect; /* INDIRECT, FINAL_GATHER */
     rl_radiosity* radiosity;         /* RADIOSITY: form factors then solve */
     uint32_t rays_per_patch;
     uint32_t ff_seed;

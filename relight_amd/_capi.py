@@ -155,6 +155,7 @@ class rl_stats(C.Structure):
 RL_JOB_DIRECT_AO = 0
 RL_JOB_INDIRECT = 1
 RL_JOB_RADIOSITY = 2
+RL_JOB_FINAL_GATHER = 3
 RL_JOB_QUEUED = 0
 RL_JOB_RUNNING = 1
 RL_JOB_DONE = 2

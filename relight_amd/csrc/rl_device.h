@@ -74,23 +74,6 @@ struct AoParams {
     int32_t sampling;
 };
 
-// One lightmap (or a lumel range of one) inside a bake launch, with the scene, lights and AO
-// settings it is baked with, so one launch can cover lightmaps of different scenes and jobs.
-struct BakeSeg {
-    const float4* pos;
-    const float4* nrm;
-    float4* color;
-    uint32_t* counts;     // per-lumel AO occluded counts
-    uint32_t count;       // lumels in this segment
-    uint32_t lm_seed;
-    uint64_t work_begin;  // first global work item (or chunk, for chunked launches) of this segment
-    SceneDev scene;
-    const LightDev* lights;
-    const float4* lverts;
-    int32_t nlights;
-    AoParams ao;
-};
-
 // A photonmap (Lightmap.h:153-170) as full w*h grids: accumulated photon power (rgb), photon
 // count, gathered colour, and the lumel validity the photon store checks.
 struct PhotonMapDev {
@@ -164,6 +147,27 @@ struct FinalGatherParams {
     float ambient[3];
     uint32_t seed;
     int32_t sampling;
+};
+
+// One lightmap (or a lumel range of one) inside a bake launch, with the scene, lights, AO and
+// final-gather settings and photon tables it is baked with, so one launch can cover lightmaps of
+// different scenes and jobs.
+struct BakeSeg {
+    const float4* pos;
+    const float4* nrm;
+    float4* color;
+    uint32_t* counts;     // per-lumel AO occluded counts
+    uint32_t count;       // lumels in this segment
+    uint32_t lm_seed;
+    uint64_t work_begin;  // first global work item (or chunk, for chunked launches) of this segment
+    SceneDev scene;
+    const LightDev* lights;
+    const float4* lverts;
+    int32_t nlights;
+    AoParams ao;
+    FinalGatherParams fg;
+    const int32_t* mesh_pm;     // scene mesh -> photonmap index (final gather)
+    const PhotonMapDev* pms;    // the scene's photonmaps (final gather)
 };
 
 }  // namespace rl

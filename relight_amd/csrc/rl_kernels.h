@@ -78,10 +78,8 @@ hipError_t launch_knn_gather(hipStream_t st, uint32_t max_blocks, const float4* 
                              const float4* spos, const PhotonRec* rec, const uint32_t* cstart,
                              const uint32_t* cend, const KnnGrid& G, const PhotonMapDev* pms, int npm,
                              unsigned long long* queue);
-hipError_t launch_final_gather(hipStream_t st, const SceneDev& S, int cap, uint32_t max_blocks,
-                               const BakeSeg* segs, int nsegs, uint64_t total_lumels,
-                               const FinalGatherParams& P, const int32_t* mesh_pm,
-                               const PhotonMapDev* pms, unsigned long long* queue,
-                               unsigned long long* rays);
+// Final gather over lumel segments; each segment carries its scene, settings and photon tables.
+hipError_t launch_final_gather(hipStream_t st, int cap, uint32_t max_blocks, const BakeSeg* segs, int nsegs,
+                               uint64_t total_lumels, unsigned long long* queue, unsigned long long* rays);
 
 }  // namespace rl

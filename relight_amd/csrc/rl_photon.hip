@@ -224,11 +224,8 @@ __global__ void k_photon_gather(const PhotonMapDev* __restrict__ pms, int npm, u
 // samples per step; the per-sample terms are added in sample order, as the reference's loop does,
 // by broadcasting them lane by lane.
 template <int CAP>
-__global__ __launch_bounds__(kBlock) void k_final_gather(SceneDev S, const BakeSeg* __restrict__ segs, int nsegs,
-                                                         uint64_t total, FinalGatherParams P,
-                                                         const int32_t* __restrict__ mesh_pm,
-                                                         const PhotonMapDev* __restrict__ pms,
-                                                         unsigned long long* queue,
+__global__ __launch_bounds__(kBlock) void k_final_gather(const BakeSeg* __restrict__ segs, int nsegs,
+                                                         uint64_t total, unsigned long long* queue,
                                                          unsigned long long* rays_out) {
     __shared__ uint32_t stack[kLdsStack * kBlock];
     uint32_t* stk = stack + threadIdx.x;
@@ -239,6 +236,10 @@ __global__ __launch_bounds__(kBlock) void k_final_gather(SceneDev S, const BakeS
     if (g >= total) break;
     const BakeSeg& sg = segs[find_seg(segs, nsegs, g)];
     const uint32_t i = (uint32_t)(g - sg.work_begin);
+    const SceneDev S = sg.scene;
+    const FinalGatherParams P = sg.fg;
+    const int32_t* mesh_pm = sg.mesh_pm;
+    const PhotonMapDev* pms = sg.pms;
     const float4 p4 = sg.pos[i], n4 = sg.nrm[i];
     const V3 p = v3(p4.x, p4.y, p4.z), n = v3(n4.x, n4.y, n4.z);
     const uint32_t texel = f_bits(p4.w);
@@ -540,17 +541,15 @@ hipError_t launch_knn_gather(hipStream_t st, uint32_t max_blocks, const float4* 
     return hipGetLastError();
 }
 
-hipError_t launch_final_gather(hipStream_t st, const SceneDev& S, int cap, uint32_t max_blocks,
-                               const BakeSeg* segs, int nsegs, uint64_t total_lumels,
-                               const FinalGatherParams& P, const int32_t* mesh_pm, const PhotonMapDev* pms,
-                               unsigned long long* queue, unsigned long long* rays) {
+hipError_t launch_final_gather(hipStream_t st, int cap, uint32_t max_blocks, const BakeSeg* segs, int nsegs,
+                               uint64_t total_lumels, unsigned long long* queue, unsigned long long* rays) {
     if (total_lumels == 0) return hipSuccess;
     hipError_t e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(queue), 2, 0u);
     if (e != hipSuccess) return e;
     const uint64_t need = blocks_for(total_lumels, kBlock / 64);
     const uint32_t grid = (uint32_t)(need < max_blocks ? need : max_blocks);
-    RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_final_gather<CAP>, dim3(grid), dim3(kBlock), 0, st, S, segs, nsegs,
-                                            total_lumels, P, mesh_pm, pms, queue, rays));
+    RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_final_gather<CAP>, dim3(grid), dim3(kBlock), 0, st, segs, nsegs,
+                                            total_lumels, queue, rays));
     return hipGetLastError();
 }
 

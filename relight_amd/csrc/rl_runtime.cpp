@@ -340,6 +340,9 @@ struct BakeItem {
     AoParams ao;
     uint32_t first;
     uint32_t count;
+    FinalGatherParams fg{};
+    const int32_t* mesh_pm = nullptr;
+    const PhotonMapDev* pms = nullptr;
 };
 
 enum SegUnit { kUnitLumel = 0, kUnitPair = 1, kUnitChunk = 2 };
@@ -406,6 +409,9 @@ int32_t upload_segments(const std::vector<BakeItem>& items, int unit, uint32_t c
         s.lverts = it.scene->d_lverts;
         s.nlights = (int32_t)it.scene->lights.size();
         s.ao = it.ao;
+        s.fg = it.fg;
+        s.mesh_pm = it.mesh_pm;
+        s.pms = it.pms;
         segs.push_back(s);
         if (unit == kUnitLumel) acc += it.count;
         else if (unit == kUnitPair) acc += (uint64_t)it.count * (uint64_t)it.ao.samples;
@@ -493,6 +499,40 @@ int32_t run_ao(rl_relight* ctx, rl_scene* scene, rl_lightmap* const* maps, int n
     std::vector<BakeItem> items;
     if ((st = whole_items(scene, maps, n, ao_params(*s), items))) return st;
     return run_ao_items(ctx, items);
+}
+
+// Final gather over items (each with its FG settings and its scene's photon tables).
+int32_t run_fg_items(rl_relight* ctx, const std::vector<BakeItem>& items) {
+    DeviceSegs ds;
+    int32_t st = upload_segments(items, kUnitLumel, 1, ds);
+    if (st) return st;
+    hipError_t e = hipSuccess;
+    if (ds.total) {
+        e = kernel_begin(ctx, RL_KERNEL_FINAL_GATHER);
+        if (e == hipSuccess)
+            e = launch_final_gather(ctx->stream, ds.cap, ctx->max_blocks, ds.d, ds.n, ds.total, ctx->d_queue,
+                                    ctx->d_rays);
+        if (e == hipSuccess) e = kernel_end(ctx);
+    }
+    ctx->stats.texels += ds.texels;
+    ctx->stats.launches += ds.total ? 1 : 0;
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return fail(RL_DEVICE_ERROR, std::string("final gather: ") + hipGetErrorString(e));
+    for (const BakeItem& it : items) it.lm->device_newer = true;
+    return RL_SUCCESS;
+}
+
+FinalGatherParams fg_params(const rl_indirect_settings* s) {
+    FinalGatherParams P;
+    P.samples = s->final_gather_samples;
+    P.distance = s->final_gather_distance;
+    for (int k = 0; k < 3; k++) {
+        P.sky[k] = s->sky_color[k];
+        P.ambient[k] = s->ambient_color[k];
+    }
+    P.seed = s->seed;
+    P.sampling = s->sampling;
+    return P;
 }
 
 // Device tables of a scene's photonmaps (distinct maps in mesh order) and the mesh -> map index.
@@ -1236,32 +1276,12 @@ int32_t rl_bake_indirect(rl_relight* ctx, rl_scene* scene, int32_t mesh, const r
     std::vector<BakeItem> items;
     AoParams none{};
     if ((st = whole_items(scene, &lm, 1, none, items))) return st;
-    DeviceSegs ds;
-    if ((st = upload_segments(items, kUnitLumel, 1, ds))) return st;
-    const uint64_t total = ds.total, tex = ds.texels;
-    BakeSeg* d_segs = ds.d;
-    FinalGatherParams P;
-    P.samples = s->final_gather_samples;
-    P.distance = s->final_gather_distance;
-    for (int k = 0; k < 3; k++) {
-        P.sky[k] = s->sky_color[k];
-        P.ambient[k] = s->ambient_color[k];
+    for (BakeItem& it : items) {
+        it.fg = fg_params(s);
+        it.mesh_pm = T.d_mesh_pm;
+        it.pms = T.d_pms;
     }
-    P.seed = s->seed;
-    P.sampling = s->sampling;
-    hipError_t e = hipSuccess;
-    if (total) {
-        e = kernel_begin(ctx, RL_KERNEL_FINAL_GATHER);
-        if (e == hipSuccess)
-            e = launch_final_gather(ctx->stream, scene->dev(), scene->cap, ctx->max_blocks, d_segs, 1, total, P,
-                                    T.d_mesh_pm, T.d_pms, ctx->d_queue, ctx->d_rays);
-        if (e == hipSuccess) e = kernel_end(ctx);
-    }
-    ctx->stats.texels += tex;
-    ctx->stats.launches += total ? 1 : 0;
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-    if (e != hipSuccess) return fail(RL_DEVICE_ERROR, std::string("final gather: ") + hipGetErrorString(e));
-    lm->device_newer = true;
+    if ((st = run_fg_items(ctx, items))) return st;
     return end_call(ctx, tm);
 }
 
@@ -1521,6 +1541,9 @@ uint32_t valid_lumels(const rl_lightmap* lm) {
     return c;
 }
 
+// Ray phases of a job, baked in shared multi-job launches.
+enum RayPhase { kPhaseNone = 0, kPhaseDirect = 1, kPhaseAo = 2, kPhaseGather = 3 };
+
 struct Job {
     uint64_t id = 0;
     rl_job_desc desc{};
@@ -1534,11 +1557,14 @@ struct Job {
     int32_t status = RL_SUCCESS;
     double submit_ms = 0, admit_ms = 0, done_ms = 0;
     uint64_t rays = 0, footprint = 0;
-    int phase = 0;        // DIRECT_AO: 0 direct, 1 AO; INDIRECT: 0 photons, 1 final gather; RADIOSITY: 0 F, 1 solve
+    // DIRECT_AO: 0 direct, 1 AO. INDIRECT: 0 photons, 1 final gather. FINAL_GATHER: 1 final gather.
+    // RADIOSITY: 0 form factors, 1 solve. 2: done.
+    int phase = 0;
     size_t map_i = 0;     // next lightmap of the phase
     uint32_t cursor = 0;  // next lumel of that lightmap
     bool cancel = false;
-    bool prepared = false;  // lumels uploaded
+    bool prepared = false;                 // lumels uploaded
+    std::unique_ptr<PhotonTables> tables;  // final gather: the scene's photon tables
     std::string error;    // message of the failing call, handed to the caller's rl_last_error
 };
 
@@ -1571,22 +1597,45 @@ bool job_finished(const Job* j) {
     return j->state == RL_JOB_DONE || j->state == RL_JOB_FAILED || j->state == RL_JOB_CANCELLED;
 }
 
-// Moves a DIRECT_AO job to its next phase with work, or to done.
-void advance_direct_ao(Job* j) {
+void fail_job(Job* j, int32_t st) {
+    j->state = RL_JOB_FAILED;
+    j->status = st;
+    j->error = g_error;
+}
+
+RayPhase ray_phase(const Job* j) {
+    if (job_finished(j)) return kPhaseNone;
+    switch (j->desc.kind) {
+        case RL_JOB_DIRECT_AO: return j->phase == 0 ? kPhaseDirect : kPhaseAo;
+        case RL_JOB_INDIRECT:
+        case RL_JOB_FINAL_GATHER: return j->phase == 1 ? kPhaseGather : kPhaseNone;
+        default: return kPhaseNone;
+    }
+}
+
+// Moves a job past a finished ray phase, skipping phases it did not ask for.
+void advance(Job* j) {
     j->map_i = 0;
     j->cursor = 0;
     j->phase++;
-    if (j->phase == 1 && !j->has_ao) j->phase = 2;
+    if (j->desc.kind == RL_JOB_DIRECT_AO && j->phase == 1 && !j->has_ao) j->phase = 2;
     if (j->phase >= 2) j->state = RL_JOB_DONE;
 }
 
 // Takes up to `budget` lumels of the job's current phase into `items`; returns lumels taken.
-uint64_t take_lumels(Job* j, uint64_t budget, const AoParams& ao, std::vector<BakeItem>& items) {
+uint64_t take_lumels(Job* j, uint64_t budget, const BakeItem& proto, std::vector<BakeItem>& items) {
     uint64_t taken = 0;
     while (budget > 0 && j->map_i < j->maps.size()) {
         const uint32_t n = j->valid[j->map_i];
         const uint32_t take = (uint32_t)std::min<uint64_t>(budget, n - j->cursor);
-        if (take) items.push_back(BakeItem{j->maps[j->map_i], j->desc.scene, ao, j->cursor, take});
+        if (take) {
+            BakeItem it = proto;
+            it.lm = j->maps[j->map_i];
+            it.scene = j->desc.scene;
+            it.first = j->cursor;
+            it.count = take;
+            items.push_back(it);
+        }
         j->cursor += take;
         budget -= take;
         taken += take;
@@ -1601,94 +1650,87 @@ uint64_t take_lumels(Job* j, uint64_t budget, const AoParams& ao, std::vector<Ba
 }  // namespace
 
 void rl_server::step(std::vector<Job*>& js) {
-    // Direct/AO work of every DIRECT_AO job, fairly shared, in one launch per pass.
-    std::vector<BakeItem> ditems, aitems;
-    std::vector<Job*> dj, aj;
-    size_t nda = 0;
-    for (Job* j : js)
-        if (j->desc.kind == RL_JOB_DIRECT_AO) nda++;
-    const uint64_t share = std::max<uint64_t>(cfg.quantum_rays / std::max<size_t>(nda, 1), 4096);
+    // Whole-step phases: photon emission and radiosity (form factors, then the solve).
     for (Job* j : js) {
-        if (j->desc.kind != RL_JOB_DIRECT_AO || job_finished(j) || j->prepared) continue;
-        int32_t st = RL_SUCCESS;
-        for (size_t k = 0; k < j->maps.size() && !st; k++) {
-            st = push_lumels(j->maps[k]);
-            if (!st && j->maps[k]->texels.size() != j->valid[k]) st = fail(RL_INVALID_CALL, "lightmap changed after submit");
-        }
-        if (st) {
-            j->state = RL_JOB_FAILED;
-            j->status = st;
-            j->error = g_error;
-        }
-        j->prepared = true;
-    }
-    for (Job* j : js) {
-        if (j->desc.kind != RL_JOB_DIRECT_AO || job_finished(j)) continue;
-        if (j->phase == 0 && !j->desc.direct) advance_direct_ao(j);
         if (job_finished(j)) continue;
-        const bool direct = j->phase == 0;
-        const uint64_t rpl = direct ? direct_rays_per_lumel(j->desc.scene) : (uint64_t)j->ao.samples;
-        const uint64_t lumels = std::max<uint64_t>(share / rpl, 64);
-        const uint64_t got = take_lumels(j, lumels, direct ? AoParams{} : ao_params(j->ao), direct ? ditems : aitems);
-        j->rays += got * rpl;
-        (direct ? dj : aj).push_back(j);
-    }
-    CallTimer tm(ctx);
-    int32_t st = begin_call(ctx);
-    if (!st && !ditems.empty()) st = run_direct_items(ctx, ditems);
-    if (!st && !aitems.empty()) st = run_ao_items(ctx, aitems);
-    if (!st) st = end_call(ctx, tm);
-    stats.rays += ctx->stats.rays;
-    for (Job* j : js) {
-        if (j->desc.kind != RL_JOB_DIRECT_AO || job_finished(j)) continue;
-        const bool in_step = std::find(dj.begin(), dj.end(), j) != dj.end() || std::find(aj.begin(), aj.end(), j) != aj.end();
-        if (!in_step) continue;
-        if (st) {
-            j->state = RL_JOB_FAILED;
-            j->status = st;
-            j->error = g_error;
-        } else if (j->map_i >= j->maps.size()) {
-            advance_direct_ao(j);
-        }
-    }
-    // Photon / final-gather / radiosity jobs: one whole phase (or one lightmap) per step.
-    for (Job* j : js) {
-        if (job_finished(j) || j->desc.kind == RL_JOB_DIRECT_AO) continue;
-        int32_t s2 = RL_SUCCESS;
-        if (j->desc.kind == RL_JOB_INDIRECT) {
+        int32_t st = RL_SUCCESS;
+        if (j->desc.kind == RL_JOB_INDIRECT && j->phase == 0) {
+            st = rl_emit_photons(ctx, j->desc.scene, &j->ind);
+            j->rays += ctx->stats.rays;
+            stats.rays += ctx->stats.rays;
+            j->phase = 1;
+        } else if (j->desc.kind == RL_JOB_RADIOSITY) {
             if (j->phase == 0) {
-                s2 = rl_emit_photons(ctx, j->desc.scene, &j->ind);
-                j->rays += ctx->stats.rays;
-                j->phase = 1;
-            } else if (j->map_i < j->maps.size()) {
-                rl_lightmap* lm = j->maps[j->map_i++];
-                int32_t mesh = -1;
-                for (size_t k = 0; k < j->desc.scene->meshes.size() && mesh < 0; k++)
-                    if (j->desc.scene->meshes[k].lightmap == lm) mesh = (int32_t)k;
-                s2 = mesh < 0 ? fail(RL_INVALID_CALL, "lightmap holds no mesh of the job's scene")
-                              : rl_bake_indirect(ctx, j->desc.scene, mesh, &j->ind);
-                j->rays += ctx->stats.rays;
-            }
-            if (!s2 && j->phase == 1 && j->map_i >= j->maps.size()) j->state = RL_JOB_DONE;
-        } else {
-            if (j->phase == 0) {
-                s2 = rl_radiosity_build_form_factors(ctx, j->desc.radiosity, j->desc.rays_per_patch, j->desc.ff_seed,
+                st = rl_radiosity_build_form_factors(ctx, j->desc.radiosity, j->desc.rays_per_patch, j->desc.ff_seed,
                                                      j->desc.ff_max_distance);
                 j->rays += ctx->stats.rays;
+                stats.rays += ctx->stats.rays;
                 j->phase = 1;
             } else {
                 rl_solve_info info{};
-                s2 = rl_radiosity_solve(ctx, j->desc.radiosity, j->E.data(), j->R.data(), j->desc.num_rhs,
+                st = rl_radiosity_solve(ctx, j->desc.radiosity, j->E.data(), j->R.data(), j->desc.num_rhs,
                                         j->desc.tolerance, 40, 16, j->desc.out_radiosity, &info);
-                if (!s2) j->state = RL_JOB_DONE;
+                if (!st) j->state = RL_JOB_DONE;
             }
         }
-        stats.rays += ctx->stats.rays;
-        if (s2) {
-            j->state = RL_JOB_FAILED;
-            j->status = s2;
-            j->error = g_error;
+        if (st) fail_job(j, st);
+    }
+    // Ray phases: upload lumels (and photon tables) on entry, then share the quantum fairly.
+    size_t nray = 0;
+    for (Job* j : js) {
+        const RayPhase ph = ray_phase(j);
+        if (ph == kPhaseNone) continue;
+        int32_t st = RL_SUCCESS;
+        if (!j->prepared) {
+            for (size_t k = 0; k < j->maps.size() && !st; k++) {
+                st = push_lumels(j->maps[k]);
+                if (!st && j->maps[k]->texels.size() != j->valid[k])
+                    st = fail(RL_INVALID_CALL, "lightmap changed after submit");
+            }
+            j->prepared = true;
         }
+        if (!st && ph == kPhaseGather && !j->tables) {
+            j->tables.reset(new PhotonTables());
+            st = make_photon_tables(j->desc.scene, *j->tables);
+        }
+        if (st) fail_job(j, st);
+        else nray++;
+    }
+    if (nray == 0) return;
+    const uint64_t share = std::max<uint64_t>(cfg.quantum_rays / nray, 4096);
+    std::vector<BakeItem> items[4];
+    std::vector<Job*> in_step;
+    for (Job* j : js) {
+        const RayPhase ph = ray_phase(j);
+        if (ph == kPhaseNone) continue;
+        BakeItem proto{};
+        uint64_t rpl = 1;
+        if (ph == kPhaseDirect) {
+            rpl = direct_rays_per_lumel(j->desc.scene);
+        } else if (ph == kPhaseAo) {
+            proto.ao = ao_params(j->ao);
+            rpl = (uint64_t)j->ao.samples;
+        } else {
+            proto.fg = fg_params(&j->ind);
+            proto.mesh_pm = j->tables->d_mesh_pm;
+            proto.pms = j->tables->d_pms;
+            rpl = (uint64_t)j->ind.final_gather_samples;
+        }
+        const uint64_t got = take_lumels(j, std::max<uint64_t>(share / rpl, 64), proto, items[ph]);
+        j->rays += got * rpl;
+        in_step.push_back(j);
+    }
+    CallTimer tm(ctx);
+    int32_t st = begin_call(ctx);
+    if (!st && !items[kPhaseDirect].empty()) st = run_direct_items(ctx, items[kPhaseDirect]);
+    if (!st && !items[kPhaseAo].empty()) st = run_ao_items(ctx, items[kPhaseAo]);
+    if (!st && !items[kPhaseGather].empty()) st = run_fg_items(ctx, items[kPhaseGather]);
+    if (!st) st = end_call(ctx, tm);
+    stats.rays += ctx->stats.rays;
+    for (Job* j : in_step) {
+        if (st) fail_job(j, st);
+        else if (j->map_i >= j->maps.size()) advance(j);
+        if (job_finished(j)) j->tables.reset();
     }
 }
 
@@ -1731,6 +1773,7 @@ void rl_server::loop() {
             if (j->cancel && !job_finished(j)) j->state = RL_JOB_CANCELLED;
             if (job_finished(j)) {
                 j->done_ms = now_ms();
+                j->tables.reset();
                 mem_used -= j->footprint;
                 if (j->state == RL_JOB_DONE) stats.completed++;
                 else if (j->state == RL_JOB_FAILED) stats.failed++;
@@ -1759,6 +1802,7 @@ uint64_t job_footprint(const Job& j) {
                                                     : (uint64_t)std::max(0, (int32_t)(L.dev.intensity * 25000.0f));
         const uint64_t recs = per_pass * (uint64_t)std::max(0, j.ind.photon_pass_count) * (uint64_t)(j.ind.photon_bounce_count + 1);
         bytes += recs * (sizeof(PhotonRec) + 4 * sizeof(uint32_t)) + recs * 2;  // records, keys/indices, sort scratch
+        if (j.ind.gather_mode == RL_GATHER_KNN) bytes += recs * (2 * sizeof(float4) + 2 * sizeof(uint32_t));
         for (const MeshHost& mh : j.desc.scene->meshes)
             if (mh.photonmap) bytes += (uint64_t)mh.photonmap->w * mh.photonmap->h * (2 * sizeof(float4) + sizeof(uint32_t) + 1);
     } else if (j.desc.kind == RL_JOB_RADIOSITY) {
@@ -1807,7 +1851,7 @@ int32_t rl_server_submit(rl_server* s, const rl_job_desc* d, uint64_t* out_id) {
     j->desc = *d;
     int32_t st = check_scene_ready(d->scene);
     if (st) return fail(RL_REJECTED, std::string("job rejected: ") + g_error);
-    if (d->kind == RL_JOB_DIRECT_AO || d->kind == RL_JOB_INDIRECT) {
+    if (d->kind == RL_JOB_DIRECT_AO || d->kind == RL_JOB_INDIRECT || d->kind == RL_JOB_FINAL_GATHER) {
         if (!d->lightmaps || d->num_lightmaps <= 0) return fail(RL_REJECTED, "job rejected: no lightmaps");
         for (int32_t i = 0; i < d->num_lightmaps; i++) {
             rl_lightmap* lm = d->lightmaps[i];
@@ -1822,9 +1866,10 @@ int32_t rl_server_submit(rl_server* s, const rl_job_desc* d, uint64_t* out_id) {
             j->ao = *d->ao;
             j->has_ao = true;
         }
-    } else if (d->kind == RL_JOB_INDIRECT) {
+    } else if (d->kind == RL_JOB_INDIRECT || d->kind == RL_JOB_FINAL_GATHER) {
         if (check_indirect_settings(d->indirect)) return fail(RL_REJECTED, std::string("job rejected: ") + g_error);
         j->ind = *d->indirect;
+        if (d->kind == RL_JOB_FINAL_GATHER) j->phase = 1;
     } else if (d->kind == RL_JOB_RADIOSITY) {
         if (!d->radiosity || !d->emission || !d->reflectance || !d->out_radiosity || d->num_rhs == 0 ||
             d->num_rhs > 128 || d->radiosity->scene != d->scene)

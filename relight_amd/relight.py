@@ -596,6 +596,15 @@ class Server:
                           num_lightmaps=len(lightmaps), indirect=C.pointer(s))
         return self._submit(d)
 
+    def submit_final_gather(self, scene: Scene, lightmaps: Sequence[Lightmap],
+                            settings: IndirectLightSettings) -> BakeJob:
+        """Final gather from the scene's photonmaps as they are (photons emitted earlier)."""
+        maps = self._maps(lightmaps)
+        s = settings.to_c()
+        d = A.rl_job_desc(kind=A.RL_JOB_FINAL_GATHER, scene=scene._h.value, lightmaps=maps,
+                          num_lightmaps=len(lightmaps), indirect=C.pointer(s))
+        return self._submit(d)
+
     def submit_radiosity(self, radiosity: Radiosity, emission, reflectance, rays_per_patch=1024,
                          seed=1, max_distance=1000.0, tolerance=1e-5) -> BakeJob:
         E = np.ascontiguousarray(emission, np.float32)

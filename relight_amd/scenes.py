@@ -12,6 +12,12 @@ Workloads:
   city       200x200 m ground + 5,000 boxes (footprint U(2,12) m, height LogNormal(2.3, 0.6),
              centres uniform, seed 7), 32 tiles = 32 meshes/lightmaps of side round(256*8^U);
              directional sun + a 2x2 m area light of 16 triangles (16 light vertices).
+  heightfield  512x512-cell fBm terrain (4 octaves of value noise, amplitude 3 m, 0.5 m cells,
+             seed 3) in 64 tiles of 64x64 cells (524,288 triangles) + 2,000 one-metre boxes
+             (24,000 triangles, no lightmap); one 2048x2048 lightmap over the terrain, a 128x128
+             photonmap per tile; 4 rectangular 4x4 m area lights 30 m up; albedo U(0.3, 0.8) per
+             mesh; 4M photons (1M per light), 3 bounces, Russian roulette; k = 64 nearest-photon
+             gather within 1 m; 512 final-gather rays per texel.
 """
 from dataclasses import dataclass, field
 from typing import List, Optional, Tuple
@@ -235,6 +241,110 @@ def box_room_gi(seed: int = 1, photonmap_size: int = 64, photons_per_light: int 
         seed=seed, sampling=RL_SAMPLE_UNIFORM, photons_per_light=photons_per_light)
     spec.name = "box_room_gi"
     return spec
+
+
+def _value_noise(rng, cells: int, period: int) -> np.ndarray:
+    """Lattice noise in [-1, 1] with smoothstep-interpolated values, sampled at the
+    (cells + 1)^2 grid vertices (rows z, columns x)."""
+    g = cells // period + 2
+    lat = rng.uniform(-1.0, 1.0, (g, g))
+    x = np.arange(cells + 1) / period
+    i = np.floor(x).astype(np.int64)
+    f = x - i
+    f = f * f * (3.0 - 2.0 * f)
+    a, b = lat[np.ix_(i, i)], lat[np.ix_(i, i + 1)]
+    c, d = lat[np.ix_(i + 1, i)], lat[np.ix_(i + 1, i + 1)]
+    fx, fz = f[None, :], f[:, None]
+    return (a * (1 - fx) + b * fx) * (1 - fz) + (c * (1 - fx) + d * fx) * fz
+
+
+def _box_mesh(lo, hi):
+    """Closed axis-aligned box, 12 triangles with outward per-face normals, no lightmap charts."""
+    quads = _box_quads(lo, hi, 1)
+    for q in quads:
+        q.charted = False
+    p, n, _, ix = _emit(quads, 1, 1)
+    return p, n, ix
+
+
+def heightfield(seed: int = 3, cells: int = 512, cell_size: float = 0.5, tile_cells: int = 64,
+                octaves: int = 4, amplitude: float = 3.0, num_boxes: int = 2000, map_size: int = 2048,
+                photonmap_size: int = 128, photons_per_light: int = 1_000_000, fg_samples: int = 512,
+                knn_k: int = 64, knn_radius: float = 1.0, box_meshes: int = 8) -> SceneSpec:
+    """fBm terrain in tiles (one mesh each, one shared lightmap, a photonmap per tile), boxes
+    flattened into a few meshes without a lightmap, four rectangular area lights."""
+    from ._capi import RL_GATHER_KNN
+    rng = np.random.default_rng(seed)
+    H = np.zeros((cells + 1, cells + 1))
+    wsum = 0.0
+    for o in range(octaves):
+        w = 0.5 ** o
+        H += w * _value_noise(rng, cells, max(1, 64 >> o))
+        wsum += w
+    H = amplitude * H / wsum
+    # Per-vertex normals from central differences of the height grid.
+    gx = np.gradient(H, cell_size, axis=1)
+    gz = np.gradient(H, cell_size, axis=0)
+    N = np.stack([-gx, np.ones_like(H), -gz], -1)
+    N /= np.linalg.norm(N, axis=-1, keepdims=True)
+    # Lightmap uv: 4 texels per cell, scaled by map/(map + 0.5) and offset by a quarter texel so
+    # that no texel centre lies on a cell diagonal (both triangles' strict inside tests would miss it).
+    scale = 1.0 / (map_size + 0.5)
+    tex_per_cell = map_size / cells
+    meshes = []
+    nt = cells // tile_cells
+    tile_albedo = rng.uniform(0.3, 0.8, (nt * nt, 3))
+    for tj in range(nt):
+        for ti in range(nt):
+            j0, i0 = tj * tile_cells, ti * tile_cells
+            jj, ii = np.meshgrid(np.arange(j0, j0 + tile_cells + 1), np.arange(i0, i0 + tile_cells + 1), indexing="ij")
+            pos = np.stack([ii * cell_size, H[jj, ii], jj * cell_size], -1).reshape(-1, 3)
+            nrm = N[jj, ii].reshape(-1, 3)
+            uv = np.stack([(ii * tex_per_cell + 0.25) * scale, (jj * tex_per_cell + 0.25) * scale], -1).reshape(-1, 2)
+            r = tile_cells + 1
+            a = (np.arange(tile_cells)[:, None] * r + np.arange(tile_cells)[None, :]).reshape(-1)
+            idx = np.stack([a, a + r, a + r + 1, a, a + r + 1, a + 1], -1).reshape(-1)
+            t = tj * nt + ti
+            meshes.append(MeshSpec(pos.astype(np.float32), nrm.astype(np.float32), uv.astype(np.float32),
+                                   idx.astype(np.uint32), 0,
+                                   np.array([*tile_albedo[t], 1.0], np.float32), photonmap=t))
+    # Boxes: 1 m cubes sunk 0.3 m into the terrain at uniform positions.
+    ext = cells * cell_size
+    bx = rng.uniform(1.0, ext - 1.0, num_boxes)
+    bz = rng.uniform(1.0, ext - 1.0, num_boxes)
+    by = H[np.clip((bz / cell_size).astype(int), 0, cells), np.clip((bx / cell_size).astype(int), 0, cells)] - 0.3
+    box_albedo = rng.uniform(0.3, 0.8, (box_meshes, 3))
+    per = (num_boxes + box_meshes - 1) // box_meshes
+    for m in range(box_meshes):
+        P, Nn, I = [], [], []
+        for b in range(m * per, min(num_boxes, (m + 1) * per)):
+            p, n, ix = _box_mesh(np.array([bx[b] - 0.5, by[b], bz[b] - 0.5]), np.array([bx[b] + 0.5, by[b] + 1.0, bz[b] + 0.5]))
+            I.append(ix + sum(len(q) for q in P))
+            P.append(p)
+            Nn.append(n)
+        if not P:
+            continue
+        P = np.concatenate(P)
+        meshes.append(MeshSpec(P.astype(np.float32), np.concatenate(Nn).astype(np.float32),
+                               np.zeros((P.shape[0], 2), np.float32), np.concatenate(I).astype(np.uint32), -1,
+                               np.array([*box_albedo[m], 1.0], np.float32)))
+    # Four 4x4 m area lights 30 m above the quadrant centres.
+    apos, aidx = _area_light_rect(4.0, 4, 4)
+    lights = []
+    for k, (qx, qz) in enumerate([(0.25, 0.25), (0.75, 0.25), (0.25, 0.75), (0.75, 0.75)]):
+        col = [(1.0, 0.95, 0.9), (0.9, 0.95, 1.0), (1.0, 1.0, 0.9), (0.95, 1.0, 0.95)][k]
+        lights.append(LightDesc.area(apos, aidx, position=(qx * ext, 30.0, qz * ext), radius=60.0,
+                                     color=col, intensity=40.0))
+    ind = IndirectLightSettings(
+        photon_pass_count=1, photon_bounce_count=3, photon_energy_threshold=0.0, photon_max_distance=100.0,
+        final_gather_samples=fg_samples, final_gather_distance=50.0, final_gather_radius=7,
+        sky_color=(0.2, 0.25, 0.3), ambient_color=(0.0, 0.0, 0.0), seed=seed, sampling=RL_SAMPLE_UNIFORM,
+        photons_per_light=photons_per_light, termination=RL_PHOTON_RUSSIAN_ROULETTE,
+        gather_mode=RL_GATHER_KNN, knn_k=knn_k, knn_radius=knn_radius)
+    return SceneSpec("heightfield", meshes, lights, [(map_size, map_size)], None,
+                     {"seed": seed, "cells": cells, "cell_size": cell_size, "octaves": octaves,
+                      "amplitude": amplitude, "boxes": num_boxes, "tiles": nt * nt},
+                     photonmaps=[(photonmap_size, photonmap_size)] * (nt * nt), indirect=ind)
 
 
 def build_gpu(relight, spec: SceneSpec):

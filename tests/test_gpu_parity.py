@@ -248,6 +248,15 @@ def test_knn_gather_city_multi_map(relight):
     _gi_compare(relight, spec)
 
 
+def test_heightfield_small_photons_knn_final_gather(relight):
+    """The heightfield workload at reduced size (64x64 cells in 4 tiles, 40 boxes without
+    photonmaps, 4 area lights, Russian roulette, k = 64 gather) against the oracle, bit for bit."""
+    print("[gpu] small heightfield photons + k-NN + final gather", flush=True)
+    spec = scenes.heightfield(cells=64, tile_cells=32, num_boxes=40, map_size=256, photonmap_size=32,
+                              photons_per_light=20000, fg_samples=64, box_meshes=2)
+    _gi_compare(relight, spec)
+
+
 def test_axis_parallel_segments_agree(relight):
     """Segments with one or two direction components exactly zero (the slab test's 1/0 case),
     from origins on box faces and inside the city, against the oracle."""

@@ -106,21 +106,27 @@ def test_server_job_alone_equals_job_in_mix(relight):
     sr2, lr2, _ = scenes.build_gpu(relight, room)
     sg2, lg2, _ = scenes.build_gpu(relight, gi)
     ps2, _, rad2 = _radiosity_case(relight)
+    # A final-gather-only job on photonmaps emitted before the server starts.
+    sg3, lg3, _ = scenes.build_gpu(relight, gi)
+    relight.emit_photons(sg3, gi.indirect)
     with relight.create_server(quantum_rays=1 << 21, max_active=8) as srv:
         jobs = [srv.submit_direct_ao(sr2, lr2, True, room.ao),
                 srv.submit_indirect(sg2, lg2, gi.indirect),
                 srv.submit_direct_ao(sc2, lc2, True, city.ao),
-                srv.submit_radiosity(rad2, ps2.emission, ps2.reflectance, 256, 9, 200.0, 1e-5)]
+                srv.submit_radiosity(rad2, ps2.emission, ps2.reflectance, 256, 9, 200.0, 1e-5),
+                srv.submit_final_gather(sg3, lg3, gi.indirect)]
         sts = [j.wait(timeout=300) for j in jobs]
         assert all(s["state"] == "done" for s in sts)
         stats = srv.stats()
-        assert stats["completed"] == 4 and stats["failed"] == 0
+        assert stats["completed"] == 5 and stats["failed"] == 0
         assert stats["peak_memory"] >= max(s["footprint"] for s in sts)
     for a, lm in zip(alone, lc2):
         assert np.array_equal(a, lm.to_hdr())
     for p, lm in zip(plain_room, lr2):
         assert np.array_equal(p, lm.to_hdr())
     for p, lm in zip(plain_gi, lg2):
+        assert np.array_equal(p, lm.to_hdr())
+    for p, lm in zip(plain_gi, lg3):
         assert np.array_equal(p, lm.to_hdr())
     assert np.array_equal(plain_B, jobs[3].result)
 
