@@ -85,6 +85,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ao-samples", type=int, default=256)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--workloads", default="gi,radiosity,serving",
+                    help="secondary legs reported under 'workloads' (comma list of gi, radiosity, "
+                         "serving; empty for none)")
+    ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device with a plain copy kernel (torch) and exit; "
                          "used to scale rocprof FETCH_SIZE/WRITE_SIZE readings")
@@ -217,6 +221,22 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    # Secondary legs (each on its own resident scenes), after the headline measurement.
+    import bench_workloads as W
+    legs = {}
+    for name in [w for w in args.workloads.split(",") if w]:
+        t0 = time.time()
+        if name == "gi":
+            legs["gi"] = W.bench_gi(relight)
+        elif name == "radiosity":
+            legs["radiosity"] = W.bench_radiosity(relight)
+        elif name == "serving":
+            legs["serving"] = W.bench_serving(relight, n_requests=args.serve_requests)
+        else:
+            raise SystemExit(f"unknown workload {name}")
+        log(f"{name} leg done in {time.time()-t0:.1f} s")
+    if legs:
+        out["workloads"] = legs
     print(json.dumps(out), flush=True)
 
 

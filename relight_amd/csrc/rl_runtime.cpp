@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -96,6 +97,10 @@ struct rl_relight {
     size_t pool_used = 0;
     struct Mark { int slot; hipEvent_t a, b; };
     std::vector<Mark> marks;
+    // Segment tables of the launches in flight (at most two at once), grown as needed and reused:
+    // every bake helper synchronises its stream before returning.
+    BakeSeg* seg_buf[2] = {nullptr, nullptr};
+    size_t seg_cap[2] = {0, 0};
 };
 
 struct rl_scene {
@@ -347,14 +352,13 @@ struct BakeItem {
 
 enum SegUnit { kUnitLumel = 0, kUnitPair = 1, kUnitChunk = 2 };
 
-// A device segment table for one launch.
+// A device segment table for one launch (in one of the context's reusable buffers).
 struct DeviceSegs {
     BakeSeg* d = nullptr;
     int n = 0;
     uint64_t total = 0;   // work items in `unit`
     uint64_t texels = 0;  // lumels covered
     int cap = 32;         // traversal-stack capacity for every scene in the table
-    ~DeviceSegs() { dev_free(d); }
 };
 
 AoParams ao_params(const rl_ao_settings& s) {
@@ -390,7 +394,8 @@ int32_t whole_items(rl_scene* scene, rl_lightmap* const* maps, int n, const AoPa
     return RL_SUCCESS;
 }
 
-int32_t upload_segments(const std::vector<BakeItem>& items, int unit, uint32_t chunk_lumels, DeviceSegs& out) {
+int32_t upload_segments(rl_relight* ctx, int slot, const std::vector<BakeItem>& items, int unit,
+                        uint32_t chunk_lumels, DeviceSegs& out) {
     std::vector<BakeSeg> segs;
     uint64_t acc = 0;
     for (const BakeItem& it : items) {
@@ -422,7 +427,14 @@ int32_t upload_segments(const std::vector<BakeItem>& items, int unit, uint32_t c
     out.total = acc;
     out.n = (int)segs.size();
     if (segs.empty()) return RL_SUCCESS;
-    RL_HIP(dev_alloc(&out.d, segs.size()));
+    if (ctx->seg_cap[slot] < segs.size()) {
+        dev_free(ctx->seg_buf[slot]);
+        ctx->seg_cap[slot] = 0;
+        const size_t cap = std::max<size_t>(segs.size(), 256);
+        RL_HIP(dev_alloc(&ctx->seg_buf[slot], cap));
+        ctx->seg_cap[slot] = cap;
+    }
+    out.d = ctx->seg_buf[slot];
     RL_HIP(hipMemcpy(out.d, segs.data(), segs.size() * sizeof(BakeSeg), hipMemcpyHostToDevice));
     return RL_SUCCESS;
 }
@@ -430,7 +442,7 @@ int32_t upload_segments(const std::vector<BakeItem>& items, int unit, uint32_t c
 // Direct light over items (one launch); the caller synchronises.
 int32_t run_direct_items(rl_relight* ctx, const std::vector<BakeItem>& items) {
     DeviceSegs ds;
-    int32_t st = upload_segments(items, kUnitLumel, 1, ds);
+    int32_t st = upload_segments(ctx, 0, items, kUnitLumel, 1, ds);
     if (st) return st;
     hipError_t e = hipSuccess;
     if (ds.total) {
@@ -454,9 +466,9 @@ int32_t run_ao_items(rl_relight* ctx, const std::vector<BakeItem>& items) {
     for (const BakeItem& it : items) refill = refill && it.ao.samples >= 4;
     const uint32_t chunk = ctx->tuning.ao_chunk_lumels;
     DeviceSegs work, lum;
-    int32_t st = upload_segments(items, refill ? kUnitChunk : kUnitPair, chunk, work);
+    int32_t st = upload_segments(ctx, 0, items, refill ? kUnitChunk : kUnitPair, chunk, work);
     if (st) return st;
-    if ((st = upload_segments(items, kUnitLumel, 1, lum))) return st;
+    if ((st = upload_segments(ctx, 1, items, kUnitLumel, 1, lum))) return st;
     hipError_t e = hipSuccess;
     if (work.total) {
         e = kernel_begin(ctx, RL_KERNEL_AO_COUNT);
@@ -504,7 +516,7 @@ int32_t run_ao(rl_relight* ctx, rl_scene* scene, rl_lightmap* const* maps, int n
 // Final gather over items (each with its FG settings and its scene's photon tables).
 int32_t run_fg_items(rl_relight* ctx, const std::vector<BakeItem>& items) {
     DeviceSegs ds;
-    int32_t st = upload_segments(items, kUnitLumel, 1, ds);
+    int32_t st = upload_segments(ctx, 0, items, kUnitLumel, 1, ds);
     if (st) return st;
     hipError_t e = hipSuccess;
     if (ds.total) {
@@ -657,6 +669,8 @@ int32_t rl_destroy(rl_relight* ctx) {
     if (!ctx) return RL_SUCCESS;
     if (ctx->d_rays) hipFree(ctx->d_rays);
     if (ctx->d_queue) hipFree(ctx->d_queue);
+    for (BakeSeg* b : ctx->seg_buf)
+        if (b) hipFree(b);
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
     for (hipEvent_t e : ctx->pool) hipEventDestroy(e);
@@ -1553,16 +1567,20 @@ struct Job {
     bool has_ao = false;
     rl_indirect_settings ind{};
     std::vector<float> E, R;
-    int32_t state = RL_JOB_QUEUED;
-    int32_t status = RL_SUCCESS;
+    // Written by the worker threads while callers poll: atomics (the error string is written
+    // before the state turns FAILED).
+    std::atomic<int32_t> state{RL_JOB_QUEUED};
+    std::atomic<int32_t> status{RL_SUCCESS};
+    std::atomic<uint64_t> rays{0};
     double submit_ms = 0, admit_ms = 0, done_ms = 0;
-    uint64_t rays = 0, footprint = 0;
+    uint64_t footprint = 0;
     // DIRECT_AO: 0 direct, 1 AO. INDIRECT: 0 photons, 1 final gather. FINAL_GATHER: 1 final gather.
     // RADIOSITY: 0 form factors, 1 solve. 2: done.
     int phase = 0;
     size_t map_i = 0;     // next lightmap of the phase
     uint32_t cursor = 0;  // next lumel of that lightmap
-    bool cancel = false;
+    std::atomic<bool> cancel{false};
+    bool taken = false;                    // radiosity: picked up by the auxiliary worker
     bool prepared = false;                 // lumels uploaded
     std::unique_ptr<PhotonTables> tables;  // final gather: the scene's photon tables
     std::string error;    // message of the failing call, handed to the caller's rl_last_error
@@ -1572,10 +1590,11 @@ struct Job {
 
 struct rl_server {
     rl_relight* ctx = nullptr;
+    rl_relight* aux = nullptr;  // second context (own stream) for radiosity jobs
     rl_server_config cfg{};
     std::mutex m;
     std::condition_variable cv_work, cv_done;
-    std::thread worker;
+    std::thread worker, aux_worker;
     bool stop = false;
     std::map<uint64_t, std::unique_ptr<Job>> jobs;
     std::deque<Job*> queue;
@@ -1588,7 +1607,10 @@ struct rl_server {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     void loop();
-    void step(std::vector<Job*>& js);
+    void aux_loop();
+    void admit_locked();
+    void retire_locked();
+    uint64_t step(std::vector<Job*>& js);
 };
 
 namespace {
@@ -1598,9 +1620,9 @@ bool job_finished(const Job* j) {
 }
 
 void fail_job(Job* j, int32_t st) {
-    j->state = RL_JOB_FAILED;
-    j->status = st;
     j->error = g_error;
+    j->status = st;
+    j->state = RL_JOB_FAILED;
 }
 
 RayPhase ray_phase(const Job* j) {
@@ -1649,30 +1671,15 @@ uint64_t take_lumels(Job* j, uint64_t budget, const BakeItem& proto, std::vector
 
 }  // namespace
 
-void rl_server::step(std::vector<Job*>& js) {
-    // Whole-step phases: photon emission and radiosity (form factors, then the solve).
+uint64_t rl_server::step(std::vector<Job*>& js) {
+    uint64_t rays = 0;
+    // Whole-step phase: photon emission.
     for (Job* j : js) {
-        if (job_finished(j)) continue;
-        int32_t st = RL_SUCCESS;
-        if (j->desc.kind == RL_JOB_INDIRECT && j->phase == 0) {
-            st = rl_emit_photons(ctx, j->desc.scene, &j->ind);
-            j->rays += ctx->stats.rays;
-            stats.rays += ctx->stats.rays;
-            j->phase = 1;
-        } else if (j->desc.kind == RL_JOB_RADIOSITY) {
-            if (j->phase == 0) {
-                st = rl_radiosity_build_form_factors(ctx, j->desc.radiosity, j->desc.rays_per_patch, j->desc.ff_seed,
-                                                     j->desc.ff_max_distance);
-                j->rays += ctx->stats.rays;
-                stats.rays += ctx->stats.rays;
-                j->phase = 1;
-            } else {
-                rl_solve_info info{};
-                st = rl_radiosity_solve(ctx, j->desc.radiosity, j->E.data(), j->R.data(), j->desc.num_rhs,
-                                        j->desc.tolerance, 40, 16, j->desc.out_radiosity, &info);
-                if (!st) j->state = RL_JOB_DONE;
-            }
-        }
+        if (job_finished(j) || j->desc.kind != RL_JOB_INDIRECT || j->phase != 0) continue;
+        const int32_t st = rl_emit_photons(ctx, j->desc.scene, &j->ind);
+        j->rays += ctx->stats.rays;
+        rays += ctx->stats.rays;
+        j->phase = 1;
         if (st) fail_job(j, st);
     }
     // Ray phases: upload lumels (and photon tables) on entry, then share the quantum fairly.
@@ -1696,7 +1703,7 @@ void rl_server::step(std::vector<Job*>& js) {
         if (st) fail_job(j, st);
         else nray++;
     }
-    if (nray == 0) return;
+    if (nray == 0) return rays;
     const uint64_t share = std::max<uint64_t>(cfg.quantum_rays / nray, 4096);
     std::vector<BakeItem> items[4];
     std::vector<Job*> in_step;
@@ -1726,65 +1733,123 @@ void rl_server::step(std::vector<Job*>& js) {
     if (!st && !items[kPhaseAo].empty()) st = run_ao_items(ctx, items[kPhaseAo]);
     if (!st && !items[kPhaseGather].empty()) st = run_fg_items(ctx, items[kPhaseGather]);
     if (!st) st = end_call(ctx, tm);
-    stats.rays += ctx->stats.rays;
+    rays += ctx->stats.rays;
     for (Job* j : in_step) {
         if (st) fail_job(j, st);
         else if (j->map_i >= j->maps.size()) advance(j);
         if (job_finished(j)) j->tables.reset();
     }
+    return rays;
 }
 
+void rl_server::admit_locked() {
+    while (!queue.empty() && (int)active.size() < cfg.max_active) {
+        Job* j = queue.front();
+        if (j->cancel) {
+            queue.pop_front();
+            j->state = RL_JOB_CANCELLED;
+            j->done_ms = now_ms();
+            stats.cancelled++;
+            cv_done.notify_all();
+            continue;
+        }
+        if (mem_used + j->footprint > cfg.memory_budget) break;
+        queue.pop_front();
+        j->state = RL_JOB_RUNNING;
+        j->admit_ms = now_ms();
+        mem_used += j->footprint;
+        stats.peak_memory = std::max(stats.peak_memory, mem_used);
+        active.push_back(j);
+        cv_work.notify_all();
+    }
+}
+
+void rl_server::retire_locked() {
+    bool any = false;
+    for (size_t k = 0; k < active.size();) {
+        Job* j = active[k];
+        const bool mine = j->desc.kind != RL_JOB_RADIOSITY;  // the auxiliary worker settles its own
+        if (mine && j->cancel && !job_finished(j)) j->state = RL_JOB_CANCELLED;
+        if (job_finished(j)) {
+            j->done_ms = now_ms();
+            j->tables.reset();
+            mem_used -= j->footprint;
+            if (j->state == RL_JOB_DONE) stats.completed++;
+            else if (j->state == RL_JOB_FAILED) stats.failed++;
+            else stats.cancelled++;
+            active.erase(active.begin() + k);
+            any = true;
+        } else {
+            k++;
+        }
+    }
+    if (any) {
+        cv_done.notify_all();
+        cv_work.notify_all();
+    }
+}
+
+// Ray work of every admitted job (direct, AO, final gather, photon emission) in shared steps.
 void rl_server::loop() {
     std::unique_lock<std::mutex> lk(m);
     for (;;) {
-        while (!queue.empty() && (int)active.size() < cfg.max_active) {
-            Job* j = queue.front();
-            if (j->cancel) {
-                queue.pop_front();
-                j->state = RL_JOB_CANCELLED;
-                j->done_ms = now_ms();
-                stats.cancelled++;
-                cv_done.notify_all();
-                continue;
-            }
-            if (mem_used + j->footprint > cfg.memory_budget) break;
-            queue.pop_front();
-            j->state = RL_JOB_RUNNING;
-            j->admit_ms = now_ms();
-            mem_used += j->footprint;
-            stats.peak_memory = std::max(stats.peak_memory, mem_used);
-            active.push_back(j);
-        }
-        if (active.empty()) {
-            if (stop) break;
+        retire_locked();
+        admit_locked();
+        std::vector<Job*> js;
+        for (Job* j : active)
+            if (j->desc.kind != RL_JOB_RADIOSITY && !job_finished(j)) js.push_back(j);
+        if (js.empty()) {
+            if (stop && active.empty() && queue.empty()) break;
             cv_work.wait(lk);
             continue;
         }
-        std::vector<Job*> js(active);
         lk.unlock();
         const double t = now_ms();
-        step(js);
+        const uint64_t rays = step(js);
         lk.lock();
+        stats.rays += rays;
         stats.busy_ms += now_ms() - t;
         stats.steps++;
-        bool any = false;
-        for (size_t k = 0; k < active.size();) {
-            Job* j = active[k];
-            if (j->cancel && !job_finished(j)) j->state = RL_JOB_CANCELLED;
-            if (job_finished(j)) {
-                j->done_ms = now_ms();
-                j->tables.reset();
-                mem_used -= j->footprint;
-                if (j->state == RL_JOB_DONE) stats.completed++;
-                else if (j->state == RL_JOB_FAILED) stats.failed++;
-                else stats.cancelled++;
-                active.erase(active.begin() + k);
-                any = true;
-            } else {
-                k++;
+    }
+    cv_work.notify_all();
+}
+
+// Radiosity jobs (form factors, then the solve) on the auxiliary context's stream, so that their
+// GEMM sweeps overlap the ray steps.
+void rl_server::aux_loop() {
+    std::unique_lock<std::mutex> lk(m);
+    for (;;) {
+        Job* j = nullptr;
+        for (Job* a : active)
+            if (a->desc.kind == RL_JOB_RADIOSITY && !a->taken && !job_finished(a)) {
+                j = a;
+                break;
             }
+        if (!j) {
+            if (stop && active.empty() && queue.empty()) break;
+            cv_work.wait(lk);
+            continue;
         }
-        if (any) cv_done.notify_all();
+        j->taken = true;
+        lk.unlock();
+        int32_t st = RL_SUCCESS;
+        uint64_t rays = 0;
+        if (!j->cancel) {
+            st = rl_radiosity_build_form_factors(aux, j->desc.radiosity, j->desc.rays_per_patch, j->desc.ff_seed,
+                                                 j->desc.ff_max_distance);
+            rays = aux->stats.rays;
+        }
+        if (!st && !j->cancel) {
+            rl_solve_info info{};
+            st = rl_radiosity_solve(aux, j->desc.radiosity, j->E.data(), j->R.data(), j->desc.num_rhs,
+                                    j->desc.tolerance, 40, 16, j->desc.out_radiosity, &info);
+        }
+        lk.lock();
+        j->rays += rays;
+        stats.rays += rays;
+        if (st) fail_job(j, st);
+        else j->state = j->cancel ? RL_JOB_CANCELLED : RL_JOB_DONE;
+        cv_work.notify_all();
     }
 }
 
@@ -1839,7 +1904,15 @@ int32_t rl_server_create(rl_relight* ctx, const rl_server_config* cfg, rl_server
     if (!s) return fail(RL_OUT_OF_MEMORY, "server allocation failed");
     s->ctx = ctx;
     s->cfg = *cfg;
+    int32_t st = rl_create(&s->aux);
+    if (st) {
+        delete s;
+        return st;
+    }
+    s->aux->tuning = ctx->tuning;
+    s->aux->max_blocks = ctx->max_blocks;
     s->worker = std::thread([s]() { s->loop(); });
+    s->aux_worker = std::thread([s]() { s->aux_loop(); });
     *out = s;
     return RL_SUCCESS;
 }
@@ -1894,7 +1967,7 @@ int32_t rl_server_submit(rl_server* s, const rl_job_desc* d, uint64_t* out_id) {
     s->jobs[raw->id] = std::move(j);
     s->queue.push_back(raw);
     if (out_id) *out_id = raw->id;
-    s->cv_work.notify_one();
+    s->cv_work.notify_all();
     return RL_SUCCESS;
 }
 
@@ -1941,7 +2014,7 @@ int32_t rl_server_cancel(rl_server* s, uint64_t id) {
     auto it = s->jobs.find(id);
     if (it == s->jobs.end()) return fail(RL_INVALID_ARGUMENT, "unknown job id");
     it->second->cancel = true;
-    s->cv_work.notify_one();
+    s->cv_work.notify_all();
     return RL_SUCCESS;
 }
 
@@ -1961,11 +2034,13 @@ int32_t rl_server_destroy(rl_server* s) {
         s->cv_work.notify_all();
     }
     s->worker.join();
+    s->aux_worker.join();
     // Jobs still queued when the loop stopped were never admitted.
     for (Job* j : s->queue) {
         j->state = RL_JOB_CANCELLED;
         s->stats.cancelled++;
     }
+    rl_destroy(s->aux);
     delete s;
     return RL_SUCCESS;
 }

@@ -270,9 +270,11 @@ def _box_mesh(lo, hi):
 def heightfield(seed: int = 3, cells: int = 512, cell_size: float = 0.5, tile_cells: int = 64,
                 octaves: int = 4, amplitude: float = 3.0, num_boxes: int = 2000, map_size: int = 2048,
                 photonmap_size: int = 128, photons_per_light: int = 1_000_000, fg_samples: int = 512,
-                knn_k: int = 64, knn_radius: float = 1.0, box_meshes: int = 8) -> SceneSpec:
+                knn_k: int = 64, knn_radius: float = 1.0, box_meshes: int = 8,
+                tile_map_sides: Optional[Tuple[int, int]] = None) -> SceneSpec:
     """fBm terrain in tiles (one mesh each, one shared lightmap, a photonmap per tile), boxes
-    flattened into a few meshes without a lightmap, four rectangular area lights."""
+    flattened into a few meshes without a lightmap, four rectangular area lights. With
+    tile_map_sides = (lo, hi), every tile gets its own lightmap of side round(lo * (hi/lo)^U)."""
     from ._capi import RL_GATHER_KNN
     rng = np.random.default_rng(seed)
     H = np.zeros((cells + 1, cells + 1))
@@ -294,19 +296,30 @@ def heightfield(seed: int = 3, cells: int = 512, cell_size: float = 0.5, tile_ce
     meshes = []
     nt = cells // tile_cells
     tile_albedo = rng.uniform(0.3, 0.8, (nt * nt, 3))
+    sides = None
+    if tile_map_sides is not None:
+        lo_s, hi_s = tile_map_sides
+        sides = [max(16, int(round(lo_s * (hi_s / lo_s) ** x))) for x in rng.uniform(0.0, 1.0, nt * nt)]
     for tj in range(nt):
         for ti in range(nt):
             j0, i0 = tj * tile_cells, ti * tile_cells
             jj, ii = np.meshgrid(np.arange(j0, j0 + tile_cells + 1), np.arange(i0, i0 + tile_cells + 1), indexing="ij")
             pos = np.stack([ii * cell_size, H[jj, ii], jj * cell_size], -1).reshape(-1, 3)
             nrm = N[jj, ii].reshape(-1, 3)
-            uv = np.stack([(ii * tex_per_cell + 0.25) * scale, (jj * tex_per_cell + 0.25) * scale], -1).reshape(-1, 2)
+            t = tj * nt + ti
+            if sides is None:
+                uv = np.stack([(ii * tex_per_cell + 0.25) * scale, (jj * tex_per_cell + 0.25) * scale], -1)
+                lm_index = 0
+            else:
+                tpc, sc = sides[t] / tile_cells, 1.0 / (sides[t] + 0.5)
+                uv = np.stack([((ii - i0) * tpc + 0.25) * sc, ((jj - j0) * tpc + 0.25) * sc], -1)
+                lm_index = t
+            uv = uv.reshape(-1, 2)
             r = tile_cells + 1
             a = (np.arange(tile_cells)[:, None] * r + np.arange(tile_cells)[None, :]).reshape(-1)
             idx = np.stack([a, a + r, a + r + 1, a, a + r + 1, a + 1], -1).reshape(-1)
-            t = tj * nt + ti
             meshes.append(MeshSpec(pos.astype(np.float32), nrm.astype(np.float32), uv.astype(np.float32),
-                                   idx.astype(np.uint32), 0,
+                                   idx.astype(np.uint32), lm_index,
                                    np.array([*tile_albedo[t], 1.0], np.float32), photonmap=t))
     # Boxes: 1 m cubes sunk 0.3 m into the terrain at uniform positions.
     ext = cells * cell_size
@@ -341,10 +354,32 @@ def heightfield(seed: int = 3, cells: int = 512, cell_size: float = 0.5, tile_ce
         sky_color=(0.2, 0.25, 0.3), ambient_color=(0.0, 0.0, 0.0), seed=seed, sampling=RL_SAMPLE_UNIFORM,
         photons_per_light=photons_per_light, termination=RL_PHOTON_RUSSIAN_ROULETTE,
         gather_mode=RL_GATHER_KNN, knn_k=knn_k, knn_radius=knn_radius)
-    return SceneSpec("heightfield", meshes, lights, [(map_size, map_size)], None,
+    maps = [(map_size, map_size)] if sides is None else [(x, x) for x in sides]
+    return SceneSpec("heightfield", meshes, lights, maps, None,
                      {"seed": seed, "cells": cells, "cell_size": cell_size, "octaves": octaves,
                       "amplitude": amplitude, "boxes": num_boxes, "tiles": nt * nt},
                      photonmaps=[(photonmap_size, photonmap_size)] * (nt * nt), indirect=ind)
+
+
+def with_photonmaps(spec: SceneSpec, size: int = 64, photons_per_light: int = 200_000,
+                    fg_samples: int = 64, knn_radius: float = 2.0, seed: int = 5) -> SceneSpec:
+    """Gives every lightmapped mesh a photonmap of size x size and an albedo U(0.3, 0.8), and sets
+    final-gather settings (Russian roulette, k = 64 nearest-photon gather)."""
+    from ._capi import RL_GATHER_KNN
+    rng = np.random.default_rng(seed)
+    k = 0
+    for m in spec.meshes:
+        if m.lightmap is not None and m.lightmap >= 0:
+            m.albedo = np.array([*rng.uniform(0.3, 0.8, 3), 1.0], np.float32)
+            m.photonmap = k
+            k += 1
+    spec.photonmaps = [(size, size)] * k
+    spec.indirect = IndirectLightSettings(
+        photon_pass_count=1, photon_bounce_count=3, photon_energy_threshold=0.0, photon_max_distance=200.0,
+        final_gather_samples=fg_samples, final_gather_distance=50.0, final_gather_radius=7,
+        sky_color=(0.2, 0.25, 0.3), seed=seed, photons_per_light=photons_per_light,
+        termination=RL_PHOTON_RUSSIAN_ROULETTE, gather_mode=RL_GATHER_KNN, knn_k=64, knn_radius=knn_radius)
+    return spec
 
 
 def build_gpu(relight, spec: SceneSpec):
