@@ -65,6 +65,7 @@ __global__ __launch_bounds__(kFfBlock) void k_form_factor_rows(SceneDev S, const
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // Y_part[split][c][row] = sum over this split's k of F[row][k] * X[k][c] with
 // v_mfma_f32_16x16x32_bf16: each wavefront owns 16 rows and all NT*16 columns. A operand lane l
@@ -79,14 +80,48 @@ __global__ __launch_bounds__(256) void k_ff_gemm(const uint16_t* __restrict__ F,
                                                  uint32_t klen, float* __restrict__ Yp) {
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t row0 = (blockIdx.x * 4u + wave) * 16u;
-    const uint32_t k0 = blockIdx.y * klen;
+    const uint32_t k0 = blockIdx.y * klen, kend = k0 + klen;
     const uint32_t sub = lane >> 4, col = lane & 15u;
     f32x4 acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; t++) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     const uint16_t* frow = F + (size_t)(row0 + col) * stride + 8u * sub;
-    for (uint32_t k = k0; k < k0 + klen; k += 32) {
-        const uint4 raw = *reinterpret_cast<const uint4*>(frow + k);
+    // F is streamed once per launch (non-temporal, so X stays in L2); the next step's F and X
+    // fragments are loaded before this step's conversions and MFMAs.
+    u32x4 raw_next = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(frow + k0));
+    bf16x8 b1n[NT], b2n[NT], b3n[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+        const size_t xo = (size_t)(16u * t + col) * npad + k0 + 8u * sub;
+        b1n[t] = *reinterpret_cast<const bf16x8*>(X1 + xo);
+        if (ACC) {
+            b2n[t] = *reinterpret_cast<const bf16x8*>(X2 + xo);
+            b3n[t] = *reinterpret_cast<const bf16x8*>(X3 + xo);
+        }
+    }
+    for (uint32_t k = k0; k < kend; k += 32) {
+        const u32x4 raw = raw_next;
+        bf16x8 b1[NT], b2[NT], b3[NT];
+#pragma unroll
+        for (int t = 0; t < NT; t++) {
+            b1[t] = b1n[t];
+            if (ACC) {
+                b2[t] = b2n[t];
+                b3[t] = b3n[t];
+            }
+        }
+        if (k + 32 < kend) {
+            raw_next = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(frow + k + 32));
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                const size_t xo = (size_t)(16u * t + col) * npad + k + 32 + 8u * sub;
+                b1n[t] = *reinterpret_cast<const bf16x8*>(X1 + xo);
+                if (ACC) {
+                    b2n[t] = *reinterpret_cast<const bf16x8*>(X2 + xo);
+                    b3n[t] = *reinterpret_cast<const bf16x8*>(X3 + xo);
+                }
+            }
+        }
         const uint32_t w[4] = {raw.x, raw.y, raw.z, raw.w};
         bf16x8 ahi, alo;
 #pragma unroll
@@ -94,20 +129,16 @@ __global__ __launch_bounds__(256) void k_ff_gemm(const uint16_t* __restrict__ F,
             const float fc = (float)((w[j >> 1] >> ((j & 1) * 16)) & 0xFFFFu);
             const __bf16 h = (__bf16)fc;
             ahi[j] = h;
-            alo[j] = (__bf16)(fc - (float)h);
+            if (ACC) alo[j] = (__bf16)(fc - (float)h);
         }
 #pragma unroll
         for (int t = 0; t < NT; t++) {
-            const size_t xo = (size_t)(16u * t + col) * npad + k + 8u * sub;
-            const bf16x8 b1 = *reinterpret_cast<const bf16x8*>(X1 + xo);
-            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, b1, acc[t], 0, 0, 0);
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, b1[t], acc[t], 0, 0, 0);
             if (ACC) {
-                const bf16x8 b2 = *reinterpret_cast<const bf16x8*>(X2 + xo);
-                const bf16x8 b3 = *reinterpret_cast<const bf16x8*>(X3 + xo);
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, b2, acc[t], 0, 0, 0);
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, b3, acc[t], 0, 0, 0);
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, b1, acc[t], 0, 0, 0);
-                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, b2, acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, b2[t], acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, b3[t], acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, b1[t], acc[t], 0, 0, 0);
+                acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, b2[t], acc[t], 0, 0, 0);
             }
         }
     }
@@ -118,6 +149,23 @@ __global__ __launch_bounds__(256) void k_ff_gemm(const uint16_t* __restrict__ F,
 #pragma unroll
         for (int i = 0; i < 4; i++) out[(size_t)(16u * t + col) * npad + row0 + 4u * sub + i] = acc[t][i];
     }
+}
+
+// Row-major (n x m) -> column-major padded (npad x mpad, zeros outside), and back.
+__global__ void k_ff_to_cols(const float* __restrict__ in, uint32_t n, uint32_t m, uint32_t npad, uint32_t mpad,
+                             float* __restrict__ out) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (uint64_t)npad * mpad) return;
+    const uint32_t c = (uint32_t)(g / npad), i = (uint32_t)(g % npad);
+    out[g] = (i < n && c < m) ? in[(size_t)i * m + c] : 0.0f;
+}
+
+__global__ void k_ff_to_rows(const float* __restrict__ cols, uint32_t n, uint32_t m, uint32_t npad,
+                             float* __restrict__ out) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (uint64_t)n * m) return;
+    const uint32_t i = (uint32_t)(g / m), c = (uint32_t)(g % m);
+    out[g] = cols[(size_t)c * npad + i];
 }
 
 __device__ __forceinline__ void split3(float x, __bf16& a, __bf16& b, __bf16& c) {
@@ -232,6 +280,19 @@ hipError_t launch_ff_gemm(hipStream_t st, bool accurate, const uint16_t* F, uint
             return hipErrorInvalidValue;
     }
 #undef RL_GEMM_CASE
+    return hipGetLastError();
+}
+
+hipError_t launch_ff_to_cols(hipStream_t st, const float* in, uint32_t n, uint32_t m, uint32_t npad, uint32_t mpad,
+                            float* out) {
+    hipLaunchKernelGGL(k_ff_to_cols, dim3(blocks_for((uint64_t)npad * mpad, 256)), dim3(256), 0, st, in, n, m, npad,
+                       mpad, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_ff_to_rows(hipStream_t st, const float* cols, uint32_t n, uint32_t m, uint32_t npad, float* out) {
+    if ((uint64_t)n * m == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ff_to_rows, dim3(blocks_for((uint64_t)n * m, 256)), dim3(256), 0, st, cols, n, m, npad, out);
     return hipGetLastError();
 }
 

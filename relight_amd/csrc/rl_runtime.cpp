@@ -1310,7 +1310,18 @@ struct rl_radiosity {
     uint16_t* d_F = nullptr;
     bool built = false;
     uint32_t rays = 0;
+    // Solve buffers, kept between solves (sized for sol_mpad columns and sol_nsplit partials).
+    uint32_t sol_mpad = 0, sol_nsplit = 0;
+    float *E = nullptr, *RHO = nullptr, *B = nullptr, *R = nullptr, *D = nullptr, *Yp = nullptr, *part = nullptr;
+    float* stage = nullptr;  // row-major emission / reflectance / result
+    __bf16 *X1 = nullptr, *X2 = nullptr, *X3 = nullptr;
+    void free_solve() {
+        dev_free(E); dev_free(RHO); dev_free(B); dev_free(R); dev_free(D); dev_free(Yp); dev_free(part);
+        dev_free(stage); dev_free(X1); dev_free(X2); dev_free(X3);
+        sol_mpad = sol_nsplit = 0;
+    }
     ~rl_radiosity() {
+        free_solve();
         dev_free(d_patches);
         dev_free(d_tri_patch);
         dev_free(d_F);
@@ -1408,46 +1419,50 @@ int32_t rl_radiosity_solve(rl_relight* ctx, rl_radiosity* r, const float* emissi
     for (uint32_t t : tiles) {
         if (t * 16 >= num_rhs) { nt = t; break; }
     }
-    const uint32_t n = r->n, npad = r->npad, mpad = nt * 16, nsplit = 4;
+    const uint32_t n = r->n, npad = r->npad, mpad = nt * 16;
+    // Split-K so that a launch has about 8 wavefronts per SIMD (npad/16 row tiles x nsplit), with
+    // at least 64 columns of K per split.
+    uint32_t nsplit = 4;
+    while (nsplit < 32 && (npad / 16) * nsplit < 8192 && npad / (nsplit * 2) >= 64 && npad % (nsplit * 2 * 32) == 0)
+        nsplit *= 2;
     const size_t vec = (size_t)mpad * npad;
-    // Row-major (patch, rhs) -> column-major padded.
-    std::vector<float> hE(vec, 0.0f), hR(vec, 0.0f);
     double e2 = 0.0;
-    for (uint32_t i = 0; i < n; i++) {
-        for (uint32_t c = 0; c < num_rhs; c++) {
-            const float ev = emission[(size_t)i * num_rhs + c], rv = reflectance[(size_t)i * num_rhs + c];
-            hE[(size_t)c * npad + i] = ev;
-            hR[(size_t)c * npad + i] = rv;
-            e2 += (double)ev * ev;
-        }
-    }
-    float *E = nullptr, *RHO = nullptr, *B = nullptr, *R = nullptr, *D = nullptr, *Yp = nullptr, *part = nullptr;
-    __bf16 *X1 = nullptr, *X2 = nullptr, *X3 = nullptr;
-    auto cleanup = [&]() {
-        dev_free(E); dev_free(RHO); dev_free(B); dev_free(R); dev_free(D); dev_free(Yp); dev_free(part);
-        dev_free(X1); dev_free(X2); dev_free(X3);
-    };
+    for (size_t i = 0; i < (size_t)n * num_rhs; i++) e2 += (double)emission[i] * emission[i];
     const uint32_t nblk = npad / 256;
-    hipError_t e = dev_alloc(&E, vec);
-    if (e == hipSuccess) e = dev_alloc(&RHO, vec);
-    if (e == hipSuccess) e = dev_alloc(&B, vec);
-    if (e == hipSuccess) e = dev_alloc(&R, vec);
-    if (e == hipSuccess) e = dev_alloc(&D, vec);
-    if (e == hipSuccess) e = dev_alloc(&Yp, vec * nsplit);
-    if (e == hipSuccess) e = dev_alloc(&part, (size_t)mpad * nblk);
-    if (e == hipSuccess) e = dev_alloc(&X1, vec);
-    if (e == hipSuccess) e = dev_alloc(&X2, vec);
-    if (e == hipSuccess) e = dev_alloc(&X3, vec);
-    if (e != hipSuccess) {
-        cleanup();
-        return fail(RL_OUT_OF_MEMORY, "radiosity solve buffers");
+    hipError_t e = hipSuccess;
+    if (r->sol_mpad != mpad || r->sol_nsplit != nsplit) {
+        r->free_solve();
+        e = dev_alloc(&r->E, vec);
+        if (e == hipSuccess) e = dev_alloc(&r->RHO, vec);
+        if (e == hipSuccess) e = dev_alloc(&r->B, vec);
+        if (e == hipSuccess) e = dev_alloc(&r->R, vec);
+        if (e == hipSuccess) e = dev_alloc(&r->D, vec);
+        if (e == hipSuccess) e = dev_alloc(&r->Yp, vec * nsplit);
+        if (e == hipSuccess) e = dev_alloc(&r->part, (size_t)mpad * nblk);
+        if (e == hipSuccess) e = dev_alloc(&r->stage, 2 * (size_t)npad * mpad);
+        if (e == hipSuccess) e = dev_alloc(&r->X1, vec);
+        if (e == hipSuccess) e = dev_alloc(&r->X2, vec);
+        if (e == hipSuccess) e = dev_alloc(&r->X3, vec);
+        if (e != hipSuccess) {
+            r->free_solve();
+            return fail(RL_OUT_OF_MEMORY, "radiosity solve buffers");
+        }
+        r->sol_mpad = mpad;
+        r->sol_nsplit = nsplit;
     }
+    float *E = r->E, *RHO = r->RHO, *B = r->B, *R = r->R, *D = r->D, *Yp = r->Yp, *part = r->part;
+    __bf16 *X1 = r->X1, *X2 = r->X2, *X3 = r->X3;
+    float* stE = r->stage;
+    float* stR = r->stage + (size_t)npad * mpad;
     CallTimer tm(ctx);
     int32_t st = begin_call(ctx);
-    if (st) { cleanup(); return st; }
-    e = hipMemcpy(E, hE.data(), vec * sizeof(float), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(RHO, hR.data(), vec * sizeof(float), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(D, hE.data(), vec * sizeof(float), hipMemcpyHostToDevice);
+    if (st) return st;
+    const size_t rowbytes = (size_t)n * num_rhs * sizeof(float);
+    e = hipMemcpyAsync(stE, emission, rowbytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(stR, reflectance, rowbytes, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = launch_ff_to_cols(ctx->stream, stE, n, num_rhs, npad, mpad, E);
+    if (e == hipSuccess) e = launch_ff_to_cols(ctx->stream, stR, n, num_rhs, npad, mpad, RHO);
+    if (e == hipSuccess) e = hipMemcpyAsync(D, E, vec * sizeof(float), hipMemcpyDeviceToDevice, ctx->stream);
     // B0 = E.
     if (e == hipSuccess) e = launch_ff_apply(ctx->stream, npad, mpad, B, D, 1, X1, X2, X3);
     const float inv_rays = 1.0f / (float)r->rays;
@@ -1476,12 +1491,10 @@ int32_t rl_radiosity_solve(rl_relight* ctx, rl_radiosity* r, const float* emissi
         }
         if (e == hipSuccess) e = launch_ff_apply(ctx->stream, npad, mpad, B, D, 0, X1, X2, X3);
     }
-    std::vector<float> hB(vec);
-    if (e == hipSuccess) e = hipMemcpy(hB.data(), B, vec * sizeof(float), hipMemcpyDeviceToHost);
-    cleanup();
+    if (e == hipSuccess) e = launch_ff_to_rows(ctx->stream, B, n, num_rhs, npad, stE);
+    if (e == hipSuccess) e = hipMemcpyAsync(out_radiosity, stE, rowbytes, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return fail(RL_DEVICE_ERROR, std::string("radiosity solve: ") + hipGetErrorString(e));
-    for (uint32_t i = 0; i < n; i++)
-        for (uint32_t c = 0; c < num_rhs; c++) out_radiosity[(size_t)i * num_rhs + c] = hB[(size_t)c * npad + i];
     st = end_call(ctx, tm);
     if (info) {
         info->outer_iterations = outer + (outer < max_outer ? 1 : 0);
