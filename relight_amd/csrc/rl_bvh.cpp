@@ -1,9 +1,10 @@
 // Host BVH builder: binned-SAH binary tree, collapsed to the 4-wide node the kernels traverse.
 //
 // Replaces the acceleration structure Embree builds in rtcCommit (rt/Embree.cpp:55-58 via
-// Scene::end, scene/Scene.cpp:122-142). Boxes are the exact fp32 min/max of the vertices; the
-// traversal kernels make the slab test conservative instead (Ize, "Robust BVH Ray Traversal",
-// JCGT 2(2), 2013), so no padding is needed here.
+// Scene::end, scene/Scene.cpp:122-142). Child boxes are the fp32 min/max of their vertices,
+// widened by a padding of 2^-20 of the largest coordinate magnitude: the kernels' fma slab test
+// shifts a plane by at most |o| * 2^-24, and the exit distance is scaled by 1 + 2*gamma(3)
+// (Ize, "Robust BVH Ray Traversal", JCGT 2(2), 2013), so no box the exact ray crosses is missed.
 #include "rl_bvh.h"
 
 #include <algorithm>

@@ -46,6 +46,8 @@ hipError_t launch_form_factor_rows(hipStream_t st, const SceneDev& S, int cap, c
 hipError_t launch_ff_gemm(hipStream_t st, bool accurate, const uint16_t* F, uint32_t stride,
                           const __bf16* X1, const __bf16* X2, const __bf16* X3, uint32_t npad,
                           uint32_t mpad, uint32_t nsplit, float* Yp);
+hipError_t launch_ff_untile(hipStream_t st, const uint16_t* F, uint32_t npad, uint32_t n, uint32_t first,
+                           uint32_t rows, uint16_t* out);
 hipError_t launch_ff_to_cols(hipStream_t st, const float* in, uint32_t n, uint32_t m, uint32_t npad, uint32_t mpad,
                             float* out);
 hipError_t launch_ff_to_rows(hipStream_t st, const float* cols, uint32_t n, uint32_t m, uint32_t npad, float* out);

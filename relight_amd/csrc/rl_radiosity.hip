@@ -25,6 +25,19 @@ using namespace trav;
 
 constexpr int kFfBlock = 128;
 
+// F (u16 hit counts, npad x npad) is stored in 16 x 32 tiles, tile-row-major: the 16 rows x 32
+// columns one MFMA A operand covers are 1 KB contiguous, so a wavefront's load of them is one
+// contiguous 1 KB.
+__device__ __forceinline__ size_t f_index(uint32_t r, uint32_t k, uint32_t npad) {
+    return ((size_t)(r >> 4) * (npad >> 5) + (k >> 5)) * 512u + (r & 15u) * 32u + (k & 31u);
+}
+
+// The BF16 parts of the iterate (x1, x2, x3; npad x mpad) are stored in 32-row blocks, each block
+// column-major: the 32 rows x 16 columns one MFMA B operand covers are 1 KB contiguous.
+__device__ __forceinline__ size_t x_index(uint32_t i, uint32_t c, uint32_t mpad) {
+    return (size_t)(i >> 5) * (mpad * 32u) + c * 32u + (i & 31u);
+}
+
 // One workgroup per patch row: `rays` closest-hit rays from uniform points on the patch with
 // cosine-weighted directions; hits are histogrammed per patch in LDS (two u16 counters per word,
 // counts <= 65535) and the row is written out as counts.
@@ -58,8 +71,8 @@ __global__ __launch_bounds__(kFfBlock) void k_form_factor_rows(SceneDev S, const
         }
     }
     __syncthreads();
-    uint32_t* out = reinterpret_cast<uint32_t*>(F + (size_t)row * P.row_stride);
-    for (uint32_t w = threadIdx.x; w < words; w += kFfBlock) out[w] = hist[w];
+    uint32_t* out = reinterpret_cast<uint32_t*>(F);
+    for (uint32_t w = threadIdx.x; w < words; w += kFfBlock) out[f_index(row, 2u * w, P.row_stride) >> 1] = hist[w];
     count_rays(rays, rays_out);
 }
 
@@ -85,14 +98,17 @@ __global__ __launch_bounds__(256) void k_ff_gemm(const uint16_t* __restrict__ F,
     f32x4 acc[NT];
 #pragma unroll
     for (int t = 0; t < NT; t++) acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    const uint16_t* frow = F + (size_t)(row0 + col) * stride + 8u * sub;
-    // F is streamed once per launch (non-temporal, so X stays in L2); the next step's F and X
+    constexpr uint32_t mp = NT * 16;
+    // Lane offsets inside a 16 x 32 F tile and inside a 32 x 16 block of x.
+    const uint16_t* ftile = F + (size_t)(row0 >> 4) * (stride >> 5) * 512u + col * 32u + 8u * sub;
+    const uint32_t xl = col * 32u + 8u * sub;
+    // F is streamed once per launch (non-temporal, so x stays in L2); the next step's F and x
     // fragments are loaded before this step's conversions and MFMAs.
-    u32x4 raw_next = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(frow + k0));
+    u32x4 raw_next = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(ftile + (size_t)(k0 >> 5) * 512u));
     bf16x8 b1n[NT], b2n[NT], b3n[NT];
 #pragma unroll
     for (int t = 0; t < NT; t++) {
-        const size_t xo = (size_t)(16u * t + col) * npad + k0 + 8u * sub;
+        const size_t xo = (size_t)(k0 >> 5) * (mp * 32u) + 512u * t + xl;
         b1n[t] = *reinterpret_cast<const bf16x8*>(X1 + xo);
         if (ACC) {
             b2n[t] = *reinterpret_cast<const bf16x8*>(X2 + xo);
@@ -111,10 +127,10 @@ __global__ __launch_bounds__(256) void k_ff_gemm(const uint16_t* __restrict__ F,
             }
         }
         if (k + 32 < kend) {
-            raw_next = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(frow + k + 32));
+            raw_next = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(ftile + (size_t)((k + 32) >> 5) * 512u));
 #pragma unroll
             for (int t = 0; t < NT; t++) {
-                const size_t xo = (size_t)(16u * t + col) * npad + k + 32 + 8u * sub;
+                const size_t xo = (size_t)((k + 32) >> 5) * (mp * 32u) + 512u * t + xl;
                 b1n[t] = *reinterpret_cast<const bf16x8*>(X1 + xo);
                 if (ACC) {
                     b2n[t] = *reinterpret_cast<const bf16x8*>(X2 + xo);
@@ -149,6 +165,76 @@ __global__ __launch_bounds__(256) void k_ff_gemm(const uint16_t* __restrict__ F,
 #pragma unroll
         for (int i = 0; i < 4; i++) out[(size_t)(16u * t + col) * npad + row0 + 4u * sub + i] = acc[t][i];
     }
+}
+
+// Fast gather product Y_part = F_hi * x1 for the Jacobi sweeps: each wavefront owns two 16-row
+// tiles (every B fragment feeds two MFMAs) and keeps F loads four K steps ahead in a register
+// ring, so about 4 KB of the streamed matrix is in flight per wavefront. Same lane layout and
+// the same per-element products and summation order as k_ff_gemm<NT, false>.
+constexpr int kFastRows = 32;  // rows per wavefront
+constexpr int kFastAhead = 4;  // K steps of F in flight
+
+template <int NT>
+__global__ __launch_bounds__(256) void k_ff_gemm_fast(const uint16_t* __restrict__ F, uint32_t stride,
+                                                      const __bf16* __restrict__ X1, uint32_t npad,
+                                                      uint32_t klen, float* __restrict__ Yp) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    const uint32_t row0 = (blockIdx.x * 4u + wave) * (uint32_t)kFastRows;
+    const uint32_t k0 = blockIdx.y * klen, kend = k0 + klen;
+    const uint32_t sub = lane >> 4, col = lane & 15u;
+    f32x4 acc[2][NT];
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int t = 0; t < NT; t++) acc[r][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+    constexpr uint32_t mp = NT * 16;
+    const uint16_t* ftile[2] = {F + (size_t)(row0 >> 4) * (stride >> 5) * 512u + col * 32u + 8u * sub,
+                                F + (size_t)((row0 >> 4) + 1u) * (stride >> 5) * 512u + col * 32u + 8u * sub};
+    const uint32_t xl = col * 32u + 8u * sub;
+    u32x4 ring[2][kFastAhead];
+#pragma unroll
+    for (int j = 0; j < kFastAhead; j++) {
+        const uint32_t kk = k0 + 32u * j;
+#pragma unroll
+        for (int r = 0; r < 2; r++)
+            ring[r][j] = kk < kend ? __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(ftile[r] + (size_t)(kk >> 5) * 512u))
+                                   : u32x4{0u, 0u, 0u, 0u};
+    }
+    for (uint32_t k = k0; k < kend; k += 32u * kFastAhead) {
+#pragma unroll
+        for (int j = 0; j < kFastAhead; j++) {
+            const uint32_t kk = k + 32u * j;
+            if (kk >= kend) break;
+            u32x4 cur[2] = {ring[0][j], ring[1][j]};
+            const uint32_t kn = kk + 32u * kFastAhead;
+            if (kn < kend) {
+#pragma unroll
+                for (int r = 0; r < 2; r++)
+                    ring[r][j] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(ftile[r] + (size_t)(kn >> 5) * 512u));
+            }
+            bf16x8 b[NT];
+#pragma unroll
+            for (int t = 0; t < NT; t++)
+                b[t] = *reinterpret_cast<const bf16x8*>(X1 + (size_t)(kk >> 5) * (mp * 32u) + 512u * t + xl);
+#pragma unroll
+            for (int r = 0; r < 2; r++) {
+                const uint32_t w[4] = {cur[r].x, cur[r].y, cur[r].z, cur[r].w};
+                bf16x8 a;
+#pragma unroll
+                for (int e = 0; e < 8; e++) a[e] = (__bf16)(float)((w[e >> 1] >> ((e & 1) * 16)) & 0xFFFFu);
+#pragma unroll
+                for (int t = 0; t < NT; t++) acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[t], acc[r][t], 0, 0, 0);
+            }
+        }
+    }
+    const uint32_t mpad = NT * 16;
+    float* out = Yp + (size_t)blockIdx.y * mpad * npad;
+#pragma unroll
+    for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) out[(size_t)(16u * t + col) * npad + row0 + 16u * r + 4u * sub + i] = acc[r][t][i];
 }
 
 // Row-major (n x m) -> column-major padded (npad x mpad, zeros outside), and back.
@@ -205,7 +291,7 @@ __global__ __launch_bounds__(256) void k_ff_residual(const float* __restrict__ Y
     const float r = E[idx] + rho[idx] * (y * inv_rays) - B[idx];
     R[idx] = r;
     D[idx] = r;
-    X1[idx] = (__bf16)r;
+    X1[x_index(i, c, mpad)] = (__bf16)r;
     block_sq_sum(r, part + (size_t)c * gridDim.x + blockIdx.x);
 }
 
@@ -219,7 +305,7 @@ __global__ __launch_bounds__(256) void k_ff_sweep(const float* __restrict__ Yp, 
     const float y = sum_parts(Yp, nsplit, (size_t)mpad * npad, idx);
     const float d = R[idx] + rho[idx] * (y * inv_rays);
     D[idx] = d;
-    X1[idx] = (__bf16)d;
+    X1[x_index(i, c, mpad)] = (__bf16)d;
 }
 
 // B += d (or B = d when init), then the three-way bf16 split of B.
@@ -232,9 +318,19 @@ __global__ __launch_bounds__(256) void k_ff_apply(uint32_t npad, float* __restri
     B[idx] = b;
     __bf16 a1, a2, a3;
     split3(b, a1, a2, a3);
-    X1[idx] = a1;
-    X2[idx] = a2;
-    X3[idx] = a3;
+    const size_t xi = x_index(i, c, gridDim.y);
+    X1[xi] = a1;
+    X2[xi] = a2;
+    X3[xi] = a3;
+}
+
+// Rows [first, first + rows) of the tiled count matrix as row-major rows of n counts.
+__global__ void k_ff_untile(const uint16_t* __restrict__ F, uint32_t npad, uint32_t n, uint32_t first, uint32_t rows,
+                            uint16_t* __restrict__ out) {
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= (uint64_t)rows * n) return;
+    const uint32_t r = (uint32_t)(g / n), k = (uint32_t)(g % n);
+    out[g] = F[f_index(first + r, k, npad)];
 }
 
 uint32_t blocks_for(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
@@ -261,11 +357,17 @@ hipError_t launch_ff_gemm(hipStream_t st, bool accurate, const uint16_t* F, uint
                           const __bf16* X2, const __bf16* X3, uint32_t npad, uint32_t mpad, uint32_t nsplit,
                           float* Yp) {
     const uint32_t klen = npad / nsplit;
+    if (klen % 32u != 0u || npad % 64u != 0u) return hipErrorInvalidValue;
     const dim3 grid(npad / 64, nsplit), block(256);
+    // The fast product takes 128 rows per block when the row count allows it.
+    const bool wide = !accurate && npad % (4u * kFastRows) == 0u;
+    const dim3 grid_fast(npad / (4u * kFastRows), nsplit);
 #define RL_GEMM_CASE(NTV)                                                                                     \
     case NTV:                                                                                                 \
         if (accurate)                                                                                         \
             hipLaunchKernelGGL((k_ff_gemm<NTV, true>), grid, block, 0, st, F, stride, X1, X2, X3, npad, klen, Yp); \
+        else if (wide)                                                                                        \
+            hipLaunchKernelGGL((k_ff_gemm_fast<NTV>), grid_fast, block, 0, st, F, stride, X1, npad, klen, Yp);  \
         else                                                                                                  \
             hipLaunchKernelGGL((k_ff_gemm<NTV, false>), grid, block, 0, st, F, stride, X1, X2, X3, npad, klen, Yp); \
         break;
@@ -280,6 +382,14 @@ hipError_t launch_ff_gemm(hipStream_t st, bool accurate, const uint16_t* F, uint
             return hipErrorInvalidValue;
     }
 #undef RL_GEMM_CASE
+    return hipGetLastError();
+}
+
+hipError_t launch_ff_untile(hipStream_t st, const uint16_t* F, uint32_t npad, uint32_t n, uint32_t first,
+                           uint32_t rows, uint16_t* out) {
+    if ((uint64_t)rows * n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ff_untile, dim3(blocks_for((uint64_t)rows * n, 256)), dim3(256), 0, st, F, npad, n, first,
+                       rows, out);
     return hipGetLastError();
 }
 

@@ -1402,8 +1402,21 @@ int32_t rl_radiosity_build_form_factors(rl_relight* ctx, rl_radiosity* r, uint32
 int32_t rl_radiosity_read_form_factors(rl_radiosity* r, uint32_t first_row, uint32_t rows, uint16_t* counts) {
     if (!r || !counts) return fail(RL_INVALID_ARGUMENT, "NULL argument");
     if ((uint64_t)first_row + rows > r->n) return fail(RL_INVALID_ARGUMENT, "row range out of bounds");
-    RL_HIP(hipMemcpy2D(counts, r->n * sizeof(uint16_t), r->d_F + (size_t)first_row * r->npad,
-                       r->npad * sizeof(uint16_t), r->n * sizeof(uint16_t), rows, hipMemcpyDeviceToHost));
+    // F is stored in 16 x 32 tiles on the device: untile 1024 rows at a time into row-major rows.
+    const uint32_t chunk = 1024;
+    uint16_t* tmp = nullptr;
+    hipError_t e = dev_alloc(&tmp, (size_t)std::min(rows, chunk) * r->n);
+    hipStream_t st = r->ctx->stream;
+    for (uint32_t r0 = 0; e == hipSuccess && r0 < rows; r0 += chunk) {
+        const uint32_t nr = std::min(chunk, rows - r0);
+        e = launch_ff_untile(st, r->d_F, r->npad, r->n, first_row + r0, nr, tmp);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(counts + (size_t)r0 * r->n, tmp, (size_t)nr * r->n * sizeof(uint16_t),
+                               hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+    }
+    dev_free(tmp);
+    if (e != hipSuccess) return fail(RL_DEVICE_ERROR, std::string("read form factors: ") + hipGetErrorString(e));
     return RL_SUCCESS;
 }
 

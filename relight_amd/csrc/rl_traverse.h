@@ -1,8 +1,9 @@
 // BVH4 traversal shared by every kernel that casts rays (device code only).
 //
-// Near-first traversal with a per-lane LDS stack (CAP entries, one column per lane), the
-// conservative slab test of Ize ("Robust BVH Ray Traversal", JCGT 2(2), 2013) and the watertight
-// triangle test of rl_math.h. Closest-hit ties at equal t go to the smaller scene-order triangle
+// Near-first traversal with a per-lane LDS stack (CAP entries, one column per lane), a
+// conservative fma slab test and the watertight triangle test of rl_math.h. Each axis's entry and
+// exit planes are picked by the sign of the ray direction when the node is loaded (a per-lane
+// load offset), so the slab test needs no per-box min/max of the two plane distances. Closest-hit ties at equal t go to the smaller scene-order triangle
 // id, so results do not depend on traversal order.
 #pragma once
 
@@ -44,10 +45,12 @@ __device__ __forceinline__ uint32_t rank_below(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-// Per-lane constants of a ray for the step-wise traversal below.
+// Per-lane constants of a ray for the step-wise traversal below. nx/ny/nz: float4 offset (0 or 1)
+// of the entry-plane row of each axis in a node (lo for a positive direction, hi for negative).
 struct TravRay {
     WtRay w;
     float ix, iy, iz, oix, oiy, oiz;
+    uint32_t nx, ny, nz;
 };
 
 __device__ __forceinline__ TravRay trav_ray(const Ray& ray) {
@@ -58,7 +61,33 @@ __device__ __forceinline__ TravRay trav_ray(const Ray& ray) {
     const float dz = fabsf(ray.d.z) > 1e-20f ? ray.d.z : copysignf(1e-20f, ray.d.z);
     r.ix = 1.0f / dx; r.iy = 1.0f / dy; r.iz = 1.0f / dz;
     r.oix = -(ray.o.x * r.ix); r.oiy = -(ray.o.y * r.iy); r.oiz = -(ray.o.z * r.iz);
+    r.nx = r.ix < 0.0f ? 1u : 0u;
+    r.ny = r.iy < 0.0f ? 1u : 0u;
+    r.nz = r.iz < 0.0f ? 1u : 0u;
     return r;
+}
+
+// Four child boxes of a node against a ray: the entry distance of each box (its sort key), +inf
+// for a box the ray misses within (tnear, lim). The entry plane of an axis is lo for a positive
+// direction component and hi for a negative one, so t_entry = fma(plane, 1/d, -o/d) equals the
+// min of the two plane distances (fma rounding is monotone in the plane) and t_exit the max.
+__device__ __forceinline__ void box4(const float4* nd, uint32_t nx, uint32_t ny, uint32_t nz, float ix, float iy,
+                                     float iz, float oix, float oiy, float oiz, float tnear, float lim,
+                                     const uint32_t* cid, float* key) {
+    const float4 ex = nd[nx], fx = nd[1u - nx], ey = nd[2u + ny], fy = nd[3u - ny], ez = nd[4u + nz], fz = nd[5u - nz];
+    const float exa[4] = {ex.x, ex.y, ex.z, ex.w}, fxa[4] = {fx.x, fx.y, fx.z, fx.w};
+    const float eya[4] = {ey.x, ey.y, ey.z, ey.w}, fya[4] = {fy.x, fy.y, fy.z, fy.w};
+    const float eza[4] = {ez.x, ez.y, ez.z, ez.w}, fza[4] = {fz.x, fz.y, fz.z, fz.w};
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const float t0x = fmaf(exa[i], ix, oix), t1x = fmaf(fxa[i], ix, oix);
+        const float t0y = fmaf(eya[i], iy, oiy), t1y = fmaf(fya[i], iy, oiy);
+        const float t0z = fmaf(eza[i], iz, oiz), t1z = fmaf(fza[i], iz, oiz);
+        const float tmin = fmaxf(fmaxf(t0x, t0y), fmaxf(t0z, tnear));
+        const float tmax = fminf(fminf(t1x, t1y), t1z) * kBoxScale;
+        const bool hit = tmin <= fminf(tmax, lim) && cid[i] != RL_EMPTY_CHILD;
+        key[i] = hit ? tmin : __builtin_inff();
+    }
 }
 
 __device__ __forceinline__ void stack_push(uint32_t v, int& sp, uint32_t* stk, uint32_t stride, uint32_t* spill) {
@@ -81,24 +110,10 @@ __device__ __forceinline__ int any_hit_step(const SceneDev& S, const TravRay& r,
                                             uint32_t* stk, uint32_t stride, uint32_t* spill) {
     if (!(node & RL_LEAF_BIT)) {
         const float4* nd = reinterpret_cast<const float4*>(S.nodes + node);
-        const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
         const uint4 ch = reinterpret_cast<const uint4*>(nd)[6];
-        const float lim = r.w.tfar * kBoxScale;
         float key[4];
         uint32_t cid[4] = {ch.x, ch.y, ch.z, ch.w};
-        const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
-        const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
-        const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-            float t0x = fmaf(lxa[i], r.ix, r.oix), t1x = fmaf(hxa[i], r.ix, r.oix);
-            float t0y = fmaf(lya[i], r.iy, r.oiy), t1y = fmaf(hya[i], r.iy, r.oiy);
-            float t0z = fmaf(lza[i], r.iz, r.oiz), t1z = fmaf(hza[i], r.iz, r.oiz);
-            float tmin = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), r.w.tnear));
-            float tmax = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * kBoxScale;
-            bool hit = tmin <= fminf(tmax, lim) && cid[i] != RL_EMPTY_CHILD;
-            key[i] = hit ? tmin : __builtin_inff();
-        }
+        box4(nd, r.nx, r.ny, r.nz, r.ix, r.iy, r.iz, r.oix, r.oiy, r.oiz, r.w.tnear, r.w.tfar * kBoxScale, cid, key);
         cswap(key[0], cid[0], key[1], cid[1]);
         cswap(key[2], cid[2], key[3], cid[3]);
         cswap(key[0], cid[0], key[2], cid[2]);
@@ -149,6 +164,7 @@ __device__ bool traverse(const SceneDev& S, const Ray& ray, uint32_t* stk, uint3
     const float dz = fabsf(ray.d.z) > 1e-20f ? ray.d.z : copysignf(1e-20f, ray.d.z);
     const float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
     const float oix = -(ray.o.x * ix), oiy = -(ray.o.y * iy), oiz = -(ray.o.z * iz);
+    const uint32_t nx = ix < 0.0f ? 1u : 0u, ny = iy < 0.0f ? 1u : 0u, nz = iz < 0.0f ? 1u : 0u;
     const float tnear = ray.tnear;
     float tlimit = ray.tfar;  // closest: shrinks to the best hit so far
     res.t = ray.tfar;
@@ -161,24 +177,10 @@ __device__ bool traverse(const SceneDev& S, const Ray& ray, uint32_t* stk, uint3
     for (;;) {
         if (!(node & RL_LEAF_BIT)) {
             const float4* nd = reinterpret_cast<const float4*>(S.nodes + node);
-            const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5];
             const uint4 ch = reinterpret_cast<const uint4*>(nd)[6];
-            const float lim = tlimit * kBoxScale;
             float key[4];
             uint32_t cid[4] = {ch.x, ch.y, ch.z, ch.w};
-            const float lxa[4] = {lx.x, lx.y, lx.z, lx.w}, hxa[4] = {hx.x, hx.y, hx.z, hx.w};
-            const float lya[4] = {ly.x, ly.y, ly.z, ly.w}, hya[4] = {hy.x, hy.y, hy.z, hy.w};
-            const float lza[4] = {lz.x, lz.y, lz.z, lz.w}, hza[4] = {hz.x, hz.y, hz.z, hz.w};
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                float t0x = fmaf(lxa[i], ix, oix), t1x = fmaf(hxa[i], ix, oix);
-                float t0y = fmaf(lya[i], iy, oiy), t1y = fmaf(hya[i], iy, oiy);
-                float t0z = fmaf(lza[i], iz, oiz), t1z = fmaf(hza[i], iz, oiz);
-                float tmin = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tnear));
-                float tmax = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * kBoxScale;
-                bool hit = tmin <= fminf(tmax, lim) && cid[i] != RL_EMPTY_CHILD;
-                key[i] = hit ? tmin : __builtin_inff();
-            }
+            box4(nd, nx, ny, nz, ix, iy, iz, oix, oiy, oiz, tnear, tlimit * kBoxScale, cid, key);
             // Sort the four (distance, child) pairs: near child next, the rest pushed far-first.
             cswap(key[0], cid[0], key[1], cid[1]);
             cswap(key[2], cid[2], key[3], cid[3]);
