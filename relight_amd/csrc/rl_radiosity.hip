@@ -167,55 +167,78 @@ __global__ __launch_bounds__(256) void k_ff_gemm(const uint16_t* __restrict__ F,
     }
 }
 
-// Fast gather product Y_part = F_hi * x1 for the Jacobi sweeps: each wavefront owns two 16-row
-// tiles (every B fragment feeds two MFMAs) and keeps F loads four K steps ahead in a register
-// ring, so about 4 KB of the streamed matrix is in flight per wavefront. Same lane layout and
-// the same per-element products and summation order as k_ff_gemm<NT, false>.
+// Fast gather product Y_part = F_hi * x1 for the Jacobi sweeps. A workgroup (four wavefronts,
+// 128 rows) stages x through LDS four K steps at a time, double-buffered, so x is read from L2
+// once per workgroup instead of once per wavefront; each wavefront owns two 16-row tiles (every B
+// fragment feeds two MFMAs) and keeps F loads four K steps ahead in a register ring. Same lane
+// layout, products and summation order as k_ff_gemm<NT, false>.
 constexpr int kFastRows = 32;  // rows per wavefront
-constexpr int kFastAhead = 4;  // K steps of F in flight
+constexpr int kFastAhead = 4;  // K steps of F in flight = K steps per LDS stage of x
 
 template <int NT>
 __global__ __launch_bounds__(256) void k_ff_gemm_fast(const uint16_t* __restrict__ F, uint32_t stride,
                                                       const __bf16* __restrict__ X1, uint32_t npad,
                                                       uint32_t klen, float* __restrict__ Yp) {
+    constexpr uint32_t mp = NT * 16;
+    constexpr uint32_t kStepVecs = mp * 32u / 8u;             // bf16x8 vectors of x per K step
+    constexpr uint32_t kStageVecs = kStepVecs * kFastAhead;  // per LDS stage
+    constexpr uint32_t kPerThread = (kStageVecs + 255u) / 256u;
+    __shared__ bf16x8 xs[2][kStageVecs];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t row0 = (blockIdx.x * 4u + wave) * (uint32_t)kFastRows;
-    const uint32_t k0 = blockIdx.y * klen, kend = k0 + klen;
+    const uint32_t k0 = blockIdx.y * klen;
+    const uint32_t nstage = klen / (32u * kFastAhead);
     const uint32_t sub = lane >> 4, col = lane & 15u;
     f32x4 acc[2][NT];
 #pragma unroll
     for (int r = 0; r < 2; r++)
 #pragma unroll
         for (int t = 0; t < NT; t++) acc[r][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-    constexpr uint32_t mp = NT * 16;
     const uint16_t* ftile[2] = {F + (size_t)(row0 >> 4) * (stride >> 5) * 512u + col * 32u + 8u * sub,
                                 F + (size_t)((row0 >> 4) + 1u) * (stride >> 5) * 512u + col * 32u + 8u * sub};
-    const uint32_t xl = col * 32u + 8u * sub;
+    const bf16x8* xg = reinterpret_cast<const bf16x8*>(X1) + (size_t)(k0 >> 5) * kStepVecs;
+    const uint32_t xl = 64u * 0u + col * 4u + sub;  // vector offset of this lane's fragment in a step
     u32x4 ring[2][kFastAhead];
 #pragma unroll
-    for (int j = 0; j < kFastAhead; j++) {
-        const uint32_t kk = k0 + 32u * j;
+    for (int j = 0; j < kFastAhead; j++)
 #pragma unroll
         for (int r = 0; r < 2; r++)
-            ring[r][j] = kk < kend ? __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(ftile[r] + (size_t)(kk >> 5) * 512u))
-                                   : u32x4{0u, 0u, 0u, 0u};
+            ring[r][j] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(ftile[r] + (size_t)((k0 >> 5) + j) * 512u));
+    bf16x8 stg[kPerThread];
+#pragma unroll
+    for (uint32_t p = 0; p < kPerThread; p++) {
+        const uint32_t v = threadIdx.x + 256u * p;
+        if (v < kStageVecs) stg[p] = xg[v];
     }
-    for (uint32_t k = k0; k < kend; k += 32u * kFastAhead) {
+#pragma unroll
+    for (uint32_t p = 0; p < kPerThread; p++) {
+        const uint32_t v = threadIdx.x + 256u * p;
+        if (v < kStageVecs) xs[0][v] = stg[p];
+    }
+    __syncthreads();
+    for (uint32_t s = 0; s < nstage; s++) {
+        const uint32_t buf = s & 1u;
+        const bool more = s + 1u < nstage;
+        if (more) {
+#pragma unroll
+            for (uint32_t p = 0; p < kPerThread; p++) {
+                const uint32_t v = threadIdx.x + 256u * p;
+                if (v < kStageVecs) stg[p] = xg[(size_t)(s + 1u) * kStageVecs + v];
+            }
+        }
 #pragma unroll
         for (int j = 0; j < kFastAhead; j++) {
-            const uint32_t kk = k + 32u * j;
-            if (kk >= kend) break;
+            const uint32_t step = (k0 >> 5) + s * kFastAhead + j;  // global K step (32 columns)
             u32x4 cur[2] = {ring[0][j], ring[1][j]};
-            const uint32_t kn = kk + 32u * kFastAhead;
-            if (kn < kend) {
+            if (more) {
 #pragma unroll
                 for (int r = 0; r < 2; r++)
-                    ring[r][j] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(ftile[r] + (size_t)(kn >> 5) * 512u));
+                    ring[r][j] = __builtin_nontemporal_load(
+                        reinterpret_cast<const u32x4*>(ftile[r] + (size_t)(step + kFastAhead) * 512u));
             }
             bf16x8 b[NT];
 #pragma unroll
-            for (int t = 0; t < NT; t++)
-                b[t] = *reinterpret_cast<const bf16x8*>(X1 + (size_t)(kk >> 5) * (mp * 32u) + 512u * t + xl);
+            for (int t = 0; t < NT; t++) b[t] = xs[buf][(uint32_t)j * kStepVecs + 64u * t + xl];
 #pragma unroll
             for (int r = 0; r < 2; r++) {
                 const uint32_t w[4] = {cur[r].x, cur[r].y, cur[r].z, cur[r].w};
@@ -226,9 +249,16 @@ __global__ __launch_bounds__(256) void k_ff_gemm_fast(const uint16_t* __restrict
                 for (int t = 0; t < NT; t++) acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[t], acc[r][t], 0, 0, 0);
             }
         }
+        if (more) {
+#pragma unroll
+            for (uint32_t p = 0; p < kPerThread; p++) {
+                const uint32_t v = threadIdx.x + 256u * p;
+                if (v < kStageVecs) xs[buf ^ 1u][v] = stg[p];
+            }
+        }
+        __syncthreads();
     }
-    const uint32_t mpad = NT * 16;
-    float* out = Yp + (size_t)blockIdx.y * mpad * npad;
+    float* out = Yp + (size_t)blockIdx.y * mp * npad;
 #pragma unroll
     for (int r = 0; r < 2; r++)
 #pragma unroll
@@ -359,8 +389,8 @@ hipError_t launch_ff_gemm(hipStream_t st, bool accurate, const uint16_t* F, uint
     const uint32_t klen = npad / nsplit;
     if (klen % 32u != 0u || npad % 64u != 0u) return hipErrorInvalidValue;
     const dim3 grid(npad / 64, nsplit), block(256);
-    // The fast product takes 128 rows per block when the row count allows it.
-    const bool wide = !accurate && npad % (4u * kFastRows) == 0u;
+    // The fast product takes 128 rows per block and whole LDS stages of K when the sizes allow it.
+    const bool wide = !accurate && npad % (4u * kFastRows) == 0u && klen % (32u * kFastAhead) == 0u;
     const dim3 grid_fast(npad / (4u * kFastRows), nsplit);
 #define RL_GEMM_CASE(NTV)                                                                                     \
     case NTV:                                                                                                 \
