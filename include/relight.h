@@ -205,6 +205,25 @@ int32_t rl_lightmap_read_lumels(rl_lightmap* lm, uint32_t* texel, float* positio
 /* Lightmap::toHdr (Lightmap.cpp:332-349): width*height*3 floats, invalid texels 0. */
 int32_t rl_lightmap_read_hdr(rl_lightmap* lm, float* out_rgb);
 
+/* Lightmap::save storage formats (StorageFormat, Relight.h:89-93). */
+#define RL_FORMAT_RAW_HDR 0
+#define RL_FORMAT_TGA_DOUBLE_LDR 1
+#define RL_FORMAT_TGA_RGBM 2
+/* Lightmap::expand (Lightmap.cpp:167-193): valid interior texels copy their colour to invalid
+ * diagonal neighbours, in the reference's raster order (on the GPU). */
+int32_t rl_lightmap_expand(rl_lightmap* lm);
+/* Lightmap::blur (Lightmap.cpp:196-220): in-place 3x3 mean over valid texels in raster order
+ * (on the GPU, bit-identical to the sequential order). */
+int32_t rl_lightmap_blur(rl_lightmap* lm);
+/* Lightmap::toDoubleLdr / toRgbmLdr (Lightmap.cpp:289-331; Rgb::doubleLdr / rgbm, Types.h:216-234):
+ * width*height*3 (RL_FORMAT_TGA_DOUBLE_LDR) or *4 (RL_FORMAT_TGA_RGBM) bytes, r g b (m) per texel.
+ * Channels the reference leaves undefined (negative or NaN before the byte cast) are 0; RGBM uses
+ * 4 bytes per texel (the reference's toRgbmLdr steps 3 bytes within a 4-byte row stride). */
+int32_t rl_lightmap_encode(rl_lightmap* lm, int32_t format, uint8_t* out);
+/* Lightmap::save (Lightmap.cpp:223-285): RawHdr (int width, int height, float rgb) or an
+ * uncompressed 24/32-bit TGA. */
+int32_t rl_lightmap_save(rl_lightmap* lm, const char* path, int32_t format);
+
 /* Relight::bakeDirectLight (Relight.cpp:216-227) -> DirectLight::bakeLumel over every valid lumel
  * of the mesh's lightmap (LumelBakeIterator, baker/Baker.cpp:52-77,176-185). One launch. */
 int32_t rl_bake_direct(rl_relight* ctx, rl_scene* scene, int32_t mesh);

@@ -105,6 +105,9 @@ def lib():
             "or_emit_photons": (C.c_int32, [C.c_void_p, C.POINTER(or_indirect), _I32, C.c_int32, _I32, _I32, _U8,
                                             _F, _F, _U32, _F, C.c_int32, _U64]),
             "or_photon_records": (C.c_int32, [C.c_void_p, _U64, _U32, _F, _F]),
+            "or_expand": (C.c_int32, [C.c_int32, C.c_int32, _U8, _F]),
+            "or_blur": (C.c_int32, [C.c_int32, C.c_int32, _U8, _F]),
+            "or_encode": (C.c_int32, [C.c_int32, C.c_int32, _F, C.c_int32, _U8]),
             "or_final_gather": (C.c_int32, [C.c_void_p, C.c_uint32, _F, _F, _U32, C.c_uint32, C.POINTER(or_indirect),
                                             _I32, C.c_int32, _I32, _I32, _F, _F, C.c_int32, _U64]),
             "or_form_factors": (C.c_int32, [C.c_void_p, _F, _U32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_float,
@@ -339,3 +342,39 @@ class OracleScene:
         valid, tri, pos, nrm = grids
         texel = np.nonzero(valid)[0].astype(np.uint32)
         return texel, pos[texel].copy(), nrm[texel].copy()
+
+
+def expand(valid, rgb):
+    """Lightmap::expand on an (h, w) validity grid and an (h, w, 3) float32 colour grid (copy)."""
+    h, w = valid.shape
+    out = np.ascontiguousarray(rgb, np.float32).copy()
+    v = np.ascontiguousarray(valid, np.uint8)
+    lib().or_expand(w, h, v.ctypes.data_as(_U8), _f(out))
+    return out
+
+
+def blur(valid, rgb):
+    """Lightmap::blur (in place, raster order) on a copy."""
+    h, w = valid.shape
+    out = np.ascontiguousarray(rgb, np.float32).copy()
+    v = np.ascontiguousarray(valid, np.uint8)
+    lib().or_blur(w, h, v.ctypes.data_as(_U8), _f(out))
+    return out
+
+
+def encode(rgb, fmt):
+    """toDoubleLdr (fmt 1, h x w x 3) / toRgbmLdr (fmt 2, h x w x 4) bytes."""
+    h, w = rgb.shape[:2]
+    c = np.ascontiguousarray(rgb, np.float32)
+    out = np.zeros((h, w, 3 if fmt == 1 else 4), np.uint8)
+    lib().or_encode(w, h, _f(c), int(fmt), out.ctypes.data_as(_U8))
+    return out
+
+
+def tga_bytes(pixels):
+    """The file Lightmap::save writes for TGA formats: 12-byte header, 6-byte info, b g r (a)."""
+    h, w, ch = pixels.shape
+    head = bytes([0, 0, 2, 0, 0, 0, 0, 0, 0, 0, 0, 0]) + bytes([w % 256, w // 256, h % 256, h // 256, ch * 8, 0])
+    px = pixels.reshape(-1, ch)
+    order = [2, 1, 0] + ([3] if ch == 4 else [])
+    return head + np.ascontiguousarray(px[:, order]).tobytes()

@@ -15,6 +15,7 @@
 //                             same normalisation (the configuration's hash-grid gather)
 //   final gather              baker/IndirectLight.cpp:47-75
 //   form factors              hemisphere-ray estimator of radiosity/RadiosityBuilder.cpp:142-176
+//   post-process, encodings   Lightmap.cpp:167-331 (expand, blur, toDoubleLdr, toRgbmLdr), Types.h:216-234
 // Project-defined parts shared by contract with the GPU path (not in the reference): PCG32 streams
 // keyed by (seed, lightmap seed, texel, technique, sample); cosine sampling (Malley + Duff et al.
 // 2017 basis); the watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2013); closest-hit ties
@@ -1021,6 +1022,83 @@ int32_t or_emit_photons(void* sp, const or_indirect* s, const int32_t* mesh_pm, 
     uint64_t tot = 0;
     for (uint64_t x : rays) tot += x;
     if (rays_out) *rays_out = tot;
+    return 0;
+}
+
+// Lightmap::expand (Lightmap.cpp:167-193) on a full grid (3 floats per texel), as written: raster
+// order over interior texels, each valid one writing its colour to invalid diagonal neighbours.
+int32_t or_expand(int32_t w, int32_t h, const uint8_t* valid, float* rgb) {
+    for (int y = 1; y < h - 1; y++) {
+        for (int x = 1; x < w - 1; x++) {
+            const size_t c = (size_t)y * w + x;
+            if (!valid[c]) continue;
+            const int nx[4] = {x - 1, x - 1, x + 1, x + 1}, ny[4] = {y - 1, y + 1, y - 1, y + 1};
+            for (int k = 0; k < 4; k++) {
+                const size_t t = (size_t)ny[k] * w + nx[k];
+                if (valid[t]) continue;
+                rgb[3 * t] = rgb[3 * c];
+                rgb[3 * t + 1] = rgb[3 * c + 1];
+                rgb[3 * t + 2] = rgb[3 * c + 2];
+            }
+        }
+    }
+    return 0;
+}
+
+// Lightmap::blur (Lightmap.cpp:196-220): in place, raster order.
+int32_t or_blur(int32_t w, int32_t h, const uint8_t* valid, float* rgb) {
+    for (int y = 1; y < h - 1; y++) {
+        for (int x = 1; x < w - 1; x++) {
+            float r = 0.0f, g = 0.0f, b = 0.0f;
+            int count = 0;
+            for (int j = y - 1; j <= y + 1; j++) {
+                for (int i = x - 1; i <= x + 1; i++) {
+                    const size_t t = (size_t)j * w + i;
+                    if (!valid[t]) continue;
+                    r += rgb[3 * t];
+                    g += rgb[3 * t + 1];
+                    b += rgb[3 * t + 2];
+                    count++;
+                }
+            }
+            if (count) {
+                const float s = 1.0f / (float)count;
+                const size_t c = (size_t)y * w + x;
+                rgb[3 * c] = r * s;
+                rgb[3 * c + 1] = g * s;
+                rgb[3 * c + 2] = b * s;
+            }
+        }
+    }
+    return 0;
+}
+
+// Rgb::doubleLdr (format 1) / Rgb::rgbm (format 2) per texel (Types.h:216-234) with
+// min2(a, b) = a < b ? a : b and max2(a, b) = a > b ? a : b; the byte cast truncates values in
+// [0, 255] and gives 0 where the reference's cast is undefined (negative, NaN).
+static inline uint8_t byte_of(float v) {
+    const float m = 255.0f < v ? 255.0f : v;
+    return m >= 1.0f ? (uint8_t)(int)m : (uint8_t)0;
+}
+
+int32_t or_encode(int32_t w, int32_t h, const float* rgb, int32_t format, uint8_t* out) {
+    const size_t n = (size_t)w * h;
+    for (size_t i = 0; i < n; i++) {
+        const float r = rgb[3 * i], g = rgb[3 * i + 1], b = rgb[3 * i + 2];
+        if (format == 1) {
+            out[3 * i] = byte_of(r * 0.5f * 255.0f);
+            out[3 * i + 1] = byte_of(g * 0.5f * 255.0f);
+            out[3 * i + 2] = byte_of(b * 0.5f * 255.0f);
+        } else {
+            const float gb = g > b ? g : b;
+            const float mx = r > gb ? r : gb;
+            const float m = 255.0f / mx;
+            out[4 * i] = byte_of(r / m * 255.0f);
+            out[4 * i + 1] = byte_of(g / m * 255.0f);
+            out[4 * i + 2] = byte_of(b / m * 255.0f);
+            out[4 * i + 3] = byte_of(m * 255.0f);
+        }
+    }
     return 0;
 }
 

@@ -4,7 +4,8 @@ The product is librelight_amd.so (HIP kernels for gfx950 + a C-ABI, include/reli
 package mirrors the reference's Relight / Scene / Lightmap API over that ABI and provides the
 seeded scene generators the tests and the benchmark use.
 """
-from ._capi import (RL_GATHER_KNN, RL_GATHER_TEXEL_DISC, RL_LIGHT_AREA, RL_LIGHT_DIRECTIONAL, RL_LIGHT_POINT, RL_LIGHT_SPOT,
+from ._capi import (RL_FORMAT_RAW_HDR, RL_FORMAT_TGA_DOUBLE_LDR, RL_FORMAT_TGA_RGBM,
+                    RL_GATHER_KNN, RL_GATHER_TEXEL_DISC, RL_LIGHT_AREA, RL_LIGHT_DIRECTIONAL, RL_LIGHT_POINT, RL_LIGHT_SPOT,
                     RL_PHOTON_RUSSIAN_ROULETTE, RL_PHOTON_THRESHOLD, RL_SAMPLE_COSINE,
                     RL_SAMPLE_UNIFORM, RL_TRACE_ANY, RL_TRACE_CLOSEST)
 from .relight import (AmbientOcclusionSettings, BakeJob, IndirectLightSettings, LightDesc,
@@ -16,4 +17,5 @@ __all__ = [
     "RL_LIGHT_POINT", "RL_LIGHT_SPOT", "RL_LIGHT_DIRECTIONAL", "RL_LIGHT_AREA",
     "RL_SAMPLE_UNIFORM", "RL_SAMPLE_COSINE", "RL_TRACE_ANY", "RL_TRACE_CLOSEST",
     "RL_PHOTON_THRESHOLD", "RL_PHOTON_RUSSIAN_ROULETTE", "RL_GATHER_TEXEL_DISC", "RL_GATHER_KNN",
+    "RL_FORMAT_RAW_HDR", "RL_FORMAT_TGA_DOUBLE_LDR", "RL_FORMAT_TGA_RGBM",
 ]

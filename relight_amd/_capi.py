@@ -103,6 +103,10 @@ class rl_indirect_settings(C.Structure):
     ]
 
 
+RL_FORMAT_RAW_HDR = 0
+RL_FORMAT_TGA_DOUBLE_LDR = 1
+RL_FORMAT_TGA_RGBM = 2
+
 RL_GATHER_TEXEL_DISC = 0
 RL_GATHER_KNN = 1
 
@@ -253,6 +257,10 @@ SIGNATURES = [
     ("rl_get_tuning", C.c_int32, [_P, C.POINTER(rl_tuning)]),
     ("rl_get_stats", C.c_int32, [_P, C.POINTER(rl_stats)]),
     ("rl_bake_items", C.c_int32, [_P, C.POINTER(rl_bake_item), C.c_int32]),
+    ("rl_lightmap_expand", C.c_int32, [_P]),
+    ("rl_lightmap_blur", C.c_int32, [_P]),
+    ("rl_lightmap_encode", C.c_int32, [_P, C.c_int32, C.POINTER(C.c_uint8)]),
+    ("rl_lightmap_save", C.c_int32, [_P, C.c_char_p, C.c_int32]),
     ("rl_server_create", C.c_int32, [_P, C.POINTER(rl_server_config), _PP]),
     ("rl_server_submit", C.c_int32, [_P, C.POINTER(rl_job_desc), C.POINTER(C.c_uint64)]),
     ("rl_server_poll", C.c_int32, [_P, C.c_uint64, C.POINTER(rl_job_status)]),

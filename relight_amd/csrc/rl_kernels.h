@@ -87,4 +87,9 @@ hipError_t launch_knn_gather(hipStream_t st, uint32_t max_blocks, const float4* 
 hipError_t launch_final_gather(hipStream_t st, int cap, uint32_t max_blocks, const BakeSeg* segs, int nsegs,
                                uint64_t total_lumels, unsigned long long* queue, unsigned long long* rays);
 
+// Lightmap post-process and encodings (rl_post.hip) on a full w x h grid.
+hipError_t launch_expand(hipStream_t st, float4* grid, const uint8_t* valid, int w, int h);
+hipError_t launch_blur(hipStream_t st, float4* grid, const uint8_t* valid, int w, int h);
+hipError_t launch_encode(hipStream_t st, const float4* grid, uint64_t n, int format, uint8_t* out);
+
 }  // namespace rl

@@ -12,6 +12,7 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdio>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -963,6 +964,139 @@ int32_t rl_lightmap_read_hdr(rl_lightmap* lm, float* out_rgb) {
     int32_t st = pull_colors(lm);
     if (st) return st;
     std::memcpy(out_rgb, lm->color.data(), lm->color.size() * sizeof(float));
+    return RL_SUCCESS;
+}
+
+}  // extern "C"
+
+namespace {
+
+// The full colour grid and validity of a lightmap on the device, for the post-process kernels.
+struct GridDev {
+    float4* grid = nullptr;
+    uint8_t* valid = nullptr;
+    ~GridDev() {
+        dev_free(grid);
+        dev_free(valid);
+    }
+};
+
+int32_t grid_upload(rl_lightmap* lm, GridDev& G) {
+    int32_t st = pull_colors(lm);
+    if (st) return st;
+    const size_t n = (size_t)lm->w * lm->h;
+    std::vector<float4> h(n);
+    for (size_t i = 0; i < n; i++) h[i] = make_float4(lm->color[3 * i], lm->color[3 * i + 1], lm->color[3 * i + 2], 0.0f);
+    RL_HIP(dev_alloc(&G.grid, n));
+    RL_HIP(dev_alloc(&G.valid, n));
+    RL_HIP(hipMemcpy(G.grid, h.data(), n * sizeof(float4), hipMemcpyHostToDevice));
+    RL_HIP(hipMemcpy(G.valid, lm->valid.data(), n, hipMemcpyHostToDevice));
+    return RL_SUCCESS;
+}
+
+// Copies a post-processed grid back into the lightmap (host grid; lumels re-uploaded on the next bake).
+int32_t grid_download(rl_lightmap* lm, const GridDev& G) {
+    const size_t n = (size_t)lm->w * lm->h;
+    std::vector<float4> h(n);
+    RL_HIP(hipStreamSynchronize(lm->ctx->stream));
+    RL_HIP(hipMemcpy(h.data(), G.grid, n * sizeof(float4), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) {
+        lm->color[3 * i] = h[i].x;
+        lm->color[3 * i + 1] = h[i].y;
+        lm->color[3 * i + 2] = h[i].z;
+    }
+    lm->dirty = true;
+    lm->device_newer = false;
+    return RL_SUCCESS;
+}
+
+int32_t post_process(rl_lightmap* lm, bool blur) {
+    if (!lm) return fail(RL_INVALID_ARGUMENT, "lightmap is NULL");
+    if (lm->is_photonmap) return fail(RL_INVALID_CALL, "not a lightmap");
+    GridDev G;
+    int32_t st = grid_upload(lm, G);
+    if (st) return st;
+    const hipError_t e = blur ? launch_blur(lm->ctx->stream, G.grid, G.valid, lm->w, lm->h)
+                              : launch_expand(lm->ctx->stream, G.grid, G.valid, lm->w, lm->h);
+    if (e != hipSuccess) return fail(RL_DEVICE_ERROR, std::string("post-process: ") + hipGetErrorString(e));
+    return grid_download(lm, G);
+}
+
+int32_t encode_ldr(rl_lightmap* lm, int32_t format, std::vector<uint8_t>& out) {
+    const int ch = format == RL_FORMAT_TGA_DOUBLE_LDR ? 3 : 4;
+    const size_t n = (size_t)lm->w * lm->h;
+    GridDev G;
+    int32_t st = grid_upload(lm, G);
+    if (st) return st;
+    uint8_t* d = nullptr;
+    hipError_t e = dev_alloc(&d, n * ch);
+    if (e == hipSuccess) e = launch_encode(lm->ctx->stream, G.grid, n, format, d);
+    if (e == hipSuccess) e = hipStreamSynchronize(lm->ctx->stream);
+    out.resize(n * ch);
+    if (e == hipSuccess) e = hipMemcpy(out.data(), d, n * ch, hipMemcpyDeviceToHost);
+    dev_free(d);
+    if (e != hipSuccess) return fail(RL_DEVICE_ERROR, std::string("encode: ") + hipGetErrorString(e));
+    return RL_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t rl_lightmap_expand(rl_lightmap* lm) { return post_process(lm, false); }
+
+int32_t rl_lightmap_blur(rl_lightmap* lm) { return post_process(lm, true); }
+
+int32_t rl_lightmap_encode(rl_lightmap* lm, int32_t format, uint8_t* out) {
+    if (!lm || !out) return fail(RL_INVALID_ARGUMENT, "NULL argument");
+    if (format != RL_FORMAT_TGA_DOUBLE_LDR && format != RL_FORMAT_TGA_RGBM)
+        return fail(RL_INVALID_ARGUMENT, "encode takes RL_FORMAT_TGA_DOUBLE_LDR or RL_FORMAT_TGA_RGBM");
+    std::vector<uint8_t> px;
+    int32_t st = encode_ldr(lm, format, px);
+    if (st) return st;
+    std::memcpy(out, px.data(), px.size());
+    return RL_SUCCESS;
+}
+
+int32_t rl_lightmap_save(rl_lightmap* lm, const char* path, int32_t format) {
+    if (!lm || !path) return fail(RL_INVALID_ARGUMENT, "NULL argument");
+    if (format != RL_FORMAT_RAW_HDR && format != RL_FORMAT_TGA_DOUBLE_LDR && format != RL_FORMAT_TGA_RGBM)
+        return fail(RL_INVALID_ARGUMENT, "unknown storage format");
+    std::vector<uint8_t> bytes;
+    const int32_t w = lm->w, h = lm->h;
+    if (format == RL_FORMAT_RAW_HDR) {
+        int32_t st = pull_colors(lm);
+        if (st) return st;
+        bytes.resize(8 + lm->color.size() * sizeof(float));
+        std::memcpy(bytes.data(), &w, 4);
+        std::memcpy(bytes.data() + 4, &h, 4);
+        std::memcpy(bytes.data() + 8, lm->color.data(), lm->color.size() * sizeof(float));
+    } else {
+        std::vector<uint8_t> px;
+        int32_t st = encode_ldr(lm, format, px);
+        if (st) return st;
+        const int ch = format == RL_FORMAT_TGA_DOUBLE_LDR ? 3 : 4;
+        // Uncompressed true-colour TGA: 12-byte header, width, height, bits per pixel, descriptor 0,
+        // then pixels bottom-up as stored, b g r (a).
+        const uint8_t head[12] = {0, 0, 2, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        const uint8_t info[6] = {(uint8_t)(w % 256), (uint8_t)(w / 256), (uint8_t)(h % 256), (uint8_t)(h / 256),
+                                 (uint8_t)(ch * 8), 0};
+        bytes.insert(bytes.end(), head, head + 12);
+        bytes.insert(bytes.end(), info, info + 6);
+        const size_t n = (size_t)w * h;
+        for (size_t i = 0; i < n; i++) {
+            const uint8_t* p = &px[i * ch];
+            bytes.push_back(p[2]);
+            bytes.push_back(p[1]);
+            bytes.push_back(p[0]);
+            if (ch == 4) bytes.push_back(p[3]);
+        }
+    }
+    FILE* f = std::fopen(path, "wb");
+    if (!f) return fail(RL_INVALID_ARGUMENT, std::string("cannot open ") + path);
+    const size_t wrote = std::fwrite(bytes.data(), 1, bytes.size(), f);
+    std::fclose(f);
+    if (wrote != bytes.size()) return fail(RL_DEVICE_ERROR, std::string("short write to ") + path);
     return RL_SUCCESS;
 }
 

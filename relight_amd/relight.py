@@ -316,6 +316,26 @@ class Lightmap:
                                                             _fptr(nrm)))
         return texel, pos, nrm
 
+    def expand(self):
+        """Lightmap::expand: invalid texels take the colour of a valid diagonal neighbour."""
+        _check(self._lib, self._lib.rl_lightmap_expand(self._h))
+
+    def blur(self):
+        """Lightmap::blur: in-place 3x3 mean over valid texels, in raster order."""
+        _check(self._lib, self._lib.rl_lightmap_blur(self._h))
+
+    def encode(self, fmt: int) -> np.ndarray:
+        """toDoubleLdr (fmt RL_FORMAT_TGA_DOUBLE_LDR, h x w x 3) or toRgbmLdr (RL_FORMAT_TGA_RGBM,
+        h x w x 4) bytes."""
+        ch = 3 if fmt == A.RL_FORMAT_TGA_DOUBLE_LDR else 4
+        out = np.zeros((self.height, self.width, ch), np.uint8)
+        _check(self._lib, self._lib.rl_lightmap_encode(self._h, int(fmt), out.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return out
+
+    def save(self, path: str, fmt: int = A.RL_FORMAT_RAW_HDR):
+        """Lightmap::save as RawHdr or TGA (double LDR / RGBM)."""
+        _check(self._lib, self._lib.rl_lightmap_save(self._h, str(path).encode(), int(fmt)))
+
     def to_hdr(self) -> np.ndarray:
         """Lightmap::toHdr: (height, width, 3) float32, invalid texels 0."""
         out = np.zeros((self.height, self.width, 3), np.float32)
