@@ -149,6 +149,24 @@ struct FinalGatherParams {
     int32_t sampling;
 };
 
+// A mesh being rasterised into a lightmap's texel grid (positions, normals: 3 floats per vertex;
+// lightmap uvs: 2; indices: 3 per face; tri_base: scene-order id of its first face).
+struct RasterMesh {
+    const float* pos;
+    const float* nrm;
+    const float* uv;
+    const uint32_t* idx;
+    uint32_t nfaces;
+    int32_t w, h;
+    uint32_t tri_base;
+};
+
+// One rasterised texel: index, scene-order triangle, position and (unnormalised) normal.
+struct RasterRec {
+    uint32_t texel, tri;
+    float pos[3], nrm[3];
+};
+
 // One lightmap (or a lumel range of one) inside a bake launch, with the scene, lights, AO and
 // final-gather settings and photon tables it is baked with, so one launch can cover lightmaps of
 // different scenes and jobs.

@@ -87,6 +87,10 @@ hipError_t launch_knn_gather(hipStream_t st, uint32_t max_blocks, const float4* 
 hipError_t launch_final_gather(hipStream_t st, int cap, uint32_t max_blocks, const BakeSeg* segs, int nsegs,
                                uint64_t total_lumels, unsigned long long* queue, unsigned long long* rays);
 
+// Texel raster of one mesh (rl_raster.hip): records of the texels it claims (last face wins).
+hipError_t launch_raster(hipStream_t st, const RasterMesh& M, uint32_t max_blocks, uint32_t* owner, RasterRec* out,
+                         unsigned int* count, unsigned long long* queue);
+
 // Lightmap post-process and encodings (rl_post.hip) on a full w x h grid.
 hipError_t launch_expand(hipStream_t st, float4* grid, const uint8_t* valid, int w, int h);
 hipError_t launch_blur(hipStream_t st, float4* grid, const uint8_t* valid, int w, int h);

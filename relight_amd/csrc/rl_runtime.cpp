@@ -102,6 +102,11 @@ struct rl_relight {
     // every bake helper synchronises its stream before returning.
     BakeSeg* seg_buf[2] = {nullptr, nullptr};
     size_t seg_cap[2] = {0, 0};
+    // Texel-raster scratch: per-texel owner and claimed-texel records (grown to the largest map).
+    uint32_t* raster_owner = nullptr;
+    RasterRec* raster_rec = nullptr;
+    unsigned int* raster_count = nullptr;
+    size_t raster_cap = 0;
 };
 
 struct rl_scene {
@@ -235,46 +240,54 @@ int32_t push_lumels(rl_lightmap* lm) {
     return RL_SUCCESS;
 }
 
-// Lightmap::initializeLumels(const Face&) (Lightmap.cpp:118-154) with Face::uvRect /
-// isUvInside / positionAt / normalAt (scene/Mesh.cpp:397-476) for face f of mesh m.
-void raster_face(rl_lightmap* lm, const MeshHost& m, uint32_t f, uint32_t tri_id) {
-    const uint32_t ia = m.idx[3 * f], ib = m.idx[3 * f + 1], ic = m.idx[3 * f + 2];
-    const float ax = m.uv[2 * ia], ay = m.uv[2 * ia + 1];
-    const float bx = m.uv[2 * ib], by = m.uv[2 * ib + 1];
-    const float cx = m.uv[2 * ic], cy = m.uv[2 * ic + 1];
-    const float minx = std::min(ax, std::min(bx, cx)), maxx = std::max(ax, std::max(bx, cx));
-    const float miny = std::min(ay, std::min(by, cy)), maxy = std::max(ay, std::max(by, cy));
-    const int x1 = (int)(minx * lm->w), x2 = (int)(maxx * lm->w);
-    const int y1 = (int)(miny * lm->h), y2 = (int)(maxy * lm->h);
-    const V3 pa = m.pos[ia], pb = m.pos[ib], pc = m.pos[ic];
-    const V3 na = m.nrm[ia], nb = m.nrm[ib], nc = m.nrm[ic];
-    for (int v = std::max(y1, 0); v <= std::min(y2, lm->h - 1); v++) {
-        for (int u = std::max(x1, 0); u <= std::min(x2, lm->w - 1); u++) {
-            const float uvx = (u + 0.5f) / float(lm->w), uvy = (v + 0.5f) / float(lm->h);
-            const float v0x = cx - ax, v0y = cy - ay;
-            const float v1x = bx - ax, v1y = by - ay;
-            const float v2x = uvx - ax, v2y = uvy - ay;
-            const float dot00 = v0x * v0x + v0y * v0y;
-            const float dot01 = v0x * v1x + v0y * v1y;
-            const float dot02 = v0x * v2x + v0y * v2y;
-            const float dot11 = v1x * v1x + v1y * v1y;
-            const float dot12 = v1x * v2x + v1y * v2y;
-            const float inv = 1.0f / (dot00 * dot11 - dot01 * dot01);
-            const float bu = (dot11 * dot02 - dot01 * dot12) * inv;
-            const float bv = (dot00 * dot12 - dot01 * dot02) * inv;
-            if (!((bu >= 0.0f) && (bv >= 0.0f) && (bu + bv < 1.0f))) continue;
-            const size_t t = (size_t)v * lm->w + u;
-            lm->valid[t] = 1;
-            lm->tri[t] = tri_id;
-            lm->pos[t] = v3(pa.x + (pb.x - pa.x) * bv + (pc.x - pa.x) * bu,
-                            pa.y + (pb.y - pa.y) * bv + (pc.y - pa.y) * bu,
-                            pa.z + (pb.z - pa.z) * bv + (pc.z - pa.z) * bu);
-            lm->nrm[t] = v3(na.x + (nb.x - na.x) * bv + (nc.x - na.x) * bu,
-                            na.y + (nb.y - na.y) * bv + (nc.y - na.y) * bu,
-                            na.z + (nb.z - na.z) * bv + (nc.z - na.z) * bu);
-            lm->color[3 * t] = lm->color[3 * t + 1] = lm->color[3 * t + 2] = 0.0f;
-        }
+// Lightmap::initializeLumels for every face of mesh m (Lightmap.cpp:118-164; Face::isUvInside /
+// positionAt / normalAt, scene/Mesh.cpp:397-476) on the GPU; the claimed texels are applied to the
+// host grid (a texel keeps the last face of the mesh that contains it, over earlier meshes).
+int32_t raster_mesh(rl_lightmap* lm, const MeshHost& m, uint32_t first_tri) {
+    rl_relight* ctx = lm->ctx;
+    const size_t texels = (size_t)lm->w * lm->h;
+    if (m.num_tris == 0 || texels == 0) return RL_SUCCESS;
+    if (lm->w > 16384 || lm->h > 16384) return fail(RL_INVALID_ARGUMENT, "lightmap side above 16384");
+    if (ctx->raster_cap < texels) {
+        dev_free(ctx->raster_owner);
+        dev_free(ctx->raster_rec);
+        ctx->raster_cap = 0;
+        RL_HIP(dev_alloc(&ctx->raster_owner, texels));
+        RL_HIP(dev_alloc(&ctx->raster_rec, texels));
+        ctx->raster_cap = texels;
     }
+    if (!ctx->raster_count) RL_HIP(dev_alloc(&ctx->raster_count, 1));
+    float *dp = nullptr, *dn = nullptr, *du = nullptr;
+    uint32_t* di = nullptr;
+    auto cleanup = [&]() { dev_free(dp); dev_free(dn); dev_free(du); dev_free(di); };
+    hipError_t e = dev_alloc(&dp, m.pos.size() * 3);
+    if (e == hipSuccess) e = dev_alloc(&dn, m.nrm.size() * 3);
+    if (e == hipSuccess) e = dev_alloc(&du, m.uv.size());
+    if (e == hipSuccess) e = dev_alloc(&di, m.idx.size());
+    if (e == hipSuccess) e = hipMemcpy(dp, m.pos.data(), m.pos.size() * sizeof(V3), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dn, m.nrm.data(), m.nrm.size() * sizeof(V3), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(du, m.uv.data(), m.uv.size() * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(di, m.idx.data(), m.idx.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+    RasterMesh M{dp, dn, du, di, m.num_tris, lm->w, lm->h, first_tri};
+    if (e == hipSuccess)
+        e = launch_raster(ctx->stream, M, ctx->max_blocks, ctx->raster_owner, ctx->raster_rec, ctx->raster_count,
+                          ctx->d_queue);
+    unsigned int n = 0;
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(&n, ctx->raster_count, sizeof(n), hipMemcpyDeviceToHost);
+    std::vector<RasterRec> rec(n);
+    if (e == hipSuccess && n) e = hipMemcpy(rec.data(), ctx->raster_rec, n * sizeof(RasterRec), hipMemcpyDeviceToHost);
+    cleanup();
+    if (e != hipSuccess) return fail(RL_DEVICE_ERROR, std::string("texel raster: ") + hipGetErrorString(e));
+    for (const RasterRec& r : rec) {
+        const size_t t = r.texel;
+        lm->valid[t] = 1;
+        lm->tri[t] = r.tri;
+        lm->pos[t] = v3(r.pos[0], r.pos[1], r.pos[2]);
+        lm->nrm[t] = v3(r.nrm[0], r.nrm[1], r.nrm[2]);
+        lm->color[3 * t] = lm->color[3 * t + 1] = lm->color[3 * t + 2] = 0.0f;
+    }
+    return RL_SUCCESS;
 }
 
 struct CallTimer {
@@ -672,6 +685,9 @@ int32_t rl_destroy(rl_relight* ctx) {
     if (ctx->d_queue) hipFree(ctx->d_queue);
     for (BakeSeg* b : ctx->seg_buf)
         if (b) hipFree(b);
+    if (ctx->raster_owner) hipFree(ctx->raster_owner);
+    if (ctx->raster_rec) hipFree(ctx->raster_rec);
+    if (ctx->raster_count) hipFree(ctx->raster_count);
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
     for (hipEvent_t e : ctx->pool) hipEventDestroy(e);
@@ -933,7 +949,7 @@ int32_t rl_lightmap_add_mesh(rl_lightmap* lm, rl_scene* scene, int32_t mesh) {
     // The scene-order triangle id is assigned at end(); until then faces are numbered per mesh.
     uint32_t first = 0;
     for (int32_t i = 0; i < mesh; i++) first += scene->meshes[i].num_tris;
-    for (uint32_t f = 0; f < m.num_tris; f++) raster_face(lm, m, f, first + f);
+    if ((st = raster_mesh(lm, m, first))) return st;
     lm->dirty = true;
     return RL_SUCCESS;
 }
@@ -1233,7 +1249,8 @@ int32_t rl_photonmap_add_mesh(rl_lightmap* pm, rl_scene* scene, int32_t mesh) {
     pm->scene = scene;
     uint32_t first = 0;
     for (int32_t i = 0; i < mesh; i++) first += scene->meshes[i].num_tris;
-    for (uint32_t f = 0; f < m.num_tris; f++) raster_face(pm, m, f, first + f);
+    int32_t st = raster_mesh(pm, m, first);
+    if (st) return st;
     pm->pm_valid_dirty = true;
     return RL_SUCCESS;
 }
