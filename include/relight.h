@@ -308,6 +308,14 @@ typedef struct rl_bake_item {
 } rl_bake_item;
 int32_t rl_bake_items(rl_relight* ctx, const rl_bake_item* items, int32_t num_items);
 
+/* Relight::bake with FullBakeJob (Relight.cpp:205-213, Worker.cpp:43-83) for a job that bakes
+ * direct light, indirect light and AO: every lightmap of the scene, one launch per pass, in the
+ * order photons (indirect != NULL), direct (direct != 0), final gather (indirect), AO (ao != NULL;
+ * it multiplies the accumulated colour). The reference's per-mesh order by bounds volume does not
+ * change results here (sample streams are keyed by lightmap seed and texel). */
+int32_t rl_bake_scene(rl_relight* ctx, rl_scene* scene, int32_t direct, const rl_ao_settings* ao,
+                      const rl_indirect_settings* indirect);
+
 /* Bake server (no counterpart in the reference; replaces the Worker/ThreadWorker job loop of
  * Worker.cpp and src/demo/Lightmapping.cpp:619-649 for serving). A background thread owns the
  * context's stream; submitted jobs wait in a FIFO queue and are admitted while the sum of admitted
@@ -321,6 +329,7 @@ int32_t rl_bake_items(rl_relight* ctx, const rl_bake_item* items, int32_t num_it
 #define RL_JOB_INDIRECT 1      /* photons, then final gather per lightmap */
 #define RL_JOB_RADIOSITY 2
 #define RL_JOB_FINAL_GATHER 3  /* final gather from the scene's resident photonmaps */
+#define RL_JOB_FULL_BAKE 4     /* photons (indirect != NULL), direct (direct != 0), final gather, AO (ao) */
 #define RL_JOB_QUEUED 0
 #define RL_JOB_RUNNING 1
 #define RL_JOB_DONE 2

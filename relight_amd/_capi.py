@@ -160,6 +160,7 @@ RL_JOB_DIRECT_AO = 0
 RL_JOB_INDIRECT = 1
 RL_JOB_RADIOSITY = 2
 RL_JOB_FINAL_GATHER = 3
+RL_JOB_FULL_BAKE = 4
 RL_JOB_QUEUED = 0
 RL_JOB_RUNNING = 1
 RL_JOB_DONE = 2
@@ -257,6 +258,7 @@ SIGNATURES = [
     ("rl_get_tuning", C.c_int32, [_P, C.POINTER(rl_tuning)]),
     ("rl_get_stats", C.c_int32, [_P, C.POINTER(rl_stats)]),
     ("rl_bake_items", C.c_int32, [_P, C.POINTER(rl_bake_item), C.c_int32]),
+    ("rl_bake_scene", C.c_int32, [_P, _P, C.c_int32, C.POINTER(rl_ao_settings), C.POINTER(rl_indirect_settings)]),
     ("rl_lightmap_expand", C.c_int32, [_P]),
     ("rl_lightmap_blur", C.c_int32, [_P]),
     ("rl_lightmap_encode", C.c_int32, [_P, C.c_int32, C.POINTER(C.c_uint8)]),

@@ -1734,6 +1734,8 @@ uint32_t valid_lumels(const rl_lightmap* lm) {
 
 // Ray phases of a job, baked in shared multi-job launches.
 enum RayPhase { kPhaseNone = 0, kPhaseDirect = 1, kPhaseAo = 2, kPhaseGather = 3 };
+// Steps of a job's plan: photon emission (a whole step) or one ray phase over all its lightmaps.
+enum JobStep { kStepEmit = 0, kStepDirect = 1, kStepAo = 2, kStepGather = 3 };
 
 struct Job {
     uint64_t id = 0;
@@ -1751,9 +1753,9 @@ struct Job {
     std::atomic<uint64_t> rays{0};
     double submit_ms = 0, admit_ms = 0, done_ms = 0;
     uint64_t footprint = 0;
-    // DIRECT_AO: 0 direct, 1 AO. INDIRECT: 0 photons, 1 final gather. FINAL_GATHER: 1 final gather.
-    // RADIOSITY: 0 form factors, 1 solve. 2: done.
-    int phase = 0;
+    // Plan (JobStep values) and the current step; radiosity jobs run on the auxiliary worker.
+    std::vector<int> plan;
+    size_t pi = 0;
     size_t map_i = 0;     // next lightmap of the phase
     uint32_t cursor = 0;  // next lumel of that lightmap
     std::atomic<bool> cancel{false};
@@ -1802,23 +1804,24 @@ void fail_job(Job* j, int32_t st) {
     j->state = RL_JOB_FAILED;
 }
 
+int current_step(const Job* j) { return j->pi < j->plan.size() ? j->plan[j->pi] : -1; }
+
 RayPhase ray_phase(const Job* j) {
-    if (job_finished(j)) return kPhaseNone;
-    switch (j->desc.kind) {
-        case RL_JOB_DIRECT_AO: return j->phase == 0 ? kPhaseDirect : kPhaseAo;
-        case RL_JOB_INDIRECT:
-        case RL_JOB_FINAL_GATHER: return j->phase == 1 ? kPhaseGather : kPhaseNone;
+    if (job_finished(j) || j->desc.kind == RL_JOB_RADIOSITY) return kPhaseNone;
+    switch (current_step(j)) {
+        case kStepDirect: return kPhaseDirect;
+        case kStepAo: return kPhaseAo;
+        case kStepGather: return kPhaseGather;
         default: return kPhaseNone;
     }
 }
 
-// Moves a job past a finished ray phase, skipping phases it did not ask for.
+// Moves a job to the next step of its plan, or to done.
 void advance(Job* j) {
     j->map_i = 0;
     j->cursor = 0;
-    j->phase++;
-    if (j->desc.kind == RL_JOB_DIRECT_AO && j->phase == 1 && !j->has_ao) j->phase = 2;
-    if (j->phase >= 2) j->state = RL_JOB_DONE;
+    j->pi++;
+    if (j->pi >= j->plan.size()) j->state = RL_JOB_DONE;
 }
 
 // Takes up to `budget` lumels of the job's current phase into `items`; returns lumels taken.
@@ -1850,14 +1853,19 @@ uint64_t take_lumels(Job* j, uint64_t budget, const BakeItem& proto, std::vector
 
 uint64_t rl_server::step(std::vector<Job*>& js) {
     uint64_t rays = 0;
-    // Whole-step phase: photon emission.
+    // Jobs with nothing left to do, then the whole-step phase: photon emission.
     for (Job* j : js) {
-        if (job_finished(j) || j->desc.kind != RL_JOB_INDIRECT || j->phase != 0) continue;
+        if (job_finished(j)) continue;
+        if (current_step(j) < 0) {
+            j->state = RL_JOB_DONE;
+            continue;
+        }
+        if (current_step(j) != kStepEmit) continue;
         const int32_t st = rl_emit_photons(ctx, j->desc.scene, &j->ind);
         j->rays += ctx->stats.rays;
         rays += ctx->stats.rays;
-        j->phase = 1;
         if (st) fail_job(j, st);
+        else advance(j);
     }
     // Ray phases: upload lumels (and photon tables) on entry, then share the quantum fairly.
     size_t nray = 0;
@@ -2037,7 +2045,8 @@ uint64_t job_footprint(const Job& j) {
     uint64_t lumels = 0;
     for (uint32_t v : j.valid) lumels += v;
     uint64_t bytes = lumels * (3 * sizeof(float4) + sizeof(uint32_t));
-    if (j.desc.kind == RL_JOB_INDIRECT) {
+    const bool emits = j.desc.kind == RL_JOB_INDIRECT || (j.desc.kind == RL_JOB_FULL_BAKE && j.desc.indirect);
+    if (emits) {
         uint64_t per_pass = 0;
         for (const LightHost& L : j.desc.scene->lights)
             per_pass += j.ind.photons_per_light > 0 ? (uint64_t)j.ind.photons_per_light
@@ -2073,6 +2082,47 @@ int32_t rl_bake_items(rl_relight* ctx, const rl_bake_item* items, int32_t num_it
     return end_call(ctx, tm);
 }
 
+int32_t rl_bake_scene(rl_relight* ctx, rl_scene* scene, int32_t direct, const rl_ao_settings* ao,
+                      const rl_indirect_settings* indirect) {
+    if (!ctx) return fail(RL_INVALID_ARGUMENT, "context is NULL");
+    int32_t st = check_scene_ready(scene);
+    if (st) return st;
+    if (ao && (st = check_ao(ao))) return st;
+    if (indirect && (st = check_indirect_settings(indirect))) return st;
+    std::vector<rl_lightmap*> maps;
+    for (const MeshHost& m : scene->meshes)
+        if (m.lightmap && std::find(maps.begin(), maps.end(), m.lightmap) == maps.end()) maps.push_back(m.lightmap);
+    uint64_t photon_rays = 0;
+    if (indirect) {
+        if ((st = rl_emit_photons(ctx, scene, indirect))) return st;
+        photon_rays = ctx->stats.rays;
+    }
+    PhotonTables T;
+    if (indirect && (st = make_photon_tables(scene, T))) return st;
+    CallTimer tm(ctx);
+    if ((st = begin_call(ctx))) return st;
+    std::vector<BakeItem> items;
+    if ((st = whole_items(scene, maps.data(), (int)maps.size(), AoParams{}, items))) return st;
+    if (direct && (st = run_direct_items(ctx, items))) return st;
+    if (indirect) {
+        std::vector<BakeItem> fg(items);
+        for (BakeItem& it : fg) {
+            it.fg = fg_params(indirect);
+            it.mesh_pm = T.d_mesh_pm;
+            it.pms = T.d_pms;
+        }
+        if ((st = run_fg_items(ctx, fg))) return st;
+    }
+    if (ao) {
+        std::vector<BakeItem> a(items);
+        for (BakeItem& it : a) it.ao = ao_params(*ao);
+        if ((st = run_ao_items(ctx, a))) return st;
+    }
+    st = end_call(ctx, tm);
+    ctx->stats.rays += photon_rays;
+    return st;
+}
+
 int32_t rl_server_create(rl_relight* ctx, const rl_server_config* cfg, rl_server** out) {
     if (!ctx || !cfg || !out) return fail(RL_INVALID_ARGUMENT, "NULL argument");
     if (cfg->max_active <= 0 || cfg->quantum_rays == 0 || cfg->memory_budget == 0)
@@ -2101,7 +2151,9 @@ int32_t rl_server_submit(rl_server* s, const rl_job_desc* d, uint64_t* out_id) {
     j->desc = *d;
     int32_t st = check_scene_ready(d->scene);
     if (st) return fail(RL_REJECTED, std::string("job rejected: ") + g_error);
-    if (d->kind == RL_JOB_DIRECT_AO || d->kind == RL_JOB_INDIRECT || d->kind == RL_JOB_FINAL_GATHER) {
+    const bool lightmap_job = d->kind == RL_JOB_DIRECT_AO || d->kind == RL_JOB_INDIRECT ||
+                              d->kind == RL_JOB_FINAL_GATHER || d->kind == RL_JOB_FULL_BAKE;
+    if (lightmap_job) {
         if (!d->lightmaps || d->num_lightmaps <= 0) return fail(RL_REJECTED, "job rejected: no lightmaps");
         for (int32_t i = 0; i < d->num_lightmaps; i++) {
             rl_lightmap* lm = d->lightmaps[i];
@@ -2110,16 +2162,24 @@ int32_t rl_server_submit(rl_server* s, const rl_job_desc* d, uint64_t* out_id) {
             j->valid.push_back(valid_lumels(lm));
         }
     }
-    if (d->kind == RL_JOB_DIRECT_AO) {
-        if (d->ao) {
+    if (lightmap_job) {
+        const bool needs_ind = d->kind != RL_JOB_DIRECT_AO && (d->kind != RL_JOB_FULL_BAKE || d->indirect);
+        const bool takes_ao = d->kind == RL_JOB_DIRECT_AO || d->kind == RL_JOB_FULL_BAKE;
+        if (takes_ao && d->ao) {
             if (check_ao(d->ao)) return fail(RL_REJECTED, std::string("job rejected: ") + g_error);
             j->ao = *d->ao;
             j->has_ao = true;
         }
-    } else if (d->kind == RL_JOB_INDIRECT || d->kind == RL_JOB_FINAL_GATHER) {
-        if (check_indirect_settings(d->indirect)) return fail(RL_REJECTED, std::string("job rejected: ") + g_error);
-        j->ind = *d->indirect;
-        if (d->kind == RL_JOB_FINAL_GATHER) j->phase = 1;
+        if (needs_ind) {
+            if (check_indirect_settings(d->indirect)) return fail(RL_REJECTED, std::string("job rejected: ") + g_error);
+            j->ind = *d->indirect;
+        }
+        // FULL_BAKE: photons, direct, final gather, AO (AO multiplies the accumulated colour).
+        const bool direct = (d->kind == RL_JOB_DIRECT_AO || d->kind == RL_JOB_FULL_BAKE) && d->direct;
+        if (needs_ind && d->kind != RL_JOB_FINAL_GATHER) j->plan.push_back(kStepEmit);
+        if (direct) j->plan.push_back(kStepDirect);
+        if (needs_ind) j->plan.push_back(kStepGather);
+        if (j->has_ao) j->plan.push_back(kStepAo);
     } else if (d->kind == RL_JOB_RADIOSITY) {
         if (!d->radiosity || !d->emission || !d->reflectance || !d->out_radiosity || d->num_rhs == 0 ||
             d->num_rhs > 128 || d->radiosity->scene != d->scene)

@@ -508,6 +508,16 @@ class Relight:
                                     C.pointer(s) if s is not None else None)
         _check(self._lib, self._lib.rl_bake_items(self._h, arr, len(items)))
 
+    def bake(self, scene: Scene, direct: bool = True, ao: Optional[AmbientOcclusionSettings] = None,
+             indirect: Optional[IndirectLightSettings] = None):
+        """Relight::bake with a FullBakeJob that bakes direct light, indirect light and AO over every
+        lightmap of the scene: photons, direct, final gather, then AO, one launch per pass."""
+        a = ao.to_c() if ao is not None else None
+        i = indirect.to_c() if indirect is not None else None
+        _check(self._lib, self._lib.rl_bake_scene(self._h, scene._h, 1 if direct else 0,
+                                                  C.byref(a) if a is not None else None,
+                                                  C.byref(i) if i is not None else None))
+
     def create_server(self, memory_budget=64 << 30, max_active=16, quantum_rays=1 << 28) -> "Server":
         return Server(self, memory_budget, max_active, quantum_rays)
 
@@ -623,6 +633,19 @@ class Server:
         s = settings.to_c()
         d = A.rl_job_desc(kind=A.RL_JOB_FINAL_GATHER, scene=scene._h.value, lightmaps=maps,
                           num_lightmaps=len(lightmaps), indirect=C.pointer(s))
+        return self._submit(d)
+
+    def submit_full_bake(self, scene: Scene, lightmaps: Sequence[Lightmap], direct: bool = True,
+                         ao: Optional[AmbientOcclusionSettings] = None,
+                         indirect: Optional[IndirectLightSettings] = None) -> BakeJob:
+        """Photons (indirect), direct, final gather (indirect), then AO over the lightmaps."""
+        maps = self._maps(lightmaps)
+        a = ao.to_c() if ao is not None else None
+        i = indirect.to_c() if indirect is not None else None
+        d = A.rl_job_desc(kind=A.RL_JOB_FULL_BAKE, scene=scene._h.value, lightmaps=maps,
+                          num_lightmaps=len(lightmaps), direct=1 if direct else 0,
+                          ao=C.pointer(a) if a is not None else None,
+                          indirect=C.pointer(i) if i is not None else None)
         return self._submit(d)
 
     def submit_radiosity(self, radiosity: Radiosity, emission, reflectance, rays_per_patch=1024,

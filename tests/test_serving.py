@@ -155,3 +155,37 @@ def test_server_rejects_and_cancels(relight):
     with relight.create_server() as srv:
         with pytest.raises(RelightError):
             srv.submit_direct_ao(scene, lms, True, bad)
+
+
+def _full_plain(relight, spec):
+    scene, lms, ids = scenes.build_gpu(relight, spec)
+    relight.emit_photons(scene, spec.indirect)
+    for mid in ids:
+        relight.bake_direct_light(scene, mid)
+    for mid in ids:
+        relight.bake_indirect_light(scene, mid, spec.indirect)
+    for mid in ids:
+        relight.bake_ambient_occlusion(scene, mid, spec.ao)
+    return [lm.to_hdr() for lm in lms]
+
+
+def test_full_bake_scene_and_server_job_equal_plain_sequence(relight):
+    """Relight::bake with a direct + indirect + AO job: rl_bake_scene (one launch per pass over every
+    lightmap) and a FULL_BAKE server job equal the per-mesh call sequence bit for bit."""
+    print("[gpu] full bake: scene call and server job vs plain calls", flush=True)
+    spec = scenes.box_room_gi(photons_per_light=10000, fg_samples=32)
+    plain = _full_plain(relight, spec)
+    scene, lms, _ = scenes.build_gpu(relight, spec)
+    relight.bake(scene, True, spec.ao, spec.indirect)
+    st = relight.stats()
+    for p, lm in zip(plain, lms):
+        assert np.array_equal(p, lm.to_hdr())
+    assert st["rays"] > 0
+    s2, l2, _ = scenes.build_gpu(relight, spec)
+    s3, l3, _ = scenes.build_gpu(relight, _small_city(ao_samples=16))
+    with relight.create_server(quantum_rays=1 << 18, max_active=4) as srv:
+        jobs = [srv.submit_full_bake(s2, l2, True, spec.ao, spec.indirect),
+                srv.submit_direct_ao(s3, l3, True, AmbientOcclusionSettings(16, 0.8, 4.0, 1.0, 1, 1))]
+        assert all(j.wait(timeout=120)["state"] == "done" for j in jobs)
+    for p, lm in zip(plain, l2):
+        assert np.array_equal(p, lm.to_hdr())
