@@ -190,6 +190,18 @@ int32_t rl_scene_end(rl_scene* scene);
 int32_t rl_scene_info(const rl_scene* scene, uint32_t* triangles, uint32_t* nodes,
                       int32_t* stack_bound);
 
+/* Wavefront OBJ with the semantics of MeshLoader::load (scene/MeshLoader.cpp:44-226): submeshes
+ * split at v/vt/vn lines after faces, corners deduplicated exactly in first-appearance order,
+ * uvs clamped to [0, 1], missing normals (0, 1, 0), triangles only. rl_obj_load parses a file:
+ * with NULL arrays it returns the counts, else it fills arrays of at least those sizes (3 floats
+ * per position and normal, 2 per uv). rl_scene_add_obj adds the file as one mesh (as
+ * rl_scene_add_mesh). Malformed files (indices out of range, unparsable numbers) give
+ * RL_INVALID_ARGUMENT where the reference reads undefined values. */
+int32_t rl_obj_load(const char* path, uint32_t* vertex_count, uint32_t* index_count, float* positions,
+                    float* normals, float* uvs, uint32_t* indices);
+int32_t rl_scene_add_obj(rl_scene* scene, const char* path, const float* transform, const float* albedo,
+                         int32_t* out_mesh);
+
 /* Relight::createLightmap (Relight.h:159) and Lightmap::addMesh (Lightmap.cpp:93-103), which
  * rasterises the mesh's texels (Lightmap.cpp:133-164). A mesh may belong to one lightmap. The
  * lightmap seed keys its texels' sample streams (default: creation order in the context). */

@@ -258,6 +258,8 @@ SIGNATURES = [
     ("rl_get_tuning", C.c_int32, [_P, C.POINTER(rl_tuning)]),
     ("rl_get_stats", C.c_int32, [_P, C.POINTER(rl_stats)]),
     ("rl_bake_items", C.c_int32, [_P, C.POINTER(rl_bake_item), C.c_int32]),
+    ("rl_obj_load", C.c_int32, [C.c_char_p, _U32, _U32, _F, _F, _F, _U32]),
+    ("rl_scene_add_obj", C.c_int32, [_P, C.c_char_p, _F, _F, _I32]),
     ("rl_bake_scene", C.c_int32, [_P, _P, C.c_int32, C.POINTER(rl_ao_settings), C.POINTER(rl_indirect_settings)]),
     ("rl_lightmap_expand", C.c_int32, [_P]),
     ("rl_lightmap_blur", C.c_int32, [_P]),

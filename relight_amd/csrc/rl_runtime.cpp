@@ -40,6 +40,15 @@ int32_t fail(int32_t code, const std::string& msg) {
     return code;
 }
 
+}  // namespace
+
+// Error reporting for the library's other host translation units (rl_last_error reads it).
+namespace rl {
+int32_t report_error(int32_t code, const std::string& msg) { return fail(code, msg); }
+}  // namespace rl
+
+namespace {
+
 #define RL_HIP(call)                                                                     \
     do {                                                                                 \
         hipError_t e_ = (call);                                                          \
