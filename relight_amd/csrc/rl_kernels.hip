@@ -275,7 +275,7 @@ __global__ __launch_bounds__(kBlock) void k_ao_refill(const BakeSeg* __restrict_
             break;
         }
         if (active) {
-            const int st = any_hit_step<CAP>(scene, tr, node, sp, stk, kBlock, spill);
+            const int st = any_hit_run<CAP>(scene, tr, node, sp, stk, kBlock, spill);
             if (st != 0) {
                 rays++;
                 if (st > 0) atomicAdd(&ring_occ[wave][slot], 1u);

@@ -101,35 +101,41 @@ __device__ __forceinline__ uint32_t stack_pop(int& sp, const uint32_t* stk, uint
     return sp < kLdsStack ? stk[sp * stride] : spill[sp - kLdsStack];
 }
 
-// One step of any-hit traversal: either one internal node (its four child boxes tested, hit
-// children pushed far-first, the nearest taken next) or one leaf (its triangles tested).
+// One internal node of any-hit traversal: its four child boxes tested, hit children pushed
+// far-first, the nearest taken next; with no child hit the next node is popped. Returns false when
+// the stack was empty (traversal ended without a hit).
+template <int CAP>
+__device__ __forceinline__ bool any_hit_node(const SceneDev& S, const TravRay& r, uint32_t& node, int& sp,
+                                             uint32_t* stk, uint32_t stride, uint32_t* spill) {
+    const float4* nd = reinterpret_cast<const float4*>(S.nodes + node);
+    const uint4 ch = reinterpret_cast<const uint4*>(nd)[6];
+    float key[4];
+    uint32_t cid[4] = {ch.x, ch.y, ch.z, ch.w};
+    box4(nd, r.nx, r.ny, r.nz, r.ix, r.iy, r.iz, r.oix, r.oiy, r.oiz, r.w.tnear, r.w.tfar * kBoxScale, cid, key);
+    cswap(key[0], cid[0], key[1], cid[1]);
+    cswap(key[2], cid[2], key[3], cid[3]);
+    cswap(key[0], cid[0], key[2], cid[2]);
+    cswap(key[1], cid[1], key[3], cid[3]);
+    cswap(key[1], cid[1], key[2], cid[2]);
+    if (key[0] == __builtin_inff()) {
+        if (sp == 0) return false;
+        node = stack_pop(sp, stk, stride, spill);
+        return true;
+    }
+#pragma unroll
+    for (int i = 3; i >= 1; i--)
+        if (key[i] != __builtin_inff() && sp < CAP) stack_push(cid[i], sp, stk, stride, spill);
+    node = cid[0];
+    return true;
+}
+
+// One step of any-hit traversal: either one internal node or one leaf (its triangles tested).
 // Returns 1 when the ray is occluded, -1 when traversal ended without a hit, 0 to continue.
 // Same box and triangle tests as traverse(), so results equal traverse<true>().
 template <int CAP>
 __device__ __forceinline__ int any_hit_step(const SceneDev& S, const TravRay& r, uint32_t& node, int& sp,
                                             uint32_t* stk, uint32_t stride, uint32_t* spill) {
-    if (!(node & RL_LEAF_BIT)) {
-        const float4* nd = reinterpret_cast<const float4*>(S.nodes + node);
-        const uint4 ch = reinterpret_cast<const uint4*>(nd)[6];
-        float key[4];
-        uint32_t cid[4] = {ch.x, ch.y, ch.z, ch.w};
-        box4(nd, r.nx, r.ny, r.nz, r.ix, r.iy, r.iz, r.oix, r.oiy, r.oiz, r.w.tnear, r.w.tfar * kBoxScale, cid, key);
-        cswap(key[0], cid[0], key[1], cid[1]);
-        cswap(key[2], cid[2], key[3], cid[3]);
-        cswap(key[0], cid[0], key[2], cid[2]);
-        cswap(key[1], cid[1], key[3], cid[3]);
-        cswap(key[1], cid[1], key[2], cid[2]);
-        if (key[0] == __builtin_inff()) {
-            if (sp == 0) return -1;
-            node = stack_pop(sp, stk, stride, spill);
-            return 0;
-        }
-#pragma unroll
-        for (int i = 3; i >= 1; i--)
-            if (key[i] != __builtin_inff() && sp < CAP) stack_push(cid[i], sp, stk, stride, spill);
-        node = cid[0];
-        return 0;
-    }
+    if (!(node & RL_LEAF_BIT)) return any_hit_node<CAP>(S, r, node, sp, stk, stride, spill) ? 0 : -1;
     const uint32_t first = node & RL_LEAF_FIRST_MASK;
     const uint32_t count = ((node >> 27) & 15u) + 1u;
     for (uint32_t j = 0; j < count; j++) {
@@ -141,6 +147,34 @@ __device__ __forceinline__ int any_hit_step(const SceneDev& S, const TravRay& r,
     if (sp == 0) return -1;
     node = stack_pop(sp, stk, stride, spill);
     return 0;
+}
+
+// A longer step of any-hit traversal in two phases: internal nodes until this lane holds a leaf
+// (a lane that reaches one first waits for the rest of its wavefront, so the triangle tests that
+// follow run with the lanes together), then one triangle per iteration through that leaf and
+// through any leaves popped right after it. Returns like any_hit_step; the set of nodes and
+// triangles a ray visits is the same (only the grouping into steps differs), so the occlusion
+// result is the same.
+template <int CAP>
+__device__ __forceinline__ int any_hit_run(const SceneDev& S, const TravRay& r, uint32_t& node, int& sp,
+                                           uint32_t* stk, uint32_t stride, uint32_t* spill) {
+    while (!(node & RL_LEAF_BIT))
+        if (!any_hit_node<CAP>(S, r, node, sp, stk, stride, spill)) return -1;
+    uint32_t cur = node & RL_LEAF_FIRST_MASK;
+    uint32_t end = cur + ((node >> 27) & 15u) + 1u;
+    for (;;) {
+        const float4* tp = S.tris + 3u * cur;
+        const float4 a = tp[0], b = tp[1], c = tp[2];
+        float t, U, V, W, det;
+        if (wt_intersect(r.w, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t, U, V, W, det)) return 1;
+        if (++cur == end) {
+            if (sp == 0) return -1;
+            node = stack_pop(sp, stk, stride, spill);
+            if (!(node & RL_LEAF_BIT)) return 0;
+            cur = node & RL_LEAF_FIRST_MASK;
+            end = cur + ((node >> 27) & 15u) + 1u;
+        }
+    }
 }
 
 // Traverses the BVH4 for one ray. ANY: stop at the first triangle with t in (tnear, tfar).
