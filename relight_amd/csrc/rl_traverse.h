@@ -101,17 +101,19 @@ __device__ __forceinline__ uint32_t stack_pop(int& sp, const uint32_t* stk, uint
     return sp < kLdsStack ? stk[sp * stride] : spill[sp - kLdsStack];
 }
 
-// One internal node of any-hit traversal: its four child boxes tested, hit children pushed
+// One internal node: its four child boxes tested against (tnear, lim), hit children pushed
 // far-first, the nearest taken next; with no child hit the next node is popped. Returns false when
-// the stack was empty (traversal ended without a hit).
+// the stack was empty (traversal ended).
 template <int CAP>
-__device__ __forceinline__ bool any_hit_node(const SceneDev& S, const TravRay& r, uint32_t& node, int& sp,
-                                             uint32_t* stk, uint32_t stride, uint32_t* spill) {
+__device__ __forceinline__ bool visit_node(const SceneDev& S, uint32_t nx, uint32_t ny, uint32_t nz, float ix, float iy,
+                                           float iz, float oix, float oiy, float oiz, float tnear, float lim,
+                                           uint32_t& node, int& sp, uint32_t* stk, uint32_t stride, uint32_t* spill) {
     const float4* nd = reinterpret_cast<const float4*>(S.nodes + node);
     const uint4 ch = reinterpret_cast<const uint4*>(nd)[6];
     float key[4];
     uint32_t cid[4] = {ch.x, ch.y, ch.z, ch.w};
-    box4(nd, r.nx, r.ny, r.nz, r.ix, r.iy, r.iz, r.oix, r.oiy, r.oiz, r.w.tnear, r.w.tfar * kBoxScale, cid, key);
+    box4(nd, nx, ny, nz, ix, iy, iz, oix, oiy, oiz, tnear, lim, cid, key);
+    // Sort the four (distance, child) pairs: near child next, the rest pushed far-first.
     cswap(key[0], cid[0], key[1], cid[1]);
     cswap(key[2], cid[2], key[3], cid[3]);
     cswap(key[0], cid[0], key[2], cid[2]);
@@ -127,6 +129,13 @@ __device__ __forceinline__ bool any_hit_node(const SceneDev& S, const TravRay& r
         if (key[i] != __builtin_inff() && sp < CAP) stack_push(cid[i], sp, stk, stride, spill);
     node = cid[0];
     return true;
+}
+
+template <int CAP>
+__device__ __forceinline__ bool any_hit_node(const SceneDev& S, const TravRay& r, uint32_t& node, int& sp,
+                                             uint32_t* stk, uint32_t stride, uint32_t* spill) {
+    return visit_node<CAP>(S, r.nx, r.ny, r.nz, r.ix, r.iy, r.iz, r.oix, r.oiy, r.oiz, r.w.tnear, r.w.tfar * kBoxScale,
+                           node, sp, stk, stride, spill);
 }
 
 // One step of any-hit traversal: either one internal node or one leaf (its triangles tested).
@@ -206,63 +215,48 @@ __device__ bool traverse(const SceneDev& S, const Ray& ray, uint32_t* stk, uint3
     res.U = res.V = res.W = res.det = 0.0f;
     uint32_t spill[CAP - kLdsStack];
 
+    // Two phases per round (Aila and Laine 2009, "while-while"): internal nodes until this lane
+    // holds a leaf (a lane that reaches one first waits for the rest of its wavefront, so the
+    // triangle tests run with the lanes together), then one triangle per iteration through that
+    // leaf and any leaves popped right after it. Each lane visits the same nodes in the same order
+    // as a one-node-or-one-leaf-per-iteration loop would.
     uint32_t node = 0;
     int sp = 0;
     for (;;) {
-        if (!(node & RL_LEAF_BIT)) {
-            const float4* nd = reinterpret_cast<const float4*>(S.nodes + node);
-            const uint4 ch = reinterpret_cast<const uint4*>(nd)[6];
-            float key[4];
-            uint32_t cid[4] = {ch.x, ch.y, ch.z, ch.w};
-            box4(nd, nx, ny, nz, ix, iy, iz, oix, oiy, oiz, tnear, tlimit * kBoxScale, cid, key);
-            // Sort the four (distance, child) pairs: near child next, the rest pushed far-first.
-            cswap(key[0], cid[0], key[1], cid[1]);
-            cswap(key[2], cid[2], key[3], cid[3]);
-            cswap(key[0], cid[0], key[2], cid[2]);
-            cswap(key[1], cid[1], key[3], cid[3]);
-            cswap(key[1], cid[1], key[2], cid[2]);
-            if (key[0] == __builtin_inff()) {
-                if (sp == 0) break;
-                --sp;
-                node = sp < kLdsStack ? stk[sp * stride] : spill[sp - kLdsStack];
-                continue;
-            }
-#pragma unroll
-            for (int i = 3; i >= 1; i--) {
-                if (key[i] != __builtin_inff() && sp < CAP) {
-                    if (sp < kLdsStack) stk[sp * stride] = cid[i];
-                    else spill[sp - kLdsStack] = cid[i];
-                    sp++;
+        while (!(node & RL_LEAF_BIT))
+            if (!visit_node<CAP>(S, nx, ny, nz, ix, iy, iz, oix, oiy, oiz, tnear, tlimit * kBoxScale, node, sp, stk,
+                                 stride, spill))
+                goto done;
+        uint32_t cur = node & RL_LEAF_FIRST_MASK;
+        uint32_t end = cur + ((node >> 27) & 15u) + 1u;
+        for (;;) {
+            const float4* tp = S.tris + 3u * cur;
+            const float4 a = tp[0], b = tp[1], c = tp[2];
+            float t, U, V, W, det;
+            if (wt_intersect(w, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t, U, V, W, det)) {
+                if (ANY) {
+                    res.t = t;
+                    res.tri = f_bits(a.w);
+                    return true;
+                }
+                const uint32_t id = f_bits(a.w);
+                if (t < res.t || (t == res.t && id < res.tri)) {
+                    res.t = t;
+                    res.tri = id;
+                    res.U = U; res.V = V; res.W = W; res.det = det;
+                    tlimit = t;
                 }
             }
-            node = cid[0];
-        } else {
-            const uint32_t first = node & RL_LEAF_FIRST_MASK;
-            const uint32_t count = ((node >> 27) & 15u) + 1u;
-            for (uint32_t j = 0; j < count; j++) {
-                const float4* tp = S.tris + 3u * (first + j);
-                const float4 a = tp[0], b = tp[1], c = tp[2];
-                float t, U, V, W, det;
-                if (wt_intersect(w, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t, U, V, W, det)) {
-                    if (ANY) {
-                        res.t = t;
-                        res.tri = f_bits(a.w);
-                        return true;
-                    }
-                    const uint32_t id = f_bits(a.w);
-                    if (t < res.t || (t == res.t && id < res.tri)) {
-                        res.t = t;
-                        res.tri = id;
-                        res.U = U; res.V = V; res.W = W; res.det = det;
-                        tlimit = t;
-                    }
-                }
+            if (++cur == end) {
+                if (sp == 0) goto done;
+                node = stack_pop(sp, stk, stride, spill);
+                if (!(node & RL_LEAF_BIT)) break;
+                cur = node & RL_LEAF_FIRST_MASK;
+                end = cur + ((node >> 27) & 15u) + 1u;
             }
-            if (sp == 0) break;
-            --sp;
-            node = sp < kLdsStack ? stk[sp * stride] : spill[sp - kLdsStack];
         }
     }
+done:
     return res.tri != 0xFFFFFFFFu;
 }
 
