@@ -225,10 +225,11 @@ private:
         std::vector<std::pair<int, int32_t>> recurse;
         for (int i = 0; i < 4; i++) {
             if (i >= (int)kids.size()) {
-                // Empty slot: a point far outside any scene, never inside a ray interval.
-                node.lox[i] = node.hix[i] = 1e30f;
-                node.loy[i] = node.hiy[i] = 1e30f;
-                node.loz[i] = node.hiz[i] = 1e30f;
+                // Empty slot: an inverted box (lo = +1e30, hi = -1e30). For either sign of a
+                // direction component its entry distance is positive and huge (or +inf) and its
+                // exit distance negative, so the slab test never accepts it.
+                node.lox[i] = node.loy[i] = node.loz[i] = 1e30f;
+                node.hix[i] = node.hiy[i] = node.hiz[i] = -1e30f;
                 node.child[i] = RL_EMPTY_CHILD;
                 continue;
             }

@@ -194,19 +194,18 @@ RL_HD WtRay wt_setup(const Ray& r) {
     return w;
 }
 
-// Returns true and the hit distance t when the triangle is hit with tnear < t < tfar. U, V, W are
-// the unnormalised barycentrics of v0, v1, v2; an exactly-zero edge function is recomputed in
-// double precision, as the paper prescribes, so shared edges are never missed.
-RL_HD bool wt_intersect(const WtRay& r, V3 v0, V3 v1, V3 v2, float& t, float& U, float& V,
-                        float& W, float& det) {
-    V3 A = v0 - r.o, B = v1 - r.o, C = v2 - r.o;
-    float Akz = comp(A, r.kz), Bkz = comp(B, r.kz), Ckz = comp(C, r.kz);
-    float Ax = comp(A, r.kx) - r.Sx * Akz;
-    float Ay = comp(A, r.ky) - r.Sy * Akz;
-    float Bx = comp(B, r.kx) - r.Sx * Bkz;
-    float By = comp(B, r.ky) - r.Sy * Bkz;
-    float Cx = comp(C, r.kx) - r.Sx * Ckz;
-    float Cy = comp(C, r.ky) - r.Sy * Ckz;
+// The watertight test from the vertices relative to the ray origin, each given by its kx, ky and
+// kz components: shear, edge functions (an exactly-zero edge function is recomputed in double
+// precision, as the paper prescribes, so shared edges are never missed), inside test and distance.
+// U, V, W are the unnormalised barycentrics of v0, v1, v2.
+RL_HD bool wt_finish(const WtRay& r, float Akx, float Aky, float Akz, float Bkx, float Bky, float Bkz, float Ckx,
+                     float Cky, float Ckz, float& t, float& U, float& V, float& W, float& det) {
+    float Ax = Akx - r.Sx * Akz;
+    float Ay = Aky - r.Sy * Akz;
+    float Bx = Bkx - r.Sx * Bkz;
+    float By = Bky - r.Sy * Bkz;
+    float Cx = Ckx - r.Sx * Ckz;
+    float Cy = Cky - r.Sy * Ckz;
     U = Cx * By - Cy * Bx;
     V = Ax * Cy - Ay * Cx;
     W = Bx * Ay - By * Ax;
@@ -222,6 +221,14 @@ RL_HD bool wt_intersect(const WtRay& r, V3 v0, V3 v1, V3 v2, float& t, float& U,
     float T = U * Az + V * Bz + W * Cz;
     t = T / det;
     return t > r.tnear && t < r.tfar;
+}
+
+// Returns true and the hit distance t when the triangle is hit with tnear < t < tfar.
+RL_HD bool wt_intersect(const WtRay& r, V3 v0, V3 v1, V3 v2, float& t, float& U, float& V,
+                        float& W, float& det) {
+    V3 A = v0 - r.o, B = v1 - r.o, C = v2 - r.o;
+    return wt_finish(r, comp(A, r.kx), comp(A, r.ky), comp(A, r.kz), comp(B, r.kx), comp(B, r.ky), comp(B, r.kz),
+                     comp(C, r.kx), comp(C, r.ky), comp(C, r.kz), t, U, V, W, det);
 }
 
 // ---------------------------------------------------------------- light models

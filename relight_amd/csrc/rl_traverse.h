@@ -20,6 +20,48 @@ constexpr float kBoxScale = 1.0000004f;
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 
+// Global-memory views of the scene arrays. A pointer read from a struct is generic to the
+// compiler, and a generic (flat) load also counts against the LDS wait counter, so every wait for
+// a node or triangle fetch would also wait for the stack's LDS traffic.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef const f32x4 __attribute__((address_space(1))) gfloat4;
+typedef const u32x4 __attribute__((address_space(1))) guint4;
+typedef const float __attribute__((address_space(1))) gfloat;
+__device__ __forceinline__ const gfloat4* gmem(const float4* p) { return (const gfloat4*)p; }
+
+// The watertight test's axis permutation applied once per ray: byte offsets of the kx, ky and kz
+// components within a vertex, and the origin's components in that order. A triangle test then
+// loads the three components it needs of each vertex directly instead of selecting them.
+struct WtPerm {
+    uint32_t okx, oky, okz;
+    float px, py, pz;
+};
+
+__device__ __forceinline__ WtPerm wt_perm(const WtRay& w) {
+    WtPerm p;
+    p.okx = 4u * (uint32_t)w.kx;
+    p.oky = 4u * (uint32_t)w.ky;
+    p.okz = 4u * (uint32_t)w.kz;
+    p.px = comp(w.o, w.kx);
+    p.py = comp(w.o, w.ky);
+    p.pz = comp(w.o, w.kz);
+    return p;
+}
+
+// wt_intersect of the triangle stored as three float4 rows (vertex xyz, then w) at `tp`: the same
+// operations on the same values, with each component read at its permuted offset.
+__device__ __forceinline__ bool wt_intersect_at(const WtRay& r, const WtPerm& p, const gfloat4* tp, float& t, float& U,
+                                                float& V, float& W, float& det) {
+    typedef const char __attribute__((address_space(1))) gchar;
+    gchar* b = reinterpret_cast<gchar*>(tp);
+    const gfloat* fx = reinterpret_cast<const gfloat*>(b + p.okx);
+    const gfloat* fy = reinterpret_cast<const gfloat*>(b + p.oky);
+    const gfloat* fz = reinterpret_cast<const gfloat*>(b + p.okz);
+    return wt_finish(r, fx[0] - p.px, fy[0] - p.py, fz[0] - p.pz, fx[4] - p.px, fy[4] - p.py, fz[4] - p.pz,
+                     fx[8] - p.px, fy[8] - p.py, fz[8] - p.pz, t, U, V, W, det);
+}
+
 __device__ __forceinline__ float bits_f(uint32_t u) { return __uint_as_float(u); }
 __device__ __forceinline__ uint32_t f_bits(float f) { return __float_as_uint(f); }
 
@@ -71,10 +113,10 @@ __device__ __forceinline__ TravRay trav_ray(const Ray& ray) {
 // for a box the ray misses within (tnear, lim). The entry plane of an axis is lo for a positive
 // direction component and hi for a negative one, so t_entry = fma(plane, 1/d, -o/d) equals the
 // min of the two plane distances (fma rounding is monotone in the plane) and t_exit the max.
-__device__ __forceinline__ void box4(const float4* nd, uint32_t nx, uint32_t ny, uint32_t nz, float ix, float iy,
+__device__ __forceinline__ void box4(const gfloat4* nd, uint32_t nx, uint32_t ny, uint32_t nz, float ix, float iy,
                                      float iz, float oix, float oiy, float oiz, float tnear, float lim,
-                                     const uint32_t* cid, float* key) {
-    const float4 ex = nd[nx], fx = nd[1u - nx], ey = nd[2u + ny], fy = nd[3u - ny], ez = nd[4u + nz], fz = nd[5u - nz];
+                                     float* key) {
+    const f32x4 ex = nd[nx], fx = nd[1u - nx], ey = nd[2u + ny], fy = nd[3u - ny], ez = nd[4u + nz], fz = nd[5u - nz];
     const float exa[4] = {ex.x, ex.y, ex.z, ex.w}, fxa[4] = {fx.x, fx.y, fx.z, fx.w};
     const float eya[4] = {ey.x, ey.y, ey.z, ey.w}, fya[4] = {fy.x, fy.y, fy.z, fy.w};
     const float eza[4] = {ez.x, ez.y, ez.z, ez.w}, fza[4] = {fz.x, fz.y, fz.z, fz.w};
@@ -85,7 +127,7 @@ __device__ __forceinline__ void box4(const float4* nd, uint32_t nx, uint32_t ny,
         const float t0z = fmaf(eza[i], iz, oiz), t1z = fmaf(fza[i], iz, oiz);
         const float tmin = fmaxf(fmaxf(t0x, t0y), fmaxf(t0z, tnear));
         const float tmax = fminf(fminf(t1x, t1y), t1z) * kBoxScale;
-        const bool hit = tmin <= fminf(tmax, lim) && cid[i] != RL_EMPTY_CHILD;
+        const bool hit = tmin <= fminf(tmax, lim);  // empty slots hold inverted boxes
         key[i] = hit ? tmin : __builtin_inff();
     }
 }
@@ -98,7 +140,10 @@ __device__ __forceinline__ void stack_push(uint32_t v, int& sp, uint32_t* stk, u
 
 __device__ __forceinline__ uint32_t stack_pop(int& sp, const uint32_t* stk, uint32_t stride, const uint32_t* spill) {
     --sp;
-    return sp < kLdsStack ? stk[sp * stride] : spill[sp - kLdsStack];
+    if (sp < kLdsStack) return stk[sp * stride];
+    // volatile keeps the compiler from merging the two loads into one generic load of a selected
+    // address (which would make every pop a flat load)
+    return *(volatile const uint32_t*)&spill[sp - kLdsStack];
 }
 
 // One internal node: its four child boxes tested against (tnear, lim), hit children pushed
@@ -108,11 +153,16 @@ template <int CAP>
 __device__ __forceinline__ bool visit_node(const SceneDev& S, uint32_t nx, uint32_t ny, uint32_t nz, float ix, float iy,
                                            float iz, float oix, float oiy, float oiz, float tnear, float lim,
                                            uint32_t& node, int& sp, uint32_t* stk, uint32_t stride, uint32_t* spill) {
-    const float4* nd = reinterpret_cast<const float4*>(S.nodes + node);
-    const uint4 ch = reinterpret_cast<const uint4*>(nd)[6];
+    const gfloat4* nd = gmem(reinterpret_cast<const float4*>(S.nodes + node));
+    const u32x4 ch = reinterpret_cast<const guint4*>(nd)[6];
     float key[4];
     uint32_t cid[4] = {ch.x, ch.y, ch.z, ch.w};
-    box4(nd, nx, ny, nz, ix, iy, iz, oix, oiy, oiz, tnear, lim, cid, key);
+    box4(nd, nx, ny, nz, ix, iy, iz, oix, oiy, oiz, tnear, lim, key);
+    // The child ids are only used when a box was hit; this keeps their load with the box rows
+    // instead of letting the compiler issue it (and wait for it) behind the test.
+    asm volatile("" ::"v"(ch.x), "v"(ch.y), "v"(ch.z), "v"(ch.w));
+    // The child ids are only used when a box was hit; this keeps their load with the box rows
+    // instead of letting the compiler issue it (and wait for it) behind the test.
     // Sort the four (distance, child) pairs: near child next, the rest pushed far-first.
     cswap(key[0], cid[0], key[1], cid[1]);
     cswap(key[2], cid[2], key[3], cid[3]);
@@ -148,8 +198,8 @@ __device__ __forceinline__ int any_hit_step(const SceneDev& S, const TravRay& r,
     const uint32_t first = node & RL_LEAF_FIRST_MASK;
     const uint32_t count = ((node >> 27) & 15u) + 1u;
     for (uint32_t j = 0; j < count; j++) {
-        const float4* tp = S.tris + 3u * (first + j);
-        const float4 a = tp[0], b = tp[1], c = tp[2];
+        const gfloat4* tp = gmem(S.tris) + 3u * (first + j);
+        const f32x4 a = tp[0], b = tp[1], c = tp[2];
         float t, U, V, W, det;
         if (wt_intersect(r.w, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t, U, V, W, det)) return 1;
     }
@@ -172,8 +222,10 @@ __device__ __forceinline__ int any_hit_run(const SceneDev& S, const TravRay& r, 
     uint32_t cur = node & RL_LEAF_FIRST_MASK;
     uint32_t end = cur + ((node >> 27) & 15u) + 1u;
     for (;;) {
-        const float4* tp = S.tris + 3u * cur;
-        const float4 a = tp[0], b = tp[1], c = tp[2];
+        // Three row loads and per-lane component selects: measured faster here than the nine
+        // component loads of wt_intersect_at (this loop is bound by vector-memory data return).
+        const gfloat4* tp = gmem(S.tris) + 3u * cur;
+        const f32x4 a = tp[0], b = tp[1], c = tp[2];
         float t, U, V, W, det;
         if (wt_intersect(r.w, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t, U, V, W, det)) return 1;
         if (++cur == end) {
@@ -200,6 +252,7 @@ template <bool ANY, int CAP>
 __device__ bool traverse(const SceneDev& S, const Ray& ray, uint32_t* stk, uint32_t stride,
                          TraceResult& res) {
     const WtRay w = wt_setup(ray);
+    const WtPerm wp = wt_perm(w);
     // A zero direction component would give fma(b, inf, -inf) = NaN; the slab test uses
     // +-1e-20 there instead (a displacement below 1e-17 m over any ray, inside the box padding).
     const float dx = fabsf(ray.d.x) > 1e-20f ? ray.d.x : copysignf(1e-20f, ray.d.x);
@@ -230,16 +283,15 @@ __device__ bool traverse(const SceneDev& S, const Ray& ray, uint32_t* stk, uint3
         uint32_t cur = node & RL_LEAF_FIRST_MASK;
         uint32_t end = cur + ((node >> 27) & 15u) + 1u;
         for (;;) {
-            const float4* tp = S.tris + 3u * cur;
-            const float4 a = tp[0], b = tp[1], c = tp[2];
+            const gfloat4* tp = gmem(S.tris) + 3u * cur;
             float t, U, V, W, det;
-            if (wt_intersect(w, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t, U, V, W, det)) {
+            if (wt_intersect_at(w, wp, tp, t, U, V, W, det)) {
+                const uint32_t id = f_bits(reinterpret_cast<const gfloat*>(tp)[3]);
                 if (ANY) {
                     res.t = t;
-                    res.tri = f_bits(a.w);
+                    res.tri = id;
                     return true;
                 }
-                const uint32_t id = f_bits(a.w);
                 if (t < res.t || (t == res.t && id < res.tri)) {
                     res.t = t;
                     res.tri = id;
