@@ -281,7 +281,26 @@ int build_bvh4(const float* tri9, uint32_t num_tris, Bvh4Build& out) {
     const float pad = maxc * (1.0f / 1048576.0f) + 1e-30f;
     Collapser c(b2.nodes_, pad);
     c.run();
-    out.nodes = std::move(c.out_);
+    // Renumber the nodes in breadth-first order (the root stays 0), so the nodes nearest the root,
+    // which most rays visit, form the prefix that the kernels keep in LDS.
+    const std::vector<Bvh4Node>& dfs = c.out_;
+    std::vector<uint32_t> bfs;
+    bfs.reserve(dfs.size());
+    bfs.push_back(0);
+    for (size_t h = 0; h < bfs.size(); h++)
+        for (int i = 0; i < 4; i++) {
+            const uint32_t ch = dfs[bfs[h]].child[i];
+            if (ch != RL_EMPTY_CHILD && !(ch & RL_LEAF_BIT)) bfs.push_back(ch);
+        }
+    std::vector<uint32_t> rank(dfs.size());
+    for (size_t r = 0; r < bfs.size(); r++) rank[bfs[r]] = (uint32_t)r;
+    out.nodes.resize(dfs.size());
+    for (size_t r = 0; r < bfs.size(); r++) {
+        Bvh4Node n = dfs[bfs[r]];
+        for (int i = 0; i < 4; i++)
+            if (n.child[i] != RL_EMPTY_CHILD && !(n.child[i] & RL_LEAF_BIT)) n.child[i] = rank[n.child[i]];
+        out.nodes[r] = n;
+    }
     out.order = std::move(b2.order_);
     out.stack_bound = c.stack_bound_;
     out.depth = c.depth_;

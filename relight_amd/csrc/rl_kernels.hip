@@ -186,6 +186,9 @@ __global__ __launch_bounds__(kBlock) void k_ao_count(const BakeSeg* __restrict__
 // only start a lumel fewer than kRing - 2 places after the oldest lumel still in flight, and the
 // lane that completes a lumel's last ray writes its count.
 constexpr int kRing = 32;
+// Nodes of the first segment's scene kept in LDS per workgroup (10.5 KB; with the stack and the
+// rings a workgroup takes 19 KB, so eight fit a compute unit's 160 KB).
+constexpr uint32_t kAoTopNodes = 96;
 
 template <int CAP>
 __global__ __launch_bounds__(kBlock) void k_ao_refill(const BakeSeg* __restrict__ segs, int nsegs,
@@ -195,6 +198,9 @@ __global__ __launch_bounds__(kBlock) void k_ao_refill(const BakeSeg* __restrict_
     __shared__ uint32_t stack[kLdsStack * kBlock];
     __shared__ uint32_t ring_occ[kBlock / 64][kRing];
     __shared__ uint32_t ring_done[kBlock / 64][kRing];
+    __shared__ u32x4 top_rows[kAoTopNodes * kNodeRows];
+    const SceneDev top_scene = segs[0].scene;
+    const uint32_t top_count = load_top_nodes(top_scene, top_rows, kAoTopNodes);
     uint32_t* stk = stack + threadIdx.x;
     uint32_t spill[CAP - kLdsStack];
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
@@ -215,6 +221,8 @@ __global__ __launch_bounds__(kBlock) void k_ao_refill(const BakeSeg* __restrict_
     int sp = 0;
     uint32_t* counter = nullptr;
     uint32_t rays = 0;
+    TopCache top{top_rows, 0u};
+    __syncthreads();
     for (;;) {
         uint64_t idle = __ballot(!active);
         if (pool_open && ((uint32_t)__popcll(idle) >= refill_min || idle == ~0ull)) {
@@ -259,6 +267,7 @@ __global__ __launch_bounds__(kBlock) void k_ao_refill(const BakeSeg* __restrict_
                     const V3 dir = sample_hemisphere(rng, n, sg.ao.sampling);
                     tr = trav_ray(ray_from_segment(p, p + dir * sg.ao.max_distance));
                     scene = sg.scene;
+                    top.count = sg.scene.nodes == top_scene.nodes ? top_count : 0u;
                     node = 0;
                     sp = 0;
                     my_stream = stream_base + lc;
@@ -275,7 +284,7 @@ __global__ __launch_bounds__(kBlock) void k_ao_refill(const BakeSeg* __restrict_
             break;
         }
         if (active) {
-            const int st = any_hit_run<CAP>(scene, tr, node, sp, stk, kBlock, spill);
+            const int st = any_hit_run<CAP>(scene, top, tr, node, sp, stk, kBlock, spill);
             if (st != 0) {
                 rays++;
                 if (st > 0) atomicAdd(&ring_occ[wave][slot], 1u);

@@ -109,14 +109,13 @@ __device__ __forceinline__ TravRay trav_ray(const Ray& ray) {
     return r;
 }
 
-// Four child boxes of a node against a ray: the entry distance of each box (its sort key), +inf
-// for a box the ray misses within (tnear, lim). The entry plane of an axis is lo for a positive
+// Four child boxes of a node against a ray, given the node's entry-plane (e*) and exit-plane (f*)
+// rows of each axis: the entry distance of each box (its sort key), +inf for a box the ray misses
+// within (tnear, lim). The entry plane of an axis is lo for a positive
 // direction component and hi for a negative one, so t_entry = fma(plane, 1/d, -o/d) equals the
 // min of the two plane distances (fma rounding is monotone in the plane) and t_exit the max.
-__device__ __forceinline__ void box4(const gfloat4* nd, uint32_t nx, uint32_t ny, uint32_t nz, float ix, float iy,
-                                     float iz, float oix, float oiy, float oiz, float tnear, float lim,
-                                     float* key) {
-    const f32x4 ex = nd[nx], fx = nd[1u - nx], ey = nd[2u + ny], fy = nd[3u - ny], ez = nd[4u + nz], fz = nd[5u - nz];
+__device__ __forceinline__ void box4(f32x4 ex, f32x4 fx, f32x4 ey, f32x4 fy, f32x4 ez, f32x4 fz, float ix, float iy,
+                                     float iz, float oix, float oiy, float oiz, float tnear, float lim, float* key) {
     const float exa[4] = {ex.x, ex.y, ex.z, ex.w}, fxa[4] = {fx.x, fx.y, fx.z, fx.w};
     const float eya[4] = {ey.x, ey.y, ey.z, ey.w}, fya[4] = {fy.x, fy.y, fy.z, fy.w};
     const float eza[4] = {ez.x, ez.y, ez.z, ez.w}, fza[4] = {fz.x, fz.y, fz.z, fz.w};
@@ -146,23 +145,58 @@ __device__ __forceinline__ uint32_t stack_pop(int& sp, const uint32_t* stk, uint
     return *(volatile const uint32_t*)&spill[sp - kLdsStack];
 }
 
+// The first nodes of a scene in breadth-first order (the levels nearest the root, which most rays
+// visit) copied into LDS by a workgroup: seven 16-B rows per node (the six plane rows and the child
+// ids). A node visit reads a cached node from LDS instead of through the vector-memory path.
+constexpr uint32_t kNodeRows = 7;
+
+struct TopCache {
+    const u32x4* rows;  // LDS, kNodeRows rows per node
+    uint32_t count;     // nodes held for this lane's scene (0: none)
+};
+
+// Copies the first min(max_nodes, S.num_nodes) nodes of S into `rows` (LDS) with the whole
+// workgroup and returns how many it holds; the caller synchronises the workgroup before use.
+__device__ __forceinline__ uint32_t load_top_nodes(const SceneDev& S, u32x4* rows, uint32_t max_nodes) {
+    const uint32_t n = S.num_nodes < max_nodes ? S.num_nodes : max_nodes;
+    const guint4* src = reinterpret_cast<const guint4*>(gmem(reinterpret_cast<const float4*>(S.nodes)));
+    for (uint32_t i = threadIdx.x; i < n * kNodeRows; i += blockDim.x) {
+        const uint32_t nd = i / kNodeRows, row = i - nd * kNodeRows;
+        rows[i] = src[8u * nd + row];
+    }
+    return n;
+}
+
 // One internal node: its four child boxes tested against (tnear, lim), hit children pushed
 // far-first, the nearest taken next; with no child hit the next node is popped. Returns false when
 // the stack was empty (traversal ended).
 template <int CAP>
-__device__ __forceinline__ bool visit_node(const SceneDev& S, uint32_t nx, uint32_t ny, uint32_t nz, float ix, float iy,
-                                           float iz, float oix, float oiy, float oiz, float tnear, float lim,
-                                           uint32_t& node, int& sp, uint32_t* stk, uint32_t stride, uint32_t* spill) {
-    const gfloat4* nd = gmem(reinterpret_cast<const float4*>(S.nodes + node));
-    const u32x4 ch = reinterpret_cast<const guint4*>(nd)[6];
+__device__ __forceinline__ bool visit_node(const SceneDev& S, const TopCache& top, uint32_t nx, uint32_t ny,
+                                           uint32_t nz, float ix, float iy, float iz, float oix, float oiy, float oiz,
+                                           float tnear, float lim, uint32_t& node, int& sp, uint32_t* stk,
+                                           uint32_t stride, uint32_t* spill) {
+    f32x4 ex, fx, ey, fy, ez, fz;
+    u32x4 ch;
+    if (node < top.count) {
+        const u32x4* nd = top.rows + kNodeRows * node;
+        ex = __builtin_bit_cast(f32x4, nd[nx]);
+        fx = __builtin_bit_cast(f32x4, nd[1u - nx]);
+        ey = __builtin_bit_cast(f32x4, nd[2u + ny]);
+        fy = __builtin_bit_cast(f32x4, nd[3u - ny]);
+        ez = __builtin_bit_cast(f32x4, nd[4u + nz]);
+        fz = __builtin_bit_cast(f32x4, nd[5u - nz]);
+        ch = nd[6];
+    } else {
+        const gfloat4* nd = gmem(reinterpret_cast<const float4*>(S.nodes + node));
+        ex = nd[nx]; fx = nd[1u - nx]; ey = nd[2u + ny]; fy = nd[3u - ny]; ez = nd[4u + nz]; fz = nd[5u - nz];
+        ch = reinterpret_cast<const guint4*>(nd)[6];
+    }
     float key[4];
     uint32_t cid[4] = {ch.x, ch.y, ch.z, ch.w};
-    box4(nd, nx, ny, nz, ix, iy, iz, oix, oiy, oiz, tnear, lim, key);
+    box4(ex, fx, ey, fy, ez, fz, ix, iy, iz, oix, oiy, oiz, tnear, lim, key);
     // The child ids are only used when a box was hit; this keeps their load with the box rows
     // instead of letting the compiler issue it (and wait for it) behind the test.
     asm volatile("" ::"v"(ch.x), "v"(ch.y), "v"(ch.z), "v"(ch.w));
-    // The child ids are only used when a box was hit; this keeps their load with the box rows
-    // instead of letting the compiler issue it (and wait for it) behind the test.
     // Sort the four (distance, child) pairs: near child next, the rest pushed far-first.
     cswap(key[0], cid[0], key[1], cid[1]);
     cswap(key[2], cid[2], key[3], cid[3]);
@@ -182,26 +216,27 @@ __device__ __forceinline__ bool visit_node(const SceneDev& S, uint32_t nx, uint3
 }
 
 template <int CAP>
-__device__ __forceinline__ bool any_hit_node(const SceneDev& S, const TravRay& r, uint32_t& node, int& sp,
-                                             uint32_t* stk, uint32_t stride, uint32_t* spill) {
-    return visit_node<CAP>(S, r.nx, r.ny, r.nz, r.ix, r.iy, r.iz, r.oix, r.oiy, r.oiz, r.w.tnear, r.w.tfar * kBoxScale,
-                           node, sp, stk, stride, spill);
+__device__ __forceinline__ bool any_hit_node(const SceneDev& S, const TopCache& top, const TravRay& r, uint32_t& node,
+                                             int& sp, uint32_t* stk, uint32_t stride, uint32_t* spill) {
+    return visit_node<CAP>(S, top, r.nx, r.ny, r.nz, r.ix, r.iy, r.iz, r.oix, r.oiy, r.oiz, r.w.tnear,
+                           r.w.tfar * kBoxScale, node, sp, stk, stride, spill);
 }
 
 // One step of any-hit traversal: either one internal node or one leaf (its triangles tested).
 // Returns 1 when the ray is occluded, -1 when traversal ended without a hit, 0 to continue.
 // Same box and triangle tests as traverse(), so results equal traverse<true>().
 template <int CAP>
-__device__ __forceinline__ int any_hit_step(const SceneDev& S, const TravRay& r, uint32_t& node, int& sp,
-                                            uint32_t* stk, uint32_t stride, uint32_t* spill) {
-    if (!(node & RL_LEAF_BIT)) return any_hit_node<CAP>(S, r, node, sp, stk, stride, spill) ? 0 : -1;
+__device__ __forceinline__ int any_hit_step(const SceneDev& S, const TopCache& top, const TravRay& r, uint32_t& node,
+                                            int& sp, uint32_t* stk, uint32_t stride, uint32_t* spill) {
+    if (!(node & RL_LEAF_BIT)) return any_hit_node<CAP>(S, top, r, node, sp, stk, stride, spill) ? 0 : -1;
     const uint32_t first = node & RL_LEAF_FIRST_MASK;
     const uint32_t count = ((node >> 27) & 15u) + 1u;
     for (uint32_t j = 0; j < count; j++) {
-        const gfloat4* tp = gmem(S.tris) + 3u * (first + j);
-        const f32x4 a = tp[0], b = tp[1], c = tp[2];
+        const gfloat* tf = reinterpret_cast<const gfloat*>(gmem(S.tris) + 3u * (first + j));
         float t, U, V, W, det;
-        if (wt_intersect(r.w, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t, U, V, W, det)) return 1;
+        if (wt_intersect(r.w, v3(tf[0], tf[1], tf[2]), v3(tf[4], tf[5], tf[6]), v3(tf[8], tf[9], tf[10]), t, U, V, W,
+                         det))
+            return 1;
     }
     if (sp == 0) return -1;
     node = stack_pop(sp, stk, stride, spill);
@@ -215,19 +250,20 @@ __device__ __forceinline__ int any_hit_step(const SceneDev& S, const TravRay& r,
 // triangles a ray visits is the same (only the grouping into steps differs), so the occlusion
 // result is the same.
 template <int CAP>
-__device__ __forceinline__ int any_hit_run(const SceneDev& S, const TravRay& r, uint32_t& node, int& sp,
-                                           uint32_t* stk, uint32_t stride, uint32_t* spill) {
+__device__ __forceinline__ int any_hit_run(const SceneDev& S, const TopCache& top, const TravRay& r, uint32_t& node,
+                                           int& sp, uint32_t* stk, uint32_t stride, uint32_t* spill) {
     while (!(node & RL_LEAF_BIT))
-        if (!any_hit_node<CAP>(S, r, node, sp, stk, stride, spill)) return -1;
+        if (!any_hit_node<CAP>(S, top, r, node, sp, stk, stride, spill)) return -1;
     uint32_t cur = node & RL_LEAF_FIRST_MASK;
     uint32_t end = cur + ((node >> 27) & 15u) + 1u;
     for (;;) {
         // Three row loads and per-lane component selects: measured faster here than the nine
         // component loads of wt_intersect_at (this loop is bound by vector-memory data return).
-        const gfloat4* tp = gmem(S.tris) + 3u * cur;
-        const f32x4 a = tp[0], b = tp[1], c = tp[2];
+        const gfloat* tf = reinterpret_cast<const gfloat*>(gmem(S.tris) + 3u * cur);
         float t, U, V, W, det;
-        if (wt_intersect(r.w, v3(a.x, a.y, a.z), v3(b.x, b.y, b.z), v3(c.x, c.y, c.z), t, U, V, W, det)) return 1;
+        if (wt_intersect(r.w, v3(tf[0], tf[1], tf[2]), v3(tf[4], tf[5], tf[6]), v3(tf[8], tf[9], tf[10]), t, U, V, W,
+                         det))
+            return 1;
         if (++cur == end) {
             if (sp == 0) return -1;
             node = stack_pop(sp, stk, stride, spill);
@@ -249,7 +285,7 @@ __device__ __forceinline__ int any_hit_run(const SceneDev& S, const TravRay& r, 
 // the builder's box padding (more than 2^-21 of the largest coordinate) covers, so no box the
 // exact ray crosses is rejected.
 template <bool ANY, int CAP>
-__device__ bool traverse(const SceneDev& S, const Ray& ray, uint32_t* stk, uint32_t stride,
+__device__ bool traverse(const SceneDev& S, const TopCache& top, const Ray& ray, uint32_t* stk, uint32_t stride,
                          TraceResult& res) {
     const WtRay w = wt_setup(ray);
     const WtPerm wp = wt_perm(w);
@@ -277,7 +313,7 @@ __device__ bool traverse(const SceneDev& S, const Ray& ray, uint32_t* stk, uint3
     int sp = 0;
     for (;;) {
         while (!(node & RL_LEAF_BIT))
-            if (!visit_node<CAP>(S, nx, ny, nz, ix, iy, iz, oix, oiy, oiz, tnear, tlimit * kBoxScale, node, sp, stk,
+            if (!visit_node<CAP>(S, top, nx, ny, nz, ix, iy, iz, oix, oiy, oiz, tnear, tlimit * kBoxScale, node, sp, stk,
                                  stride, spill))
                 goto done;
         uint32_t cur = node & RL_LEAF_FIRST_MASK;
@@ -310,6 +346,13 @@ __device__ bool traverse(const SceneDev& S, const Ray& ray, uint32_t* stk, uint3
     }
 done:
     return res.tri != 0xFFFFFFFFu;
+}
+
+// traverse() without an LDS copy of the top nodes.
+template <bool ANY, int CAP>
+__device__ __forceinline__ bool traverse(const SceneDev& S, const Ray& ray, uint32_t* stk, uint32_t stride,
+                                         TraceResult& res) {
+    return traverse<ANY, CAP>(S, TopCache{nullptr, 0u}, ray, stk, stride, res);
 }
 
 __device__ __forceinline__ int find_seg(const BakeSeg* segs, int nsegs, uint64_t g) {
