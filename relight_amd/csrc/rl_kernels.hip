@@ -186,18 +186,20 @@ __global__ __launch_bounds__(kBlock) void k_ao_count(const BakeSeg* __restrict__
 // only start a lumel fewer than kRing - 2 places after the oldest lumel still in flight, and the
 // lane that completes a lumel's last ray writes its count.
 constexpr int kRing = 32;
-// Nodes of the first segment's scene kept in LDS per workgroup (10.5 KB; with the stack and the
-// rings a workgroup takes 19 KB, so eight fit a compute unit's 160 KB).
-constexpr uint32_t kAoTopNodes = 96;
+// Workgroups of four wavefronts, each keeping the first 192 nodes of the first segment's scene in
+// LDS (21 KB; with the stacks and rings a workgroup takes 38 KB, so four fit a compute unit's
+// 160 KB: sixteen wavefronts, the occupancy the kernel's registers allow).
+constexpr int kAoBlock = 256;
+constexpr uint32_t kAoTopNodes = 192;
 
 template <int CAP>
-__global__ __launch_bounds__(kBlock) void k_ao_refill(const BakeSeg* __restrict__ segs, int nsegs,
+__global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restrict__ segs, int nsegs,
                                                       uint64_t total_chunks, uint32_t chunk_lumels,
                                                       uint32_t refill_min, unsigned long long* queue,
                                                       unsigned long long* rays_out) {
-    __shared__ uint32_t stack[kLdsStack * kBlock];
-    __shared__ uint32_t ring_occ[kBlock / 64][kRing];
-    __shared__ uint32_t ring_done[kBlock / 64][kRing];
+    __shared__ uint32_t stack[kLdsStack * kAoBlock];
+    __shared__ uint32_t ring_occ[kAoBlock / 64][kRing];
+    __shared__ uint32_t ring_done[kAoBlock / 64][kRing];
     __shared__ u32x4 top_rows[kAoTopNodes * kNodeRows];
     const SceneDev top_scene = segs[0].scene;
     const uint32_t top_count = load_top_nodes(top_scene, top_rows, kAoTopNodes);
@@ -284,7 +286,7 @@ __global__ __launch_bounds__(kBlock) void k_ao_refill(const BakeSeg* __restrict_
             break;
         }
         if (active) {
-            const int st = any_hit_run<CAP>(scene, top, tr, node, sp, stk, kBlock, spill);
+            const int st = any_hit_run<CAP>(scene, top, tr, node, sp, stk, kAoBlock, spill);
             if (st != 0) {
                 rays++;
                 if (st > 0) atomicAdd(&ring_occ[wave][slot], 1u);
@@ -417,8 +419,11 @@ hipError_t launch_ao_refill(hipStream_t st, int cap, uint32_t max_blocks, const 
     if (chunk_lumels == 0 || refill_min == 0 || refill_min > 64) return hipErrorInvalidValue;
     hipError_t e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(queue), 2, 0u);
     if (e != hipSuccess) return e;
-    const uint32_t grid = grid_for_waves(total_chunks, max_blocks);
-    RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_ao_refill<CAP>, dim3(grid), dim3(kBlock), 0, st, segs, nsegs,
+    // max_blocks counts kBlock-thread workgroups; this kernel's are kAoBlock threads.
+    const uint64_t need = (total_chunks + (kAoBlock / 64) - 1) / (kAoBlock / 64);
+    const uint32_t cap_blocks = std::max<uint32_t>(1u, max_blocks * (uint32_t)kBlock / (uint32_t)kAoBlock);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(need, cap_blocks);
+    RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_ao_refill<CAP>, dim3(grid), dim3(kAoBlock), 0, st, segs, nsegs,
                                             total_chunks, chunk_lumels, refill_min, queue, rays));
     return hipGetLastError();
 }
