@@ -223,11 +223,20 @@ __global__ void k_photon_gather(const PhotonMapDev* __restrict__ pms, int npm, u
 // IndirectLight::bakeLumel: one wavefront per lumel (taken from the persistent queue), 64
 // samples per step; the per-sample terms are added in sample order, as the reference's loop does,
 // by broadcasting them lane by lane.
+// Workgroups of four wavefronts, each keeping the first 192 nodes of the first segment's scene in
+// LDS (21 KB; with the stacks a workgroup takes 37 KB, so four fit a compute unit's 160 KB).
+constexpr int kFgBlock = 256;
+constexpr uint32_t kFgTopNodes = 192;
+
 template <int CAP>
-__global__ __launch_bounds__(kBlock) void k_final_gather(const BakeSeg* __restrict__ segs, int nsegs,
-                                                         uint64_t total, unsigned long long* queue,
-                                                         unsigned long long* rays_out) {
-    __shared__ uint32_t stack[kLdsStack * kBlock];
+__global__ __launch_bounds__(kFgBlock) void k_final_gather(const BakeSeg* __restrict__ segs, int nsegs,
+                                                           uint64_t total, unsigned long long* queue,
+                                                           unsigned long long* rays_out) {
+    __shared__ uint32_t stack[kLdsStack * kFgBlock];
+    __shared__ u32x4 top_rows[kFgTopNodes * kNodeRows];
+    const SceneDev top_scene = segs[0].scene;
+    const uint32_t top_count = load_top_nodes(top_scene, top_rows, kFgTopNodes);
+    __syncthreads();
     uint32_t* stk = stack + threadIdx.x;
     const uint32_t lane = lane_id();
     uint32_t rays = 0;
@@ -237,6 +246,7 @@ __global__ __launch_bounds__(kBlock) void k_final_gather(const BakeSeg* __restri
     const BakeSeg& sg = segs[find_seg(segs, nsegs, g)];
     const uint32_t i = (uint32_t)(g - sg.work_begin);
     const SceneDev S = sg.scene;
+    const TopCache top{top_rows, S.nodes == top_scene.nodes ? top_count : 0u};
     const FinalGatherParams P = sg.fg;
     const int32_t* mesh_pm = sg.mesh_pm;
     const PhotonMapDev* pms = sg.pms;
@@ -256,7 +266,7 @@ __global__ __launch_bounds__(kBlock) void k_final_gather(const BakeSeg* __restri
             const Ray ray = ray_from_segment(p, p + dir * P.distance);
             TraceResult tr;
             rays++;
-            if (!traverse<false, CAP>(S, ray, stk, kBlock, tr)) {
+            if (!traverse<false, CAP>(S, top, ray, stk, kFgBlock, tr)) {
                 tr_ = P.sky[0] * influence + P.ambient[0];
                 tg_ = P.sky[1] * influence + P.ambient[1];
                 tb_ = P.sky[2] * influence + P.ambient[2];
@@ -546,9 +556,11 @@ hipError_t launch_final_gather(hipStream_t st, int cap, uint32_t max_blocks, con
     if (total_lumels == 0) return hipSuccess;
     hipError_t e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(queue), 2, 0u);
     if (e != hipSuccess) return e;
-    const uint64_t need = blocks_for(total_lumels, kBlock / 64);
-    const uint32_t grid = (uint32_t)(need < max_blocks ? need : max_blocks);
-    RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_final_gather<CAP>, dim3(grid), dim3(kBlock), 0, st, segs, nsegs,
+    // max_blocks counts kBlock-thread workgroups; this kernel's are kFgBlock threads.
+    const uint64_t need = blocks_for(total_lumels, kFgBlock / 64);
+    const uint64_t cap_blocks = std::max<uint64_t>(1, (uint64_t)max_blocks * kBlock / kFgBlock);
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(need, cap_blocks);
+    RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_final_gather<CAP>, dim3(grid), dim3(kFgBlock), 0, st, segs, nsegs,
                                             total_lumels, queue, rays));
     return hipGetLastError();
 }
