@@ -406,11 +406,17 @@ int32_t rl_server_destroy(rl_server* server);
  * grids (128-thread blocks). Results do not depend on the tuning. */
 #define RL_AO_KERNEL_CHUNKED 0
 #define RL_AO_KERNEL_REFILL 1
+/* Final-gather kernel: AUTO gives each lane its own lumel when a pass has at least 8 lumels per
+   resident lane, else each lumel a wavefront; WAVE and LANE force one of the two. */
+#define RL_FG_KERNEL_AUTO 0
+#define RL_FG_KERNEL_WAVE 1
+#define RL_FG_KERNEL_LANE 2
 typedef struct rl_tuning {
     int32_t ao_kernel;
     uint32_t refill_min;
     uint32_t ao_chunk_lumels;
     uint32_t blocks_per_cu;
+    int32_t fg_kernel;
 } rl_tuning;
 int32_t rl_set_tuning(rl_relight* ctx, const rl_tuning* tuning);
 int32_t rl_get_tuning(const rl_relight* ctx, rl_tuning* out);

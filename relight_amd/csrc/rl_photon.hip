@@ -220,85 +220,287 @@ __global__ void k_photon_gather(const PhotonMapDev* __restrict__ pms, int npm, u
     M.gathered[t] = out;
 }
 
-// IndirectLight::bakeLumel: one wavefront per lumel (taken from the persistent queue), 64
-// samples per step; the per-sample terms are added in sample order, as the reference's loop does,
-// by broadcasting them lane by lane.
-// Workgroups of four wavefronts, each keeping the first 192 nodes of the first segment's scene in
-// LDS (21 KB; with the stacks a workgroup takes 37 KB, so four fit a compute unit's 160 KB).
+// Final-gather workgroups: four wavefronts, each workgroup keeping the first nodes of the first
+// segment's scene in LDS (kFgTopNodes for the lane-per-lumel kernel: 21 KB, with the stacks 37 KB
+// per workgroup, so four fit a compute unit's 160 KB).
 constexpr int kFgBlock = 256;
 constexpr uint32_t kFgTopNodes = 192;
 
+// IndirectLight::bakeLumel with lane refill. A wavefront takes lumels from the persistent queue
+// and hands their (lumel, sample) pairs, in order, to idle lanes whenever at least `refill_min`
+// lanes are idle; pair number j of the wavefront's stream owns slot j mod kFgRing of the
+// wavefront's LDS ring. A lane traces its sample in rounds (closest_run), so finished lanes take
+// new samples while others still traverse; when its ray is done it shades the sample and writes
+// the sample's term into its slot. The wavefront adds finished terms to the current lumel's sum
+// strictly in stream order, that is in sample order as the reference's loop does, and writes the
+// lumel when its last sample has been added. A lane may only start pair j < consumed + kFgRing.
+constexpr uint32_t kFgRing = 128;
+// LDS per workgroup: 16 KB stacks + 10 KB rings + 13 KB of the scene's first 120 nodes, so four
+// workgroups (sixteen wavefronts) fit a compute unit's 160 KB.
+constexpr uint32_t kFgRefillTopNodes = 120;
+
 template <int CAP>
-__global__ __launch_bounds__(kFgBlock) void k_final_gather(const BakeSeg* __restrict__ segs, int nsegs,
-                                                           uint64_t total, unsigned long long* queue,
-                                                           unsigned long long* rays_out) {
+__global__ __launch_bounds__(kFgBlock) void k_fg_refill(const BakeSeg* __restrict__ segs, int nsegs, uint64_t total,
+                                                        uint32_t refill_min, unsigned long long* queue,
+                                                        unsigned long long* rays_out) {
+    __shared__ uint32_t stack[kLdsStack * kFgBlock];
+    __shared__ u32x4 top_rows[kFgRefillTopNodes * kNodeRows];
+    __shared__ float4 ring[kFgBlock / 64][kFgRing];          // x, y, z: term; w: flags (0 = pending)
+    __shared__ uint32_t ring_lumel[kFgBlock / 64][kFgRing];  // queue index of the slot's lumel
+    const SceneDev top_scene = segs[0].scene;
+    const uint32_t top_count = load_top_nodes(top_scene, top_rows, kFgRefillTopNodes);
+    uint32_t* stk = stack + threadIdx.x;
+    uint32_t spill[CAP - kLdsStack];
+    const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
+    for (uint32_t k = lane; k < kFgRing; k += 64) ring[wave][k] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    __syncthreads();
+    constexpr uint32_t kUsed = 1u, kUnused = 2u, kLast = 4u;
+
+    // Wavefront state, identical in every lane.
+    uint64_t take_g = 0;                 // lumel being handed out (queue index)
+    int take_seg = 0;
+    uint32_t take_s = 0, take_n = 0;     // its next sample and its sample count
+    uint32_t j_take = 0, j_cons = 0;     // stream pairs handed out / added
+    bool pool_open = true;
+    float ar = 0.0f, ag = 0.0f, ab = 0.0f;  // sum of the lumel being added
+    // Lane state.
+    bool active = false;
+    TravRay tr;
+    WtPerm wp;
+    SceneDev S;
+    TopCache top{top_rows, 0u};
+    TraceResult res;
+    uint32_t node = 0, my_j = 0, my_seg = 0, my_last = 0;
+    int sp = 0;
+    V3 dir = v3(0.0f, 0.0f, 0.0f);
+    float influence = 0.0f;
+    uint32_t rays = 0;
+    for (;;) {
+        const uint64_t idle = __ballot(!active);
+        if (pool_open && ((uint32_t)__popcll(idle) >= refill_min || idle == ~0ull)) {
+            uint64_t want = idle;
+            while (want && pool_open) {
+                if (take_s == take_n) {
+                    const uint64_t c = wave_next_chunk(queue);
+                    if (c >= total) {
+                        pool_open = false;
+                        break;
+                    }
+                    take_g = c;
+                    take_seg = find_seg(segs, nsegs, c);
+                    take_s = 0;
+                    take_n = (uint32_t)segs[take_seg].fg.samples;
+                }
+                const uint32_t room = j_cons + kFgRing - j_take;
+                const uint32_t take = min(min(room, take_n - take_s), (uint32_t)__popcll(want));
+                if (take == 0) break;
+                const uint32_t rank = rank_below(want);
+                const bool mine = ((want >> lane) & 1ull) && rank < take;
+                if (mine) {
+                    const BakeSeg& sg = segs[take_seg];
+                    const FinalGatherParams& P = sg.fg;
+                    const uint32_t i = (uint32_t)(take_g - sg.work_begin);
+                    const uint32_t s = take_s + rank;
+                    my_j = j_take + rank;
+                    my_seg = (uint32_t)take_seg;
+                    my_last = s == take_n - 1u ? kLast : 0u;
+                    ring_lumel[wave][my_j % kFgRing] = (uint32_t)take_g;
+                    const float4 p4 = sg.pos[i], n4 = sg.nrm[i];
+                    const V3 p = v3(p4.x, p4.y, p4.z), n = v3(n4.x, n4.y, n4.z);
+                    Pcg32 rng = sample_stream(P.seed, sg.lm_seed, f_bits(p4.w), TECH_FINAL_GATHER, s);
+                    dir = sample_hemisphere(rng, n, P.sampling);
+                    const float d0 = dot(n, dir);
+                    influence = d0 > 0.0f ? d0 : 0.0f;
+                    const Ray ray = ray_from_segment(p, p + dir * P.distance);
+                    tr = trav_ray(ray);
+                    wp = wt_perm(tr.w);
+                    S = sg.scene;
+                    top.count = S.nodes == top_scene.nodes ? top_count : 0u;
+                    node = 0;
+                    sp = 0;
+                    res.t = ray.tfar;
+                    res.tri = 0xFFFFFFFFu;
+                    res.U = res.V = res.W = res.det = 0.0f;
+                    active = true;
+                }
+                take_s += take;
+                j_take += take;
+                want &= ~__ballot(mine);
+            }
+        }
+        if (active && closest_run<CAP>(S, top, tr, wp, node, sp, stk, kFgBlock, spill, res) != 0) {
+            rays++;
+            const BakeSeg& sg = segs[my_seg];
+            const FinalGatherParams& P = sg.fg;
+            float xr = 0.0f, xg = 0.0f, xb = 0.0f;
+            uint32_t f = kUnused;
+            if (res.tri == 0xFFFFFFFFu) {
+                xr = P.sky[0] * influence + P.ambient[0];
+                xg = P.sky[1] * influence + P.ambient[1];
+                xb = P.sky[2] * influence + P.ambient[2];
+                f = kUsed;
+            } else {
+                const HitAttr h = hit_attributes(S, res);
+                if (dot(dir, h.n) < 0.0f) {
+                    const int32_t pm = sg.mesh_pm[h.mesh];
+                    if (pm >= 0) {
+                        const PhotonMapDev M = sg.pms[pm];
+                        const int x = lumel_coord(h.uvx, M.w), y = lumel_coord(h.uvy, M.h);
+                        const float4 gc = M.gathered[(uint32_t)y * (uint32_t)M.w + (uint32_t)x];
+                        xr = gc.x * influence + P.ambient[0];
+                        xg = gc.y * influence + P.ambient[1];
+                        xb = gc.z * influence + P.ambient[2];
+                        f = kUsed;
+                    }
+                }
+            }
+            ring[wave][my_j % kFgRing] = make_float4(xr, xg, xb, __uint_as_float(f | my_last));
+            active = false;
+        }
+        // Add finished terms in stream order; a slot read here is the same LDS word in every lane.
+        while (j_cons != j_take) {
+            const uint32_t slot = j_cons % kFgRing;
+            const float4 e = ring[wave][slot];
+            const uint32_t f = (uint32_t)wave_uniform(__float_as_uint(e.w));
+            if (f == 0u) break;
+            if (f & kUsed) {
+                ar += e.x;
+                ag += e.y;
+                ab += e.z;
+            }
+            if (f & kLast) {
+                const uint32_t g = ring_lumel[wave][slot];
+                if (lane == 0) {
+                    const BakeSeg& sg = segs[find_seg(segs, nsegs, g)];
+                    const uint32_t i = (uint32_t)(g - sg.work_begin);
+                    const float fs = (float)sg.fg.samples;
+                    float4 c = sg.color[i];
+                    c.x += ar / fs; c.y += ag / fs; c.z += ab / fs;
+                    sg.color[i] = c;
+                }
+                ar = ag = ab = 0.0f;
+            }
+            if (lane == 0) ring[wave][slot].w = 0.0f;
+            j_cons++;
+        }
+        if (!pool_open && __ballot(active) == 0ull && j_cons == j_take) break;
+    }
+    count_rays(rays, rays_out);
+}
+
+// IndirectLight::bakeLumel with one lumel per lane. Each lane takes a lumel from the persistent
+// queue and traces its samples one after another, adding each sample's term to the lumel's sum as
+// it finishes, so the sum runs in sample order as the reference's loop does, with no exchange
+// between lanes. A lane traces in rounds (closest_run); lanes whose ray is done wait until at least
+// `refill_min` lanes of the wavefront are without a ray, then start their next samples together
+// (lanes whose lumel is done first take new lumels), so the ray set-up runs for many lanes at once.
+// LDS per workgroup: 16 KB stacks + 21 KB of the scene's first 192 nodes.
+constexpr uint64_t kFgLanesMinPerLane = 8;  // lumels per resident lane for the lane-per-lumel kernel
+
+template <int CAP>
+__global__ __launch_bounds__(kFgBlock) void k_fg_lanes(const BakeSeg* __restrict__ segs, int nsegs, uint64_t total,
+                                                       uint32_t refill_min, unsigned long long* queue,
+                                                       unsigned long long* rays_out) {
     __shared__ uint32_t stack[kLdsStack * kFgBlock];
     __shared__ u32x4 top_rows[kFgTopNodes * kNodeRows];
     const SceneDev top_scene = segs[0].scene;
     const uint32_t top_count = load_top_nodes(top_scene, top_rows, kFgTopNodes);
     __syncthreads();
     uint32_t* stk = stack + threadIdx.x;
+    uint32_t spill[CAP - kLdsStack];
     const uint32_t lane = lane_id();
+    bool pool_open = true;  // identical in every lane
+    // Lane state: the lumel (queue index g, its segment, next sample, sample count, sums) and the
+    // sample's ray in flight.
+    bool has_lumel = false, ray_on = false;
+    uint32_t g = 0, seg = 0, s = 0, nsamples = 0;
+    float ar = 0.0f, ag = 0.0f, ab = 0.0f;
+    TravRay tr;
+    WtPerm wp;
+    SceneDev S;
+    TopCache top{top_rows, 0u};
+    TraceResult res;
+    uint32_t node = 0;
+    int sp = 0;
+    V3 dir = v3(0.0f, 0.0f, 0.0f);
+    float influence = 0.0f;
     uint32_t rays = 0;
     for (;;) {
-    const uint64_t g = wave_next_chunk(queue);  // lumel of this wavefront
-    if (g >= total) break;
-    const BakeSeg& sg = segs[find_seg(segs, nsegs, g)];
-    const uint32_t i = (uint32_t)(g - sg.work_begin);
-    const SceneDev S = sg.scene;
-    const TopCache top{top_rows, S.nodes == top_scene.nodes ? top_count : 0u};
-    const FinalGatherParams P = sg.fg;
-    const int32_t* mesh_pm = sg.mesh_pm;
-    const PhotonMapDev* pms = sg.pms;
-    const float4 p4 = sg.pos[i], n4 = sg.nrm[i];
-    const V3 p = v3(p4.x, p4.y, p4.z), n = v3(n4.x, n4.y, n4.z);
-    const uint32_t texel = f_bits(p4.w);
-    float ar = 0.0f, ag = 0.0f, ab = 0.0f;
-    for (int base = 0; base < P.samples; base += 64) {
-        const int s = base + (int)lane;
-        float tr_ = 0.0f, tg_ = 0.0f, tb_ = 0.0f;
-        int use = 0;
-        if (s < P.samples) {
-            Pcg32 rng = sample_stream(P.seed, sg.lm_seed, texel, TECH_FINAL_GATHER, (uint32_t)s);
-            const V3 dir = sample_hemisphere(rng, n, P.sampling);
-            const float d0 = dot(n, dir);
-            const float influence = d0 > 0.0f ? d0 : 0.0f;
-            const Ray ray = ray_from_segment(p, p + dir * P.distance);
-            TraceResult tr;
-            rays++;
-            if (!traverse<false, CAP>(S, top, ray, stk, kFgBlock, tr)) {
-                tr_ = P.sky[0] * influence + P.ambient[0];
-                tg_ = P.sky[1] * influence + P.ambient[1];
-                tb_ = P.sky[2] * influence + P.ambient[2];
-                use = 1;
-            } else {
-                const HitAttr h = hit_attributes(S, tr);
-                if (dot(dir, h.n) < 0.0f) {
-                    const int32_t pm = mesh_pm[h.mesh];
-                    if (pm >= 0) {
-                        const PhotonMapDev M = pms[pm];
-                        const int x = lumel_coord(h.uvx, M.w), y = lumel_coord(h.uvy, M.h);
-                        const float4 gc = M.gathered[(uint32_t)y * (uint32_t)M.w + (uint32_t)x];
-                        tr_ = gc.x * influence + P.ambient[0];
-                        tg_ = gc.y * influence + P.ambient[1];
-                        tb_ = gc.z * influence + P.ambient[2];
-                        use = 1;
+        const uint64_t idle = __ballot(!ray_on);
+        if ((uint32_t)__popcll(idle) >= refill_min || idle == ~0ull) {
+            const uint64_t fresh = __ballot(!ray_on && !has_lumel);
+            if (fresh && pool_open) {
+                const uint32_t n = (uint32_t)__popcll(fresh);
+                const uint64_t base = wave_take(queue, n);
+                if (base + n >= total) pool_open = false;
+                if (!ray_on && !has_lumel) {
+                    const uint64_t mine = base + rank_below(fresh);
+                    if (mine < total) {
+                        has_lumel = true;
+                        g = (uint32_t)mine;
+                        seg = (uint32_t)find_seg(segs, nsegs, mine);
+                        s = 0;
+                        nsamples = (uint32_t)segs[seg].fg.samples;
+                        ar = ag = ab = 0.0f;
                     }
                 }
             }
+            if (!ray_on && has_lumel) {
+                const BakeSeg& sg = segs[seg];
+                const FinalGatherParams& P = sg.fg;
+                const uint32_t i = g - (uint32_t)sg.work_begin;
+                const float4 p4 = sg.pos[i], n4 = sg.nrm[i];
+                const V3 p = v3(p4.x, p4.y, p4.z), n = v3(n4.x, n4.y, n4.z);
+                Pcg32 rng = sample_stream(P.seed, sg.lm_seed, f_bits(p4.w), TECH_FINAL_GATHER, s);
+                dir = sample_hemisphere(rng, n, P.sampling);
+                const float d0 = dot(n, dir);
+                influence = d0 > 0.0f ? d0 : 0.0f;
+                const Ray ray = ray_from_segment(p, p + dir * P.distance);
+                tr = trav_ray(ray);
+                wp = wt_perm(tr.w);
+                S = sg.scene;
+                top.count = S.nodes == top_scene.nodes ? top_count : 0u;
+                node = 0;
+                sp = 0;
+                res.t = ray.tfar;
+                res.tri = 0xFFFFFFFFu;
+                res.U = res.V = res.W = res.det = 0.0f;
+                ray_on = true;
+            }
         }
-        const int nb = min(64, P.samples - base);
-        for (int k = 0; k < nb; k++) {
-            const float xr = __shfl(tr_, k, 64), xg = __shfl(tg_, k, 64), xb = __shfl(tb_, k, 64);
-            if (__shfl(use, k, 64)) { ar += xr; ag += xg; ab += xb; }
+        if (ray_on && closest_run<CAP>(S, top, tr, wp, node, sp, stk, kFgBlock, spill, res) != 0) {
+            rays++;
+            ray_on = false;
+            const BakeSeg& sg = segs[seg];
+            const FinalGatherParams& P = sg.fg;
+            if (res.tri == 0xFFFFFFFFu) {
+                ar += P.sky[0] * influence + P.ambient[0];
+                ag += P.sky[1] * influence + P.ambient[1];
+                ab += P.sky[2] * influence + P.ambient[2];
+            } else {
+                const HitAttr h = hit_attributes(S, res);
+                if (dot(dir, h.n) < 0.0f) {
+                    const int32_t pm = sg.mesh_pm[h.mesh];
+                    if (pm >= 0) {
+                        const PhotonMapDev M = sg.pms[pm];
+                        const int x = lumel_coord(h.uvx, M.w), y = lumel_coord(h.uvy, M.h);
+                        const float4 gc = M.gathered[(uint32_t)y * (uint32_t)M.w + (uint32_t)x];
+                        ar += gc.x * influence + P.ambient[0];
+                        ag += gc.y * influence + P.ambient[1];
+                        ab += gc.z * influence + P.ambient[2];
+                    }
+                }
+            }
+            if (++s == nsamples) {
+                const uint32_t i = g - (uint32_t)sg.work_begin;
+                const float fs = (float)nsamples;
+                float4 c = sg.color[i];
+                c.x += ar / fs; c.y += ag / fs; c.z += ab / fs;
+                sg.color[i] = c;
+                has_lumel = false;
+            }
         }
-    }
-    if (lane == 0) {
-        float4 c = sg.color[i];
-        const float fs = (float)P.samples;
-        c.x += ar / fs; c.y += ag / fs; c.z += ab / fs;
-        sg.color[i] = c;
-    }
+        if (!pool_open && __ballot(ray_on || has_lumel) == 0ull) break;
     }
     count_rays(rays, rays_out);
 }
@@ -552,16 +754,27 @@ hipError_t launch_knn_gather(hipStream_t st, uint32_t max_blocks, const float4* 
 }
 
 hipError_t launch_final_gather(hipStream_t st, int cap, uint32_t max_blocks, const BakeSeg* segs, int nsegs,
-                               uint64_t total_lumels, unsigned long long* queue, unsigned long long* rays) {
+                               uint64_t total_lumels, int kernel, uint32_t refill_min, unsigned long long* queue,
+                               unsigned long long* rays) {
     if (total_lumels == 0) return hipSuccess;
+    if (refill_min == 0 || refill_min > 64) return hipErrorInvalidValue;
     hipError_t e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(queue), 2, 0u);
     if (e != hipSuccess) return e;
-    // max_blocks counts kBlock-thread workgroups; this kernel's are kFgBlock threads.
-    const uint64_t need = blocks_for(total_lumels, kFgBlock / 64);
+    // max_blocks counts kBlock-thread workgroups; these kernels' are kFgBlock threads.
     const uint64_t cap_blocks = std::max<uint64_t>(1, (uint64_t)max_blocks * kBlock / kFgBlock);
-    const uint32_t grid = (uint32_t)std::min<uint64_t>(need, cap_blocks);
-    RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_final_gather<CAP>, dim3(grid), dim3(kFgBlock), 0, st, segs, nsegs,
-                                            total_lumels, queue, rays));
+    // One lumel per lane needs many lumels per resident lane to keep lanes busy to the end; smaller
+    // passes give each lumel a wavefront.
+    const bool lanes = kernel == kFgKernelLane ||
+                       (kernel == kFgKernelAuto && total_lumels >= kFgLanesMinPerLane * cap_blocks * kFgBlock);
+    if (lanes) {
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(blocks_for(total_lumels, kFgBlock), cap_blocks);
+        RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_fg_lanes<CAP>, dim3(grid), dim3(kFgBlock), 0, st, segs, nsegs,
+                                                total_lumels, refill_min, queue, rays));
+    } else {
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(blocks_for(total_lumels, kFgBlock / 64), cap_blocks);
+        RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_fg_refill<CAP>, dim3(grid), dim3(kFgBlock), 0, st, segs, nsegs,
+                                                total_lumels, refill_min, queue, rays));
+    }
     return hipGetLastError();
 }
 

@@ -99,7 +99,7 @@ struct rl_relight {
     unsigned long long* d_queue = nullptr;  // persistent-kernel work counter
     uint32_t max_blocks = 4096;             // persistent grid: blocks_per_cu blocks per compute unit
     int cus = 256;
-    rl_tuning tuning{RL_AO_KERNEL_REFILL, 16u, 8u, 16u};
+    rl_tuning tuning{RL_AO_KERNEL_REFILL, 16u, 8u, 16u, RL_FG_KERNEL_AUTO};
     rl_stats stats{};
     uint32_t next_lm_seed = 0;
     // Event pairs around each kernel launch of the current call: (slot, start, end).
@@ -545,8 +545,8 @@ int32_t run_fg_items(rl_relight* ctx, const std::vector<BakeItem>& items) {
     if (ds.total) {
         e = kernel_begin(ctx, RL_KERNEL_FINAL_GATHER);
         if (e == hipSuccess)
-            e = launch_final_gather(ctx->stream, ds.cap, ctx->max_blocks, ds.d, ds.n, ds.total, ctx->d_queue,
-                                    ctx->d_rays);
+            e = launch_final_gather(ctx->stream, ds.cap, ctx->max_blocks, ds.d, ds.n, ds.total,
+                                    ctx->tuning.fg_kernel, ctx->tuning.refill_min, ctx->d_queue, ctx->d_rays);
         if (e == hipSuccess) e = kernel_end(ctx);
     }
     ctx->stats.texels += ds.texels;
@@ -1680,6 +1680,10 @@ int32_t rl_radiosity_solve(rl_relight* ctx, rl_radiosity* r, const float* emissi
     return RL_SUCCESS;
 }
 
+static_assert(RL_FG_KERNEL_AUTO == rl::kFgKernelAuto && RL_FG_KERNEL_WAVE == rl::kFgKernelWave &&
+                  RL_FG_KERNEL_LANE == rl::kFgKernelLane,
+              "final-gather kernel ids");
+
 int32_t rl_set_tuning(rl_relight* ctx, const rl_tuning* t) {
     if (!ctx || !t) return fail(RL_INVALID_ARGUMENT, "NULL argument");
     if (t->ao_kernel != RL_AO_KERNEL_CHUNKED && t->ao_kernel != RL_AO_KERNEL_REFILL)
@@ -1687,6 +1691,8 @@ int32_t rl_set_tuning(rl_relight* ctx, const rl_tuning* t) {
     if (t->refill_min < 1 || t->refill_min > 64) return fail(RL_INVALID_ARGUMENT, "refill_min must be in [1, 64]");
     if (t->ao_chunk_lumels < 1 || t->ao_chunk_lumels > 4096) return fail(RL_INVALID_ARGUMENT, "bad chunk size");
     if (t->blocks_per_cu < 1 || t->blocks_per_cu > 64) return fail(RL_INVALID_ARGUMENT, "blocks_per_cu must be in [1, 64]");
+    if (t->fg_kernel != RL_FG_KERNEL_AUTO && t->fg_kernel != RL_FG_KERNEL_WAVE && t->fg_kernel != RL_FG_KERNEL_LANE)
+        return fail(RL_INVALID_ARGUMENT, "unknown final-gather kernel");
     ctx->tuning = *t;
     ctx->max_blocks = (uint32_t)ctx->cus * t->blocks_per_cu;
     return RL_SUCCESS;

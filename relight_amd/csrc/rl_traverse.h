@@ -178,7 +178,7 @@ __device__ __forceinline__ bool visit_node(const SceneDev& S, const TopCache& to
     f32x4 ex, fx, ey, fy, ez, fz;
     u32x4 ch;
     if (node < top.count) {
-        const u32x4* nd = top.rows + kNodeRows * node;
+        const u32x4* nd = top.rows + __umul24(node, kNodeRows);  // node < top.count < 2^24: full-rate multiply
         ex = __builtin_bit_cast(f32x4, nd[nx]);
         fx = __builtin_bit_cast(f32x4, nd[1u - nx]);
         ey = __builtin_bit_cast(f32x4, nd[2u + ny]);
@@ -264,6 +264,44 @@ __device__ __forceinline__ int any_hit_run(const SceneDev& S, const TopCache& to
         if (wt_intersect(r.w, v3(tf[0], tf[1], tf[2]), v3(tf[4], tf[5], tf[6]), v3(tf[8], tf[9], tf[10]), t, U, V, W,
                          det))
             return 1;
+        if (++cur == end) {
+            if (sp == 0) return -1;
+            node = stack_pop(sp, stk, stride, spill);
+            if (!(node & RL_LEAF_BIT)) return 0;
+            cur = node & RL_LEAF_FIRST_MASK;
+            end = cur + ((node >> 27) & 15u) + 1u;
+        }
+    }
+}
+
+// One round of closest-hit traversal in the two phases of traverse(): internal nodes until this
+// lane holds a leaf, then the triangles of that leaf and of any leaves popped right after it.
+// Returns 0 when the round ends at an internal node (call again), -1 when traversal is finished:
+// res then holds the closest hit (tri = 0xFFFFFFFF for a miss). Start with node = 0, sp = 0 and
+// res = {r.w.tfar, 0xFFFFFFFF, 0, 0, 0, 0}. A lane tests the nodes and triangles traverse<false>()
+// tests, in the same order and against the same shrinking interval (res.t), so the result equals
+// traverse<false>()'s; rounds only let a caller give finished lanes new rays in between.
+template <int CAP>
+__device__ __forceinline__ int closest_run(const SceneDev& S, const TopCache& top, const TravRay& r, const WtPerm& p,
+                                           uint32_t& node, int& sp, uint32_t* stk, uint32_t stride, uint32_t* spill,
+                                           TraceResult& res) {
+    while (!(node & RL_LEAF_BIT))
+        if (!visit_node<CAP>(S, top, r.nx, r.ny, r.nz, r.ix, r.iy, r.iz, r.oix, r.oiy, r.oiz, r.w.tnear,
+                             res.t * kBoxScale, node, sp, stk, stride, spill))
+            return -1;
+    uint32_t cur = node & RL_LEAF_FIRST_MASK;
+    uint32_t end = cur + ((node >> 27) & 15u) + 1u;
+    for (;;) {
+        const gfloat4* tp = gmem(S.tris) + 3u * cur;
+        float t, U, V, W, det;
+        if (wt_intersect_at(r.w, p, tp, t, U, V, W, det)) {
+            const uint32_t id = f_bits(reinterpret_cast<const gfloat*>(tp)[3]);
+            if (t < res.t || (t == res.t && id < res.tri)) {
+                res.t = t;
+                res.tri = id;
+                res.U = U; res.V = V; res.W = W; res.det = det;
+            }
+        }
         if (++cur == end) {
             if (sp == 0) return -1;
             node = stack_pop(sp, stk, stride, spill);
@@ -372,16 +410,20 @@ __device__ __forceinline__ uint64_t wave_uniform(uint64_t v) {
     return ((uint64_t)hi << 32) | lo;
 }
 
-// Persistent work queue: a wavefront takes the next chunk index and every lane gets it. A
-// wavefront keeps taking chunks until the index passes the chunk count, so every wavefront reaches
-// its exit and the grid drains. Every lane executes the atomic (lane 0 adds 1, the others 0) and
+// Persistent work queue: a wavefront takes the next n items (wave_take returns the first index,
+// wave_next_chunk takes one) and every lane gets the index. A wavefront keeps taking until the
+// index passes the item count, so every wavefront reaches its exit and the grid drains. Every
+// lane executes the atomic (lane 0 adds n, the others 0) and
 // the index is returned wave-uniform: with a lane-0-only atomic at the loop head and a lane-0-only
 // store at the loop end, the compiler may rotate the loop so that the two regions merge and the
 // other lanes re-read the index from lane 0 after it has left their loop, never reaching the exit.
-__device__ __forceinline__ uint64_t wave_next_chunk(unsigned long long* counter) {
-    const unsigned long long c = atomicAdd(counter, lane_id() == 0 ? 1ull : 0ull);
+__device__ __forceinline__ uint64_t wave_take(unsigned long long* counter, uint32_t n) {
+    const unsigned long long c = atomicAdd(counter, lane_id() == 0 ? (unsigned long long)n : 0ull);
     return wave_uniform(__shfl(c, 0, 64));
 }
+
+// wave_take of one item (the common case: one chunk per wavefront).
+__device__ __forceinline__ uint64_t wave_next_chunk(unsigned long long* counter) { return wave_take(counter, 1u); }
 
 __device__ __forceinline__ void count_rays(uint32_t n, unsigned long long* counter) {
     // Wave sum of per-lane ray counts, one atomic per wavefront.

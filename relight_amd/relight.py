@@ -464,8 +464,8 @@ class Relight:
         return {name: getattr(t, name) for name, _ in A.rl_tuning._fields_}
 
     def set_tuning(self, **kw):
-        """Launch tuning (ao_kernel, refill_min, ao_chunk_lumels, blocks_per_cu); results do not
-        depend on it."""
+        """Launch tuning (ao_kernel, refill_min, ao_chunk_lumels, blocks_per_cu, fg_kernel); results
+        do not depend on it."""
         cur = self.tuning()
         cur.update(kw)
         t = A.rl_tuning(*[int(cur[name]) for name, _ in A.rl_tuning._fields_])

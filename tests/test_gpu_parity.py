@@ -257,6 +257,37 @@ def test_heightfield_small_photons_knn_final_gather(relight):
     _gi_compare(relight, spec)
 
 
+@pytest.mark.parametrize("fg_kernel", ["wave", "lane"])
+def test_final_gather_kernels_bit_exact(relight, fg_kernel):
+    """Both final-gather kernels forced on small scenes (a wavefront per lumel summing through an
+    in-order LDS ring; a lane per lumel summing in registers), against the oracle bit for bit:
+    the city with 40 cosine samples per texel (lumels end inside a wavefront's sample batch) and
+    the small heightfield with the k-NN gather."""
+    from relight_amd._capi import RL_FG_KERNEL_LANE, RL_FG_KERNEL_WAVE
+    from relight_amd import RL_PHOTON_RUSSIAN_ROULETTE, RL_SAMPLE_COSINE, IndirectLightSettings
+    print(f"[gpu] final gather kernel {fg_kernel}: city (40 cosine samples) + small heightfield", flush=True)
+    saved = relight.tuning()
+    relight.set_tuning(fg_kernel=RL_FG_KERNEL_WAVE if fg_kernel == "wave" else RL_FG_KERNEL_LANE, refill_min=8)
+    try:
+        spec = scenes.city(num_boxes=120, tiles=(2, 1), side_range=(40, 56))
+        rng = np.random.default_rng(11)
+        for i, m in enumerate(spec.meshes):
+            m.albedo = np.array([*rng.uniform(0.3, 0.8, 3), 1.0], np.float32)
+            m.photonmap = i
+        spec.photonmaps = [(32, 32) for _ in spec.meshes]
+        spec.indirect = IndirectLightSettings(
+            photon_pass_count=1, photon_bounce_count=3, photon_energy_threshold=0.0,
+            photon_max_distance=300.0, final_gather_samples=40, final_gather_distance=60.0,
+            final_gather_radius=2, sky_color=(0.3, 0.4, 0.5), ambient_color=(0.01, 0.0, 0.02), seed=13,
+            sampling=RL_SAMPLE_COSINE, photons_per_light=12000, termination=RL_PHOTON_RUSSIAN_ROULETTE)
+        _gi_compare(relight, spec)
+        _gi_compare(relight, scenes.heightfield(cells=64, tile_cells=32, num_boxes=40, map_size=256,
+                                                photonmap_size=32, photons_per_light=20000, fg_samples=64,
+                                                box_meshes=2))
+    finally:
+        relight.set_tuning(**saved)
+
+
 def test_axis_parallel_segments_agree(relight):
     """Segments with one or two direction components exactly zero (the slab test's 1/0 case),
     from origins on box faces and inside the city, against the oracle."""
