@@ -86,11 +86,9 @@ hipError_t launch_knn_gather(hipStream_t st, uint32_t max_blocks, const float4* 
 // Final gather over lumel segments; each segment carries its scene, settings and photon tables.
 // Final-gather kernel choice, the values of RL_FG_KERNEL_AUTO / _WAVE / _LANE.
 constexpr int kFgKernelAuto = 0, kFgKernelWave = 1, kFgKernelLane = 2;
-// kernel: kFgKernel*; refill_min: idle lanes a wavefront waits for before starting more
-// samples (1..64).
+// kernel: kFgKernel*.
 hipError_t launch_final_gather(hipStream_t st, int cap, uint32_t max_blocks, const BakeSeg* segs, int nsegs,
-                               uint64_t total_lumels, int kernel, uint32_t refill_min, unsigned long long* queue,
-                               unsigned long long* rays);
+                               uint64_t total_lumels, int kernel, unsigned long long* queue, unsigned long long* rays);
 
 // Texel raster of one mesh (rl_raster.hip): records of the texels it claims (last face wins).
 hipError_t launch_raster(hipStream_t st, const RasterMesh& M, uint32_t max_blocks, uint32_t* owner, RasterRec* out,

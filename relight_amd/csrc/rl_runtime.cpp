@@ -546,7 +546,7 @@ int32_t run_fg_items(rl_relight* ctx, const std::vector<BakeItem>& items) {
         e = kernel_begin(ctx, RL_KERNEL_FINAL_GATHER);
         if (e == hipSuccess)
             e = launch_final_gather(ctx->stream, ds.cap, ctx->max_blocks, ds.d, ds.n, ds.total,
-                                    ctx->tuning.fg_kernel, ctx->tuning.refill_min, ctx->d_queue, ctx->d_rays);
+                                    ctx->tuning.fg_kernel, ctx->d_queue, ctx->d_rays);
         if (e == hipSuccess) e = kernel_end(ctx);
     }
     ctx->stats.texels += ds.texels;
