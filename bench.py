@@ -85,9 +85,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--ao-samples", type=int, default=256)
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--workloads", default="gi,radiosity,serving",
+    ap.add_argument("--workloads", default="gi,radiosity,serving,serving_small",
                     help="secondary legs reported under 'workloads' (comma list of gi, radiosity, "
-                         "serving; empty for none)")
+                         "serving, serving_small; empty for none)")
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device-to-device with a plain copy kernel (torch) and exit; "
@@ -232,6 +232,8 @@ def main():
             legs["radiosity"] = W.bench_radiosity(relight)
         elif name == "serving":
             legs["serving"] = W.bench_serving(relight, n_requests=args.serve_requests)
+        elif name == "serving_small":
+            legs["serving_small"] = W.bench_serving_small(relight)
         else:
             raise SystemExit(f"unknown workload {name}")
         log(f"{name} leg done in {time.time()-t0:.1f} s")

@@ -9,6 +9,9 @@
              triangles; lightmaps 128^2-2048^2; 50% direct+AO, 30% final gather, 20% radiosity):
              saturated throughput and Poisson-arrival latency, served (continuous batching) vs
              plain (one request at a time through the per-call entry points).
+  serving_small  the same comparison for a stream of small requests: 256 direct+AO requests
+             (64 AO rays) on 128^2-256^2 lightmaps of one resident city, where one request does not
+             fill the GPU and the per-call overhead of the plain path dominates.
 """
 import sys
 import threading
@@ -296,4 +299,50 @@ def bench_serving(relight, n_requests=64, seed=5, ao_samples=64, fg_samples=64, 
             "served_makespan_ms": poisson_served_ms, "plain_makespan_ms": poisson_plain_ms,
         },
         "rays": plain_rays,
+    }
+
+
+# ---------------------------------------------------------------------------------- serving_small
+
+def bench_serving_small(relight, n_requests=256, ao_samples=64, quantum_rays=1 << 27, max_active=256):
+    from relight_amd import AmbientOcclusionSettings, RL_SAMPLE_COSINE, scenes
+    t0 = time.time()
+    spec = scenes.city(seed=31, num_boxes=2500, tiles=(16, 16), side_range=(128, 256), ao_samples=ao_samples)
+    scene, lms, ids = scenes.build_gpu(relight, spec)
+    relight.bake_items([(scene, lm, 0, 0, False, None) for lm in lms])  # resident lumels
+    ao = AmbientOcclusionSettings(ao_samples, 0.8, 4.0, 1.0, 1, RL_SAMPLE_COSINE)
+    n = min(n_requests, len(lms))
+    texels = [lms[i].valid_count() for i in range(n)]
+    log(f"serving_small: {n} direct+AO requests, {sum(texels)} texels, setup {time.time()-t0:.1f} s")
+    for i in range(2):  # warm both paths
+        relight.bake_direct_light(scene, ids[i])
+        relight.bake_ambient_occlusion(scene, ids[i], ao)
+    t = time.perf_counter()
+    plain_done, plain_rays = [], 0
+    for i in range(n):
+        relight.bake_direct_light(scene, ids[i])
+        plain_rays += relight.stats()["rays"]
+        relight.bake_ambient_occlusion(scene, ids[i], ao)
+        plain_rays += relight.stats()["rays"]
+        plain_done.append((time.perf_counter() - t) * 1e3)
+    plain_ms = (time.perf_counter() - t) * 1e3
+    with relight.create_server(quantum_rays=quantum_rays, max_active=max_active) as srv:
+        t = time.perf_counter()
+        jobs = [srv.submit_direct_ao(scene, [lms[i]], True, ao) for i in range(n)]
+        sts = [j.wait(timeout=600) for j in jobs]
+        served_ms = (time.perf_counter() - t) * 1e3
+        sstats = srv.stats()
+    assert all(s_["state"] == "done" for s_ in sts)
+    served_lat = [s_["done_ms"] - s_["submit_ms"] for s_ in sts]
+    log(f"serving_small: saturated plain {plain_ms:.1f} ms, served {served_ms:.1f} ms "
+        f"({plain_ms / served_ms:.2f}x), steps {sstats['steps']}")
+    return {
+        "workload": f"{n} direct+AO requests ({ao_samples} cosine AO rays + sun and 16-vertex area light) on "
+                    f"128^2-256^2 lightmaps of one resident city (2,500 boxes, mean {np.mean(texels):.0f} texels)",
+        "plain_ms": plain_ms, "served_ms": served_ms, "served_vs_plain": plain_ms / served_ms,
+        "plain_mrays_per_s": plain_rays / (plain_ms / 1e3) / 1e6,
+        "served_mrays_per_s": plain_rays / (served_ms / 1e3) / 1e6,
+        "served_latency_ms": {"p50": percentile(served_lat, 50), "p99": percentile(served_lat, 99)},
+        "plain_latency_ms": {"p50": percentile(plain_done, 50), "p99": percentile(plain_done, 99)},
+        "server_steps": sstats["steps"], "rays": plain_rays,
     }
