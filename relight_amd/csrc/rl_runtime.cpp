@@ -109,8 +109,15 @@ struct rl_relight {
     std::vector<Mark> marks;
     // Segment tables of the launches in flight (at most two at once), grown as needed and reused:
     // every bake helper synchronises its stream before returning.
-    BakeSeg* seg_buf[2] = {nullptr, nullptr};
-    size_t seg_cap[2] = {0, 0};
+    // Segment tables, one slot per phase of a pass (kSegDirect..kSegGather), uploaded on the stream
+    // from pinned staging; seg_ev[k] marks when the staging of slot k may be rewritten.
+    static constexpr int kSegSlots = 4;
+    BakeSeg* seg_buf[kSegSlots] = {};
+    size_t seg_cap[kSegSlots] = {};
+    BakeSeg* seg_host[kSegSlots] = {};
+    size_t seg_host_cap[kSegSlots] = {};
+    hipEvent_t seg_ev[kSegSlots] = {};
+    bool seg_ev_live[kSegSlots] = {};
     // Texel-raster scratch: per-texel owner and claimed-texel records (grown to the largest map).
     uint32_t* raster_owner = nullptr;
     RasterRec* raster_rec = nullptr;
@@ -374,6 +381,8 @@ struct BakeItem {
 };
 
 enum SegUnit { kUnitLumel = 0, kUnitPair = 1, kUnitChunk = 2 };
+// Segment-table slots: one per phase, so the phases of a pass can be queued back to back.
+enum SegSlot { kSegDirect = 0, kSegAoWork = 1, kSegAoLumel = 2, kSegGather = 3 };
 
 // A device segment table for one launch (in one of the context's reusable buffers).
 struct DeviceSegs {
@@ -451,6 +460,7 @@ int32_t upload_segments(rl_relight* ctx, int slot, const std::vector<BakeItem>& 
     out.n = (int)segs.size();
     if (segs.empty()) return RL_SUCCESS;
     if (ctx->seg_cap[slot] < segs.size()) {
+        RL_HIP(hipStreamSynchronize(ctx->stream));  // queued kernels may still read the old table
         dev_free(ctx->seg_buf[slot]);
         ctx->seg_cap[slot] = 0;
         const size_t cap = std::max<size_t>(segs.size(), 256);
@@ -458,14 +468,31 @@ int32_t upload_segments(rl_relight* ctx, int slot, const std::vector<BakeItem>& 
         ctx->seg_cap[slot] = cap;
     }
     out.d = ctx->seg_buf[slot];
-    RL_HIP(hipMemcpy(out.d, segs.data(), segs.size() * sizeof(BakeSeg), hipMemcpyHostToDevice));
+    // The copy runs on the stream, so it lands after the kernels queued before it (which may still
+    // read the slot's previous table) and before the kernels queued after it.
+    if (ctx->seg_ev_live[slot]) RL_HIP(hipEventSynchronize(ctx->seg_ev[slot]));
+    if (ctx->seg_host_cap[slot] < segs.size()) {
+        if (ctx->seg_host[slot]) RL_HIP(hipHostFree(ctx->seg_host[slot]));
+        ctx->seg_host[slot] = nullptr;
+        ctx->seg_host_cap[slot] = 0;
+        const size_t cap = std::max<size_t>(segs.size(), 256);
+        RL_HIP(hipHostMalloc(reinterpret_cast<void**>(&ctx->seg_host[slot]), cap * sizeof(BakeSeg), hipHostMallocDefault));
+        ctx->seg_host_cap[slot] = cap;
+    }
+    if (!ctx->seg_ev[slot]) RL_HIP(hipEventCreateWithFlags(&ctx->seg_ev[slot], hipEventDisableTiming));
+    std::memcpy(ctx->seg_host[slot], segs.data(), segs.size() * sizeof(BakeSeg));
+    RL_HIP(hipMemcpyAsync(out.d, ctx->seg_host[slot], segs.size() * sizeof(BakeSeg), hipMemcpyHostToDevice,
+                          ctx->stream));
+    RL_HIP(hipEventRecord(ctx->seg_ev[slot], ctx->stream));
+    ctx->seg_ev_live[slot] = true;
     return RL_SUCCESS;
 }
 
-// Direct light over items (one launch); the caller synchronises.
-int32_t run_direct_items(rl_relight* ctx, const std::vector<BakeItem>& items) {
+// Direct light over items (one launch). sync = false leaves the launch queued on the stream (a
+// later synchronisation, e.g. end_call's, reports its errors).
+int32_t run_direct_items(rl_relight* ctx, const std::vector<BakeItem>& items, bool sync = true) {
     DeviceSegs ds;
-    int32_t st = upload_segments(ctx, 0, items, kUnitLumel, 1, ds);
+    int32_t st = upload_segments(ctx, kSegDirect, items, kUnitLumel, 1, ds);
     if (st) return st;
     hipError_t e = hipSuccess;
     if (ds.total) {
@@ -476,7 +503,7 @@ int32_t run_direct_items(rl_relight* ctx, const std::vector<BakeItem>& items) {
     }
     ctx->stats.texels += ds.texels;
     ctx->stats.launches += ds.total ? 1 : 0;
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess && sync) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return fail(RL_DEVICE_ERROR, std::string("direct bake: ") + hipGetErrorString(e));
     for (const BakeItem& it : items) it.lm->device_newer = true;
     return RL_SUCCESS;
@@ -484,14 +511,14 @@ int32_t run_direct_items(rl_relight* ctx, const std::vector<BakeItem>& items) {
 
 // AO over items: count pass (lane refill when every item has >= 4 samples, else chunked) and
 // the finalize pass.
-int32_t run_ao_items(rl_relight* ctx, const std::vector<BakeItem>& items) {
+int32_t run_ao_items(rl_relight* ctx, const std::vector<BakeItem>& items, bool sync = true) {
     bool refill = ctx->tuning.ao_kernel == RL_AO_KERNEL_REFILL;
     for (const BakeItem& it : items) refill = refill && it.ao.samples >= 4;
     const uint32_t chunk = ctx->tuning.ao_chunk_lumels;
     DeviceSegs work, lum;
-    int32_t st = upload_segments(ctx, 0, items, refill ? kUnitChunk : kUnitPair, chunk, work);
+    int32_t st = upload_segments(ctx, kSegAoWork, items, refill ? kUnitChunk : kUnitPair, chunk, work);
     if (st) return st;
-    if ((st = upload_segments(ctx, 1, items, kUnitLumel, 1, lum))) return st;
+    if ((st = upload_segments(ctx, kSegAoLumel, items, kUnitLumel, 1, lum))) return st;
     hipError_t e = hipSuccess;
     if (work.total) {
         e = kernel_begin(ctx, RL_KERNEL_AO_COUNT);
@@ -513,33 +540,33 @@ int32_t run_ao_items(rl_relight* ctx, const std::vector<BakeItem>& items) {
     }
     ctx->stats.texels += lum.texels;
     ctx->stats.launches += work.total ? 2 : 0;
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess && sync) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return fail(RL_DEVICE_ERROR, std::string("AO bake: ") + hipGetErrorString(e));
     for (const BakeItem& it : items) it.lm->device_newer = true;
     return RL_SUCCESS;
 }
 
-int32_t run_direct(rl_relight* ctx, rl_scene* scene, rl_lightmap* const* maps, int n) {
+int32_t run_direct(rl_relight* ctx, rl_scene* scene, rl_lightmap* const* maps, int n, bool sync = true) {
     std::vector<BakeItem> items;
     AoParams none{};
     int32_t st = whole_items(scene, maps, n, none, items);
     if (st) return st;
-    return run_direct_items(ctx, items);
+    return run_direct_items(ctx, items, sync);
 }
 
 int32_t run_ao(rl_relight* ctx, rl_scene* scene, rl_lightmap* const* maps, int n,
-               const rl_ao_settings* s) {
+               const rl_ao_settings* s, bool sync = true) {
     int32_t st = check_ao(s);
     if (st) return st;
     std::vector<BakeItem> items;
     if ((st = whole_items(scene, maps, n, ao_params(*s), items))) return st;
-    return run_ao_items(ctx, items);
+    return run_ao_items(ctx, items, sync);
 }
 
 // Final gather over items (each with its FG settings and its scene's photon tables).
-int32_t run_fg_items(rl_relight* ctx, const std::vector<BakeItem>& items) {
+int32_t run_fg_items(rl_relight* ctx, const std::vector<BakeItem>& items, bool sync = true) {
     DeviceSegs ds;
-    int32_t st = upload_segments(ctx, 0, items, kUnitLumel, 1, ds);
+    int32_t st = upload_segments(ctx, kSegGather, items, kUnitLumel, 1, ds);
     if (st) return st;
     hipError_t e = hipSuccess;
     if (ds.total) {
@@ -551,7 +578,7 @@ int32_t run_fg_items(rl_relight* ctx, const std::vector<BakeItem>& items) {
     }
     ctx->stats.texels += ds.texels;
     ctx->stats.launches += ds.total ? 1 : 0;
-    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess && sync) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return fail(RL_DEVICE_ERROR, std::string("final gather: ") + hipGetErrorString(e));
     for (const BakeItem& it : items) it.lm->device_newer = true;
     return RL_SUCCESS;
@@ -692,8 +719,13 @@ int32_t rl_destroy(rl_relight* ctx) {
     if (!ctx) return RL_SUCCESS;
     if (ctx->d_rays) hipFree(ctx->d_rays);
     if (ctx->d_queue) hipFree(ctx->d_queue);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
     for (BakeSeg* b : ctx->seg_buf)
         if (b) hipFree(b);
+    for (BakeSeg* b : ctx->seg_host)
+        if (b) hipHostFree(b);
+    for (hipEvent_t e : ctx->seg_ev)
+        if (e) hipEventDestroy(e);
     if (ctx->raster_owner) hipFree(ctx->raster_owner);
     if (ctx->raster_rec) hipFree(ctx->raster_rec);
     if (ctx->raster_count) hipFree(ctx->raster_count);
@@ -1162,8 +1194,9 @@ int32_t rl_bake_batch(rl_relight* ctx, rl_scene* scene, rl_lightmap* const* maps
     if (st) return st;
     CallTimer tm(ctx);
     if ((st = begin_call(ctx))) return st;
-    if (do_direct && (st = run_direct(ctx, scene, maps, num_maps))) return st;
-    if (ao && (st = run_ao(ctx, scene, maps, num_maps, ao))) return st;
+    // Both phases queued back to back; end_call synchronises.
+    if (do_direct && (st = run_direct(ctx, scene, maps, num_maps, false))) return st;
+    if (ao && (st = run_ao(ctx, scene, maps, num_maps, ao, false))) return st;
     return end_call(ctx, tm);
 }
 
@@ -1904,12 +1937,15 @@ uint64_t rl_server::step(std::vector<Job*>& js) {
         else nray++;
     }
     if (nray == 0) return rays;
-    const uint64_t share = std::max<uint64_t>(cfg.quantum_rays / nray, 4096);
+    // The step's quantum: every job first gets a floor of a quarter of an even share, so a small
+    // job that arrives behind large ones progresses at once; the rest goes to the jobs in
+    // admission order, so the oldest finish first (latency close to one job at a time, with the
+    // small jobs batched alongside).
     std::vector<BakeItem> items[4];
     std::vector<Job*> in_step;
-    for (Job* j : js) {
+    uint64_t left = cfg.quantum_rays;
+    auto take = [&](Job* j, uint64_t rays_wanted) {
         const RayPhase ph = ray_phase(j);
-        if (ph == kPhaseNone) continue;
         BakeItem proto{};
         uint64_t rpl = 1;
         if (ph == kPhaseDirect) {
@@ -1923,15 +1959,28 @@ uint64_t rl_server::step(std::vector<Job*>& js) {
             proto.pms = j->tables->d_pms;
             rpl = (uint64_t)j->ind.final_gather_samples;
         }
-        const uint64_t got = take_lumels(j, std::max<uint64_t>(share / rpl, 64), proto, items[ph]);
-        j->rays += got * rpl;
-        in_step.push_back(j);
+        const uint64_t got = take_lumels(j, std::max<uint64_t>(rays_wanted / rpl, 64), proto, items[ph]) * rpl;
+        j->rays += got;
+        left -= std::min(left, got);
+        if (got && std::find(in_step.begin(), in_step.end(), j) == in_step.end()) in_step.push_back(j);
+    };
+    const uint64_t floor_rays = std::max<uint64_t>(cfg.quantum_rays / (4 * nray), 4096);
+    for (Job* j : js)
+        if (ray_phase(j) != kPhaseNone) take(j, floor_rays);
+    for (Job* j : js) {
+        if (left == 0) break;
+        if (ray_phase(j) != kPhaseNone && j->map_i < j->maps.size()) take(j, left);
     }
+    for (Job* j : js)  // jobs whose phase had no lumels left still advance below
+        if (ray_phase(j) != kPhaseNone && std::find(in_step.begin(), in_step.end(), j) == in_step.end())
+            in_step.push_back(j);
     CallTimer tm(ctx);
     int32_t st = begin_call(ctx);
-    if (!st && !items[kPhaseDirect].empty()) st = run_direct_items(ctx, items[kPhaseDirect]);
-    if (!st && !items[kPhaseAo].empty()) st = run_ao_items(ctx, items[kPhaseAo]);
-    if (!st && !items[kPhaseGather].empty()) st = run_fg_items(ctx, items[kPhaseGather]);
+    // The phases are queued back to back (their segment tables use separate slots); end_call
+    // synchronises once.
+    if (!st && !items[kPhaseDirect].empty()) st = run_direct_items(ctx, items[kPhaseDirect], false);
+    if (!st && !items[kPhaseAo].empty()) st = run_ao_items(ctx, items[kPhaseAo], false);
+    if (!st && !items[kPhaseGather].empty()) st = run_fg_items(ctx, items[kPhaseGather], false);
     if (!st) st = end_call(ctx, tm);
     rays += ctx->stats.rays;
     for (Job* j : in_step) {
@@ -2092,8 +2141,8 @@ int32_t rl_bake_items(rl_relight* ctx, const rl_bake_item* items, int32_t num_it
     CallTimer tm(ctx);
     int32_t st = begin_call(ctx);
     if (st) return st;
-    if (!direct.empty() && (st = run_direct_items(ctx, direct))) return st;
-    if (!ao.empty() && (st = run_ao_items(ctx, ao))) return st;
+    if (!direct.empty() && (st = run_direct_items(ctx, direct, false))) return st;
+    if (!ao.empty() && (st = run_ao_items(ctx, ao, false))) return st;
     return end_call(ctx, tm);
 }
 
@@ -2118,7 +2167,7 @@ int32_t rl_bake_scene(rl_relight* ctx, rl_scene* scene, int32_t direct, const rl
     if ((st = begin_call(ctx))) return st;
     std::vector<BakeItem> items;
     if ((st = whole_items(scene, maps.data(), (int)maps.size(), AoParams{}, items))) return st;
-    if (direct && (st = run_direct_items(ctx, items))) return st;
+    if (direct && (st = run_direct_items(ctx, items, false))) return st;
     if (indirect) {
         std::vector<BakeItem> fg(items);
         for (BakeItem& it : fg) {
@@ -2126,12 +2175,12 @@ int32_t rl_bake_scene(rl_relight* ctx, rl_scene* scene, int32_t direct, const rl
             it.mesh_pm = T.d_mesh_pm;
             it.pms = T.d_pms;
         }
-        if ((st = run_fg_items(ctx, fg))) return st;
+        if ((st = run_fg_items(ctx, fg, false))) return st;
     }
     if (ao) {
         std::vector<BakeItem> a(items);
         for (BakeItem& it : a) it.ao = ao_params(*ao);
-        if ((st = run_ao_items(ctx, a))) return st;
+        if ((st = run_ao_items(ctx, a, false))) return st;
     }
     st = end_call(ctx, tm);
     ctx->stats.rays += photon_rays;
