@@ -336,6 +336,23 @@ def bench_serving_small(relight, n_requests=256, ao_samples=64, quantum_rays=1 <
     served_lat = [s_["done_ms"] - s_["submit_ms"] for s_ in sts]
     log(f"serving_small: saturated plain {plain_ms:.1f} ms, served {served_ms:.1f} ms "
         f"({plain_ms / served_ms:.2f}x), steps {sstats['steps']}")
+    # One small lightmap baked repeatedly: eager bake_batch calls vs replays of the recorded pass.
+    reps = 50
+    lm0 = [lms[0]]
+    relight.bake_batch(scene, lm0, direct=True, ao=ao)
+    t = time.perf_counter()
+    for _ in range(reps):
+        relight.bake_batch(scene, lm0, direct=True, ao=ao)
+    eager_ms = (time.perf_counter() - t) * 1e3 / reps
+    pas = relight.record_pass(scene, lm0, direct=True, ao=ao)
+    pas.launch()
+    t = time.perf_counter()
+    for _ in range(reps):
+        pas.launch()
+    graph_ms = (time.perf_counter() - t) * 1e3 / reps
+    pas.close()
+    log(f"serving_small: one {lms[0].valid_count()}-texel lightmap, bake_batch {eager_ms:.3f} ms/pass, "
+        f"recorded pass {graph_ms:.3f} ms/pass")
     return {
         "workload": f"{n} direct+AO requests ({ao_samples} cosine AO rays + sun and 16-vertex area light) on "
                     f"128^2-256^2 lightmaps of one resident city (2,500 boxes, mean {np.mean(texels):.0f} texels)",
@@ -345,4 +362,6 @@ def bench_serving_small(relight, n_requests=256, ao_samples=64, quantum_rays=1 <
         "served_latency_ms": {"p50": percentile(served_lat, 50), "p99": percentile(served_lat, 99)},
         "plain_latency_ms": {"p50": percentile(plain_done, 50), "p99": percentile(plain_done, 99)},
         "server_steps": sstats["steps"], "rays": plain_rays,
+        "repeated_small_pass": {"texels": lms[0].valid_count(), "bake_batch_ms": eager_ms,
+                                "recorded_pass_ms": graph_ms},
     }

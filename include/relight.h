@@ -248,6 +248,18 @@ int32_t rl_bake_ambient_occlusion(rl_relight* ctx, rl_scene* scene, int32_t mesh
 int32_t rl_bake_batch(rl_relight* ctx, rl_scene* scene, rl_lightmap* const* maps,
                       int32_t num_maps, int32_t do_direct, const rl_ao_settings* ao);
 
+/* The rl_bake_batch pass recorded once as a HIP graph (segment tables in buffers of its own,
+ * kernel choice and launch shape fixed at recording) and replayed by rl_pass_launch with one
+ * graph launch; each launch equals the same rl_bake_batch call bit for bit. Replays check that
+ * the scene and lightmaps still hold the device arrays the pass was recorded with (rebuilding the
+ * scene or re-rasterising a lightmap makes the pass stale: RL_INVALID_CALL). Destroy a pass before
+ * its scene and lightmaps. */
+typedef struct rl_pass rl_pass;
+int32_t rl_pass_create(rl_relight* ctx, rl_scene* scene, rl_lightmap* const* maps, int32_t num_maps,
+                       int32_t do_direct, const rl_ao_settings* ao, rl_pass** out);
+int32_t rl_pass_launch(rl_relight* ctx, rl_pass* pass);
+int32_t rl_pass_destroy(rl_pass* pass);
+
 /* rt::ITracer::traceSegment (rt/Tracer.h:87) for n segments start[i] -> end[i] (xyz each):
  * direction normalize(end-start), interval (0.01, |end-start|) as Embree::initializeRay
  * (rt/Embree.cpp:107-130). RL_TRACE_ANY writes hit flags (0/1) to out_flags; RL_TRACE_CLOSEST

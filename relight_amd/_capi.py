@@ -244,6 +244,10 @@ SIGNATURES = [
     ("rl_bake_ambient_occlusion", C.c_int32, [_P, _P, C.c_int32, C.POINTER(rl_ao_settings)]),
     ("rl_bake_batch", C.c_int32, [_P, _P, C.POINTER(C.c_void_p), C.c_int32, C.c_int32,
                                   C.POINTER(rl_ao_settings)]),
+    ("rl_pass_create", C.c_int32, [_P, _P, C.POINTER(C.c_void_p), C.c_int32, C.c_int32,
+                                   C.POINTER(rl_ao_settings), _PP]),
+    ("rl_pass_launch", C.c_int32, [_P, _P]),
+    ("rl_pass_destroy", C.c_int32, [_P]),
     ("rl_trace_segments", C.c_int32, [_P, _P, _F, _F, C.c_uint32, C.c_int32,
                                       C.POINTER(C.c_uint8), C.POINTER(rl_hit)]),
     ("rl_photonmap_create", C.c_int32, [_P, C.c_int32, C.c_int32, _PP]),

@@ -426,8 +426,9 @@ int32_t whole_items(rl_scene* scene, rl_lightmap* const* maps, int n, const AoPa
     return RL_SUCCESS;
 }
 
-int32_t upload_segments(rl_relight* ctx, int slot, const std::vector<BakeItem>& items, int unit,
-                        uint32_t chunk_lumels, DeviceSegs& out) {
+// The segment table of `items` in work units `unit` (out gets the counts; out.d stays unset).
+std::vector<BakeSeg> build_segments(const std::vector<BakeItem>& items, int unit, uint32_t chunk_lumels,
+                                    DeviceSegs& out) {
     std::vector<BakeSeg> segs;
     uint64_t acc = 0;
     for (const BakeItem& it : items) {
@@ -458,6 +459,12 @@ int32_t upload_segments(rl_relight* ctx, int slot, const std::vector<BakeItem>& 
     }
     out.total = acc;
     out.n = (int)segs.size();
+    return segs;
+}
+
+int32_t upload_segments(rl_relight* ctx, int slot, const std::vector<BakeItem>& items, int unit,
+                        uint32_t chunk_lumels, DeviceSegs& out) {
+    const std::vector<BakeSeg> segs = build_segments(items, unit, chunk_lumels, out);
     if (segs.empty()) return RL_SUCCESS;
     if (ctx->seg_cap[slot] < segs.size()) {
         RL_HIP(hipStreamSynchronize(ctx->stream));  // queued kernels may still read the old table
@@ -1199,6 +1206,150 @@ int32_t rl_bake_batch(rl_relight* ctx, rl_scene* scene, rl_lightmap* const* maps
     if (ao && (st = run_ao(ctx, scene, maps, num_maps, ao, false))) return st;
     return end_call(ctx, tm);
 }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- recorded passes (HIP graphs)
+
+// A batched direct/AO pass recorded once as a HIP graph and replayed: its segment tables live in
+// buffers of its own, and each replay checks that the scene and lightmaps still have the device
+// arrays the graph was recorded with.
+struct rl_pass {
+    rl_relight* ctx = nullptr;
+    rl_scene* scene = nullptr;
+    const Bvh4Node* nodes = nullptr;
+    struct MapRef {
+        rl_lightmap* lm;
+        const float4* pos;
+        float4* color;
+        uint32_t* counts;
+        size_t texels;
+    };
+    std::vector<MapRef> maps;
+    BakeSeg* d_segs[3] = {};  // direct, AO work units, AO lumels
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t texels = 0;
+    int launches = 0;
+
+    ~rl_pass() {
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+        for (BakeSeg* d : d_segs) dev_free(d);
+    }
+};
+
+namespace {
+
+int32_t pass_upload(const std::vector<BakeSeg>& segs, BakeSeg** d) {
+    if (segs.empty()) return RL_SUCCESS;
+    RL_HIP(dev_alloc(d, segs.size()));
+    RL_HIP(hipMemcpy(*d, segs.data(), segs.size() * sizeof(BakeSeg), hipMemcpyHostToDevice));
+    return RL_SUCCESS;
+}
+
+// The launches of one pass on ctx->stream (recorded into the graph while it is captured). The
+// fills are kernels, not memset nodes.
+hipError_t pass_launches(rl_relight* ctx, const rl_pass& p, const DeviceSegs& dir, const DeviceSegs& work,
+                         const DeviceSegs& lum, bool refill, uint32_t chunk, const std::vector<BakeItem>& ao_items) {
+    hipError_t e = hipSuccess;
+    if (dir.total)
+        e = launch_bake_direct(ctx->stream, dir.cap, ctx->max_blocks, p.d_segs[0], dir.n, dir.total, ctx->d_queue,
+                               ctx->d_rays);
+    if (e == hipSuccess && work.total) {
+        if (refill) {
+            e = launch_ao_refill(ctx->stream, work.cap, ctx->max_blocks, p.d_segs[1], work.n, work.total, chunk,
+                                 ctx->tuning.refill_min, ctx->d_queue, ctx->d_rays);
+        } else {
+            for (const BakeItem& it : ao_items)
+                if (e == hipSuccess && it.count) e = launch_fill_u32(ctx->stream, it.lm->d_counts + it.first, it.count, 0u);
+            if (e == hipSuccess)
+                e = launch_ao_count(ctx->stream, work.cap, ctx->max_blocks, p.d_segs[1], work.n, work.total,
+                                    ctx->d_queue, ctx->d_rays);
+        }
+        if (e == hipSuccess) e = launch_ao_finalize(ctx->stream, p.d_segs[2], lum.n, lum.total);
+    }
+    return e;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t rl_pass_create(rl_relight* ctx, rl_scene* scene, rl_lightmap* const* maps, int32_t num_maps,
+                       int32_t do_direct, const rl_ao_settings* ao, rl_pass** out) {
+    if (!ctx || !maps || num_maps <= 0 || !out) return fail(RL_INVALID_ARGUMENT, "bad pass arguments");
+    *out = nullptr;
+    int32_t st = check_scene_ready(scene);
+    if (st) return st;
+    if (ao && (st = check_ao(ao))) return st;
+    if (!do_direct && !ao) return fail(RL_INVALID_ARGUMENT, "a pass needs direct light or AO");
+    std::vector<BakeItem> items;
+    if ((st = whole_items(scene, maps, num_maps, ao ? ao_params(*ao) : AoParams{}, items))) return st;
+    std::unique_ptr<rl_pass> p(new rl_pass());
+    p->ctx = ctx;
+    p->scene = scene;
+    p->nodes = scene->dev().nodes;
+    for (const BakeItem& it : items) {
+        p->maps.push_back({it.lm, it.lm->d_pos, it.lm->d_color, it.lm->d_counts, it.lm->texels.size()});
+        p->texels += it.count;
+    }
+    // The same kernel choice as a direct bake_batch call, made once at recording.
+    bool refill = ctx->tuning.ao_kernel == RL_AO_KERNEL_REFILL;
+    for (const BakeItem& it : items) refill = refill && it.ao.samples >= 4;
+    const uint32_t chunk = ctx->tuning.ao_chunk_lumels;
+    DeviceSegs dir, work, lum;
+    if (do_direct && (st = pass_upload(build_segments(items, kUnitLumel, 1, dir), &p->d_segs[0]))) return st;
+    if (ao) {
+        if ((st = pass_upload(build_segments(items, refill ? kUnitChunk : kUnitPair, chunk, work), &p->d_segs[1])))
+            return st;
+        if ((st = pass_upload(build_segments(items, kUnitLumel, 1, lum), &p->d_segs[2]))) return st;
+    }
+    p->launches = (dir.total ? 1 : 0) + (work.total ? 2 : 0);
+    RL_HIP(hipStreamSynchronize(ctx->stream));
+    RL_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = pass_launches(ctx, *p, dir, work, lum, refill, chunk, items);
+    hipGraph_t g = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
+    p->graph = g;
+    if (e == hipSuccess) e = e2;
+    if (e == hipSuccess) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) return fail(RL_DEVICE_ERROR, std::string("recording the pass: ") + hipGetErrorString(e));
+    *out = p.release();
+    return RL_SUCCESS;
+}
+
+int32_t rl_pass_launch(rl_relight* ctx, rl_pass* pass) {
+    if (!ctx || !pass) return fail(RL_INVALID_ARGUMENT, "NULL argument");
+    if (pass->ctx != ctx) return fail(RL_INVALID_CALL, "the pass was recorded on another context");
+    if (pass->scene->dev().nodes != pass->nodes) return fail(RL_INVALID_CALL, "the scene changed since recording");
+    for (const rl_pass::MapRef& m : pass->maps) {
+        int32_t st = push_lumels(m.lm);
+        if (st) return st;
+        if (m.lm->d_pos != m.pos || m.lm->d_color != m.color || m.lm->d_counts != m.counts ||
+            m.lm->texels.size() != m.texels)
+            return fail(RL_INVALID_CALL, "a lightmap changed since recording");
+    }
+    CallTimer tm(ctx);
+    int32_t st = begin_call(ctx);
+    if (st) return st;
+    RL_HIP(hipGraphLaunch(pass->exec, ctx->stream));
+    ctx->stats.texels += pass->texels;
+    ctx->stats.launches += pass->launches;
+    for (const rl_pass::MapRef& m : pass->maps) m.lm->device_newer = true;
+    return end_call(ctx, tm);
+}
+
+int32_t rl_pass_destroy(rl_pass* pass) {
+    if (!pass) return RL_SUCCESS;
+    hipStreamSynchronize(pass->ctx->stream);
+    delete pass;
+    return RL_SUCCESS;
+}
+
+}  // extern "C"
+
+extern "C" {
 
 int32_t rl_trace_segments(rl_relight* ctx, rl_scene* scene, const float* starts, const float* ends,
                           uint32_t n, int32_t mode, uint8_t* out_flags, rl_hit* out_hits) {

@@ -495,6 +495,11 @@ class Relight:
                                                   1 if direct else 0,
                                                   C.byref(s) if s is not None else None))
 
+    def record_pass(self, scene: Scene, lightmaps: List[Lightmap], direct: bool = True,
+                    ao: Optional[AmbientOcclusionSettings] = None) -> "Pass":
+        """The bake_batch pass recorded once as a HIP graph; Pass.launch() replays it."""
+        return Pass(self, scene, lightmaps, direct, ao)
+
     def bake_items(self, items):
         """One multi-scene launch per pass over (scene, lightmap, first, count, direct, ao) items
         (count None: to the end of the lightmap); equals the per-lightmap bakes."""
@@ -528,6 +533,37 @@ class Relight:
                    for i, name in A.KERNEL_NAMES.items() if st.kernel_launches[i]}
         return {"rays": st.rays, "texels": st.texels, "gpu_ms": st.gpu_ms,
                 "wall_ms": st.wall_ms, "launches": st.launches, "kernels": kernels}
+
+
+class Pass:
+    """A recorded bake_batch pass (rl_pass_create): launch() replays it with one graph launch and
+    equals the same bake_batch call bit for bit. Close it before its scene and lightmaps."""
+
+    def __init__(self, relight: "Relight", scene: Scene, lightmaps: List[Lightmap], direct: bool,
+                 ao: Optional[AmbientOcclusionSettings]):
+        self._relight = relight
+        self._lib = relight._lib
+        self.scene, self.lightmaps = scene, list(lightmaps)
+        arr = (C.c_void_p * len(lightmaps))(*[lm._h.value for lm in lightmaps])
+        s = ao.to_c() if ao is not None else None
+        h = C.c_void_p()
+        _check(self._lib, self._lib.rl_pass_create(relight._h, scene._h, arr, len(lightmaps), 1 if direct else 0,
+                                                   C.byref(s) if s is not None else None, C.byref(h)))
+        self._h = h
+
+    def launch(self):
+        _check(self._lib, self._lib.rl_pass_launch(self._relight._h, self._h))
+
+    def close(self):
+        if self._h:
+            self._lib.rl_pass_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class BakeJob:

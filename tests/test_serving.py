@@ -189,3 +189,29 @@ def test_full_bake_scene_and_server_job_equal_plain_sequence(relight):
         assert all(j.wait(timeout=120)["state"] == "done" for j in jobs)
     for p, lm in zip(plain, l2):
         assert np.array_equal(p, lm.to_hdr())
+
+
+@pytest.mark.gpu
+def test_recorded_pass_replays_equal_eager_passes():
+    """A bake_batch pass recorded as a HIP graph, replayed four times with other eager bakes and
+    synchronisations in between, equals the same eager passes on an identical second set of
+    lightmaps after every replay (direct light accumulates, so each pass changes the colours)."""
+    print("[gpu] recorded pass (HIP graph) vs eager bake_batch, 4 replays", flush=True)
+    r = Relight()
+    spec = scenes.city(num_boxes=200, tiles=(3, 2), side_range=(48, 96), ao_samples=16)
+    scene_a, lms_a, _ = scenes.build_gpu(r, spec)
+    scene_b, lms_b, _ = scenes.build_gpu(r, spec)
+    scene_c, lms_c, ids_c = scenes.build_gpu(r, spec)
+    ao = AmbientOcclusionSettings(16, 0.8, 4.0, 1.0, 3, spec.ao.sampling)
+    p = r.record_pass(scene_b, lms_b, direct=True, ao=ao)
+    for k in range(4):
+        r.bake_batch(scene_a, lms_a, direct=True, ao=ao)
+        p.launch()
+        st = r.stats()
+        assert st["rays"] > 0 and st["texels"] == sum(lm.valid_count() for lm in lms_b)
+        for a, b in zip(lms_a, lms_b):
+            assert np.array_equal(a.to_hdr(), b.to_hdr()), f"replay {k}"
+        r.bake_direct_light(scene_c, ids_c[k % len(ids_c)])  # unrelated eager work between replays
+        r.bake_ambient_occlusion(scene_c, ids_c[0], ao)
+    p.close()
+    r.close()
