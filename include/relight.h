@@ -253,7 +253,7 @@ int32_t rl_bake_batch(rl_relight* ctx, rl_scene* scene, rl_lightmap* const* maps
  * graph launch; each launch equals the same rl_bake_batch call bit for bit. Replays check that
  * the scene and lightmaps still hold the device arrays the pass was recorded with (rebuilding the
  * scene or re-rasterising a lightmap makes the pass stale: RL_INVALID_CALL). Destroy a pass before
- * its scene and lightmaps. */
+ * its scene, its lightmaps and its context. */
 typedef struct rl_pass rl_pass;
 int32_t rl_pass_create(rl_relight* ctx, rl_scene* scene, rl_lightmap* const* maps, int32_t num_maps,
                        int32_t do_direct, const rl_ao_settings* ao, rl_pass** out);
