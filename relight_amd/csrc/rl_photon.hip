@@ -226,14 +226,15 @@ __global__ void k_photon_gather(const PhotonMapDev* __restrict__ pms, int npm, u
 constexpr int kFgBlock = 256;
 constexpr uint32_t kFgTopNodes = 192;
 
-// IndirectLight::bakeLumel with lane refill. A wavefront takes lumels from the persistent queue
-// and hands their (lumel, sample) pairs, in order, to idle lanes whenever at least `refill_min`
-// lanes are idle; pair number j of the wavefront's stream owns slot j mod kFgRing of the
-// wavefront's LDS ring. A lane traces its sample in rounds (closest_run), so finished lanes take
-// new samples while others still traverse; when its ray is done it shades the sample and writes
-// the sample's term into its slot. The wavefront adds finished terms to the current lumel's sum
-// strictly in stream order, that is in sample order as the reference's loop does, and writes the
-// lumel when its last sample has been added. A lane may only start pair j < consumed + kFgRing.
+// IndirectLight::bakeLumel (baker/IndirectLight.cpp:47-75) with lane refill. A wavefront takes
+// lumels from the persistent queue and hands their (lumel, sample) pairs, in order, to idle lanes
+// whenever at least `refill_min` lanes are idle; pair number j of the wavefront's stream owns slot
+// j mod kFgRing of the wavefront's LDS ring. A lane traces its sample in rounds (closest_run), so
+// finished lanes take new samples while others still traverse; when its ray is done it shades the
+// sample and writes the sample's term into its slot. The wavefront adds finished terms to the
+// current lumel's sum strictly in stream order, that is in sample order as the reference's loop
+// does, and writes the lumel when its last sample has been added. A lane may only start pair j <
+// consumed + kFgRing.
 constexpr uint32_t kFgRing = 128;
 // LDS per workgroup: 16 KB stacks + 10 KB rings + 13 KB of the scene's first 120 nodes, so four
 // workgroups (sixteen wavefronts) fit a compute unit's 160 KB.
@@ -388,13 +389,13 @@ __global__ __launch_bounds__(kFgBlock) void k_fg_refill(const BakeSeg* __restric
     count_rays(rays, rays_out);
 }
 
-// IndirectLight::bakeLumel with one lumel per lane. Each lane takes a lumel from the persistent
-// queue and traces its samples one after another, adding each sample's term to the lumel's sum as
-// it finishes, so the sum runs in sample order as the reference's loop does, with no exchange
-// between lanes. A lane traces in rounds (closest_run); lanes whose ray is done wait until at least
-// `refill_min` lanes of the wavefront are without a ray, then start their next samples together
-// (lanes whose lumel is done first take new lumels), so the ray set-up runs for many lanes at once.
-// LDS per workgroup: 16 KB stacks + 21 KB of the scene's first 192 nodes.
+// IndirectLight::bakeLumel (baker/IndirectLight.cpp:47-75) with one lumel per lane. Each lane takes
+// a lumel from the persistent queue and traces its samples one after another, adding each sample's
+// term to the lumel's sum as it finishes, so the sum runs in sample order as the reference's loop
+// does, with no exchange between lanes. A lane traces in rounds (closest_run); lanes whose ray is
+// done wait until at least `refill_min` lanes of the wavefront are without a ray, then start their
+// next samples together (lanes whose lumel is done first take new lumels), so the ray set-up runs
+// for many lanes at once. LDS per workgroup: 16 KB stacks + 21 KB of the scene's first 192 nodes.
 constexpr uint64_t kFgLanesMinPerLane = 8;  // lumels per resident lane for the lane-per-lumel kernel
 
 template <int CAP>
