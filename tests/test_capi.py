@@ -81,3 +81,18 @@ def test_argument_errors_without_a_device():
     assert lib.rl_server_destroy(None) == _capi.RL_SUCCESS
     st = _capi.rl_job_status()
     assert lib.rl_server_poll(None, 1, ctypes.byref(st)) == _capi.RL_INVALID_ARGUMENT
+
+
+def test_pass_and_tuning_argument_errors_without_a_device():
+    """Recorded-pass and tuning entry points reject NULL handles and bad values before any device
+    work (callable without a GPU)."""
+    if not os.path.exists(_capi.LIB_PATH):
+        pytest.skip("librelight_amd.so not built")
+    lib = _capi.load()
+    out = ctypes.c_void_p()
+    assert lib.rl_pass_create(None, None, None, 0, 1, None, ctypes.byref(out)) == _capi.RL_INVALID_ARGUMENT
+    assert lib.rl_pass_launch(None, None) == _capi.RL_INVALID_ARGUMENT
+    assert lib.rl_pass_destroy(None) == _capi.RL_SUCCESS
+    t = _capi.rl_tuning(_capi.RL_AO_KERNEL_REFILL, 16, 8, 8, _capi.RL_FG_KERNEL_AUTO)
+    assert lib.rl_set_tuning(None, ctypes.byref(t)) == _capi.RL_INVALID_ARGUMENT
+    assert lib.rl_get_tuning(None, ctypes.byref(t)) == _capi.RL_INVALID_ARGUMENT
