@@ -274,23 +274,40 @@ __device__ __forceinline__ int any_hit_run(const SceneDev& S, const TopCache& to
     }
 }
 
-// One round of closest-hit traversal in the two phases of traverse(): internal nodes until this
-// lane holds a leaf, then the triangles of that leaf and of any leaves popped right after it.
-// Returns 0 when the round ends at an internal node (call again), -1 when traversal is finished:
-// res then holds the closest hit (tri = 0xFFFFFFFF for a miss). Start with node = 0, sp = 0 and
-// res = {r.w.tfar, 0xFFFFFFFF, 0, 0, 0, 0}. A lane tests the nodes and triangles traverse<false>()
-// tests, in the same order and against the same shrinking interval (res.t), so the result equals
-// traverse<false>()'s; rounds only let a caller give finished lanes new rays in between.
+// One round of closest-hit traversal: a node phase, then the triangles of the leaves it found.
+// In the node phase a lane that reaches a leaf holds it and goes on visiting nodes (speculatively:
+// its interval has not yet been shortened by that leaf's triangles) until it reaches a second leaf
+// or runs out of nodes; the phase ends when every lane holds a leaf or has finished (Aila and
+// Laine 2009, speculative while-while), so lanes do not idle while others search. The leaf phase
+// tests the held leaf's triangles, one per iteration, then any leaf the lane stopped at and the
+// leaves popped right after it. Returns 0 when the round ends at an internal node (call again),
+// -1 when traversal is finished: res then holds the closest hit (tri = 0xFFFFFFFF for a miss).
+// Start with node = 0, sp = 0 and res = {r.w.tfar, 0xFFFFFFFF, 0, 0, 0, 0}. Every node a lane
+// skips is culled against an interval no shorter than the final one, so the lane tests every
+// triangle that traverse<false>() could return, and the result (nearest t, ties to the smaller
+// id) equals traverse<false>()'s; rounds only let a caller give finished lanes new rays.
 template <int CAP>
 __device__ __forceinline__ int closest_run(const SceneDev& S, const TopCache& top, const TravRay& r, const WtPerm& p,
                                            uint32_t& node, int& sp, uint32_t* stk, uint32_t stride, uint32_t* spill,
                                            TraceResult& res) {
-    while (!(node & RL_LEAF_BIT))
-        if (!visit_node<CAP>(S, top, r.nx, r.ny, r.nz, r.ix, r.iy, r.iz, r.oix, r.oiy, r.oiz, r.w.tnear,
+    uint32_t held = 0;
+    bool has = false, done = false;
+    for (;;) {
+        if (!done && !has && (node & RL_LEAF_BIT)) {
+            held = node;
+            has = true;
+            if (sp == 0) done = true;
+            else node = stack_pop(sp, stk, stride, spill);
+        }
+        if (__ballot(!has && !done) == 0ull) break;
+        if (!done && !(node & RL_LEAF_BIT) &&
+            !visit_node<CAP>(S, top, r.nx, r.ny, r.nz, r.ix, r.iy, r.iz, r.oix, r.oiy, r.oiz, r.w.tnear,
                              res.t * kBoxScale, node, sp, stk, stride, spill))
-            return -1;
-    uint32_t cur = node & RL_LEAF_FIRST_MASK;
-    uint32_t end = cur + ((node >> 27) & 15u) + 1u;
+            done = true;
+    }
+    if (!has) return -1;  // finished without reaching another leaf
+    uint32_t cur = held & RL_LEAF_FIRST_MASK;
+    uint32_t end = cur + ((held >> 27) & 15u) + 1u;
     for (;;) {
         const gfloat4* tp = gmem(S.tris) + 3u * cur;
         float t, U, V, W, det;
@@ -303,13 +320,14 @@ __device__ __forceinline__ int closest_run(const SceneDev& S, const TopCache& to
             }
         }
         if (++cur == end) {
-            if (sp == 0) return -1;
-            node = stack_pop(sp, stk, stride, spill);
-            if (!(node & RL_LEAF_BIT)) return 0;
+            if (done || !(node & RL_LEAF_BIT)) break;
             cur = node & RL_LEAF_FIRST_MASK;
             end = cur + ((node >> 27) & 15u) + 1u;
+            if (sp == 0) done = true;
+            else node = stack_pop(sp, stk, stride, spill);
         }
     }
+    return done ? -1 : 0;
 }
 
 // Traverses the BVH4 for one ray. ANY: stop at the first triangle with t in (tnear, tfar).
