@@ -2,9 +2,10 @@
 //
 // Replaces the acceleration structure Embree builds in rtcCommit (rt/Embree.cpp:55-58 via
 // Scene::end, scene/Scene.cpp:122-142). Child boxes are the fp32 min/max of their vertices,
-// widened by a padding of 2^-20 of the largest coordinate magnitude: the kernels' fma slab test
-// shifts a plane by at most |o| * 2^-24, and the exit distance is scaled by 1 + 2*gamma(3)
-// (Ize, "Robust BVH Ray Traversal", JCGT 2(2), 2013), so no box the exact ray crosses is missed.
+// widened by a padding of 2^-18 of the largest coordinate magnitude: the kernels' fma slab test
+// shifts a plane by at most 3 |o| * 2^-24, and the exit distances are scaled by 1 + 2*gamma(3)
+// (Ize, "Robust BVH Ray Traversal", JCGT 2(2), 2013), so no box the exact ray crosses is missed
+// for ray origins within 20 times the largest coordinate magnitude.
 #include "rl_bvh.h"
 
 #include <algorithm>
@@ -274,11 +275,11 @@ int build_bvh4(const float* tri9, uint32_t num_tris, Bvh4Build& out) {
     for (const Node2& n : b2.nodes_) {
         if (n.left < 0 && n.count > RL_LEAF_MAX) return -3;
     }
-    // Box padding for the fma slab test: more than 2^-21 of the largest coordinate magnitude
-    // (ray origins lie inside the scene, so |o| is bounded by it), plus a tiny absolute term.
+    // Box padding for the fma slab test: 2^-18 of the largest coordinate magnitude (covers the
+    // test's plane shift of 3 |o| * 2^-24 for origins up to 20 times it), plus a tiny absolute term.
     float maxc = 0.0f;
     for (size_t i = 0; i < (size_t)num_tris * 9; i++) maxc = std::max(maxc, std::fabs(tri9[i]));
-    const float pad = maxc * (1.0f / 1048576.0f) + 1e-30f;
+    const float pad = maxc * (1.0f / 262144.0f) + 1e-30f;
     Collapser c(b2.nodes_, pad);
     c.run();
     // Renumber the nodes in breadth-first order (the root stays 0), so the nodes nearest the root,

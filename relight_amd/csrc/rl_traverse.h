@@ -3,8 +3,9 @@
 // Near-first traversal with a per-lane LDS stack (CAP entries, one column per lane), a
 // conservative fma slab test and the watertight triangle test of rl_math.h. Each axis's entry and
 // exit planes are picked by the sign of the ray direction when the node is loaded (a per-lane
-// load offset), so the slab test needs no per-box min/max of the two plane distances. Closest-hit ties at equal t go to the smaller scene-order triangle
-// id, so results do not depend on traversal order.
+// load offset), so the slab test needs no per-box min/max of the two plane distances.
+// Closest-hit ties at equal t go to the smaller scene-order triangle id, so results do not depend
+// on traversal order.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -14,7 +15,7 @@
 namespace rl {
 namespace trav {
 
-// Slab-test safety factor 1 + 2*gamma(3) (Ize 2013): scales the exit distance so that the
+// Slab-test safety factor 1 + 2*gamma(3) (Ize 2013): scales the exit distances so that the
 // rounded slab distances never reject a box the exact ray passes through.
 constexpr float kBoxScale = 1.0000004f;
 
@@ -87,45 +88,70 @@ __device__ __forceinline__ uint32_t rank_below(uint64_t mask) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-// Per-lane constants of a ray for the step-wise traversal below. nx/ny/nz: float4 offset (0 or 1)
-// of the entry-plane row of each axis in a node (lo for a positive direction, hi for negative).
-struct TravRay {
-    WtRay w;
+// Per-lane constants of the slab test. Entry planes: t = fma(b, i, oi) with i = 1/d and
+// oi = -o/d per axis; exit planes: t = fma(b, si, soi) with the same two constants multiplied by
+// kBoxScale. nx/ny/nz: float4 offset (0 or 1) of the entry-plane row of each axis in a node (lo
+// for a positive direction, hi for a negative one).
+//
+// Rounding: an entry distance is the exact distance to its plane shifted by at most |o| * 2^-24,
+// times a factor within 2 * 2^-24 of 1; an exit distance is the exact distance to its plane
+// shifted by at most 3 |o| * 2^-24, times kBoxScale * (1 +- 3 * 2^-24). kBoxScale (1 + 6 * 2^-24
+// once rounded) covers the five relative roundings, and the builder's box padding (2^-18 of the
+// largest coordinate magnitude) covers the shifts for any origin within 20 times that magnitude,
+// so no box the exact ray crosses within (tnear, lim) is rejected.
+struct SlabRay {
     float ix, iy, iz, oix, oiy, oiz;
+    float six, siy, siz, soix, soiy, soiz;
     uint32_t nx, ny, nz;
 };
 
-__device__ __forceinline__ TravRay trav_ray(const Ray& ray) {
-    TravRay r;
-    r.w = wt_setup(ray);
+__device__ __forceinline__ SlabRay slab_ray(const Ray& ray) {
+    SlabRay r;
+    // A zero direction component would give fma(b, inf, -inf) = NaN; the slab test uses
+    // +-1e-20 there instead (a displacement below 1e-17 m over any ray, inside the box padding).
     const float dx = fabsf(ray.d.x) > 1e-20f ? ray.d.x : copysignf(1e-20f, ray.d.x);
     const float dy = fabsf(ray.d.y) > 1e-20f ? ray.d.y : copysignf(1e-20f, ray.d.y);
     const float dz = fabsf(ray.d.z) > 1e-20f ? ray.d.z : copysignf(1e-20f, ray.d.z);
     r.ix = 1.0f / dx; r.iy = 1.0f / dy; r.iz = 1.0f / dz;
     r.oix = -(ray.o.x * r.ix); r.oiy = -(ray.o.y * r.iy); r.oiz = -(ray.o.z * r.iz);
+    r.six = r.ix * kBoxScale; r.siy = r.iy * kBoxScale; r.siz = r.iz * kBoxScale;
+    r.soix = r.oix * kBoxScale; r.soiy = r.oiy * kBoxScale; r.soiz = r.oiz * kBoxScale;
     r.nx = r.ix < 0.0f ? 1u : 0u;
     r.ny = r.iy < 0.0f ? 1u : 0u;
     r.nz = r.iz < 0.0f ? 1u : 0u;
     return r;
 }
 
+// Per-lane constants of a ray for the step-wise traversal below.
+struct TravRay {
+    WtRay w;
+    SlabRay s;
+};
+
+__device__ __forceinline__ TravRay trav_ray(const Ray& ray) {
+    TravRay r;
+    r.w = wt_setup(ray);
+    r.s = slab_ray(ray);
+    return r;
+}
+
 // Four child boxes of a node against a ray, given the node's entry-plane (e*) and exit-plane (f*)
 // rows of each axis: the entry distance of each box (its sort key), +inf for a box the ray misses
-// within (tnear, lim). The entry plane of an axis is lo for a positive
-// direction component and hi for a negative one, so t_entry = fma(plane, 1/d, -o/d) equals the
-// min of the two plane distances (fma rounding is monotone in the plane) and t_exit the max.
-__device__ __forceinline__ void box4(f32x4 ex, f32x4 fx, f32x4 ey, f32x4 fy, f32x4 ez, f32x4 fz, float ix, float iy,
-                                     float iz, float oix, float oiy, float oiz, float tnear, float lim, float* key) {
+// within (tnear, lim). The entry plane of an axis is lo for a positive direction component and hi
+// for a negative one, so the entry distance is the min of the two plane distances (fma rounding is
+// monotone in the plane) and the exit distance the max.
+__device__ __forceinline__ void box4(f32x4 ex, f32x4 fx, f32x4 ey, f32x4 fy, f32x4 ez, f32x4 fz, const SlabRay& r,
+                                     float tnear, float lim, float* key) {
     const float exa[4] = {ex.x, ex.y, ex.z, ex.w}, fxa[4] = {fx.x, fx.y, fx.z, fx.w};
     const float eya[4] = {ey.x, ey.y, ey.z, ey.w}, fya[4] = {fy.x, fy.y, fy.z, fy.w};
     const float eza[4] = {ez.x, ez.y, ez.z, ez.w}, fza[4] = {fz.x, fz.y, fz.z, fz.w};
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        const float t0x = fmaf(exa[i], ix, oix), t1x = fmaf(fxa[i], ix, oix);
-        const float t0y = fmaf(eya[i], iy, oiy), t1y = fmaf(fya[i], iy, oiy);
-        const float t0z = fmaf(eza[i], iz, oiz), t1z = fmaf(fza[i], iz, oiz);
+        const float t0x = fmaf(exa[i], r.ix, r.oix), t1x = fmaf(fxa[i], r.six, r.soix);
+        const float t0y = fmaf(eya[i], r.iy, r.oiy), t1y = fmaf(fya[i], r.siy, r.soiy);
+        const float t0z = fmaf(eza[i], r.iz, r.oiz), t1z = fmaf(fza[i], r.siz, r.soiz);
         const float tmin = fmaxf(fmaxf(t0x, t0y), fmaxf(t0z, tnear));
-        const float tmax = fminf(fminf(t1x, t1y), t1z) * kBoxScale;
+        const float tmax = fminf(fminf(t1x, t1y), t1z);
         const bool hit = tmin <= fminf(tmax, lim);  // empty slots hold inverted boxes
         key[i] = hit ? tmin : __builtin_inff();
     }
@@ -167,33 +193,42 @@ __device__ __forceinline__ uint32_t load_top_nodes(const SceneDev& S, u32x4* row
     return n;
 }
 
+// 24-bit unsigned multiply (full rate; the compiler otherwise turns a constant multiply whose
+// operand range it cannot bound into a quarter-rate 32-bit multiply).
+__device__ __forceinline__ uint32_t mul_u24(uint32_t a, uint32_t b) {
+    uint32_t r;
+    asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
 // One internal node: its four child boxes tested against (tnear, lim), hit children pushed
 // far-first, the nearest taken next; with no child hit the next node is popped. Returns false when
 // the stack was empty (traversal ended).
 template <int CAP>
-__device__ __forceinline__ bool visit_node(const SceneDev& S, const TopCache& top, uint32_t nx, uint32_t ny,
-                                           uint32_t nz, float ix, float iy, float iz, float oix, float oiy, float oiz,
-                                           float tnear, float lim, uint32_t& node, int& sp, uint32_t* stk,
-                                           uint32_t stride, uint32_t* spill) {
+__device__ __forceinline__ bool visit_node(const SceneDev& S, const TopCache& top, const SlabRay& r, float tnear,
+                                           float lim, uint32_t& node, int& sp, uint32_t* stk, uint32_t stride,
+                                           uint32_t* spill) {
     f32x4 ex, fx, ey, fy, ez, fz;
     u32x4 ch;
     if (node < top.count) {
-        const u32x4* nd = top.rows + __umul24(node, kNodeRows);  // node < top.count < 2^24: full-rate multiply
-        ex = __builtin_bit_cast(f32x4, nd[nx]);
-        fx = __builtin_bit_cast(f32x4, nd[1u - nx]);
-        ey = __builtin_bit_cast(f32x4, nd[2u + ny]);
-        fy = __builtin_bit_cast(f32x4, nd[3u - ny]);
-        ez = __builtin_bit_cast(f32x4, nd[4u + nz]);
-        fz = __builtin_bit_cast(f32x4, nd[5u - nz]);
+        // node < top.count < 2^24
+        const u32x4* nd = reinterpret_cast<const u32x4*>(reinterpret_cast<const char*>(top.rows) +
+                                                         mul_u24(node, kNodeRows * 16u));
+        ex = __builtin_bit_cast(f32x4, nd[r.nx]);
+        fx = __builtin_bit_cast(f32x4, nd[1u - r.nx]);
+        ey = __builtin_bit_cast(f32x4, nd[2u + r.ny]);
+        fy = __builtin_bit_cast(f32x4, nd[3u - r.ny]);
+        ez = __builtin_bit_cast(f32x4, nd[4u + r.nz]);
+        fz = __builtin_bit_cast(f32x4, nd[5u - r.nz]);
         ch = nd[6];
     } else {
         const gfloat4* nd = gmem(reinterpret_cast<const float4*>(S.nodes + node));
-        ex = nd[nx]; fx = nd[1u - nx]; ey = nd[2u + ny]; fy = nd[3u - ny]; ez = nd[4u + nz]; fz = nd[5u - nz];
+        ex = nd[r.nx]; fx = nd[1u - r.nx]; ey = nd[2u + r.ny]; fy = nd[3u - r.ny]; ez = nd[4u + r.nz]; fz = nd[5u - r.nz];
         ch = reinterpret_cast<const guint4*>(nd)[6];
     }
     float key[4];
     uint32_t cid[4] = {ch.x, ch.y, ch.z, ch.w};
-    box4(ex, fx, ey, fy, ez, fz, ix, iy, iz, oix, oiy, oiz, tnear, lim, key);
+    box4(ex, fx, ey, fy, ez, fz, r, tnear, lim, key);
     // The child ids are only used when a box was hit; this keeps their load with the box rows
     // instead of letting the compiler issue it (and wait for it) behind the test.
     asm volatile("" ::"v"(ch.x), "v"(ch.y), "v"(ch.z), "v"(ch.w));
@@ -208,9 +243,20 @@ __device__ __forceinline__ bool visit_node(const SceneDev& S, const TopCache& to
         node = stack_pop(sp, stk, stride, spill);
         return true;
     }
-#pragma unroll
-    for (int i = 3; i >= 1; i--)
-        if (key[i] != __builtin_inff() && sp < CAP) stack_push(cid[i], sp, stk, stride, spill);
+    const bool h1 = key[1] != __builtin_inff(), h2 = key[2] != __builtin_inff(), h3 = key[3] != __builtin_inff();
+    if (sp <= kLdsStack - 3) {
+        // The k = h1 + h2 + h3 other hits (sorted to the front) go to slots sp .. sp+k-1, farthest
+        // first. All three slots are written: the ones from sp+k up are above the new top.
+        uint32_t* p = stk + (uint32_t)sp * stride;
+        p[0] = h3 ? cid[3] : (h2 ? cid[2] : cid[1]);
+        p[stride] = h3 ? cid[2] : cid[1];
+        p[2u * stride] = cid[1];
+        sp += (int)h1 + (int)h2 + (int)h3;
+    } else {
+        if (h3 && sp < CAP) stack_push(cid[3], sp, stk, stride, spill);
+        if (h2 && sp < CAP) stack_push(cid[2], sp, stk, stride, spill);
+        if (h1 && sp < CAP) stack_push(cid[1], sp, stk, stride, spill);
+    }
     node = cid[0];
     return true;
 }
@@ -218,8 +264,7 @@ __device__ __forceinline__ bool visit_node(const SceneDev& S, const TopCache& to
 template <int CAP>
 __device__ __forceinline__ bool any_hit_node(const SceneDev& S, const TopCache& top, const TravRay& r, uint32_t& node,
                                              int& sp, uint32_t* stk, uint32_t stride, uint32_t* spill) {
-    return visit_node<CAP>(S, top, r.nx, r.ny, r.nz, r.ix, r.iy, r.iz, r.oix, r.oiy, r.oiz, r.w.tnear,
-                           r.w.tfar * kBoxScale, node, sp, stk, stride, spill);
+    return visit_node<CAP>(S, top, r.s, r.w.tnear, r.w.tfar * kBoxScale, node, sp, stk, stride, spill);
 }
 
 // One step of any-hit traversal: either one internal node or one leaf (its triangles tested).
@@ -246,14 +291,24 @@ __device__ __forceinline__ int any_hit_step(const SceneDev& S, const TopCache& t
 // A longer step of any-hit traversal in two phases: internal nodes until this lane holds a leaf
 // (a lane that reaches one first waits for the rest of its wavefront, so the triangle tests that
 // follow run with the lanes together), then one triangle per iteration through that leaf and
-// through any leaves popped right after it. Returns like any_hit_step; the set of nodes and
-// triangles a ray visits is the same (only the grouping into steps differs), so the occlusion
-// result is the same.
+// through any leaves popped right after it. After its first node, the node phase ends early for
+// the whole wavefront once no more than `min_lanes` of the calling lanes are still searching;
+// those lanes return 0 and continue on the next call (every call visits at least one node of
+// each searching lane, so repeated calls always finish). Returns like any_hit_step; the set of
+// nodes and triangles a ray visits is the same (only the grouping into steps differs), so the
+// occlusion result is the same.
 template <int CAP>
 __device__ __forceinline__ int any_hit_run(const SceneDev& S, const TopCache& top, const TravRay& r, uint32_t& node,
-                                           int& sp, uint32_t* stk, uint32_t stride, uint32_t* spill) {
-    while (!(node & RL_LEAF_BIT))
-        if (!any_hit_node<CAP>(S, top, r, node, sp, stk, stride, spill)) return -1;
+                                           int& sp, uint32_t* stk, uint32_t stride, uint32_t* spill,
+                                           uint32_t min_lanes) {
+    bool ended = false;
+    for (uint32_t floor = 0;; floor = min_lanes) {
+        const bool searching = !ended && !(node & RL_LEAF_BIT);
+        if ((uint32_t)__popcll(__ballot(searching)) <= floor) break;
+        if (searching && !any_hit_node<CAP>(S, top, r, node, sp, stk, stride, spill)) ended = true;
+    }
+    if (ended) return -1;
+    if (!(node & RL_LEAF_BIT)) return 0;
     uint32_t cur = node & RL_LEAF_FIRST_MASK;
     uint32_t end = cur + ((node >> 27) & 15u) + 1u;
     for (;;) {
@@ -301,8 +356,7 @@ __device__ __forceinline__ int closest_run(const SceneDev& S, const TopCache& to
         }
         if (__ballot(!has && !done) == 0ull) break;
         if (!done && !(node & RL_LEAF_BIT) &&
-            !visit_node<CAP>(S, top, r.nx, r.ny, r.nz, r.ix, r.iy, r.iz, r.oix, r.oiy, r.oiz, r.w.tnear,
-                             res.t * kBoxScale, node, sp, stk, stride, spill))
+            !visit_node<CAP>(S, top, r.s, r.w.tnear, res.t * kBoxScale, node, sp, stk, stride, spill))
             done = true;
     }
     if (!has) return -1;  // finished without reaching another leaf
@@ -336,23 +390,13 @@ __device__ __forceinline__ int closest_run(const SceneDev& S, const TopCache& to
 // consecutive entries are `stride` words apart (one column per lane, conflict-free). CAP is the
 // total stack capacity (LDS + spill), at least the BVH's worst-case bound.
 //
-// Slab test: t = fma(b, 1/d, -o/d). Its error is a relative term (covered by scaling the exit
-// distance by kBoxScale) plus an absolute shift of the slab plane of at most |o| * 2^-24, which
-// the builder's box padding (more than 2^-21 of the largest coordinate) covers, so no box the
-// exact ray crosses is rejected.
+// Slab test: as described at SlabRay.
 template <bool ANY, int CAP>
 __device__ bool traverse(const SceneDev& S, const TopCache& top, const Ray& ray, uint32_t* stk, uint32_t stride,
                          TraceResult& res) {
     const WtRay w = wt_setup(ray);
     const WtPerm wp = wt_perm(w);
-    // A zero direction component would give fma(b, inf, -inf) = NaN; the slab test uses
-    // +-1e-20 there instead (a displacement below 1e-17 m over any ray, inside the box padding).
-    const float dx = fabsf(ray.d.x) > 1e-20f ? ray.d.x : copysignf(1e-20f, ray.d.x);
-    const float dy = fabsf(ray.d.y) > 1e-20f ? ray.d.y : copysignf(1e-20f, ray.d.y);
-    const float dz = fabsf(ray.d.z) > 1e-20f ? ray.d.z : copysignf(1e-20f, ray.d.z);
-    const float ix = 1.0f / dx, iy = 1.0f / dy, iz = 1.0f / dz;
-    const float oix = -(ray.o.x * ix), oiy = -(ray.o.y * iy), oiz = -(ray.o.z * iz);
-    const uint32_t nx = ix < 0.0f ? 1u : 0u, ny = iy < 0.0f ? 1u : 0u, nz = iz < 0.0f ? 1u : 0u;
+    const SlabRay sr = slab_ray(ray);
     const float tnear = ray.tnear;
     float tlimit = ray.tfar;  // closest: shrinks to the best hit so far
     res.t = ray.tfar;
@@ -369,8 +413,7 @@ __device__ bool traverse(const SceneDev& S, const TopCache& top, const Ray& ray,
     int sp = 0;
     for (;;) {
         while (!(node & RL_LEAF_BIT))
-            if (!visit_node<CAP>(S, top, nx, ny, nz, ix, iy, iz, oix, oiy, oiz, tnear, tlimit * kBoxScale, node, sp, stk,
-                                 stride, spill))
+            if (!visit_node<CAP>(S, top, sr, tnear, tlimit * kBoxScale, node, sp, stk, stride, spill))
                 goto done;
         uint32_t cur = node & RL_LEAF_FIRST_MASK;
         uint32_t end = cur + ((node >> 27) & 15u) + 1u;
