@@ -225,6 +225,10 @@ __global__ void k_photon_gather(const PhotonMapDev* __restrict__ pms, int npm, u
 // per workgroup, so four fit a compute unit's 160 KB).
 constexpr int kFgBlock = 256;
 constexpr uint32_t kFgTopNodes = 192;
+// A closest-hit round's node phase stops once no more than this many lanes are still searching
+// without a leaf (heightfield final gather: 338 ms with no early stop, 327/328/328/330/345 ms at
+// 2/3/4/6/16 lanes, 8 to 32 idle lanes before a refill within 1 ms of each other at 4).
+constexpr uint32_t kFgNodeExit = 4;
 
 // IndirectLight::bakeLumel (baker/IndirectLight.cpp:47-75) with lane refill. A wavefront takes
 // lumels from the persistent queue and hands their (lumel, sample) pairs, in order, to idle lanes
@@ -329,7 +333,7 @@ __global__ __launch_bounds__(kFgBlock) void k_fg_refill(const BakeSeg* __restric
                 want &= ~__ballot(mine);
             }
         }
-        if (active && closest_run<CAP>(S, top, tr, wp, node, sp, stk, kFgBlock, spill, res) != 0) {
+        if (active && closest_run<CAP>(S, top, tr, wp, node, sp, stk, kFgBlock, spill, res, kFgNodeExit) != 0) {
             rays++;
             const BakeSeg& sg = segs[my_seg];
             const FinalGatherParams& P = sg.fg;
@@ -469,7 +473,7 @@ __global__ __launch_bounds__(kFgBlock) void k_fg_lanes(const BakeSeg* __restrict
                 ray_on = true;
             }
         }
-        if (ray_on && closest_run<CAP>(S, top, tr, wp, node, sp, stk, kFgBlock, spill, res) != 0) {
+        if (ray_on && closest_run<CAP>(S, top, tr, wp, node, sp, stk, kFgBlock, spill, res, kFgNodeExit) != 0) {
             rays++;
             ray_on = false;
             const BakeSeg& sg = segs[seg];

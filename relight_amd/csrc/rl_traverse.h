@@ -15,9 +15,10 @@
 namespace rl {
 namespace trav {
 
-// Slab-test safety factor 1 + 2*gamma(3) (Ize 2013): scales the exit distances so that the
-// rounded slab distances never reject a box the exact ray passes through.
-constexpr float kBoxScale = 1.0000004f;
+// Slab-test safety factor: scales the exit distances so that the rounded slab distances never
+// reject a box the exact ray passes through (1 + 5 * 2^-23 once rounded; the bound is derived
+// at SlabRay).
+constexpr float kBoxScale = 1.0000006f;
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 
@@ -93,12 +94,13 @@ __device__ __forceinline__ uint32_t rank_below(uint64_t mask) {
 // kBoxScale. nx/ny/nz: float4 offset (0 or 1) of the entry-plane row of each axis in a node (lo
 // for a positive direction, hi for a negative one).
 //
-// Rounding: an entry distance is the exact distance to its plane shifted by at most |o| * 2^-24,
-// times a factor within 2 * 2^-24 of 1; an exit distance is the exact distance to its plane
-// shifted by at most 3 |o| * 2^-24, times kBoxScale * (1 +- 3 * 2^-24). kBoxScale (1 + 6 * 2^-24
-// once rounded) covers the five relative roundings, and the builder's box padding (2^-18 of the
-// largest coordinate magnitude) covers the shifts for any origin within 20 times that magnitude,
-// so no box the exact ray crosses within (tnear, lim) is rejected.
+// Rounding: i is the hardware reciprocal (within 1 ulp, a relative 2^-23, of 1/d). An entry
+// distance is the exact distance to its plane shifted by at most |o| * 2^-24, times a factor
+// within 3 * 2^-24 of 1; an exit distance is the exact distance to its plane shifted by at most
+// 3 |o| * 2^-24, times kBoxScale * (1 +- 4 * 2^-24). kBoxScale (1 + 10 * 2^-24) covers the seven
+// relative roundings, and the builder's box padding (2^-18 of the largest coordinate magnitude)
+// covers the shifts for any origin within 20 times that magnitude, so no box the exact ray
+// crosses within (tnear, lim) is rejected.
 struct SlabRay {
     float ix, iy, iz, oix, oiy, oiz;
     float six, siy, siz, soix, soiy, soiz;
@@ -112,7 +114,7 @@ __device__ __forceinline__ SlabRay slab_ray(const Ray& ray) {
     const float dx = fabsf(ray.d.x) > 1e-20f ? ray.d.x : copysignf(1e-20f, ray.d.x);
     const float dy = fabsf(ray.d.y) > 1e-20f ? ray.d.y : copysignf(1e-20f, ray.d.y);
     const float dz = fabsf(ray.d.z) > 1e-20f ? ray.d.z : copysignf(1e-20f, ray.d.z);
-    r.ix = 1.0f / dx; r.iy = 1.0f / dy; r.iz = 1.0f / dz;
+    r.ix = __builtin_amdgcn_rcpf(dx); r.iy = __builtin_amdgcn_rcpf(dy); r.iz = __builtin_amdgcn_rcpf(dz);
     r.oix = -(ray.o.x * r.ix); r.oiy = -(ray.o.y * r.iy); r.oiz = -(ray.o.z * r.iz);
     r.six = r.ix * kBoxScale; r.siy = r.iy * kBoxScale; r.siz = r.iz * kBoxScale;
     r.soix = r.oix * kBoxScale; r.soiy = r.oiy * kBoxScale; r.soiz = r.oiz * kBoxScale;
@@ -333,33 +335,35 @@ __device__ __forceinline__ int any_hit_run(const SceneDev& S, const TopCache& to
 // In the node phase a lane that reaches a leaf holds it and goes on visiting nodes (speculatively:
 // its interval has not yet been shortened by that leaf's triangles) until it reaches a second leaf
 // or runs out of nodes; the phase ends when every lane holds a leaf or has finished (Aila and
-// Laine 2009, speculative while-while), so lanes do not idle while others search. The leaf phase
-// tests the held leaf's triangles, one per iteration, then any leaf the lane stopped at and the
-// leaves popped right after it. Returns 0 when the round ends at an internal node (call again),
-// -1 when traversal is finished: res then holds the closest hit (tri = 0xFFFFFFFF for a miss).
-// Start with node = 0, sp = 0 and res = {r.w.tfar, 0xFFFFFFFF, 0, 0, 0, 0}. Every node a lane
-// skips is culled against an interval no shorter than the final one, so the lane tests every
-// triangle that traverse<false>() could return, and the result (nearest t, ties to the smaller
-// id) equals traverse<false>()'s; rounds only let a caller give finished lanes new rays.
+// Laine 2009, speculative while-while), so lanes do not idle while others search, or, after its
+// first node, once no more than `min_lanes` lanes are still searching without a leaf (those return
+// 0 and continue on the next call). The leaf phase tests the held leaf's triangles, one per
+// iteration, then any leaf the lane stopped at and the leaves popped right after it. Returns 0
+// when the round ends at an internal node (call again), -1 when traversal is finished: res then
+// holds the closest hit (tri = 0xFFFFFFFF for a miss). Start with node = 0, sp = 0 and
+// res = {r.w.tfar, 0xFFFFFFFF, 0, 0, 0, 0}. Every node a lane skips is culled against an interval
+// no shorter than the final one, so the lane tests every triangle that traverse<false>() could
+// return, and the result (nearest t, ties to the smaller id) equals traverse<false>()'s; rounds
+// only let a caller give finished lanes new rays.
 template <int CAP>
 __device__ __forceinline__ int closest_run(const SceneDev& S, const TopCache& top, const TravRay& r, const WtPerm& p,
                                            uint32_t& node, int& sp, uint32_t* stk, uint32_t stride, uint32_t* spill,
-                                           TraceResult& res) {
+                                           TraceResult& res, uint32_t min_lanes) {
     uint32_t held = 0;
     bool has = false, done = false;
-    for (;;) {
+    for (uint32_t floor = 0;; floor = min_lanes) {
         if (!done && !has && (node & RL_LEAF_BIT)) {
             held = node;
             has = true;
             if (sp == 0) done = true;
             else node = stack_pop(sp, stk, stride, spill);
         }
-        if (__ballot(!has && !done) == 0ull) break;
+        if ((uint32_t)__popcll(__ballot(!has && !done)) <= floor) break;
         if (!done && !(node & RL_LEAF_BIT) &&
             !visit_node<CAP>(S, top, r.s, r.w.tnear, res.t * kBoxScale, node, sp, stk, stride, spill))
             done = true;
     }
-    if (!has) return -1;  // finished without reaching another leaf
+    if (!has) return done ? -1 : 0;  // finished without another leaf, or still searching
     uint32_t cur = held & RL_LEAF_FIRST_MASK;
     uint32_t end = cur + ((held >> 27) & 15u) + 1u;
     for (;;) {
