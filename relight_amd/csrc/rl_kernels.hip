@@ -190,11 +190,11 @@ constexpr int kRing = 32;
 // finished lanes take new rays instead of idling until the slowest ray of the wavefront reaches a
 // leaf (city AO: 399 ms with no early stop, 397/393/387/384/387/398 ms at 1/2/4/8/16/24 lanes).
 constexpr uint32_t kAoNodeExit = 8;
-// Workgroups of four wavefronts, each keeping the first 192 nodes of the first segment's scene in
-// LDS (21 KB; with the stacks and rings a workgroup takes 38 KB, so four fit a compute unit's
+// Workgroups of four wavefronts, each keeping the first 184 nodes of the first segment's scene in
+// LDS (23 KB; with the stacks and rings a workgroup takes 40 KB, so four fill a compute unit's
 // 160 KB: sixteen wavefronts, the occupancy the kernel's registers allow).
 constexpr int kAoBlock = 256;
-constexpr uint32_t kAoTopNodes = 192;
+constexpr uint32_t kAoTopNodes = 184;
 
 template <int CAP>
 __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restrict__ segs, int nsegs,
@@ -290,7 +290,9 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
             break;
         }
         if (active) {
-            const int st = any_hit_run<CAP>(scene, top, tr, node, sp, stk, kAoBlock, spill, pool_open ? kAoNodeExit : 0u);
+            const uint32_t floor = pool_open ? kAoNodeExit : 0u;
+            const int st = one_scene(scene) ? any_hit_run<CAP, true>(scene, top, tr, node, sp, stk, kAoBlock, spill, floor)
+                                            : any_hit_run<CAP, false>(scene, top, tr, node, sp, stk, kAoBlock, spill, floor);
             if (st != 0) {
                 rays++;
                 if (st > 0) atomicAdd(&ring_occ[wave][slot], 1u);

@@ -221,10 +221,10 @@ __global__ void k_photon_gather(const PhotonMapDev* __restrict__ pms, int npm, u
 }
 
 // Final-gather workgroups: four wavefronts, each workgroup keeping the first nodes of the first
-// segment's scene in LDS (kFgTopNodes for the lane-per-lumel kernel: 21 KB, with the stacks 37 KB
+// segment's scene in LDS (kFgTopNodes for the lane-per-lumel kernel: 23 KB, with the stacks 39 KB
 // per workgroup, so four fit a compute unit's 160 KB).
 constexpr int kFgBlock = 256;
-constexpr uint32_t kFgTopNodes = 192;
+constexpr uint32_t kFgTopNodes = 184;
 // A closest-hit round's node phase stops once no more than this many lanes are still searching
 // without a leaf (heightfield final gather: 338 ms with no early stop, 327/328/328/330/345 ms at
 // 2/3/4/6/16 lanes, 8 to 32 idle lanes before a refill within 1 ms of each other at 4).
@@ -240,9 +240,9 @@ constexpr uint32_t kFgNodeExit = 4;
 // does, and writes the lumel when its last sample has been added. A lane may only start pair j <
 // consumed + kFgRing.
 constexpr uint32_t kFgRing = 128;
-// LDS per workgroup: 16 KB stacks + 10 KB rings + 13 KB of the scene's first 120 nodes, so four
+// LDS per workgroup: 16 KB stacks + 10 KB rings + 13 KB of the scene's first 104 nodes, so four
 // workgroups (sixteen wavefronts) fit a compute unit's 160 KB.
-constexpr uint32_t kFgRefillTopNodes = 120;
+constexpr uint32_t kFgRefillTopNodes = 104;
 
 template <int CAP>
 __global__ __launch_bounds__(kFgBlock) void k_fg_refill(const BakeSeg* __restrict__ segs, int nsegs, uint64_t total,
@@ -333,7 +333,9 @@ __global__ __launch_bounds__(kFgBlock) void k_fg_refill(const BakeSeg* __restric
                 want &= ~__ballot(mine);
             }
         }
-        if (active && closest_run<CAP>(S, top, tr, wp, node, sp, stk, kFgBlock, spill, res, kFgNodeExit) != 0) {
+        if (active && (one_scene(S) ? closest_run<CAP, true>(S, top, tr, wp, node, sp, stk, kFgBlock, spill, res, kFgNodeExit)
+                                    : closest_run<CAP, false>(S, top, tr, wp, node, sp, stk, kFgBlock, spill, res,
+                                                              kFgNodeExit)) != 0) {
             rays++;
             const BakeSeg& sg = segs[my_seg];
             const FinalGatherParams& P = sg.fg;
@@ -399,7 +401,7 @@ __global__ __launch_bounds__(kFgBlock) void k_fg_refill(const BakeSeg* __restric
 // does, with no exchange between lanes. A lane traces in rounds (closest_run); lanes whose ray is
 // done wait until at least `refill_min` lanes of the wavefront are without a ray, then start their
 // next samples together (lanes whose lumel is done first take new lumels), so the ray set-up runs
-// for many lanes at once. LDS per workgroup: 16 KB stacks + 21 KB of the scene's first 192 nodes.
+// for many lanes at once. LDS per workgroup: 16 KB stacks + 23 KB of the scene's first 184 nodes.
 constexpr uint64_t kFgLanesMinPerLane = 8;  // lumels per resident lane for the lane-per-lumel kernel
 
 template <int CAP>
@@ -473,7 +475,9 @@ __global__ __launch_bounds__(kFgBlock) void k_fg_lanes(const BakeSeg* __restrict
                 ray_on = true;
             }
         }
-        if (ray_on && closest_run<CAP>(S, top, tr, wp, node, sp, stk, kFgBlock, spill, res, kFgNodeExit) != 0) {
+        if (ray_on && (one_scene(S) ? closest_run<CAP, true>(S, top, tr, wp, node, sp, stk, kFgBlock, spill, res, kFgNodeExit)
+                                    : closest_run<CAP, false>(S, top, tr, wp, node, sp, stk, kFgBlock, spill, res,
+                                                              kFgNodeExit)) != 0) {
             rays++;
             ray_on = false;
             const BakeSeg& sg = segs[seg];

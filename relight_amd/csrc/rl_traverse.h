@@ -22,6 +22,15 @@ constexpr float kBoxScale = 1.0000006f;
 
 __device__ __forceinline__ uint32_t lane_id() { return threadIdx.x & 63u; }
 
+// A 64-bit value equal in every active lane, moved to scalar registers so the compiler treats
+// branches on it as wave-uniform.
+__device__ __forceinline__ uint64_t wave_uniform(uint64_t v) {
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+
 // Global-memory views of the scene arrays. A pointer read from a struct is generic to the
 // compiler, and a generic (flat) load also counts against the LDS wait counter, so every wait for
 // a node or triangle fetch would also wait for the stack's LDS traffic.
@@ -31,6 +40,23 @@ typedef const f32x4 __attribute__((address_space(1))) gfloat4;
 typedef const u32x4 __attribute__((address_space(1))) guint4;
 typedef const float __attribute__((address_space(1))) gfloat;
 __device__ __forceinline__ const gfloat4* gmem(const float4* p) { return (const gfloat4*)p; }
+
+// The scene's array addresses taken from the first active lane into scalar registers, for callers
+// whose active lanes all trace the same scene (one_scene): loads then use a scalar base and a
+// 32-bit per-lane offset instead of per-lane 64-bit address arithmetic.
+__device__ __forceinline__ SceneDev uniform_scene(const SceneDev& S) {
+    SceneDev U = S;
+    U.nodes = reinterpret_cast<const Bvh4Node*>(wave_uniform(reinterpret_cast<uint64_t>(S.nodes)));
+    U.tris = reinterpret_cast<const float4*>(wave_uniform(reinterpret_cast<uint64_t>(S.tris)));
+    U.attr = reinterpret_cast<const TriAttr*>(wave_uniform(reinterpret_cast<uint64_t>(S.attr)));
+    return U;
+}
+
+// Whether every active lane holds the scene of the first active lane.
+__device__ __forceinline__ bool one_scene(const SceneDev& S) {
+    const uint64_t lead = wave_uniform(reinterpret_cast<uint64_t>(S.nodes));
+    return __ballot(reinterpret_cast<uint64_t>(S.nodes) != lead) == 0ull;
+}
 
 // The watertight test's axis permutation applied once per ray: byte offsets of the kx, ky and kz
 // components within a vertex, and the origin's components in that order. A triangle test then
@@ -91,8 +117,8 @@ __device__ __forceinline__ uint32_t rank_below(uint64_t mask) {
 
 // Per-lane constants of the slab test. Entry planes: t = fma(b, i, oi) with i = 1/d and
 // oi = -o/d per axis; exit planes: t = fma(b, si, soi) with the same two constants multiplied by
-// kBoxScale. nx/ny/nz: float4 offset (0 or 1) of the entry-plane row of each axis in a node (lo
-// for a positive direction, hi for a negative one).
+// kBoxScale. re*/rf*: byte offsets in a node of each axis's entry-plane row (lo for a positive
+// direction, hi for a negative one) and exit-plane row.
 //
 // Rounding: i is the hardware reciprocal (within 1 ulp, a relative 2^-23, of 1/d). An entry
 // distance is the exact distance to its plane shifted by at most |o| * 2^-24, times a factor
@@ -104,7 +130,7 @@ __device__ __forceinline__ uint32_t rank_below(uint64_t mask) {
 struct SlabRay {
     float ix, iy, iz, oix, oiy, oiz;
     float six, siy, siz, soix, soiy, soiz;
-    uint32_t nx, ny, nz;
+    uint32_t rex, rfx, rey, rfy, rez, rfz;
 };
 
 __device__ __forceinline__ SlabRay slab_ray(const Ray& ray) {
@@ -118,9 +144,13 @@ __device__ __forceinline__ SlabRay slab_ray(const Ray& ray) {
     r.oix = -(ray.o.x * r.ix); r.oiy = -(ray.o.y * r.iy); r.oiz = -(ray.o.z * r.iz);
     r.six = r.ix * kBoxScale; r.siy = r.iy * kBoxScale; r.siz = r.iz * kBoxScale;
     r.soix = r.oix * kBoxScale; r.soiy = r.oiy * kBoxScale; r.soiz = r.oiz * kBoxScale;
-    r.nx = r.ix < 0.0f ? 1u : 0u;
-    r.ny = r.iy < 0.0f ? 1u : 0u;
-    r.nz = r.iz < 0.0f ? 1u : 0u;
+    // Node rows: lox, hix, loy, hiy, loz, hiz (16 B each).
+    r.rex = r.ix < 0.0f ? 16u : 0u;
+    r.rfx = 16u - r.rex;
+    r.rey = r.iy < 0.0f ? 48u : 32u;
+    r.rfy = 80u - r.rey;
+    r.rez = r.iz < 0.0f ? 80u : 64u;
+    r.rfz = 144u - r.rez;
     return r;
 }
 
@@ -174,9 +204,10 @@ __device__ __forceinline__ uint32_t stack_pop(int& sp, const uint32_t* stk, uint
 }
 
 // The first nodes of a scene in breadth-first order (the levels nearest the root, which most rays
-// visit) copied into LDS by a workgroup: seven 16-B rows per node (the six plane rows and the child
-// ids). A node visit reads a cached node from LDS instead of through the vector-memory path.
-constexpr uint32_t kNodeRows = 7;
+// visit) copied into LDS by a workgroup, each node's 128 bytes as they are in memory, so a node's
+// rows lie at the same byte offsets in LDS as in the scene's node array and one offset computation
+// serves both. A node visit reads a cached node from LDS instead of through the vector-memory path.
+constexpr uint32_t kNodeRows = 8;
 
 struct TopCache {
     const u32x4* rows;  // LDS, kNodeRows rows per node
@@ -188,19 +219,8 @@ struct TopCache {
 __device__ __forceinline__ uint32_t load_top_nodes(const SceneDev& S, u32x4* rows, uint32_t max_nodes) {
     const uint32_t n = S.num_nodes < max_nodes ? S.num_nodes : max_nodes;
     const guint4* src = reinterpret_cast<const guint4*>(gmem(reinterpret_cast<const float4*>(S.nodes)));
-    for (uint32_t i = threadIdx.x; i < n * kNodeRows; i += blockDim.x) {
-        const uint32_t nd = i / kNodeRows, row = i - nd * kNodeRows;
-        rows[i] = src[8u * nd + row];
-    }
+    for (uint32_t i = threadIdx.x; i < n * kNodeRows; i += blockDim.x) rows[i] = src[i];
     return n;
-}
-
-// 24-bit unsigned multiply (full rate; the compiler otherwise turns a constant multiply whose
-// operand range it cannot bound into a quarter-rate 32-bit multiply).
-__device__ __forceinline__ uint32_t mul_u24(uint32_t a, uint32_t b) {
-    uint32_t r;
-    asm("v_mul_u32_u24 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
-    return r;
 }
 
 // One internal node: its four child boxes tested against (tnear, lim), hit children pushed
@@ -212,21 +232,28 @@ __device__ __forceinline__ bool visit_node(const SceneDev& S, const TopCache& to
                                            uint32_t* spill) {
     f32x4 ex, fx, ey, fy, ez, fz;
     u32x4 ch;
+    const uint32_t off = node << 7;  // byte offset of the node (node < 2^25)
+    const uint32_t aex = off + r.rex, afx = off + r.rfx, aey = off + r.rey, afy = off + r.rfy, aez = off + r.rez,
+                   afz = off + r.rfz;
     if (node < top.count) {
-        // node < top.count < 2^24
-        const u32x4* nd = reinterpret_cast<const u32x4*>(reinterpret_cast<const char*>(top.rows) +
-                                                         mul_u24(node, kNodeRows * 16u));
-        ex = __builtin_bit_cast(f32x4, nd[r.nx]);
-        fx = __builtin_bit_cast(f32x4, nd[1u - r.nx]);
-        ey = __builtin_bit_cast(f32x4, nd[2u + r.ny]);
-        fy = __builtin_bit_cast(f32x4, nd[3u - r.ny]);
-        ez = __builtin_bit_cast(f32x4, nd[4u + r.nz]);
-        fz = __builtin_bit_cast(f32x4, nd[5u - r.nz]);
-        ch = nd[6];
+        const char* b = reinterpret_cast<const char*>(top.rows);
+        ex = __builtin_bit_cast(f32x4, *reinterpret_cast<const u32x4*>(b + aex));
+        fx = __builtin_bit_cast(f32x4, *reinterpret_cast<const u32x4*>(b + afx));
+        ey = __builtin_bit_cast(f32x4, *reinterpret_cast<const u32x4*>(b + aey));
+        fy = __builtin_bit_cast(f32x4, *reinterpret_cast<const u32x4*>(b + afy));
+        ez = __builtin_bit_cast(f32x4, *reinterpret_cast<const u32x4*>(b + aez));
+        fz = __builtin_bit_cast(f32x4, *reinterpret_cast<const u32x4*>(b + afz));
+        ch = *reinterpret_cast<const u32x4*>(b + off + 96u);
     } else {
-        const gfloat4* nd = gmem(reinterpret_cast<const float4*>(S.nodes + node));
-        ex = nd[r.nx]; fx = nd[1u - r.nx]; ey = nd[2u + r.ny]; fy = nd[3u - r.ny]; ez = nd[4u + r.nz]; fz = nd[5u - r.nz];
-        ch = reinterpret_cast<const guint4*>(nd)[6];
+        typedef const char __attribute__((address_space(1))) gchar;
+        gchar* b = reinterpret_cast<gchar*>(gmem(reinterpret_cast<const float4*>(S.nodes)));
+        ex = *reinterpret_cast<const gfloat4*>(b + aex);
+        fx = *reinterpret_cast<const gfloat4*>(b + afx);
+        ey = *reinterpret_cast<const gfloat4*>(b + aey);
+        fy = *reinterpret_cast<const gfloat4*>(b + afy);
+        ez = *reinterpret_cast<const gfloat4*>(b + aez);
+        fz = *reinterpret_cast<const gfloat4*>(b + afz);
+        ch = *reinterpret_cast<const guint4*>(b + off + 96u);
     }
     float key[4];
     uint32_t cid[4] = {ch.x, ch.y, ch.z, ch.w};
@@ -299,10 +326,12 @@ __device__ __forceinline__ int any_hit_step(const SceneDev& S, const TopCache& t
 // each searching lane, so repeated calls always finish). Returns like any_hit_step; the set of
 // nodes and triangles a ray visits is the same (only the grouping into steps differs), so the
 // occlusion result is the same.
-template <int CAP>
-__device__ __forceinline__ int any_hit_run(const SceneDev& S, const TopCache& top, const TravRay& r, uint32_t& node,
+// UNI: the caller has checked one_scene(S0).
+template <int CAP, bool UNI>
+__device__ __forceinline__ int any_hit_run(const SceneDev& S0, const TopCache& top, const TravRay& r, uint32_t& node,
                                            int& sp, uint32_t* stk, uint32_t stride, uint32_t* spill,
                                            uint32_t min_lanes) {
+    const SceneDev S = UNI ? uniform_scene(S0) : S0;
     bool ended = false;
     for (uint32_t floor = 0;; floor = min_lanes) {
         const bool searching = !ended && !(node & RL_LEAF_BIT);
@@ -345,10 +374,12 @@ __device__ __forceinline__ int any_hit_run(const SceneDev& S, const TopCache& to
 // no shorter than the final one, so the lane tests every triangle that traverse<false>() could
 // return, and the result (nearest t, ties to the smaller id) equals traverse<false>()'s; rounds
 // only let a caller give finished lanes new rays.
-template <int CAP>
-__device__ __forceinline__ int closest_run(const SceneDev& S, const TopCache& top, const TravRay& r, const WtPerm& p,
+// UNI: the caller has checked one_scene(S0).
+template <int CAP, bool UNI>
+__device__ __forceinline__ int closest_run(const SceneDev& S0, const TopCache& top, const TravRay& r, const WtPerm& p,
                                            uint32_t& node, int& sp, uint32_t* stk, uint32_t stride, uint32_t* spill,
                                            TraceResult& res, uint32_t min_lanes) {
+    const SceneDev S = UNI ? uniform_scene(S0) : S0;
     uint32_t held = 0;
     bool has = false, done = false;
     for (uint32_t floor = 0;; floor = min_lanes) {
@@ -467,13 +498,6 @@ __device__ __forceinline__ int find_seg(const BakeSeg* segs, int nsegs, uint64_t
     return lo;
 }
 
-// A 64-bit value equal in every active lane, moved to scalar registers so the compiler treats
-// branches on it as wave-uniform.
-__device__ __forceinline__ uint64_t wave_uniform(uint64_t v) {
-    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
-    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
-    return ((uint64_t)hi << 32) | lo;
-}
 
 // Persistent work queue: a wavefront takes the next n items (wave_take returns the first index,
 // wave_next_chunk takes one) and every lane gets the index. A wavefront keeps taking until the
