@@ -218,6 +218,7 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
     int pseg = 0;
     uint32_t chunk_first = 0, chunk_count = 0, chunk_ns = 1;
     uint32_t pool_next = 0, pool_end = 0, stream_base = 0;
+    uint32_t pool_lc = 0, pool_si = 0;  // pool_next = pool_lc * chunk_ns + pool_si
     bool pool_open = true;
     // Lane state.
     bool active = false;
@@ -251,6 +252,7 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
                     chunk_count = min(chunk_lumels, sg.count - chunk_first);
                     chunk_ns = (uint32_t)sg.ao.samples;
                     pool_next = 0;
+                    pool_lc = pool_si = 0;
                     pool_end = chunk_count * chunk_ns;
                 }
                 uint64_t room = pool_end - pool_next;
@@ -263,9 +265,21 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
                 const uint32_t rank = rank_below(want);
                 const bool mine = ((want >> lane) & 1ull) && rank < take;
                 if (mine) {
-                    const BakeSeg& sg = segs[pseg];
-                    const uint32_t rr = pool_next + rank;
-                    const uint32_t lc = rr / chunk_ns, si = rr % chunk_ns;
+                    // The segment index is the same in every lane: read as such, the segment's
+                    // fields load through the scalar path.
+                    const BakeSeg& sg = segs[__builtin_amdgcn_readfirstlane(pseg)];
+                    // Pair pool_next + rank as (lumel, sample): with at least 64 samples per lumel
+                    // it is at most one lumel past (pool_lc, pool_si).
+                    const uint32_t t = pool_si + rank;
+                    uint32_t lc, si;
+                    if (chunk_ns >= 64u) {
+                        const bool wrap = t >= chunk_ns;
+                        lc = pool_lc + (wrap ? 1u : 0u);
+                        si = wrap ? t - chunk_ns : t;
+                    } else {
+                        lc = pool_lc + t / chunk_ns;
+                        si = t % chunk_ns;
+                    }
                     const uint32_t li = chunk_first + lc;
                     const float4 p4 = sg.pos[li], n4 = sg.nrm[li];
                     const V3 p = v3(p4.x, p4.y, p4.z), n = v3(n4.x, n4.y, n4.z);
@@ -283,6 +297,13 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
                     active = true;
                 }
                 pool_next += take;
+                pool_si += take;
+                if (chunk_ns >= 64u) {
+                    if (pool_si >= chunk_ns) { pool_si -= chunk_ns; pool_lc++; }
+                } else {
+                    pool_lc += pool_si / chunk_ns;
+                    pool_si %= chunk_ns;
+                }
                 want &= ~__ballot(mine);
             }
             if (!pool_open && __ballot(active) == 0ull) break;
