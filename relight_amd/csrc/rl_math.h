@@ -87,12 +87,22 @@ enum Technique : uint32_t {
     TECH_FORM_FACTOR = 4,
 };
 
+// The initial state shared by a texel's streams: (bake seed, lightmap seed, texel) spread by
+// SplitMix64.
+RL_HD uint64_t texel_key(uint32_t seed, uint32_t lm_seed, uint32_t texel) {
+    const uint64_t key = splitmix64(((uint64_t)seed << 32) | (uint64_t)lm_seed);
+    return splitmix64(key ^ (uint64_t)texel);
+}
+
+// The stream of one (technique, sample) of a texel, from its texel_key.
+RL_HD Pcg32 texel_stream(uint64_t key, uint32_t tech, uint32_t sample) {
+    return pcg32_seed(key, ((uint64_t)tech << 32) | (uint64_t)sample);
+}
+
 // One independent stream per (bake seed, lightmap seed, texel, technique, sample).
 RL_HD Pcg32 sample_stream(uint32_t seed, uint32_t lm_seed, uint32_t texel, uint32_t tech,
                           uint32_t sample) {
-    uint64_t key = splitmix64(((uint64_t)seed << 32) | (uint64_t)lm_seed);
-    key = splitmix64(key ^ (uint64_t)texel);
-    return pcg32_seed(key, ((uint64_t)tech << 32) | (uint64_t)sample);
+    return texel_stream(texel_key(seed, lm_seed, texel), tech, sample);
 }
 
 // ---------------------------------------------------------------- hemisphere sampling
@@ -125,7 +135,23 @@ RL_HD V3 sample_uniform_hemisphere(Pcg32& r, V3 n) {
 // Cosine-weighted direction about n: a point (x,y) uniform in the unit disk lifted to
 // z = sqrt(1 - x^2 - y^2) (Malley), rotated into the frame of n with the branchless orthonormal
 // basis of Duff et al., "Building an Orthonormal Basis, Revisited", JCGT 6(1), 2017.
-RL_HD V3 sample_cosine_hemisphere(Pcg32& r, V3 n) {
+// The orthonormal frame (t1, t2, n) the cosine sampler rotates into.
+struct CosineFrame {
+    V3 t1, t2, n;
+};
+
+RL_HD CosineFrame cosine_frame(V3 n) {
+    CosineFrame f;
+    float s = n.z >= 0.0f ? 1.0f : -1.0f;
+    float a = -1.0f / (s + n.z);
+    float b = n.x * n.y * a;
+    f.t1 = v3(1.0f + s * n.x * n.x * a, s * b, -s * n.x);
+    f.t2 = v3(b, s + n.y * n.y * a, -n.y);
+    f.n = n;
+    return f;
+}
+
+RL_HD V3 sample_cosine_in_frame(Pcg32& r, const CosineFrame& f) {
     float x, y, r2;
     do {
         x = pcg32_float(r) * 2.0f - 1.0f;
@@ -133,15 +159,12 @@ RL_HD V3 sample_cosine_hemisphere(Pcg32& r, V3 n) {
         r2 = x * x + y * y;
     } while (r2 >= 1.0f);
     float z = sqrtf(1.0f - r2);
-    float s = n.z >= 0.0f ? 1.0f : -1.0f;
-    float a = -1.0f / (s + n.z);
-    float b = n.x * n.y * a;
-    V3 t1 = v3(1.0f + s * n.x * n.x * a, s * b, -s * n.x);
-    V3 t2 = v3(b, s + n.y * n.y * a, -n.y);
-    return v3(t1.x * x + t2.x * y + n.x * z,
-              t1.y * x + t2.y * y + n.y * z,
-              t1.z * x + t2.z * y + n.z * z);
+    return v3(f.t1.x * x + f.t2.x * y + f.n.x * z,
+              f.t1.y * x + f.t2.y * y + f.n.y * z,
+              f.t1.z * x + f.t2.z * y + f.n.z * z);
 }
+
+RL_HD V3 sample_cosine_hemisphere(Pcg32& r, V3 n) { return sample_cosine_in_frame(r, cosine_frame(n)); }
 
 RL_HD V3 sample_hemisphere(Pcg32& r, V3 n, int mode) {
     return mode == SAMPLE_COSINE ? sample_cosine_hemisphere(r, n) : sample_uniform_hemisphere(r, n);

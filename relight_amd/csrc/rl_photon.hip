@@ -422,6 +422,13 @@ __global__ __launch_bounds__(kFgBlock) void k_fg_lanes(const BakeSeg* __restrict
     bool has_lumel = false, ray_on = false;
     uint32_t g = 0, seg = 0, s = 0, nsamples = 0;
     float ar = 0.0f, ag = 0.0f, ab = 0.0f;
+    // The lumel's position, sampling frame, texel key and settings, read and computed once for
+    // all its samples.
+    V3 lp = v3(0.0f, 0.0f, 0.0f);
+    CosineFrame frame{};
+    uint64_t key = 0;
+    float fg_distance = 0.0f;
+    int32_t fg_sampling = 0;
     TravRay tr;
     WtPerm wp;
     SceneDev S;
@@ -446,27 +453,31 @@ __global__ __launch_bounds__(kFgBlock) void k_fg_lanes(const BakeSeg* __restrict
                         has_lumel = true;
                         g = (uint32_t)mine;
                         seg = (uint32_t)find_seg(segs, nsegs, mine);
+                        const BakeSeg& sg = segs[seg];
                         s = 0;
-                        nsamples = (uint32_t)segs[seg].fg.samples;
+                        nsamples = (uint32_t)sg.fg.samples;
                         ar = ag = ab = 0.0f;
+                        const uint32_t i = g - (uint32_t)sg.work_begin;
+                        const float4 p4 = sg.pos[i], n4 = sg.nrm[i];
+                        lp = v3(p4.x, p4.y, p4.z);
+                        frame = cosine_frame(v3(n4.x, n4.y, n4.z));
+                        key = texel_key(sg.fg.seed, sg.lm_seed, f_bits(p4.w));
+                        fg_distance = sg.fg.distance;
+                        fg_sampling = sg.fg.sampling;
+                        S = sg.scene;
+                        top.count = S.nodes == top_scene.nodes ? top_count : 0u;
                     }
                 }
             }
             if (!ray_on && has_lumel) {
-                const BakeSeg& sg = segs[seg];
-                const FinalGatherParams& P = sg.fg;
-                const uint32_t i = g - (uint32_t)sg.work_begin;
-                const float4 p4 = sg.pos[i], n4 = sg.nrm[i];
-                const V3 p = v3(p4.x, p4.y, p4.z), n = v3(n4.x, n4.y, n4.z);
-                Pcg32 rng = sample_stream(P.seed, sg.lm_seed, f_bits(p4.w), TECH_FINAL_GATHER, s);
-                dir = sample_hemisphere(rng, n, P.sampling);
-                const float d0 = dot(n, dir);
+                Pcg32 rng = texel_stream(key, TECH_FINAL_GATHER, s);
+                dir = fg_sampling == SAMPLE_COSINE ? sample_cosine_in_frame(rng, frame)
+                                                   : sample_uniform_hemisphere(rng, frame.n);
+                const float d0 = dot(frame.n, dir);
                 influence = d0 > 0.0f ? d0 : 0.0f;
-                const Ray ray = ray_from_segment(p, p + dir * P.distance);
+                const Ray ray = ray_from_segment(lp, lp + dir * fg_distance);
                 tr = trav_ray(ray);
                 wp = wt_perm(tr.w);
-                S = sg.scene;
-                top.count = S.nodes == top_scene.nodes ? top_count : 0u;
                 node = 0;
                 sp = 0;
                 res.t = ray.tfar;
