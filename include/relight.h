@@ -413,7 +413,8 @@ int32_t rl_server_destroy(rl_server* server);
 
 /* Launch tuning of a context (no counterpart in the reference; chosen by the autotuner in
  * relight_amd/autotune.py). ao_kernel: RL_AO_KERNEL_REFILL (lanes take new rays from a per-
- * wavefront pool when at least refill_min of them are idle; chunks of ao_chunk_lumels lumels) or
+ * wavefront pool when at least refill_min of them are idle; chunks of ao_chunk_lumels lumels, at
+ * most 16) or
  * RL_AO_KERNEL_CHUNKED (64 rays per wavefront per chunk). blocks_per_cu sizes the persistent
  * grids (128-thread blocks). Results do not depend on the tuning. */
 #define RL_AO_KERNEL_CHUNKED 0

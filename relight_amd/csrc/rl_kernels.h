@@ -27,7 +27,8 @@ hipError_t launch_ao_count(hipStream_t st, int cap, uint32_t max_blocks, const B
                            int nsegs, uint64_t total_pairs, unsigned long long* queue,
                            unsigned long long* rays);
 
-// AO with lane refill (counts written, not accumulated).
+// AO with lane refill (counts written, not accumulated); chunks of at most kAoChunkMax lumels.
+constexpr uint32_t kAoChunkMax = 16;
 hipError_t launch_ao_refill(hipStream_t st, int cap, uint32_t max_blocks, const BakeSeg* segs,
                             int nsegs, uint64_t total_chunks, uint32_t chunk_lumels,
                             uint32_t refill_min, unsigned long long* queue,

@@ -207,11 +207,11 @@ constexpr int kRing = 32;
 // finished lanes take new rays instead of idling until the slowest ray of the wavefront reaches a
 // leaf (city AO: 399 ms with no early stop, 397/393/387/384/387/398 ms at 1/2/4/8/16/24 lanes).
 constexpr uint32_t kAoNodeExit = 8;
-// Workgroups of four wavefronts, each keeping the first 184 nodes of the first segment's scene in
-// LDS (23 KB; with the stacks and rings a workgroup takes 40 KB, so four fill a compute unit's
-// 160 KB: sixteen wavefronts, the occupancy the kernel's registers allow).
+// Workgroups of four wavefronts, each keeping the first 152 nodes of the first segment's scene in
+// LDS (19 KB; with the stacks, rings and lumel tables a workgroup takes 40 KB, so four fill a
+// compute unit's 160 KB: sixteen wavefronts, the occupancy the kernel's registers allow).
 constexpr int kAoBlock = 256;
-constexpr uint32_t kAoTopNodes = 184;
+constexpr uint32_t kAoTopNodes = 152;
 
 template <int CAP>
 __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restrict__ segs, int nsegs,
@@ -222,6 +222,9 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
     __shared__ uint32_t ring_occ[kAoBlock / 64][kRing];
     __shared__ uint32_t ring_done[kAoBlock / 64][kRing];
     __shared__ u32x4 top_rows[kAoTopNodes * kNodeRows];
+    // Per wavefront, the current chunk's lumels: position and texel key, cosine-sampling frame
+    // (t1, t2, n), computed once per lumel when the chunk is taken instead of once per ray.
+    __shared__ u32x4 lumel_rows[kAoBlock / 64][kAoChunkMax][4];
     const SceneDev top_scene = segs[0].scene;
     const uint32_t top_count = load_top_nodes(top_scene, top_rows, kAoTopNodes);
     uint32_t* stk = stack + threadIdx.x;
@@ -271,6 +274,20 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
                     pool_next = 0;
                     pool_lc = pool_si = 0;
                     pool_end = chunk_count * chunk_ns;
+                    if (lane < chunk_count) {  // chunk_count <= kAoChunkMax (launcher)
+                        const uint32_t li = chunk_first + lane;
+                        const float4 p4 = sg.pos[li], n4 = sg.nrm[li];
+                        const uint64_t key = texel_key(sg.ao.seed, sg.lm_seed, f_bits(p4.w));
+                        const CosineFrame f = cosine_frame(v3(n4.x, n4.y, n4.z));
+                        u32x4* row = lumel_rows[wave][lane];
+                        row[0] = u32x4{f_bits(p4.x), f_bits(p4.y), f_bits(p4.z), (uint32_t)key};
+                        row[1] = u32x4{f_bits(f.t1.x), f_bits(f.t1.y), f_bits(f.t1.z), (uint32_t)(key >> 32)};
+                        row[2] = u32x4{f_bits(f.t2.x), f_bits(f.t2.y), f_bits(f.t2.z), 0u};
+                        row[3] = u32x4{f_bits(n4.x), f_bits(n4.y), f_bits(n4.z), 0u};
+                    }
+                    // Other lanes of this wavefront read the rows below; the wavefront's LDS
+                    // operations complete in order.
+                    __builtin_amdgcn_wave_barrier();
                 }
                 uint64_t room = pool_end - pool_next;
                 if (allow != ~0ull) {
@@ -298,10 +315,16 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
                         si = t % chunk_ns;
                     }
                     const uint32_t li = chunk_first + lc;
-                    const float4 p4 = sg.pos[li], n4 = sg.nrm[li];
-                    const V3 p = v3(p4.x, p4.y, p4.z), n = v3(n4.x, n4.y, n4.z);
-                    Pcg32 rng = sample_stream(sg.ao.seed, sg.lm_seed, f_bits(p4.w), TECH_AO, si);
-                    const V3 dir = sample_hemisphere(rng, n, sg.ao.sampling);
+                    const u32x4* row = lumel_rows[wave][lc];
+                    const u32x4 r0 = row[0], r1 = row[1], r2 = row[2], r3 = row[3];
+                    const V3 p = v3(bits_f(r0.x), bits_f(r0.y), bits_f(r0.z));
+                    CosineFrame f;
+                    f.t1 = v3(bits_f(r1.x), bits_f(r1.y), bits_f(r1.z));
+                    f.t2 = v3(bits_f(r2.x), bits_f(r2.y), bits_f(r2.z));
+                    f.n = v3(bits_f(r3.x), bits_f(r3.y), bits_f(r3.z));
+                    Pcg32 rng = texel_stream(((uint64_t)r1.w << 32) | r0.w, TECH_AO, si);
+                    const V3 dir = sg.ao.sampling == SAMPLE_COSINE ? sample_cosine_in_frame(rng, f)
+                                                                   : sample_uniform_hemisphere(rng, f.n);
                     tr = trav_ray(ray_from_segment(p, p + dir * sg.ao.max_distance));
                     scene = sg.scene;
                     top.count = sg.scene.nodes == top_scene.nodes ? top_count : 0u;
@@ -460,7 +483,8 @@ hipError_t launch_ao_refill(hipStream_t st, int cap, uint32_t max_blocks, const 
                             uint64_t total_chunks, uint32_t chunk_lumels, uint32_t refill_min,
                             unsigned long long* queue, unsigned long long* rays) {
     if (total_chunks == 0) return hipSuccess;
-    if (chunk_lumels == 0 || refill_min == 0 || refill_min > 64) return hipErrorInvalidValue;
+    if (chunk_lumels == 0 || chunk_lumels > kAoChunkMax || refill_min == 0 || refill_min > 64)
+        return hipErrorInvalidValue;
     hipError_t e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(queue), 2, 0u);
     if (e != hipSuccess) return e;
     // max_blocks counts kBlock-thread workgroups; this kernel's are kAoBlock threads.

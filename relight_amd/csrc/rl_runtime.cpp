@@ -516,12 +516,17 @@ int32_t run_direct_items(rl_relight* ctx, const std::vector<BakeItem>& items, bo
     return RL_SUCCESS;
 }
 
+// Lumels per AO work chunk: the refill kernel's chunks hold at most kAoChunkMax lumels.
+uint32_t ao_chunk(const rl_relight* ctx, bool refill) {
+    return refill ? std::min(ctx->tuning.ao_chunk_lumels, kAoChunkMax) : ctx->tuning.ao_chunk_lumels;
+}
+
 // AO over items: count pass (lane refill when every item has >= 4 samples, else chunked) and
 // the finalize pass.
 int32_t run_ao_items(rl_relight* ctx, const std::vector<BakeItem>& items, bool sync = true) {
     bool refill = ctx->tuning.ao_kernel == RL_AO_KERNEL_REFILL;
     for (const BakeItem& it : items) refill = refill && it.ao.samples >= 4;
-    const uint32_t chunk = ctx->tuning.ao_chunk_lumels;
+    const uint32_t chunk = ao_chunk(ctx, refill);
     DeviceSegs work, lum;
     int32_t st = upload_segments(ctx, kSegAoWork, items, refill ? kUnitChunk : kUnitPair, chunk, work);
     if (st) return st;
@@ -1297,7 +1302,7 @@ int32_t rl_pass_create(rl_relight* ctx, rl_scene* scene, rl_lightmap* const* map
     // The same kernel choice as a direct bake_batch call, made once at recording.
     bool refill = ctx->tuning.ao_kernel == RL_AO_KERNEL_REFILL;
     for (const BakeItem& it : items) refill = refill && it.ao.samples >= 4;
-    const uint32_t chunk = ctx->tuning.ao_chunk_lumels;
+    const uint32_t chunk = ao_chunk(ctx, refill);
     DeviceSegs dir, work, lum;
     if (do_direct && (st = pass_upload(build_segments(items, kUnitLumel, 1, dir), &p->d_segs[0]))) return st;
     if (ao) {

@@ -311,14 +311,18 @@ def test_axis_parallel_segments_agree(relight):
     assert np.array_equal(h["triangle"], S.trace(a, b, closest=True)["triangle"])
 
 
-@pytest.mark.parametrize("samples", [4, 5, 48, 100, 256])
-def test_ao_refill_equals_chunked_and_oracle(relight, samples):
+@pytest.mark.parametrize("samples,sampling", [(4, "cosine"), (5, "cosine"), (48, "cosine"), (100, "cosine"),
+                                              (256, "cosine"), (5, "uniform"), (100, "uniform")])
+def test_ao_refill_equals_chunked_and_oracle(relight, samples, sampling):
     """The lane-refill AO kernel against the chunked kernel and the oracle, for sample counts
-    below, between and above multiples of 64, under several refill thresholds and chunk sizes."""
-    print(f"[gpu] AO refill vs chunked, samples={samples}", flush=True)
-    from relight_amd import RL_SAMPLE_COSINE
+    below, between and above multiples of 64, under several refill thresholds and chunk sizes
+    (64 is above the refill kernel's 16-lumel chunk limit, which the runtime applies), with cosine
+    and uniform hemisphere sampling."""
+    print(f"[gpu] AO refill vs chunked, samples={samples}, {sampling}", flush=True)
+    from relight_amd import RL_SAMPLE_COSINE, RL_SAMPLE_UNIFORM
     spec = scenes.city(num_boxes=300, tiles=(2, 2), side_range=(48, 96))
-    ao = AmbientOcclusionSettings(samples, 0.8, 4.0, 1.0, 3, RL_SAMPLE_COSINE)
+    ao = AmbientOcclusionSettings(samples, 0.8, 4.0, 1.0, 3,
+                                  RL_SAMPLE_COSINE if sampling == "cosine" else RL_SAMPLE_UNIFORM)
     outs = []
     for kernel, rmin, chunk in [(0, 16, 8), (1, 1, 1), (1, 16, 8), (1, 64, 3), (1, 8, 64)]:
         relight.set_tuning(ao_kernel=kernel, refill_min=rmin, ao_chunk_lumels=chunk)
