@@ -27,13 +27,17 @@ def candidates(grid=None):
     return out
 
 
-def autotune(relight, scene, lightmaps, ao: AmbientOcclusionSettings, probe_samples=32,
+def autotune(relight, scene, lightmaps, ao: AmbientOcclusionSettings, probe_samples=None,
              grid=None, repeats=1):
     """Times an AO pass with `probe_samples` rays per texel for each candidate; keeps the fastest.
-    Returns (best tuning, [(tuning, ms)]).
+    Returns (best tuning, [(tuning, ms)]). By default the probe uses min(ao.samples, 64) rays per
+    texel: the refill kernel pairs rays with lumels one way for 64 or more samples per lumel and
+    another way below, so a probe of at least 64 times the production path of a large pass.
 
     The probe passes bake AO into `lightmaps`: AO multiplies the texel colour, so call this
     before baking them (zero colours stay zero)."""
+    if probe_samples is None:
+        probe_samples = min(int(ao.samples), 64)
     probe = AmbientOcclusionSettings(probe_samples, ao.occluded_fraction, ao.max_distance,
                                      ao.exponent, ao.seed, ao.sampling)
     results = []
