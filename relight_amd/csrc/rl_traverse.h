@@ -333,9 +333,9 @@ __device__ __forceinline__ int any_hit_run(const SceneDev& S0, const TopCache& t
                                            uint32_t min_lanes) {
     const SceneDev S = UNI ? uniform_scene(S0) : S0;
     bool ended = false;
-    for (uint32_t floor = 0;; floor = min_lanes) {
+    for (uint32_t stop_at = 0;; stop_at = min_lanes) {
         const bool searching = !ended && !(node & RL_LEAF_BIT);
-        if ((uint32_t)__popcll(__ballot(searching)) <= floor) break;
+        if ((uint32_t)__popcll(__ballot(searching)) <= stop_at) break;
         if (searching && !any_hit_node<CAP>(S, top, r, node, sp, stk, stride, spill)) ended = true;
     }
     if (ended) return -1;
@@ -382,14 +382,14 @@ __device__ __forceinline__ int closest_run(const SceneDev& S0, const TopCache& t
     const SceneDev S = UNI ? uniform_scene(S0) : S0;
     uint32_t held = 0;
     bool has = false, done = false;
-    for (uint32_t floor = 0;; floor = min_lanes) {
+    for (uint32_t stop_at = 0;; stop_at = min_lanes) {
         if (!done && !has && (node & RL_LEAF_BIT)) {
             held = node;
             has = true;
             if (sp == 0) done = true;
             else node = stack_pop(sp, stk, stride, spill);
         }
-        if ((uint32_t)__popcll(__ballot(!has && !done)) <= floor) break;
+        if ((uint32_t)__popcll(__ballot(!has && !done)) <= stop_at) break;
         if (!done && !(node & RL_LEAF_BIT) &&
             !visit_node<CAP>(S, top, r.s, r.w.tnear, res.t * kBoxScale, node, sp, stk, stride, spill))
             done = true;
