@@ -352,8 +352,9 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
         }
         if (active) {
             const uint32_t min_lanes = pool_open ? kAoNodeExit : 0u;
-            const int st = one_scene(scene) ? any_hit_run<CAP, true>(scene, top, tr, node, sp, stk, kAoBlock, spill, min_lanes)
-                                            : any_hit_run<CAP, false>(scene, top, tr, node, sp, stk, kAoBlock, spill, min_lanes);
+            const int st = one_scene(scene)
+                               ? any_hit_run<CAP, true>(scene, top, tr, node, sp, stk, kAoBlock, spill, min_lanes)
+                               : any_hit_run<CAP, false>(scene, top, tr, node, sp, stk, kAoBlock, spill, min_lanes);
             if (st != 0) {
                 rays++;
                 if (st > 0) atomicAdd(&ring_occ[wave][slot], 1u);
