@@ -229,6 +229,11 @@ constexpr uint32_t kFgTopNodes = 184;
 // without a leaf (heightfield final gather: 338 ms with no early stop, 327/328/328/330/345 ms at
 // 2/3/4/6/16 lanes, 8 to 32 idle lanes before a refill within 1 ms of each other at 4).
 constexpr uint32_t kFgNodeExit = 4;
+// Likewise its leaf phase, once no more than this many lanes still have triangles to test (the
+// rest of their leaf carries over). Heightfield final gather: 299 ms with a per-lane leaf loop;
+// this loop 307/290/289/288/292/293 ms at 0/8/12/16/20/24 lanes; at 16, a node-phase stop at 2
+// or 8 lanes instead of 4: 288/293 ms.
+constexpr uint32_t kFgLeafExit = 16;
 
 // IndirectLight::bakeLumel (baker/IndirectLight.cpp:47-75) with lane refill. A wavefront takes
 // lumels from the persistent queue and hands their (lumel, sample) pairs, in order, to idle lanes
@@ -275,7 +280,7 @@ __global__ __launch_bounds__(kFgBlock) void k_fg_refill(const BakeSeg* __restric
     SceneDev S;
     TopCache top{top_rows, 0u};
     TraceResult res;
-    uint32_t node = 0, my_j = 0, my_seg = 0, my_last = 0;
+    uint32_t node = 0, pend = 0, my_j = 0, my_seg = 0, my_last = 0;
     int sp = 0;
     V3 dir = v3(0.0f, 0.0f, 0.0f);
     float influence = 0.0f;
@@ -322,6 +327,7 @@ __global__ __launch_bounds__(kFgBlock) void k_fg_refill(const BakeSeg* __restric
                     S = sg.scene;
                     top.count = S.nodes == top_scene.nodes ? top_count : 0u;
                     node = 0;
+                    pend = 0;
                     sp = 0;
                     res.t = ray.tfar;
                     res.tri = 0xFFFFFFFFu;
@@ -333,9 +339,10 @@ __global__ __launch_bounds__(kFgBlock) void k_fg_refill(const BakeSeg* __restric
                 want &= ~__ballot(mine);
             }
         }
-        if (active && (one_scene(S) ? closest_run<CAP, true>(S, top, tr, wp, node, sp, stk, kFgBlock, spill, res, kFgNodeExit)
-                                    : closest_run<CAP, false>(S, top, tr, wp, node, sp, stk, kFgBlock, spill, res,
-                                                              kFgNodeExit)) != 0) {
+        if (active && (one_scene(S) ? closest_run<CAP, true>(S, top, tr, wp, node, sp, stk, kFgBlock, spill, res, pend,
+                                                             kFgNodeExit, kFgLeafExit)
+                                    : closest_run<CAP, false>(S, top, tr, wp, node, sp, stk, kFgBlock, spill, res, pend,
+                                                              kFgNodeExit, kFgLeafExit)) != 0) {
             rays++;
             const BakeSeg& sg = segs[my_seg];
             const FinalGatherParams& P = sg.fg;
@@ -434,7 +441,7 @@ __global__ __launch_bounds__(kFgBlock) void k_fg_lanes(const BakeSeg* __restrict
     SceneDev S;
     TopCache top{top_rows, 0u};
     TraceResult res;
-    uint32_t node = 0;
+    uint32_t node = 0, pend = 0;
     int sp = 0;
     V3 dir = v3(0.0f, 0.0f, 0.0f);
     float influence = 0.0f;
@@ -479,6 +486,7 @@ __global__ __launch_bounds__(kFgBlock) void k_fg_lanes(const BakeSeg* __restrict
                 tr = trav_ray(ray);
                 wp = wt_perm(tr.w);
                 node = 0;
+                pend = 0;
                 sp = 0;
                 res.t = ray.tfar;
                 res.tri = 0xFFFFFFFFu;
@@ -486,9 +494,10 @@ __global__ __launch_bounds__(kFgBlock) void k_fg_lanes(const BakeSeg* __restrict
                 ray_on = true;
             }
         }
-        if (ray_on && (one_scene(S) ? closest_run<CAP, true>(S, top, tr, wp, node, sp, stk, kFgBlock, spill, res, kFgNodeExit)
-                                    : closest_run<CAP, false>(S, top, tr, wp, node, sp, stk, kFgBlock, spill, res,
-                                                              kFgNodeExit)) != 0) {
+        if (ray_on && (one_scene(S) ? closest_run<CAP, true>(S, top, tr, wp, node, sp, stk, kFgBlock, spill, res, pend,
+                                                             kFgNodeExit, kFgLeafExit)
+                                    : closest_run<CAP, false>(S, top, tr, wp, node, sp, stk, kFgBlock, spill, res, pend,
+                                                              kFgNodeExit, kFgLeafExit)) != 0) {
             rays++;
             ray_on = false;
             const BakeSeg& sg = segs[seg];

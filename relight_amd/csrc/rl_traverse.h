@@ -378,10 +378,11 @@ __device__ __forceinline__ int any_hit_run(const SceneDev& S0, const TopCache& t
 template <int CAP, bool UNI>
 __device__ __forceinline__ int closest_run(const SceneDev& S0, const TopCache& top, const TravRay& r, const WtPerm& p,
                                            uint32_t& node, int& sp, uint32_t* stk, uint32_t stride, uint32_t* spill,
-                                           TraceResult& res, uint32_t min_lanes) {
+                                           TraceResult& res, uint32_t& pend, uint32_t min_lanes, uint32_t leaf_lanes) {
     const SceneDev S = UNI ? uniform_scene(S0) : S0;
-    uint32_t held = 0;
-    bool has = false, done = false;
+    uint32_t held = pend;  // the rest of a leaf an earlier round stopped in
+    bool has = pend != 0u, done = false;
+    pend = 0u;
     for (uint32_t stop_at = 0;; stop_at = min_lanes) {
         if (!done && !has && (node & RL_LEAF_BIT)) {
             held = node;
@@ -397,24 +398,40 @@ __device__ __forceinline__ int closest_run(const SceneDev& S0, const TopCache& t
     if (!has) return done ? -1 : 0;  // finished without another leaf, or still searching
     uint32_t cur = held & RL_LEAF_FIRST_MASK;
     uint32_t end = cur + ((held >> 27) & 15u) + 1u;
-    for (;;) {
-        const gfloat4* tp = gmem(S.tris) + 3u * cur;
-        float t, U, V, W, det;
-        if (wt_intersect_at(r.w, p, tp, t, U, V, W, det)) {
-            const uint32_t id = f_bits(reinterpret_cast<const gfloat*>(tp)[3]);
-            if (t < res.t || (t == res.t && id < res.tri)) {
-                res.t = t;
-                res.tri = id;
-                res.U = U; res.V = V; res.W = W; res.det = det;
+    // The leaf phase, one triangle per iteration; after its first, it ends for the wavefront once
+    // no more than `leaf_lanes` lanes still have triangles, and those keep the rest of their leaf
+    // for the next round (in `pend`, or as the next node when nothing follows it).
+    bool leafing = true;
+    for (uint32_t stop_at = 0;; stop_at = leaf_lanes) {
+        if ((uint32_t)__popcll(__ballot(leafing)) <= stop_at) break;
+        if (leafing) {
+            const gfloat4* tp = gmem(S.tris) + 3u * cur;
+            float t, U, V, W, det;
+            if (wt_intersect_at(r.w, p, tp, t, U, V, W, det)) {
+                const uint32_t id = f_bits(reinterpret_cast<const gfloat*>(tp)[3]);
+                if (t < res.t || (t == res.t && id < res.tri)) {
+                    res.t = t;
+                    res.tri = id;
+                    res.U = U; res.V = V; res.W = W; res.det = det;
+                }
+            }
+            if (++cur == end) {
+                if (done || !(node & RL_LEAF_BIT)) {
+                    leafing = false;
+                } else {
+                    cur = node & RL_LEAF_FIRST_MASK;
+                    end = cur + ((node >> 27) & 15u) + 1u;
+                    if (sp == 0) done = true;
+                    else node = stack_pop(sp, stk, stride, spill);
+                }
             }
         }
-        if (++cur == end) {
-            if (done || !(node & RL_LEAF_BIT)) break;
-            cur = node & RL_LEAF_FIRST_MASK;
-            end = cur + ((node >> 27) & 15u) + 1u;
-            if (sp == 0) done = true;
-            else node = stack_pop(sp, stk, stride, spill);
-        }
+    }
+    if (leafing) {
+        const uint32_t rest = RL_LEAF_BIT | ((end - cur - 1u) << 27) | cur;
+        if (done) node = rest;  // the stack is empty: the next round takes it as its leaf and ends
+        else pend = rest;
+        return 0;
     }
     return done ? -1 : 0;
 }
