@@ -419,7 +419,7 @@ int32_t rl_server_destroy(rl_server* server);
  * grids (128-thread blocks). Results do not depend on the tuning. */
 #define RL_AO_KERNEL_CHUNKED 0
 #define RL_AO_KERNEL_REFILL 1
-/* Final-gather kernel: AUTO gives each lane its own lumel when a pass has at least 8 lumels per
+/* Final-gather kernel: AUTO gives each lane its own lumel when a pass has at least 2 lumels per
    resident lane, else each lumel a wavefront; WAVE and LANE force one of the two. */
 #define RL_FG_KERNEL_AUTO 0
 #define RL_FG_KERNEL_WAVE 1

@@ -409,7 +409,7 @@ __global__ __launch_bounds__(kFgBlock) void k_fg_refill(const BakeSeg* __restric
 // done wait until at least `refill_min` lanes of the wavefront are without a ray, then start their
 // next samples together (lanes whose lumel is done first take new lumels), so the ray set-up runs
 // for many lanes at once. LDS per workgroup: 16 KB stacks + 23 KB of the scene's first 184 nodes.
-constexpr uint64_t kFgLanesMinPerLane = 8;  // lumels per resident lane for the lane-per-lumel kernel
+constexpr uint64_t kFgLanesMinPerLane = 2;  // lumels per resident lane for the lane-per-lumel kernel
 
 template <int CAP>
 __global__ __launch_bounds__(kFgBlock) void k_fg_lanes(const BakeSeg* __restrict__ segs, int nsegs, uint64_t total,
