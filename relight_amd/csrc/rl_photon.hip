@@ -787,8 +787,8 @@ hipError_t launch_final_gather(hipStream_t st, int cap, uint32_t max_blocks, con
     if (total_lumels == 0) return hipSuccess;
     // Idle lanes a wavefront waits for before starting more samples: the heightfield pass measured
     // 370 ms at 8 and 16, 388 ms at 32 (lane kernel); 407-411 ms from 4 to 48 (ring kernel); after
-    // the early node-phase stop 328/328/329/330 ms at 8/16/24/32 and after the per-lumel set-up
-    // 298/299/299 ms at 8/16/32.
+    // the early node-phase stop 329/328/329/330 ms at 8/16/24/32 and after the per-lumel set-up
+    // 298/298/299 ms at 8/16/32.
     const uint32_t refill_min = 16;
     hipError_t e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(queue), 2, 0u);
     if (e != hipSuccess) return e;
