@@ -124,6 +124,32 @@ def test_trace_segments_agreement(relight, which):
         assert hits["mesh"][i] == m[0] and hits["face"][i] == f[0]
 
 
+def test_far_origin_segments_agree(relight):
+    """Segments from origins far outside the scene (2 to 18 times the largest coordinate
+    magnitude away, inside the range the slab test's rounding analysis and box padding cover)
+    into random points of the city, any and closest hit, against the oracle."""
+    print("[gpu] far-origin segments", flush=True)
+    spec = scenes.city(num_boxes=600, tiles=(2, 2), side_range=(64, 128))
+    scene, lms, ids = scenes.build_gpu(relight, spec)
+    allp = np.concatenate([m.positions for m in spec.meshes])
+    maxc = float(np.abs(allp).max())
+    rng = np.random.default_rng(23)
+    n = 60000
+    d = rng.normal(size=(n, 3))
+    d[:, 2] = np.abs(d[:, 2])  # from above the ground
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    a = (d * rng.uniform(2.0, 18.0, (n, 1)) * maxc).astype(np.float32)
+    b = rng.uniform(allp.min(0), allp.max(0), (n, 3)).astype(np.float32)
+    S = O.OracleScene.from_spec(spec)
+    assert np.array_equal(scene.trace_segments(a, b), S.trace(a, b)["flags"])
+    h = scene.trace_segments(a, b, RL_TRACE_CLOSEST)
+    ref = S.trace(a, b, closest=True)
+    assert np.array_equal(h["triangle"], ref["triangle"])
+    hit = ref["triangle"] >= 0
+    assert hit.mean() > 0.2
+    assert np.array_equal(h["t"][hit], ref["t"][hit])
+
+
 def test_city_small_batch_equals_plain_and_oracle(relight):
     print("[gpu] city_small batch vs plain vs oracle", flush=True)
     spec = scenes.city(num_boxes=400, tiles=(2, 2), side_range=(64, 160), ao_samples=32)
