@@ -369,7 +369,7 @@ __device__ __forceinline__ int any_hit_run(const SceneDev& S0, const TopCache& t
 // 0 and continue on the next call). The leaf phase tests the held leaf's triangles, one per
 // iteration, then any leaf the lane stopped at and the leaves popped right after it. Returns 0
 // when the round ends at an internal node (call again), -1 when traversal is finished: res then
-// holds the closest hit (tri = 0xFFFFFFFF for a miss). Start with node = 0, sp = 0 and
+// holds the closest hit (tri = 0xFFFFFFFF for a miss). Start with node = 0, sp = 0, pend = 0 and
 // res = {r.w.tfar, 0xFFFFFFFF, 0, 0, 0, 0}. Every node a lane skips is culled against an interval
 // no shorter than the final one, so the lane tests every triangle that traverse<false>() could
 // return, and the result (nearest t, ties to the smaller id) equals traverse<false>()'s; rounds
