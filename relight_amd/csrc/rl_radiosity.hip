@@ -168,12 +168,15 @@ __global__ __launch_bounds__(256) void k_ff_gemm(const uint16_t* __restrict__ F,
 }
 
 // Fast gather product Y_part = F_hi * x1 for the Jacobi sweeps. A workgroup (four wavefronts,
-// 128 rows) stages x through LDS four K steps at a time, double-buffered, so x is read from L2
+// 128 rows) stages x through LDS eight K steps at a time, double-buffered, so x is read from L2
 // once per workgroup instead of once per wavefront; each wavefront owns two 16-row tiles (every B
-// fragment feeds two MFMAs) and keeps F loads four K steps ahead in a register ring. Same lane
+// fragment feeds two MFMAs) and keeps F loads eight K steps ahead in a register ring. Same lane
 // layout, products and summation order as k_ff_gemm<NT, false>.
 constexpr int kFastRows = 32;  // rows per wavefront
-constexpr int kFastAhead = 4;  // K steps of F in flight = K steps per LDS stage of x
+// K steps of F in flight = K steps per LDS stage of x. Eight (96 KB of x stages for 96 columns,
+// one workgroup per compute unit) measured 121 us per 16,384-patch sweep product against 129 us
+// with four.
+constexpr int kFastAhead = 8;
 
 template <int NT>
 __global__ __launch_bounds__(256) void k_ff_gemm_fast(const uint16_t* __restrict__ F, uint32_t stride,
