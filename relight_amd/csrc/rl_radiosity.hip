@@ -167,30 +167,37 @@ __global__ __launch_bounds__(256) void k_ff_gemm(const uint16_t* __restrict__ F,
     }
 }
 
-// Fast gather product Y_part = F_hi * x1 for the Jacobi sweeps. A workgroup (four wavefronts,
-// 128 rows) stages x through LDS eight K steps at a time, double-buffered, so x is read from L2
-// once per workgroup instead of once per wavefront; each wavefront owns two 16-row tiles (every B
-// fragment feeds two MFMAs) and keeps F loads eight K steps ahead in a register ring. Same lane
-// layout, products and summation order as k_ff_gemm<NT, false>.
+// Staged gather products. A workgroup (four wavefronts, 128 rows) stages x through LDS AHEAD K
+// steps at a time, double-buffered, so x is read from L2 once per workgroup instead of once per
+// wavefront; each wavefront owns two 16-row tiles (every B fragment feeds two row tiles' MFMAs)
+// and keeps F loads AHEAD K steps ahead in a register ring. ACC = false: F_hi * x1, the Jacobi
+// sweeps' product. ACC = true: the refinement residual's (F_hi + F_lo)(x1 + x2 + x3) without
+// F_lo * x3, with x1, x2 and x3 all staged. Same lane layout, products and summation order per
+// row tile as k_ff_gemm<NT, ACC>.
 constexpr int kFastRows = 32;  // rows per wavefront
-// K steps of F in flight = K steps per LDS stage of x. Eight (96 KB of x stages for 96 columns,
-// one workgroup per compute unit) measured 121 us per 16,384-patch sweep product against 129 us
-// with four.
+// K steps of F in flight = K steps per LDS stage of x. For the sweeps eight (96 KB of x stages for
+// 96 columns, one workgroup per compute unit) measured 121 us per 16,384-patch product against
+// 129 us with four; the residual product stages three x vectors, two steps at a time (72 KB).
 constexpr int kFastAhead = 8;
+constexpr int kAccAhead = 2;
 
-template <int NT>
-__global__ __launch_bounds__(256) void k_ff_gemm_fast(const uint16_t* __restrict__ F, uint32_t stride,
-                                                      const __bf16* __restrict__ X1, uint32_t npad,
-                                                      uint32_t klen, float* __restrict__ Yp) {
+template <int NT, bool ACC>
+__global__ __launch_bounds__(256) void k_ff_gemm_staged(const uint16_t* __restrict__ F, uint32_t stride,
+                                                        const __bf16* __restrict__ X1,
+                                                        const __bf16* __restrict__ X2,
+                                                        const __bf16* __restrict__ X3, uint32_t npad,
+                                                        uint32_t klen, float* __restrict__ Yp) {
+    constexpr int AHEAD = ACC ? kAccAhead : kFastAhead;
+    constexpr int NX = ACC ? 3 : 1;                           // x vectors staged
     constexpr uint32_t mp = NT * 16;
-    constexpr uint32_t kStepVecs = mp * 32u / 8u;             // bf16x8 vectors of x per K step
-    constexpr uint32_t kStageVecs = kStepVecs * kFastAhead;  // per LDS stage
+    constexpr uint32_t kStepVecs = mp * 32u / 8u;             // bf16x8 vectors of one x per K step
+    constexpr uint32_t kStageVecs = kStepVecs * AHEAD;        // per x per LDS stage
     constexpr uint32_t kPerThread = (kStageVecs + 255u) / 256u;
-    __shared__ bf16x8 xs[2][kStageVecs];
+    __shared__ bf16x8 xs[2][NX][kStageVecs];
     const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
     const uint32_t row0 = (blockIdx.x * 4u + wave) * (uint32_t)kFastRows;
     const uint32_t k0 = blockIdx.y * klen;
-    const uint32_t nstage = klen / (32u * kFastAhead);
+    const uint32_t nstage = klen / (32u * AHEAD);
     const uint32_t sub = lane >> 4, col = lane & 15u;
     f32x4 acc[2][NT];
 #pragma unroll
@@ -199,65 +206,90 @@ __global__ __launch_bounds__(256) void k_ff_gemm_fast(const uint16_t* __restrict
         for (int t = 0; t < NT; t++) acc[r][t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
     const uint16_t* ftile[2] = {F + (size_t)(row0 >> 4) * (stride >> 5) * 512u + col * 32u + 8u * sub,
                                 F + (size_t)((row0 >> 4) + 1u) * (stride >> 5) * 512u + col * 32u + 8u * sub};
-    const bf16x8* xg = reinterpret_cast<const bf16x8*>(X1) + (size_t)(k0 >> 5) * kStepVecs;
-    const uint32_t xl = 64u * 0u + col * 4u + sub;  // vector offset of this lane's fragment in a step
-    u32x4 ring[2][kFastAhead];
+    const bf16x8* xg[3] = {reinterpret_cast<const bf16x8*>(X1) + (size_t)(k0 >> 5) * kStepVecs,
+                           reinterpret_cast<const bf16x8*>(ACC ? X2 : X1) + (size_t)(k0 >> 5) * kStepVecs,
+                           reinterpret_cast<const bf16x8*>(ACC ? X3 : X1) + (size_t)(k0 >> 5) * kStepVecs};
+    const uint32_t xl = col * 4u + sub;  // vector offset of this lane's fragment in a step
+    u32x4 ring[2][AHEAD];
 #pragma unroll
-    for (int j = 0; j < kFastAhead; j++)
+    for (int j = 0; j < AHEAD; j++)
 #pragma unroll
         for (int r = 0; r < 2; r++)
             ring[r][j] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(ftile[r] + (size_t)((k0 >> 5) + j) * 512u));
-    bf16x8 stg[kPerThread];
+    bf16x8 stg[NX][kPerThread];
 #pragma unroll
-    for (uint32_t p = 0; p < kPerThread; p++) {
-        const uint32_t v = threadIdx.x + 256u * p;
-        if (v < kStageVecs) stg[p] = xg[v];
-    }
+    for (int q = 0; q < NX; q++)
 #pragma unroll
-    for (uint32_t p = 0; p < kPerThread; p++) {
-        const uint32_t v = threadIdx.x + 256u * p;
-        if (v < kStageVecs) xs[0][v] = stg[p];
-    }
+        for (uint32_t p = 0; p < kPerThread; p++) {
+            const uint32_t v = threadIdx.x + 256u * p;
+            if (v < kStageVecs) stg[q][p] = xg[q][v];
+        }
+#pragma unroll
+    for (int q = 0; q < NX; q++)
+#pragma unroll
+        for (uint32_t p = 0; p < kPerThread; p++) {
+            const uint32_t v = threadIdx.x + 256u * p;
+            if (v < kStageVecs) xs[0][q][v] = stg[q][p];
+        }
     __syncthreads();
     for (uint32_t s = 0; s < nstage; s++) {
         const uint32_t buf = s & 1u;
         const bool more = s + 1u < nstage;
         if (more) {
 #pragma unroll
-            for (uint32_t p = 0; p < kPerThread; p++) {
-                const uint32_t v = threadIdx.x + 256u * p;
-                if (v < kStageVecs) stg[p] = xg[(size_t)(s + 1u) * kStageVecs + v];
-            }
+            for (int q = 0; q < NX; q++)
+#pragma unroll
+                for (uint32_t p = 0; p < kPerThread; p++) {
+                    const uint32_t v = threadIdx.x + 256u * p;
+                    if (v < kStageVecs) stg[q][p] = xg[q][(size_t)(s + 1u) * kStageVecs + v];
+                }
         }
 #pragma unroll
-        for (int j = 0; j < kFastAhead; j++) {
-            const uint32_t step = (k0 >> 5) + s * kFastAhead + j;  // global K step (32 columns)
+        for (int j = 0; j < AHEAD; j++) {
+            const uint32_t step = (k0 >> 5) + s * AHEAD + j;  // global K step (32 columns)
             u32x4 cur[2] = {ring[0][j], ring[1][j]};
             if (more) {
 #pragma unroll
                 for (int r = 0; r < 2; r++)
                     ring[r][j] = __builtin_nontemporal_load(
-                        reinterpret_cast<const u32x4*>(ftile[r] + (size_t)(step + kFastAhead) * 512u));
+                        reinterpret_cast<const u32x4*>(ftile[r] + (size_t)(step + AHEAD) * 512u));
             }
-            bf16x8 b[NT];
+            bf16x8 b[NX][NT];
 #pragma unroll
-            for (int t = 0; t < NT; t++) b[t] = xs[buf][(uint32_t)j * kStepVecs + 64u * t + xl];
+            for (int q = 0; q < NX; q++)
+#pragma unroll
+                for (int t = 0; t < NT; t++) b[q][t] = xs[buf][q][(uint32_t)j * kStepVecs + 64u * t + xl];
 #pragma unroll
             for (int r = 0; r < 2; r++) {
                 const uint32_t w[4] = {cur[r].x, cur[r].y, cur[r].z, cur[r].w};
-                bf16x8 a;
+                bf16x8 ahi, alo;
 #pragma unroll
-                for (int e = 0; e < 8; e++) a[e] = (__bf16)(float)((w[e >> 1] >> ((e & 1) * 16)) & 0xFFFFu);
+                for (int e = 0; e < 8; e++) {
+                    const float fc = (float)((w[e >> 1] >> ((e & 1) * 16)) & 0xFFFFu);
+                    const __bf16 h = (__bf16)fc;
+                    ahi[e] = h;
+                    if (ACC) alo[e] = (__bf16)(fc - (float)h);
+                }
 #pragma unroll
-                for (int t = 0; t < NT; t++) acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[t], acc[r][t], 0, 0, 0);
+                for (int t = 0; t < NT; t++) {
+                    acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, b[0][t], acc[r][t], 0, 0, 0);
+                    if (ACC) {
+                        acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, b[NX > 1 ? 1 : 0][t], acc[r][t], 0, 0, 0);
+                        acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ahi, b[NX > 2 ? 2 : 0][t], acc[r][t], 0, 0, 0);
+                        acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, b[0][t], acc[r][t], 0, 0, 0);
+                        acc[r][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(alo, b[NX > 1 ? 1 : 0][t], acc[r][t], 0, 0, 0);
+                    }
+                }
             }
         }
         if (more) {
 #pragma unroll
-            for (uint32_t p = 0; p < kPerThread; p++) {
-                const uint32_t v = threadIdx.x + 256u * p;
-                if (v < kStageVecs) xs[buf ^ 1u][v] = stg[p];
-            }
+            for (int q = 0; q < NX; q++)
+#pragma unroll
+                for (uint32_t p = 0; p < kPerThread; p++) {
+                    const uint32_t v = threadIdx.x + 256u * p;
+                    if (v < kStageVecs) xs[buf ^ 1u][q][v] = stg[q][p];
+                }
         }
         __syncthreads();
     }
@@ -392,15 +424,20 @@ hipError_t launch_ff_gemm(hipStream_t st, bool accurate, const uint16_t* F, uint
     const uint32_t klen = npad / nsplit;
     if (klen % 32u != 0u || npad % 64u != 0u) return hipErrorInvalidValue;
     const dim3 grid(npad / 64, nsplit), block(256);
-    // The fast product takes 128 rows per block and whole LDS stages of K when the sizes allow it.
-    const bool wide = !accurate && npad % (4u * kFastRows) == 0u && klen % (32u * kFastAhead) == 0u;
+    // The staged products take 128 rows per block and whole LDS stages of K when the sizes allow it.
+    const uint32_t ahead = accurate ? (uint32_t)kAccAhead : (uint32_t)kFastAhead;
+    const bool wide = npad % (4u * kFastRows) == 0u && klen % (32u * ahead) == 0u;
     const dim3 grid_fast(npad / (4u * kFastRows), nsplit);
 #define RL_GEMM_CASE(NTV)                                                                                     \
     case NTV:                                                                                                 \
-        if (accurate)                                                                                         \
-            hipLaunchKernelGGL((k_ff_gemm<NTV, true>), grid, block, 0, st, F, stride, X1, X2, X3, npad, klen, Yp); \
+        if (wide && accurate)                                                                                 \
+            hipLaunchKernelGGL((k_ff_gemm_staged<NTV, true>), grid_fast, block, 0, st, F, stride, X1, X2, X3, npad, \
+                               klen, Yp);                                                                     \
         else if (wide)                                                                                        \
-            hipLaunchKernelGGL((k_ff_gemm_fast<NTV>), grid_fast, block, 0, st, F, stride, X1, npad, klen, Yp);  \
+            hipLaunchKernelGGL((k_ff_gemm_staged<NTV, false>), grid_fast, block, 0, st, F, stride, X1, X1, X1, npad, \
+                               klen, Yp);                                                                     \
+        else if (accurate)                                                                                    \
+            hipLaunchKernelGGL((k_ff_gemm<NTV, true>), grid, block, 0, st, F, stride, X1, X2, X3, npad, klen, Yp); \
         else                                                                                                  \
             hipLaunchKernelGGL((k_ff_gemm<NTV, false>), grid, block, 0, st, F, stride, X1, X2, X3, npad, klen, Yp); \
         break;
