@@ -207,11 +207,12 @@ constexpr int kRing = 32;
 // finished lanes take new rays instead of idling until the slowest ray of the wavefront reaches a
 // leaf (city AO: 399 ms with no early stop, 397/393/387/384/387/398 ms at 1/2/4/8/16/24 lanes).
 constexpr uint32_t kAoNodeExit = 8;
-// Workgroups of four wavefronts, each keeping the first 152 nodes of the first segment's scene in
-// LDS (19 KB; with the stacks, rings and lumel tables a workgroup takes 40 KB, so four fill a
-// compute unit's 160 KB: sixteen wavefronts, the occupancy the kernel's registers allow).
-constexpr int kAoBlock = 256;
-constexpr uint32_t kAoTopNodes = 152;
+// One workgroup of sixteen wavefronts per compute unit (the occupancy the kernel's registers
+// allow), so one LDS copy of the scene's first 600 nodes (75 KB; with the stacks, rings and lumel
+// tables 159 KB) serves all of them: city AO 360 ms with four-wavefront workgroups and 152 nodes,
+// 350 ms with eight and 304, 352/347 ms with sixteen and 400/600.
+constexpr int kAoBlock = 1024;
+constexpr uint32_t kAoTopNodes = 600;
 
 template <int CAP>
 __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restrict__ segs, int nsegs,
