@@ -220,11 +220,12 @@ __global__ void k_photon_gather(const PhotonMapDev* __restrict__ pms, int npm, u
     M.gathered[t] = out;
 }
 
-// Final-gather workgroups: four wavefronts, each workgroup keeping the first nodes of the first
-// segment's scene in LDS (kFgTopNodes for the lane-per-lumel kernel: 23 KB, with the stacks 39 KB
-// per workgroup, so four fit a compute unit's 160 KB).
-constexpr int kFgBlock = 256;
-constexpr uint32_t kFgTopNodes = 184;
+// Final-gather workgroups: sixteen wavefronts, one workgroup per compute unit (the occupancy the
+// registers allow), keeping the first nodes of the first segment's scene in LDS (kFgTopNodes for
+// the lane-per-lumel kernel: 88 KB, with the stacks 152 KB). Heightfield final gather: 289 ms with
+// four-wavefront workgroups and 184 nodes, 282 ms with sixteen and 700.
+constexpr int kFgBlock = 1024;
+constexpr uint32_t kFgTopNodes = 700;
 // A closest-hit round's node phase stops once no more than this many lanes are still searching
 // without a leaf (heightfield final gather: 338 ms with no early stop, 327/328/328/330/345 ms at
 // 2/3/4/6/16 lanes, 8 to 32 idle lanes before a refill within 1 ms of each other at 4).
@@ -245,8 +246,7 @@ constexpr uint32_t kFgLeafExit = 16;
 // does, and writes the lumel when its last sample has been added. A lane may only start pair j <
 // consumed + kFgRing.
 constexpr uint32_t kFgRing = 128;
-// LDS per workgroup: 16 KB stacks + 10 KB rings + 13 KB of the scene's first 104 nodes, so four
-// workgroups (sixteen wavefronts) fit a compute unit's 160 KB.
+// LDS per workgroup: 64 KB stacks + 40 KB rings + 13 KB of the scene's first 104 nodes.
 constexpr uint32_t kFgRefillTopNodes = 104;
 
 template <int CAP>
@@ -408,7 +408,7 @@ __global__ __launch_bounds__(kFgBlock) void k_fg_refill(const BakeSeg* __restric
 // does, with no exchange between lanes. A lane traces in rounds (closest_run); lanes whose ray is
 // done wait until at least `refill_min` lanes of the wavefront are without a ray, then start their
 // next samples together (lanes whose lumel is done first take new lumels), so the ray set-up runs
-// for many lanes at once. LDS per workgroup: 16 KB stacks + 23 KB of the scene's first 184 nodes.
+// for many lanes at once. LDS per workgroup: 64 KB stacks + 88 KB of the scene's first 700 nodes.
 constexpr uint64_t kFgLanesMinPerLane = 2;  // lumels per resident lane for the lane-per-lumel kernel
 
 template <int CAP>
