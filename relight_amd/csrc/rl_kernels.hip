@@ -20,6 +20,7 @@ namespace rl {
 namespace {
 
 constexpr int kBlock = 128;  // two wavefronts
+constexpr int kDirectBlock = 768;  // direct light: twelve wavefronts, one workgroup per compute unit
 using namespace trav;
 
 // LightInfluence::calculate / DirectionalLightInfluence::calculate with LightCutoff and
@@ -31,7 +32,7 @@ using namespace trav;
 template <int CAP>
 __device__ __forceinline__ bool occluded(const SceneDev& S, const TopCache& top, const Ray& r, uint32_t* stk) {
     TraceResult tr;
-    return traverse<true, CAP>(S, top, r, stk, kBlock, tr);
+    return traverse<true, CAP>(S, top, r, stk, kDirectBlock, tr);
 }
 
 template <int CAP>
@@ -76,17 +77,17 @@ __device__ float influence_from_point(const SceneDev& S, const TopCache& top, co
     return inf * att * cut;
 }
 
-// One lane per lumel; lights and light vertices in the reference's order. Each workgroup keeps the
-// first 120 nodes of the first segment's scene in LDS (15 KB; with the stacks 23 KB per two-wave
-// workgroup, so the six workgroups a compute unit holds at this kernel's register use fit):
-// city direct light 21.2 -> 20.0 ms.
-constexpr uint32_t kDirectTopNodes = 120;
+// One lane per lumel; lights and light vertices in the reference's order. One workgroup of twelve
+// wavefronts per compute unit (the occupancy the kernel's registers allow) keeps the first 880
+// nodes of the first segment's scene in LDS (110 KB; with the stacks 158 KB): city direct light
+// 21.2 ms without the cache, 20.0 ms with 120 nodes per two-wavefront workgroup, 19.6 ms so.
+constexpr uint32_t kDirectTopNodes = 880;
 
 template <int CAP>
-__global__ __launch_bounds__(kBlock) void k_bake_direct(const BakeSeg* __restrict__ segs, int nsegs,
-                                                        uint64_t total, unsigned long long* queue,
-                                                        unsigned long long* rays_out) {
-    __shared__ uint32_t stack[kLdsStack * kBlock];
+__global__ __launch_bounds__(kDirectBlock) void k_bake_direct(const BakeSeg* __restrict__ segs, int nsegs,
+                                                              uint64_t total, unsigned long long* queue,
+                                                              unsigned long long* rays_out) {
+    __shared__ uint32_t stack[kLdsStack * kDirectBlock];
     __shared__ u32x4 top_rows[kDirectTopNodes * kNodeRows];
     const SceneDev top_scene = segs[0].scene;
     const uint32_t top_count = load_top_nodes(top_scene, top_rows, kDirectTopNodes);
@@ -464,8 +465,11 @@ hipError_t launch_bake_direct(hipStream_t st, int cap, uint32_t max_blocks, cons
     if (total == 0) return hipSuccess;
     hipError_t e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(queue), 2, 0u);
     if (e != hipSuccess) return e;
-    const uint32_t grid = grid_for_waves((total + 63) / 64, max_blocks);
-    RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_bake_direct<CAP>, dim3(grid), dim3(kBlock), 0, st, segs, nsegs, total,
+    // max_blocks counts kBlock-thread workgroups; this kernel's are kDirectBlock threads.
+    const uint64_t need = ((total + 63) / 64 + (kDirectBlock / 64) - 1) / (kDirectBlock / 64);
+    const uint32_t cap_blocks = std::max<uint32_t>(1u, max_blocks * (uint32_t)kBlock / (uint32_t)kDirectBlock);
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(need, cap_blocks));
+    RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_bake_direct<CAP>, dim3(grid), dim3(kDirectBlock), 0, st, segs, nsegs, total,
                                             queue, rays));
     return hipGetLastError();
 }
