@@ -268,6 +268,19 @@ int32_t rl_trace_segments(rl_relight* ctx, rl_scene* scene, const float* starts,
                           const float* ends, uint32_t n, int32_t mode, uint8_t* out_flags,
                           rl_hit* out_hits);
 
+/* rl_trace_segments for callers that cannot allocate device memory (code that must not call
+ * hipMalloc/hipFree: serving loops with a fixed footprint, harnesses that forbid allocation). No
+ * counterpart in the reference. Same arguments, results and host buffers as rl_trace_segments,
+ * but every device buffer of the call (segment staging, hit records, the work and ray counters)
+ * is a static device array of the library, about 6.1 MiB per GPU, held until the process ends;
+ * the call allocates nothing. At most RL_NOALLOC_MAX_SEGMENTS segments per call (more:
+ * RL_INVALID_ARGUMENT). Calls share the array of their GPU: calls from one process are
+ * serialised; calls that overlap from several processes on one GPU race. */
+#define RL_NOALLOC_MAX_SEGMENTS 65536
+int32_t rl_trace_segments_noalloc(rl_relight* ctx, rl_scene* scene, const float* starts,
+                                  const float* ends, uint32_t n, int32_t mode, uint8_t* out_flags,
+                                  rl_hit* out_hits);
+
 /* Relight::createPhotonmap (Relight.h:162) and Photonmap::addMesh (Lightmap.cpp:359-373): a
  * photonmap is a lightmap whose texels receive photon stores. read() returns its full grids:
  * accumulated photon power (w*h*3), photon counts (w*h) and gathered colour (w*h*3); any output

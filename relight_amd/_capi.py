@@ -27,6 +27,7 @@ RL_SAMPLE_COSINE = 1
 
 RL_TRACE_ANY = 0
 RL_TRACE_CLOSEST = 1
+RL_NOALLOC_MAX_SEGMENTS = 65536
 
 STATUS_NAMES = {
     0: "Success", 1: "InvalidCall", 2: "NotImplemented", 3: "OutOfMemory",
@@ -250,6 +251,8 @@ SIGNATURES = [
     ("rl_pass_destroy", C.c_int32, [_P]),
     ("rl_trace_segments", C.c_int32, [_P, _P, _F, _F, C.c_uint32, C.c_int32,
                                       C.POINTER(C.c_uint8), C.POINTER(rl_hit)]),
+    ("rl_trace_segments_noalloc", C.c_int32, [_P, _P, _F, _F, C.c_uint32, C.c_int32,
+                                              C.POINTER(C.c_uint8), C.POINTER(rl_hit)]),
     ("rl_photonmap_create", C.c_int32, [_P, C.c_int32, C.c_int32, _PP]),
     ("rl_photonmap_add_mesh", C.c_int32, [_P, _P, C.c_int32]),
     ("rl_photonmap_read", C.c_int32, [_P, _F, _U32, _F]),

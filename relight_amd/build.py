@@ -1,26 +1,43 @@
-"""Builds librelight_amd.so in-tree with hipcc for gfx950."""
+"""Builds librelight_amd.so in-tree with hipcc for gfx950 (one object per source, compiled in
+parallel, then linked)."""
+import concurrent.futures as cf
 import os
 import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "librelight_amd.so")
-SOURCES = ["rl_kernels.hip", "rl_photon.hip", "rl_radiosity.hip", "rl_post.hip", "rl_raster.hip", "rl_runtime.cpp", "rl_bvh.cpp", "rl_obj.cpp"]
+SOURCES = ["rl_kernels.hip", "rl_photon.hip", "rl_radiosity.hip", "rl_post.hip", "rl_raster.hip",
+           "rl_noalloc.hip", "rl_runtime.cpp", "rl_bvh.cpp", "rl_obj.cpp"]
 
 # -ffp-contract=off keeps host (lumel setup) and device (ray setup, shading) arithmetic rounding
 # as written; -packed-fp32-ops keeps the compiler from emitting packed fp32 instructions.
-FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
          "-Wno-unused-value", "-Wno-unused-command-line-argument",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
-def build(verbose: bool = False) -> str:
-    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    tmp = OUT + ".tmp"
-    cmd = [hipcc] + FLAGS + [os.path.join(CSRC, s) for s in SOURCES] + ["-o", tmp]
+def _compile(hipcc, src):
+    obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+    cmd = [hipcc] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
+        raise RuntimeError(f"hipcc failed on {src}:\n" + r.stdout + r.stderr)
+    return obj
+
+
+def build(verbose: bool = False) -> str:
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    os.makedirs(OBJ, exist_ok=True)
+    jobs = max(1, min(len(SOURCES), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
+    with cf.ThreadPoolExecutor(jobs) as ex:
+        objs = list(ex.map(lambda s: _compile(hipcc, s), SOURCES))
+    tmp = OUT + ".tmp"
+    cmd = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC"] + objs + ["-o", tmp]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, OUT)
     if verbose:
         print(" ".join(cmd))

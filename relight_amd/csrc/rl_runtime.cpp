@@ -28,6 +28,7 @@
 #include "rl_device.h"
 #include "rl_kernels.h"
 #include "rl_math.h"
+#include "rl_noalloc.h"
 
 using namespace rl;
 
@@ -717,6 +718,11 @@ int32_t rl_create(rl_relight** out) {
         hipMalloc((void**)&ctx->d_queue, 4 * sizeof(unsigned long long)) != hipSuccess) {
         delete ctx;
         return fail(RL_DEVICE_ERROR, "stream/event/counter creation failed");
+    }
+    // The static scratch of rl_trace_segments_noalloc: its address on this device, looked up once.
+    if (hipError_t se = noalloc_setup(ctx->device); se != hipSuccess) {
+        rl_destroy(ctx);
+        return fail(RL_DEVICE_ERROR, std::string("scratch lookup: ") + hipGetErrorString(se));
     }
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device) == hipSuccess && cus > 0) {
@@ -1426,6 +1432,32 @@ int32_t rl_trace_segments(rl_relight* ctx, rl_scene* scene, const float* starts,
     }
     cleanup();
     return end_call(ctx, tm);
+}
+
+int32_t rl_trace_segments_noalloc(rl_relight* ctx, rl_scene* scene, const float* starts, const float* ends,
+                                  uint32_t n, int32_t mode, uint8_t* out_flags, rl_hit* out_hits) {
+    if (!ctx) return fail(RL_INVALID_ARGUMENT, "context is NULL");
+    int32_t st = check_scene_ready(scene);
+    if (st) return st;
+    if (mode != RL_TRACE_ANY && mode != RL_TRACE_CLOSEST) return fail(RL_INVALID_ARGUMENT, "bad trace mode");
+    if (n > kNoAllocMaxSegments)
+        return fail(RL_INVALID_ARGUMENT, "rl_trace_segments_noalloc takes at most " +
+                                             std::to_string(kNoAllocMaxSegments) + " segments per call, got " +
+                                             std::to_string(n));
+    if (n == 0) return RL_SUCCESS;
+    if (!starts || !ends) return fail(RL_INVALID_ARGUMENT, "segment arrays are NULL");
+    if (mode == RL_TRACE_ANY && !out_flags) return fail(RL_INVALID_ARGUMENT, "out_flags is NULL");
+    if (mode == RL_TRACE_CLOSEST && !out_hits) return fail(RL_INVALID_ARGUMENT, "out_hits is NULL");
+    const auto t0 = std::chrono::steady_clock::now();
+    unsigned long long rays = 0;
+    hipError_t e = noalloc_trace(ctx->device, ctx->stream, scene->dev(), scene->cap, ctx->max_blocks,
+                                 mode == RL_TRACE_ANY, starts, ends, n, out_flags, out_hits, &rays);
+    if (e != hipSuccess) return fail(RL_DEVICE_ERROR, std::string("trace (no-alloc): ") + hipGetErrorString(e));
+    ctx->stats = rl_stats{};
+    ctx->stats.rays = rays;
+    ctx->stats.launches = 1;
+    ctx->stats.wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return RL_SUCCESS;
 }
 
 int32_t rl_photonmap_create(rl_relight* ctx, int32_t width, int32_t height, rl_lightmap** out) {
