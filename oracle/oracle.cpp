@@ -27,6 +27,11 @@
 // cases; first-bounce photon energy; texel-disc gather by hand; k-nearest gather against a
 // numpy brute force; final gather under a constant sky in closed form; form factors against
 // Howell's parallel and perpendicular plate formulas and closed-enclosure row sums.
+// tests/test_oracle_pins.py: the photon bounce loop in a closed sphere (per-depth power
+// I * albedo^(d+1) / 2^d; Russian roulette survivors and per-depth sums), the directional
+// emitter's start plane, the final gather's hit branch (front face, back face, the
+// floor(uv * (size - 1)) lookup at the barycentric-swapped uv) against ray/plane predictions,
+// and the mesh transform against R p + t with renormalised normals.
 // Compiled with -ffp-contract=off so every expression rounds as written.
 #include <stdint.h>
 #include <math.h>
