@@ -250,7 +250,8 @@ class OracleScene:
     def from_spec(cls, spec, use_bvh=True):
         S = cls(use_bvh)
         for m in spec.meshes:
-            S.add_mesh(m.positions, m.normals, m.uvs, m.indices, albedo=m.albedo)
+            S.add_mesh(m.positions, m.normals, m.uvs, m.indices, transform=getattr(m, "transform", None),
+                       albedo=m.albedo)
         for L in spec.lights:
             S.add_light(L)
         S.build()

@@ -64,6 +64,11 @@ inline float normalize3(Vec& a) {
     if (l) { a.x /= l; a.y /= l; a.z /= l; }
     return l;
 }
+// powf of LightSpotCutoff (scene/Light.cpp:464-477) and AmbientOcclusion::bakeLumel
+// (baker/AmbientOcclusion.cpp:61-63), evaluated as the double pow rounded to float: the
+// correctly rounded powf except within about 2^-52 of a float rounding midpoint, and the same
+// bits as the GPU library (single-precision powf implementations differ in the last bit).
+inline float ref_powf(float x, float y) { return (float)pow((double)x, (double)y); }
 inline float axis(Vec a, int k) { return k == 0 ? a.x : (k == 1 ? a.y : a.z); }
 
 // ------------------------------------------------------------------ PCG32 (O'Neill 2014)
@@ -457,7 +462,7 @@ float influence(const Scene& S, const Light& L, Vec lp, Vec p, Vec n, uint64_t& 
         if (v <= L.cutoff) cut = 0.0f;
         else {
             v = (1.0f - (1.0f - v) * 1.0f / (1.0f - L.cutoff));
-            if (fabsf(1.0f - L.sexp) > 0.01f) v = powf(v, L.sexp);
+            if (fabsf(1.0f - L.sexp) > 0.01f) v = ref_powf(v, L.sexp);
             cut = v;
         }
     }
@@ -698,7 +703,7 @@ int32_t or_bake_ao(void* sp, uint32_t n, const float* pos, const float* nrm, con
             if (trace(S, make_seg(p, add(p, scale(d, a->max_distance))), true, h, nullptr)) occ++;
         }
         float value = 1.0f - occ / (a->samples * a->occluded_fraction);
-        if (fabsf(1.0f - a->exponent) > 0.01f) value = powf(value, a->exponent);
+        if (fabsf(1.0f - a->exponent) > 0.01f) value = ref_powf(value, a->exponent);
         color[3 * i] *= value;
         color[3 * i + 1] *= value;
         color[3 * i + 2] *= value;
@@ -933,7 +938,7 @@ int32_t or_emit_photons(void* sp, const or_indirect* s, const int32_t* mesh_pm, 
             if (v <= L.cutoff) cut = 0.0f;
             else {
                 v = (1.0f - (1.0f - v) * 1.0f / (1.0f - L.cutoff));
-                if (fabsf(1.0f - L.sexp) > 0.01f) v = powf(v, L.sexp);
+                if (fabsf(1.0f - L.sexp) > 0.01f) v = ref_powf(v, L.sexp);
                 cut = v;
             }
         }

@@ -69,7 +69,7 @@ __device__ float influence_from_point(const SceneDev& S, const TopCache& top, co
             cut = 0.0f;
         } else {
             value = (1.0f - (1.0f - value) * 1.0f / (1.0f - L.cutoff));
-            if (fabsf(1.0f - L.spot_exponent) > 0.01f) value = powf(value, L.spot_exponent);
+            if (fabsf(1.0f - L.spot_exponent) > 0.01f) value = ref_powf(value, L.spot_exponent);
             cut = value;
         }
     }
@@ -383,7 +383,7 @@ __global__ __launch_bounds__(kBlock) void k_ao_finalize(const BakeSeg* __restric
         const uint32_t i = (uint32_t)(g - sg.work_begin);
         const uint32_t occluded = sg.counts[i];
         float value = 1.0f - (float)occluded / ((float)P.samples * P.occluded_fraction);
-        if (fabsf(1.0f - P.exponent) > 0.01f) value = powf(value, P.exponent);
+        if (fabsf(1.0f - P.exponent) > 0.01f) value = ref_powf(value, P.exponent);
         float4 c = sg.color[i];
         c.x *= value; c.y *= value; c.z *= value;
         sg.color[i] = c;

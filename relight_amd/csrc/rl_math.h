@@ -30,6 +30,14 @@ RL_HD V3 operator*(V3 a, float s) { return v3(a.x * s, a.y * s, a.z * s); }
 RL_HD V3 operator/(V3 a, float s) { return v3(a.x / s, a.y / s, a.z / s); }
 // Vec3::operator*(Vec3) is the dot product, summed left to right (Types.h:792-794).
 RL_HD float dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+// The reference's powf (LightSpotCutoff::cutoffForDirection, scene/Light.cpp:464-477, and
+// AmbientOcclusion::bakeLumel, baker/AmbientOcclusion.cpp:61-63) as the double-precision pow
+// rounded to float. Single-precision powf implementations differ in the last bit (the GPU's and
+// glibc's disagree on about 1 % of spot-light cutoffs); a double pow within an ulp of the exact
+// value rounds to the correctly rounded powf except when x^y lies within about 2^-52 of a float
+// rounding midpoint, so device and host agree bit for bit.
+RL_HD float ref_powf(float x, float y) { return (float)pow((double)x, (double)y); }
 RL_HD float length(V3 a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
 // Vec3::normalize (Types.h:852-859): divide by the length unless it is zero, return the length.
 RL_HD float normalize(V3& a) {

@@ -101,7 +101,7 @@ __global__ __launch_bounds__(kBlock) void k_photon_trace(SceneDev S, const Light
                 cut = 0.0f;
             } else {
                 value = (1.0f - (1.0f - value) * 1.0f / (1.0f - L.cutoff));
-                if (fabsf(1.0f - L.spot_exponent) > 0.01f) value = powf(value, L.spot_exponent);
+                if (fabsf(1.0f - L.spot_exponent) > 0.01f) value = ref_powf(value, L.spot_exponent);
                 cut = value;
             }
         }

@@ -37,6 +37,7 @@ class MeshSpec:
     lightmap: int
     albedo: Optional[np.ndarray] = None
     photonmap: int = -1
+    transform: Optional[np.ndarray] = None  # 16 floats, the reference's Matrix4 layout
 
     @property
     def num_tris(self):
@@ -389,7 +390,8 @@ def build_gpu(relight, spec: SceneSpec):
     scene.begin()
     ids = []
     for m in spec.meshes:
-        ids.append(scene.add_mesh(m.positions, m.normals, m.uvs, m.indices, albedo=m.albedo))
+        ids.append(scene.add_mesh(m.positions, m.normals, m.uvs, m.indices, transform=m.transform,
+                                  albedo=m.albedo))
     for L in spec.lights:
         scene.add_light(L)
     lms = [relight.create_lightmap(w, h) for (w, h) in spec.lightmaps]
