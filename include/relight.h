@@ -174,9 +174,12 @@ int32_t rl_destroy(rl_relight* ctx);
  * (scene/Scene.cpp:98-142). add_mesh copies and transforms the mesh like Scene::addMesh +
  * Mesh::transformed (scene/Mesh.cpp:277-297): transform is 16 floats in the reference's
  * Matrix4 layout (Types.h:1188-1196) or NULL; albedo is the material colour (rgba) or NULL for
- * Face::colorAt's (1,1,1,1) default (scene/Mesh.cpp:478-485). Indices are 32-bit; meshes are not
- * limited to the reference's 16-bit Index (Relight.h:59). end() builds the BVH and uploads the
- * scene to HBM (the rtcCommit of rt/Embree.cpp:55-58). */
+ * Face::colorAt's (1,1,1,1) default (scene/Mesh.cpp:478-485). The colour is the photon
+ * reflectance (baker/Photons.cpp:119), and its alpha is the one shadow rays test (HitUseAlpha,
+ * rt/Embree.cpp:163-165: a hit with alpha <= 0.1 does not block the light; the reference's
+ * Material(Rgb) always has alpha 1, so a constant alpha here acts as a 1x1 RGBA texture would).
+ * Indices are 32-bit; meshes are not limited to the reference's 16-bit Index (Relight.h:59).
+ * end() builds the BVH and uploads the scene to HBM (the rtcCommit of rt/Embree.cpp:55-58). */
 int32_t rl_scene_create(rl_relight* ctx, rl_scene** out);
 int32_t rl_scene_destroy(rl_scene* scene);
 int32_t rl_scene_begin(rl_scene* scene);
@@ -184,6 +187,23 @@ int32_t rl_scene_add_mesh(rl_scene* scene, const float* positions, const float* 
                           const float* lightmap_uvs, uint32_t vertex_count,
                           const uint32_t* indices, uint32_t index_count, const float* transform,
                           const float* albedo, int32_t* out_mesh);
+/* Texture::create (scene/Material.h:95, scene/Material.cpp:78-82): width x height pixels of
+ * `channels` (3 or 4) bytes, copied. A scene keeps its own copy of the textures its meshes use,
+ * so a texture may be destroyed after rl_scene_add_mesh_textured. */
+typedef struct rl_texture rl_texture;
+int32_t rl_texture_create(rl_relight* ctx, int32_t width, int32_t height, int32_t channels,
+                          const uint8_t* pixels, rl_texture** out);
+int32_t rl_texture_destroy(rl_texture* texture);
+/* Scene::addMesh with a TexturedMaterial(texture, color) (scene/Material.cpp:61-70): the mesh's
+ * colour at a hit is Rgba(color, 1) times Texture::colorAt (scene/Material.cpp:87-98: texel
+ * (int(u*w) % w, int(v*h) % h), magenta outside) at the hit's diffuse uv (Face::colorAt, the
+ * Vertex::Diffuse layer, scene/Mesh.cpp:478-485), with 2 floats per vertex in diffuse_uvs. color
+ * NULL is white. Other arguments as rl_scene_add_mesh. */
+int32_t rl_scene_add_mesh_textured(rl_scene* scene, const float* positions, const float* normals,
+                                   const float* lightmap_uvs, const float* diffuse_uvs,
+                                   uint32_t vertex_count, const uint32_t* indices,
+                                   uint32_t index_count, const float* transform, const float* color,
+                                   const rl_texture* texture, int32_t* out_mesh);
 int32_t rl_scene_add_light(rl_scene* scene, const rl_light_desc* light, int32_t* out_light);
 int32_t rl_scene_end(rl_scene* scene);
 /* Scene statistics after end(): triangle count, BVH node count, traversal stack bound. */

@@ -90,6 +90,11 @@ def lib():
             "or_scene_destroy": (None, [C.c_void_p]),
             "or_scene_add_mesh": (C.c_int32, [C.c_void_p, _F, _F, _F, C.c_uint32, _U32, C.c_uint32, _F, _F]),
             "or_scene_add_light": (C.c_int32, [C.c_void_p, C.POINTER(or_light)]),
+            "or_scene_add_texture": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _U8]),
+            "or_scene_add_mesh_textured": (C.c_int32, [C.c_void_p, _F, _F, _F, _F, C.c_uint32, _U32, C.c_uint32,
+                                                       _F, _F, C.c_int32]),
+            "or_texture_color": (None, [C.c_void_p, C.c_int32, C.c_float, C.c_float, _F]),
+            "or_face_color": (None, [C.c_void_p, C.c_uint32, C.c_float, C.c_float, _F]),
             "or_scene_build": (C.c_int32, [C.c_void_p, C.c_int32]),
             "or_raster": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _U8, _U32, _F, _F]),
             "or_bake_direct": (C.c_int32, [C.c_void_p, C.c_uint32, _F, _F, _F, C.c_int32, _U64]),
@@ -177,6 +182,35 @@ class OracleScene:
         return self._L.or_scene_add_mesh(self._h, _f(p), _f(n), _f(uv), p.shape[0], _u(ix), ix.size,
                                          _f(tr), _f(al))
 
+    def add_texture(self, pixels):
+        """Texture::create from (h, w, channels) uint8 pixels; returns the texture index."""
+        px = np.ascontiguousarray(pixels, np.uint8)
+        self._keep.append(px)
+        return self._L.or_scene_add_texture(self._h, px.shape[1], px.shape[0], px.shape[2], px.ctypes.data_as(_U8))
+
+    def add_mesh_textured(self, positions, normals, uvs, diffuse_uvs, indices, texture, color=None, transform=None):
+        p = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+        n = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+        uv = np.ascontiguousarray(uvs, np.float32).reshape(-1, 2)
+        duv = np.ascontiguousarray(diffuse_uvs, np.float32).reshape(-1, 2)
+        ix = np.ascontiguousarray(indices, np.uint32).reshape(-1)
+        tr = None if transform is None else np.ascontiguousarray(transform, np.float32).reshape(16)
+        col = np.ascontiguousarray([1.0, 1.0, 1.0] if color is None else np.asarray(color)[:3], np.float32)
+        self._keep += [p, n, uv, duv, ix, tr, col]
+        self.meshes += 1
+        return self._L.or_scene_add_mesh_textured(self._h, _f(p), _f(n), _f(uv), _f(duv), p.shape[0], _u(ix), ix.size,
+                                                  _f(tr), _f(col), int(texture))
+
+    def texture_color(self, texture, u, v):
+        out = np.zeros(4, np.float32)
+        self._L.or_texture_color(self._h, int(texture), C.c_float(u), C.c_float(v), _f(out))
+        return out
+
+    def face_color(self, tri, u, v):
+        out = np.zeros(4, np.float32)
+        self._L.or_face_color(self._h, int(tri), C.c_float(u), C.c_float(v), _f(out))
+        return out
+
     def add_light(self, L):
         d = or_light()
         d.type = L.type
@@ -249,9 +283,15 @@ class OracleScene:
     @classmethod
     def from_spec(cls, spec, use_bvh=True):
         S = cls(use_bvh)
+        tex = [S.add_texture(px) for px in getattr(spec, "textures", [])]
         for m in spec.meshes:
-            S.add_mesh(m.positions, m.normals, m.uvs, m.indices, transform=getattr(m, "transform", None),
-                       albedo=m.albedo)
+            if getattr(m, "texture", None) is not None:
+                S.add_mesh_textured(m.positions, m.normals, m.uvs, m.diffuse_uvs, m.indices, tex[m.texture],
+                                    color=None if m.albedo is None else m.albedo[:3],
+                                    transform=getattr(m, "transform", None))
+            else:
+                S.add_mesh(m.positions, m.normals, m.uvs, m.indices, transform=getattr(m, "transform", None),
+                           albedo=m.albedo)
         for L in spec.lights:
             S.add_light(L)
         S.build()

@@ -225,7 +225,12 @@ struct Scene {
     std::vector<uint32_t> tmesh, tface;
     std::vector<float> tuv;       // 6 per triangle (lightmap uv of a, b, c)
     std::vector<float> tnrm;      // 9 per triangle
-    std::vector<float> talb;      // 4 per triangle
+    std::vector<float> talb;      // 4 per triangle (material colour rgba)
+    std::vector<int32_t> ttex;    // per triangle: texture index or -1 (TexturedMaterial)
+    std::vector<float> tduv;      // 6 per triangle (diffuse uv of a, b, c)
+    struct Tex { int32_t w, h, ch; std::vector<uint8_t> px; };
+    std::vector<Tex> textures;
+    bool alpha = false;           // some material can have alpha <= 0.1 (set by or_scene_build)
     // per mesh
     std::vector<std::vector<Vec>> mpos, mnrm;
     std::vector<std::vector<float>> muv;
@@ -426,6 +431,63 @@ bool trace(const Scene& S, const Seg& s, bool any, HitRec& h, Counters* ctr) {
     return h.tri >= 0;
 }
 
+// ------------------------------------------------------------------ materials
+// Texture::colorAt (scene/Material.cpp:87-98): texel (int(u * w) % w, int(v * h) % h) with C's
+// truncating conversion and remainder; magenta (1, 0, 1, 1) outside; channels / 255 (alpha 1
+// with 3 channels). A product outside the int range (undefined in the reference) is magenta.
+void texture_color(const Scene::Tex& T, float u, float v, float out[4]) {
+    const float fx = u * (float)T.w, fy = v * (float)T.h;
+    int x = -1, y = -1;
+    if (fx > -2147483648.0f && fx < 2147483648.0f && fy > -2147483648.0f && fy < 2147483648.0f) {
+        x = (int)fx % T.w;
+        y = (int)fy % T.h;
+    }
+    if (x >= 0 && x < T.w && y >= 0 && y < T.h) {
+        const uint8_t* p = &T.px[(size_t)y * T.w * T.ch + (size_t)x * T.ch];
+        out[0] = p[0] / 255.0f;
+        out[1] = p[1] / 255.0f;
+        out[2] = p[2] / 255.0f;
+        out[3] = T.ch == 4 ? p[3] / 255.0f : 1.0f;
+    } else {
+        out[0] = 1.0f; out[1] = 0.0f; out[2] = 1.0f; out[3] = 1.0f;
+    }
+}
+
+// Face::colorAt (scene/Mesh.cpp:478-485) at Embree's (u, v): the material colour, times the
+// texel at the diffuse uv a + (b - a) v + (c - a) u for a TexturedMaterial (Rgba(colour, 1) *
+// Texture::colorAt, scene/Material.cpp:67-70).
+void face_color(const Scene& S, uint32_t tri, float u, float v, float out[4]) {
+    const float* A = &S.talb[4 * (size_t)tri];
+    out[0] = A[0]; out[1] = A[1]; out[2] = A[2]; out[3] = A[3];
+    const int32_t t = S.ttex[tri];
+    if (t < 0) return;
+    const float* D = &S.tduv[6 * (size_t)tri];
+    const float du = D[0] + (D[2] - D[0]) * v + (D[4] - D[0]) * u;
+    const float dv = D[1] + (D[3] - D[1]) * v + (D[5] - D[1]) * u;
+    float c[4];
+    texture_color(S.textures[t], du, dv, c);
+    out[0] = out[0] * c[0]; out[1] = out[1] * c[1]; out[2] = out[2] * c[2]; out[3] = out[3] * c[3];
+}
+
+// The shadow test of LightInfluence::calculate / DirectionalLightInfluence::calculate
+// (scene/Light.cpp:385,417): Embree::traceSegment(start, end, HitUseAlpha) (rt/Embree.cpp:145-177)
+// as written: the closest hit blocks unless its colour's alpha is <= 0.1 and fewer than 25
+// re-traces were made, in which case it recurses from point + direction (one unit past the
+// hit). Each trace is a ray.
+bool shadow_hit(const Scene& S, Vec start, Vec end, uint64_t& rays, int step = 0) {
+    const Seg sg = make_seg(start, end);
+    HitRec h;
+    rays++;
+    // With no transparent material the closest hit exists exactly when any hit does.
+    if (!S.alpha) return trace(S, sg, true, h, nullptr);
+    if (!trace(S, sg, false, h, nullptr)) return false;
+    const Vec point = add(start, scale(sg.d, h.t));
+    float c[4];
+    face_color(S, (uint32_t)h.tri, h.u, h.v, c);
+    if (c[3] <= 0.1f && step < 25) return shadow_hit(S, add(point, sg.d), end, rays, step + 1);
+    return true;
+}
+
 // ------------------------------------------------------------------ bakers
 // DirectLight::influenceFromPoint (baker/DirectLight.cpp:101-125) and the light models of
 // scene/Light.cpp:372-500.
@@ -438,8 +500,7 @@ float influence(const Scene& S, const Light& L, Vec lp, Vec p, Vec n, uint64_t& 
         if (inf <= 0.001f) {
             inf = 0.0f;
         } else if (L.shadow) {
-            rays++;
-            if (trace(S, make_seg(p, sub(p, scale(L.dir, 1000.0f))), true, h, nullptr)) inf *= 0.0f;
+            if (shadow_hit(S, p, sub(p, scale(L.dir, 1000.0f)), rays)) inf *= 0.0f;
             else inf *= 1.0f;
         }
     } else {
@@ -450,8 +511,7 @@ float influence(const Scene& S, const Light& L, Vec lp, Vec p, Vec n, uint64_t& 
         if (inf <= 0.001f) {
             inf = 0.0f;
         } else if (L.shadow) {
-            rays++;
-            if (trace(S, make_seg(p, lp), true, h, nullptr)) inf *= 0.0f;
+            if (shadow_hit(S, p, lp, rays)) inf *= 0.0f;
             else inf *= 1.0f;
         }
     }
@@ -582,6 +642,8 @@ int32_t or_scene_add_mesh(void* sp, const float* pos, const float* nrm, const fl
             S.tnrm.push_back(N[v].x); S.tnrm.push_back(N[v].y); S.tnrm.push_back(N[v].z);
         }
         for (int k = 0; k < 4; k++) S.talb.push_back(albedo ? albedo[k] : 1.0f);
+        S.ttex.push_back(-1);
+        for (int k = 0; k < 6; k++) S.tduv.push_back(0.0f);
         S.tmesh.push_back(mesh);
         S.tface.push_back(f);
     }
@@ -591,6 +653,41 @@ int32_t or_scene_add_mesh(void* sp, const float* pos, const float* nrm, const fl
     S.midx.push_back(std::vector<uint32_t>(idx, idx + ni));
     S.mfirst.push_back(first);
     return (int32_t)mesh;
+}
+
+// Texture::create (scene/Material.cpp:78-82): a copy of w*h*ch bytes; returns its index.
+int32_t or_scene_add_texture(void* sp, int32_t w, int32_t h, int32_t ch, const uint8_t* px) {
+    Scene& S = *(Scene*)sp;
+    Scene::Tex t{w, h, ch, std::vector<uint8_t>(px, px + (size_t)w * h * ch)};
+    S.textures.push_back(std::move(t));
+    return (int32_t)S.textures.size() - 1;
+}
+
+// Scene::addMesh with TexturedMaterial(texture, colour): colour rgb (alpha 1), diffuse uvs
+// (2 floats per vertex).
+int32_t or_scene_add_mesh_textured(void* sp, const float* pos, const float* nrm, const float* uv, const float* duv,
+                                   uint32_t nv, const uint32_t* idx, uint32_t ni, const float* M, const float* color,
+                                   int32_t tex) {
+    Scene& S = *(Scene*)sp;
+    const float c4[4] = {color[0], color[1], color[2], 1.0f};
+    const uint32_t first = (uint32_t)S.ttex.size();
+    const int32_t mesh = or_scene_add_mesh(sp, pos, nrm, uv, nv, idx, ni, M, c4);
+    for (uint32_t f = 0; f < ni / 3; f++) {
+        S.ttex[first + f] = tex;
+        for (int k = 0; k < 3; k++) {
+            const uint32_t v = idx[3 * f + k];
+            S.tduv[6 * (size_t)(first + f) + 2 * k] = duv[2 * v];
+            S.tduv[6 * (size_t)(first + f) + 2 * k + 1] = duv[2 * v + 1];
+        }
+    }
+    return mesh;
+}
+
+// Face::colorAt of triangle `tri` at Embree barycentrics (u, v), and Texture::colorAt of texture
+// `tex` at (u, v) (for the pins).
+void or_face_color(void* sp, uint32_t tri, float u, float v, float* out) { face_color(*(Scene*)sp, tri, u, v, out); }
+void or_texture_color(void* sp, int32_t tex, float u, float v, float* out) {
+    texture_color(((Scene*)sp)->textures[tex], u, v, out);
 }
 
 int32_t or_scene_add_light(void* sp, const or_light* d) {
@@ -624,6 +721,14 @@ int32_t or_scene_build(void* sp, int32_t use_bvh) {
     Scene& S = *(Scene*)sp;
     S.use_bvh = use_bvh != 0;
     if (S.use_bvh) build_bvh(S);
+    std::vector<bool> cut(S.textures.size(), false);
+    for (size_t t = 0; t < S.textures.size(); t++)
+        if (S.textures[t].ch == 4)
+            for (size_t i = 3; i < S.textures[t].px.size(); i += 4)
+                if (S.textures[t].px[i] / 255.0f <= 0.1f) cut[t] = true;
+    S.alpha = false;
+    for (size_t i = 0; i < S.ttex.size(); i++)
+        if (S.ttex[i] >= 0 ? cut[S.ttex[i]] : S.talb[4 * i + 3] <= 0.1f) S.alpha = true;
     return 0;
 }
 
@@ -953,7 +1058,8 @@ int32_t or_emit_photons(void* sp, const or_indirect* s, const int32_t* mesh_pm, 
             const Vec hp = add(sg.o, scale(sg.d, h.t));
             const float* N = &S.tnrm[9 * (size_t)h.tri];
             const float* UV = &S.tuv[6 * (size_t)h.tri];
-            const float* A = &S.talb[4 * (size_t)h.tri];
+            float A[4];
+            face_color(S, (uint32_t)h.tri, h.u, h.v, A);
             const Vec nh = mk(flerp(N[0], N[3], N[6], h.u, h.v), flerp(N[1], N[4], N[7], h.u, h.v), flerp(N[2], N[5], N[8], h.u, h.v));
             const float ux = flerp(UV[0], UV[2], UV[4], h.u, h.v), uy = flerp(UV[1], UV[3], UV[5], h.u, h.v);
             float lam = dot3(neg(dir), nh);

@@ -231,6 +231,10 @@ SIGNATURES = [
     ("rl_scene_destroy", C.c_int32, [_P]),
     ("rl_scene_begin", C.c_int32, [_P]),
     ("rl_scene_add_mesh", C.c_int32, [_P, _F, _F, _F, C.c_uint32, _U32, C.c_uint32, _F, _F, _I32]),
+    ("rl_texture_create", C.c_int32, [_P, C.c_int32, C.c_int32, C.c_int32, C.POINTER(C.c_uint8), _PP]),
+    ("rl_texture_destroy", C.c_int32, [_P]),
+    ("rl_scene_add_mesh_textured", C.c_int32, [_P, _F, _F, _F, _F, C.c_uint32, _U32, C.c_uint32, _F, _F, _P,
+                                               _I32]),
     ("rl_scene_add_light", C.c_int32, [_P, C.POINTER(rl_light_desc), _I32]),
     ("rl_scene_end", C.c_int32, [_P]),
     ("rl_scene_info", C.c_int32, [_P, _U32, _U32, _I32]),

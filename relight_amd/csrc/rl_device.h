@@ -41,12 +41,30 @@ struct TriAttr {
     float albedo[4];
 };
 
+// A texture (Texture, scene/Material.h:66-123): width x height pixels of `channels` (3 or 4)
+// bytes, rows in the caller's order.
+struct TexDev {
+    const uint8_t* px;
+    int32_t w, h, channels;
+    int32_t pad;
+};
+
+// Per scene-order triangle of a scene with textured meshes: the diffuse uvs of its vertices a, b,
+// c (Vertex::Diffuse, scene/Mesh.h:37-72) and its texture (-1: the mesh's constant colour only).
+struct TriMat {
+    float duv[6];
+    int32_t tex;
+    uint32_t pad;
+};
+
 struct SceneDev {
     const Bvh4Node* nodes;
     const float4* tris;
     const TriAttr* attr;
     uint32_t num_nodes;
     uint32_t num_tris;
+    const TriMat* mat;    // NULL when no mesh of the scene is textured
+    const TexDev* texs;
 };
 
 // One light as the bake kernels see it (scene/Light.h models, flattened).

@@ -19,8 +19,10 @@ hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v);
 // Segments carry their scene, lights and AO settings; `cap` is the largest traversal-stack
 // capacity any of their scenes needs. work_begin units: lumels (direct, finalize), (lumel,
 // sample) pairs (AO chunked) or chunks of chunk_lumels lumels (AO refill).
+// alpha: some segment's scene has a material whose alpha can be <= 0.1, so shadow rays take the
+// closest-hit alpha re-trace (HitUseAlpha) instead of any-hit.
 hipError_t launch_bake_direct(hipStream_t st, int cap, uint32_t max_blocks, const BakeSeg* segs,
-                              int nsegs, uint64_t total, unsigned long long* queue,
+                              int nsegs, uint64_t total, bool alpha, unsigned long long* queue,
                               unsigned long long* rays);
 
 hipError_t launch_ao_count(hipStream_t st, int cap, uint32_t max_blocks, const BakeSeg* segs,

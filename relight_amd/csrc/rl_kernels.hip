@@ -35,7 +35,39 @@ __device__ __forceinline__ bool occluded(const SceneDev& S, const TopCache& top,
     return traverse<true, CAP>(S, top, r, stk, kDirectBlock, tr);
 }
 
+// Embree::traceSegment with HitUseAlpha (rt/Embree.cpp:145-177), as the shadow tests of
+// LightInfluence::calculate and DirectionalLightInfluence::calculate call it (scene/Light.cpp:385,
+// 417): the closest hit of start -> end blocks the light unless its colour's alpha is <= 0.1, in
+// which case the segment is traced again from the hit point plus one unit along the direction
+// (the reference's `point + direction`), at most 25 times; the 26th closest hit blocks whatever
+// its alpha. Every trace counts as a ray.
 template <int CAP>
+__device__ bool occluded_alpha(const SceneDev& S, const TopCache& top, V3 start, V3 end, uint32_t* stk,
+                               uint32_t& rays) {
+    for (int step = 0;; step++) {
+        const Ray r = ray_from_segment(start, end);
+        TraceResult tr;
+        rays++;
+        if (!traverse<false, CAP>(S, top, r, stk, kDirectBlock, tr)) return false;
+        const float4 c = face_color(S, tr.tri, tr.V / tr.det, tr.W / tr.det);
+        if (!(c.w <= 0.1f) || step >= 25) return true;
+        const V3 point = start + r.d * tr.t;
+        start = point + r.d;
+    }
+}
+
+// A shadow segment p -> e: any hit (ALPHA = false: no material of the scene can be transparent,
+// and then the closest hit exists exactly when any hit does) or the alpha re-trace.
+template <int CAP, bool ALPHA>
+__device__ __forceinline__ bool shadowed(const SceneDev& S, const TopCache& top, V3 p, V3 e, uint32_t* stk,
+                                         uint32_t& rays) {
+    if (ALPHA) return occluded_alpha<CAP>(S, top, p, e, stk, rays);
+    Ray r = ray_from_segment(p, e);
+    rays++;
+    return occluded<CAP>(S, top, r, stk);
+}
+
+template <int CAP, bool ALPHA>
 __device__ float influence_from_point(const SceneDev& S, const TopCache& top, const LightDev& L, V3 lp, V3 p, V3 n,
                                       uint32_t* stk, uint32_t& rays) {
     float inf = 0.0f, att = 1.0f, cut = 1.0f, distance = 0.0f;
@@ -45,9 +77,7 @@ __device__ float influence_from_point(const SceneDev& S, const TopCache& top, co
         if (inf <= 0.001f) {
             inf = 0.0f;
         } else if (L.casts_shadow) {
-            Ray r = ray_from_segment(p, p - ld * 1000.0f);
-            rays++;
-            inf *= occluded<CAP>(S, top, r, stk) ? 0.0f : 1.0f;
+            inf *= shadowed<CAP, ALPHA>(S, top, p, p - ld * 1000.0f, stk, rays) ? 0.0f : 1.0f;
         }
     } else {
         V3 dir = lp - p;
@@ -56,9 +86,7 @@ __device__ float influence_from_point(const SceneDev& S, const TopCache& top, co
         if (inf <= 0.001f) {
             inf = 0.0f;
         } else if (L.casts_shadow) {
-            Ray r = ray_from_segment(p, lp);
-            rays++;
-            inf *= occluded<CAP>(S, top, r, stk) ? 0.0f : 1.0f;
+            inf *= shadowed<CAP, ALPHA>(S, top, p, lp, stk, rays) ? 0.0f : 1.0f;
         }
     }
     if (L.has_spot) {
@@ -83,7 +111,7 @@ __device__ float influence_from_point(const SceneDev& S, const TopCache& top, co
 // 21.2 ms without the cache, 20.0 ms with 120 nodes per two-wavefront workgroup, 19.6 ms so.
 constexpr uint32_t kDirectTopNodes = 880;
 
-template <int CAP>
+template <int CAP, bool ALPHA>
 __global__ __launch_bounds__(kDirectBlock) void k_bake_direct(const BakeSeg* __restrict__ segs, int nsegs,
                                                               uint64_t total, unsigned long long* queue,
                                                               unsigned long long* rays_out) {
@@ -116,7 +144,7 @@ __global__ __launch_bounds__(kDirectBlock) void k_bake_direct(const BakeSeg* __r
                 for (int v = 0; v < L.num_vertices; v++) {
                     const float4 lv = sg.lverts[L.vertex_offset + v];
                     V3 lp = v3(lv.x + L.pos[0], lv.y + L.pos[1], lv.z + L.pos[2]);
-                    float inf = influence_from_point<CAP>(S, top, L, lp, p, n, stk, rays);
+                    float inf = influence_from_point<CAP, ALPHA>(S, top, L, lp, p, n, stk, rays);
                     if (inf > 0.0f) {
                         cr += L.color[0] * L.intensity * inf;
                         cg += L.color[1] * L.intensity * inf;
@@ -127,7 +155,7 @@ __global__ __launch_bounds__(kDirectBlock) void k_bake_direct(const BakeSeg* __r
                 cr = cr / nv; cg = cg / nv; cb = cb / nv;
             } else {
                 // DirectLight::lightFromPoint (baker/DirectLight.cpp:63-72).
-                float inf = influence_from_point<CAP>(S, top, L, v3(L.pos[0], L.pos[1], L.pos[2]), p, n, stk, rays);
+                float inf = influence_from_point<CAP, ALPHA>(S, top, L, v3(L.pos[0], L.pos[1], L.pos[2]), p, n, stk, rays);
                 if (inf > 0.0f) {
                     cr = L.color[0] * L.intensity * inf;
                     cg = L.color[1] * L.intensity * inf;
@@ -461,7 +489,7 @@ hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v) 
 }
 
 hipError_t launch_bake_direct(hipStream_t st, int cap, uint32_t max_blocks, const BakeSeg* segs, int nsegs,
-                              uint64_t total, unsigned long long* queue, unsigned long long* rays) {
+                              uint64_t total, bool alpha, unsigned long long* queue, unsigned long long* rays) {
     if (total == 0) return hipSuccess;
     hipError_t e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(queue), 2, 0u);
     if (e != hipSuccess) return e;
@@ -469,8 +497,13 @@ hipError_t launch_bake_direct(hipStream_t st, int cap, uint32_t max_blocks, cons
     const uint64_t need = ((total + 63) / 64 + (kDirectBlock / 64) - 1) / (kDirectBlock / 64);
     const uint32_t cap_blocks = std::max<uint32_t>(1u, max_blocks * (uint32_t)kBlock / (uint32_t)kDirectBlock);
     const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(need, cap_blocks));
-    RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_bake_direct<CAP>, dim3(grid), dim3(kDirectBlock), 0, st, segs, nsegs, total,
-                                            queue, rays));
+    if (alpha) {
+        RL_DISPATCH_CAP(cap, hipLaunchKernelGGL((k_bake_direct<CAP, true>), dim3(grid), dim3(kDirectBlock), 0, st, segs,
+                                                nsegs, total, queue, rays));
+    } else {
+        RL_DISPATCH_CAP(cap, hipLaunchKernelGGL((k_bake_direct<CAP, false>), dim3(grid), dim3(kDirectBlock), 0, st, segs,
+                                                nsegs, total, queue, rays));
+    }
     return hipGetLastError();
 }
 

@@ -6,6 +6,7 @@
 // IEEE correctly rounded on both sides (HIP's default for fp32 div/sqrt on gfx950).
 #pragma once
 
+#include <stddef.h>
 #include <stdint.h>
 #include <math.h>
 
@@ -284,6 +285,28 @@ RL_HD float luminance(float r, float g, float b) { return 0.2126f * r + 0.7152f 
 // Face::*At interpolation with Embree's hit barycentrics (rt/Embree.cpp:160,169-171 pass (u, v)
 // straight into Face::normalAt/uvAt, scene/Mesh.cpp:454-495): a + (b - a) * v + (c - a) * u.
 RL_HD float face_lerp(float a, float b, float c, float u, float v) { return a + (b - a) * v + (c - a) * u; }
+
+// Texture::colorAt (scene/Material.cpp:87-98): texel (int(u * w) % w, int(v * h) % h) with C's
+// truncating conversion and remainder, so a negative coordinate below -1 texel gives the
+// reference's magenta (1, 0, 1, 1); channels / 255 (alpha 1 for 3-channel textures). A product
+// u * w outside the int range (undefined in the reference) is also magenta.
+RL_HD void texture_color(const uint8_t* px, int w, int h, int channels, float u, float v, float out[4]) {
+    const float fx = u * (float)w, fy = v * (float)h;
+    int x = -1, y = -1;
+    if (fx > -2147483648.0f && fx < 2147483648.0f && fy > -2147483648.0f && fy < 2147483648.0f) {
+        x = (int)fx % w;
+        y = (int)fy % h;
+    }
+    if (x >= 0 && x < w && y >= 0 && y < h) {
+        const uint8_t* p = px + ((size_t)y * (size_t)w * (size_t)channels + (size_t)x * (size_t)channels);
+        out[0] = p[0] / 255.0f;
+        out[1] = p[1] / 255.0f;
+        out[2] = p[2] / 255.0f;
+        out[3] = channels == 4 ? p[3] / 255.0f : 1.0f;
+    } else {
+        out[0] = 1.0f; out[1] = 0.0f; out[2] = 1.0f; out[3] = 1.0f;
+    }
+}
 
 // Lightmap::lumel(const Uv&) (Lightmap.cpp:61-67): int(uv * (size - 1)); clamped to the map
 // (the reference asserts instead).

@@ -47,6 +47,11 @@ __device__ __forceinline__ HitAttr hit_attributes(const SceneDev& S, const Trace
     h.uvy = face_lerp(A.uv[1], A.uv[3], A.uv[5], u, v);
     h.mesh = A.mesh;
     h.albedo[0] = A.albedo[0]; h.albedo[1] = A.albedo[1]; h.albedo[2] = A.albedo[2]; h.albedo[3] = A.albedo[3];
+    if (S.mat) {
+        // Rgb(hit.m_color) of Photons::trace (baker/Photons.cpp:119) for textured materials.
+        const float4 c = face_color(S, tr.tri, u, v);
+        h.albedo[0] = c.x; h.albedo[1] = c.y; h.albedo[2] = c.z; h.albedo[3] = c.w;
+    }
     return h;
 }
 

@@ -72,11 +72,21 @@ void dev_free(T*& p) {
     p = nullptr;
 }
 
+// A texture's pixels (Texture::create copies them, scene/Material.cpp:78-82). cutout: some texel's
+// alpha is <= 0.1 (4 channels), so shadow rays through its meshes need the alpha re-trace.
+struct TexHost {
+    int32_t w = 0, h = 0, channels = 0;
+    std::vector<uint8_t> px;
+    bool cutout = false;
+};
+
 struct MeshHost {
     std::vector<V3> pos, nrm;
     std::vector<float> uv;  // lightmap uv, 2 per vertex
+    std::vector<float> duv; // diffuse uv, 2 per vertex (textured meshes)
     std::vector<uint32_t> idx;
     float albedo[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+    int32_t tex = -1;       // index into the scene's textures (TexturedMaterial), -1: constant colour
     uint32_t first_tri = 0;
     uint32_t num_tris = 0;
     rl_lightmap* lightmap = nullptr;
@@ -142,6 +152,13 @@ struct rl_scene {
     LightDev* d_lights = nullptr;
     float4* d_lverts = nullptr;
     std::vector<uint32_t> tri_mesh, tri_face;
+    // Textures of the scene's textured meshes (shared with the rl_texture handles they came from)
+    // and their device copies; alpha_shadows: some material can have alpha <= 0.1.
+    std::vector<std::shared_ptr<const TexHost>> textures;
+    TriMat* d_mat = nullptr;
+    TexDev* d_texs = nullptr;
+    uint8_t* d_texpx = nullptr;
+    bool alpha_shadows = false;
 
     SceneDev dev() const {
         SceneDev s;
@@ -150,6 +167,8 @@ struct rl_scene {
         s.attr = d_attr;
         s.num_nodes = num_nodes;
         s.num_tris = num_tris;
+        s.mat = d_mat;
+        s.texs = d_texs;
         return s;
     }
     ~rl_scene() {
@@ -158,7 +177,14 @@ struct rl_scene {
         dev_free(d_attr);
         dev_free(d_lights);
         dev_free(d_lverts);
+        dev_free(d_mat);
+        dev_free(d_texs);
+        dev_free(d_texpx);
     }
+};
+
+struct rl_texture {
+    std::shared_ptr<const TexHost> t;
 };
 
 struct rl_lightmap {
@@ -392,6 +418,7 @@ struct DeviceSegs {
     uint64_t total = 0;   // work items in `unit`
     uint64_t texels = 0;  // lumels covered
     int cap = 32;         // traversal-stack capacity for every scene in the table
+    bool alpha = false;   // some scene in the table needs the alpha re-trace for shadow rays
 };
 
 AoParams ao_params(const rl_ao_settings& s) {
@@ -457,6 +484,7 @@ std::vector<BakeSeg> build_segments(const std::vector<BakeItem>& items, int unit
         else acc += (it.count + chunk_lumels - 1) / chunk_lumels;
         out.texels += it.count;
         out.cap = std::max(out.cap, it.scene->cap);
+        out.alpha = out.alpha || it.scene->alpha_shadows;
     }
     out.total = acc;
     out.n = (int)segs.size();
@@ -506,7 +534,7 @@ int32_t run_direct_items(rl_relight* ctx, const std::vector<BakeItem>& items, bo
     if (ds.total) {
         e = kernel_begin(ctx, RL_KERNEL_DIRECT);
         if (e == hipSuccess) e = launch_bake_direct(ctx->stream, ds.cap, ctx->max_blocks, ds.d, ds.n, ds.total,
-                                                    ctx->d_queue, ctx->d_rays);
+                                                    ds.alpha, ctx->d_queue, ctx->d_rays);
         if (e == hipSuccess) e = kernel_end(ctx);
     }
     ctx->stats.texels += ds.texels;
@@ -776,10 +804,16 @@ int32_t rl_scene_begin(rl_scene* scene) {
     return RL_SUCCESS;
 }
 
-int32_t rl_scene_add_mesh(rl_scene* scene, const float* positions, const float* normals,
-                          const float* lightmap_uvs, uint32_t vertex_count,
-                          const uint32_t* indices, uint32_t index_count, const float* transform,
-                          const float* albedo, int32_t* out_mesh) {
+}  // extern "C"
+
+namespace {
+
+// Scene::addMesh (scene/Scene.cpp:98-107) with a Material (colour rgba, or (1,1,1,1) when albedo
+// is NULL) or a TexturedMaterial (texture != NULL: colour rgb with alpha 1, diffuse uvs).
+int32_t add_mesh_impl(rl_scene* scene, const float* positions, const float* normals, const float* lightmap_uvs,
+                      const float* diffuse_uvs, uint32_t vertex_count, const uint32_t* indices, uint32_t index_count,
+                      const float* transform, const float* albedo, int albedo_channels, const rl_texture* texture,
+                      int32_t* out_mesh) {
     if (!scene) return fail(RL_INVALID_ARGUMENT, "scene is NULL");
     if (scene->state != kAdding) return fail(RL_INVALID_CALL, "add_mesh outside begin/end");
     if (!positions || !normals || !lightmap_uvs || !indices)
@@ -812,11 +846,73 @@ int32_t rl_scene_add_mesh(rl_scene* scene, const float* positions, const float* 
         m.pos[i] = p;
         m.nrm[i] = n;
     }
-    if (albedo) std::memcpy(m.albedo, albedo, sizeof(m.albedo));
+    if (albedo)
+        for (int k = 0; k < albedo_channels; k++) m.albedo[k] = albedo[k];
+    if (texture) {
+        m.duv.assign(diffuse_uvs, diffuse_uvs + 2 * (size_t)vertex_count);
+        int32_t ti = -1;
+        for (size_t k = 0; k < scene->textures.size(); k++)
+            if (scene->textures[k] == texture->t) ti = (int32_t)k;
+        if (ti < 0) {
+            ti = (int32_t)scene->textures.size();
+            scene->textures.push_back(texture->t);
+        }
+        m.tex = ti;
+    }
     m.num_tris = index_count / 3;
     scene->meshes.push_back(std::move(m));
     if (out_mesh) *out_mesh = (int32_t)scene->meshes.size() - 1;
     return RL_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t rl_scene_add_mesh(rl_scene* scene, const float* positions, const float* normals,
+                          const float* lightmap_uvs, uint32_t vertex_count,
+                          const uint32_t* indices, uint32_t index_count, const float* transform,
+                          const float* albedo, int32_t* out_mesh) {
+    return add_mesh_impl(scene, positions, normals, lightmap_uvs, nullptr, vertex_count, indices, index_count,
+                         transform, albedo, 4, nullptr, out_mesh);
+}
+
+int32_t rl_texture_create(rl_relight* ctx, int32_t width, int32_t height, int32_t channels, const uint8_t* pixels,
+                          rl_texture** out) {
+    if (!ctx || !out || !pixels) return fail(RL_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    if (width <= 0 || height <= 0) return fail(RL_INVALID_ARGUMENT, "texture size must be positive");
+    if (channels != 3 && channels != 4) return fail(RL_INVALID_ARGUMENT, "textures have 3 or 4 channels");
+    auto t = std::make_shared<TexHost>();
+    t->w = width;
+    t->h = height;
+    t->channels = channels;
+    t->px.assign(pixels, pixels + (size_t)width * (size_t)height * (size_t)channels);
+    if (channels == 4)
+        for (size_t i = 3; i < t->px.size(); i += 4)
+            if (t->px[i] / 255.0f <= 0.1f) t->cutout = true;
+    rl_texture* h = new (std::nothrow) rl_texture();
+    if (!h) return fail(RL_OUT_OF_MEMORY, "texture allocation failed");
+    h->t = std::move(t);
+    *out = h;
+    return RL_SUCCESS;
+}
+
+int32_t rl_texture_destroy(rl_texture* tex) {
+    delete tex;
+    return RL_SUCCESS;
+}
+
+int32_t rl_scene_add_mesh_textured(rl_scene* scene, const float* positions, const float* normals,
+                                   const float* lightmap_uvs, const float* diffuse_uvs, uint32_t vertex_count,
+                                   const uint32_t* indices, uint32_t index_count, const float* transform,
+                                   const float* color, const rl_texture* texture, int32_t* out_mesh) {
+    if (!texture) return fail(RL_INVALID_ARGUMENT, "texture is NULL");
+    if (!diffuse_uvs) return fail(RL_INVALID_ARGUMENT, "diffuse uvs are NULL");
+    // TexturedMaterial(texture, color): Material::colorAt is Rgba(color) with alpha 1.
+    const float white[3] = {1.0f, 1.0f, 1.0f};
+    return add_mesh_impl(scene, positions, normals, lightmap_uvs, diffuse_uvs, vertex_count, indices, index_count,
+                         transform, color ? color : white, 3, texture, out_mesh);
 }
 
 int32_t rl_scene_add_light(rl_scene* scene, const rl_light_desc* d, int32_t* out_light) {
@@ -925,6 +1021,46 @@ int32_t rl_scene_end(rl_scene* scene) {
         tris[3 * (size_t)i + 1] = make_float4(v[3], v[4], v[5], 0.0f);
         tris[3 * (size_t)i + 2] = make_float4(v[6], v[7], v[8], 0.0f);
     }
+    // Textured meshes: per-triangle diffuse uvs and texture, and the textures' pixels.
+    bool textured = false, alpha = false;
+    for (const MeshHost& m : scene->meshes) {
+        if (m.tex >= 0) {
+            textured = true;
+            if (scene->textures[m.tex]->cutout) alpha = true;
+        } else if (m.albedo[3] <= 0.1f && m.num_tris > 0) {
+            alpha = true;
+        }
+    }
+    std::vector<TriMat> mats;
+    std::vector<TexDev> texs;
+    std::vector<uint8_t> texpx;
+    if (textured) {
+        mats.resize(ntri);
+        for (const MeshHost& m : scene->meshes) {
+            for (uint32_t f = 0; f < m.num_tris; f++) {
+                TriMat& tm = mats[m.first_tri + f];
+                tm.tex = m.tex;
+                tm.pad = 0;
+                for (int k = 0; k < 3; k++) {
+                    const uint32_t vi = m.idx[3 * f + k];
+                    tm.duv[2 * k] = m.tex >= 0 ? m.duv[2 * vi] : 0.0f;
+                    tm.duv[2 * k + 1] = m.tex >= 0 ? m.duv[2 * vi + 1] : 0.0f;
+                }
+            }
+        }
+        std::vector<size_t> offs;
+        for (const auto& t : scene->textures) {
+            offs.push_back(texpx.size());
+            texpx.insert(texpx.end(), t->px.begin(), t->px.end());
+        }
+        for (size_t k = 0; k < scene->textures.size(); k++) {
+            TexDev d{};
+            d.w = scene->textures[k]->w;
+            d.h = scene->textures[k]->h;
+            d.channels = scene->textures[k]->channels;
+            texs.push_back(d);
+        }
+    }
     std::vector<LightDev> lights;
     std::vector<float4> lverts;
     for (LightHost& L : scene->lights) {
@@ -947,6 +1083,20 @@ int32_t rl_scene_end(rl_scene* scene) {
         RL_HIP(dev_alloc(&scene->d_lverts, lverts.size()));
         RL_HIP(hipMemcpy(scene->d_lverts, lverts.data(), lverts.size() * sizeof(float4), hipMemcpyHostToDevice));
     }
+    if (textured) {
+        RL_HIP(dev_alloc(&scene->d_texpx, texpx.size()));
+        RL_HIP(hipMemcpy(scene->d_texpx, texpx.data(), texpx.size(), hipMemcpyHostToDevice));
+        size_t off = 0;
+        for (size_t k = 0; k < texs.size(); k++) {
+            texs[k].px = scene->d_texpx + off;
+            off += scene->textures[k]->px.size();
+        }
+        RL_HIP(dev_alloc(&scene->d_texs, texs.size()));
+        RL_HIP(hipMemcpy(scene->d_texs, texs.data(), texs.size() * sizeof(TexDev), hipMemcpyHostToDevice));
+        RL_HIP(dev_alloc(&scene->d_mat, mats.size()));
+        RL_HIP(hipMemcpy(scene->d_mat, mats.data(), mats.size() * sizeof(TriMat), hipMemcpyHostToDevice));
+    }
+    scene->alpha_shadows = alpha;
     scene->num_tris = (uint32_t)ntri;
     scene->num_nodes = (uint32_t)bvh.nodes.size();
     scene->stack_bound = bvh.stack_bound;
@@ -1265,8 +1415,8 @@ hipError_t pass_launches(rl_relight* ctx, const rl_pass& p, const DeviceSegs& di
                          const DeviceSegs& lum, bool refill, uint32_t chunk, const std::vector<BakeItem>& ao_items) {
     hipError_t e = hipSuccess;
     if (dir.total)
-        e = launch_bake_direct(ctx->stream, dir.cap, ctx->max_blocks, p.d_segs[0], dir.n, dir.total, ctx->d_queue,
-                               ctx->d_rays);
+        e = launch_bake_direct(ctx->stream, dir.cap, ctx->max_blocks, p.d_segs[0], dir.n, dir.total, dir.alpha,
+                               ctx->d_queue, ctx->d_rays);
     if (e == hipSuccess && work.total) {
         if (refill) {
             e = launch_ao_refill(ctx->stream, work.cap, ctx->max_blocks, p.d_segs[1], work.n, work.total, chunk,

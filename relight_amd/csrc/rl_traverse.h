@@ -506,6 +506,25 @@ __device__ __forceinline__ bool traverse(const SceneDev& S, const Ray& ray, uint
     return traverse<ANY, CAP>(S, TopCache{nullptr, 0u}, ray, stk, stride, res);
 }
 
+// Face::colorAt (scene/Mesh.cpp:478-485) at Embree's (u, v): the mesh's material colour (rgba;
+// (1, 1, 1, 1) without a material), times Texture::colorAt at the diffuse uv for a textured
+// material (TexturedMaterial::colorAt, scene/Material.cpp:67-70: Rgba(colour, 1) * texel).
+__device__ __forceinline__ float4 face_color(const SceneDev& S, uint32_t tri, float u, float v) {
+    const TriAttr& A = S.attr[tri];
+    float4 c = make_float4(A.albedo[0], A.albedo[1], A.albedo[2], A.albedo[3]);
+    if (S.mat) {
+        const TriMat m = S.mat[tri];
+        if (m.tex >= 0) {
+            const TexDev T = S.texs[m.tex];
+            float tc[4];
+            texture_color(T.px, T.w, T.h, T.channels, face_lerp(m.duv[0], m.duv[2], m.duv[4], u, v),
+                          face_lerp(m.duv[1], m.duv[3], m.duv[5], u, v), tc);
+            c = make_float4(c.x * tc[0], c.y * tc[1], c.z * tc[2], c.w * tc[3]);
+        }
+    }
+    return c;
+}
+
 __device__ __forceinline__ int find_seg(const BakeSeg* segs, int nsegs, uint64_t g) {
     int lo = 0, hi = nsegs - 1;
     while (lo < hi) {

@@ -202,6 +202,31 @@ class LightDesc:
         return d
 
 
+class Texture:
+    """Texture::create (scene/Material.h:95): (h, w, channels) uint8 pixels, channels 3 or 4."""
+
+    def __init__(self, relight: "Relight", pixels):
+        px = np.ascontiguousarray(pixels, np.uint8)
+        if px.ndim != 3 or px.shape[2] not in (3, 4):
+            raise ValueError("pixels must be (height, width, 3 or 4) uint8")
+        self._lib = relight._lib
+        h = C.c_void_p()
+        _check(self._lib, self._lib.rl_texture_create(relight._h, px.shape[1], px.shape[0], px.shape[2],
+                                                      px.ctypes.data_as(C.POINTER(C.c_uint8)), C.byref(h)))
+        self._h = h
+
+    def close(self):
+        if self._h:
+            self._lib.rl_texture_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Scene:
     """Scene (scene/Scene.h): begin -> add_mesh/add_light -> end (BVH build + upload)."""
 
@@ -228,14 +253,25 @@ class Scene:
         _check(self._lib, self._lib.rl_scene_begin(self._h))
 
     def add_mesh(self, positions, normals, lightmap_uvs, indices, transform=None,
-                 albedo=None) -> int:
+                 albedo=None, texture: Optional["Texture"] = None, diffuse_uvs=None) -> int:
+        """Scene::addMesh with a Material (albedo rgba) or, when `texture` is given, a
+        TexturedMaterial(texture, albedo[:3]) sampled at `diffuse_uvs`."""
         p = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
         n = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
         uv = np.ascontiguousarray(lightmap_uvs, np.float32).reshape(-1, 2)
         ix = np.ascontiguousarray(indices, np.uint32).reshape(-1)
         tr = None if transform is None else np.ascontiguousarray(transform, np.float32).reshape(16)
-        al = None if albedo is None else np.ascontiguousarray(albedo, np.float32).reshape(4)
         out = C.c_int32()
+        if texture is not None:
+            duv = np.ascontiguousarray(diffuse_uvs, np.float32).reshape(-1, 2)
+            if duv.shape[0] != p.shape[0]:
+                raise ValueError("diffuse_uvs needs one uv per vertex")
+            col = None if albedo is None else np.ascontiguousarray(np.asarray(albedo)[:3], np.float32)
+            _check(self._lib, self._lib.rl_scene_add_mesh_textured(
+                self._h, _fptr(p), _fptr(n), _fptr(uv), _fptr(duv), p.shape[0], _uptr(ix), ix.size,
+                _fptr(tr), _fptr(col), texture._h, C.byref(out)))
+            return out.value
+        al = None if albedo is None else np.ascontiguousarray(albedo, np.float32).reshape(4)
         _check(self._lib, self._lib.rl_scene_add_mesh(
             self._h, _fptr(p), _fptr(n), _fptr(uv), p.shape[0], _uptr(ix), ix.size, _fptr(tr),
             _fptr(al), C.byref(out)))
@@ -451,6 +487,9 @@ class Relight:
 
     def create_lightmap(self, width: int, height: int) -> Lightmap:
         return Lightmap(self, width, height)
+
+    def create_texture(self, pixels) -> Texture:
+        return Texture(self, pixels)
 
     def create_photonmap(self, width: int, height: int) -> Photonmap:
         return Photonmap(self, width, height)

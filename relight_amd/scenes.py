@@ -38,6 +38,8 @@ class MeshSpec:
     albedo: Optional[np.ndarray] = None
     photonmap: int = -1
     transform: Optional[np.ndarray] = None  # 16 floats, the reference's Matrix4 layout
+    texture: Optional[int] = None           # index into SceneSpec.textures (TexturedMaterial)
+    diffuse_uvs: Optional[np.ndarray] = None
 
     @property
     def num_tris(self):
@@ -54,6 +56,7 @@ class SceneSpec:
     params: dict = field(default_factory=dict)
     photonmaps: List[Tuple[int, int]] = field(default_factory=list)
     indirect: Optional[IndirectLightSettings] = None
+    textures: List[np.ndarray] = field(default_factory=list)  # (h, w, 3 or 4) uint8 pixels
 
     @property
     def num_tris(self):
@@ -389,9 +392,11 @@ def build_gpu(relight, spec: SceneSpec):
     scene = relight.create_scene()
     scene.begin()
     ids = []
+    textures = [relight.create_texture(px) for px in spec.textures]
     for m in spec.meshes:
+        tex = None if m.texture is None else textures[m.texture]
         ids.append(scene.add_mesh(m.positions, m.normals, m.uvs, m.indices, transform=m.transform,
-                                  albedo=m.albedo))
+                                  albedo=m.albedo, texture=tex, diffuse_uvs=m.diffuse_uvs))
     for L in spec.lights:
         scene.add_light(L)
     lms = [relight.create_lightmap(w, h) for (w, h) in spec.lightmaps]
