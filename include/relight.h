@@ -380,8 +380,9 @@ int32_t rl_bake_scene(rl_relight* ctx, rl_scene* scene, int32_t direct, const rl
  * loop takes up to quantum_rays rays of direct, AO and final-gather work from all admitted jobs,
  * fairly shared, into one multi-scene launch per pass (continuous batching: a job submitted while
  * others run joins at the next step); photon emission and radiosity phases run as whole steps. Scenes, lightmaps
- * and radiosity objects are borrowed until the job finishes; while a server runs, make no other
- * call on its context. Results are identical to the same job run alone. */
+ * (and the scene's photonmaps) and radiosity objects are borrowed until the job finishes, fails or
+ * is cancelled: their destroy calls return RL_INVALID_CALL meanwhile. While a server runs, make no
+ * other call on its context. Results are identical to the same job run alone. */
 #define RL_JOB_DIRECT_AO 0
 #define RL_JOB_INDIRECT 1      /* photons, then final gather per lightmap */
 #define RL_JOB_RADIOSITY 2

@@ -159,6 +159,7 @@ struct rl_scene {
     TexDev* d_texs = nullptr;
     uint8_t* d_texpx = nullptr;
     bool alpha_shadows = false;
+    std::atomic<int32_t> server_jobs{0};  // unfinished server jobs that use the scene
 
     SceneDev dev() const {
         SceneDev s;
@@ -190,6 +191,7 @@ struct rl_texture {
 struct rl_lightmap {
     rl_relight* ctx = nullptr;
     rl_scene* scene = nullptr;
+    std::atomic<int32_t> server_jobs{0};  // unfinished server jobs that bake into the lightmap
     int w = 0, h = 0;
     uint32_t seed = 0;
     // Host lumel grid (Lightmap::m_lumels).
@@ -793,6 +795,8 @@ int32_t rl_scene_create(rl_relight* ctx, rl_scene** out) {
 }
 
 int32_t rl_scene_destroy(rl_scene* scene) {
+    if (scene && scene->server_jobs.load() > 0)
+        return fail(RL_INVALID_CALL, "scene is in use by an unfinished server job");
     delete scene;
     return RL_SUCCESS;
 }
@@ -1134,6 +1138,8 @@ int32_t rl_lightmap_create(rl_relight* ctx, int32_t width, int32_t height, rl_li
 }
 
 int32_t rl_lightmap_destroy(rl_lightmap* lm) {
+    if (lm && lm->server_jobs.load() > 0)
+        return fail(RL_INVALID_CALL, "lightmap is in use by an unfinished server job");
     delete lm;
     return RL_SUCCESS;
 }
@@ -1835,6 +1841,7 @@ int32_t rl_bake_indirect(rl_relight* ctx, rl_scene* scene, int32_t mesh, const r
 struct rl_radiosity {
     rl_relight* ctx = nullptr;
     rl_scene* scene = nullptr;
+    std::atomic<int32_t> server_jobs{0};  // unfinished server jobs that solve it
     uint32_t n = 0, npad = 0;
     PatchDev* d_patches = nullptr;
     uint32_t* d_tri_patch = nullptr;
@@ -1896,6 +1903,8 @@ int32_t rl_radiosity_create(rl_relight* ctx, rl_scene* scene, const float* patch
 }
 
 int32_t rl_radiosity_destroy(rl_radiosity* r) {
+    if (r && r->server_jobs.load() > 0)
+        return fail(RL_INVALID_CALL, "radiosity object is in use by an unfinished server job");
     delete r;
     return RL_SUCCESS;
 }
@@ -2150,6 +2159,23 @@ struct Job {
     bool prepared = false;                 // lumels uploaded
     std::unique_ptr<PhotonTables> tables;  // final gather: the scene's photon tables
     std::string error;    // message of the failing call, handed to the caller's rl_last_error
+    bool holds = false;   // counted in the server_jobs of its scene, lightmaps and radiosity
+
+    // Marks the job's scene, lightmaps and radiosity object as used (at submit) or no longer used
+    // (once the job has finished, failed or been cancelled): their destroy calls refuse meanwhile.
+    void hold(bool on) {
+        if (holds == on) return;
+        holds = on;
+        const int32_t d = on ? 1 : -1;
+        if (desc.scene) {
+            desc.scene->server_jobs += d;
+            // Photon steps store into, and final gathers read, the scene's photonmaps.
+            for (const MeshHost& m : desc.scene->meshes)
+                if (m.photonmap) m.photonmap->server_jobs += d;
+        }
+        for (rl_lightmap* lm : maps) lm->server_jobs += d;
+        if (desc.kind == RL_JOB_RADIOSITY && desc.radiosity) desc.radiosity->server_jobs += d;
+    }
 };
 
 }  // namespace
@@ -2336,6 +2362,7 @@ void rl_server::admit_locked() {
         if (j->cancel) {
             queue.pop_front();
             j->state = RL_JOB_CANCELLED;
+            j->hold(false);
             j->done_ms = now_ms();
             stats.cancelled++;
             cv_done.notify_all();
@@ -2361,6 +2388,7 @@ void rl_server::retire_locked() {
         if (job_finished(j)) {
             j->done_ms = now_ms();
             j->tables.reset();
+            j->hold(false);
             mem_used -= j->footprint;
             if (j->state == RL_JOB_DONE) stats.completed++;
             else if (j->state == RL_JOB_FAILED) stats.failed++;
@@ -2604,6 +2632,7 @@ int32_t rl_server_submit(rl_server* s, const rl_job_desc* d, uint64_t* out_id) {
     j->id = s->next_id++;
     j->submit_ms = s->now_ms();
     Job* raw = j.get();
+    raw->hold(true);
     s->jobs[raw->id] = std::move(j);
     s->queue.push_back(raw);
     if (out_id) *out_id = raw->id;
@@ -2678,8 +2707,10 @@ int32_t rl_server_destroy(rl_server* s) {
     // Jobs still queued when the loop stopped were never admitted.
     for (Job* j : s->queue) {
         j->state = RL_JOB_CANCELLED;
+        j->hold(false);
         s->stats.cancelled++;
     }
+    for (Job* j : s->active) j->hold(false);
     rl_destroy(s->aux);
     delete s;
     return RL_SUCCESS;

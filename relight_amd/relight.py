@@ -239,8 +239,9 @@ class Scene:
         self._keep = []
 
     def close(self):
+        """Raises RelightError (and keeps the handle) while an unfinished server job uses it."""
         if self._h:
-            self._lib.rl_scene_destroy(self._h)
+            _check(self._lib, self._lib.rl_scene_destroy(self._h))
             self._h = None
 
     def __del__(self):
@@ -322,8 +323,9 @@ class Lightmap:
         self._h = h
 
     def close(self):
+        """Raises RelightError (and keeps the handle) while an unfinished server job uses it."""
         if self._h:
-            self._lib.rl_lightmap_destroy(self._h)
+            _check(self._lib, self._lib.rl_lightmap_destroy(self._h))
             self._h = None
 
     def __del__(self):
@@ -419,8 +421,9 @@ class Radiosity:
         self._h = h
 
     def close(self):
+        """Raises RelightError (and keeps the handle) while an unfinished server job uses it."""
         if self._h:
-            self._lib.rl_radiosity_destroy(self._h)
+            _check(self._lib, self._lib.rl_radiosity_destroy(self._h))
             self._h = None
 
     def __del__(self):
@@ -606,13 +609,24 @@ class Pass:
 
 
 class BakeJob:
-    """A job submitted to a Server; holds what must outlive it (the radiosity output array)."""
+    """A job submitted to a Server. It holds what the native job borrows until it finishes: the
+    scene, the lightmaps, the radiosity object, the settings and the radiosity output array (the
+    server also keeps the job until then), so dropping the caller's references cannot free them
+    under the server's worker threads."""
 
-    def __init__(self, server: "Server", job_id: int, kind: int, out=None):
+    _FINAL = ("done", "failed", "cancelled")
+
+    def __init__(self, server: "Server", job_id: int, kind: int, out=None, keep=()):
         self.server = server
         self.id = job_id
         self.kind = kind
         self.result = out
+        self._keep = keep
+
+    def _settle(self, status: dict) -> dict:
+        if status["state"] in self._FINAL:
+            self.server._jobs.pop(self.id, None)
+        return status
 
     @staticmethod
     def _status(st: A.rl_job_status) -> dict:
@@ -623,7 +637,7 @@ class BakeJob:
     def poll(self) -> dict:
         st = A.rl_job_status()
         _check(self.server._lib, self.server._lib.rl_server_poll(self.server._h, self.id, C.byref(st)))
-        return self._status(st)
+        return self._settle(self._status(st))
 
     def wait(self, timeout: Optional[float] = None) -> dict:
         """Blocks until the job finishes (timeout in seconds); raises RelightError if it failed
@@ -635,9 +649,10 @@ class BakeJob:
         if rc == A.RL_REJECTED and st.state in (A.RL_JOB_QUEUED, A.RL_JOB_RUNNING):
             raise TimeoutError(f"job {self.id} still {A.JOB_STATE_NAMES[st.state]}")
         _check(lib, rc)
+        status = self._settle(self._status(st))
         if st.state == A.RL_JOB_FAILED:
             raise RelightError(st.status, lib.rl_last_error().decode())
-        return self._status(st)
+        return status
 
     def cancel(self):
         _check(self.server._lib, self.server._lib.rl_server_cancel(self.server._h, self.id))
@@ -657,11 +672,14 @@ class Server:
         h = C.c_void_p()
         _check(self._lib, self._lib.rl_server_create(relight._h, C.byref(cfg), C.byref(h)))
         self._h = h
+        self._jobs = {}  # id -> BakeJob, until the job is seen finished
 
     def close(self):
+        """Cancels queued jobs, finishes running ones and stops the loop."""
         if self._h:
             self._lib.rl_server_destroy(self._h)
             self._h = None
+            self._jobs.clear()
 
     def __enter__(self):
         return self
@@ -675,10 +693,15 @@ class Server:
         except Exception:
             pass
 
-    def _submit(self, d: A.rl_job_desc, out=None) -> BakeJob:
+    def _submit(self, d: A.rl_job_desc, out=None, keep=()) -> BakeJob:
+        if len(self._jobs) >= 256:
+            for job in list(self._jobs.values()):
+                job.poll()  # drops finished jobs
         jid = C.c_uint64()
         _check(self._lib, self._lib.rl_server_submit(self._h, C.byref(d), C.byref(jid)))
-        return BakeJob(self, jid.value, d.kind, out)
+        job = BakeJob(self, jid.value, d.kind, out, keep)
+        self._jobs[job.id] = job
+        return job
 
     @staticmethod
     def _maps(lightmaps):
@@ -691,7 +714,7 @@ class Server:
         d = A.rl_job_desc(kind=A.RL_JOB_DIRECT_AO, scene=scene._h.value, lightmaps=maps,
                           num_lightmaps=len(lightmaps), direct=1 if direct else 0,
                           ao=C.pointer(s) if s is not None else None)
-        return self._submit(d)
+        return self._submit(d, keep=(scene, list(lightmaps)))
 
     def submit_indirect(self, scene: Scene, lightmaps: Sequence[Lightmap],
                         settings: IndirectLightSettings) -> BakeJob:
@@ -699,7 +722,7 @@ class Server:
         s = settings.to_c()
         d = A.rl_job_desc(kind=A.RL_JOB_INDIRECT, scene=scene._h.value, lightmaps=maps,
                           num_lightmaps=len(lightmaps), indirect=C.pointer(s))
-        return self._submit(d)
+        return self._submit(d, keep=(scene, list(lightmaps)))
 
     def submit_final_gather(self, scene: Scene, lightmaps: Sequence[Lightmap],
                             settings: IndirectLightSettings) -> BakeJob:
@@ -708,7 +731,7 @@ class Server:
         s = settings.to_c()
         d = A.rl_job_desc(kind=A.RL_JOB_FINAL_GATHER, scene=scene._h.value, lightmaps=maps,
                           num_lightmaps=len(lightmaps), indirect=C.pointer(s))
-        return self._submit(d)
+        return self._submit(d, keep=(scene, list(lightmaps)))
 
     def submit_full_bake(self, scene: Scene, lightmaps: Sequence[Lightmap], direct: bool = True,
                          ao: Optional[AmbientOcclusionSettings] = None,
@@ -721,7 +744,7 @@ class Server:
                           num_lightmaps=len(lightmaps), direct=1 if direct else 0,
                           ao=C.pointer(a) if a is not None else None,
                           indirect=C.pointer(i) if i is not None else None)
-        return self._submit(d)
+        return self._submit(d, keep=(scene, list(lightmaps)))
 
     def submit_radiosity(self, radiosity: Radiosity, emission, reflectance, rays_per_patch=1024,
                          seed=1, max_distance=1000.0, tolerance=1e-5) -> BakeJob:
@@ -734,7 +757,7 @@ class Server:
                           ff_seed=int(seed) & 0xFFFFFFFF, ff_max_distance=float(max_distance),
                           emission=_fptr(E), reflectance=_fptr(R), num_rhs=E.shape[1],
                           tolerance=float(tolerance), out_radiosity=_fptr(out))
-        return self._submit(d, out)
+        return self._submit(d, out, keep=(radiosity, radiosity.scene))
 
     def stats(self) -> dict:
         st = A.rl_server_stats()

@@ -215,3 +215,40 @@ def test_recorded_pass_replays_equal_eager_passes():
         r.bake_ambient_occlusion(scene_c, ids_c[0], ao)
     p.close()
     r.close()
+
+
+def test_server_jobs_keep_their_objects_alive(relight):
+    """A job borrows its scene, lightmaps and radiosity object until it finishes: dropping the
+    caller's references after submit neither frees them nor changes the result, and destroying a
+    borrowed object through the C-ABI is refused (RL_INVALID_CALL) until the job is done."""
+    import gc
+
+    from relight_amd import _capi as A
+    print("[gpu] server: job lifetimes", flush=True)
+    spec = _small_city(ao_samples=16)
+    ref_scene, ref_lms, _ = scenes.build_gpu(relight, spec)
+    relight.bake_batch(ref_scene, ref_lms, direct=True, ao=spec.ao)
+    want = [lm.to_hdr() for lm in ref_lms]
+    big = scenes.city(num_boxes=800, tiles=(2, 2), side_range=(512, 768), ao_samples=256)
+    with relight.create_server(max_active=1, quantum_rays=1 << 24) as srv:
+        s1, l1, _ = scenes.build_gpu(relight, big)
+        first = srv.submit_direct_ao(s1, l1, True, big.ao)  # keeps the only slot busy for a while
+        s2, l2, _ = scenes.build_gpu(relight, spec)
+        job = srv.submit_direct_ao(s2, l2, True, spec.ao)   # queued behind it
+        handle = s2._h
+        assert relight._lib.rl_scene_destroy(handle) == A.RL_INVALID_CALL
+        assert b"server job" in relight._lib.rl_last_error()
+        with pytest.raises(RelightError):
+            l2[0].close()
+        kept = job._keep[1]
+        del s1, l1, s2, l2
+        gc.collect()
+        assert first.wait(timeout=120)["state"] == "done"
+        assert job.wait(timeout=120)["state"] == "done"
+        assert job.id not in srv._jobs
+    for lm, w in zip(kept, want):
+        assert np.array_equal(lm.to_hdr(), w)
+    # Finished: the objects can be destroyed again.
+    del job
+    for lm in kept:
+        lm.close()
