@@ -18,6 +18,12 @@ sys.path.insert(0, HERE)
 
 METRIC = "Mrays/s + bake p50/p99 latency, batched vs plain GPU at ≤1e-3 texel RMSE"
 HBM_PEAK_GBS = 8000.0
+# MI355X peak engine clock; a compute unit's four SIMDs execute 16 lanes of a VALU instruction
+# per clock (a wave64 instruction issues over 4 clocks), so the VALU ceiling is 64 lane-ops per
+# CU per clock.
+ENGINE_CLOCK_GHZ = 2.4
+MI355X_CUS = 256
+ORACLE_BUILD = "g++ -O3 -march=x86-64-v3 -ffp-contract=off (oracle/Makefile)"
 
 
 def log(*a):
@@ -32,12 +38,59 @@ def load_traffic():
         return json.load(f)
 
 
-def load_pmc(kernel):
-    path = os.path.join(HERE, "profiles", f"{kernel}_city_pmc.json")
+def load_pmc(kernel, workload="city"):
+    name = f"{kernel}_{workload}_pmc.json"
+    path = os.path.join(HERE, "profiles", name)
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f)
+        d = json.load(f)
+    d["file"] = name
+    return d
+
+
+def cpu_share():
+    """Host threads this process may run on (the box's CPU share for one GPU)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def valu_roofline(pmc, avg_s, cus, texels):
+    """VALU-issue roofline of a traversal kernel from its committed counters (same workload and
+    kernel): useful lane-operations per launch (SQ_THREAD_CYCLES_VALU: active lanes summed over
+    the VALU instructions) over the live launch time, against the VALU ceiling."""
+    sq = pmc["sq"]
+    lane_ops = sq["SQ_THREAD_CYCLES_VALU"]
+    peak = cus * 64 * ENGINE_CLOCK_GHZ  # G lane-op/s
+    achieved = lane_ops / avg_s / 1e9
+    return {
+        "bound": "valu_issue", "achieved": achieved, "peak": peak, "unit": "G lane-op/s",
+        "frac": achieved / peak, "traffic": pmc["hbm_bytes_per_launch"],
+        "counters": os.path.join("profiles", pmc["file"]),
+        "lane_ops_per_launch": lane_ops,
+        # Wave-instruction issue (1 = each of a CU's four SIMDs issues a wave64 VALU instruction
+        # every 4 clocks) and the mean fraction of a VALU instruction's 64 lanes that are active;
+        # frac = their product.
+        "valu_issue_util": sq["SQ_INSTS_VALU"] / (cus * ENGINE_CLOCK_GHZ * 1e9 * avg_s),
+        "lane_efficiency": sq["SQ_THREAD_CYCLES_VALU"] / (64.0 * sq["SQ_INSTS_VALU"]),
+        "hbm_gbs_measured": pmc["hbm_bytes_per_launch"] / avg_s / 1e9,
+        # Compulsory traffic: per texel the 24-B lumel (position, normal) read and the 12-B RGB write.
+        "hbm_vs_compulsory": pmc["hbm_bytes_per_launch"] / (36.0 * texels),
+        "l2_hit_rate": pmc["l2_hit_rate"],
+    }
 
 
 def percentile(x, q):
@@ -70,7 +123,9 @@ def cpu_baseline(spec, gpu_maps, stride, threads):
         den += float((col.astype(np.float64) ** 2).sum())
     base = {"value": rays / elapsed / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"oracle direct+AO on every {stride}th valid lumel of all 32 lightmaps "
-                      f"({lumels} lumels, {rays} rays, {elapsed:.1f} s)"}
+                      f"({lumels} lumels, {rays} rays, {elapsed:.1f} s)",
+            "cpu_model": cpu_model(), "build": ORACLE_BUILD,
+            "threads_note": "all host threads this process may use (sched_getaffinity)"}
     rmse = (num / den) ** 0.5 if den > 0 else 0.0
     return base, rmse, lumels
 
@@ -167,35 +222,24 @@ def main():
     plain_value = (float(np.sum(plain_rays)) / (float(np.sum(plain_lat)) / 1e3) / 1e6
                    if plain_lat else None)
 
-    # Roofline of the dominant kernel (AO any-hit traversal).
+    # Roofline of the dominant kernel (the AO traversal): VALU issue, from its committed counters.
     traffic = load_traffic()
     roofline = None
-    if ao_launch and traffic:
-        bpr = traffic["ao"]["bytes_per_ray"]
+    kernel = "k_ao_refill" if relight.tuning()["ao_kernel"] == 1 else "k_ao_count"
+    pmc = load_pmc(kernel)
+    if ao_launch and pmc:
+        cus = MI355X_CUS
         avg_s = ao_ms / ao_launch / 1e3
         rays_per_launch = ao_rays / ao_launch
-        achieved = rays_per_launch * bpr / avg_s / 1e9
-        kernel = "k_ao_refill" if relight.tuning()["ao_kernel"] == 1 else "k_ao_count"
-        pmc = load_pmc(kernel)
-        roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK_GBS,
-                    "traffic": pmc["hbm_bytes_per_launch"] if pmc else None,
-                    "kernel": kernel, "avg_launch_ms": avg_s * 1e3,
-                    "bytes_per_ray": bpr, "rays_per_launch": rays_per_launch}
-        if pmc:
-            # Measured on the same workload: physical HBM rate, L2 hit rate and VALU issue rate
-            # (wave instructions per SIMD-cycle; 0.25 is the wave64 issue ceiling).
-            sq = pmc["sq"]
-            roofline["hbm_gbs_measured"] = pmc["hbm_bytes_per_launch"] / avg_s / 1e9
-            roofline["l2_hit_rate"] = pmc["l2_hit_rate"]
-            roofline["valu_issue_per_simd_cycle"] = sq["SQ_INSTS_VALU"] / 1024 / (sq["GRBM_GUI_ACTIVE"] / 8)
-
-    cpu = None
-    rmse = None
-    if not args.no_cpu:
-        threads = max(1, min(16, os.cpu_count() or 1))
-        cpu, rmse, n = cpu_baseline(spec, first, args.cpu_stride, threads)
-        log(f"cpu baseline: {cpu}, rel RMSE on {n} texels: {rmse:.3e}")
+        roofline = valu_roofline(pmc, avg_s, cus, texels)
+        roofline.update({"kernel": kernel, "avg_launch_ms": avg_s * 1e3, "rays_per_launch": rays_per_launch})
+        if traffic:
+            # Work metric: bytes a binary BVH of the oracle's kind would fetch per ray (node and
+            # triangle visits counted on the oracle's traversal of this workload's rays), not HBM
+            # traffic: the BVH stays in the caches.
+            bpr = traffic["ao"]["bytes_per_ray"]
+            roofline["work_metric"] = {"bytes_per_ray": bpr, "gbs": rays_per_launch * bpr / avg_s / 1e9,
+                                       "source": "tests/golden/city_traffic.json"}
 
     out = {
         "metric": METRIC, "value": value, "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
@@ -216,20 +260,20 @@ def main():
                   "latency_ms": {"p50": percentile(plain_lat, 50), "p99": percentile(plain_lat, 99)}
                   if plain_lat else None},
         "batched_vs_plain": (value / plain_value) if plain_value else None,
-        "rmse_vs_oracle": rmse,
+        "rmse_vs_oracle": None,
         "tuning": relight.tuning(),
         "roofline": roofline,
-        "cpu_baseline": cpu,
+        "cpu_baseline": None,
     }
     # Secondary legs (each on its own resident scenes), after the headline measurement.
     import bench_workloads as W
-    legs = {}
+    legs, checks = {}, {}
     for name in [w for w in args.workloads.split(",") if w]:
         t0 = time.time()
         if name == "gi":
-            legs["gi"] = W.bench_gi(relight)
+            checks["gi"], legs["gi"] = W.bench_gi(relight)
         elif name == "radiosity":
-            legs["radiosity"] = W.bench_radiosity(relight)
+            legs["radiosity"], checks["radiosity"] = W.bench_radiosity(relight)
         elif name == "serving":
             legs["serving"] = W.bench_serving(relight, n_requests=args.serve_requests)
         elif name == "serving_small":
@@ -239,6 +283,22 @@ def main():
         log(f"{name} leg done in {time.time()-t0:.1f} s")
     if legs:
         out["workloads"] = legs
+    # CPU leg: the oracle timed on a sample of the headline workload, and every GPU result of this
+    # run compared with it on the same inputs.
+    if not args.no_cpu:
+        threads = cpu_share()
+        cpu, rmse, n = cpu_baseline(spec, first, args.cpu_stride, threads)
+        out["cpu_baseline"], out["rmse_vs_oracle"] = cpu, rmse
+        log(f"cpu baseline: {cpu}, rel RMSE on {n} texels: {rmse:.3e}")
+        if "gi" in checks:
+            t0 = time.time()
+            legs["gi"]["parity"] = W.gi_oracle_check(checks["gi"], threads=threads)
+            legs["gi"]["rmse_vs_oracle"] = legs["gi"]["parity"]["rmse_vs_oracle"]
+            log(f"gi oracle check ({time.time()-t0:.1f} s): {legs['gi']['parity']}")
+        if "radiosity" in checks:
+            t0 = time.time()
+            legs["radiosity"]["parity"] = W.radiosity_oracle_check(checks["radiosity"], threads=threads)
+            log(f"radiosity oracle check ({time.time()-t0:.1f} s): {legs['radiosity']['parity']}")
     print(json.dumps(out), flush=True)
 
 

@@ -43,9 +43,12 @@ def bench_gi(relight, steps=2):
     texels = lms[0].valid_count()
     ind = spec.indirect
     log(f"gi: heightfield built: {info}, {texels} texels, {len(spec.photonmaps)} photonmaps ({time.time()-t0:.1f} s)")
-    # Warm-up pass (photon buffers, lumel upload).
+    # Check pass (also the warm-up): one photon emission and one final gather from zero, kept for
+    # the comparison with the oracle in bench.py's CPU leg (later passes accumulate onto them).
     relight.emit_photons(scene, ind)
+    check_maps = [pm.read() for pm in scene.photonmaps]
     relight.bake_indirect_light(scene, ids[0], ind)
+    check = {"spec": spec, "maps": check_maps, "fg": lms[0].to_hdr().reshape(-1, 3).copy()}
     rec = []
     for s in range(steps):
         t = time.perf_counter()
@@ -67,7 +70,13 @@ def bench_gi(relight, steps=2):
     trace_ms = float(np.mean([k(r[1], "photon_trace") for r in rec]))
     store_ms = float(np.mean([k(r[1], "photon_store") for r in rec]))
     nq = sum(w * h for w, h in spec.photonmaps)
-    return {
+    roofline = None
+    import bench
+    pmc = bench.load_pmc("k_fg_lanes", "heightfield")
+    if pmc and fg_ms and relight.tuning()["fg_kernel"] in (0, 2):
+        roofline = bench.valu_roofline(pmc, fg_ms / 1e3, bench.MI355X_CUS, texels)
+        roofline["kernel"] = "k_fg_lanes"
+    return check, {
         "workload": f"heightfield: 512^2-cell fBm terrain + 2000 boxes ({spec.num_tris} tris), 4 area lights, "
                     f"{ind.photons_per_light * len(spec.lights)} photons (RR, 3 bounces), k={ind.knn_k} "
                     f"hash-grid gather r={ind.knn_radius} m on 64 photonmaps 128^2, FG "
@@ -80,6 +89,7 @@ def bench_gi(relight, steps=2):
                     "knn_gather_ms": knn_ms, "knn_queries": nq,
                     "knn_queries_per_s": nq / (knn_ms / 1e3) if knn_ms else None},
         "triangles": spec.num_tris, "bvh_nodes": info["nodes"],
+        "roofline": roofline,
     }
 
 
@@ -111,6 +121,9 @@ def bench_radiosity(relight, rays=1024, max_distance=1000.0):
     for r0 in range(0, n, 2048):
         FB[r0:r0 + 2048] = (C[r0:r0 + 2048].astype(np.float64) / rays) @ B64
     res64 = float(np.linalg.norm(ps.emission + ps.reflectance * FB - B64) / np.linalg.norm(ps.emission))
+    del FB
+    err64, lu_s = fp64_solve_error(C, rays, ps, B64)
+    log(f"radiosity: error vs FP64 LU solve {err64:.3e} ({lu_s:.1f} s)")
     npad = (n + 63) // 64 * 64
     gemm = {}
     for name in ("ff_gemm_fast", "ff_gemm_accurate"):
@@ -129,8 +142,86 @@ def bench_radiosity(relight, rays=1024, max_distance=1000.0):
         "form_factors_ms": ff_ms, "form_factor_rays": ff["rays"],
         "form_factor_mrays_per_s": ff["rays"] / (ff_ms / 1e3) / 1e6,
         "solve_ms": solve_ms, "outer_iterations": info["outer_iterations"], "sweeps": info["sweeps"],
-        "residual_fp32": info["residual"], "residual_fp64_host": res64, "gemm": gemm,
-    }
+        "residual_fp32": info["residual"], "residual_fp64_host": res64,
+        "error_vs_fp64": err64,
+        "fp64_check": f"host FP64 LU (scipy getrf) of (I - diag(rho_c) F) for the 3 RGB channels of {n}^2, "
+                      f"{m // 3} right-hand sides each, {lu_s:.1f} s; error = ||B - B64|| / ||B64||",
+        "gemm": gemm,
+    }, {"ps": ps, "counts": C, "rays": rays, "max_distance": max_distance}
+
+
+def fp64_solve_error(C, rays, ps, B):
+    """Relative error of B against an FP64 LU solve of (I - diag(rho_c) F) B_c = E_c per RGB
+    channel c (rho is the same for every emission set, so three factorizations cover all
+    right-hand sides)."""
+    import scipy.linalg as sl
+    t = time.perf_counter()
+    n, m = ps.emission.shape
+    R = ps.reflectance.astype(np.float64)
+    E = ps.emission.astype(np.float64)
+    ref = np.empty((n, m))
+    for ch in range(3):
+        cols = list(range(ch, m, 3))
+        rho = R[:, ch]
+        assert all(np.array_equal(R[:, c], rho) for c in cols)
+        A = np.empty((n, n))
+        for r0 in range(0, n, 2048):
+            A[r0:r0 + 2048] = C[r0:r0 + 2048].astype(np.float64) * (-rho[r0:r0 + 2048, None] / rays)
+        A[np.arange(n), np.arange(n)] += 1.0
+        lu = sl.lu_factor(A, overwrite_a=True, check_finite=False)
+        ref[:, cols] = sl.lu_solve(lu, E[:, cols], check_finite=False)
+        del A, lu
+    err = float(np.linalg.norm(B - ref) / np.linalg.norm(ref))
+    return err, time.perf_counter() - t
+
+
+def gi_oracle_check(check, stride=256, threads=0):
+    """The gi leg's first pass against the oracle (bench.py's CPU leg): photon counts and k-NN
+    gathered colour of every photonmap, and the final gather on every `stride`-th valid lumel of
+    the 2048^2 map. Also times the oracle on that sample."""
+    import oracle as O
+    spec = check["spec"]
+    t = time.perf_counter()
+    S = O.OracleScene.from_spec(spec)
+    maps, prays = S.emit_photons(spec, spec.indirect, threads)
+    emit_s = time.perf_counter() - t
+    counts_eq = all(np.array_equal(check["maps"][k][1], maps[k][1]) for k in range(len(maps)))
+    color_eq = all(np.array_equal(check["maps"][k][0], maps[k][0]) for k in range(len(maps)))
+    gathered_eq = all(np.array_equal(check["maps"][k][2], maps[k][2]) for k in range(len(maps)))
+    texel, pos, nrm = S.lumels(spec, 0)
+    sel = slice(0, None, stride)
+    col = np.zeros((texel[sel].size, 3), np.float32)
+    t = time.perf_counter()
+    frays = S.final_gather(spec, pos[sel], nrm[sel], texel[sel], 0, spec.indirect, maps, col, threads)
+    fg_s = time.perf_counter() - t
+    g = check["fg"][texel[sel]].astype(np.float64)
+    c = col.astype(np.float64)
+    rmse = float(np.sqrt(np.mean((g - c) ** 2)) / np.sqrt(np.mean(c * c)))
+    return {"photonmaps": len(maps), "photon_counts_equal": counts_eq, "photon_power_equal": color_eq,
+            "knn_gathered_equal": gathered_eq, "fg_sample_lumels": int(texel[sel].size),
+            "rmse_vs_oracle": rmse, "fg_bit_exact_frac": float(np.mean(np.all(g == c, axis=1))),
+            "oracle_mrays_per_s": {"photons": prays / emit_s / 1e6, "final_gather": frays / fg_s / 1e6},
+            "sample": f"every {stride}th valid lumel of the 2048^2 map ({texel[sel].size} lumels, {frays} rays)"}
+
+
+def radiosity_oracle_check(check, rows=32, threads=0):
+    """Form-factor rows of the radiosity leg against the oracle: `rows` evenly spaced rows of the
+    16,384^2 hit-count matrix, compared exactly (bench.py's CPU leg)."""
+    import oracle as O
+    ps, C = check["ps"], check["counts"]
+    S = O.OracleScene()
+    S.add_mesh(ps.positions, ps.normals, np.zeros((ps.positions.shape[0], 2), np.float32), ps.indices)
+    S.build()
+    n = ps.num_patches
+    idx = np.linspace(0, n - 1, rows).astype(int)
+    t = time.perf_counter()
+    same = 0
+    for r in idx:
+        Co = S.form_factors(ps.patches, ps.tri_patch, check["rays"], 1, check["max_distance"], rows=1, row0=int(r),
+                            threads=threads)
+        same += int(np.array_equal(Co[0], C[r]))
+    return {"rows_checked": int(rows), "rows_equal": same, "form_factor_rows_equal": same == rows,
+            "oracle_s": time.perf_counter() - t}
 
 
 # ----------------------------------------------------------------------------------------- serving
