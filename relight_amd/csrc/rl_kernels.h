@@ -69,15 +69,15 @@ hipError_t launch_photon_trace(hipStream_t st, const SceneDev& S, int cap, uint3
                                const PhotonParams& P, const int32_t* mesh_pm,
                                const PhotonMapDev* pms, PhotonRec* out, float4* out_pos,
                                unsigned long long* queue, unsigned long long* rays);
-hipError_t photon_sort_temp_bytes(uint32_t n, size_t* bytes);
+// Photons::store of n records (sorted by texel key with the radix sort of rl_sort.h, scratch
+// sort_pairs_temp_bytes(n)); total_keys = texels of all photonmaps (the keys' upper bound).
 hipError_t launch_photon_store(hipStream_t st, const PhotonRec* rec, uint32_t n, uint32_t* keys_a,
                                uint32_t* keys_b, uint32_t* idx_a, uint32_t* idx_b, void* temp,
-                               size_t temp_bytes, const PhotonMapDev* pms, int npm);
+                               size_t temp_bytes, const PhotonMapDev* pms, int npm, uint32_t total_keys);
 hipError_t launch_photon_gather(hipStream_t st, const PhotonMapDev* pms, int npm,
                                 uint32_t total_texels, int radius);
 // k-nearest gather: hash grid over the stored records (key_bits = log2(mask + 1) + 1), then one
 // wavefront per query (float4: surface point, w = global texel key bits).
-hipError_t knn_sort_temp_bytes(uint32_t n, int key_bits, size_t* bytes);
 hipError_t launch_knn_build(hipStream_t st, const PhotonRec* rec, const float4* ppos, uint32_t n,
                             const KnnGrid& G, int key_bits, uint32_t* keys_a, uint32_t* keys_b, uint32_t* idx_a,
                             uint32_t* idx_b, void* temp, size_t temp_bytes, float4* spos, uint32_t* cstart,

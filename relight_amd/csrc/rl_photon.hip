@@ -6,10 +6,10 @@
 // a stable radix sort and summed in record order, so the photonmap equals a sequential store in
 // the reference's emission order, bit for bit, on every run.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include "rl_device.h"
 #include "rl_kernels.h"
+#include "rl_sort.h"
 #include "rl_traverse.h"
 
 namespace rl {
@@ -165,11 +165,14 @@ __global__ void k_photon_clear(PhotonRec* out, uint64_t n) {
     }
 }
 
+// Sort keys of the photon records: the texel key, or `empty` (the number of texel keys, above
+// every texel key) for a slot with nothing stored, so the keys need only bit_length(empty) bits.
 __global__ void k_keys_iota(const PhotonRec* __restrict__ rec, uint32_t* __restrict__ keys,
-                            uint32_t* __restrict__ idx, uint32_t n) {
+                            uint32_t* __restrict__ idx, uint32_t n, uint32_t empty) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g < n) {
-        keys[g] = rec[g].key;
+        const uint32_t k = rec[g].key;
+        keys[g] = k == 0xFFFFFFFFu ? empty : k;
         idx[g] = g;
     }
 }
@@ -178,11 +181,11 @@ __global__ void k_keys_iota(const PhotonRec* __restrict__ rec, uint32_t* __restr
 // head, summing in record order on top of the texel's current value.
 __global__ void k_photon_accumulate(const uint32_t* __restrict__ keys, const uint32_t* __restrict__ idx,
                                     const PhotonRec* __restrict__ rec, uint32_t n,
-                                    const PhotonMapDev* __restrict__ pms, int npm) {
+                                    const PhotonMapDev* __restrict__ pms, int npm, uint32_t empty) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const uint32_t key = keys[g];
-    if (key == 0xFFFFFFFFu) return;
+    if (key >= empty) return;
     if (g > 0 && keys[g - 1] == key) return;
     const PhotonMapDev M = pms[find_map(pms, npm, key)];
     const uint32_t t = key - M.key_base;
@@ -722,22 +725,19 @@ hipError_t launch_photon_trace(hipStream_t st, const SceneDev& S, int cap, uint3
     return hipGetLastError();
 }
 
-hipError_t photon_sort_temp_bytes(uint32_t n, size_t* bytes) {
-    *bytes = 0;
-    return hipcub::DeviceRadixSort::SortPairs(nullptr, *bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                              (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n);
-}
-
 hipError_t launch_photon_store(hipStream_t st, const PhotonRec* rec, uint32_t n, uint32_t* keys_a,
                                uint32_t* keys_b, uint32_t* idx_a, uint32_t* idx_b, void* temp,
-                               size_t temp_bytes, const PhotonMapDev* pms, int npm) {
+                               size_t temp_bytes, const PhotonMapDev* pms, int npm, uint32_t total_keys) {
     if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_keys_iota, dim3(blocks_for(n, 256)), dim3(256), 0, st, rec, keys_a, idx_a, n);
-    hipError_t e = hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys_a, keys_b, idx_a, idx_b, (int)n, 0,
-                                                      32, st);
+    if (total_keys == 0xFFFFFFFFu) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_keys_iota, dim3(blocks_for(n, 256)), dim3(256), 0, st, rec, keys_a, idx_a, n, total_keys);
+    int bits = 1;
+    while (bits < 32 && (total_keys >> bits) != 0u) bits++;
+    uint32_t *keys, *idx;
+    hipError_t e = launch_sort_pairs(st, keys_a, keys_b, idx_a, idx_b, n, bits, temp, temp_bytes, &keys, &idx);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_photon_accumulate, dim3(blocks_for(n, 256)), dim3(256), 0, st, keys_b, idx_b, rec, n,
-                       pms, npm);
+    hipLaunchKernelGGL(k_photon_accumulate, dim3(blocks_for(n, 256)), dim3(256), 0, st, keys, idx, rec, n,
+                       pms, npm, total_keys);
     return hipGetLastError();
 }
 
@@ -750,12 +750,6 @@ hipError_t launch_photon_gather(hipStream_t st, const PhotonMapDev* pms, int npm
     return hipGetLastError();
 }
 
-hipError_t knn_sort_temp_bytes(uint32_t n, int key_bits, size_t* bytes) {
-    *bytes = 0;
-    return hipcub::DeviceRadixSort::SortPairs(nullptr, *bytes, (const uint32_t*)nullptr, (uint32_t*)nullptr,
-                                              (const uint32_t*)nullptr, (uint32_t*)nullptr, (int)n, 0, key_bits);
-}
-
 hipError_t launch_knn_build(hipStream_t st, const PhotonRec* rec, const float4* ppos, uint32_t n,
                             const KnnGrid& G, int key_bits, uint32_t* keys_a, uint32_t* keys_b, uint32_t* idx_a,
                             uint32_t* idx_b, void* temp, size_t temp_bytes, float4* spos, uint32_t* cstart,
@@ -765,9 +759,10 @@ hipError_t launch_knn_build(hipStream_t st, const PhotonRec* rec, const float4* 
     if (e == hipSuccess) e = launch_fill_u32(st, cend, (uint64_t)G.mask + 1u, 0u);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_knn_keys, dim3(blocks_for(n, 256)), dim3(256), 0, st, rec, ppos, n, G, keys_a, idx_a);
-    e = hipcub::DeviceRadixSort::SortPairs(temp, temp_bytes, keys_a, keys_b, idx_a, idx_b, (int)n, 0, key_bits, st);
+    uint32_t *keys, *idx;
+    e = launch_sort_pairs(st, keys_a, keys_b, idx_a, idx_b, n, key_bits, temp, temp_bytes, &keys, &idx);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_knn_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, st, keys_b, idx_b, n, ppos, G, spos,
+    hipLaunchKernelGGL(k_knn_scatter, dim3(blocks_for(n, 256)), dim3(256), 0, st, keys, idx, n, ppos, G, spos,
                        cstart, cend);
     return hipGetLastError();
 }

@@ -29,6 +29,7 @@
 #include "rl_kernels.h"
 #include "rl_math.h"
 #include "rl_noalloc.h"
+#include "rl_sort.h"
 
 using namespace rl;
 
@@ -1755,8 +1756,7 @@ int32_t rl_emit_photons(rl_relight* ctx, rl_scene* scene, const rl_indirect_sett
     if (e == hipSuccess) e = dev_alloc(&kb, (size_t)nrec);
     if (e == hipSuccess) e = dev_alloc(&ia, (size_t)nrec);
     if (e == hipSuccess) e = dev_alloc(&ib, (size_t)nrec);
-    if (e == hipSuccess) e = photon_sort_temp_bytes((uint32_t)nrec, &temp_bytes);
-    if (e == hipSuccess && knn) e = knn_sort_temp_bytes((uint32_t)nrec, key_bits, &knn_temp);
+    temp_bytes = knn_temp = sort_pairs_temp_bytes((uint32_t)nrec);
     if (e == hipSuccess) e = dev_alloc(&temp, std::max<size_t>(std::max(temp_bytes, knn_temp), 1));
     if (e == hipSuccess && knn) e = dev_alloc(&d_ppos, (size_t)nrec);
     if (e == hipSuccess && knn) e = dev_alloc(&d_spos, (size_t)nrec);
@@ -1778,7 +1778,7 @@ int32_t rl_emit_photons(rl_relight* ctx, rl_scene* scene, const rl_indirect_sett
     if (e == hipSuccess) e = kernel_begin(ctx, RL_KERNEL_PHOTON_STORE);
     if (e == hipSuccess)
         e = launch_photon_store(ctx->stream, d_rec, (uint32_t)nrec, ka, kb, ia, ib, temp, temp_bytes, T.d_pms,
-                                (int)T.dev.size());
+                                (int)T.dev.size(), (uint32_t)T.total_texels);
     if (e == hipSuccess) e = kernel_end(ctx);
     // Relight::emitPhotons gathers every photonmap after the passes (Relight.cpp:250-254).
     if (knn) {
