@@ -318,6 +318,53 @@ int32_t rl_emit_photons(rl_relight* ctx, rl_scene* scene, const rl_indirect_sett
 int32_t rl_bake_indirect(rl_relight* ctx, rl_scene* scene, int32_t mesh,
                          const rl_indirect_settings* settings);
 
+/* BakeIterator (baker/Baker.h, baker/Baker.cpp:148-234): which lumels a bake visits.
+ * RL_ITERATE_LUMELS is LumelBakeIterator(first, step): texel indices first, first + step, ... of the
+ * mesh's whole lightmap (the texels of every mesh in it). RL_ITERATE_FACES is
+ * FaceBakeIterator(first, step): the lumels owned by faces first, first + step, ... of the mesh (a
+ * lumel belongs to the last face of the lightmap that covers its texel centre; the reference would
+ * also bake another mesh's lumel of the same face index where two meshes' charts overlap). Workers
+ * split a bake with iterators (i, N), i = 0..N-1; the union equals the whole bake bit for bit.
+ * first >= 0 and step >= 1 (the reference reads out of range for a first past the end; here such
+ * an iterator visits nothing). */
+#define RL_ITERATE_LUMELS 0
+#define RL_ITERATE_FACES 1
+typedef struct rl_bake_iterator {
+    int32_t kind;
+    int32_t first;
+    int32_t step;
+} rl_bake_iterator;
+
+/* Progress::notify (Relight.h:96-105, called by Baker::bakeMesh, baker/Baker.cpp:71-74): with a
+ * progress record the selected lumels are baked in `updates` parts on the calling thread, and
+ * notify(user, mesh, step, step_count) follows each part but the last (step = iterator steps whose
+ * lumels are all baked, step_count = the iterator's itemCount: texels or faces), then
+ * notify(user, mesh, step_count, step_count) when the mesh is done, as the reference ends. */
+typedef void (*rl_progress_fn)(void* user, int32_t mesh, int32_t step, int32_t step_count);
+typedef struct rl_progress {
+    rl_progress_fn notify;
+    void* user;
+    int32_t updates;
+} rl_progress;
+
+/* Relight::bakeDirectLight / bakeAmbientOcclusion / bakeIndirectLight with their progress and
+ * iterator arguments (Relight.cpp:216-271). iterator NULL = LumelBakeIterator(0, 1), the whole
+ * lightmap (Relight.cpp:218-220); progress may be NULL. */
+int32_t rl_bake_direct_iter(rl_relight* ctx, rl_scene* scene, int32_t mesh,
+                            const rl_bake_iterator* iterator, const rl_progress* progress);
+int32_t rl_bake_ambient_occlusion_iter(rl_relight* ctx, rl_scene* scene, int32_t mesh,
+                                       const rl_ao_settings* settings,
+                                       const rl_bake_iterator* iterator,
+                                       const rl_progress* progress);
+int32_t rl_bake_indirect_iter(rl_relight* ctx, rl_scene* scene, int32_t mesh,
+                              const rl_indirect_settings* settings,
+                              const rl_bake_iterator* iterator, const rl_progress* progress);
+
+/* Progress as a polled counter: lumels queued (`total`) and finished (`done`) by the bake call
+ * running on ctx, readable from any thread while it runs (between calls: the last call's). A
+ * single-launch phase counts as finished when its launch has completed. */
+int32_t rl_get_progress(const rl_relight* ctx, uint64_t* done, uint64_t* total);
+
 /* Radiosity: patches are quads origin + s*U + t*V with normal n (12 floats each: o, U, V, n);
  * tri_patch maps every scene-order triangle to its patch or 0xFFFFFFFF. Replaces
  * RadiosityBuilder::build / computeFormFactors (radiosity/RadiosityBuilder.cpp:44-176) and the empty

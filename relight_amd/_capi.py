@@ -68,6 +68,21 @@ class rl_ao_settings(C.Structure):
     ]
 
 
+class rl_bake_iterator(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("first", C.c_int32), ("step", C.c_int32)]
+
+
+rl_progress_fn = C.CFUNCTYPE(None, C.c_void_p, C.c_int32, C.c_int32, C.c_int32)
+
+
+class rl_progress(C.Structure):
+    _fields_ = [("notify", rl_progress_fn), ("user", C.c_void_p), ("updates", C.c_int32)]
+
+
+RL_ITERATE_LUMELS = 0
+RL_ITERATE_FACES = 1
+
+
 class rl_hit(C.Structure):
     _fields_ = [
         ("t", C.c_float),
@@ -262,6 +277,12 @@ SIGNATURES = [
     ("rl_photonmap_read", C.c_int32, [_P, _F, _U32, _F]),
     ("rl_emit_photons", C.c_int32, [_P, _P, C.POINTER(rl_indirect_settings)]),
     ("rl_bake_indirect", C.c_int32, [_P, _P, C.c_int32, C.POINTER(rl_indirect_settings)]),
+    ("rl_bake_direct_iter", C.c_int32, [_P, _P, C.c_int32, C.POINTER(rl_bake_iterator), C.POINTER(rl_progress)]),
+    ("rl_bake_ambient_occlusion_iter", C.c_int32, [_P, _P, C.c_int32, C.POINTER(rl_ao_settings),
+                                                   C.POINTER(rl_bake_iterator), C.POINTER(rl_progress)]),
+    ("rl_bake_indirect_iter", C.c_int32, [_P, _P, C.c_int32, C.POINTER(rl_indirect_settings),
+                                          C.POINTER(rl_bake_iterator), C.POINTER(rl_progress)]),
+    ("rl_get_progress", C.c_int32, [_P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("rl_radiosity_create", C.c_int32, [_P, _P, _F, C.c_uint32, _U32, _PP]),
     ("rl_radiosity_destroy", C.c_int32, [_P]),
     ("rl_radiosity_build_form_factors", C.c_int32, [_P, _P, C.c_uint32, C.c_uint32, C.c_float]),

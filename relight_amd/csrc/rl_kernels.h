@@ -16,6 +16,12 @@ int stack_capacity_for(int bound);
 
 hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v);
 
+// Copies lumels sel[0..n) of a lightmap's compact arrays to a compact segment, and the segment's
+// colours back (bakes of an iterator's subset of a lightmap).
+hipError_t launch_gather_lumels(hipStream_t st, const uint32_t* sel, uint32_t n, const float4* pos, const float4* nrm,
+                                const float4* col, float4* opos, float4* onrm, float4* ocol);
+hipError_t launch_scatter_colors(hipStream_t st, const uint32_t* sel, uint32_t n, const float4* col, float4* out);
+
 // Segments carry their scene, lights and AO settings; `cap` is the largest traversal-stack
 // capacity any of their scenes needs. work_begin units: lumels (direct, finalize), (lumel,
 // sample) pairs (AO chunked) or chunks of chunk_lumels lumels (AO refill).

@@ -454,6 +454,26 @@ __global__ __launch_bounds__(kBlock) void k_trace(SceneDev S, const float4* __re
     count_rays(rays, rays_out);
 }
 
+// Lumels `sel` of a lightmap copied to a compact segment (position, normal, colour) and the
+// baked colours copied back: the bake kernels run on the selection as on a whole lightmap (the
+// texel id in position.w keys the sample streams, so results equal a whole-lightmap bake).
+__global__ void k_gather_lumels(const uint32_t* __restrict__ sel, uint32_t n, const float4* __restrict__ pos,
+                                const float4* __restrict__ nrm, const float4* __restrict__ col,
+                                float4* __restrict__ opos, float4* __restrict__ onrm, float4* __restrict__ ocol) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t j = sel[i];
+    opos[i] = pos[j];
+    onrm[i] = nrm[j];
+    ocol[i] = col[j];
+}
+
+__global__ void k_scatter_colors(const uint32_t* __restrict__ sel, uint32_t n, const float4* __restrict__ col,
+                                 float4* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[sel[i]] = col[i];
+}
+
 uint32_t blocks_for(uint64_t n, uint32_t b) { return (uint32_t)((n + b - 1) / b); }
 
 // Persistent grid: enough blocks for `waves` wavefronts of work, at most max_blocks.
@@ -480,6 +500,20 @@ int stack_capacity_for(int bound) {
             __VA_ARGS__;                   \
         }                                  \
     } while (0)
+
+hipError_t launch_gather_lumels(hipStream_t st, const uint32_t* sel, uint32_t n, const float4* pos, const float4* nrm,
+                                const float4* col, float4* opos, float4* onrm, float4* ocol) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_lumels, dim3(blocks_for(n, 256)), dim3(256), 0, st, sel, n, pos, nrm, col, opos, onrm,
+                       ocol);
+    return hipGetLastError();
+}
+
+hipError_t launch_scatter_colors(hipStream_t st, const uint32_t* sel, uint32_t n, const float4* col, float4* out) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_scatter_colors, dim3(blocks_for(n, 256)), dim3(256), 0, st, sel, n, col, out);
+    return hipGetLastError();
+}
 
 hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v) {
     if (n == 0) return hipSuccess;

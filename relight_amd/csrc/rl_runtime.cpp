@@ -130,6 +130,9 @@ struct rl_relight {
     size_t seg_host_cap[kSegSlots] = {};
     hipEvent_t seg_ev[kSegSlots] = {};
     bool seg_ev_live[kSegSlots] = {};
+    // Lumels queued and finished by the current bake call (rl_get_progress reads them from other
+    // threads while the call runs).
+    std::atomic<uint64_t> progress_done{0}, progress_total{0};
     // Texel-raster scratch: per-texel owner and claimed-texel records (grown to the largest map).
     uint32_t* raster_owner = nullptr;
     RasterRec* raster_rec = nullptr;
@@ -363,6 +366,8 @@ hipError_t kernel_end(rl_relight* ctx) { return hipEventRecord(ctx->marks.back()
 
 int32_t begin_call(rl_relight* ctx) {
     ctx->stats = rl_stats{};
+    ctx->progress_done = 0;
+    ctx->progress_total = 0;
     ctx->pool_used = 0;
     ctx->marks.clear();
     RL_HIP(launch_fill_u32(ctx->stream, reinterpret_cast<uint32_t*>(ctx->d_rays), 2, 0u));
@@ -373,6 +378,7 @@ int32_t begin_call(rl_relight* ctx) {
 int32_t end_call(rl_relight* ctx, const CallTimer& tm) {
     RL_HIP(hipEventRecord(ctx->ev1, ctx->stream));
     RL_HIP(hipEventSynchronize(ctx->ev1));
+    ctx->progress_done = ctx->progress_total.load();
     float ms = 0.0f;
     RL_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
     unsigned long long rays = 0;
@@ -542,8 +548,10 @@ int32_t run_direct_items(rl_relight* ctx, const std::vector<BakeItem>& items, bo
     }
     ctx->stats.texels += ds.texels;
     ctx->stats.launches += ds.total ? 1 : 0;
+    ctx->progress_total += ds.texels;
     if (e == hipSuccess && sync) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return fail(RL_DEVICE_ERROR, std::string("direct bake: ") + hipGetErrorString(e));
+    if (sync) ctx->progress_done += ds.texels;
     for (const BakeItem& it : items) it.lm->device_newer = true;
     return RL_SUCCESS;
 }
@@ -584,8 +592,10 @@ int32_t run_ao_items(rl_relight* ctx, const std::vector<BakeItem>& items, bool s
     }
     ctx->stats.texels += lum.texels;
     ctx->stats.launches += work.total ? 2 : 0;
+    ctx->progress_total += lum.texels;
     if (e == hipSuccess && sync) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return fail(RL_DEVICE_ERROR, std::string("AO bake: ") + hipGetErrorString(e));
+    if (sync) ctx->progress_done += lum.texels;
     for (const BakeItem& it : items) it.lm->device_newer = true;
     return RL_SUCCESS;
 }
@@ -622,8 +632,10 @@ int32_t run_fg_items(rl_relight* ctx, const std::vector<BakeItem>& items, bool s
     }
     ctx->stats.texels += ds.texels;
     ctx->stats.launches += ds.total ? 1 : 0;
+    ctx->progress_total += ds.texels;
     if (e == hipSuccess && sync) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return fail(RL_DEVICE_ERROR, std::string("final gather: ") + hipGetErrorString(e));
+    if (sync) ctx->progress_done += ds.texels;
     for (const BakeItem& it : items) it.lm->device_newer = true;
     return RL_SUCCESS;
 }
@@ -1332,6 +1344,176 @@ int32_t rl_lightmap_save(rl_lightmap* lm, const char* path, int32_t format) {
     return RL_SUCCESS;
 }
 
+}  // extern "C"
+
+namespace {
+
+// The lumels of `lm` a BakeIterator visits for `mesh` (baker/Baker.cpp:148-234), as indices into
+// the lightmap's valid lumels, in visiting order: LumelBakeIterator(first, step) takes texel
+// indices first, first + step, ... of the whole lightmap (every mesh's texels, as the reference
+// iterates lightmap->lumels()); FaceBakeIterator(first, step) takes the lumels owned by faces
+// first, first + step, ... of `mesh`, face by face. `steps` gets the number of iterator steps
+// (indices visited), `items` the iterator's itemCount.
+int32_t iterator_selection(const rl_scene* scene, int32_t mesh, const rl_lightmap* lm, const rl_bake_iterator& it,
+                           std::vector<uint32_t>& sel, std::vector<uint32_t>& step_of, int64_t& steps,
+                           int64_t& items) {
+    if (it.first < 0 || it.step < 1) return fail(RL_INVALID_ARGUMENT, "iterator needs first >= 0 and step >= 1");
+    const MeshHost& m = scene->meshes[mesh];
+    sel.clear();
+    step_of.clear();
+    if (it.kind == RL_ITERATE_LUMELS) {
+        items = (int64_t)lm->w * lm->h;
+        steps = it.first < items ? (items - it.first + it.step - 1) / it.step : 0;
+        for (uint32_t j = 0; j < (uint32_t)lm->texels.size(); j++) {
+            const int64_t t = lm->texels[j];
+            if (t >= it.first && (t - it.first) % it.step == 0) {
+                sel.push_back(j);
+                step_of.push_back((uint32_t)((t - it.first) / it.step));
+            }
+        }
+        return RL_SUCCESS;
+    }
+    if (it.kind != RL_ITERATE_FACES) return fail(RL_INVALID_ARGUMENT, "unknown iterator kind");
+    items = m.num_tris;
+    steps = it.first < items ? (items - it.first + it.step - 1) / it.step : 0;
+    std::vector<std::pair<uint32_t, uint32_t>> by_face;  // (iterator step, lumel)
+    for (uint32_t j = 0; j < (uint32_t)lm->texels.size(); j++) {
+        const uint32_t tri = lm->tri[lm->texels[j]];
+        if (tri < m.first_tri || tri >= m.first_tri + m.num_tris) continue;
+        const int64_t f = tri - m.first_tri;
+        if (f >= it.first && (f - it.first) % it.step == 0) by_face.push_back({(uint32_t)((f - it.first) / it.step), j});
+    }
+    std::stable_sort(by_face.begin(), by_face.end(),
+                     [](const std::pair<uint32_t, uint32_t>& a, const std::pair<uint32_t, uint32_t>& b) {
+                         return a.first < b.first;
+                     });
+    for (const auto& p : by_face) {
+        step_of.push_back(p.first);
+        sel.push_back(p.second);
+    }
+    return RL_SUCCESS;
+}
+
+// Bakes the selected lumels of lm (indices into its valid lumels) with `bake`, which runs on the
+// selection copied to a compact segment (same texel ids and lightmap seed, so the same sample
+// streams), then writes the colours back. With a progress callback the selection is baked in
+// progress->updates parts, and Progress::notify's (mesh, step, stepCount) follows each part
+// (steps of the iterator whose lumels are all baked), then the final (items, items) as
+// Baker::bakeMesh ends (baker/Baker.cpp:71-74).
+template <class Bake>
+int32_t bake_selection(rl_relight* ctx, rl_scene* scene, int32_t mesh, rl_lightmap* lm,
+                       const std::vector<uint32_t>& sel, const std::vector<uint32_t>& step_of, int64_t steps,
+                       int64_t items, const rl_progress* progress, const AoParams& ao, Bake bake) {
+    int32_t st = RL_SUCCESS;
+    const uint32_t n = (uint32_t)sel.size();
+    const uint32_t parts = progress && progress->updates > 1 ? std::min<uint32_t>((uint32_t)progress->updates,
+                                                                                    std::max<uint32_t>(n, 1u))
+                                                              : 1u;
+    if (n) {
+        rl_lightmap tmp;
+        tmp.ctx = ctx;
+        tmp.scene = scene;
+        tmp.seed = lm->seed;
+        tmp.w = lm->w;
+        tmp.h = lm->h;
+        tmp.dirty = false;
+        uint32_t* d_sel = nullptr;
+        hipError_t e = dev_alloc(&d_sel, n);
+        if (e == hipSuccess) e = dev_alloc(&tmp.d_pos, n);
+        if (e == hipSuccess) e = dev_alloc(&tmp.d_nrm, n);
+        if (e == hipSuccess) e = dev_alloc(&tmp.d_color, n);
+        if (e == hipSuccess) e = dev_alloc(&tmp.d_counts, n);
+        tmp.d_capacity = n;
+        if (e == hipSuccess) e = hipMemcpy(d_sel, sel.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = launch_gather_lumels(ctx->stream, d_sel, n, lm->d_pos, lm->d_nrm, lm->d_color, tmp.d_pos, tmp.d_nrm,
+                                     tmp.d_color);
+        if (e != hipSuccess) {
+            dev_free(d_sel);
+            return fail(RL_DEVICE_ERROR, std::string("iterator selection: ") + hipGetErrorString(e));
+        }
+        for (uint32_t p = 0; p < parts && st == RL_SUCCESS; p++) {
+            const uint32_t a = (uint32_t)((uint64_t)n * p / parts), b = (uint32_t)((uint64_t)n * (p + 1) / parts);
+            std::vector<BakeItem> part{BakeItem{&tmp, scene, ao, a, b - a}};
+            st = bake(part);
+            if (st == RL_SUCCESS && progress && progress->notify && p + 1 < parts) {
+                // Steps below the next part's first step are finished.
+                const int64_t done = b < n ? (int64_t)step_of[b] : steps;
+                progress->notify(progress->user, mesh, (int32_t)std::min<int64_t>(done, INT32_MAX),
+                                 (int32_t)std::min<int64_t>(items, INT32_MAX));
+            }
+        }
+        if (st == RL_SUCCESS) {
+            e = launch_scatter_colors(ctx->stream, d_sel, n, tmp.d_color, lm->d_color);
+            if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+            if (e != hipSuccess) st = fail(RL_DEVICE_ERROR, std::string("iterator write-back: ") + hipGetErrorString(e));
+            lm->device_newer = true;
+        }
+        dev_free(d_sel);
+    }
+    if (st == RL_SUCCESS && progress && progress->notify)
+        progress->notify(progress->user, mesh, (int32_t)std::min<int64_t>(items, INT32_MAX),
+                         (int32_t)std::min<int64_t>(items, INT32_MAX));
+    return st;
+}
+
+// Baker::bakeMesh preconditions (baker/Baker.cpp:54-65) and the mesh's lightmap.
+int32_t mesh_lightmap(rl_relight* ctx, rl_scene* scene, int32_t mesh, rl_lightmap** out) {
+    if (!ctx) return fail(RL_INVALID_ARGUMENT, "context is NULL");
+    int32_t st = check_scene_ready(scene);
+    if (st) return st;
+    if (mesh < 0 || mesh >= (int32_t)scene->meshes.size()) return fail(RL_INVALID_ARGUMENT, "bad mesh index");
+    if (scene->meshes[mesh].num_tris == 0) return fail(RL_INVALID_CALL, "mesh has no faces");
+    *out = scene->meshes[mesh].lightmap;
+    if (!*out) return fail(RL_INVALID_CALL, "mesh has no lightmap");
+    return RL_SUCCESS;
+}
+
+// A bake call through an iterator: selection, then bake_selection.
+template <class Bake>
+int32_t bake_iterated(rl_relight* ctx, rl_scene* scene, int32_t mesh, const rl_bake_iterator* iterator,
+                      const rl_progress* progress, const AoParams& ao, Bake bake) {
+    rl_lightmap* lm = nullptr;
+    int32_t st = mesh_lightmap(ctx, scene, mesh, &lm);
+    if (st) return st;
+    // Relight::bake* with a NULL iterator uses LumelBakeIterator(0, 1) (Relight.cpp:218-220).
+    const rl_bake_iterator all{RL_ITERATE_LUMELS, 0, 1};
+    const rl_bake_iterator& it = iterator ? *iterator : all;
+    if ((st = push_lumels(lm))) return st;
+    std::vector<uint32_t> sel, step_of;
+    int64_t steps = 0, items = 0;
+    if ((st = iterator_selection(scene, mesh, lm, it, sel, step_of, steps, items))) return st;
+    CallTimer tm(ctx);
+    if ((st = begin_call(ctx))) return st;
+    if ((st = bake_selection(ctx, scene, mesh, lm, sel, step_of, steps, items, progress, ao, bake))) return st;
+    return end_call(ctx, tm);
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t rl_bake_direct_iter(rl_relight* ctx, rl_scene* scene, int32_t mesh, const rl_bake_iterator* iterator,
+                            const rl_progress* progress) {
+    return bake_iterated(ctx, scene, mesh, iterator, progress, AoParams{},
+                         [ctx](const std::vector<BakeItem>& items) { return run_direct_items(ctx, items); });
+}
+
+int32_t rl_bake_ambient_occlusion_iter(rl_relight* ctx, rl_scene* scene, int32_t mesh, const rl_ao_settings* settings,
+                                       const rl_bake_iterator* iterator, const rl_progress* progress) {
+    int32_t st = check_ao(settings);
+    if (st) return st;
+    return bake_iterated(ctx, scene, mesh, iterator, progress, ao_params(*settings),
+                         [ctx](const std::vector<BakeItem>& items) { return run_ao_items(ctx, items); });
+}
+
+int32_t rl_get_progress(const rl_relight* ctx, uint64_t* done, uint64_t* total) {
+    if (!ctx) return fail(RL_INVALID_ARGUMENT, "context is NULL");
+    if (done) *done = ctx->progress_done.load();
+    if (total) *total = ctx->progress_total.load();
+    return RL_SUCCESS;
+}
+
 int32_t rl_bake_direct(rl_relight* ctx, rl_scene* scene, int32_t mesh) {
     if (!ctx) return fail(RL_INVALID_ARGUMENT, "context is NULL");
     int32_t st = check_scene_ready(scene);
@@ -1809,6 +1991,26 @@ int32_t rl_emit_photons(rl_relight* ctx, rl_scene* scene, const rl_indirect_sett
     cleanup();
     if (e != hipSuccess) return fail(RL_DEVICE_ERROR, std::string("photon pass: ") + hipGetErrorString(e));
     return end_call(ctx, tm);
+}
+
+int32_t rl_bake_indirect_iter(rl_relight* ctx, rl_scene* scene, int32_t mesh, const rl_indirect_settings* s,
+                              const rl_bake_iterator* iterator, const rl_progress* progress) {
+    if (!ctx) return fail(RL_INVALID_ARGUMENT, "context is NULL");
+    int32_t st = check_scene_ready(scene);
+    if (st) return st;
+    if ((st = check_indirect_settings(s))) return st;
+    PhotonTables T;
+    if ((st = make_photon_tables(scene, T))) return st;
+    const FinalGatherParams fg = fg_params(s);
+    return bake_iterated(ctx, scene, mesh, iterator, progress, AoParams{},
+                         [ctx, &T, &fg](std::vector<BakeItem>& items) {
+                             for (BakeItem& it : items) {
+                                 it.fg = fg;
+                                 it.mesh_pm = T.d_mesh_pm;
+                                 it.pms = T.d_pms;
+                             }
+                             return run_fg_items(ctx, items);
+                         });
 }
 
 int32_t rl_bake_indirect(rl_relight* ctx, rl_scene* scene, int32_t mesh, const rl_indirect_settings* s) {

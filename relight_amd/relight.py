@@ -517,17 +517,55 @@ class Relight:
         s = settings.to_c()
         _check(self._lib, self._lib.rl_emit_photons(self._h, scene._h, C.byref(s)))
 
-    def bake_indirect_light(self, scene: Scene, mesh: int, settings: IndirectLightSettings):
-        s = settings.to_c()
-        _check(self._lib, self._lib.rl_bake_indirect(self._h, scene._h, int(mesh), C.byref(s)))
+    @staticmethod
+    def _iter_args(iterator, progress, updates):
+        """(rl_bake_iterator or None, rl_progress or None, keep-alive) for the *_iter entry points.
+        iterator: ("lumels" | "faces", first, step); progress: callable(mesh, step, step_count)."""
+        it = None
+        if iterator is not None:
+            kind, first, step = iterator
+            it = A.rl_bake_iterator({"lumels": A.RL_ITERATE_LUMELS, "faces": A.RL_ITERATE_FACES}[kind],
+                                    int(first), int(step))
+        pr, fn = None, None
+        if progress is not None:
+            fn = A.rl_progress_fn(lambda user, mesh, step, count: progress(mesh, step, count))
+            pr = A.rl_progress(fn, None, int(updates))
+        return (C.byref(it) if it is not None else None, C.byref(pr) if pr is not None else None, (it, pr, fn))
 
-    def bake_direct_light(self, scene: Scene, mesh: int):
-        _check(self._lib, self._lib.rl_bake_direct(self._h, scene._h, int(mesh)))
-
-    def bake_ambient_occlusion(self, scene: Scene, mesh: int, settings: AmbientOcclusionSettings):
+    def bake_indirect_light(self, scene: Scene, mesh: int, settings: IndirectLightSettings, iterator=None,
+                            progress=None, updates: int = 8):
+        """Relight::bakeIndirectLight (Relight.cpp:230-241); iterator ("lumels"|"faces", first,
+        step) and progress callback(mesh, step, step_count) as rl_bake_indirect_iter."""
         s = settings.to_c()
-        _check(self._lib, self._lib.rl_bake_ambient_occlusion(self._h, scene._h, int(mesh),
-                                                              C.byref(s)))
+        if iterator is None and progress is None:
+            _check(self._lib, self._lib.rl_bake_indirect(self._h, scene._h, int(mesh), C.byref(s)))
+            return
+        it, pr, keep = self._iter_args(iterator, progress, updates)
+        _check(self._lib, self._lib.rl_bake_indirect_iter(self._h, scene._h, int(mesh), C.byref(s), it, pr))
+
+    def bake_direct_light(self, scene: Scene, mesh: int, iterator=None, progress=None, updates: int = 8):
+        if iterator is None and progress is None:
+            _check(self._lib, self._lib.rl_bake_direct(self._h, scene._h, int(mesh)))
+            return
+        it, pr, keep = self._iter_args(iterator, progress, updates)
+        _check(self._lib, self._lib.rl_bake_direct_iter(self._h, scene._h, int(mesh), it, pr))
+
+    def bake_ambient_occlusion(self, scene: Scene, mesh: int, settings: AmbientOcclusionSettings, iterator=None,
+                               progress=None, updates: int = 8):
+        s = settings.to_c()
+        if iterator is None and progress is None:
+            _check(self._lib, self._lib.rl_bake_ambient_occlusion(self._h, scene._h, int(mesh), C.byref(s)))
+            return
+        it, pr, keep = self._iter_args(iterator, progress, updates)
+        _check(self._lib, self._lib.rl_bake_ambient_occlusion_iter(self._h, scene._h, int(mesh), C.byref(s), it,
+                                                                   pr))
+
+    def progress(self):
+        """(done, total) lumels of the bake call running on this context (pollable from another
+        thread)."""
+        d, t = C.c_uint64(), C.c_uint64()
+        _check(self._lib, self._lib.rl_get_progress(self._h, C.byref(d), C.byref(t)))
+        return d.value, t.value
 
     def bake_batch(self, scene: Scene, lightmaps: List[Lightmap], direct: bool = True,
                    ao: Optional[AmbientOcclusionSettings] = None):

@@ -42,7 +42,7 @@ def test_bindings_cover_the_header():
 
 
 # Every struct of the header with its ctypes mirror; the C compiler's offsetof decides the layout.
-_STRUCTS = ["rl_light_desc", "rl_ao_settings", "rl_hit", "rl_indirect_settings", "rl_tuning",
+_STRUCTS = ["rl_light_desc", "rl_ao_settings", "rl_hit", "rl_indirect_settings", "rl_tuning", "rl_bake_iterator", "rl_progress",
             "rl_solve_info", "rl_stats", "rl_bake_item", "rl_server_config", "rl_job_desc",
             "rl_job_status", "rl_server_stats"]
 
