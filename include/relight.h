@@ -222,6 +222,21 @@ int32_t rl_obj_load(const char* path, uint32_t* vertex_count, uint32_t* index_co
 int32_t rl_scene_add_obj(rl_scene* scene, const char* path, const float* transform, const float* albedo,
                          int32_t* out_mesh);
 
+/* Mesh::generateUv(angle, margin, padding) (scene/Mesh.cpp:229-238; UvGenerator, AngularChartifier,
+ * RectanglePacker, Types.h:2293-3326) on a mesh given as arrays: charts of faces within `angle`
+ * degrees of a seed face (connected through shared vertex indices), each projected onto its
+ * dominant axis plane and turned onto its principal axis when that shrinks it, packed with
+ * `padding` (in the mesh's units) and scaled to [0, 1]; the returned mesh's lightmap uvs are the
+ * generated set. Vertices are split and merged as the reference's indexer does (exact equality of
+ * position, normal and uvs), so the counts change. margin is unused, as in the reference. Faces
+ * with a zero normal join no chart at angles below 90 and are dropped, as in the reference. With
+ * NULL output arrays only the counts are written. diffuse_uvs may be NULL (zeros). */
+int32_t rl_mesh_generate_uv(const float* positions, const float* normals, const float* diffuse_uvs,
+                            uint32_t vertex_count, const uint32_t* indices, uint32_t index_count,
+                            float angle, float margin, float padding, uint32_t* out_vertex_count,
+                            uint32_t* out_index_count, float* out_positions, float* out_normals,
+                            float* out_diffuse_uvs, float* out_lightmap_uvs, uint32_t* out_indices);
+
 /* Relight::createLightmap (Relight.h:159) and Lightmap::addMesh (Lightmap.cpp:93-103), which
  * rasterises the mesh's texels (Lightmap.cpp:133-164). A mesh may belong to one lightmap. The
  * lightmap seed keys its texels' sample streams (default: creation order in the context). */

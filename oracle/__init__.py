@@ -94,6 +94,9 @@ def lib():
             "or_scene_add_mesh_textured": (C.c_int32, [C.c_void_p, _F, _F, _F, _F, C.c_uint32, _U32, C.c_uint32,
                                                        _F, _F, C.c_int32]),
             "or_texture_color": (None, [C.c_void_p, C.c_int32, C.c_float, C.c_float, _F]),
+            "or_generate_uv": (C.c_int32, [_F, _F, _F, C.c_uint32, _U32, C.c_uint32, C.c_float, C.c_float, _U32,
+                                           _U32]),
+            "or_generate_uv_get": (None, [_F, _F, _F, _F, _U32]),
             "or_face_color": (None, [C.c_void_p, C.c_uint32, C.c_float, C.c_float, _F]),
             "or_scene_build": (C.c_int32, [C.c_void_p, C.c_int32]),
             "or_raster": (C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, _U8, _U32, _F, _F]),
@@ -383,6 +386,25 @@ class OracleScene:
         valid, tri, pos, nrm = grids
         texel = np.nonzero(valid)[0].astype(np.uint32)
         return texel, pos[texel].copy(), nrm[texel].copy()
+
+
+def generate_uv(positions, normals, indices, angle=88.0, padding=0.0, diffuse_uvs=None):
+    """Mesh::generateUv(angle, margin, padding): (positions, normals, diffuse uvs, lightmap uvs,
+    indices) of the regenerated mesh."""
+    p = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+    n = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+    ix = np.ascontiguousarray(indices, np.uint32).reshape(-1)
+    d = None if diffuse_uvs is None else np.ascontiguousarray(diffuse_uvs, np.float32).reshape(-1, 2)
+    nv, ni = C.c_uint32(), C.c_uint32()
+    lib().or_generate_uv(_f(p), _f(n), _f(d), p.shape[0], _u(ix), ix.size, angle, padding, C.byref(nv),
+                         C.byref(ni))
+    op = np.zeros((nv.value, 3), np.float32)
+    on = np.zeros((nv.value, 3), np.float32)
+    od = np.zeros((nv.value, 2), np.float32)
+    ol = np.zeros((nv.value, 2), np.float32)
+    oi = np.zeros(ni.value, np.uint32)
+    lib().or_generate_uv_get(_f(op), _f(on), _f(od), _f(ol), _u(oi))
+    return op, on, od, ol, oi
 
 
 def expand(valid, rgb):

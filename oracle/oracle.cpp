@@ -16,6 +16,7 @@
 //   final gather              baker/IndirectLight.cpp:47-75
 //   form factors              hemisphere-ray estimator of radiosity/RadiosityBuilder.cpp:142-176
 //   post-process, encodings   Lightmap.cpp:167-331 (expand, blur, toDoubleLdr, toRgbmLdr), Types.h:216-234
+//   lightmap uv generation    scene/Mesh.cpp:229-238, Types.h:2293-3326 (charts, flatten, rotate, pack)
 // Project-defined parts shared by contract with the GPU path (not in the reference): PCG32 streams
 // keyed by (seed, lightmap seed, texel, technique, sample); cosine sampling (Malley + Duff et al.
 // 2017 basis); the watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2013); closest-hit ties
@@ -40,6 +41,8 @@
 #include <algorithm>
 #include <atomic>
 #include <cfloat>
+#include <map>
+#include <memory>
 #include <thread>
 #include <unordered_map>
 #include <vector>
@@ -1299,6 +1302,275 @@ int32_t or_form_factors(void* sp, const float* patches, const uint32_t* tri_patc
         }
     });
     return 0;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------ lightmap uv generation
+// Mesh::generateUv (scene/Mesh.cpp:229-238): UvGenerator::generate (Types.h:3063-3160) with
+// AngularChartifier (Types.h:2427-2595), DCEL::create (Types.h:2380-2423), Face::flatten / normal
+// (Types.h:2825-2886), UvGenerator::rotate (Types.h:3031-3060; covarianceMatrix, CovMatrix::converge,
+// Types.h:2200-2290) and RectanglePacker::place (Types.h:3163-3326), as written (recursive flood,
+// the plain grow-by-0.1 loop).
+namespace uvgen {
+
+struct V {
+    float p[3], n[3], t0[2], t1[2];
+    bool operator<(const V& o) const {  // Vertex::Compare (scene/Mesh.h:54-70)
+        for (int i = 0; i < 3; i++) if (p[i] != o.p[i]) return p[i] < o.p[i];
+        for (int i = 0; i < 3; i++) if (n[i] != o.n[i]) return n[i] < o.n[i];
+        for (int i = 0; i < 2; i++) if (t0[i] != o.t0[i]) return t0[i] < o.t0[i];
+        for (int i = 0; i < 2; i++) if (t1[i] != o.t1[i]) return t1[i] < o.t1[i];
+        return false;
+    }
+};
+
+struct Indexer {
+    std::map<V, uint32_t> cache;
+    std::vector<V> vb;
+    std::vector<uint32_t> ib;
+    uint32_t operator+=(const V& v) {
+        auto it = cache.find(v);
+        uint32_t idx;
+        if (it != cache.end()) idx = it->second;
+        else { idx = (uint32_t)vb.size(); cache[v] = idx; vb.push_back(v); }
+        ib.push_back(idx);
+        return idx;
+    }
+};
+
+struct Half { uint32_t face, vertex; Half* next; Half* twin; };
+
+struct Gen {
+    const std::vector<V>& vs;
+    const std::vector<uint32_t>& is;
+    float angle;
+    std::map<int, int> chart_by_face;
+    std::vector<std::vector<uint32_t>> charts;
+    Gen(const std::vector<V>& v, const std::vector<uint32_t>& i, float a) : vs(v), is(i), angle(a) {}
+
+    Vec normal(uint32_t f) const {
+        const float* a = vs[is[3 * f]].p; const float* b = vs[is[3 * f + 1]].p; const float* c = vs[is[3 * f + 2]].p;
+        const Vec u = mk(b[0] - a[0], b[1] - a[1], b[2] - a[2]), w = mk(c[0] - a[0], c[1] - a[1], c[2] - a[2]);
+        Vec n = mk(u.y * w.z - u.z * w.y, u.z * w.x - u.x * w.z, u.x * w.y - u.y * w.x);
+        const float len = sqrtf(n.x * n.x + n.y * n.y + n.z * n.z);
+        if (len) { n.x /= len; n.y /= len; n.z /= len; }
+        return n;
+    }
+
+    void add_to_chart(const Half* e, Vec axis, int index) {
+        if (chart_by_face.count((int)e->face)) return;
+        const float ang = acosf(dot3(axis, normal(e->face))) * 180.0f / 3.1415926535897932f;
+        if (ang > angle) return;
+        chart_by_face[(int)e->face] = index;
+        if ((int)charts.size() <= index) charts.resize(index + 1);
+        charts[index].push_back(e->face);
+        const Half* i = e;
+        do {
+            if (i->twin) add_to_chart(i->twin, axis, index);
+            i = i->next;
+        } while (i != e);
+    }
+};
+
+struct R { float x, y, w, h; };
+
+struct Node {
+    bool free = true;
+    std::unique_ptr<Node> l, r;
+    R rc;
+    explicit Node(R q) : rc(q) {}
+    bool assign(R& pl) {
+        if (l || r) return l->assign(pl) ? true : r->assign(pl);
+        if ((pl.w > rc.w || pl.h > rc.h) || !free) return false;
+        if (pl.w == rc.w && pl.h == rc.h) { pl.x = rc.x; pl.y = rc.y; free = false; return true; }
+        const float dw = rc.w - pl.w, dh = rc.h - pl.h;
+        if (dw > dh) {
+            l.reset(new Node(R{rc.x, rc.y, pl.w, rc.h}));
+            r.reset(new Node(R{rc.x + pl.w, rc.y, rc.w - pl.w, rc.h}));
+        } else {
+            l.reset(new Node(R{rc.x, rc.y, rc.w, pl.h}));
+            r.reset(new Node(R{rc.x, rc.y + pl.h, rc.w, rc.h - pl.h}));
+        }
+        return assign(pl);
+    }
+};
+
+bool place(std::vector<R>& rects, float w, float h) {
+    std::vector<R*> order;
+    for (R& q : rects) order.push_back(&q);
+    std::sort(order.begin(), order.end(), [](const R* a, const R* b) { return a->w * a->h > b->w * b->h; });
+    Node root(R{0.0f, 0.0f, w - 0.0f, h - 0.0f});
+    for (R* q : order) if (!root.assign(*q)) return false;
+    return true;
+}
+
+void bounds(const std::vector<std::pair<float, float>>& uv, float mn[2], float mx[2]) {
+    mn[0] = mn[1] = FLT_MAX; mx[0] = mx[1] = -FLT_MAX;
+    for (const auto& t : uv) {
+        mn[0] = std::min(mn[0], t.first); mx[0] = std::max(mx[0], t.first);
+        mn[1] = std::min(mn[1], t.second); mx[1] = std::max(mx[1], t.second);
+    }
+}
+
+void rotate(std::vector<std::pair<float, float>>& uv) {
+    const uint32_t n = (uint32_t)uv.size();
+    float sx = 0.0f, sy = 0.0f;
+    for (const auto& t : uv) { sx += t.first; sy += t.second; }
+    const float mx0 = sx / (float)n, my0 = sy / (float)n;
+    std::vector<std::pair<float, float>> c;
+    for (const auto& t : uv) c.push_back({t.first - mx0, t.second - my0});
+    float cx = 0.0f, cy = 0.0f;
+    for (const auto& t : c) { cx += t.first; cy += t.second; }
+    const float m[2] = {cx / (float)n, cy / (float)n};
+    float M[4];  // M[row * 2 + col]
+    for (int row = 0; row < 2; row++)
+        for (int col = 0; col < 2; col++) {
+            float cov = 0.0f;
+            for (uint32_t i = 0; i < n; i++) {
+                const float a = (row == 0 ? c[i].first : c[i].second) - m[row];
+                const float b = (col == 0 ? c[i].first : c[i].second) - m[col];
+                cov += a * b;
+            }
+            M[row * 2 + col] = cov / (float)n;
+        }
+    float v[2] = {1.0f, 1.0f};
+    for (int it = 0; it < 10; it++) {
+        const float r0 = (0.0f + M[0] * v[0]) + M[1] * v[1];
+        const float r1 = (0.0f + M[2] * v[0]) + M[3] * v[1];
+        v[0] = r0; v[1] = r1;
+    }
+    float len = 0.0f;
+    len += v[0] * v[0];
+    len += v[1] * v[1];
+    len = sqrtf(len);
+    if (len != 0.0f) { v[0] /= len; v[1] /= len; }
+    const float e2[2] = {v[1], -v[0]};
+    std::vector<std::pair<float, float>> rot;
+    for (const auto& t : uv) rot.push_back({v[0] * t.first + v[1] * t.second, e2[0] * t.first + e2[1] * t.second});
+    float bmn[2], bmx[2], amn[2], amx[2];
+    bounds(uv, bmn, bmx);
+    bounds(rot, amn, amx);
+    const float bw = bmx[0] - bmn[0], bh = bmx[1] - bmn[1], aw = amx[0] - amn[0], ah = amx[1] - amn[1];
+    if ((aw * ah) < (bw * bh)) uv = rot;
+}
+
+}  // namespace uvgen
+
+// The generated mesh is kept until the next call; or_generate_uv_get copies it out.
+static std::vector<uvgen::V> g_uv_vb;
+static std::vector<uint32_t> g_uv_ib;
+
+extern "C" {
+
+int32_t or_generate_uv(const float* pos, const float* nrm, const float* duv, uint32_t nv, const uint32_t* idx,
+                       uint32_t ni, float angle, float padding, uint32_t* out_nv, uint32_t* out_ni) {
+    using namespace uvgen;
+    std::vector<V> vs(nv);
+    for (uint32_t i = 0; i < nv; i++) {
+        for (int k = 0; k < 3; k++) { vs[i].p[k] = pos[3 * i + k]; vs[i].n[k] = nrm[3 * i + k]; }
+        vs[i].t0[0] = duv ? duv[2 * i] : 0.0f;
+        vs[i].t0[1] = duv ? duv[2 * i + 1] : 0.0f;
+        vs[i].t1[0] = vs[i].t1[1] = 0.0f;
+    }
+    std::vector<uint32_t> is(idx, idx + ni);
+    const uint32_t nf = ni / 3;
+    // DCEL::create: a multimap of (vertex, next vertex) -> edge, twins by lookup.
+    std::vector<Half> edges(3 * (size_t)nf);
+    std::multimap<std::pair<uint32_t, uint32_t>, Half*> cache;
+    for (uint32_t f = 0; f < nf; f++) {
+        Half* fe = &edges[3 * (size_t)f];
+        for (int i = 0; i < 3; i++) { fe[i].face = f; fe[i].vertex = is[3 * f + i]; fe[i].twin = nullptr; }
+        for (int i = 0; i < 3; i++) {
+            fe[i].next = &fe[(i + 1) % 3];
+            cache.insert({{fe[i].vertex, fe[i].next->vertex}, &fe[i]});
+        }
+    }
+    for (Half& e : edges) {
+        auto z = cache.find({e.next->vertex, e.vertex});
+        if (z != cache.end() && &e != z->second) { e.twin = z->second; z->second->twin = &e; }
+    }
+    Gen G(vs, is, angle);
+    for (Half& e : edges) G.add_to_chart(&e, G.normal(e.face), (int)G.charts.size());
+    Indexer ix;
+    std::vector<std::vector<uint32_t>> cv(G.charts.size());
+    for (size_t c = 0; c < G.charts.size(); c++) {
+        Vec n = mk(0, 0, 0);
+        for (uint32_t f : G.charts[c]) n = add(n, G.normal(f));
+        n = divs(n, (float)G.charts[c].size());
+        normalize3(n);
+        const float ax = fabsf(n.x), ay = fabsf(n.y), az = fabsf(n.z);
+        int axis = 2;
+        if (ax > ay && ax > az) axis = 0;
+        else if (ay > ax && ay > az) axis = 1;
+        std::vector<std::pair<float, float>> uv;
+        for (uint32_t f : G.charts[c])
+            for (int k = 0; k < 3; k++) {
+                const float* p = vs[is[3 * f + k]].p;
+                uv.push_back(axis == 0 ? std::make_pair(p[1], p[2]) : axis == 1 ? std::make_pair(p[0], p[2])
+                                                                                 : std::make_pair(p[0], p[1]));
+            }
+        rotate(uv);
+        float mn[2], mx[2];
+        bounds(uv, mn, mx);
+        const float sw = mx[0] - mn[0], sh = mx[1] - mn[1];
+        size_t q = 0;
+        for (uint32_t f : G.charts[c])
+            for (int k = 0; k < 3; k++) {
+                V v = vs[is[3 * f + k]];
+                const auto t = uv[q++];
+                if (sw > sh) { v.t1[0] = t.first - mn[0]; v.t1[1] = t.second - mn[1]; }
+                else { v.t1[0] = t.second - mn[1]; v.t1[1] = t.first - mn[0]; }
+                cv[c].push_back(ix += v);
+            }
+    }
+    std::vector<R> rects;
+    for (size_t c = 0; c < G.charts.size(); c++) {
+        float mn[2] = {FLT_MAX, FLT_MAX}, mx[2] = {-FLT_MAX, -FLT_MAX};
+        for (uint32_t i : cv[c]) {
+            const float* t = ix.vb[i].t1;
+            mn[0] = std::min(t[0], mn[0]); mn[1] = std::min(t[1], mn[1]);
+            mx[0] = std::max(t[0], mx[0]); mx[1] = std::max(t[1], mx[1]);
+        }
+        const float sw = mx[0] - mn[0], sh = mx[1] - mn[1];
+        rects.push_back(R{0.0f, 0.0f, std::max(sw, sh) + padding, std::min(sw, sh) + padding});
+    }
+    float w = 1.0f, h = 1.0f;
+    bool ew = true;
+    while (!place(rects, w, h)) {
+        if (ew) { w += 0.1f; ew = false; } else { h += 0.1f; ew = true; }
+    }
+    Indexer out;
+    for (size_t c = 0; c < G.charts.size(); c++)
+        for (uint32_t i : cv[c]) {
+            V v = ix.vb[i];
+            v.t1[0] = rects[c].x + v.t1[0];
+            v.t1[1] = rects[c].y + v.t1[1];
+            out += v;
+        }
+    float mn[2] = {FLT_MAX, FLT_MAX}, mx[2] = {-FLT_MAX, -FLT_MAX};
+    for (uint32_t i : out.ib) {
+        const float* t = out.vb[i].t1;
+        mn[0] = std::min(t[0], mn[0]); mn[1] = std::min(t[1], mn[1]);
+        mx[0] = std::max(t[0], mx[0]); mx[1] = std::max(t[1], mx[1]);
+    }
+    for (V& v : out.vb) {
+        v.t1[0] = (v.t1[0] - mn[0]) / (mx[0] - mn[0]);
+        v.t1[1] = (v.t1[1] - mn[1]) / (mx[1] - mn[1]);
+    }
+    g_uv_vb = out.vb;
+    g_uv_ib = out.ib;
+    *out_nv = (uint32_t)out.vb.size();
+    *out_ni = (uint32_t)out.ib.size();
+    return 0;
+}
+
+void or_generate_uv_get(float* pos, float* nrm, float* duv, float* luv, uint32_t* idx) {
+    for (size_t i = 0; i < g_uv_vb.size(); i++) {
+        for (int k = 0; k < 3; k++) { pos[3 * i + k] = g_uv_vb[i].p[k]; nrm[3 * i + k] = g_uv_vb[i].n[k]; }
+        for (int k = 0; k < 2; k++) { duv[2 * i + k] = g_uv_vb[i].t0[k]; luv[2 * i + k] = g_uv_vb[i].t1[k]; }
+    }
+    std::copy(g_uv_ib.begin(), g_uv_ib.end(), idx);
 }
 
 }  // extern "C"

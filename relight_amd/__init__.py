@@ -9,11 +9,12 @@ from ._capi import (RL_FORMAT_RAW_HDR, RL_FORMAT_TGA_DOUBLE_LDR, RL_FORMAT_TGA_R
                     RL_PHOTON_RUSSIAN_ROULETTE, RL_PHOTON_THRESHOLD, RL_SAMPLE_COSINE,
                     RL_SAMPLE_UNIFORM, RL_TRACE_ANY, RL_TRACE_CLOSEST)
 from .relight import (AmbientOcclusionSettings, BakeJob, IndirectLightSettings, LightDesc,
-                      Lightmap, Pass, Photonmap, Radiosity, Relight, RelightError, Scene, Server, Texture)
+                      Lightmap, Pass, Photonmap, Radiosity, Relight, RelightError, Scene, Server, Texture,
+                      generate_uv)
 
 __all__ = [
     "Relight", "Scene", "Lightmap", "Photonmap", "LightDesc", "AmbientOcclusionSettings",
-    "IndirectLightSettings", "RelightError", "Radiosity", "Server", "BakeJob", "Pass", "Texture",
+    "IndirectLightSettings", "RelightError", "Radiosity", "Server", "BakeJob", "Pass", "Texture", "generate_uv",
     "RL_LIGHT_POINT", "RL_LIGHT_SPOT", "RL_LIGHT_DIRECTIONAL", "RL_LIGHT_AREA",
     "RL_SAMPLE_UNIFORM", "RL_SAMPLE_COSINE", "RL_TRACE_ANY", "RL_TRACE_CLOSEST",
     "RL_PHOTON_THRESHOLD", "RL_PHOTON_RUSSIAN_ROULETTE", "RL_GATHER_TEXEL_DISC", "RL_GATHER_KNN",

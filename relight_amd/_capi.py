@@ -253,6 +253,8 @@ SIGNATURES = [
     ("rl_scene_add_light", C.c_int32, [_P, C.POINTER(rl_light_desc), _I32]),
     ("rl_scene_end", C.c_int32, [_P]),
     ("rl_scene_info", C.c_int32, [_P, _U32, _U32, _I32]),
+    ("rl_mesh_generate_uv", C.c_int32, [_F, _F, _F, C.c_uint32, _U32, C.c_uint32, C.c_float, C.c_float, C.c_float,
+                                        _U32, _U32, _F, _F, _F, _F, _U32]),
     ("rl_lightmap_create", C.c_int32, [_P, C.c_int32, C.c_int32, _PP]),
     ("rl_lightmap_destroy", C.c_int32, [_P]),
     ("rl_lightmap_add_mesh", C.c_int32, [_P, _P, C.c_int32]),

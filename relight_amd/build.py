@@ -9,7 +9,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 OUT = os.path.join(HERE, "librelight_amd.so")
 SOURCES = ["rl_kernels.hip", "rl_photon.hip", "rl_radiosity.hip", "rl_post.hip", "rl_raster.hip",
-           "rl_noalloc.hip", "rl_sort.hip", "rl_runtime.cpp", "rl_bvh.cpp", "rl_obj.cpp"]
+           "rl_noalloc.hip", "rl_sort.hip", "rl_runtime.cpp", "rl_bvh.cpp", "rl_obj.cpp", "rl_uv.cpp"]
 
 # -ffp-contract=off keeps host (lumel setup) and device (ray setup, shading) arithmetic rounding
 # as written; -packed-fp32-ops keeps the compiler from emitting packed fp32 instructions.

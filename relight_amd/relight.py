@@ -202,6 +202,27 @@ class LightDesc:
         return d
 
 
+def generate_uv(positions, normals, indices, angle=88.0, margin=0.0, padding=0.0, diffuse_uvs=None):
+    """Mesh::generateUv (scene/Mesh.cpp:229-238) through rl_mesh_generate_uv: returns the mesh
+    (positions, normals, diffuse uvs, lightmap uvs, indices) with a generated lightmap uv set."""
+    lib = A.load()
+    p = np.ascontiguousarray(positions, np.float32).reshape(-1, 3)
+    n = np.ascontiguousarray(normals, np.float32).reshape(-1, 3)
+    ix = np.ascontiguousarray(indices, np.uint32).reshape(-1)
+    d = None if diffuse_uvs is None else np.ascontiguousarray(diffuse_uvs, np.float32).reshape(-1, 2)
+    nv, ni = C.c_uint32(), C.c_uint32()
+    args = (_fptr(p), _fptr(n), _fptr(d), p.shape[0], _uptr(ix), ix.size, float(angle), float(margin), float(padding))
+    _check(lib, lib.rl_mesh_generate_uv(*args, C.byref(nv), C.byref(ni), None, None, None, None, None))
+    op = np.zeros((nv.value, 3), np.float32)
+    on = np.zeros((nv.value, 3), np.float32)
+    od = np.zeros((nv.value, 2), np.float32)
+    ol = np.zeros((nv.value, 2), np.float32)
+    oi = np.zeros(ni.value, np.uint32)
+    _check(lib, lib.rl_mesh_generate_uv(*args, C.byref(nv), C.byref(ni), _fptr(op), _fptr(on), _fptr(od), _fptr(ol),
+                                        _uptr(oi)))
+    return op, on, od, ol, oi
+
+
 class Texture:
     """Texture::create (scene/Material.h:95): (h, w, channels) uint8 pixels, channels 3 or 4."""
 
