@@ -254,17 +254,20 @@ constexpr uint32_t kFgLeafExit = 16;
 // does, and writes the lumel when its last sample has been added. A lane may only start pair j <
 // consumed + kFgRing.
 constexpr uint32_t kFgRing = 128;
-// LDS per workgroup: 64 KB stacks + 40 KB rings + 13 KB of the scene's first 104 nodes.
+// Workgroups of four wavefronts, four to a compute unit (registers allow four wavefronts per
+// SIMD): passes of a few thousand lumels, a lumel per wavefront, still spread over every compute
+// unit. LDS per workgroup: 16 KB stacks + 10 KB rings + 13 KB of the scene's first 104 nodes.
+constexpr int kFgRefillBlock = 256;
 constexpr uint32_t kFgRefillTopNodes = 104;
 
 template <int CAP>
-__global__ __launch_bounds__(kFgBlock) void k_fg_refill(const BakeSeg* __restrict__ segs, int nsegs, uint64_t total,
-                                                        uint32_t refill_min, unsigned long long* queue,
-                                                        unsigned long long* rays_out) {
-    __shared__ uint32_t stack[kLdsStack * kFgBlock];
+__global__ __launch_bounds__(kFgRefillBlock) void k_fg_refill(const BakeSeg* __restrict__ segs, int nsegs,
+                                                              uint64_t total, uint32_t refill_min,
+                                                              unsigned long long* queue, unsigned long long* rays_out) {
+    __shared__ uint32_t stack[kLdsStack * kFgRefillBlock];
     __shared__ u32x4 top_rows[kFgRefillTopNodes * kNodeRows];
-    __shared__ float4 ring[kFgBlock / 64][kFgRing];          // x, y, z: term; w: flags (0 = pending)
-    __shared__ uint32_t ring_lumel[kFgBlock / 64][kFgRing];  // queue index of the slot's lumel
+    __shared__ float4 ring[kFgRefillBlock / 64][kFgRing];          // x, y, z: term; w: flags (0 = pending)
+    __shared__ uint32_t ring_lumel[kFgRefillBlock / 64][kFgRing];  // queue index of the slot's lumel
     const SceneDev top_scene = segs[0].scene;
     const uint32_t top_count = load_top_nodes(top_scene, top_rows, kFgRefillTopNodes);
     uint32_t* stk = stack + threadIdx.x;
@@ -347,10 +350,10 @@ __global__ __launch_bounds__(kFgBlock) void k_fg_refill(const BakeSeg* __restric
                 want &= ~__ballot(mine);
             }
         }
-        if (active && (one_scene(S) ? closest_run<CAP, true>(S, top, tr, wp, node, sp, stk, kFgBlock, spill, res, pend,
-                                                             kFgNodeExit, kFgLeafExit)
-                                    : closest_run<CAP, false>(S, top, tr, wp, node, sp, stk, kFgBlock, spill, res, pend,
-                                                              kFgNodeExit, kFgLeafExit)) != 0) {
+        if (active && (one_scene(S) ? closest_run<CAP, true>(S, top, tr, wp, node, sp, stk, kFgRefillBlock, spill, res,
+                                                             pend, kFgNodeExit, kFgLeafExit)
+                                    : closest_run<CAP, false>(S, top, tr, wp, node, sp, stk, kFgRefillBlock, spill,
+                                                              res, pend, kFgNodeExit, kFgLeafExit)) != 0) {
             rays++;
             const BakeSeg& sg = segs[my_seg];
             const FinalGatherParams& P = sg.fg;
@@ -792,7 +795,7 @@ hipError_t launch_final_gather(hipStream_t st, int cap, uint32_t max_blocks, con
     const uint32_t refill_min = 16;
     hipError_t e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(queue), 2, 0u);
     if (e != hipSuccess) return e;
-    // max_blocks counts kBlock-thread workgroups; these kernels' are kFgBlock threads.
+    // max_blocks counts kBlock-thread workgroups; these kernels' are kFgBlock and kFgRefillBlock threads.
     const uint64_t cap_blocks = std::max<uint64_t>(1, (uint64_t)max_blocks * kBlock / kFgBlock);
     // One lumel per lane needs many lumels per resident lane to keep lanes busy to the end; smaller
     // passes give each lumel a wavefront.
@@ -803,8 +806,9 @@ hipError_t launch_final_gather(hipStream_t st, int cap, uint32_t max_blocks, con
         RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_fg_lanes<CAP>, dim3(grid), dim3(kFgBlock), 0, st, segs, nsegs,
                                                 total_lumels, refill_min, queue, rays));
     } else {
-        const uint32_t grid = (uint32_t)std::min<uint64_t>(blocks_for(total_lumels, kFgBlock / 64), cap_blocks);
-        RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_fg_refill<CAP>, dim3(grid), dim3(kFgBlock), 0, st, segs, nsegs,
+        const uint64_t refill_cap = std::max<uint64_t>(1, (uint64_t)max_blocks * kBlock / kFgRefillBlock);
+        const uint32_t grid = (uint32_t)std::min<uint64_t>(blocks_for(total_lumels, kFgRefillBlock / 64), refill_cap);
+        RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_fg_refill<CAP>, dim3(grid), dim3(kFgRefillBlock), 0, st, segs, nsegs,
                                                 total_lumels, refill_min, queue, rays));
     }
     return hipGetLastError();

@@ -8,7 +8,7 @@
 // the fraction of `rays` cosine-weighted rays from patch i whose closest hit lies on patch j
 // (rows sum to <= 1), stored exactly as 16-bit hit counts.
 //
-// F layout: u16 counts, row-major, rows padded to a multiple of 64 entries. Vectors (E, rho, B,
+// F layout: u16 counts, npad x npad (npad a multiple of 64), in 16 x 32 tiles (below). Vectors (E, rho, B,
 // residual, correction) are column-major fp32 [rhs][patch]; MFMA operands X are column-major
 // bf16 splits of a vector (x = x1 + x2 + x3 to ~24 bits).
 #include <hip/hip_runtime.h>
