@@ -439,9 +439,13 @@ int32_t rl_bake_scene(rl_relight* ctx, rl_scene* scene, int32_t direct, const rl
  * Worker.cpp and src/demo/Lightmapping.cpp:619-649 for serving). A background thread owns the
  * context's stream; submitted jobs wait in a FIFO queue and are admitted while the sum of admitted
  * jobs' device footprints stays within memory_budget and at most max_active run. Every step the
- * loop takes up to quantum_rays rays of direct, AO and final-gather work from all admitted jobs,
- * fairly shared, into one multi-scene launch per pass (continuous batching: a job submitted while
- * others run joins at the next step); photon emission and radiosity phases run as whole steps. Scenes, lightmaps
+ * loop takes up to quantum_rays rays of direct, AO and final-gather work, one launch per phase, from
+ * the jobs of one scene: that of the oldest job with ray work. The step takes that scene's jobs
+ * older than any other scene's oldest job and its small jobs (whole remaining work within a
+ * quarter of an even share), each first a quarter of an even share, the rest in admission order
+ * (continuous batching: a job submitted while others run joins at the next step). The next step is
+ * planned and queued while one runs. Photon emission runs as a whole step; radiosity jobs run
+ * beside the steps on a second stream. Scenes, lightmaps
  * (and the scene's photonmaps) and radiosity objects are borrowed until the job finishes, fails or
  * is cancelled: their destroy calls return RL_INVALID_CALL meanwhile. While a server runs, make no
  * other call on its context. Results are identical to the same job run alone. */
@@ -459,7 +463,7 @@ typedef struct rl_server rl_server;
 typedef struct rl_server_config {
     uint64_t memory_budget;  /* bytes of per-job device memory admitted at once */
     int32_t max_active;      /* jobs admitted at once */
-    uint64_t quantum_rays;   /* rays of direct/AO work per step */
+    uint64_t quantum_rays;   /* rays of direct/AO/final-gather work per step */
 } rl_server_config;
 typedef struct rl_job_desc {
     int32_t kind;
@@ -492,7 +496,8 @@ typedef struct rl_server_stats {
     uint64_t submitted, completed, failed, cancelled, rejected;
     uint64_t steps;
     uint64_t rays;
-    double busy_ms;      /* wall time spent in steps */
+    double busy_ms;      /* wall time with a step on the GPU (the next step is planned and queued
+                          * while one runs) */
     uint64_t peak_memory;
 } rl_server_stats;
 int32_t rl_server_create(rl_relight* ctx, const rl_server_config* config, rl_server** out);

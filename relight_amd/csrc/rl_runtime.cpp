@@ -2362,6 +2362,10 @@ struct Job {
     std::unique_ptr<PhotonTables> tables;  // final gather: the scene's photon tables
     std::string error;    // message of the failing call, handed to the caller's rl_last_error
     bool holds = false;   // counted in the server_jobs of its scene, lightmaps and radiosity
+    // Steps queued on the stream that bake part of this job and have not finished yet, and
+    // whether every step of its plan has been queued (it is done once those have finished).
+    int inflight = 0;
+    bool planned = false;
 
     // Marks the job's scene, lightmaps and radiosity object as used (at submit) or no longer used
     // (once the job has finished, failed or been cancelled): their destroy calls refuse meanwhile.
@@ -2396,6 +2400,22 @@ struct rl_server {
     uint64_t next_id = 1, mem_used = 0;
     rl_server_stats stats{};
     std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    // A step queued on the context's stream: the worker plans and queues step k + 1 while step k
+    // runs, then waits for step k, so the GPU does not idle while the host prepares a step.
+    struct StepSlot {
+        hipEvent_t done = nullptr;             // recorded after the step's launches
+        std::vector<Job*> jobs;                // jobs with lumels in the step
+        unsigned long long* d_rays = nullptr;  // the step's ray counter
+        unsigned long long* h_rays = nullptr;  // pinned copy of it, valid once `done` has completed
+        uint64_t extra_rays = 0;               // photon rays of the emissions run while planning it
+        double t_queued = 0;
+        bool live = false;
+    };
+    StepSlot slots[2];
+    double last_finish_ms = 0;
+    // Photon tables of finished final gathers, freed once no step is in flight (hipFree waits for
+    // the device, which would drain the queued step).
+    std::vector<std::unique_ptr<PhotonTables>> retired_tables;
 
     double now_ms() const {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -2404,7 +2424,8 @@ struct rl_server {
     void aux_loop();
     void admit_locked();
     void retire_locked();
-    uint64_t step(std::vector<Job*>& js);
+    bool queue_step(std::vector<Job*>& js, StepSlot& sl);
+    void finish_step(StepSlot& sl);
 };
 
 namespace {
@@ -2431,12 +2452,40 @@ RayPhase ray_phase(const Job* j) {
     }
 }
 
-// Moves a job to the next step of its plan, or to done.
+// Moves a job to the next step of its plan; after the last, the job is planned (it becomes done
+// once the steps that bake it have finished).
 void advance(Job* j) {
     j->map_i = 0;
     j->cursor = 0;
     j->pi++;
-    if (j->pi >= j->plan.size()) j->state = RL_JOB_DONE;
+    if (j->pi >= j->plan.size()) j->planned = true;
+}
+
+uint64_t phase_rpl(const Job* j, int step) {
+    if (step == kStepDirect) return direct_rays_per_lumel(j->desc.scene);
+    if (step == kStepAo) return (uint64_t)j->ao.samples;
+    if (step == kStepGather) return (uint64_t)j->ind.final_gather_samples;
+    return 0;
+}
+
+// Rays of the current ray phase of a job that are not planned yet.
+uint64_t phase_rays_left(const Job* j) {
+    uint64_t left = 0;
+    for (size_t k = j->map_i; k < j->valid.size(); k++) left += j->valid[k];
+    left -= std::min<uint64_t>(left, j->cursor);
+    return left * phase_rpl(j, current_step(j));
+}
+
+// Rays of the ray phases a job has still to plan (photon emission not counted).
+uint64_t remaining_rays(const Job* j) {
+    uint64_t lumels = 0, total = 0;
+    for (uint32_t v : j->valid) lumels += v;
+    uint64_t done_now = 0;  // lumels of the current phase already planned
+    for (size_t k = 0; k < j->map_i && k < j->valid.size(); k++) done_now += j->valid[k];
+    done_now += j->cursor;
+    for (size_t p = j->pi; p < j->plan.size(); p++)
+        total += (p == j->pi ? lumels - std::min(lumels, done_now) : lumels) * phase_rpl(j, j->plan[p]);
+    return total;
 }
 
 // Takes up to `budget` lumels of the job's current phase into `items`; returns lumels taken.
@@ -2466,53 +2515,79 @@ uint64_t take_lumels(Job* j, uint64_t budget, const BakeItem& proto, std::vector
 
 }  // namespace
 
-uint64_t rl_server::step(std::vector<Job*>& js) {
-    uint64_t rays = 0;
-    // Jobs with nothing left to do, then the whole-step phase: photon emission.
+// Plans the next step of the jobs `js` and queues its launches on the stream without waiting for
+// them: photon emissions first (synchronous calls), then the ray phases. Job cursors and plans
+// advance here, at planning time; the stream runs a job's phases in plan order. Returns false when
+// nothing was queued.
+bool rl_server::queue_step(std::vector<Job*>& js, StepSlot& sl) {
+    sl.jobs.clear();
+    sl.extra_rays = 0;
     for (Job* j : js) {
-        if (job_finished(j)) continue;
+        if (job_finished(j) || j->planned) continue;
         if (current_step(j) < 0) {
-            j->state = RL_JOB_DONE;
+            j->planned = true;
             continue;
         }
         if (current_step(j) != kStepEmit) continue;
         const int32_t st = rl_emit_photons(ctx, j->desc.scene, &j->ind);
         j->rays += ctx->stats.rays;
-        rays += ctx->stats.rays;
+        sl.extra_rays += ctx->stats.rays;
         if (st) fail_job(j, st);
         else advance(j);
     }
-    // Ray phases: upload lumels (and photon tables) on entry, then share the quantum fairly.
-    size_t nray = 0;
-    for (Job* j : js) {
-        const RayPhase ph = ray_phase(j);
-        if (ph == kPhaseNone) continue;
+    // Ray phases. A step bakes one scene, that of the oldest job with ray work, in one launch per
+    // phase: the launch holds the scene's top nodes in LDS and reads scene data wave-uniformly (a
+    // launch over several scenes loses both; launches per scene within a step measured slower
+    // still, being smaller). The step's jobs are the scene's jobs older than the oldest job of any
+    // other scene, so large jobs keep admission order across scenes, plus the scene's small jobs
+    // (whole remaining plan within a floor share), which are batched alongside. Each gets the floor
+    // (a quarter of an even share of the quantum) and the rest goes to them in admission order.
+    // A step queues its direct, AO and gather launches in that order, whatever the jobs' plans,
+    // hence one phase per job per step. Lumels (and photon tables) are uploaded on entry to a phase.
+    std::vector<Job*> cand;
+    for (Job* j : js)
+        if (!j->planned && !job_finished(j) && ray_phase(j) != kPhaseNone) cand.push_back(j);
+    if (cand.empty()) return false;
+    const rl_scene* scene = cand[0]->desc.scene;
+    const uint64_t floor_rays = std::max<uint64_t>(cfg.quantum_rays / (4 * cand.size()), 4096);
+    std::vector<Job*> mine;
+    bool prefix = true;
+    for (Job* j : cand) {
+        if (j->desc.scene != scene) {
+            prefix = false;
+            continue;
+        }
+        if (prefix || remaining_rays(j) <= floor_rays) mine.push_back(j);
+    }
+    for (size_t k = 0; k < mine.size();) {
+        Job* j = mine[k];
         int32_t st = RL_SUCCESS;
         if (!j->prepared) {
-            for (size_t k = 0; k < j->maps.size() && !st; k++) {
-                st = push_lumels(j->maps[k]);
-                if (!st && j->maps[k]->texels.size() != j->valid[k])
+            for (size_t m = 0; m < j->maps.size() && !st; m++) {
+                st = push_lumels(j->maps[m]);
+                if (!st && j->maps[m]->texels.size() != j->valid[m])
                     st = fail(RL_INVALID_CALL, "lightmap changed after submit");
             }
             j->prepared = true;
         }
-        if (!st && ph == kPhaseGather && !j->tables) {
+        if (!st && ray_phase(j) == kPhaseGather && !j->tables) {
             j->tables.reset(new PhotonTables());
             st = make_photon_tables(j->desc.scene, *j->tables);
         }
-        if (st) fail_job(j, st);
-        else nray++;
+        if (st) {
+            fail_job(j, st);
+            mine.erase(mine.begin() + k);
+        } else {
+            k++;
+        }
     }
-    if (nray == 0) return rays;
-    // The step's quantum: every job first gets a floor of a quarter of an even share, so a small
-    // job that arrives behind large ones progresses at once; the rest goes to the jobs in
-    // admission order, so the oldest finish first (latency close to one job at a time, with the
-    // small jobs batched alongside).
+    if (mine.empty()) return false;
     std::vector<BakeItem> items[4];
-    std::vector<Job*> in_step;
+    std::vector<std::pair<Job*, RayPhase>> phase_of;
+    for (Job* j : mine) phase_of.emplace_back(j, ray_phase(j));
     uint64_t left = cfg.quantum_rays;
-    auto take = [&](Job* j, uint64_t rays_wanted) {
-        const RayPhase ph = ray_phase(j);
+    auto take = [&](Job* j, RayPhase ph, uint64_t rays_wanted) {
+        if (j->planned || ray_phase(j) != ph) return;  // its phase was completed earlier in this step
         BakeItem proto{};
         uint64_t rpl = 1;
         if (ph == kPhaseDirect) {
@@ -2529,33 +2604,66 @@ uint64_t rl_server::step(std::vector<Job*>& js) {
         const uint64_t got = take_lumels(j, std::max<uint64_t>(rays_wanted / rpl, 64), proto, items[ph]) * rpl;
         j->rays += got;
         left -= std::min(left, got);
-        if (got && std::find(in_step.begin(), in_step.end(), j) == in_step.end()) in_step.push_back(j);
+        if (got && std::find(sl.jobs.begin(), sl.jobs.end(), j) == sl.jobs.end()) sl.jobs.push_back(j);
+        if (j->map_i >= j->maps.size()) advance(j);
     };
-    const uint64_t floor_rays = std::max<uint64_t>(cfg.quantum_rays / (4 * nray), 4096);
-    for (Job* j : js)
-        if (ray_phase(j) != kPhaseNone) take(j, floor_rays);
-    for (Job* j : js) {
+    for (const auto& jp : phase_of) take(jp.first, jp.second, floor_rays);
+    for (const auto& jp : phase_of) {
         if (left == 0) break;
-        if (ray_phase(j) != kPhaseNone && j->map_i < j->maps.size()) take(j, left);
+        // A phase that would leave less than half a quantum for a later step is taken whole: a
+        // launch's tail (lanes that finish early while the last lumels run) costs every launch.
+        const uint64_t rest = phase_rays_left(jp.first);
+        take(jp.first, jp.second, rest <= left + left / 2 ? rest : left);
     }
-    for (Job* j : js)  // jobs whose phase had no lumels left still advance below
-        if (ray_phase(j) != kPhaseNone && std::find(in_step.begin(), in_step.end(), j) == in_step.end())
-            in_step.push_back(j);
-    CallTimer tm(ctx);
-    int32_t st = begin_call(ctx);
-    // The phases are queued back to back (their segment tables use separate slots); end_call
-    // synchronises once.
+    if (sl.jobs.empty()) return false;
+    // The phases are queued back to back, counting rays into the step's own counter.
+    ctx->pool_used = 0;
+    ctx->marks.clear();
+    unsigned long long* const own_rays = ctx->d_rays;
+    ctx->d_rays = sl.d_rays;
+    int32_t st = RL_SUCCESS;
+    hipError_t e = launch_fill_u32(ctx->stream, reinterpret_cast<uint32_t*>(sl.d_rays), 2, 0u);
+    if (e != hipSuccess) st = fail(RL_DEVICE_ERROR, std::string("server step: ") + hipGetErrorString(e));
     if (!st && !items[kPhaseDirect].empty()) st = run_direct_items(ctx, items[kPhaseDirect], false);
     if (!st && !items[kPhaseAo].empty()) st = run_ao_items(ctx, items[kPhaseAo], false);
     if (!st && !items[kPhaseGather].empty()) st = run_fg_items(ctx, items[kPhaseGather], false);
-    if (!st) st = end_call(ctx, tm);
-    rays += ctx->stats.rays;
-    for (Job* j : in_step) {
-        if (st) fail_job(j, st);
-        else if (j->map_i >= j->maps.size()) advance(j);
-        if (job_finished(j)) j->tables.reset();
+    ctx->d_rays = own_rays;
+    if (!st) {
+        e = hipMemcpyAsync(sl.h_rays, sl.d_rays, sizeof(unsigned long long), hipMemcpyDeviceToHost, ctx->stream);
+        if (e == hipSuccess) e = hipEventRecord(sl.done, ctx->stream);
+        if (e != hipSuccess) st = fail(RL_DEVICE_ERROR, std::string("server step: ") + hipGetErrorString(e));
     }
-    return rays;
+    if (st) {
+        // Launches queued before the failure may still run: wait for them before the jobs retire.
+        hipStreamSynchronize(ctx->stream);
+        for (Job* j : sl.jobs) fail_job(j, st);
+        sl.jobs.clear();
+        return false;
+    }
+    for (Job* j : sl.jobs) j->inflight++;
+    sl.t_queued = now_ms();
+    sl.live = true;
+    return true;
+}
+
+// Waits for a queued step; its jobs that no other queued step bakes retire at once.
+void rl_server::finish_step(StepSlot& sl) {
+    const hipError_t e = hipEventSynchronize(sl.done);
+    int32_t st = RL_SUCCESS;
+    if (e != hipSuccess) st = fail(RL_DEVICE_ERROR, std::string("server step: ") + hipGetErrorString(e));
+    const double t = now_ms();
+    std::lock_guard<std::mutex> lk(m);
+    for (Job* j : sl.jobs) {
+        j->inflight--;
+        if (st && !job_finished(j)) fail_job(j, st);
+    }
+    retire_locked();
+    stats.rays += (st ? 0 : *sl.h_rays) + sl.extra_rays;
+    stats.busy_ms += t - std::max(sl.t_queued, last_finish_ms);
+    stats.steps++;
+    last_finish_ms = t;
+    sl.jobs.clear();
+    sl.live = false;
 }
 
 void rl_server::admit_locked() {
@@ -2581,15 +2689,20 @@ void rl_server::admit_locked() {
     }
 }
 
+// Ends jobs that no queued step bakes any more: planned jobs become done, cancelled ones
+// cancelled; finished jobs release their memory and their hold on the caller's objects.
 void rl_server::retire_locked() {
     bool any = false;
     for (size_t k = 0; k < active.size();) {
         Job* j = active[k];
         const bool mine = j->desc.kind != RL_JOB_RADIOSITY;  // the auxiliary worker settles its own
-        if (mine && j->cancel && !job_finished(j)) j->state = RL_JOB_CANCELLED;
-        if (job_finished(j)) {
+        if (mine && j->inflight == 0 && !job_finished(j)) {
+            if (j->cancel) j->state = RL_JOB_CANCELLED;
+            else if (j->planned) j->state = RL_JOB_DONE;
+        }
+        if (job_finished(j) && j->inflight == 0) {
             j->done_ms = now_ms();
-            j->tables.reset();
+            if (j->tables) retired_tables.push_back(std::move(j->tables));
             j->hold(false);
             mem_used -= j->footprint;
             if (j->state == RL_JOB_DONE) stats.completed++;
@@ -2607,27 +2720,39 @@ void rl_server::retire_locked() {
     }
 }
 
-// Ray work of every admitted job (direct, AO, final gather, photon emission) in shared steps.
+// Ray work of every admitted job (direct, AO, final gather, photon emission) in shared steps, one
+// step queued ahead of the one running.
 void rl_server::loop() {
     std::unique_lock<std::mutex> lk(m);
+    int cur = 0;
     for (;;) {
         retire_locked();
         admit_locked();
         std::vector<Job*> js;
         for (Job* j : active)
-            if (j->desc.kind != RL_JOB_RADIOSITY && !job_finished(j)) js.push_back(j);
-        if (js.empty()) {
+            if (j->desc.kind != RL_JOB_RADIOSITY && !job_finished(j) && !j->planned && !j->cancel) js.push_back(j);
+        StepSlot& prev = slots[1 - cur];
+        if (js.empty() && !prev.live) {
+            if (!retired_tables.empty() && !slots[cur].live) {
+                lk.unlock();
+                retired_tables.clear();  // nothing is queued: the frees do not stall a step
+                lk.lock();
+                continue;
+            }
             if (stop && active.empty() && queue.empty()) break;
             cv_work.wait(lk);
             continue;
         }
+        if (retired_tables.size() > 64) {  // bounded: free them even with a step in flight
+            lk.unlock();
+            retired_tables.clear();
+            lk.lock();
+        }
         lk.unlock();
-        const double t = now_ms();
-        const uint64_t rays = step(js);
+        const bool queued = !js.empty() && queue_step(js, slots[cur]);
+        if (prev.live) finish_step(prev);
         lk.lock();
-        stats.rays += rays;
-        stats.busy_ms += now_ms() - t;
-        stats.steps++;
+        if (queued) cur = 1 - cur;
     }
     cv_work.notify_all();
 }
@@ -2756,6 +2881,16 @@ int32_t rl_bake_scene(rl_relight* ctx, rl_scene* scene, int32_t direct, const rl
     return st;
 }
 
+static void free_slots(rl_server* s) {
+    for (rl_server::StepSlot& sl : s->slots) {
+        if (sl.done) hipEventDestroy(sl.done);
+        sl.done = nullptr;
+        dev_free(sl.d_rays);
+        if (sl.h_rays) hipHostFree(sl.h_rays);
+        sl.h_rays = nullptr;
+    }
+}
+
 int32_t rl_server_create(rl_relight* ctx, const rl_server_config* cfg, rl_server** out) {
     if (!ctx || !cfg || !out) return fail(RL_INVALID_ARGUMENT, "NULL argument");
     if (cfg->max_active <= 0 || cfg->quantum_rays == 0 || cfg->memory_budget == 0)
@@ -2771,6 +2906,19 @@ int32_t rl_server_create(rl_relight* ctx, const rl_server_config* cfg, rl_server
     }
     s->aux->tuning = ctx->tuning;
     s->aux->max_blocks = ctx->max_blocks;
+    hipError_t e = hipSuccess;
+    for (rl_server::StepSlot& sl : s->slots) {
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&sl.done, hipEventDisableTiming);
+        if (e == hipSuccess) e = dev_alloc(&sl.d_rays, 1);
+        if (e == hipSuccess)
+            e = hipHostMalloc(reinterpret_cast<void**>(&sl.h_rays), sizeof(unsigned long long), hipHostMallocDefault);
+    }
+    if (e != hipSuccess) {
+        free_slots(s);
+        rl_destroy(s->aux);
+        delete s;
+        return fail(RL_DEVICE_ERROR, std::string("server set-up: ") + hipGetErrorString(e));
+    }
     s->worker = std::thread([s]() { s->loop(); });
     s->aux_worker = std::thread([s]() { s->aux_loop(); });
     *out = s;
@@ -2913,6 +3061,8 @@ int32_t rl_server_destroy(rl_server* s) {
         s->stats.cancelled++;
     }
     for (Job* j : s->active) j->hold(false);
+    s->retired_tables.clear();
+    free_slots(s);
     rl_destroy(s->aux);
     delete s;
     return RL_SUCCESS;
