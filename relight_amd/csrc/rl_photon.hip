@@ -287,7 +287,6 @@ __global__ __launch_bounds__(kFgRefillBlock) void k_fg_refill(const BakeSeg* __r
     // Lane state.
     bool active = false;
     TravRay tr;
-    WtPerm wp;
     SceneDev S;
     TopCache top{top_rows, 0u};
     TraceResult res;
@@ -334,7 +333,6 @@ __global__ __launch_bounds__(kFgRefillBlock) void k_fg_refill(const BakeSeg* __r
                     influence = d0 > 0.0f ? d0 : 0.0f;
                     const Ray ray = ray_from_segment(p, p + dir * P.distance);
                     tr = trav_ray(ray);
-                    wp = wt_perm(tr.w);
                     S = sg.scene;
                     top.count = S.nodes == top_scene.nodes ? top_count : 0u;
                     node = 0;
@@ -350,9 +348,9 @@ __global__ __launch_bounds__(kFgRefillBlock) void k_fg_refill(const BakeSeg* __r
                 want &= ~__ballot(mine);
             }
         }
-        if (active && (one_scene(S) ? closest_run<CAP, true>(S, top, tr, wp, node, sp, stk, kFgRefillBlock, spill, res,
+        if (active && (one_scene(S) ? closest_run<CAP, true>(S, top, tr, node, sp, stk, kFgRefillBlock, spill, res,
                                                              pend, kFgNodeExit, kFgLeafExit)
-                                    : closest_run<CAP, false>(S, top, tr, wp, node, sp, stk, kFgRefillBlock, spill,
+                                    : closest_run<CAP, false>(S, top, tr, node, sp, stk, kFgRefillBlock, spill,
                                                               res, pend, kFgNodeExit, kFgLeafExit)) != 0) {
             rays++;
             const BakeSeg& sg = segs[my_seg];
@@ -448,7 +446,6 @@ __global__ __launch_bounds__(kFgBlock) void k_fg_lanes(const BakeSeg* __restrict
     float fg_distance = 0.0f;
     int32_t fg_sampling = 0;
     TravRay tr;
-    WtPerm wp;
     SceneDev S;
     TopCache top{top_rows, 0u};
     TraceResult res;
@@ -495,7 +492,6 @@ __global__ __launch_bounds__(kFgBlock) void k_fg_lanes(const BakeSeg* __restrict
                 influence = d0 > 0.0f ? d0 : 0.0f;
                 const Ray ray = ray_from_segment(lp, lp + dir * fg_distance);
                 tr = trav_ray(ray);
-                wp = wt_perm(tr.w);
                 node = 0;
                 pend = 0;
                 sp = 0;
@@ -505,9 +501,9 @@ __global__ __launch_bounds__(kFgBlock) void k_fg_lanes(const BakeSeg* __restrict
                 ray_on = true;
             }
         }
-        if (ray_on && (one_scene(S) ? closest_run<CAP, true>(S, top, tr, wp, node, sp, stk, kFgBlock, spill, res, pend,
+        if (ray_on && (one_scene(S) ? closest_run<CAP, true>(S, top, tr, node, sp, stk, kFgBlock, spill, res, pend,
                                                              kFgNodeExit, kFgLeafExit)
-                                    : closest_run<CAP, false>(S, top, tr, wp, node, sp, stk, kFgBlock, spill, res, pend,
+                                    : closest_run<CAP, false>(S, top, tr, node, sp, stk, kFgBlock, spill, res, pend,
                                                               kFgNodeExit, kFgLeafExit)) != 0) {
             rays++;
             ray_on = false;

@@ -58,38 +58,6 @@ __device__ __forceinline__ bool one_scene(const SceneDev& S) {
     return __ballot(reinterpret_cast<uint64_t>(S.nodes) != lead) == 0ull;
 }
 
-// The watertight test's axis permutation applied once per ray: byte offsets of the kx, ky and kz
-// components within a vertex, and the origin's components in that order. A triangle test then
-// loads the three components it needs of each vertex directly instead of selecting them.
-struct WtPerm {
-    uint32_t okx, oky, okz;
-    float px, py, pz;
-};
-
-__device__ __forceinline__ WtPerm wt_perm(const WtRay& w) {
-    WtPerm p;
-    p.okx = 4u * (uint32_t)w.kx;
-    p.oky = 4u * (uint32_t)w.ky;
-    p.okz = 4u * (uint32_t)w.kz;
-    p.px = comp(w.o, w.kx);
-    p.py = comp(w.o, w.ky);
-    p.pz = comp(w.o, w.kz);
-    return p;
-}
-
-// wt_intersect of the triangle stored as three float4 rows (vertex xyz, then w) at `tp`: the same
-// operations on the same values, with each component read at its permuted offset.
-__device__ __forceinline__ bool wt_intersect_at(const WtRay& r, const WtPerm& p, const gfloat4* tp, float& t, float& U,
-                                                float& V, float& W, float& det) {
-    typedef const char __attribute__((address_space(1))) gchar;
-    gchar* b = reinterpret_cast<gchar*>(tp);
-    const gfloat* fx = reinterpret_cast<const gfloat*>(b + p.okx);
-    const gfloat* fy = reinterpret_cast<const gfloat*>(b + p.oky);
-    const gfloat* fz = reinterpret_cast<const gfloat*>(b + p.okz);
-    return wt_finish(r, fx[0] - p.px, fy[0] - p.py, fz[0] - p.pz, fx[4] - p.px, fy[4] - p.py, fz[4] - p.pz,
-                     fx[8] - p.px, fy[8] - p.py, fz[8] - p.pz, t, U, V, W, det);
-}
-
 __device__ __forceinline__ float bits_f(uint32_t u) { return __uint_as_float(u); }
 __device__ __forceinline__ uint32_t f_bits(float f) { return __float_as_uint(f); }
 
@@ -343,8 +311,8 @@ __device__ __forceinline__ int any_hit_run(const SceneDev& S0, const TopCache& t
     uint32_t cur = node & RL_LEAF_FIRST_MASK;
     uint32_t end = cur + ((node >> 27) & 15u) + 1u;
     for (;;) {
-        // Three row loads and per-lane component selects: measured faster here than the nine
-        // component loads of wt_intersect_at (this loop is bound by vector-memory data return).
+        // Three 16-B row loads and per-lane component selects (nine component loads at per-ray
+        // offsets measured slower: this loop is bound by vector-memory data return).
         const gfloat* tf = reinterpret_cast<const gfloat*>(gmem(S.tris) + 3u * cur);
         float t, U, V, W, det;
         if (wt_intersect(r.w, v3(tf[0], tf[1], tf[2]), v3(tf[4], tf[5], tf[6]), v3(tf[8], tf[9], tf[10]), t, U, V, W,
@@ -376,7 +344,7 @@ __device__ __forceinline__ int any_hit_run(const SceneDev& S0, const TopCache& t
 // only let a caller give finished lanes new rays.
 // UNI: the caller has checked one_scene(S0).
 template <int CAP, bool UNI>
-__device__ __forceinline__ int closest_run(const SceneDev& S0, const TopCache& top, const TravRay& r, const WtPerm& p,
+__device__ __forceinline__ int closest_run(const SceneDev& S0, const TopCache& top, const TravRay& r,
                                            uint32_t& node, int& sp, uint32_t* stk, uint32_t stride, uint32_t* spill,
                                            TraceResult& res, uint32_t& pend, uint32_t min_lanes, uint32_t leaf_lanes) {
     const SceneDev S = UNI ? uniform_scene(S0) : S0;
@@ -405,10 +373,14 @@ __device__ __forceinline__ int closest_run(const SceneDev& S0, const TopCache& t
     for (uint32_t stop_at = 0;; stop_at = leaf_lanes) {
         if ((uint32_t)__popcll(__ballot(leafing)) <= stop_at) break;
         if (leafing) {
-            const gfloat4* tp = gmem(S.tris) + 3u * cur;
+            // Three 16-B row loads and per-lane component selects: this loop waits on the loads'
+            // return, and three loads beat nine component loads at per-ray offsets (heightfield
+            // final gather 283 -> 261 ms).
+            const gfloat* tf = reinterpret_cast<const gfloat*>(gmem(S.tris) + 3u * cur);
             float t, U, V, W, det;
-            if (wt_intersect_at(r.w, p, tp, t, U, V, W, det)) {
-                const uint32_t id = f_bits(reinterpret_cast<const gfloat*>(tp)[3]);
+            if (wt_intersect(r.w, v3(tf[0], tf[1], tf[2]), v3(tf[4], tf[5], tf[6]), v3(tf[8], tf[9], tf[10]), t, U, V, W,
+                             det)) {
+                const uint32_t id = f_bits(tf[3]);
                 if (t < res.t || (t == res.t && id < res.tri)) {
                     res.t = t;
                     res.tri = id;
@@ -447,7 +419,6 @@ template <bool ANY, int CAP>
 __device__ bool traverse(const SceneDev& S, const TopCache& top, const Ray& ray, uint32_t* stk, uint32_t stride,
                          TraceResult& res) {
     const WtRay w = wt_setup(ray);
-    const WtPerm wp = wt_perm(w);
     const SlabRay sr = slab_ray(ray);
     const float tnear = ray.tnear;
     float tlimit = ray.tfar;  // closest: shrinks to the best hit so far
@@ -470,10 +441,11 @@ __device__ bool traverse(const SceneDev& S, const TopCache& top, const Ray& ray,
         uint32_t cur = node & RL_LEAF_FIRST_MASK;
         uint32_t end = cur + ((node >> 27) & 15u) + 1u;
         for (;;) {
-            const gfloat4* tp = gmem(S.tris) + 3u * cur;
+            const gfloat* tf = reinterpret_cast<const gfloat*>(gmem(S.tris) + 3u * cur);
             float t, U, V, W, det;
-            if (wt_intersect_at(w, wp, tp, t, U, V, W, det)) {
-                const uint32_t id = f_bits(reinterpret_cast<const gfloat*>(tp)[3]);
+            if (wt_intersect(w, v3(tf[0], tf[1], tf[2]), v3(tf[4], tf[5], tf[6]), v3(tf[8], tf[9], tf[10]), t, U, V, W,
+                             det)) {
+                const uint32_t id = f_bits(tf[3]);
                 if (ANY) {
                     res.t = t;
                     res.tri = id;
