@@ -287,7 +287,7 @@ __device__ __forceinline__ int any_hit_step(const SceneDev& S, const TopCache& t
 
 // A longer step of any-hit traversal in two phases: internal nodes until this lane holds a leaf
 // (a lane that reaches one first waits for the rest of its wavefront, so the triangle tests that
-// follow run with the lanes together), then one triangle per iteration through that leaf and
+// follow run with the lanes together), then two triangles per iteration through that leaf and
 // through any leaves popped right after it. After its first node, the node phase ends early for
 // the whole wavefront once no more than `min_lanes` of the calling lanes are still searching;
 // those lanes return 0 and continue on the next call (every call visits at least one node of
@@ -311,13 +311,20 @@ __device__ __forceinline__ int any_hit_run(const SceneDev& S0, const TopCache& t
     uint32_t cur = node & RL_LEAF_FIRST_MASK;
     uint32_t end = cur + ((node >> 27) & 15u) + 1u;
     for (;;) {
-        // Three 16-B row loads and per-lane component selects (nine component loads at per-ray
-        // offsets measured slower: this loop is bound by vector-memory data return).
-        const gfloat* tf = reinterpret_cast<const gfloat*>(gmem(S.tris) + 3u * cur);
+        // Two triangles per iteration, each as three 16-B row loads with per-lane component
+        // selects, both triangles' loads issued before either test (the last triangle of a leaf is
+        // tested twice when one is left): this loop is bound by vector-memory data return (city AO
+        // 346.5 -> 341.6 ms against one triangle per iteration; nine per-component loads of one
+        // triangle measured slower still).
+        const uint32_t c2 = cur + 1u < end ? cur + 1u : cur;
+        const gfloat* ta = reinterpret_cast<const gfloat*>(gmem(S.tris) + 3u * cur);
+        const gfloat* tb = reinterpret_cast<const gfloat*>(gmem(S.tris) + 3u * c2);
+        const V3 a0 = v3(ta[0], ta[1], ta[2]), a1 = v3(ta[4], ta[5], ta[6]), a2 = v3(ta[8], ta[9], ta[10]);
+        const V3 b0 = v3(tb[0], tb[1], tb[2]), b1 = v3(tb[4], tb[5], tb[6]), b2 = v3(tb[8], tb[9], tb[10]);
         float t, U, V, W, det;
-        if (wt_intersect(r.w, v3(tf[0], tf[1], tf[2]), v3(tf[4], tf[5], tf[6]), v3(tf[8], tf[9], tf[10]), t, U, V, W,
-                         det))
-            return 1;
+        if (wt_intersect(r.w, a0, a1, a2, t, U, V, W, det)) return 1;
+        if (wt_intersect(r.w, b0, b1, b2, t, U, V, W, det)) return 1;
+        cur = c2;
         if (++cur == end) {
             if (sp == 0) return -1;
             node = stack_pop(sp, stk, stride, spill);
@@ -334,7 +341,7 @@ __device__ __forceinline__ int any_hit_run(const SceneDev& S0, const TopCache& t
 // or runs out of nodes; the phase ends when every lane holds a leaf or has finished (Aila and
 // Laine 2009, speculative while-while), so lanes do not idle while others search, or, after its
 // first node, once no more than `min_lanes` lanes are still searching without a leaf (those return
-// 0 and continue on the next call). The leaf phase tests the held leaf's triangles, one per
+// 0 and continue on the next call). The leaf phase tests the held leaf's triangles, two per
 // iteration, then any leaf the lane stopped at and the leaves popped right after it. Returns 0
 // when the round ends at an internal node (call again), -1 when traversal is finished: res then
 // holds the closest hit (tri = 0xFFFFFFFF for a miss). Start with node = 0, sp = 0, pend = 0 and
@@ -366,27 +373,36 @@ __device__ __forceinline__ int closest_run(const SceneDev& S0, const TopCache& t
     if (!has) return done ? -1 : 0;  // finished without another leaf, or still searching
     uint32_t cur = held & RL_LEAF_FIRST_MASK;
     uint32_t end = cur + ((held >> 27) & 15u) + 1u;
-    // The leaf phase, one triangle per iteration; after its first, it ends for the wavefront once
+    // The leaf phase, two triangles per iteration (the last one of a leaf tested twice when one is
+    // left: the tie rule makes the repeat a no-op); after its first, it ends for the wavefront once
     // no more than `leaf_lanes` lanes still have triangles, and those keep the rest of their leaf
     // for the next round (in `pend`, or as the next node when nothing follows it).
     bool leafing = true;
     for (uint32_t stop_at = 0;; stop_at = leaf_lanes) {
         if ((uint32_t)__popcll(__ballot(leafing)) <= stop_at) break;
         if (leafing) {
-            // Three 16-B row loads and per-lane component selects: this loop waits on the loads'
-            // return, and three loads beat nine component loads at per-ray offsets (heightfield
-            // final gather 283 -> 261 ms).
-            const gfloat* tf = reinterpret_cast<const gfloat*>(gmem(S.tris) + 3u * cur);
+            // Each triangle as three 16-B row loads with per-lane component selects, both
+            // triangles' loads issued before either test: this loop waits on the loads' return
+            // (heightfield final gather: nine per-component loads of one triangle 283 ms, three
+            // row loads 261 ms, two triangles per iteration 252 ms, three 265 ms, four 287 ms).
+            const uint32_t c2 = cur + 1u < end ? cur + 1u : cur;
+            const gfloat* ta = reinterpret_cast<const gfloat*>(gmem(S.tris) + 3u * cur);
+            const gfloat* tb = reinterpret_cast<const gfloat*>(gmem(S.tris) + 3u * c2);
+            const V3 a0 = v3(ta[0], ta[1], ta[2]), a1 = v3(ta[4], ta[5], ta[6]), a2 = v3(ta[8], ta[9], ta[10]);
+            const V3 b0 = v3(tb[0], tb[1], tb[2]), b1 = v3(tb[4], tb[5], tb[6]), b2 = v3(tb[8], tb[9], tb[10]);
+            const uint32_t ida = f_bits(ta[3]), idb = f_bits(tb[3]);
             float t, U, V, W, det;
-            if (wt_intersect(r.w, v3(tf[0], tf[1], tf[2]), v3(tf[4], tf[5], tf[6]), v3(tf[8], tf[9], tf[10]), t, U, V, W,
-                             det)) {
-                const uint32_t id = f_bits(tf[3]);
-                if (t < res.t || (t == res.t && id < res.tri)) {
-                    res.t = t;
-                    res.tri = id;
-                    res.U = U; res.V = V; res.W = W; res.det = det;
-                }
+            if (wt_intersect(r.w, a0, a1, a2, t, U, V, W, det) && (t < res.t || (t == res.t && ida < res.tri))) {
+                res.t = t;
+                res.tri = ida;
+                res.U = U; res.V = V; res.W = W; res.det = det;
             }
+            if (wt_intersect(r.w, b0, b1, b2, t, U, V, W, det) && (t < res.t || (t == res.t && idb < res.tri))) {
+                res.t = t;
+                res.tri = idb;
+                res.U = U; res.V = V; res.W = W; res.det = det;
+            }
+            cur = c2;
             if (++cur == end) {
                 if (done || !(node & RL_LEAF_BIT)) {
                     leafing = false;
