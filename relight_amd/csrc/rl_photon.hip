@@ -418,7 +418,10 @@ __global__ __launch_bounds__(kFgRefillBlock) void k_fg_refill(const BakeSeg* __r
 // done wait until at least `refill_min` lanes of the wavefront are without a ray, then start their
 // next samples together (lanes whose lumel is done first take new lumels), so the ray set-up runs
 // for many lanes at once. LDS per workgroup: 64 KB stacks + 88 KB of the scene's first 700 nodes.
-constexpr uint64_t kFgLanesMinPerLane = 2;  // lumels per resident lane for the lane-per-lumel kernel
+// Lumels per resident lane from which the lane-per-lumel kernel runs. Heightfield maps at 64 samples
+// (wave / lane kernel): 65 K lumels 1.14 / 3.13 ms, 147 K 2.29 / 3.22 ms, 262 K 3.71 / 3.25 ms, 590 K
+// 7.91 / 6.72 ms, 1 M 13.87 / 10.99 ms; at 256 samples, 262 K: 12.30 / 12.57 ms.
+constexpr uint64_t kFgLanesMinPerLane = 1;
 
 template <int CAP>
 __global__ __launch_bounds__(kFgBlock) void k_fg_lanes(const BakeSeg* __restrict__ segs, int nsegs, uint64_t total,
