@@ -1,7 +1,8 @@
 """Radiosity: form factors by hemisphere rays and the BF16-MFMA gather solve.
 
 CPU: the oracle's form-factor estimator against closed forms (closed enclosure row sums; Howell
-catalogue factors for parallel and perpendicular unit plates).
+catalogue factors for parallel and perpendicular unit plates) and reciprocity (A_i F_ij = A_j F_ji
+for unequal plates).
 GPU: form-factor counts bit-exact against the oracle; the solve against an FP64 solve of the same
 system (numpy), with the FP64 residual reported.
 """
@@ -62,6 +63,24 @@ def test_form_factor_perpendicular_plates_closed_form():
     F = S.form_factors(P, tp, rays, 5, 100.0).astype(np.float64) / rays
     sd = math.sqrt(0.2 * 0.8 / rays)
     assert abs(F[0, 1] - 0.20004) < 5 * sd and abs(F[1, 0] - 0.20004) < 5 * sd
+
+
+def test_form_factor_reciprocity_unequal_plates():
+    # A_i F_ij = A_j F_ji: a unit square under a parallel 1 x 2 rectangle 0.7 m above (offset
+    # towards -x) and a 1 x 0.7 wall standing on the square's +x edge; every patch sees only the
+    # others' front faces. Each estimate is a binomial fraction of the rays from one patch, so the
+    # check is within five standard deviations of the difference.
+    S, P, tp = _plates([((0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1)),
+                        ((-1.5, 0.3, 0.7), (0, 1, 0), (2, 0, 0), (0, 0, -1)),
+                        ((1, 0, 0), (0, 1, 0), (0, 0, 0.7), (-1, 0, 0))])
+    areas = np.array([1.0, 2.0, 0.7])
+    rays = 80000
+    F = S.form_factors(P, tp, rays, 11, 100.0).astype(np.float64) / rays
+    for i, j in [(0, 1), (0, 2), (1, 2)]:
+        assert F[i, j] > 0.02 and F[j, i] > 0.02
+        lhs, rhs = areas[i] * F[i, j], areas[j] * F[j, i]
+        sd = math.sqrt(areas[i] ** 2 * F[i, j] * (1 - F[i, j]) / rays + areas[j] ** 2 * F[j, i] * (1 - F[j, i]) / rays)
+        assert abs(lhs - rhs) < 5 * sd, (i, j, lhs, rhs, sd)
 
 
 def test_form_factor_closed_enclosure_rows_sum_to_one():
