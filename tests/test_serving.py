@@ -252,3 +252,35 @@ def test_server_jobs_keep_their_objects_alive(relight):
     del job
     for lm in kept:
         lm.close()
+
+
+def test_server_interleaved_scenes_and_running_cancel(relight):
+    """Jobs of four scenes submitted alternately with a small quantum, so the server's steps (one
+    scene each, the next queued while one runs) switch scenes many times: every job equals its
+    plain bake bit for bit. A large job cancelled while its steps run ends cancelled, the other
+    jobs still finish, and its scene can be destroyed once the server is done."""
+    print("[gpu] server: interleaved scenes, pipelined steps, running cancel", flush=True)
+    specs = [scenes.box_room(seed=1), _small_city(seed=3, ao_samples=16), scenes.box_room(seed=2),
+             _small_city(seed=4, ao_samples=16)]
+    plain = [_plain(relight, sp) for sp in specs]
+    built = [scenes.build_gpu(relight, sp) for sp in specs * 2]
+    big_spec = _small_city(seed=5, ao_samples=256)
+    big_scene, big_lms, _ = scenes.build_gpu(relight, big_spec)
+    with relight.create_server(quantum_rays=1 << 17, max_active=16) as srv:
+        jobs = []
+        for k, (sc, lms, _) in enumerate(built):
+            jobs.append(srv.submit_direct_ao(sc, lms, True, specs[k % 4].ao))
+            if k == 1:
+                big = srv.submit_direct_ao(big_scene, big_lms, True, big_spec.ao)
+        for _ in range(2000):
+            if big.poll()["state"] != "queued":
+                break
+        big.cancel()
+        sts = [j.wait(timeout=300) for j in jobs]
+        assert all(s["state"] == "done" for s in sts)
+        assert big.wait(timeout=300)["state"] in ("cancelled", "done")
+        assert srv.stats()["steps"] > 8
+    for k, (sc, lms, _) in enumerate(built):
+        for want, lm in zip(plain[k % 4], lms):
+            assert np.array_equal(want, lm.to_hdr())
+    big_scene.close()
