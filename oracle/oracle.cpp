@@ -33,6 +33,13 @@
 // emitter's start plane, the final gather's hit branch (front face, back face, the
 // floor(uv * (size - 1)) lookup at the barycentric-swapped uv) against ray/plane predictions,
 // and the mesh transform against R p + t with renormalised normals.
+// tests/test_materials.py: Texture::colorAt known answers (truncating wrap, channel order, magenta
+// outside), the alpha re-trace (one-unit restart, 25-step limit, checker alpha texture) and
+// textured photon reflectance. tests/test_radiosity.py: also reciprocity A_i F_ij = A_j F_ji on
+// unequal plates and the solve against an FP64 LU solve. tests/test_uv.py: uv generation against
+// hand-derived layouts and invariants (uvs in [0, 1], constant uv/3-D area ratio per chart, no
+// texel centre in two charts). tests/test_postprocess.py and test_obj.py: expand, blur and the
+// encodings by hand; the OBJ parse on the reference's own data files.
 // Compiled with -ffp-contract=off so every expression rounds as written.
 #include <stdint.h>
 #include <math.h>
