@@ -520,17 +520,23 @@ int32_t rl_server_destroy(rl_server* server);
  * grids (128-thread blocks). Results do not depend on the tuning. */
 #define RL_AO_KERNEL_CHUNKED 0
 #define RL_AO_KERNEL_REFILL 1
-/* Final-gather kernel: AUTO gives each lane its own lumel when a pass has at least 2 lumels per
+/* Final-gather kernel: AUTO gives each lane its own lumel when a pass has at least one lumel per
    resident lane, else each lumel a wavefront; WAVE and LANE force one of the two. */
 #define RL_FG_KERNEL_AUTO 0
 #define RL_FG_KERNEL_WAVE 1
 #define RL_FG_KERNEL_LANE 2
+/* Direct-light kernel: AUTO traces small passes as (lumel, shadow ray) pairs summed per lumel
+   afterwards and larger ones with a lane per lumel; LUMEL and PAIRS force one of the two. */
+#define RL_DIRECT_KERNEL_AUTO 0
+#define RL_DIRECT_KERNEL_LUMEL 1
+#define RL_DIRECT_KERNEL_PAIRS 2
 typedef struct rl_tuning {
     int32_t ao_kernel;
     uint32_t refill_min;
     uint32_t ao_chunk_lumels;
     uint32_t blocks_per_cu;
     int32_t fg_kernel;
+    int32_t direct_kernel;
 } rl_tuning;
 int32_t rl_set_tuning(rl_relight* ctx, const rl_tuning* tuning);
 int32_t rl_get_tuning(const rl_relight* ctx, rl_tuning* out);

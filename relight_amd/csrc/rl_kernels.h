@@ -30,6 +30,12 @@ hipError_t launch_scatter_colors(hipStream_t st, const uint32_t* sel, uint32_t n
 hipError_t launch_bake_direct(hipStream_t st, int cap, uint32_t max_blocks, const BakeSeg* segs,
                               int nsegs, uint64_t total, bool alpha, unsigned long long* queue,
                               unsigned long long* rays);
+// Direct light of small passes over (lumel, shadow ray) pairs: influences into `inf` (one float per
+// pair, pairs counted from pair_segs' work_begin), then summed per lumel in the reference's light
+// order (lumel_segs: the same segments in lumel units). Results equal launch_bake_direct's.
+hipError_t launch_bake_direct_pairs(hipStream_t st, int cap, uint32_t max_blocks, const BakeSeg* pair_segs,
+                                   const BakeSeg* lumel_segs, int nsegs, uint64_t total_pairs, uint64_t total_lumels,
+                                   bool alpha, float* inf, unsigned long long* queue, unsigned long long* rays);
 
 hipError_t launch_ao_count(hipStream_t st, int cap, uint32_t max_blocks, const BakeSeg* segs,
                            int nsegs, uint64_t total_pairs, unsigned long long* queue,

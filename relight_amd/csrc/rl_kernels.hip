@@ -171,6 +171,118 @@ __global__ __launch_bounds__(kDirectBlock) void k_bake_direct(const BakeSeg* __r
     count_rays(rays, rays_out);
 }
 
+// DirectLight::bakeLumel over (lumel, shadow ray) pairs, for passes with too few lumels to fill
+// the machine with a lane per lumel (a lane then traces its lumel's rays one after another, so a
+// pass takes the time of the slowest lumel's rays in series). Pass 1 (k_direct_pairs) computes
+// each pair's influence, the same influence_from_point call k_bake_direct makes for that light
+// and vertex, into `inf`, pairs lumel-major with a lumel's rays in the reference's light and
+// vertex order; pass 2 (k_direct_sum) adds them per lumel in that order with k_bake_direct's
+// arithmetic, so the colour is the same bit for bit. Segments' work_begin counts pairs in pass 1
+// and lumels in pass 2 (two tables of the same segments in the same order).
+
+// Shadow rays per lumel of a segment: one per point, spot or directional light, one per vertex of
+// a vertex-generated (area) light; at least 1 (a pair without a light has influence 0).
+__device__ __forceinline__ uint32_t direct_rays(const BakeSeg& sg) {
+    uint32_t r = 0;
+    for (int l = 0; l < sg.nlights; l++)
+        r += sg.lights[l].num_vertices > 0 ? (uint32_t)sg.lights[l].num_vertices : 1u;
+    return r > 0u ? r : 1u;
+}
+
+template <int CAP, bool ALPHA>
+__global__ __launch_bounds__(kDirectBlock) void k_direct_pairs(const BakeSeg* __restrict__ segs, int nsegs,
+                                                               uint64_t total, float* __restrict__ inf_out,
+                                                               unsigned long long* queue,
+                                                               unsigned long long* rays_out) {
+    __shared__ uint32_t stack[kLdsStack * kDirectBlock];
+    __shared__ u32x4 top_rows[kDirectTopNodes * kNodeRows];
+    const SceneDev top_scene = segs[0].scene;
+    const uint32_t top_count = load_top_nodes(top_scene, top_rows, kDirectTopNodes);
+    __syncthreads();
+    uint32_t* stk = stack + threadIdx.x;
+    const uint64_t nchunks = (total + 63) / 64;
+    uint32_t rays = 0;
+    for (;;) {
+        const uint64_t chunk = wave_next_chunk(queue);
+        if (chunk >= nchunks) break;
+        const uint64_t g = chunk * 64 + lane_id();
+        if (g >= total) continue;
+        const BakeSeg& sg = segs[find_seg(segs, nsegs, g)];
+        const uint64_t local = g - sg.work_begin;
+        const uint32_t R = direct_rays(sg);
+        const uint32_t i = (uint32_t)(local / R), r = (uint32_t)(local % R);
+        // The light of ray r and, for an area light, its vertex.
+        int l = 0, v = -1;
+        for (uint32_t first = 0; l < sg.nlights; l++) {
+            const int nv = sg.lights[l].num_vertices;
+            const uint32_t k = nv > 0 ? (uint32_t)nv : 1u;
+            if (r < first + k) {
+                v = nv > 0 ? (int)(r - first) : -1;
+                break;
+            }
+            first += k;
+        }
+        float inf = 0.0f;
+        if (l < sg.nlights) {
+            const SceneDev S = sg.scene;
+            const TopCache top{top_rows, S.nodes == top_scene.nodes ? top_count : 0u};
+            const LightDev L = sg.lights[l];
+            const float4 p4 = sg.pos[i], n4 = sg.nrm[i];
+            const V3 p = v3(p4.x, p4.y, p4.z), n = v3(n4.x, n4.y, n4.z);
+            V3 lp = v3(L.pos[0], L.pos[1], L.pos[2]);
+            if (v >= 0) {
+                const float4 lv = sg.lverts[L.vertex_offset + v];
+                lp = v3(lv.x + L.pos[0], lv.y + L.pos[1], lv.z + L.pos[2]);
+            }
+            inf = influence_from_point<CAP, ALPHA>(S, top, L, lp, p, n, stk, rays);
+        }
+        inf_out[g] = inf;
+    }
+    count_rays(rays, rays_out);
+}
+
+__global__ __launch_bounds__(kBlock) void k_direct_sum(const BakeSeg* __restrict__ lumel_segs,
+                                                       const BakeSeg* __restrict__ pair_segs, int nsegs,
+                                                       uint64_t total, const float* __restrict__ inf) {
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t g = (uint64_t)blockIdx.x * kBlock + threadIdx.x; g < total; g += stride) {
+        const int s = find_seg(lumel_segs, nsegs, g);
+        const BakeSeg& sg = lumel_segs[s];
+        const uint32_t i = (uint32_t)(g - sg.work_begin);
+        const float* f = inf + pair_segs[s].work_begin + (uint64_t)i * direct_rays(sg);
+        float4 col = sg.color[i];
+        uint32_t r = 0;
+        for (int l = 0; l < sg.nlights; l++) {
+            const LightDev L = sg.lights[l];
+            float cr, cg, cb;
+            if (L.num_vertices > 0) {
+                cr = cg = cb = 0.0f;
+                for (int v = 0; v < L.num_vertices; v++) {
+                    const float x = f[r++];
+                    if (x > 0.0f) {
+                        cr += L.color[0] * L.intensity * x;
+                        cg += L.color[1] * L.intensity * x;
+                        cb += L.color[2] * L.intensity * x;
+                    }
+                }
+                float nv = (float)L.num_vertices;
+                cr = cr / nv; cg = cg / nv; cb = cb / nv;
+            } else {
+                const float x = f[r++];
+                if (x > 0.0f) {
+                    cr = L.color[0] * L.intensity * x;
+                    cg = L.color[1] * L.intensity * x;
+                    cb = L.color[2] * L.intensity * x;
+                } else {
+                    cr = cg = cb = 0.0f;
+                }
+            }
+            col.x += cr; col.y += cg; col.z += cb;
+        }
+        sg.color[i] = col;
+    }
+}
+
 // AO pass 1 (chunked): one lane per (lumel, sample) pair, pairs lumel-major so a lumel's rays
 // share a wavefront; segments' work_begin counts pairs. Occluded rays are counted per lumel with
 // a ballot, one integer atomic per run.
@@ -538,6 +650,29 @@ hipError_t launch_bake_direct(hipStream_t st, int cap, uint32_t max_blocks, cons
         RL_DISPATCH_CAP(cap, hipLaunchKernelGGL((k_bake_direct<CAP, false>), dim3(grid), dim3(kDirectBlock), 0, st, segs,
                                                 nsegs, total, queue, rays));
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_bake_direct_pairs(hipStream_t st, int cap, uint32_t max_blocks, const BakeSeg* pair_segs,
+                                   const BakeSeg* lumel_segs, int nsegs, uint64_t total_pairs, uint64_t total_lumels,
+                                   bool alpha, float* inf, unsigned long long* queue, unsigned long long* rays) {
+    if (total_pairs == 0) return hipSuccess;
+    hipError_t e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(queue), 2, 0u);
+    if (e != hipSuccess) return e;
+    const uint64_t need = ((total_pairs + 63) / 64 + (kDirectBlock / 64) - 1) / (kDirectBlock / 64);
+    const uint32_t cap_blocks = std::max<uint32_t>(1u, max_blocks * (uint32_t)kBlock / (uint32_t)kDirectBlock);
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(need, cap_blocks));
+    if (alpha) {
+        RL_DISPATCH_CAP(cap, hipLaunchKernelGGL((k_direct_pairs<CAP, true>), dim3(grid), dim3(kDirectBlock), 0, st,
+                                                pair_segs, nsegs, total_pairs, inf, queue, rays));
+    } else {
+        RL_DISPATCH_CAP(cap, hipLaunchKernelGGL((k_direct_pairs<CAP, false>), dim3(grid), dim3(kDirectBlock), 0, st,
+                                                pair_segs, nsegs, total_pairs, inf, queue, rays));
+    }
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>(blocks_for(total_lumels, kBlock), 65536);
+    hipLaunchKernelGGL(k_direct_sum, dim3(blocks), dim3(kBlock), 0, st, lumel_segs, pair_segs, nsegs, total_lumels, inf);
     return hipGetLastError();
 }
 

@@ -111,7 +111,7 @@ struct rl_relight {
     unsigned long long* d_queue = nullptr;  // persistent-kernel work counter
     uint32_t max_blocks = 4096;             // persistent grid: blocks_per_cu blocks per compute unit
     int cus = 256;
-    rl_tuning tuning{RL_AO_KERNEL_REFILL, 16u, 8u, 16u, RL_FG_KERNEL_AUTO};
+    rl_tuning tuning{RL_AO_KERNEL_REFILL, 16u, 8u, 16u, RL_FG_KERNEL_AUTO, RL_DIRECT_KERNEL_AUTO};
     rl_stats stats{};
     uint32_t next_lm_seed = 0;
     // Event pairs around each kernel launch of the current call: (slot, start, end).
@@ -121,9 +121,9 @@ struct rl_relight {
     std::vector<Mark> marks;
     // Segment tables of the launches in flight (at most two at once), grown as needed and reused:
     // every bake helper synchronises its stream before returning.
-    // Segment tables, one slot per phase of a pass (kSegDirect..kSegGather), uploaded on the stream
-    // from pinned staging; seg_ev[k] marks when the staging of slot k may be rewritten.
-    static constexpr int kSegSlots = 4;
+    // Segment tables, one slot per phase of a pass (kSegDirect..kSegDirectPairs), uploaded on the
+    // stream from pinned staging; seg_ev[k] marks when the staging of slot k may be rewritten.
+    static constexpr int kSegSlots = 5;
     BakeSeg* seg_buf[kSegSlots] = {};
     size_t seg_cap[kSegSlots] = {};
     BakeSeg* seg_host[kSegSlots] = {};
@@ -138,6 +138,9 @@ struct rl_relight {
     RasterRec* raster_rec = nullptr;
     unsigned int* raster_count = nullptr;
     size_t raster_cap = 0;
+    // Influence of each (lumel, shadow ray) pair of a small direct-light pass (grown as needed).
+    float* direct_inf = nullptr;
+    size_t direct_inf_cap = 0;
 };
 
 struct rl_scene {
@@ -416,9 +419,9 @@ struct BakeItem {
     const PhotonMapDev* pms = nullptr;
 };
 
-enum SegUnit { kUnitLumel = 0, kUnitPair = 1, kUnitChunk = 2 };
+enum SegUnit { kUnitLumel = 0, kUnitPair = 1, kUnitChunk = 2, kUnitDirectPair = 3 };
 // Segment-table slots: one per phase, so the phases of a pass can be queued back to back.
-enum SegSlot { kSegDirect = 0, kSegAoWork = 1, kSegAoLumel = 2, kSegGather = 3 };
+enum SegSlot { kSegDirect = 0, kSegAoWork = 1, kSegAoLumel = 2, kSegGather = 3, kSegDirectPairs = 4 };
 
 // A device segment table for one launch (in one of the context's reusable buffers).
 struct DeviceSegs {
@@ -463,6 +466,14 @@ int32_t whole_items(rl_scene* scene, rl_lightmap* const* maps, int n, const AoPa
     return RL_SUCCESS;
 }
 
+// Shadow rays per lumel: one per point, spot or directional light, one per vertex of an area light
+// (the count the direct-light kernels derive from the scene's device lights), at least 1.
+uint32_t direct_rays_per_lumel(const rl_scene* s) {
+    uint32_t r = 0;
+    for (const LightHost& L : s->lights) r += L.verts.empty() ? 1u : (uint32_t)L.verts.size();
+    return std::max(r, 1u);
+}
+
 // The segment table of `items` in work units `unit` (out gets the counts; out.d stays unset).
 std::vector<BakeSeg> build_segments(const std::vector<BakeItem>& items, int unit, uint32_t chunk_lumels,
                                     DeviceSegs& out) {
@@ -490,6 +501,7 @@ std::vector<BakeSeg> build_segments(const std::vector<BakeItem>& items, int unit
         segs.push_back(s);
         if (unit == kUnitLumel) acc += it.count;
         else if (unit == kUnitPair) acc += (uint64_t)it.count * (uint64_t)it.ao.samples;
+        else if (unit == kUnitDirectPair) acc += (uint64_t)it.count * direct_rays_per_lumel(it.scene);
         else acc += (it.count + chunk_lumels - 1) / chunk_lumels;
         out.texels += it.count;
         out.cap = std::max(out.cap, it.scene->cap);
@@ -533,21 +545,55 @@ int32_t upload_segments(rl_relight* ctx, int slot, const std::vector<BakeItem>& 
     return RL_SUCCESS;
 }
 
-// Direct light over items (one launch). sync = false leaves the launch queued on the stream (a
-// later synchronisation, e.g. end_call's, reports its errors).
+// Lumels below which a direct-light pass runs as (lumel, shadow ray) pairs. City lumel ranges
+// with a sun and a 16-vertex area light (lane per lumel / pairs): 64 lumels 33 / 30 us, 1,024
+// 45 / 40 us, 8,192 278 / 84 us, 32,768 347 / 220 us, 65,536 454 / 325 us, 131,072 446 / 536 us,
+// 524,288 466 / 1,816 us: the lumel kernel is bound by one lane's serial rays until the machine
+// is full, the pair kernel grows with the rays.
+constexpr uint64_t kDirectPairsBelow = 98304;
+
+// Whether a direct-light pass over `items` (`lumels` lumels) runs as pairs.
+bool direct_uses_pairs(const rl_relight* ctx, const std::vector<BakeItem>& items, uint64_t lumels) {
+    uint32_t rays_per_lumel = 1;
+    for (const BakeItem& it : items) rays_per_lumel = std::max(rays_per_lumel, direct_rays_per_lumel(it.scene));
+    if (lumels == 0 || rays_per_lumel < 2) return false;
+    return ctx->tuning.direct_kernel == RL_DIRECT_KERNEL_PAIRS ||
+           (ctx->tuning.direct_kernel == RL_DIRECT_KERNEL_AUTO && lumels < kDirectPairsBelow);
+}
+
+// Direct light over items (one launch, or two for a small pass). sync = false leaves the launch
+// queued on the stream (a later synchronisation, e.g. end_call's, reports its errors).
 int32_t run_direct_items(rl_relight* ctx, const std::vector<BakeItem>& items, bool sync = true) {
     DeviceSegs ds;
     int32_t st = upload_segments(ctx, kSegDirect, items, kUnitLumel, 1, ds);
     if (st) return st;
+    // A lane per lumel traces the lumel's shadow rays one after another; a pass with fewer
+    // lumels than kDirectPairsBelow takes the pair kernels, which spread them over lanes.
+    const bool pairs = direct_uses_pairs(ctx, items, ds.total);
+    DeviceSegs ps;
+    if (pairs) {
+        if ((st = upload_segments(ctx, kSegDirectPairs, items, kUnitDirectPair, 1, ps))) return st;
+        if (ctx->direct_inf_cap < ps.total) {
+            RL_HIP(hipStreamSynchronize(ctx->stream));  // queued kernels may still write the old buffer
+            dev_free(ctx->direct_inf);
+            ctx->direct_inf_cap = 0;
+            RL_HIP(dev_alloc(&ctx->direct_inf, (size_t)ps.total));
+            ctx->direct_inf_cap = ps.total;
+        }
+    }
     hipError_t e = hipSuccess;
     if (ds.total) {
         e = kernel_begin(ctx, RL_KERNEL_DIRECT);
-        if (e == hipSuccess) e = launch_bake_direct(ctx->stream, ds.cap, ctx->max_blocks, ds.d, ds.n, ds.total,
-                                                    ds.alpha, ctx->d_queue, ctx->d_rays);
+        if (e == hipSuccess && pairs)
+            e = launch_bake_direct_pairs(ctx->stream, ds.cap, ctx->max_blocks, ps.d, ds.d, ds.n, ps.total, ds.total,
+                                         ds.alpha, ctx->direct_inf, ctx->d_queue, ctx->d_rays);
+        if (e == hipSuccess && !pairs)
+            e = launch_bake_direct(ctx->stream, ds.cap, ctx->max_blocks, ds.d, ds.n, ds.total, ds.alpha, ctx->d_queue,
+                                   ctx->d_rays);
         if (e == hipSuccess) e = kernel_end(ctx);
     }
     ctx->stats.texels += ds.texels;
-    ctx->stats.launches += ds.total ? 1 : 0;
+    ctx->stats.launches += ds.total ? (pairs ? 2 : 1) : 0;
     ctx->progress_total += ds.texels;
     if (e == hipSuccess && sync) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return fail(RL_DEVICE_ERROR, std::string("direct bake: ") + hipGetErrorString(e));
@@ -557,8 +603,17 @@ int32_t run_direct_items(rl_relight* ctx, const std::vector<BakeItem>& items, bo
 }
 
 // Lumels per AO work chunk: the refill kernel's chunks hold at most kAoChunkMax lumels.
-uint32_t ao_chunk(const rl_relight* ctx, bool refill) {
-    return refill ? std::min(ctx->tuning.ao_chunk_lumels, kAoChunkMax) : ctx->tuning.ao_chunk_lumels;
+uint32_t ao_chunk(const rl_relight* ctx, bool refill, const std::vector<BakeItem>& items) {
+    if (!refill) return ctx->tuning.ao_chunk_lumels;
+    // A refill chunk goes to one wavefront: a pass with fewer lumels than two tuned chunks per
+    // resident wavefront (max_blocks counts 128-thread workgroups) gets smaller chunks, so every
+    // wavefront has work (a 13,120-lumel pass at 64 samples: 16-lumel chunks 325 us, 2-lumel ...).
+    uint64_t lumels = 0;
+    for (const BakeItem& it : items) lumels += it.count;
+    const uint64_t waves = (uint64_t)ctx->max_blocks * 2u;
+    const uint64_t fit = (lumels + 2 * waves - 1) / (2 * waves);
+    const uint32_t tuned = std::min(ctx->tuning.ao_chunk_lumels, kAoChunkMax);
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(tuned, fit));
 }
 
 // AO over items: count pass (lane refill when every item has >= 4 samples, else chunked) and
@@ -566,7 +621,7 @@ uint32_t ao_chunk(const rl_relight* ctx, bool refill) {
 int32_t run_ao_items(rl_relight* ctx, const std::vector<BakeItem>& items, bool sync = true) {
     bool refill = ctx->tuning.ao_kernel == RL_AO_KERNEL_REFILL;
     for (const BakeItem& it : items) refill = refill && it.ao.samples >= 4;
-    const uint32_t chunk = ao_chunk(ctx, refill);
+    const uint32_t chunk = ao_chunk(ctx, refill, items);
     DeviceSegs work, lum;
     int32_t st = upload_segments(ctx, kSegAoWork, items, refill ? kUnitChunk : kUnitPair, chunk, work);
     if (st) return st;
@@ -790,6 +845,7 @@ int32_t rl_destroy(rl_relight* ctx) {
     if (ctx->raster_owner) hipFree(ctx->raster_owner);
     if (ctx->raster_rec) hipFree(ctx->raster_rec);
     if (ctx->raster_count) hipFree(ctx->raster_count);
+    if (ctx->direct_inf) hipFree(ctx->direct_inf);
     if (ctx->ev0) hipEventDestroy(ctx->ev0);
     if (ctx->ev1) hipEventDestroy(ctx->ev1);
     for (hipEvent_t e : ctx->pool) hipEventDestroy(e);
@@ -1576,7 +1632,8 @@ struct rl_pass {
         size_t texels;
     };
     std::vector<MapRef> maps;
-    BakeSeg* d_segs[3] = {};  // direct, AO work units, AO lumels
+    BakeSeg* d_segs[4] = {};  // direct (lumels), AO work units, AO lumels, direct (pairs)
+    float* d_inf = nullptr;   // pair influences of a small pass's direct light
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     uint64_t texels = 0;
@@ -1586,6 +1643,7 @@ struct rl_pass {
         if (exec) hipGraphExecDestroy(exec);
         if (graph) hipGraphDestroy(graph);
         for (BakeSeg* d : d_segs) dev_free(d);
+        dev_free(d_inf);
     }
 };
 
@@ -1600,10 +1658,14 @@ int32_t pass_upload(const std::vector<BakeSeg>& segs, BakeSeg** d) {
 
 // The launches of one pass on ctx->stream (recorded into the graph while it is captured). The
 // fills are kernels, not memset nodes.
-hipError_t pass_launches(rl_relight* ctx, const rl_pass& p, const DeviceSegs& dir, const DeviceSegs& work,
-                         const DeviceSegs& lum, bool refill, uint32_t chunk, const std::vector<BakeItem>& ao_items) {
+hipError_t pass_launches(rl_relight* ctx, const rl_pass& p, const DeviceSegs& dir, const DeviceSegs& pairs,
+                         const DeviceSegs& work, const DeviceSegs& lum, bool refill, uint32_t chunk,
+                         const std::vector<BakeItem>& ao_items) {
     hipError_t e = hipSuccess;
-    if (dir.total)
+    if (dir.total && pairs.total)
+        e = launch_bake_direct_pairs(ctx->stream, dir.cap, ctx->max_blocks, p.d_segs[3], p.d_segs[0], dir.n,
+                                     pairs.total, dir.total, dir.alpha, p.d_inf, ctx->d_queue, ctx->d_rays);
+    else if (dir.total)
         e = launch_bake_direct(ctx->stream, dir.cap, ctx->max_blocks, p.d_segs[0], dir.n, dir.total, dir.alpha,
                                ctx->d_queue, ctx->d_rays);
     if (e == hipSuccess && work.total) {
@@ -1647,18 +1709,22 @@ int32_t rl_pass_create(rl_relight* ctx, rl_scene* scene, rl_lightmap* const* map
     // The same kernel choice as a direct bake_batch call, made once at recording.
     bool refill = ctx->tuning.ao_kernel == RL_AO_KERNEL_REFILL;
     for (const BakeItem& it : items) refill = refill && it.ao.samples >= 4;
-    const uint32_t chunk = ao_chunk(ctx, refill);
-    DeviceSegs dir, work, lum;
+    const uint32_t chunk = ao_chunk(ctx, refill, items);
+    DeviceSegs dir, pairs, work, lum;
     if (do_direct && (st = pass_upload(build_segments(items, kUnitLumel, 1, dir), &p->d_segs[0]))) return st;
+    if (do_direct && direct_uses_pairs(ctx, items, dir.total)) {
+        if ((st = pass_upload(build_segments(items, kUnitDirectPair, 1, pairs), &p->d_segs[3]))) return st;
+        RL_HIP(dev_alloc(&p->d_inf, (size_t)pairs.total));
+    }
     if (ao) {
         if ((st = pass_upload(build_segments(items, refill ? kUnitChunk : kUnitPair, chunk, work), &p->d_segs[1])))
             return st;
         if ((st = pass_upload(build_segments(items, kUnitLumel, 1, lum), &p->d_segs[2]))) return st;
     }
-    p->launches = (dir.total ? 1 : 0) + (work.total ? 2 : 0);
+    p->launches = (dir.total ? (pairs.total ? 2 : 1) : 0) + (work.total ? 2 : 0);
     RL_HIP(hipStreamSynchronize(ctx->stream));
     RL_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-    hipError_t e = pass_launches(ctx, *p, dir, work, lum, refill, chunk, items);
+    hipError_t e = pass_launches(ctx, *p, dir, pairs, work, lum, refill, chunk, items);
     hipGraph_t g = nullptr;
     const hipError_t e2 = hipStreamEndCapture(ctx->stream, &g);
     p->graph = g;
@@ -2276,6 +2342,9 @@ int32_t rl_set_tuning(rl_relight* ctx, const rl_tuning* t) {
     if (t->blocks_per_cu < 1 || t->blocks_per_cu > 64) return fail(RL_INVALID_ARGUMENT, "blocks_per_cu must be in [1, 64]");
     if (t->fg_kernel != RL_FG_KERNEL_AUTO && t->fg_kernel != RL_FG_KERNEL_WAVE && t->fg_kernel != RL_FG_KERNEL_LANE)
         return fail(RL_INVALID_ARGUMENT, "unknown final-gather kernel");
+    if (t->direct_kernel != RL_DIRECT_KERNEL_AUTO && t->direct_kernel != RL_DIRECT_KERNEL_LUMEL &&
+        t->direct_kernel != RL_DIRECT_KERNEL_PAIRS)
+        return fail(RL_INVALID_ARGUMENT, "unknown direct-light kernel");
     ctx->tuning = *t;
     ctx->max_blocks = (uint32_t)ctx->cus * t->blocks_per_cu;
     return RL_SUCCESS;
@@ -2315,13 +2384,6 @@ int32_t add_item(const rl_bake_item& d, std::vector<BakeItem>& direct, std::vect
         ao.push_back(BakeItem{d.lightmap, d.scene, ao_params(*d.ao), d.first_lumel, cnt});
     }
     return RL_SUCCESS;
-}
-
-// Shadow rays per lumel at most (one per point light, one per light vertex).
-uint32_t direct_rays_per_lumel(const rl_scene* s) {
-    uint32_t r = 0;
-    for (const LightHost& L : s->lights) r += L.verts.empty() ? 1u : (uint32_t)L.verts.size();
-    return std::max(r, 1u);
 }
 
 uint32_t valid_lumels(const rl_lightmap* lm) {

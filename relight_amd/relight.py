@@ -527,7 +527,8 @@ class Relight:
         return {name: getattr(t, name) for name, _ in A.rl_tuning._fields_}
 
     def set_tuning(self, **kw):
-        """Launch tuning (ao_kernel, refill_min, ao_chunk_lumels, blocks_per_cu, fg_kernel); results
+        """Launch tuning (ao_kernel, refill_min, ao_chunk_lumels, blocks_per_cu, fg_kernel,
+        direct_kernel); results
         do not depend on it. The refill AO kernel uses chunks of at most 16 lumels."""
         cur = self.tuning()
         cur.update(kw)

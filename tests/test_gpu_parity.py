@@ -178,6 +178,36 @@ def test_city_small_batch_equals_plain_and_oracle(relight):
     assert st["texels"] == 2 * lumels
 
 
+@pytest.mark.parametrize("direct_kernel", ["lumel", "pairs"])
+def test_direct_kernels_bit_exact(relight, direct_kernel):
+    """Both direct-light kernels forced (a lane per lumel; (lumel, shadow ray) pairs whose
+    influences are summed per lumel afterwards, in the reference's light and vertex order) against
+    the oracle bit for bit: the box room (point light) and a small city (sun and a 16-vertex area
+    light), the city's lightmaps in one launch."""
+    from relight_amd._capi import RL_DIRECT_KERNEL_LUMEL, RL_DIRECT_KERNEL_PAIRS
+    print(f"[gpu] direct kernel {direct_kernel}: box room + small city", flush=True)
+    saved = relight.tuning()
+    relight.set_tuning(direct_kernel=RL_DIRECT_KERNEL_LUMEL if direct_kernel == "lumel" else RL_DIRECT_KERNEL_PAIRS)
+    try:
+        room = scenes.box_room()
+        scene, lms, ids = scenes.build_gpu(relight, room)
+        relight.bake_direct_light(scene, ids[0])
+        texel, _, _, col, _ = oracle_bake(room, 0, True, None)
+        assert np.array_equal(lms[0].to_hdr().reshape(-1, 3)[texel], col)
+        city = scenes.city(num_boxes=400, tiles=(2, 2), side_range=(64, 160), ao_samples=32)
+        scene, lms, ids = scenes.build_gpu(relight, city)
+        relight.bake_batch(scene, lms, direct=True, ao=None)
+        st = relight.stats()
+        lumels = 0
+        for li in range(len(city.lightmaps)):
+            texel, _, _, col, _ = oracle_bake(city, li, True, None)
+            lumels += texel.size
+            assert np.array_equal(lms[li].to_hdr().reshape(-1, 3)[texel], col)
+        assert st["texels"] == lumels and 0 < st["rays"] <= 17 * lumels
+    finally:
+        relight.set_tuning(**saved)
+
+
 def test_city_full_size_properties(relight):
     """The full procedural city (60k triangles, 32 lightmaps 256^2-2048^2) at reduced AO count:
     size-independent properties (ray counts exact, values finite and in range, lightmaps
