@@ -442,7 +442,8 @@ int32_t rl_bake_scene(rl_relight* ctx, rl_scene* scene, int32_t direct, const rl
  * loop takes up to quantum_rays rays of direct, AO and final-gather work, one launch per phase, from
  * the jobs of one scene: that of the oldest job with ray work. The step takes that scene's jobs
  * older than any other scene's oldest job and its small jobs (whole remaining work within a
- * quarter of an even share), each first a quarter of an even share, the rest in admission order
+ * quarter of an even share; none while that oldest job has waited over 60 ms since submission),
+ * each first a quarter of an even share, the rest in admission order
  * (continuous batching: a job submitted while others run joins at the next step). The next step is
  * planned and queued while one runs. Photon emission runs as a whole step; radiosity jobs run
  * beside the steps on a second stream. Scenes, lightmaps
