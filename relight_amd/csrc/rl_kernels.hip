@@ -353,7 +353,38 @@ constexpr uint32_t kAoNodeExit = 8;
 // tables 159 KB) serves all of them: city AO 360 ms with four-wavefront workgroups and 152 nodes,
 // 350 ms with eight and 304, 352/347 ms with sixteen and 400/600.
 constexpr int kAoBlock = 1024;
-constexpr uint32_t kAoTopNodes = 600;
+constexpr uint32_t kAoTopNodes = 360;
+// Each wavefront builds a local tree (rl_traverse.h) for a chunk of at least kVtMinRays rays, over
+// the union of the boxes its lumels' AO segments stay in, and starts the chunk's rays at its
+// virtual root. Rays of several chunks can be in flight at once, so a wavefront keeps three trees
+// and builds into one only when no ray of the chunk that used it last is still in flight; otherwise
+// (and for smaller chunks) the chunk's rays start at the scene's root. LDS: 30 KB of virtual
+// nodes, paid for with the top-node cache (600 -> 360 nodes), which local trees mostly skip.
+constexpr uint32_t kVtBuffers = 3;
+constexpr uint32_t kVtMinRays = 64;
+
+// The box that the AO segments of a lumel stay in: from p, length D, directions of the sampler.
+// Cosine: d = t1 x + t2 y + n z with x^2 + y^2 <= 1 and 0 <= z <= 1, so each component of d lies
+// within sqrt(t1_a^2 + t2_a^2) of [min(0, n_a), max(0, n_a)]; uniform: any unit vector. The margin
+// covers the rounding of the direction, its normalisation and the segment end point.
+__device__ __forceinline__ QueryBox ao_segment_box(V3 p, const CosineFrame& f, float D, bool cosine) {
+    QueryBox q;
+    const float pa[3] = {p.x, p.y, p.z};
+    const float t1[3] = {f.t1.x, f.t1.y, f.t1.z}, t2[3] = {f.t2.x, f.t2.y, f.t2.z}, n[3] = {f.n.x, f.n.y, f.n.z};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        float dlo = -1.0f, dhi = 1.0f;
+        if (cosine) {
+            const float s = sqrtf(t1[a] * t1[a] + t2[a] * t2[a]);
+            dlo = fminf(n[a], 0.0f) - s;
+            dhi = fmaxf(n[a], 0.0f) + s;
+        }
+        const float pad = 1e-3f * D + 1e-5f * (fabsf(pa[a]) + D);
+        q.lo[a] = pa[a] + D * dlo - pad;
+        q.hi[a] = pa[a] + D * dhi + pad;
+    }
+    return q;
+}
 
 template <int CAP>
 __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restrict__ segs, int nsegs,
@@ -367,6 +398,7 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
     // Per wavefront, the current chunk's lumels: position and texel key, cosine-sampling frame
     // (t1, t2, n), computed once per lumel when the chunk is taken instead of once per ray.
     __shared__ u32x4 lumel_rows[kAoBlock / 64][kAoChunkMax][4];
+    __shared__ u32x4 vt_rows[kAoBlock / 64 * kVtBuffers * kVtNodes * kNodeRows];
     const SceneDev top_scene = segs[0].scene;
     const uint32_t top_count = load_top_nodes(top_scene, top_rows, kAoTopNodes);
     uint32_t* stk = stack + threadIdx.x;
@@ -382,6 +414,9 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
     uint32_t pool_next = 0, pool_end = 0, stream_base = 0;
     uint32_t pool_lc = 0, pool_si = 0;  // pool_next = pool_lc * chunk_ns + pool_si
     bool pool_open = true;
+    // Local trees: trees built, each buffer's end stream (one past its chunk's last lumel), the
+    // current chunk's start node.
+    uint32_t vt_seq = 0, vt_end0 = 0, vt_end1 = 0, vt_end2 = 0, chunk_root = 0;
     // Lane state.
     bool active = false;
     TravRay tr;
@@ -390,7 +425,7 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
     int sp = 0;
     uint32_t* counter = nullptr;
     uint32_t rays = 0;
-    TopCache top{top_rows, 0u};
+    TopCache top{top_rows, 0u, vt_rows};
     __syncthreads();
     for (;;) {
         uint64_t idle = __ballot(!active);
@@ -399,6 +434,9 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
 #pragma unroll
             for (int off = 32; off >= 1; off >>= 1) oldest = min(oldest, (uint32_t)__shfl_xor((int)oldest, off, 64));
             const uint64_t allow = oldest == 0xFFFFFFFFu ? ~0ull : (uint64_t)oldest + (uint64_t)(kRing - 2);
+            // The lowest stream in flight: the oldest active ray's, or the first one this refill
+            // hands out (streams are handed out in increasing order).
+            uint32_t lowest = oldest;
             uint64_t want = idle;
             while (want && pool_open) {
                 if (pool_next == pool_end) {
@@ -416,6 +454,8 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
                     pool_next = 0;
                     pool_lc = pool_si = 0;
                     pool_end = chunk_count * chunk_ns;
+                    QueryBox q = {{__builtin_inff(), __builtin_inff(), __builtin_inff()},
+                                  {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()}};
                     if (lane < chunk_count) {  // chunk_count <= kAoChunkMax (launcher)
                         const uint32_t li = chunk_first + lane;
                         const float4 p4 = sg.pos[li], n4 = sg.nrm[li];
@@ -426,10 +466,34 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
                         row[1] = u32x4{f_bits(f.t1.x), f_bits(f.t1.y), f_bits(f.t1.z), (uint32_t)(key >> 32)};
                         row[2] = u32x4{f_bits(f.t2.x), f_bits(f.t2.y), f_bits(f.t2.z), 0u};
                         row[3] = u32x4{f_bits(n4.x), f_bits(n4.y), f_bits(n4.z), 0u};
+                        q = ao_segment_box(v3(p4.x, p4.y, p4.z), f, sg.ao.max_distance,
+                                           sg.ao.sampling == SAMPLE_COSINE);
                     }
                     // Other lanes of this wavefront read the rows below; the wavefront's LDS
                     // operations complete in order.
                     __builtin_amdgcn_wave_barrier();
+                    // The chunk's local tree over the union of its lumels' segment boxes.
+#pragma unroll
+                    for (int off = 32; off >= 1; off >>= 1) {
+#pragma unroll
+                        for (int a = 0; a < 3; a++) {
+                            q.lo[a] = fminf(q.lo[a], __shfl_xor(q.lo[a], off, 64));
+                            q.hi[a] = fmaxf(q.hi[a], __shfl_xor(q.hi[a], off, 64));
+                        }
+                    }
+                    const uint32_t buf = vt_seq % kVtBuffers;
+                    const uint32_t buf_end = buf == 0 ? vt_end0 : (buf == 1 ? vt_end1 : vt_end2);
+                    chunk_root = 0u;
+                    if (chunk_count * chunk_ns >= kVtMinRays && lowest >= buf_end) {
+                        const uint32_t first = (wave * kVtBuffers + buf) * kVtNodes;
+                        const TopCache btop{top_rows, sg.scene.nodes == top_scene.nodes ? top_count : 0u, nullptr};
+                        chunk_root = build_local_tree(sg.scene, btop, q, vt_rows + first * kNodeRows, first);
+                        const uint32_t end = stream_base + chunk_count;
+                        if (buf == 0) vt_end0 = end;
+                        else if (buf == 1) vt_end1 = end;
+                        else vt_end2 = end;
+                        vt_seq++;
+                    }
                 }
                 uint64_t room = pool_end - pool_next;
                 if (allow != ~0ull) {
@@ -438,6 +502,7 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
                 }
                 const uint32_t take = (uint32_t)min(room, (uint64_t)__popcll(want));
                 if (take == 0) break;
+                lowest = min(lowest, stream_base + pool_lc);
                 const uint32_t rank = rank_below(want);
                 const bool mine = ((want >> lane) & 1ull) && rank < take;
                 if (mine) {
@@ -470,7 +535,7 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
                     tr = trav_ray(ray_from_segment(p, p + dir * sg.ao.max_distance));
                     scene = sg.scene;
                     top.count = sg.scene.nodes == top_scene.nodes ? top_count : 0u;
-                    node = 0;
+                    node = chunk_root;
                     sp = 0;
                     my_stream = stream_base + lc;
                     my_ns = chunk_ns;
@@ -495,8 +560,9 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
         if (active) {
             const uint32_t min_lanes = pool_open ? kAoNodeExit : 0u;
             const int st = one_scene(scene)
-                               ? any_hit_run<CAP, true>(scene, top, tr, node, sp, stk, kAoBlock, spill, min_lanes)
-                               : any_hit_run<CAP, false>(scene, top, tr, node, sp, stk, kAoBlock, spill, min_lanes);
+                               ? any_hit_run<CAP, true, true>(scene, top, tr, node, sp, stk, kAoBlock, spill, min_lanes)
+                               : any_hit_run<CAP, false, true>(scene, top, tr, node, sp, stk, kAoBlock, spill,
+                                                              min_lanes);
             if (st != 0) {
                 rays++;
                 if (st > 0) atomicAdd(&ring_occ[wave][slot], 1u);

@@ -178,9 +178,15 @@ __device__ __forceinline__ uint32_t stack_pop(int& sp, const uint32_t* stk, uint
 constexpr uint32_t kNodeRows = 8;
 
 struct TopCache {
-    const u32x4* rows;  // LDS, kNodeRows rows per node
-    uint32_t count;     // nodes held for this lane's scene (0: none)
+    const u32x4* rows;   // LDS, kNodeRows rows per node
+    uint32_t count;      // nodes held for this lane's scene (0: none)
+    const u32x4* vrows;  // LDS virtual nodes of the workgroup (local trees, below), or null
 };
+
+// Virtual node k of a workgroup (a node built in LDS for a box of rays, see build_local_tree) has
+// the node index kVirtualBase + k. Real node indices stay below 2^25, and the node's byte offset
+// (index << 7, in 32 bits) is then k * 128: its offset in the LDS array of virtual nodes.
+constexpr uint32_t kVirtualBase = 1u << 25;
 
 // Copies the first min(max_nodes, S.num_nodes) nodes of S into `rows` (LDS) with the whole
 // workgroup and returns how many it holds; the caller synchronises the workgroup before use.
@@ -191,20 +197,197 @@ __device__ __forceinline__ uint32_t load_top_nodes(const SceneDev& S, u32x4* row
     return n;
 }
 
+// ---------------------------------------------------------------- local trees
+// Rays known to stay inside a box q (the AO rays of a chunk of lumels: segments of length D from
+// each lumel into its hemisphere) can only hit triangles in BVH subtrees whose boxes overlap q: a
+// hit point lies in q and in every box on the path from the root to the hit triangle's leaf. A
+// local tree replaces the top of the BVH for such rays: a virtual root whose entries are a
+// frontier of subtrees overlapping q (copies of their child boxes and words), and up to four
+// second-level virtual nodes that refine the root's internal entries the same way. Traversal from
+// the virtual root tests the same child boxes below the frontier as traversal from the real root,
+// and skips only subtrees that cannot hold a hit, so any-hit results are unchanged; it skips the
+// several node visits the real root's levels cost every ray (city AO: 14.1 node visits per ray
+// from the root, 7.7 plus 2.3 virtual ones from a 16-lumel chunk's local tree).
+
+struct QueryBox {
+    float lo[3], hi[3];
+};
+
+__device__ __forceinline__ bool box_overlaps(const QueryBox& q, float lx, float hx, float ly, float hy, float lz,
+                                             float hz) {
+    return !(hx < q.lo[0] || lx > q.hi[0] || hy < q.lo[1] || ly > q.hi[1] || hz < q.lo[2] || lz > q.hi[2]);
+}
+
+// A virtual node in LDS: the node layout (rows lox, hix, loy, hiy, loz, hiz, child, pad) written
+// entry by entry.
+__device__ __forceinline__ void vt_set(u32x4* v, uint32_t j, const float* lo, const float* hi, uint32_t child) {
+    float* f = reinterpret_cast<float*>(v);
+    f[0 * 4 + j] = lo[0]; f[1 * 4 + j] = hi[0];
+    f[2 * 4 + j] = lo[1]; f[3 * 4 + j] = hi[1];
+    f[4 * 4 + j] = lo[2]; f[5 * 4 + j] = hi[2];
+    reinterpret_cast<uint32_t*>(v)[6 * 4 + j] = child;
+}
+
+__device__ __forceinline__ void vt_get(const u32x4* v, uint32_t j, float* lo, float* hi, uint32_t& child) {
+    const float* f = reinterpret_cast<const float*>(v);
+    lo[0] = f[0 * 4 + j]; hi[0] = f[1 * 4 + j];
+    lo[1] = f[2 * 4 + j]; hi[1] = f[3 * 4 + j];
+    lo[2] = f[4 * 4 + j]; hi[2] = f[5 * 4 + j];
+    child = reinterpret_cast<const uint32_t*>(v)[6 * 4 + j];
+}
+
+__device__ __forceinline__ void vt_clear(u32x4* v, uint32_t j) {
+    const float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+    vt_set(v, j, lo, hi, RL_EMPTY_CHILD);  // an inverted box: no ray enters it
+}
+
+// A real node's six box rows (lox, hix, loy, hiy, loz, hiz) and child words.
+__device__ __forceinline__ void read_node(const SceneDev& S, const TopCache& top, uint32_t c, f32x4* rows, u32x4& ch) {
+    if (c < top.count) {
+        const u32x4* b = top.rows + c * kNodeRows;
+#pragma unroll
+        for (int k = 0; k < 6; k++) rows[k] = __builtin_bit_cast(f32x4, b[k]);
+        ch = b[6];
+    } else {
+        const guint4* b = reinterpret_cast<const guint4*>(gmem(reinterpret_cast<const float4*>(S.nodes))) +
+                          (size_t)c * kNodeRows;
+#pragma unroll
+        for (int k = 0; k < 6; k++) rows[k] = __builtin_bit_cast(f32x4, b[k]);
+        ch = b[6];
+    }
+}
+
+// Builds virtual node v (cleared beforehand) for box q from the children of real node `start`
+// that overlap q, then repeatedly replaces an internal entry by its children overlapping q while
+// the entries still fit in four (in entry order, re-examining an entry after it is replaced; an
+// entry none of whose children overlaps q is dropped). At most max_loads nodes are read; every
+// step keeps the entries a cover of the subtrees under `start` that overlap q, so stopping early
+// is still correct. Returns the number of entries. Run by one lane per node.
+__device__ __forceinline__ uint32_t build_local_node(const SceneDev& S, const TopCache& top, uint32_t start,
+                                                     const QueryBox& q, u32x4* v, uint32_t max_loads) {
+    uint32_t n = 0, loads = 0;
+    for (uint32_t j = 0; loads < max_loads;) {
+        uint32_t c;
+        if (loads == 0) {
+            c = start;
+        } else {
+            if (j >= n) break;
+            float lo[3], hi[3];
+            vt_get(v, j, lo, hi, c);
+            if (c & RL_LEAF_BIT) {
+                j++;
+                continue;
+            }
+        }
+        f32x4 rows[6];
+        u32x4 ch;
+        read_node(S, top, c, rows, ch);
+        loads++;
+        const float lox[4] = {rows[0].x, rows[0].y, rows[0].z, rows[0].w}, hix[4] = {rows[1].x, rows[1].y, rows[1].z, rows[1].w};
+        const float loy[4] = {rows[2].x, rows[2].y, rows[2].z, rows[2].w}, hiy[4] = {rows[3].x, rows[3].y, rows[3].z, rows[3].w};
+        const float loz[4] = {rows[4].x, rows[4].y, rows[4].z, rows[4].w}, hiz[4] = {rows[5].x, rows[5].y, rows[5].z, rows[5].w};
+        const uint32_t cw[4] = {ch.x, ch.y, ch.z, ch.w};
+        bool ov[4];
+        uint32_t k = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            ov[i] = cw[i] != RL_EMPTY_CHILD && box_overlaps(q, lox[i], hix[i], loy[i], hiy[i], loz[i], hiz[i]);
+            k += ov[i] ? 1u : 0u;
+        }
+        if (loads == 1) {
+            // The start node's overlapping children are the initial entries (at most four).
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                if (ov[i]) {
+                    const float lo[3] = {lox[i], loy[i], loz[i]}, hi[3] = {hix[i], hiy[i], hiz[i]};
+                    vt_set(v, n++, lo, hi, cw[i]);
+                }
+            }
+            continue;
+        }
+        if (n - 1u + k > 4u) {
+            j++;
+            continue;
+        }
+        // Entry j becomes the first overlapping child, the others are appended; with none, the last
+        // entry moves into slot j.
+        uint32_t slot = j;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if (ov[i]) {
+                const float lo[3] = {lox[i], loy[i], loz[i]}, hi[3] = {hix[i], hiy[i], hiz[i]};
+                vt_set(v, slot, lo, hi, cw[i]);
+                slot = slot == j ? n : slot + 1u;
+            }
+        }
+        if (k == 0) {
+            n--;
+            if (j != n) {
+                float lo[3], hi[3];
+                uint32_t c2;
+                vt_get(v, n, lo, hi, c2);
+                vt_set(v, j, lo, hi, c2);
+            }
+            vt_clear(v, n);
+        } else {
+            n += k - 1u;
+        }
+    }
+    return n;
+}
+
+// Builds a wavefront's local tree for box q in the kVtNodes virtual nodes at `v` (node 0 the root,
+// node 1 + j refining root entry j), whose virtual indices are kVirtualBase + first .. + 4, and
+// returns the root's index. Every lane of the wavefront calls it with the same arguments.
+constexpr uint32_t kVtNodes = 5;
+constexpr uint32_t kVtMaxLoads = 48;
+
+__device__ __forceinline__ uint32_t build_local_tree(const SceneDev& S, const TopCache& top, const QueryBox& q,
+                                                     u32x4* v, uint32_t first) {
+    const uint32_t lane = lane_id();
+    if (lane < kVtNodes * 4u) vt_clear(v + (lane >> 2) * kNodeRows, lane & 3u);
+    __builtin_amdgcn_wave_barrier();
+    if (lane == 0) build_local_node(S, top, 0u, q, v, kVtMaxLoads);
+    __builtin_amdgcn_wave_barrier();
+    uint32_t c = RL_EMPTY_CHILD, k = 0;
+    float lo[3], hi[3];
+    if (lane < 4u) {
+        vt_get(v, lane, lo, hi, c);
+        if (c != RL_EMPTY_CHILD && !(c & RL_LEAF_BIT))
+            k = build_local_node(S, top, c, q, v + (1u + lane) * kNodeRows, kVtMaxLoads);
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (lane < 4u && c != RL_EMPTY_CHILD && !(c & RL_LEAF_BIT)) {
+        if (k == 0) {
+            vt_clear(v, lane);  // no subtree under this entry overlaps q
+        } else if (k == 1) {
+            uint32_t c1;
+            vt_get(v + (1u + lane) * kNodeRows, 0, lo, hi, c1);
+            vt_set(v, lane, lo, hi, c1);
+        } else {
+            reinterpret_cast<uint32_t*>(v)[6 * 4 + lane] = kVirtualBase + first + 1u + lane;
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    return kVirtualBase + first;
+}
+
 // One internal node: its four child boxes tested against (tnear, lim), hit children pushed
 // far-first, the nearest taken next; with no child hit the next node is popped. Returns false when
 // the stack was empty (traversal ended).
-template <int CAP>
+// VT: the node may be a virtual node (index kVirtualBase + k, read from top.vrows).
+template <int CAP, bool VT = false>
 __device__ __forceinline__ bool visit_node(const SceneDev& S, const TopCache& top, const SlabRay& r, float tnear,
                                            float lim, uint32_t& node, int& sp, uint32_t* stk, uint32_t stride,
                                            uint32_t* spill) {
     f32x4 ex, fx, ey, fy, ez, fz;
     u32x4 ch;
-    const uint32_t off = node << 7;  // byte offset of the node (node < 2^25)
+    const uint32_t off = node << 7;  // byte offset of the node (k * 128 for virtual node k)
     const uint32_t aex = off + r.rex, afx = off + r.rfx, aey = off + r.rey, afy = off + r.rfy, aez = off + r.rez,
                    afz = off + r.rfz;
-    if (node < top.count) {
-        const char* b = reinterpret_cast<const char*>(top.rows);
+    const bool virt = VT && node >= kVirtualBase;
+    if (node < top.count || virt) {
+        const char* b = reinterpret_cast<const char*>(virt ? top.vrows : top.rows);
         ex = __builtin_bit_cast(f32x4, *reinterpret_cast<const u32x4*>(b + aex));
         fx = __builtin_bit_cast(f32x4, *reinterpret_cast<const u32x4*>(b + afx));
         ey = __builtin_bit_cast(f32x4, *reinterpret_cast<const u32x4*>(b + aey));
@@ -258,10 +441,10 @@ __device__ __forceinline__ bool visit_node(const SceneDev& S, const TopCache& to
     return true;
 }
 
-template <int CAP>
+template <int CAP, bool VT = false>
 __device__ __forceinline__ bool any_hit_node(const SceneDev& S, const TopCache& top, const TravRay& r, uint32_t& node,
                                              int& sp, uint32_t* stk, uint32_t stride, uint32_t* spill) {
-    return visit_node<CAP>(S, top, r.s, r.w.tnear, r.w.tfar * kBoxScale, node, sp, stk, stride, spill);
+    return visit_node<CAP, VT>(S, top, r.s, r.w.tnear, r.w.tfar * kBoxScale, node, sp, stk, stride, spill);
 }
 
 // One step of any-hit traversal: either one internal node or one leaf (its triangles tested).
@@ -294,8 +477,8 @@ __device__ __forceinline__ int any_hit_step(const SceneDev& S, const TopCache& t
 // each searching lane, so repeated calls always finish). Returns like any_hit_step; the set of
 // nodes and triangles a ray visits is the same (only the grouping into steps differs), so the
 // occlusion result is the same.
-// UNI: the caller has checked one_scene(S0).
-template <int CAP, bool UNI>
+// UNI: the caller has checked one_scene(S0). VT: nodes may be virtual (local trees).
+template <int CAP, bool UNI, bool VT = false>
 __device__ __forceinline__ int any_hit_run(const SceneDev& S0, const TopCache& top, const TravRay& r, uint32_t& node,
                                            int& sp, uint32_t* stk, uint32_t stride, uint32_t* spill,
                                            uint32_t min_lanes) {
@@ -304,7 +487,7 @@ __device__ __forceinline__ int any_hit_run(const SceneDev& S0, const TopCache& t
     for (uint32_t stop_at = 0;; stop_at = min_lanes) {
         const bool searching = !ended && !(node & RL_LEAF_BIT);
         if ((uint32_t)__popcll(__ballot(searching)) <= stop_at) break;
-        if (searching && !any_hit_node<CAP>(S, top, r, node, sp, stk, stride, spill)) ended = true;
+        if (searching && !any_hit_node<CAP, VT>(S, top, r, node, sp, stk, stride, spill)) ended = true;
     }
     if (ended) return -1;
     if (!(node & RL_LEAF_BIT)) return 0;
