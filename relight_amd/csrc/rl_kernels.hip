@@ -394,11 +394,11 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
     __shared__ uint32_t stack[kLdsStack * kAoBlock];
     __shared__ uint32_t ring_occ[kAoBlock / 64][kRing];
     __shared__ uint32_t ring_done[kAoBlock / 64][kRing];
-    __shared__ u32x4 top_rows[kAoTopNodes * kNodeRows];
+    // The scene's first kAoTopNodes nodes, then the wavefronts' local trees (virtual nodes).
+    __shared__ u32x4 top_rows[(kAoTopNodes + kAoBlock / 64 * kVtBuffers * kVtNodes) * kNodeRows];
     // Per wavefront, the current chunk's lumels: position and texel key, cosine-sampling frame
     // (t1, t2, n), computed once per lumel when the chunk is taken instead of once per ray.
     __shared__ u32x4 lumel_rows[kAoBlock / 64][kAoChunkMax][4];
-    __shared__ u32x4 vt_rows[kAoBlock / 64 * kVtBuffers * kVtNodes * kNodeRows];
     const SceneDev top_scene = segs[0].scene;
     const uint32_t top_count = load_top_nodes(top_scene, top_rows, kAoTopNodes);
     uint32_t* stk = stack + threadIdx.x;
@@ -425,7 +425,7 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
     int sp = 0;
     uint32_t* counter = nullptr;
     uint32_t rays = 0;
-    TopCache top{top_rows, 0u, vt_rows};
+    TopCache top{top_rows, 0u};
     __syncthreads();
     for (;;) {
         uint64_t idle = __ballot(!active);
@@ -485,9 +485,9 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
                     const uint32_t buf_end = buf == 0 ? vt_end0 : (buf == 1 ? vt_end1 : vt_end2);
                     chunk_root = 0u;
                     if (chunk_count * chunk_ns >= kVtMinRays && lowest >= buf_end) {
-                        const uint32_t first = (wave * kVtBuffers + buf) * kVtNodes;
-                        const TopCache btop{top_rows, sg.scene.nodes == top_scene.nodes ? top_count : 0u, nullptr};
-                        chunk_root = build_local_tree(sg.scene, btop, q, vt_rows + first * kNodeRows, first);
+                        const uint32_t first = kAoTopNodes + (wave * kVtBuffers + buf) * kVtNodes;
+                        const TopCache btop{top_rows, sg.scene.nodes == top_scene.nodes ? top_count : 0u};
+                        chunk_root = build_local_tree(sg.scene, btop, q, top_rows + first * kNodeRows, first);
                         const uint32_t end = stream_base + chunk_count;
                         if (buf == 0) vt_end0 = end;
                         else if (buf == 1) vt_end1 = end;

@@ -178,14 +178,15 @@ __device__ __forceinline__ uint32_t stack_pop(int& sp, const uint32_t* stk, uint
 constexpr uint32_t kNodeRows = 8;
 
 struct TopCache {
-    const u32x4* rows;   // LDS, kNodeRows rows per node
-    uint32_t count;      // nodes held for this lane's scene (0: none)
-    const u32x4* vrows;  // LDS virtual nodes of the workgroup (local trees, below), or null
+    const u32x4* rows;  // LDS, kNodeRows rows per node
+    uint32_t count;     // nodes held for this lane's scene (0: none)
 };
 
-// Virtual node k of a workgroup (a node built in LDS for a box of rays, see build_local_tree) has
-// the node index kVirtualBase + k. Real node indices stay below 2^25, and the node's byte offset
-// (index << 7, in 32 bits) is then k * 128: its offset in the LDS array of virtual nodes.
+// Virtual nodes (nodes built in LDS for a box of rays, see build_local_tree) sit in the same LDS
+// array as the top nodes, after its top-node capacity C: the node in slot C + k has the node index
+// kVirtualBase + C + k. Real node indices stay below kVirtualBase = 2^25, so the node's byte offset
+// (index << 7, in 32 bits) is (C + k) * 128, its offset in that array, and one test
+// (index < count or index >= 2^25) tells LDS nodes from nodes in memory.
 constexpr uint32_t kVirtualBase = 1u << 25;
 
 // Copies the first min(max_nodes, S.num_nodes) nodes of S into `rows` (LDS) with the whole
@@ -337,8 +338,8 @@ __device__ __forceinline__ uint32_t build_local_node(const SceneDev& S, const To
 }
 
 // Builds a wavefront's local tree for box q in the kVtNodes virtual nodes at `v` (node 0 the root,
-// node 1 + j refining root entry j), whose virtual indices are kVirtualBase + first .. + 4, and
-// returns the root's index. Every lane of the wavefront calls it with the same arguments.
+// node 1 + j refining root entry j), which are slots first .. first + 4 of the LDS node array, and
+// returns the root's node index. Every lane of the wavefront calls it with the same arguments.
 constexpr uint32_t kVtNodes = 5;
 constexpr uint32_t kVtMaxLoads = 48;
 
@@ -375,19 +376,18 @@ __device__ __forceinline__ uint32_t build_local_tree(const SceneDev& S, const To
 // One internal node: its four child boxes tested against (tnear, lim), hit children pushed
 // far-first, the nearest taken next; with no child hit the next node is popped. Returns false when
 // the stack was empty (traversal ended).
-// VT: the node may be a virtual node (index kVirtualBase + k, read from top.vrows).
+// VT: the node may be a virtual node (index >= kVirtualBase, held in top.rows).
 template <int CAP, bool VT = false>
 __device__ __forceinline__ bool visit_node(const SceneDev& S, const TopCache& top, const SlabRay& r, float tnear,
                                            float lim, uint32_t& node, int& sp, uint32_t* stk, uint32_t stride,
                                            uint32_t* spill) {
     f32x4 ex, fx, ey, fy, ez, fz;
     u32x4 ch;
-    const uint32_t off = node << 7;  // byte offset of the node (k * 128 for virtual node k)
+    const uint32_t off = node << 7;  // byte offset of the node (in 32 bits, also for virtual nodes)
     const uint32_t aex = off + r.rex, afx = off + r.rfx, aey = off + r.rey, afy = off + r.rfy, aez = off + r.rez,
                    afz = off + r.rfz;
-    const bool virt = VT && node >= kVirtualBase;
-    if (node < top.count || virt) {
-        const char* b = reinterpret_cast<const char*>(virt ? top.vrows : top.rows);
+    if (node < top.count || (VT && node >= kVirtualBase)) {
+        const char* b = reinterpret_cast<const char*>(top.rows);
         ex = __builtin_bit_cast(f32x4, *reinterpret_cast<const u32x4*>(b + aex));
         fx = __builtin_bit_cast(f32x4, *reinterpret_cast<const u32x4*>(b + afx));
         ey = __builtin_bit_cast(f32x4, *reinterpret_cast<const u32x4*>(b + aey));
