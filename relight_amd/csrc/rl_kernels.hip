@@ -362,6 +362,8 @@ constexpr uint32_t kAoTopNodes = 360;
 // nodes, paid for with the top-node cache (600 -> 360 nodes), which local trees mostly skip.
 constexpr uint32_t kVtBuffers = 3;
 constexpr uint32_t kVtMinRays = 64;
+// An empty occluder hint (above every triangle index).
+constexpr uint32_t kNoHint = 0xFFFFFFFFu;
 
 // The box that the AO segments of a lumel stay in: from p, length D, directions of the sampler.
 // Cosine: d = t1 x + t2 y + n z with x^2 + y^2 <= 1 and 0 <= z <= 1, so each component of d lies
@@ -397,7 +399,8 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
     // The scene's first kAoTopNodes nodes, then the wavefronts' local trees (virtual nodes).
     __shared__ u32x4 top_rows[(kAoTopNodes + kAoBlock / 64 * kVtBuffers * kVtNodes) * kNodeRows];
     // Per wavefront, the current chunk's lumels: position and texel key, cosine-sampling frame
-    // (t1, t2, n), computed once per lumel when the chunk is taken instead of once per ray.
+    // (t1, t2, n), computed once per lumel when the chunk is taken instead of once per ray, and
+    // the lumel's two occluder hints (the w words of rows 2 and 3).
     __shared__ u32x4 lumel_rows[kAoBlock / 64][kAoChunkMax][4];
     const SceneDev top_scene = segs[0].scene;
     const uint32_t top_count = load_top_nodes(top_scene, top_rows, kAoTopNodes);
@@ -418,14 +421,29 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
     // current chunk's start node.
     uint32_t vt_seq = 0, vt_end0 = 0, vt_end1 = 0, vt_end2 = 0, chunk_root = 0;
     // Lane state.
-    bool active = false;
+    bool active = false, fresh = false;
     TravRay tr;
     SceneDev scene;
     uint32_t node = 0, slot = 0, my_stream = 0, my_ns = 1;
+    uint32_t hint0 = kNoHint, hint1 = kNoHint, my_lc = 0, my_key = 0;
     int sp = 0;
     uint32_t* counter = nullptr;
     uint32_t rays = 0;
     TopCache top{top_rows, 0u};
+    // A lane's ray is done: count it, add it to its lumel's ring slot, and publish the lumel's
+    // occlusion count when this was its last ray.
+    auto finish = [&](bool occluded) {
+        rays++;
+        if (occluded) atomicAdd(&ring_occ[wave][slot], 1u);
+        const uint32_t done = atomicAdd(&ring_done[wave][slot], 1u);
+        if (done == my_ns - 1) {
+            // Last ray of the lumel: every earlier increment of this slot precedes this one in
+            // the wavefront's LDS order; read, publish, clear.
+            *counter = atomicExch(&ring_occ[wave][slot], 0u);
+            atomicExch(&ring_done[wave][slot], 0u);
+        }
+        active = false;
+    };
     __syncthreads();
     for (;;) {
         uint64_t idle = __ballot(!active);
@@ -464,8 +482,8 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
                         u32x4* row = lumel_rows[wave][lane];
                         row[0] = u32x4{f_bits(p4.x), f_bits(p4.y), f_bits(p4.z), (uint32_t)key};
                         row[1] = u32x4{f_bits(f.t1.x), f_bits(f.t1.y), f_bits(f.t1.z), (uint32_t)(key >> 32)};
-                        row[2] = u32x4{f_bits(f.t2.x), f_bits(f.t2.y), f_bits(f.t2.z), 0u};
-                        row[3] = u32x4{f_bits(n4.x), f_bits(n4.y), f_bits(n4.z), 0u};
+                        row[2] = u32x4{f_bits(f.t2.x), f_bits(f.t2.y), f_bits(f.t2.z), kNoHint};
+                        row[3] = u32x4{f_bits(n4.x), f_bits(n4.y), f_bits(n4.z), kNoHint};
                         q = ao_segment_box(v3(p4.x, p4.y, p4.z), f, sg.ao.max_distance,
                                            sg.ao.sampling == SAMPLE_COSINE);
                     }
@@ -541,7 +559,12 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
                     my_ns = chunk_ns;
                     slot = my_stream % (uint32_t)kRing;
                     counter = sg.counts + li;
+                    hint0 = r2.w;
+                    hint1 = r3.w;
+                    my_lc = lc;
+                    my_key = r0.w;
                     active = true;
+                    fresh = true;
                 }
                 pool_next += take;
                 pool_si += take;
@@ -553,28 +576,45 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
                 }
                 want &= ~__ballot(mine);
             }
+            // Occluder hints: a new ray first tests the last two triangles that occluded rays of
+            // its lumel. A hit there is a hit the traversal would also find (the same watertight
+            // test in the ray's interval), so the ray is occluded and completes at once; the lanes
+            // it frees take new rays before the wavefront traverses. Hints only choose which
+            // triangles are tested first: the occlusion result does not depend on them.
+            bool by_hint = false;
+            if (fresh) {
+                fresh = false;
+                if (hint0 < scene.num_tris) by_hint = hits_triangle(scene, tr, hint0);
+                if (!by_hint && hint1 < scene.num_tris) by_hint = hits_triangle(scene, tr, hint1);
+                if (by_hint) finish(true);
+            }
+            if (__ballot(by_hint) != 0ull && pool_open && (uint32_t)__popcll(__ballot(!active)) >= refill_min)
+                continue;
             if (!pool_open && __ballot(active) == 0ull) break;
         } else if (!pool_open && idle == ~0ull) {
             break;
         }
         if (active) {
             const uint32_t min_lanes = pool_open ? kAoNodeExit : 0u;
+            uint32_t hit = kNoHint;
             const int st = one_scene(scene)
-                               ? any_hit_run<CAP, true, true>(scene, top, tr, node, sp, stk, kAoBlock, spill, min_lanes)
+                               ? any_hit_run<CAP, true, true>(scene, top, tr, node, sp, stk, kAoBlock, spill, min_lanes,
+                                                              hit)
                                : any_hit_run<CAP, false, true>(scene, top, tr, node, sp, stk, kAoBlock, spill,
-                                                              min_lanes);
-            if (st != 0) {
-                rays++;
-                if (st > 0) atomicAdd(&ring_occ[wave][slot], 1u);
-                const uint32_t done = atomicAdd(&ring_done[wave][slot], 1u);
-                if (done == my_ns - 1) {
-                    // Last ray of the lumel: every earlier increment of this slot precedes this
-                    // one in the wavefront's LDS order; read, publish, clear.
-                    *counter = atomicExch(&ring_occ[wave][slot], 0u);
-                    atomicExch(&ring_done[wave][slot], 0u);
+                                                               min_lanes, hit);
+            if (st > 0) {
+                // Record the occluder as the lumel's first hint (the previous one becomes the
+                // second) while the row still holds this ray's lumel.
+                uint32_t* w = reinterpret_cast<uint32_t*>(lumel_rows[wave][my_lc]);
+                if (w[3] == my_key) {
+                    const uint32_t h0 = w[2 * 4 + 3];
+                    if (h0 != hit) {
+                        w[3 * 4 + 3] = h0;
+                        w[2 * 4 + 3] = hit;
+                    }
                 }
-                active = false;
             }
+            if (st != 0) finish(st > 0);
         }
     }
     count_rays(rays, rays_out);

@@ -477,11 +477,12 @@ __device__ __forceinline__ int any_hit_step(const SceneDev& S, const TopCache& t
 // each searching lane, so repeated calls always finish). Returns like any_hit_step; the set of
 // nodes and triangles a ray visits is the same (only the grouping into steps differs), so the
 // occlusion result is the same.
-// UNI: the caller has checked one_scene(S0). VT: nodes may be virtual (local trees).
+// UNI: the caller has checked one_scene(S0). VT: nodes may be virtual (local trees). On a hit,
+// hit_tri receives the occluding triangle's leaf-order index.
 template <int CAP, bool UNI, bool VT = false>
 __device__ __forceinline__ int any_hit_run(const SceneDev& S0, const TopCache& top, const TravRay& r, uint32_t& node,
                                            int& sp, uint32_t* stk, uint32_t stride, uint32_t* spill,
-                                           uint32_t min_lanes) {
+                                           uint32_t min_lanes, uint32_t& hit_tri) {
     const SceneDev S = UNI ? uniform_scene(S0) : S0;
     bool ended = false;
     for (uint32_t stop_at = 0;; stop_at = min_lanes) {
@@ -505,8 +506,14 @@ __device__ __forceinline__ int any_hit_run(const SceneDev& S0, const TopCache& t
         const V3 a0 = v3(ta[0], ta[1], ta[2]), a1 = v3(ta[4], ta[5], ta[6]), a2 = v3(ta[8], ta[9], ta[10]);
         const V3 b0 = v3(tb[0], tb[1], tb[2]), b1 = v3(tb[4], tb[5], tb[6]), b2 = v3(tb[8], tb[9], tb[10]);
         float t, U, V, W, det;
-        if (wt_intersect(r.w, a0, a1, a2, t, U, V, W, det)) return 1;
-        if (wt_intersect(r.w, b0, b1, b2, t, U, V, W, det)) return 1;
+        if (wt_intersect(r.w, a0, a1, a2, t, U, V, W, det)) {
+            hit_tri = cur;
+            return 1;
+        }
+        if (wt_intersect(r.w, b0, b1, b2, t, U, V, W, det)) {
+            hit_tri = c2;
+            return 1;
+        }
         cur = c2;
         if (++cur == end) {
             if (sp == 0) return -1;
@@ -516,6 +523,15 @@ __device__ __forceinline__ int any_hit_run(const SceneDev& S0, const TopCache& t
             end = cur + ((node >> 27) & 15u) + 1u;
         }
     }
+}
+
+// Whether the ray hits triangle `tri` (leaf order) in its interval: the watertight test of the
+// traversal's leaves, so a hit here is a hit the traversal would also find.
+__device__ __forceinline__ bool hits_triangle(const SceneDev& S, const TravRay& r, uint32_t tri) {
+    const gfloat* ta = reinterpret_cast<const gfloat*>(gmem(S.tris) + 3u * tri);
+    float t, U, V, W, det;
+    return wt_intersect(r.w, v3(ta[0], ta[1], ta[2]), v3(ta[4], ta[5], ta[6]), v3(ta[8], ta[9], ta[10]), t, U, V, W,
+                        det);
 }
 
 // One round of closest-hit traversal: a node phase, then the triangles of the leaves it found.
