@@ -353,13 +353,14 @@ constexpr uint32_t kAoNodeExit = 8;
 // tables 159 KB) serves all of them: city AO 360 ms with four-wavefront workgroups and 152 nodes,
 // 350 ms with eight and 304, 352/347 ms with sixteen and 400/600.
 constexpr int kAoBlock = 1024;
-constexpr uint32_t kAoTopNodes = 360;
+constexpr uint32_t kAoTopNodes = 328;
 // Each wavefront builds a local tree (rl_traverse.h) for a chunk of at least kVtMinRays rays, over
 // the union of the boxes its lumels' AO segments stay in, and starts the chunk's rays at its
 // virtual root. Rays of several chunks can be in flight at once, so a wavefront keeps three trees
 // and builds into one only when no ray of the chunk that used it last is still in flight; otherwise
 // (and for smaller chunks) the chunk's rays start at the scene's root. LDS: 30 KB of virtual
-// nodes, paid for with the top-node cache (600 -> 360 nodes), which local trees mostly skip.
+// nodes, paid for with the top-node cache (600 -> 328 nodes with the hint rows below), which local
+// trees mostly skip.
 constexpr uint32_t kVtBuffers = 3;
 constexpr uint32_t kVtMinRays = 64;
 // An empty occluder hint (above every triangle index).
@@ -400,8 +401,8 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
     __shared__ u32x4 top_rows[(kAoTopNodes + kAoBlock / 64 * kVtBuffers * kVtNodes) * kNodeRows];
     // Per wavefront, the current chunk's lumels: position and texel key, cosine-sampling frame
     // (t1, t2, n), computed once per lumel when the chunk is taken instead of once per ray, and
-    // the lumel's two occluder hints (the w words of rows 2 and 3).
-    __shared__ u32x4 lumel_rows[kAoBlock / 64][kAoChunkMax][4];
+    // the lumel's three occluder hints (row 4).
+    __shared__ u32x4 lumel_rows[kAoBlock / 64][kAoChunkMax][5];
     const SceneDev top_scene = segs[0].scene;
     const uint32_t top_count = load_top_nodes(top_scene, top_rows, kAoTopNodes);
     uint32_t* stk = stack + threadIdx.x;
@@ -425,7 +426,7 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
     TravRay tr;
     SceneDev scene;
     uint32_t node = 0, slot = 0, my_stream = 0, my_ns = 1;
-    uint32_t hint0 = kNoHint, hint1 = kNoHint, my_lc = 0, my_key = 0;
+    uint32_t hint0 = kNoHint, hint1 = kNoHint, hint2 = kNoHint, my_lc = 0, my_key = 0;
     int sp = 0;
     uint32_t* counter = nullptr;
     uint32_t rays = 0;
@@ -482,8 +483,9 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
                         u32x4* row = lumel_rows[wave][lane];
                         row[0] = u32x4{f_bits(p4.x), f_bits(p4.y), f_bits(p4.z), (uint32_t)key};
                         row[1] = u32x4{f_bits(f.t1.x), f_bits(f.t1.y), f_bits(f.t1.z), (uint32_t)(key >> 32)};
-                        row[2] = u32x4{f_bits(f.t2.x), f_bits(f.t2.y), f_bits(f.t2.z), kNoHint};
-                        row[3] = u32x4{f_bits(n4.x), f_bits(n4.y), f_bits(n4.z), kNoHint};
+                        row[2] = u32x4{f_bits(f.t2.x), f_bits(f.t2.y), f_bits(f.t2.z), 0u};
+                        row[3] = u32x4{f_bits(n4.x), f_bits(n4.y), f_bits(n4.z), 0u};
+                        row[4] = u32x4{kNoHint, kNoHint, kNoHint, 0u};
                         q = ao_segment_box(v3(p4.x, p4.y, p4.z), f, sg.ao.max_distance,
                                            sg.ao.sampling == SAMPLE_COSINE);
                     }
@@ -559,8 +561,13 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
                     my_ns = chunk_ns;
                     slot = my_stream % (uint32_t)kRing;
                     counter = sg.counts + li;
-                    hint0 = r2.w;
-                    hint1 = r3.w;
+                    // A lumel none of whose rays has been occluded yet starts from the hints of
+                    // the lumel before it (the previous texel of its lightmap row, mostly).
+                    u32x4 h = row[4];
+                    if (h.x == kNoHint && lc > 0) h = lumel_rows[wave][lc - 1][4];
+                    hint0 = h.x;
+                    hint1 = h.y;
+                    hint2 = h.z;
                     my_lc = lc;
                     my_key = r0.w;
                     active = true;
@@ -576,7 +583,7 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
                 }
                 want &= ~__ballot(mine);
             }
-            // Occluder hints: a new ray first tests the last two triangles that occluded rays of
+            // Occluder hints: a new ray first tests the last three triangles that occluded rays of
             // its lumel. A hit there is a hit the traversal would also find (the same watertight
             // test in the ray's interval), so the ray is occluded and completes at once; the lanes
             // it frees take new rays before the wavefront traverses. Hints only choose which
@@ -586,6 +593,7 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
                 fresh = false;
                 if (hint0 < scene.num_tris) by_hint = hits_triangle(scene, tr, hint0);
                 if (!by_hint && hint1 < scene.num_tris) by_hint = hits_triangle(scene, tr, hint1);
+                if (!by_hint && hint2 < scene.num_tris) by_hint = hits_triangle(scene, tr, hint2);
                 if (by_hint) finish(true);
             }
             if (__ballot(by_hint) != 0ull && pool_open && (uint32_t)__popcll(__ballot(!active)) >= refill_min)
@@ -603,14 +611,15 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
                                : any_hit_run<CAP, false, true>(scene, top, tr, node, sp, stk, kAoBlock, spill,
                                                                min_lanes, hit);
             if (st > 0) {
-                // Record the occluder as the lumel's first hint (the previous one becomes the
-                // second) while the row still holds this ray's lumel.
+                // Move the occluder to the front of the lumel's hints (most recent first) while the
+                // row still holds this ray's lumel.
                 uint32_t* w = reinterpret_cast<uint32_t*>(lumel_rows[wave][my_lc]);
                 if (w[3] == my_key) {
-                    const uint32_t h0 = w[2 * 4 + 3];
+                    const uint32_t h0 = w[4 * 4 + 0], h1 = w[4 * 4 + 1];
                     if (h0 != hit) {
-                        w[3 * 4 + 3] = h0;
-                        w[2 * 4 + 3] = hit;
+                        if (h1 != hit) w[4 * 4 + 2] = h1;
+                        w[4 * 4 + 1] = h0;
+                        w[4 * 4 + 0] = hit;
                     }
                 }
             }
