@@ -22,6 +22,8 @@ namespace {
 constexpr int kBlock = 128;  // two wavefronts
 constexpr int kDirectBlock = 768;  // direct light: twelve wavefronts, one workgroup per compute unit
 using namespace trav;
+// An empty occluder hint (above every triangle index).
+constexpr uint32_t kNoHint = 0xFFFFFFFFu;
 
 // LightInfluence::calculate / DirectionalLightInfluence::calculate with LightCutoff and
 // LinearLightAttenuation, i.e. DirectLight::influenceFromPoint (baker/DirectLight.cpp:101-125,
@@ -29,10 +31,17 @@ using namespace trav;
 // Whether the shadow segment r is blocked (any hit). A scalar-base variant for wavefronts of one
 // scene (as the AO kernel has) raised this kernel to 171 VGPRs, two wavefronts per SIMD, and
 // measured 25.7 ms against 21.2 ms on the city.
+// `hint` is the leaf-order index of the triangle that blocked the lane's previous shadow segment
+// (or above every index): it is tested first, and a hit there is a hit the traversal would also
+// find, so the result does not depend on it; a blocking triangle found by traversal replaces it.
 template <int CAP>
-__device__ __forceinline__ bool occluded(const SceneDev& S, const TopCache& top, const Ray& r, uint32_t* stk) {
+__device__ __forceinline__ bool occluded(const SceneDev& S, const TopCache& top, const Ray& r, uint32_t* stk,
+                                         uint32_t& hint) {
+    if (hint < S.num_tris && hits_triangle(S, wt_setup(r), hint)) return true;
     TraceResult tr;
-    return traverse<true, CAP>(S, top, r, stk, kDirectBlock, tr);
+    const bool h = traverse<true, CAP>(S, top, r, stk, kDirectBlock, tr);
+    if (h) hint = tr.leaf;
+    return h;
 }
 
 // Embree::traceSegment with HitUseAlpha (rt/Embree.cpp:145-177), as the shadow tests of
@@ -60,16 +69,16 @@ __device__ bool occluded_alpha(const SceneDev& S, const TopCache& top, V3 start,
 // and then the closest hit exists exactly when any hit does) or the alpha re-trace.
 template <int CAP, bool ALPHA>
 __device__ __forceinline__ bool shadowed(const SceneDev& S, const TopCache& top, V3 p, V3 e, uint32_t* stk,
-                                         uint32_t& rays) {
+                                         uint32_t& rays, uint32_t& hint) {
     if (ALPHA) return occluded_alpha<CAP>(S, top, p, e, stk, rays);
     Ray r = ray_from_segment(p, e);
     rays++;
-    return occluded<CAP>(S, top, r, stk);
+    return occluded<CAP>(S, top, r, stk, hint);
 }
 
 template <int CAP, bool ALPHA>
 __device__ float influence_from_point(const SceneDev& S, const TopCache& top, const LightDev& L, V3 lp, V3 p, V3 n,
-                                      uint32_t* stk, uint32_t& rays) {
+                                      uint32_t* stk, uint32_t& rays, uint32_t& hint) {
     float inf = 0.0f, att = 1.0f, cut = 1.0f, distance = 0.0f;
     if (L.type == LIGHT_DIRECTIONAL) {
         V3 ld = v3(L.dir[0], L.dir[1], L.dir[2]);
@@ -77,7 +86,7 @@ __device__ float influence_from_point(const SceneDev& S, const TopCache& top, co
         if (inf <= 0.001f) {
             inf = 0.0f;
         } else if (L.casts_shadow) {
-            inf *= shadowed<CAP, ALPHA>(S, top, p, p - ld * 1000.0f, stk, rays) ? 0.0f : 1.0f;
+            inf *= shadowed<CAP, ALPHA>(S, top, p, p - ld * 1000.0f, stk, rays, hint) ? 0.0f : 1.0f;
         }
     } else {
         V3 dir = lp - p;
@@ -86,7 +95,7 @@ __device__ float influence_from_point(const SceneDev& S, const TopCache& top, co
         if (inf <= 0.001f) {
             inf = 0.0f;
         } else if (L.casts_shadow) {
-            inf *= shadowed<CAP, ALPHA>(S, top, p, lp, stk, rays) ? 0.0f : 1.0f;
+            inf *= shadowed<CAP, ALPHA>(S, top, p, lp, stk, rays, hint) ? 0.0f : 1.0f;
         }
     }
     if (L.has_spot) {
@@ -122,7 +131,7 @@ __global__ __launch_bounds__(kDirectBlock) void k_bake_direct(const BakeSeg* __r
     __syncthreads();
     uint32_t* stk = stack + threadIdx.x;
     const uint64_t nchunks = (total + 63) / 64;
-    uint32_t rays = 0;
+    uint32_t rays = 0, hint = kNoHint;  // hint: the lane's last blocking triangle
     for (;;) {
         const uint64_t chunk = wave_next_chunk(queue);
         if (chunk >= nchunks) break;
@@ -144,7 +153,7 @@ __global__ __launch_bounds__(kDirectBlock) void k_bake_direct(const BakeSeg* __r
                 for (int v = 0; v < L.num_vertices; v++) {
                     const float4 lv = sg.lverts[L.vertex_offset + v];
                     V3 lp = v3(lv.x + L.pos[0], lv.y + L.pos[1], lv.z + L.pos[2]);
-                    float inf = influence_from_point<CAP, ALPHA>(S, top, L, lp, p, n, stk, rays);
+                    float inf = influence_from_point<CAP, ALPHA>(S, top, L, lp, p, n, stk, rays, hint);
                     if (inf > 0.0f) {
                         cr += L.color[0] * L.intensity * inf;
                         cg += L.color[1] * L.intensity * inf;
@@ -155,7 +164,8 @@ __global__ __launch_bounds__(kDirectBlock) void k_bake_direct(const BakeSeg* __r
                 cr = cr / nv; cg = cg / nv; cb = cb / nv;
             } else {
                 // DirectLight::lightFromPoint (baker/DirectLight.cpp:63-72).
-                float inf = influence_from_point<CAP, ALPHA>(S, top, L, v3(L.pos[0], L.pos[1], L.pos[2]), p, n, stk, rays);
+                float inf = influence_from_point<CAP, ALPHA>(S, top, L, v3(L.pos[0], L.pos[1], L.pos[2]), p, n, stk, rays,
+                                                             hint);
                 if (inf > 0.0f) {
                     cr = L.color[0] * L.intensity * inf;
                     cg = L.color[1] * L.intensity * inf;
@@ -201,7 +211,7 @@ __global__ __launch_bounds__(kDirectBlock) void k_direct_pairs(const BakeSeg* __
     __syncthreads();
     uint32_t* stk = stack + threadIdx.x;
     const uint64_t nchunks = (total + 63) / 64;
-    uint32_t rays = 0;
+    uint32_t rays = 0, hint = kNoHint;  // hint: the lane's last blocking triangle
     for (;;) {
         const uint64_t chunk = wave_next_chunk(queue);
         if (chunk >= nchunks) break;
@@ -234,7 +244,7 @@ __global__ __launch_bounds__(kDirectBlock) void k_direct_pairs(const BakeSeg* __
                 const float4 lv = sg.lverts[L.vertex_offset + v];
                 lp = v3(lv.x + L.pos[0], lv.y + L.pos[1], lv.z + L.pos[2]);
             }
-            inf = influence_from_point<CAP, ALPHA>(S, top, L, lp, p, n, stk, rays);
+            inf = influence_from_point<CAP, ALPHA>(S, top, L, lp, p, n, stk, rays, hint);
         }
         inf_out[g] = inf;
     }
@@ -363,8 +373,6 @@ constexpr uint32_t kAoTopNodes = 328;
 // trees mostly skip.
 constexpr uint32_t kVtBuffers = 3;
 constexpr uint32_t kVtMinRays = 64;
-// An empty occluder hint (above every triangle index).
-constexpr uint32_t kNoHint = 0xFFFFFFFFu;
 
 // The box that the AO segments of a lumel stay in: from p, length D, directions of the sampler.
 // Cosine: d = t1 x + t2 y + n z with x^2 + y^2 <= 1 and 0 <= z <= 1, so each component of d lies

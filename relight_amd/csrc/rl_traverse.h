@@ -65,6 +65,7 @@ struct TraceResult {
     float t;
     uint32_t tri;  // scene-order id, 0xFFFFFFFF on a miss
     float U, V, W, det;
+    uint32_t leaf;  // any-hit traversal: the hit triangle's leaf-order index
 };
 
 __device__ __forceinline__ void cswap(float& ka, uint32_t& va, float& kb, uint32_t& vb) {
@@ -527,11 +528,14 @@ __device__ __forceinline__ int any_hit_run(const SceneDev& S0, const TopCache& t
 
 // Whether the ray hits triangle `tri` (leaf order) in its interval: the watertight test of the
 // traversal's leaves, so a hit here is a hit the traversal would also find.
-__device__ __forceinline__ bool hits_triangle(const SceneDev& S, const TravRay& r, uint32_t tri) {
+__device__ __forceinline__ bool hits_triangle(const SceneDev& S, const WtRay& w, uint32_t tri) {
     const gfloat* ta = reinterpret_cast<const gfloat*>(gmem(S.tris) + 3u * tri);
     float t, U, V, W, det;
-    return wt_intersect(r.w, v3(ta[0], ta[1], ta[2]), v3(ta[4], ta[5], ta[6]), v3(ta[8], ta[9], ta[10]), t, U, V, W,
-                        det);
+    return wt_intersect(w, v3(ta[0], ta[1], ta[2]), v3(ta[4], ta[5], ta[6]), v3(ta[8], ta[9], ta[10]), t, U, V, W, det);
+}
+
+__device__ __forceinline__ bool hits_triangle(const SceneDev& S, const TravRay& r, uint32_t tri) {
+    return hits_triangle(S, r.w, tri);
 }
 
 // One round of closest-hit traversal: a node phase, then the triangles of the leaves it found.
@@ -664,6 +668,7 @@ __device__ bool traverse(const SceneDev& S, const TopCache& top, const Ray& ray,
                 if (ANY) {
                     res.t = t;
                     res.tri = id;
+                    res.leaf = cur;
                     return true;
                 }
                 if (t < res.t || (t == res.t && id < res.tri)) {
