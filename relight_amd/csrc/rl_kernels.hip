@@ -363,14 +363,14 @@ constexpr uint32_t kAoNodeExit = 8;
 // tables 159 KB) serves all of them: city AO 360 ms with four-wavefront workgroups and 152 nodes,
 // 350 ms with eight and 304, 352/347 ms with sixteen and 400/600.
 constexpr int kAoBlock = 1024;
-constexpr uint32_t kAoTopNodes = 328;
+constexpr uint32_t kAoTopNodes = 296;
 // Each wavefront builds a local tree (rl_traverse.h) for a chunk of at least kVtMinRays rays, over
 // the union of the boxes its lumels' AO segments stay in, and starts the chunk's rays at its
 // virtual root. Rays of several chunks can be in flight at once, so a wavefront keeps three trees
 // and builds into one only when no ray of the chunk that used it last is still in flight; otherwise
 // (and for smaller chunks) the chunk's rays start at the scene's root. LDS: 30 KB of virtual
-// nodes, paid for with the top-node cache (600 -> 328 nodes with the hint rows below), which local
-// trees mostly skip.
+// nodes, paid for with the top-node cache (600 -> 296 nodes with the hint rows and ring records
+// below), which local trees mostly skip.
 constexpr uint32_t kVtBuffers = 3;
 constexpr uint32_t kVtMinRays = 64;
 
@@ -403,23 +403,24 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
                                                       uint32_t refill_min, unsigned long long* queue,
                                                       unsigned long long* rays_out) {
     __shared__ uint32_t stack[kLdsStack * kAoBlock];
+    // Per wavefront and ring slot (one lumel in flight): occluded rays so far, rays still to
+    // finish, and where the lumel's count goes. A lumel's first ray sets them up.
     __shared__ uint32_t ring_occ[kAoBlock / 64][kRing];
-    __shared__ uint32_t ring_done[kAoBlock / 64][kRing];
+    __shared__ uint32_t ring_left[kAoBlock / 64][kRing];
+    __shared__ uint32_t* ring_out[kAoBlock / 64][kRing];
     // The scene's first kAoTopNodes nodes, then the wavefronts' local trees (virtual nodes).
     __shared__ u32x4 top_rows[(kAoTopNodes + kAoBlock / 64 * kVtBuffers * kVtNodes) * kNodeRows];
-    // Per wavefront, the current chunk's lumels: position and texel key, cosine-sampling frame
-    // (t1, t2, n), computed once per lumel when the chunk is taken instead of once per ray, and
-    // the lumel's three occluder hints (row 4).
+    // Per wavefront, rows of the lumels of the chunks taken, lumel stream number s in row
+    // s % kAoChunkMax: position and texel key, cosine-sampling frame (t1, t2, n), computed once
+    // per lumel when its chunk is taken instead of once per ray, and row 4: the lumel's three
+    // occluder hints and its stream number (which tells whether a later chunk has reused the row).
     __shared__ u32x4 lumel_rows[kAoBlock / 64][kAoChunkMax][5];
     const SceneDev top_scene = segs[0].scene;
     const uint32_t top_count = load_top_nodes(top_scene, top_rows, kAoTopNodes);
     uint32_t* stk = stack + threadIdx.x;
     uint32_t spill[CAP - kLdsStack];
     const uint32_t lane = lane_id(), wave = threadIdx.x >> 6;
-    for (uint32_t k = lane; k < (uint32_t)kRing; k += 64) {
-        ring_occ[wave][k] = 0u;
-        ring_done[wave][k] = 0u;
-    }
+
     // Pool state, identical in every lane of the wavefront.
     int pseg = 0;
     uint32_t chunk_first = 0, chunk_count = 0, chunk_ns = 1;
@@ -429,27 +430,24 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
     // Local trees: trees built, each buffer's end stream (one past its chunk's last lumel), the
     // current chunk's start node.
     uint32_t vt_seq = 0, vt_end0 = 0, vt_end1 = 0, vt_end2 = 0, chunk_root = 0;
+    uint32_t rays = 0;  // rays finished by the wavefront
     // Lane state.
     bool active = false, fresh = false;
     TravRay tr;
     SceneDev scene;
-    uint32_t node = 0, slot = 0, my_stream = 0, my_ns = 1;
-    uint32_t hint0 = kNoHint, hint1 = kNoHint, hint2 = kNoHint, my_lc = 0, my_key = 0;
+    uint32_t node = 0, my_stream = 0;
+    uint32_t hint0 = kNoHint, hint1 = kNoHint, hint2 = kNoHint;
     int sp = 0;
-    uint32_t* counter = nullptr;
-    uint32_t rays = 0;
     TopCache top{top_rows, 0u};
-    // A lane's ray is done: count it, add it to its lumel's ring slot, and publish the lumel's
-    // occlusion count when this was its last ray.
+    // A lane's ray is done: add it to its lumel's ring slot and publish the lumel's occlusion count
+    // when this was its last ray.
     auto finish = [&](bool occluded) {
-        rays++;
+        const uint32_t slot = my_stream % (uint32_t)kRing;
         if (occluded) atomicAdd(&ring_occ[wave][slot], 1u);
-        const uint32_t done = atomicAdd(&ring_done[wave][slot], 1u);
-        if (done == my_ns - 1) {
+        if (atomicSub(&ring_left[wave][slot], 1u) == 1u) {
             // Last ray of the lumel: every earlier increment of this slot precedes this one in
-            // the wavefront's LDS order; read, publish, clear.
-            *counter = atomicExch(&ring_occ[wave][slot], 0u);
-            atomicExch(&ring_done[wave][slot], 0u);
+            // the wavefront's LDS order.
+            *ring_out[wave][slot] = atomicAdd(&ring_occ[wave][slot], 0u);
         }
         active = false;
     };
@@ -485,15 +483,16 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
                                   {-__builtin_inff(), -__builtin_inff(), -__builtin_inff()}};
                     if (lane < chunk_count) {  // chunk_count <= kAoChunkMax (launcher)
                         const uint32_t li = chunk_first + lane;
+                        const uint32_t stream = stream_base + lane;
                         const float4 p4 = sg.pos[li], n4 = sg.nrm[li];
                         const uint64_t key = texel_key(sg.ao.seed, sg.lm_seed, f_bits(p4.w));
                         const CosineFrame f = cosine_frame(v3(n4.x, n4.y, n4.z));
-                        u32x4* row = lumel_rows[wave][lane];
+                        u32x4* row = lumel_rows[wave][stream % kAoChunkMax];
                         row[0] = u32x4{f_bits(p4.x), f_bits(p4.y), f_bits(p4.z), (uint32_t)key};
                         row[1] = u32x4{f_bits(f.t1.x), f_bits(f.t1.y), f_bits(f.t1.z), (uint32_t)(key >> 32)};
                         row[2] = u32x4{f_bits(f.t2.x), f_bits(f.t2.y), f_bits(f.t2.z), 0u};
                         row[3] = u32x4{f_bits(n4.x), f_bits(n4.y), f_bits(n4.z), 0u};
-                        row[4] = u32x4{kNoHint, kNoHint, kNoHint, 0u};
+                        row[4] = u32x4{kNoHint, kNoHint, kNoHint, stream};
                         q = ao_segment_box(v3(p4.x, p4.y, p4.z), f, sg.ao.max_distance,
                                            sg.ao.sampling == SAMPLE_COSINE);
                     }
@@ -550,7 +549,8 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
                         si = t % chunk_ns;
                     }
                     const uint32_t li = chunk_first + lc;
-                    const u32x4* row = lumel_rows[wave][lc];
+                    my_stream = stream_base + lc;
+                    const u32x4* row = lumel_rows[wave][my_stream % kAoChunkMax];
                     const u32x4 r0 = row[0], r1 = row[1], r2 = row[2], r3 = row[3];
                     const V3 p = v3(bits_f(r0.x), bits_f(r0.y), bits_f(r0.z));
                     CosineFrame f;
@@ -565,19 +565,25 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
                     top.count = sg.scene.nodes == top_scene.nodes ? top_count : 0u;
                     node = chunk_root;
                     sp = 0;
-                    my_stream = stream_base + lc;
-                    my_ns = chunk_ns;
-                    slot = my_stream % (uint32_t)kRing;
-                    counter = sg.counts + li;
+                    if (si == 0) {
+                        // The lumel's first ray: its ring slot is free (a ray starts only lumels
+                        // fewer than kRing - 2 places after the oldest one in flight).
+                        const uint32_t slot = my_stream % (uint32_t)kRing;
+                        ring_occ[wave][slot] = 0u;
+                        ring_left[wave][slot] = chunk_ns;
+                        ring_out[wave][slot] = sg.counts + li;
+                    }
                     // A lumel none of whose rays has been occluded yet starts from the hints of
-                    // the lumel before it (the previous texel of its lightmap row, mostly).
+                    // the lumel before it (the previous texel of its lightmap row, mostly), if its
+                    // row still holds them.
                     u32x4 h = row[4];
-                    if (h.x == kNoHint && lc > 0) h = lumel_rows[wave][lc - 1][4];
+                    if (h.x == kNoHint) {
+                        const u32x4 hp = lumel_rows[wave][(my_stream - 1u) % kAoChunkMax][4];
+                        if (hp.w == my_stream - 1u) h = hp;
+                    }
                     hint0 = h.x;
                     hint1 = h.y;
                     hint2 = h.z;
-                    my_lc = lc;
-                    my_key = r0.w;
                     active = true;
                     fresh = true;
                 }
@@ -604,12 +610,14 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
                 if (!by_hint && hint2 < scene.num_tris) by_hint = hits_triangle(scene, tr, hint2);
                 if (by_hint) finish(true);
             }
-            if (__ballot(by_hint) != 0ull && pool_open && (uint32_t)__popcll(__ballot(!active)) >= refill_min)
-                continue;
+            const uint64_t hinted = __ballot(by_hint);
+            rays += (uint32_t)__popcll(hinted);
+            if (hinted != 0ull && pool_open && (uint32_t)__popcll(__ballot(!active)) >= refill_min) continue;
             if (!pool_open && __ballot(active) == 0ull) break;
         } else if (!pool_open && idle == ~0ull) {
             break;
         }
+        bool ended = false;
         if (active) {
             const uint32_t min_lanes = pool_open ? kAoNodeExit : 0u;
             uint32_t hit = kNoHint;
@@ -621,8 +629,8 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
             if (st > 0) {
                 // Move the occluder to the front of the lumel's hints (most recent first) while the
                 // row still holds this ray's lumel.
-                uint32_t* w = reinterpret_cast<uint32_t*>(lumel_rows[wave][my_lc]);
-                if (w[3] == my_key) {
+                uint32_t* w = reinterpret_cast<uint32_t*>(lumel_rows[wave][my_stream % kAoChunkMax]);
+                if (w[4 * 4 + 3] == my_stream) {
                     const uint32_t h0 = w[4 * 4 + 0], h1 = w[4 * 4 + 1];
                     if (h0 != hit) {
                         if (h1 != hit) w[4 * 4 + 2] = h1;
@@ -632,9 +640,11 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
                 }
             }
             if (st != 0) finish(st > 0);
+            ended = st != 0;
         }
+        rays += (uint32_t)__popcll(__ballot(ended));
     }
-    count_rays(rays, rays_out);
+    if (lane == 0 && rays) atomicAdd(rays_out, (unsigned long long)rays);
 }
 
 // AO pass 2: AmbientOcclusion::bakeLumel's value and multiply (baker/AmbientOcclusion.cpp:59-65).
