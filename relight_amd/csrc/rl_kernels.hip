@@ -359,9 +359,10 @@ constexpr int kRing = 32;
 // leaf (city AO: 399 ms with no early stop, 397/393/387/384/387/398 ms at 1/2/4/8/16/24 lanes).
 constexpr uint32_t kAoNodeExit = 8;
 // One workgroup of sixteen wavefronts per compute unit (the occupancy the kernel's registers
-// allow), so one LDS copy of the scene's first 600 nodes (75 KB; with the stacks, rings and lumel
-// tables 159 KB) serves all of them: city AO 360 ms with four-wavefront workgroups and 152 nodes,
-// 350 ms with eight and 304, 352/347 ms with sixteen and 400/600.
+// allow), so one LDS copy of the scene's top nodes serves all of them: before local trees, city AO
+// 360 ms with four-wavefront workgroups and 152 nodes, 350 ms with eight and 304, 352/347 ms with
+// sixteen and 400/600. With local trees, 768-thread workgroups (no register spills, 560 nodes)
+// measured 256.6 ms against 237.8 ms for sixteen wavefronts.
 constexpr int kAoBlock = 1024;
 constexpr uint32_t kAoTopNodes = 296;
 // Each wavefront builds a local tree (rl_traverse.h) for a chunk of at least kVtMinRays rays, over
