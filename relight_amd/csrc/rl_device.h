@@ -4,7 +4,12 @@
 //   nodes : Bvh4Node[num_nodes]       128 B per node, child boxes stored SoA inside the node so
 //                                     one lane reads a node as eight 16-B loads.
 //   tris  : float4[3 * num_tris]      48 B per triangle in leaf order: v0, v1, v2; v0.w carries
-//                                     the scene-order triangle id (bit pattern of a uint32).
+//                                     the scene-order triangle id (bit pattern of a uint32). Read
+//                                     by the closest-hit traversals.
+//   tris_rot : float4[4 * num_tris]   the same triangles, 64 B each: vertex k as (x, y, z, x, y)
+//                                     at floats 5k .. 5k + 4 (a ray's watertight component order,
+//                                     a rotation of (x, y, z), is three consecutive floats), float
+//                                     15 the scene-order id. Read by the any-hit traversals.
 //   tri_attr : TriAttr[num_tris]      scene-order attributes used only after a closest hit.
 // Lumels of a lightmap are stored compacted (valid texels only) as three float4 streams: position
 // (w = texel index y*width+x), normal (w = scene-order triangle id), colour (w unused).
@@ -60,6 +65,7 @@ struct TriMat {
 struct SceneDev {
     const Bvh4Node* nodes;
     const float4* tris;
+    const float4* tris_rot;
     const TriAttr* attr;
     uint32_t num_nodes;
     uint32_t num_tris;

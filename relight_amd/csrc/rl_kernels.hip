@@ -37,9 +37,10 @@ constexpr uint32_t kNoHint = 0xFFFFFFFFu;
 template <int CAP>
 __device__ __forceinline__ bool occluded(const SceneDev& S, const TopCache& top, const Ray& r, uint32_t* stk,
                                          uint32_t& hint) {
-    if (hint < S.num_tris && hits_triangle(S, wt_setup(r), hint)) return true;
+    const WtRay w = wt_setup(r);
+    if (hint < S.num_tris && hits_triangle(S, w, hint)) return true;
     TraceResult tr;
-    const bool h = traverse<true, CAP>(S, top, r, stk, kDirectBlock, tr);
+    const bool h = traverse<true, CAP>(S, top, r, w, stk, kDirectBlock, tr);
     if (h) hint = tr.leaf;
     return h;
 }

@@ -199,31 +199,41 @@ RL_HD Ray ray_from_segment(V3 start, V3 end) {
 }
 
 // Per-ray constants of the watertight ray/triangle test (Woop, Benthin, Wald, "Watertight
-// Ray/Triangle Intersection", JCGT 2(1), 2013): the dominant axis kz, the two others kx, ky
-// (swapped when d[kz] < 0 to keep the winding), and the shear constants.
+// Ray/Triangle Intersection", JCGT 2(1), 2013): the dominant axis kz, the two others
+// kx = kz + 1 and ky = kz + 2 (mod 3), and the shear constants. The paper swaps kx and ky when
+// d[kz] < 0 to keep the winding; the swap exchanges the two sheared coordinates, which negates U,
+// V, W, det and T exactly (each is an antisymmetric expression of them, and IEEE subtraction is
+// antisymmetric), so the hit, t and the ratios V/det, W/det are the same without it. Without the
+// swap the component order (kx, ky, kz) is a rotation of (x, y, z) by kx, and the origin is kept
+// in that order: rot 0 = (x, y, z), 1 = (y, z, x), 2 = (z, x, y).
 struct WtRay {
-    V3 o;
-    int kx, ky, kz;
+    V3 o;          // origin components in the order (kx, ky, kz)
+    uint32_t rot;  // kx
     float Sx, Sy, Sz;
     float tnear, tfar;
 };
 
 RL_HD WtRay wt_setup(const Ray& r) {
     WtRay w;
-    w.o = r.o;
     float ax = fabsf(r.d.x), ay = fabsf(r.d.y), az = fabsf(r.d.z);
     int kz = ax > ay ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
     int kx = kz == 2 ? 0 : kz + 1;
     int ky = kx == 2 ? 0 : kx + 1;
     float dz = comp(r.d, kz);
-    if (dz < 0.0f) { int t = kx; kx = ky; ky = t; }
-    w.kx = kx; w.ky = ky; w.kz = kz;
+    w.o = v3(comp(r.o, kx), comp(r.o, ky), comp(r.o, kz));
+    w.rot = (uint32_t)kx;
     w.Sx = comp(r.d, kx) / dz;
     w.Sy = comp(r.d, ky) / dz;
     w.Sz = 1.0f / dz;
     w.tnear = r.tnear;
     w.tfar = r.tfar;
     return w;
+}
+
+// A point's components in the ray's order (kx, ky, kz), selected component by component.
+RL_HD V3 wt_rotate(const WtRay& r, V3 p) {
+    const bool r0 = r.rot == 0, r1 = r.rot == 1;
+    return v3(r0 ? p.x : (r1 ? p.y : p.z), r0 ? p.y : (r1 ? p.z : p.x), r0 ? p.z : (r1 ? p.x : p.y));
 }
 
 // The watertight test from the vertices relative to the ray origin, each given by its kx, ky and
@@ -255,12 +265,18 @@ RL_HD bool wt_finish(const WtRay& r, float Akx, float Aky, float Akz, float Bkx,
     return t > r.tnear && t < r.tfar;
 }
 
-// Returns true and the hit distance t when the triangle is hit with tnear < t < tfar.
+// Returns true and the hit distance t when the triangle is hit with tnear < t < tfar; the
+// vertices are given with their components in the ray's order (wt_rotate).
+RL_HD bool wt_intersect_rotated(const WtRay& r, V3 v0, V3 v1, V3 v2, float& t, float& U, float& V,
+                                float& W, float& det) {
+    V3 A = v0 - r.o, B = v1 - r.o, C = v2 - r.o;
+    return wt_finish(r, A.x, A.y, A.z, B.x, B.y, B.z, C.x, C.y, C.z, t, U, V, W, det);
+}
+
+// The same for vertices in (x, y, z) order.
 RL_HD bool wt_intersect(const WtRay& r, V3 v0, V3 v1, V3 v2, float& t, float& U, float& V,
                         float& W, float& det) {
-    V3 A = v0 - r.o, B = v1 - r.o, C = v2 - r.o;
-    return wt_finish(r, comp(A, r.kx), comp(A, r.ky), comp(A, r.kz), comp(B, r.kx), comp(B, r.ky), comp(B, r.kz),
-                     comp(C, r.kx), comp(C, r.ky), comp(C, r.kz), t, U, V, W, det);
+    return wt_intersect_rotated(r, wt_rotate(r, v0), wt_rotate(r, v1), wt_rotate(r, v2), t, U, V, W, det);
 }
 
 // ---------------------------------------------------------------- light models

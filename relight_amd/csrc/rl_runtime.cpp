@@ -155,6 +155,7 @@ struct rl_scene {
     float bounds_lo[3] = {0, 0, 0}, bounds_hi[3] = {0, 0, 0};
     Bvh4Node* d_nodes = nullptr;
     float4* d_tris = nullptr;
+    float4* d_tris_rot = nullptr;
     TriAttr* d_attr = nullptr;
     LightDev* d_lights = nullptr;
     float4* d_lverts = nullptr;
@@ -172,6 +173,7 @@ struct rl_scene {
         SceneDev s;
         s.nodes = d_nodes;
         s.tris = d_tris;
+        s.tris_rot = d_tris_rot;
         s.attr = d_attr;
         s.num_nodes = num_nodes;
         s.num_tris = num_tris;
@@ -182,6 +184,7 @@ struct rl_scene {
     ~rl_scene() {
         dev_free(d_nodes);
         dev_free(d_tris);
+        dev_free(d_tris_rot);
         dev_free(d_attr);
         dev_free(d_lights);
         dev_free(d_lverts);
@@ -1078,7 +1081,11 @@ int32_t rl_scene_end(rl_scene* scene) {
     if (rc) return fail(RL_REJECTED, "BVH build failed");
     int cap = stack_capacity_for(bvh.stack_bound);
     if (cap < 0) return fail(RL_REJECTED, "BVH traversal stack bound exceeds the LDS stack");
-    std::vector<float4> tris((size_t)ntri * 3);
+    // Triangles in leaf order twice: 48 B each (v0, v1, v2 as float4, the scene-order id in v0.w)
+    // for the closest-hit traversals, and 64 B each for the any-hit ones, vertex k as
+    // (x, y, z, x, y) at floats 5k .. 5k + 4 (a ray's watertight component order is a rotation of
+    // (x, y, z), read as three consecutive floats) and the id in float 15.
+    std::vector<float4> tris((size_t)ntri * 3), tris_rot((size_t)ntri * 4);
     for (uint32_t i = 0; i < ntri; i++) {
         const uint32_t t = bvh.order[i];
         const float* v = &tri9[9 * (size_t)t];
@@ -1087,6 +1094,16 @@ int32_t rl_scene_end(rl_scene* scene) {
         tris[3 * (size_t)i + 0] = make_float4(v[0], v[1], v[2], idw);
         tris[3 * (size_t)i + 1] = make_float4(v[3], v[4], v[5], 0.0f);
         tris[3 * (size_t)i + 2] = make_float4(v[6], v[7], v[8], 0.0f);
+        float* f = reinterpret_cast<float*>(&tris_rot[4 * (size_t)i]);
+        for (int k = 0; k < 3; k++) {
+            const float* p = v + 3 * k;
+            f[5 * k + 0] = p[0];
+            f[5 * k + 1] = p[1];
+            f[5 * k + 2] = p[2];
+            f[5 * k + 3] = p[0];
+            f[5 * k + 4] = p[1];
+        }
+        f[15] = idw;
     }
     // Textured meshes: per-triangle diffuse uvs and texture, and the textures' pixels.
     bool textured = false, alpha = false;
@@ -1138,9 +1155,11 @@ int32_t rl_scene_end(rl_scene* scene) {
     }
     RL_HIP(dev_alloc(&scene->d_nodes, bvh.nodes.size()));
     RL_HIP(dev_alloc(&scene->d_tris, tris.size()));
+    RL_HIP(dev_alloc(&scene->d_tris_rot, tris_rot.size()));
     RL_HIP(dev_alloc(&scene->d_attr, attr.size()));
     RL_HIP(hipMemcpy(scene->d_nodes, bvh.nodes.data(), bvh.nodes.size() * sizeof(Bvh4Node), hipMemcpyHostToDevice));
     RL_HIP(hipMemcpy(scene->d_tris, tris.data(), tris.size() * sizeof(float4), hipMemcpyHostToDevice));
+    RL_HIP(hipMemcpy(scene->d_tris_rot, tris_rot.data(), tris_rot.size() * sizeof(float4), hipMemcpyHostToDevice));
     RL_HIP(hipMemcpy(scene->d_attr, attr.data(), attr.size() * sizeof(TriAttr), hipMemcpyHostToDevice));
     if (!lights.empty()) {
         RL_HIP(dev_alloc(&scene->d_lights, lights.size()));

@@ -41,6 +41,30 @@ typedef const u32x4 __attribute__((address_space(1))) guint4;
 typedef const float __attribute__((address_space(1))) gfloat;
 __device__ __forceinline__ const gfloat4* gmem(const float4* p) { return (const gfloat4*)p; }
 
+// Triangles of the any-hit traversals (SceneDev::tris_rot), in leaf order, 64 B each: vertex k as
+// (x, y, z, x, y) at floats 5k .. 5k + 4, so its components in a ray's watertight order (wt_rotate:
+// a rotation by WtRay::rot) are the three floats from 5k + rot, one 12-B load with no selects.
+// The closest-hit traversals read the 48-B triangles (SceneDev::tris: v0, v1, v2 as float4, the
+// scene-order id in v0.w) and rotate the components with selects: the larger triangles measured
+// 4 % slower in the final gather, which waits on triangle and node fetches, while the AO kernel,
+// which issues VALU at the ceiling, gained 3 % from the 24 VALU fewer per triangle test.
+constexpr uint32_t kRotTriFloats = 16;
+
+__device__ __forceinline__ const gfloat* rtri_ptr(const float4* tris_rot, uint32_t tri) {
+    return reinterpret_cast<const gfloat*>(gmem(tris_rot)) + kRotTriFloats * tri;
+}
+
+__device__ __forceinline__ V3 rtri_vertex(const gfloat* t, uint32_t k, uint32_t rot) {
+    const gfloat* p = t + 5u * k + rot;
+    return v3(p[0], p[1], p[2]);
+}
+
+__device__ __forceinline__ uint32_t rtri_id(const gfloat* t) { return __float_as_uint(t[15]); }
+
+__device__ __forceinline__ const gfloat* tri_ptr(const float4* tris, uint32_t tri) {
+    return reinterpret_cast<const gfloat*>(gmem(tris) + 3u * tri);
+}
+
 // The scene's array addresses taken from the first active lane into scalar registers, for callers
 // whose active lanes all trace the same scene (one_scene): loads then use a scalar base and a
 // 32-bit per-lane offset instead of per-lane 64-bit address arithmetic.
@@ -48,6 +72,7 @@ __device__ __forceinline__ SceneDev uniform_scene(const SceneDev& S) {
     SceneDev U = S;
     U.nodes = reinterpret_cast<const Bvh4Node*>(wave_uniform(reinterpret_cast<uint64_t>(S.nodes)));
     U.tris = reinterpret_cast<const float4*>(wave_uniform(reinterpret_cast<uint64_t>(S.tris)));
+    U.tris_rot = reinterpret_cast<const float4*>(wave_uniform(reinterpret_cast<uint64_t>(S.tris_rot)));
     U.attr = reinterpret_cast<const TriAttr*>(wave_uniform(reinterpret_cast<uint64_t>(S.attr)));
     return U;
 }
@@ -458,10 +483,10 @@ __device__ __forceinline__ int any_hit_step(const SceneDev& S, const TopCache& t
     const uint32_t first = node & RL_LEAF_FIRST_MASK;
     const uint32_t count = ((node >> 27) & 15u) + 1u;
     for (uint32_t j = 0; j < count; j++) {
-        const gfloat* tf = reinterpret_cast<const gfloat*>(gmem(S.tris) + 3u * (first + j));
+        const gfloat* tf = rtri_ptr(S.tris_rot, first + j);
         float t, U, V, W, det;
-        if (wt_intersect(r.w, v3(tf[0], tf[1], tf[2]), v3(tf[4], tf[5], tf[6]), v3(tf[8], tf[9], tf[10]), t, U, V, W,
-                         det))
+        if (wt_intersect_rotated(r.w, rtri_vertex(tf, 0, r.w.rot), rtri_vertex(tf, 1, r.w.rot),
+                                 rtri_vertex(tf, 2, r.w.rot), t, U, V, W, det))
             return 1;
     }
     if (sp == 0) return -1;
@@ -496,22 +521,23 @@ __device__ __forceinline__ int any_hit_run(const SceneDev& S0, const TopCache& t
     uint32_t cur = node & RL_LEAF_FIRST_MASK;
     uint32_t end = cur + ((node >> 27) & 15u) + 1u;
     for (;;) {
-        // Two triangles per iteration, each as three 16-B row loads with per-lane component
-        // selects, both triangles' loads issued before either test (the last triangle of a leaf is
-        // tested twice when one is left): this loop is bound by vector-memory data return (city AO
-        // 346.5 -> 341.6 ms against one triangle per iteration; nine per-component loads of one
-        // triangle measured slower still).
+        // Two triangles per iteration, each as three 12-B loads of its vertices already in the
+        // ray's component order (rtri_vertex), both triangles' loads issued before either test (the
+        // last triangle of a leaf is tested twice when one is left): this loop is bound by
+        // vector-memory data return (city AO 346.5 -> 341.6 ms against one triangle per iteration;
+        // nine per-component loads of one triangle measured slower still).
         const uint32_t c2 = cur + 1u < end ? cur + 1u : cur;
-        const gfloat* ta = reinterpret_cast<const gfloat*>(gmem(S.tris) + 3u * cur);
-        const gfloat* tb = reinterpret_cast<const gfloat*>(gmem(S.tris) + 3u * c2);
-        const V3 a0 = v3(ta[0], ta[1], ta[2]), a1 = v3(ta[4], ta[5], ta[6]), a2 = v3(ta[8], ta[9], ta[10]);
-        const V3 b0 = v3(tb[0], tb[1], tb[2]), b1 = v3(tb[4], tb[5], tb[6]), b2 = v3(tb[8], tb[9], tb[10]);
+        const gfloat* ta = rtri_ptr(S.tris_rot, cur);
+        const gfloat* tb = rtri_ptr(S.tris_rot, c2);
+        const uint32_t rot = r.w.rot;
+        const V3 a0 = rtri_vertex(ta, 0, rot), a1 = rtri_vertex(ta, 1, rot), a2 = rtri_vertex(ta, 2, rot);
+        const V3 b0 = rtri_vertex(tb, 0, rot), b1 = rtri_vertex(tb, 1, rot), b2 = rtri_vertex(tb, 2, rot);
         float t, U, V, W, det;
-        if (wt_intersect(r.w, a0, a1, a2, t, U, V, W, det)) {
+        if (wt_intersect_rotated(r.w, a0, a1, a2, t, U, V, W, det)) {
             hit_tri = cur;
             return 1;
         }
-        if (wt_intersect(r.w, b0, b1, b2, t, U, V, W, det)) {
+        if (wt_intersect_rotated(r.w, b0, b1, b2, t, U, V, W, det)) {
             hit_tri = c2;
             return 1;
         }
@@ -529,9 +555,10 @@ __device__ __forceinline__ int any_hit_run(const SceneDev& S0, const TopCache& t
 // Whether the ray hits triangle `tri` (leaf order) in its interval: the watertight test of the
 // traversal's leaves, so a hit here is a hit the traversal would also find.
 __device__ __forceinline__ bool hits_triangle(const SceneDev& S, const WtRay& w, uint32_t tri) {
-    const gfloat* ta = reinterpret_cast<const gfloat*>(gmem(S.tris) + 3u * tri);
+    const gfloat* ta = rtri_ptr(S.tris_rot, tri);
     float t, U, V, W, det;
-    return wt_intersect(w, v3(ta[0], ta[1], ta[2]), v3(ta[4], ta[5], ta[6]), v3(ta[8], ta[9], ta[10]), t, U, V, W, det);
+    return wt_intersect_rotated(w, rtri_vertex(ta, 0, w.rot), rtri_vertex(ta, 1, w.rot), rtri_vertex(ta, 2, w.rot), t, U,
+                                V, W, det);
 }
 
 __device__ __forceinline__ bool hits_triangle(const SceneDev& S, const TravRay& r, uint32_t tri) {
@@ -589,8 +616,8 @@ __device__ __forceinline__ int closest_run(const SceneDev& S0, const TopCache& t
             // (heightfield final gather: nine per-component loads of one triangle 283 ms, three
             // row loads 261 ms, two triangles per iteration 252 ms, three 265 ms, four 287 ms).
             const uint32_t c2 = cur + 1u < end ? cur + 1u : cur;
-            const gfloat* ta = reinterpret_cast<const gfloat*>(gmem(S.tris) + 3u * cur);
-            const gfloat* tb = reinterpret_cast<const gfloat*>(gmem(S.tris) + 3u * c2);
+            const gfloat* ta = tri_ptr(S.tris, cur);
+            const gfloat* tb = tri_ptr(S.tris, c2);
             const V3 a0 = v3(ta[0], ta[1], ta[2]), a1 = v3(ta[4], ta[5], ta[6]), a2 = v3(ta[8], ta[9], ta[10]);
             const V3 b0 = v3(tb[0], tb[1], tb[2]), b1 = v3(tb[4], tb[5], tb[6]), b2 = v3(tb[8], tb[9], tb[10]);
             const uint32_t ida = f_bits(ta[3]), idb = f_bits(tb[3]);
@@ -634,10 +661,10 @@ __device__ __forceinline__ int closest_run(const SceneDev& S0, const TopCache& t
 // total stack capacity (LDS + spill), at least the BVH's worst-case bound.
 //
 // Slab test: as described at SlabRay.
+// w: wt_setup(ray), for callers that have it already.
 template <bool ANY, int CAP>
-__device__ bool traverse(const SceneDev& S, const TopCache& top, const Ray& ray, uint32_t* stk, uint32_t stride,
-                         TraceResult& res) {
-    const WtRay w = wt_setup(ray);
+__device__ bool traverse(const SceneDev& S, const TopCache& top, const Ray& ray, const WtRay& w, uint32_t* stk,
+                         uint32_t stride, TraceResult& res) {
     const SlabRay sr = slab_ray(ray);
     const float tnear = ray.tnear;
     float tlimit = ray.tfar;  // closest: shrinks to the best hit so far
@@ -660,11 +687,21 @@ __device__ bool traverse(const SceneDev& S, const TopCache& top, const Ray& ray,
         uint32_t cur = node & RL_LEAF_FIRST_MASK;
         uint32_t end = cur + ((node >> 27) & 15u) + 1u;
         for (;;) {
-            const gfloat* tf = reinterpret_cast<const gfloat*>(gmem(S.tris) + 3u * cur);
             float t, U, V, W, det;
-            if (wt_intersect(w, v3(tf[0], tf[1], tf[2]), v3(tf[4], tf[5], tf[6]), v3(tf[8], tf[9], tf[10]), t, U, V, W,
-                             det)) {
-                const uint32_t id = f_bits(tf[3]);
+            bool hit;
+            uint32_t id;
+            if (ANY) {
+                const gfloat* tf = rtri_ptr(S.tris_rot, cur);
+                hit = wt_intersect_rotated(w, rtri_vertex(tf, 0, w.rot), rtri_vertex(tf, 1, w.rot),
+                                           rtri_vertex(tf, 2, w.rot), t, U, V, W, det);
+                id = rtri_id(tf);
+            } else {
+                const gfloat* tf = tri_ptr(S.tris, cur);
+                hit = wt_intersect(w, v3(tf[0], tf[1], tf[2]), v3(tf[4], tf[5], tf[6]), v3(tf[8], tf[9], tf[10]), t, U,
+                                   V, W, det);
+                id = f_bits(tf[3]);
+            }
+            if (hit) {
                 if (ANY) {
                     res.t = t;
                     res.tri = id;
@@ -693,9 +730,15 @@ done:
 
 // traverse() without an LDS copy of the top nodes.
 template <bool ANY, int CAP>
+__device__ __forceinline__ bool traverse(const SceneDev& S, const TopCache& top, const Ray& ray, uint32_t* stk,
+                                         uint32_t stride, TraceResult& res) {
+    return traverse<ANY, CAP>(S, top, ray, wt_setup(ray), stk, stride, res);
+}
+
+template <bool ANY, int CAP>
 __device__ __forceinline__ bool traverse(const SceneDev& S, const Ray& ray, uint32_t* stk, uint32_t stride,
                                          TraceResult& res) {
-    return traverse<ANY, CAP>(S, TopCache{nullptr, 0u}, ray, stk, stride, res);
+    return traverse<ANY, CAP>(S, TopCache{nullptr, 0u}, ray, wt_setup(ray), stk, stride, res);
 }
 
 // Face::colorAt (scene/Mesh.cpp:478-485) at Embree's (u, v): the mesh's material colour (rgba;
