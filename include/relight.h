@@ -531,6 +531,10 @@ int32_t rl_server_destroy(rl_server* server);
 #define RL_DIRECT_KERNEL_AUTO 0
 #define RL_DIRECT_KERNEL_LUMEL 1
 #define RL_DIRECT_KERNEL_PAIRS 2
+/* local_trees (1, the default; or 0): a wavefront's AO rays start at a small tree built in LDS over
+   the BVH subtrees they can reach instead of at the scene's root. occluder_hints (1 or 0): AO and
+   shadow rays first test the triangles that blocked recent rays of their lumel (or lane). Both
+   only change which exact tests run and in which order, never a result. */
 typedef struct rl_tuning {
     int32_t ao_kernel;
     uint32_t refill_min;
@@ -538,6 +542,8 @@ typedef struct rl_tuning {
     uint32_t blocks_per_cu;
     int32_t fg_kernel;
     int32_t direct_kernel;
+    int32_t local_trees;
+    int32_t occluder_hints;
 } rl_tuning;
 int32_t rl_set_tuning(rl_relight* ctx, const rl_tuning* tuning);
 int32_t rl_get_tuning(const rl_relight* ctx, rl_tuning* out);

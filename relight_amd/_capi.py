@@ -157,7 +157,8 @@ RL_DIRECT_KERNEL_PAIRS = 2
 
 class rl_tuning(C.Structure):
     _fields_ = [("ao_kernel", C.c_int32), ("refill_min", C.c_uint32), ("ao_chunk_lumels", C.c_uint32),
-                ("blocks_per_cu", C.c_uint32), ("fg_kernel", C.c_int32), ("direct_kernel", C.c_int32)]
+                ("blocks_per_cu", C.c_uint32), ("fg_kernel", C.c_int32), ("direct_kernel", C.c_int32),
+                ("local_trees", C.c_int32), ("occluder_hints", C.c_int32)]
 
 
 class rl_solve_info(C.Structure):

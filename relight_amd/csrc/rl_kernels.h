@@ -27,15 +27,19 @@ hipError_t launch_scatter_colors(hipStream_t st, const uint32_t* sel, uint32_t n
 // sample) pairs (AO chunked) or chunks of chunk_lumels lumels (AO refill).
 // alpha: some segment's scene has a material whose alpha can be <= 0.1, so shadow rays take the
 // closest-hit alpha re-trace (HitUseAlpha) instead of any-hit.
+// shortcuts: kShortcut* bits, exact shortcuts of the any-hit kernels (local trees for AO rays,
+// occluder hints for AO and shadow rays); results do not depend on them.
+constexpr uint32_t kShortcutTrees = 1u, kShortcutHints = 2u;
 hipError_t launch_bake_direct(hipStream_t st, int cap, uint32_t max_blocks, const BakeSeg* segs,
-                              int nsegs, uint64_t total, bool alpha, unsigned long long* queue,
+                              int nsegs, uint64_t total, bool alpha, uint32_t shortcuts, unsigned long long* queue,
                               unsigned long long* rays);
 // Direct light of small passes over (lumel, shadow ray) pairs: influences into `inf` (one float per
 // pair, pairs counted from pair_segs' work_begin), then summed per lumel in the reference's light
 // order (lumel_segs: the same segments in lumel units). Results equal launch_bake_direct's.
 hipError_t launch_bake_direct_pairs(hipStream_t st, int cap, uint32_t max_blocks, const BakeSeg* pair_segs,
                                    const BakeSeg* lumel_segs, int nsegs, uint64_t total_pairs, uint64_t total_lumels,
-                                   bool alpha, float* inf, unsigned long long* queue, unsigned long long* rays);
+                                   bool alpha, uint32_t shortcuts, float* inf, unsigned long long* queue,
+                                   unsigned long long* rays);
 
 hipError_t launch_ao_count(hipStream_t st, int cap, uint32_t max_blocks, const BakeSeg* segs,
                            int nsegs, uint64_t total_pairs, unsigned long long* queue,
@@ -45,7 +49,7 @@ hipError_t launch_ao_count(hipStream_t st, int cap, uint32_t max_blocks, const B
 constexpr uint32_t kAoChunkMax = 16;
 hipError_t launch_ao_refill(hipStream_t st, int cap, uint32_t max_blocks, const BakeSeg* segs,
                             int nsegs, uint64_t total_chunks, uint32_t chunk_lumels,
-                            uint32_t refill_min, unsigned long long* queue,
+                            uint32_t refill_min, uint32_t shortcuts, unsigned long long* queue,
                             unsigned long long* rays);
 
 hipError_t launch_ao_finalize(hipStream_t st, const BakeSeg* segs, int nsegs, uint64_t total);

@@ -123,8 +123,10 @@ constexpr uint32_t kDirectTopNodes = 880;
 
 template <int CAP, bool ALPHA>
 __global__ __launch_bounds__(kDirectBlock) void k_bake_direct(const BakeSeg* __restrict__ segs, int nsegs,
-                                                              uint64_t total, unsigned long long* queue,
+                                                              uint64_t total, uint32_t shortcuts,
+                                                              unsigned long long* queue,
                                                               unsigned long long* rays_out) {
+    const bool use_hints = (shortcuts & kShortcutHints) != 0;
     __shared__ uint32_t stack[kLdsStack * kDirectBlock];
     __shared__ u32x4 top_rows[kDirectTopNodes * kNodeRows];
     const SceneDev top_scene = segs[0].scene;
@@ -154,6 +156,7 @@ __global__ __launch_bounds__(kDirectBlock) void k_bake_direct(const BakeSeg* __r
                 for (int v = 0; v < L.num_vertices; v++) {
                     const float4 lv = sg.lverts[L.vertex_offset + v];
                     V3 lp = v3(lv.x + L.pos[0], lv.y + L.pos[1], lv.z + L.pos[2]);
+                    if (!use_hints) hint = kNoHint;
                     float inf = influence_from_point<CAP, ALPHA>(S, top, L, lp, p, n, stk, rays, hint);
                     if (inf > 0.0f) {
                         cr += L.color[0] * L.intensity * inf;
@@ -165,6 +168,7 @@ __global__ __launch_bounds__(kDirectBlock) void k_bake_direct(const BakeSeg* __r
                 cr = cr / nv; cg = cg / nv; cb = cb / nv;
             } else {
                 // DirectLight::lightFromPoint (baker/DirectLight.cpp:63-72).
+                if (!use_hints) hint = kNoHint;
                 float inf = influence_from_point<CAP, ALPHA>(S, top, L, v3(L.pos[0], L.pos[1], L.pos[2]), p, n, stk, rays,
                                                              hint);
                 if (inf > 0.0f) {
@@ -202,9 +206,10 @@ __device__ __forceinline__ uint32_t direct_rays(const BakeSeg& sg) {
 
 template <int CAP, bool ALPHA>
 __global__ __launch_bounds__(kDirectBlock) void k_direct_pairs(const BakeSeg* __restrict__ segs, int nsegs,
-                                                               uint64_t total, float* __restrict__ inf_out,
-                                                               unsigned long long* queue,
+                                                               uint64_t total, uint32_t shortcuts,
+                                                               float* __restrict__ inf_out, unsigned long long* queue,
                                                                unsigned long long* rays_out) {
+    const bool use_hints = (shortcuts & kShortcutHints) != 0;
     __shared__ uint32_t stack[kLdsStack * kDirectBlock];
     __shared__ u32x4 top_rows[kDirectTopNodes * kNodeRows];
     const SceneDev top_scene = segs[0].scene;
@@ -245,6 +250,7 @@ __global__ __launch_bounds__(kDirectBlock) void k_direct_pairs(const BakeSeg* __
                 const float4 lv = sg.lverts[L.vertex_offset + v];
                 lp = v3(lv.x + L.pos[0], lv.y + L.pos[1], lv.z + L.pos[2]);
             }
+            if (!use_hints) hint = kNoHint;
             inf = influence_from_point<CAP, ALPHA>(S, top, L, lp, p, n, stk, rays, hint);
         }
         inf_out[g] = inf;
@@ -402,8 +408,9 @@ __device__ __forceinline__ QueryBox ao_segment_box(V3 p, const CosineFrame& f, f
 template <int CAP>
 __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restrict__ segs, int nsegs,
                                                       uint64_t total_chunks, uint32_t chunk_lumels,
-                                                      uint32_t refill_min, unsigned long long* queue,
-                                                      unsigned long long* rays_out) {
+                                                      uint32_t refill_min, uint32_t shortcuts,
+                                                      unsigned long long* queue, unsigned long long* rays_out) {
+    const bool use_trees = (shortcuts & kShortcutTrees) != 0, use_hints = (shortcuts & kShortcutHints) != 0;
     __shared__ uint32_t stack[kLdsStack * kAoBlock];
     // Per wavefront and ring slot (one lumel in flight): occluded rays so far, rays still to
     // finish, and where the lumel's count goes. A lumel's first ray sets them up.
@@ -513,7 +520,7 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
                     const uint32_t buf = vt_seq % kVtBuffers;
                     const uint32_t buf_end = buf == 0 ? vt_end0 : (buf == 1 ? vt_end1 : vt_end2);
                     chunk_root = 0u;
-                    if (chunk_count * chunk_ns >= kVtMinRays && lowest >= buf_end) {
+                    if (use_trees && chunk_count * chunk_ns >= kVtMinRays && lowest >= buf_end) {
                         const uint32_t first = kAoTopNodes + (wave * kVtBuffers + buf) * kVtNodes;
                         const TopCache btop{top_rows, sg.scene.nodes == top_scene.nodes ? top_count : 0u};
                         chunk_root = build_local_tree(sg.scene, btop, q, top_rows + first * kNodeRows, first);
@@ -605,13 +612,13 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
             // it frees take new rays before the wavefront traverses. Hints only choose which
             // triangles are tested first: the occlusion result does not depend on them.
             bool by_hint = false;
-            if (fresh) {
-                fresh = false;
+            if (fresh && use_hints) {
                 if (hint0 < scene.num_tris) by_hint = hits_triangle(scene, tr, hint0);
                 if (!by_hint && hint1 < scene.num_tris) by_hint = hits_triangle(scene, tr, hint1);
                 if (!by_hint && hint2 < scene.num_tris) by_hint = hits_triangle(scene, tr, hint2);
                 if (by_hint) finish(true);
             }
+            fresh = false;
             const uint64_t hinted = __ballot(by_hint);
             rays += (uint32_t)__popcll(hinted);
             if (hinted != 0ull && pool_open && (uint32_t)__popcll(__ballot(!active)) >= refill_min) continue;
@@ -628,7 +635,7 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
                                                               hit)
                                : any_hit_run<CAP, false, true>(scene, top, tr, node, sp, stk, kAoBlock, spill,
                                                                min_lanes, hit);
-            if (st > 0) {
+            if (st > 0 && use_hints) {
                 // Move the occluder to the front of the lumel's hints (most recent first) while the
                 // row still holds this ray's lumel.
                 uint32_t* w = reinterpret_cast<uint32_t*>(lumel_rows[wave][my_stream % kAoChunkMax]);
@@ -770,7 +777,8 @@ hipError_t launch_fill_u32(hipStream_t st, uint32_t* p, uint64_t n, uint32_t v) 
 }
 
 hipError_t launch_bake_direct(hipStream_t st, int cap, uint32_t max_blocks, const BakeSeg* segs, int nsegs,
-                              uint64_t total, bool alpha, unsigned long long* queue, unsigned long long* rays) {
+                              uint64_t total, bool alpha, uint32_t shortcuts, unsigned long long* queue,
+                              unsigned long long* rays) {
     if (total == 0) return hipSuccess;
     hipError_t e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(queue), 2, 0u);
     if (e != hipSuccess) return e;
@@ -780,17 +788,18 @@ hipError_t launch_bake_direct(hipStream_t st, int cap, uint32_t max_blocks, cons
     const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(need, cap_blocks));
     if (alpha) {
         RL_DISPATCH_CAP(cap, hipLaunchKernelGGL((k_bake_direct<CAP, true>), dim3(grid), dim3(kDirectBlock), 0, st, segs,
-                                                nsegs, total, queue, rays));
+                                                nsegs, total, shortcuts, queue, rays));
     } else {
         RL_DISPATCH_CAP(cap, hipLaunchKernelGGL((k_bake_direct<CAP, false>), dim3(grid), dim3(kDirectBlock), 0, st, segs,
-                                                nsegs, total, queue, rays));
+                                                nsegs, total, shortcuts, queue, rays));
     }
     return hipGetLastError();
 }
 
 hipError_t launch_bake_direct_pairs(hipStream_t st, int cap, uint32_t max_blocks, const BakeSeg* pair_segs,
                                    const BakeSeg* lumel_segs, int nsegs, uint64_t total_pairs, uint64_t total_lumels,
-                                   bool alpha, float* inf, unsigned long long* queue, unsigned long long* rays) {
+                                   bool alpha, uint32_t shortcuts, float* inf, unsigned long long* queue,
+                                   unsigned long long* rays) {
     if (total_pairs == 0) return hipSuccess;
     hipError_t e = launch_fill_u32(st, reinterpret_cast<uint32_t*>(queue), 2, 0u);
     if (e != hipSuccess) return e;
@@ -799,10 +808,10 @@ hipError_t launch_bake_direct_pairs(hipStream_t st, int cap, uint32_t max_blocks
     const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(need, cap_blocks));
     if (alpha) {
         RL_DISPATCH_CAP(cap, hipLaunchKernelGGL((k_direct_pairs<CAP, true>), dim3(grid), dim3(kDirectBlock), 0, st,
-                                                pair_segs, nsegs, total_pairs, inf, queue, rays));
+                                                pair_segs, nsegs, total_pairs, shortcuts, inf, queue, rays));
     } else {
         RL_DISPATCH_CAP(cap, hipLaunchKernelGGL((k_direct_pairs<CAP, false>), dim3(grid), dim3(kDirectBlock), 0, st,
-                                                pair_segs, nsegs, total_pairs, inf, queue, rays));
+                                                pair_segs, nsegs, total_pairs, shortcuts, inf, queue, rays));
     }
     e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -823,7 +832,7 @@ hipError_t launch_ao_count(hipStream_t st, int cap, uint32_t max_blocks, const B
 }
 
 hipError_t launch_ao_refill(hipStream_t st, int cap, uint32_t max_blocks, const BakeSeg* segs, int nsegs,
-                            uint64_t total_chunks, uint32_t chunk_lumels, uint32_t refill_min,
+                            uint64_t total_chunks, uint32_t chunk_lumels, uint32_t refill_min, uint32_t shortcuts,
                             unsigned long long* queue, unsigned long long* rays) {
     if (total_chunks == 0) return hipSuccess;
     if (chunk_lumels == 0 || chunk_lumels > kAoChunkMax || refill_min == 0 || refill_min > 64)
@@ -835,7 +844,7 @@ hipError_t launch_ao_refill(hipStream_t st, int cap, uint32_t max_blocks, const 
     const uint32_t cap_blocks = std::max<uint32_t>(1u, max_blocks * (uint32_t)kBlock / (uint32_t)kAoBlock);
     const uint32_t grid = (uint32_t)std::min<uint64_t>(need, cap_blocks);
     RL_DISPATCH_CAP(cap, hipLaunchKernelGGL(k_ao_refill<CAP>, dim3(grid), dim3(kAoBlock), 0, st, segs, nsegs,
-                                            total_chunks, chunk_lumels, refill_min, queue, rays));
+                                            total_chunks, chunk_lumels, refill_min, shortcuts, queue, rays));
     return hipGetLastError();
 }
 

@@ -111,7 +111,7 @@ struct rl_relight {
     unsigned long long* d_queue = nullptr;  // persistent-kernel work counter
     uint32_t max_blocks = 4096;             // persistent grid: blocks_per_cu blocks per compute unit
     int cus = 256;
-    rl_tuning tuning{RL_AO_KERNEL_REFILL, 16u, 8u, 16u, RL_FG_KERNEL_AUTO, RL_DIRECT_KERNEL_AUTO};
+    rl_tuning tuning{RL_AO_KERNEL_REFILL, 16u, 8u, 16u, RL_FG_KERNEL_AUTO, RL_DIRECT_KERNEL_AUTO, 1, 1};
     rl_stats stats{};
     uint32_t next_lm_seed = 0;
     // Event pairs around each kernel launch of the current call: (slot, start, end).
@@ -555,6 +555,11 @@ int32_t upload_segments(rl_relight* ctx, int slot, const std::vector<BakeItem>& 
 // is full, the pair kernel grows with the rays.
 constexpr uint64_t kDirectPairsBelow = 98304;
 
+// The any-hit kernels' exact shortcuts the context's tuning enables (kShortcut* bits).
+uint32_t shortcuts(const rl_relight* ctx) {
+    return (ctx->tuning.local_trees ? kShortcutTrees : 0u) | (ctx->tuning.occluder_hints ? kShortcutHints : 0u);
+}
+
 // Whether a direct-light pass over `items` (`lumels` lumels) runs as pairs.
 bool direct_uses_pairs(const rl_relight* ctx, const std::vector<BakeItem>& items, uint64_t lumels) {
     uint32_t rays_per_lumel = 1;
@@ -589,10 +594,10 @@ int32_t run_direct_items(rl_relight* ctx, const std::vector<BakeItem>& items, bo
         e = kernel_begin(ctx, RL_KERNEL_DIRECT);
         if (e == hipSuccess && pairs)
             e = launch_bake_direct_pairs(ctx->stream, ds.cap, ctx->max_blocks, ps.d, ds.d, ds.n, ps.total, ds.total,
-                                         ds.alpha, ctx->direct_inf, ctx->d_queue, ctx->d_rays);
+                                         ds.alpha, shortcuts(ctx), ctx->direct_inf, ctx->d_queue, ctx->d_rays);
         if (e == hipSuccess && !pairs)
-            e = launch_bake_direct(ctx->stream, ds.cap, ctx->max_blocks, ds.d, ds.n, ds.total, ds.alpha, ctx->d_queue,
-                                   ctx->d_rays);
+            e = launch_bake_direct(ctx->stream, ds.cap, ctx->max_blocks, ds.d, ds.n, ds.total, ds.alpha, shortcuts(ctx),
+                                   ctx->d_queue, ctx->d_rays);
         if (e == hipSuccess) e = kernel_end(ctx);
     }
     ctx->stats.texels += ds.texels;
@@ -628,7 +633,7 @@ int32_t run_ao_items(rl_relight* ctx, const std::vector<BakeItem>& items, bool s
         e = kernel_begin(ctx, RL_KERNEL_AO_COUNT);
         if (e == hipSuccess && refill)
             e = launch_ao_refill(ctx->stream, work.cap, ctx->max_blocks, work.d, work.n, work.total, chunk,
-                                 ctx->tuning.refill_min, ctx->d_queue, ctx->d_rays);
+                                 ctx->tuning.refill_min, shortcuts(ctx), ctx->d_queue, ctx->d_rays);
         if (e == hipSuccess && !refill) {
             for (const BakeItem& it : items)
                 if (e == hipSuccess && it.count)
@@ -1677,14 +1682,15 @@ hipError_t pass_launches(rl_relight* ctx, const rl_pass& p, const DeviceSegs& di
     hipError_t e = hipSuccess;
     if (dir.total && pairs.total)
         e = launch_bake_direct_pairs(ctx->stream, dir.cap, ctx->max_blocks, p.d_segs[3], p.d_segs[0], dir.n,
-                                     pairs.total, dir.total, dir.alpha, p.d_inf, ctx->d_queue, ctx->d_rays);
+                                     pairs.total, dir.total, dir.alpha, shortcuts(ctx), p.d_inf, ctx->d_queue,
+                                     ctx->d_rays);
     else if (dir.total)
         e = launch_bake_direct(ctx->stream, dir.cap, ctx->max_blocks, p.d_segs[0], dir.n, dir.total, dir.alpha,
-                               ctx->d_queue, ctx->d_rays);
+                               shortcuts(ctx), ctx->d_queue, ctx->d_rays);
     if (e == hipSuccess && work.total) {
         if (refill) {
             e = launch_ao_refill(ctx->stream, work.cap, ctx->max_blocks, p.d_segs[1], work.n, work.total, chunk,
-                                 ctx->tuning.refill_min, ctx->d_queue, ctx->d_rays);
+                                 ctx->tuning.refill_min, shortcuts(ctx), ctx->d_queue, ctx->d_rays);
         } else {
             for (const BakeItem& it : ao_items)
                 if (e == hipSuccess && it.count) e = launch_fill_u32(ctx->stream, it.lm->d_counts + it.first, it.count, 0u);
@@ -2358,6 +2364,8 @@ int32_t rl_set_tuning(rl_relight* ctx, const rl_tuning* t) {
     if (t->direct_kernel != RL_DIRECT_KERNEL_AUTO && t->direct_kernel != RL_DIRECT_KERNEL_LUMEL &&
         t->direct_kernel != RL_DIRECT_KERNEL_PAIRS)
         return fail(RL_INVALID_ARGUMENT, "unknown direct-light kernel");
+    if ((t->local_trees != 0 && t->local_trees != 1) || (t->occluder_hints != 0 && t->occluder_hints != 1))
+        return fail(RL_INVALID_ARGUMENT, "local_trees and occluder_hints must be 0 or 1");
     ctx->tuning = *t;
     ctx->max_blocks = (uint32_t)ctx->cus * t->blocks_per_cu;
     return RL_SUCCESS;

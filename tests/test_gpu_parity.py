@@ -396,6 +396,37 @@ def test_ao_refill_equals_chunked_and_oracle(relight, samples, sampling):
         assert np.array_equal(outs[2][li].reshape(-1, 3)[texel], col)
 
 
+@pytest.mark.parametrize("sampling,direct_kernel", [("cosine", "lumel"), ("uniform", "lumel"), ("cosine", "pairs")])
+def test_any_hit_shortcuts_change_no_result(relight, sampling, direct_kernel):
+    """Local trees (AO rays start at a tree built in LDS over the subtrees their chunk can reach)
+    and occluder hints (AO and shadow rays first test recently blocking triangles) switched off and
+    on through rl_tuning: direct light and AO equal bit for bit in all four combinations, and equal
+    the oracle. 64 AO rays per lumel, so every 16-lumel chunk builds a tree."""
+    print(f"[gpu] any-hit shortcuts on/off, {sampling} AO, {direct_kernel} direct kernel", flush=True)
+    from relight_amd import RL_SAMPLE_COSINE, RL_SAMPLE_UNIFORM
+    from relight_amd._capi import RL_DIRECT_KERNEL_LUMEL, RL_DIRECT_KERNEL_PAIRS
+    spec = scenes.city(num_boxes=300, tiles=(2, 2), side_range=(48, 96))
+    ao = AmbientOcclusionSettings(64, 0.8, 4.0, 1.0, 3,
+                                  RL_SAMPLE_COSINE if sampling == "cosine" else RL_SAMPLE_UNIFORM)
+    dk = RL_DIRECT_KERNEL_LUMEL if direct_kernel == "lumel" else RL_DIRECT_KERNEL_PAIRS
+    saved = relight.tuning()
+    outs = []
+    try:
+        for trees, hints in [(0, 0), (1, 0), (0, 1), (1, 1)]:
+            relight.set_tuning(ao_kernel=1, refill_min=16, ao_chunk_lumels=16, direct_kernel=dk,
+                               local_trees=trees, occluder_hints=hints)
+            scene, lms, ids = scenes.build_gpu(relight, spec)
+            relight.bake_batch(scene, lms, direct=True, ao=ao)
+            outs.append([lm.to_hdr() for lm in lms])
+    finally:
+        relight.set_tuning(**saved)
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            assert np.array_equal(a, b)
+    texel, _, _, col, _ = oracle_bake(spec, 1, True, ao)
+    assert np.array_equal(outs[3][1].reshape(-1, 3)[texel], col)
+
+
 def _spot_spec():
     """Box room (photons + final gather) lit by two spot lights: exponent 1 (createSpotLight) and
     exponent 2 (the powf branch of LightSpotCutoff, scene/Light.cpp:452-477)."""
