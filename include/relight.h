@@ -516,7 +516,7 @@ int32_t rl_server_destroy(rl_server* server);
 /* Launch tuning of a context (no counterpart in the reference; chosen by the autotuner in
  * relight_amd/autotune.py). ao_kernel: RL_AO_KERNEL_REFILL (lanes take new rays from a per-
  * wavefront pool when at least refill_min of them are idle; chunks of ao_chunk_lumels lumels, at
- * most 16) or
+ * most 16, halved down to 4 for passes with fewer chunks than the kernel has resident wavefronts) or
  * RL_AO_KERNEL_CHUNKED (64 rays per wavefront per chunk). blocks_per_cu sizes the persistent
  * grids (128-thread blocks). Results do not depend on the tuning. */
 #define RL_AO_KERNEL_CHUNKED 0

@@ -46,7 +46,11 @@ hipError_t launch_ao_count(hipStream_t st, int cap, uint32_t max_blocks, const B
                            unsigned long long* rays);
 
 // AO with lane refill (counts written, not accumulated); chunks of at most kAoChunkMax lumels.
+// The kernel runs one workgroup of kAoRefillWavesPerCu wavefronts per compute unit; passes with
+// fewer chunks than that many wavefronts on the device use smaller chunks, down to kAoSmallChunk.
 constexpr uint32_t kAoChunkMax = 16;
+constexpr uint32_t kAoRefillWavesPerCu = 16;
+constexpr uint32_t kAoSmallChunk = 4;
 hipError_t launch_ao_refill(hipStream_t st, int cap, uint32_t max_blocks, const BakeSeg* segs,
                             int nsegs, uint64_t total_chunks, uint32_t chunk_lumels,
                             uint32_t refill_min, uint32_t shortcuts, unsigned long long* queue,

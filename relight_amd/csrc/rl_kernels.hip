@@ -371,6 +371,7 @@ constexpr uint32_t kAoNodeExit = 8;
 // sixteen and 400/600. With local trees, 768-thread workgroups (no register spills, 560 nodes)
 // measured 256.6 ms against 237.8 ms for sixteen wavefronts.
 constexpr int kAoBlock = 1024;
+static_assert(kAoBlock / 64 == (int)kAoRefillWavesPerCu, "the launcher's chunk choice counts these wavefronts");
 constexpr uint32_t kAoTopNodes = 296;
 // Each wavefront builds a local tree (rl_traverse.h) for a chunk of at least kVtMinRays rays, over
 // the union of the boxes its lumels' AO segments stay in, and starts the chunk's rays at its

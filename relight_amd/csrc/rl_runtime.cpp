@@ -610,12 +610,23 @@ int32_t run_direct_items(rl_relight* ctx, const std::vector<BakeItem>& items, bo
     return RL_SUCCESS;
 }
 
-// Lumels per AO work chunk: the refill kernel's chunks hold at most kAoChunkMax lumels.
-uint32_t ao_chunk(const rl_relight* ctx, bool refill) {
-    // Chunks smaller than the tuned size for passes with fewer lumels than two chunks per resident
-    // wavefront measured no change (8.5K-, 19K- and 48K-lumel 64-sample passes within 4%): the
-    // slowest wavefront's rounds set a small pass's time, not idle wavefronts.
-    return refill ? std::min(ctx->tuning.ao_chunk_lumels, kAoChunkMax) : ctx->tuning.ao_chunk_lumels;
+// Lumels per AO work chunk: the refill kernel's chunks hold at most kAoChunkMax lumels. A pass with
+// fewer chunks than the refill kernel has resident wavefronts (kAoRefillWavesPerCu per compute
+// unit) halves the chunk, down to kAoSmallChunk lumels, so more wavefronts share it: a 13,120-lumel
+// city pass at 64 samples takes 0.184 ms with 8-lumel chunks and 0.154 ms with 4 (2: 0.197 ms, the
+// local-tree builds per chunk then cost more than the wavefronts gain); an 80,583-lumel pass,
+// already more chunks than wavefronts at 8, takes 0.410 ms with 8 and 0.436 ms with 4.
+uint32_t ao_chunk(const rl_relight* ctx, bool refill, const std::vector<BakeItem>& items) {
+    if (!refill) return ctx->tuning.ao_chunk_lumels;
+    uint32_t c = std::min(ctx->tuning.ao_chunk_lumels, kAoChunkMax);
+    uint64_t chunks = 0;
+    const uint64_t waves = (uint64_t)ctx->cus * kAoRefillWavesPerCu;
+    for (;;) {
+        chunks = 0;
+        for (const BakeItem& it : items) chunks += ((uint64_t)it.count + c - 1) / c;
+        if (c / 2 < kAoSmallChunk || chunks >= waves) return c;
+        c /= 2;
+    }
 }
 
 // AO over items: count pass (lane refill when every item has >= 4 samples, else chunked) and
@@ -623,7 +634,7 @@ uint32_t ao_chunk(const rl_relight* ctx, bool refill) {
 int32_t run_ao_items(rl_relight* ctx, const std::vector<BakeItem>& items, bool sync = true) {
     bool refill = ctx->tuning.ao_kernel == RL_AO_KERNEL_REFILL;
     for (const BakeItem& it : items) refill = refill && it.ao.samples >= 4;
-    const uint32_t chunk = ao_chunk(ctx, refill);
+    const uint32_t chunk = ao_chunk(ctx, refill, items);
     DeviceSegs work, lum;
     int32_t st = upload_segments(ctx, kSegAoWork, items, refill ? kUnitChunk : kUnitPair, chunk, work);
     if (st) return st;
@@ -1728,7 +1739,7 @@ int32_t rl_pass_create(rl_relight* ctx, rl_scene* scene, rl_lightmap* const* map
     // The same kernel choice as a direct bake_batch call, made once at recording.
     bool refill = ctx->tuning.ao_kernel == RL_AO_KERNEL_REFILL;
     for (const BakeItem& it : items) refill = refill && it.ao.samples >= 4;
-    const uint32_t chunk = ao_chunk(ctx, refill);
+    const uint32_t chunk = ao_chunk(ctx, refill, items);
     DeviceSegs dir, pairs, work, lum;
     if (do_direct && (st = pass_upload(build_segments(items, kUnitLumel, 1, dir), &p->d_segs[0]))) return st;
     if (do_direct && direct_uses_pairs(ctx, items, dir.total)) {

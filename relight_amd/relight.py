@@ -528,8 +528,9 @@ class Relight:
 
     def set_tuning(self, **kw):
         """Launch tuning (ao_kernel, refill_min, ao_chunk_lumels, blocks_per_cu, fg_kernel,
-        direct_kernel); results
-        do not depend on it. The refill AO kernel uses chunks of at most 16 lumels."""
+        direct_kernel, local_trees, occluder_hints); results do not depend on it. The refill AO
+        kernel uses chunks of at most 16 lumels, halved down to 4 for passes with fewer chunks
+        than it has resident wavefronts."""
         cur = self.tuning()
         cur.update(kw)
         t = A.rl_tuning(*[int(cur[name]) for name, _ in A.rl_tuning._fields_])
