@@ -468,6 +468,10 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
             uint32_t oldest = active ? my_stream : 0xFFFFFFFFu;
 #pragma unroll
             for (int off = 32; off >= 1; off >>= 1) oldest = min(oldest, (uint32_t)__shfl_xor((int)oldest, off, 64));
+            // Every lane holds the minimum now. Said so, the pool arithmetic below (the ring window,
+            // room, take, the pool counters) runs in scalar registers and branches on the sample
+            // count are scalar branches instead of selects over both paths.
+            oldest = wave_uniform32(oldest);
             const uint64_t allow = oldest == 0xFFFFFFFFu ? ~0ull : (uint64_t)oldest + (uint64_t)(kRing - 2);
             // The lowest stream in flight: the oldest active ray's, or the first one this refill
             // hands out (streams are handed out in increasing order).
@@ -537,27 +541,28 @@ __global__ __launch_bounds__(kAoBlock) void k_ao_refill(const BakeSeg* __restric
                     const uint64_t lim = allow > stream_base ? (allow - stream_base) * (uint64_t)chunk_ns : 0ull;
                     room = lim > pool_next ? min(room, lim - pool_next) : 0ull;
                 }
-                const uint32_t take = (uint32_t)min(room, (uint64_t)__popcll(want));
+                const uint32_t take = wave_uniform32((uint32_t)min(room, (uint64_t)__popcll(want)));
                 if (take == 0) break;
                 lowest = min(lowest, stream_base + pool_lc);
                 const uint32_t rank = rank_below(want);
                 const bool mine = ((want >> lane) & 1ull) && rank < take;
+                // Pair pool_next + rank as (lumel, sample): with at least 64 samples per lumel it
+                // is at most one lumel past (pool_lc, pool_si). The branch is on a wave-uniform
+                // count and outside the lanes' own branch, so only one of the two ways runs.
+                const uint32_t t = pool_si + rank;
+                uint32_t lc, si;
+                if (chunk_ns >= 64u) {
+                    const bool wrap = t >= chunk_ns;
+                    lc = pool_lc + (wrap ? 1u : 0u);
+                    si = wrap ? t - chunk_ns : t;
+                } else {
+                    lc = pool_lc + t / chunk_ns;
+                    si = t % chunk_ns;
+                }
                 if (mine) {
                     // The segment index is the same in every lane: read as such, the segment's
                     // fields load through the scalar path.
                     const BakeSeg& sg = segs[__builtin_amdgcn_readfirstlane(pseg)];
-                    // Pair pool_next + rank as (lumel, sample): with at least 64 samples per lumel
-                    // it is at most one lumel past (pool_lc, pool_si).
-                    const uint32_t t = pool_si + rank;
-                    uint32_t lc, si;
-                    if (chunk_ns >= 64u) {
-                        const bool wrap = t >= chunk_ns;
-                        lc = pool_lc + (wrap ? 1u : 0u);
-                        si = wrap ? t - chunk_ns : t;
-                    } else {
-                        lc = pool_lc + t / chunk_ns;
-                        si = t % chunk_ns;
-                    }
                     const uint32_t li = chunk_first + lc;
                     my_stream = stream_base + lc;
                     const u32x4* row = lumel_rows[wave][my_stream % kAoChunkMax];

@@ -30,6 +30,10 @@ __device__ __forceinline__ uint64_t wave_uniform(uint64_t v) {
     return ((uint64_t)hi << 32) | lo;
 }
 
+// The same for a 32-bit value (a cross-lane reduction's result, for instance, which the compiler
+// cannot tell is uniform: values computed from it would otherwise live in vector registers).
+__device__ __forceinline__ uint32_t wave_uniform32(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
 
 // Global-memory views of the scene arrays. A pointer read from a struct is generic to the
 // compiler, and a generic (flat) load also counts against the LDS wait counter, so every wait for
