@@ -514,8 +514,10 @@ __device__ __forceinline__ int any_hit_run(const SceneDev& S0, const TopCache& t
                                            int& sp, uint32_t* stk, uint32_t stride, uint32_t* spill,
                                            uint32_t min_lanes, uint32_t& hit_tri) {
     const SceneDev S = UNI ? uniform_scene(S0) : S0;
+    // min_lanes is the same in every lane; said so, the loop's exit test is a scalar compare.
+    const uint32_t exit_lanes = wave_uniform32(min_lanes);
     bool ended = false;
-    for (uint32_t stop_at = 0;; stop_at = min_lanes) {
+    for (uint32_t stop_at = 0;; stop_at = exit_lanes) {
         const bool searching = !ended && !(node & RL_LEAF_BIT);
         if ((uint32_t)__popcll(__ballot(searching)) <= stop_at) break;
         if (searching && !any_hit_node<CAP, VT>(S, top, r, node, sp, stk, stride, spill)) ended = true;
