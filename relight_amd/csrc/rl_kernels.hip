@@ -138,12 +138,14 @@ __global__ __launch_bounds__(kDirectBlock) void k_bake_direct(const BakeSeg* __r
     for (;;) {
         const uint64_t chunk = wave_next_chunk(queue);
         if (chunk >= nchunks) break;
+        // Segments start at multiples of 64 lumels, so the chunk's lanes share one segment: its
+        // scene, lights and light vertices are read into scalar registers.
+        const BakeSeg& sg = segs[find_seg(segs, nsegs, chunk * 64)];
         const uint64_t g = chunk * 64 + lane_id();
-        if (g >= total) continue;
-        const BakeSeg& sg = segs[find_seg(segs, nsegs, g)];
+        const uint32_t i = (uint32_t)(g - sg.work_begin);
+        if (g >= total || i >= sg.count) continue;  // past the end, or the gap before the next segment
         const SceneDev S = sg.scene;
         const TopCache top{top_rows, S.nodes == top_scene.nodes ? top_count : 0u};
-        const uint32_t i = (uint32_t)(g - sg.work_begin);
         const float4 p4 = sg.pos[i], n4 = sg.nrm[i];
         const V3 p = v3(p4.x, p4.y, p4.z), n = v3(n4.x, n4.y, n4.z);
         float4 col = sg.color[i];
@@ -266,6 +268,7 @@ __global__ __launch_bounds__(kBlock) void k_direct_sum(const BakeSeg* __restrict
         const int s = find_seg(lumel_segs, nsegs, g);
         const BakeSeg& sg = lumel_segs[s];
         const uint32_t i = (uint32_t)(g - sg.work_begin);
+        if (i >= sg.count) continue;  // the gap before the next segment (lumel tables aligned to 64)
         const float* f = inf + pair_segs[s].work_begin + (uint64_t)i * direct_rays(sg);
         float4 col = sg.color[i];
         uint32_t r = 0;

@@ -422,7 +422,10 @@ struct BakeItem {
     const PhotonMapDev* pms = nullptr;
 };
 
-enum SegUnit { kUnitLumel = 0, kUnitPair = 1, kUnitChunk = 2, kUnitDirectPair = 3 };
+// kUnitLumelAligned: lumels, each segment's range starting at a multiple of 64, so that a wavefront's
+// 64 consecutive work items lie in one segment (the direct-light lumel kernel reads the segment, its
+// scene and lights into scalar registers); the items in the gaps are skipped.
+enum SegUnit { kUnitLumel = 0, kUnitPair = 1, kUnitChunk = 2, kUnitDirectPair = 3, kUnitLumelAligned = 4 };
 // Segment-table slots: one per phase, so the phases of a pass can be queued back to back.
 enum SegSlot { kSegDirect = 0, kSegAoWork = 1, kSegAoLumel = 2, kSegGather = 3, kSegDirectPairs = 4 };
 
@@ -492,6 +495,7 @@ std::vector<BakeSeg> build_segments(const std::vector<BakeItem>& items, int unit
         s.counts = lm->d_counts + it.first;
         s.count = it.count;
         s.lm_seed = lm->seed;
+        if (unit == kUnitLumelAligned) acc = (acc + 63u) & ~uint64_t(63);
         s.work_begin = acc;
         s.scene = it.scene->dev();
         s.lights = it.scene->d_lights;
@@ -502,7 +506,7 @@ std::vector<BakeSeg> build_segments(const std::vector<BakeItem>& items, int unit
         s.mesh_pm = it.mesh_pm;
         s.pms = it.pms;
         segs.push_back(s);
-        if (unit == kUnitLumel) acc += it.count;
+        if (unit == kUnitLumel || unit == kUnitLumelAligned) acc += it.count;
         else if (unit == kUnitPair) acc += (uint64_t)it.count * (uint64_t)it.ao.samples;
         else if (unit == kUnitDirectPair) acc += (uint64_t)it.count * direct_rays_per_lumel(it.scene);
         else acc += (it.count + chunk_lumels - 1) / chunk_lumels;
@@ -573,11 +577,11 @@ bool direct_uses_pairs(const rl_relight* ctx, const std::vector<BakeItem>& items
 // queued on the stream (a later synchronisation, e.g. end_call's, reports its errors).
 int32_t run_direct_items(rl_relight* ctx, const std::vector<BakeItem>& items, bool sync = true) {
     DeviceSegs ds;
-    int32_t st = upload_segments(ctx, kSegDirect, items, kUnitLumel, 1, ds);
+    int32_t st = upload_segments(ctx, kSegDirect, items, kUnitLumelAligned, 1, ds);
     if (st) return st;
     // A lane per lumel traces the lumel's shadow rays one after another; a pass with fewer
     // lumels than kDirectPairsBelow takes the pair kernels, which spread them over lanes.
-    const bool pairs = direct_uses_pairs(ctx, items, ds.total);
+    const bool pairs = direct_uses_pairs(ctx, items, ds.texels);
     DeviceSegs ps;
     if (pairs) {
         if ((st = upload_segments(ctx, kSegDirectPairs, items, kUnitDirectPair, 1, ps))) return st;
@@ -1741,8 +1745,9 @@ int32_t rl_pass_create(rl_relight* ctx, rl_scene* scene, rl_lightmap* const* map
     for (const BakeItem& it : items) refill = refill && it.ao.samples >= 4;
     const uint32_t chunk = ao_chunk(ctx, refill, items);
     DeviceSegs dir, pairs, work, lum;
-    if (do_direct && (st = pass_upload(build_segments(items, kUnitLumel, 1, dir), &p->d_segs[0]))) return st;
-    if (do_direct && direct_uses_pairs(ctx, items, dir.total)) {
+    if (do_direct && (st = pass_upload(build_segments(items, kUnitLumelAligned, 1, dir), &p->d_segs[0])))
+        return st;
+    if (do_direct && direct_uses_pairs(ctx, items, dir.texels)) {
         if ((st = pass_upload(build_segments(items, kUnitDirectPair, 1, pairs), &p->d_segs[3]))) return st;
         RL_HIP(dev_alloc(&p->d_inf, (size_t)pairs.total));
     }
