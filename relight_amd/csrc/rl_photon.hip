@@ -231,7 +231,9 @@ __global__ void k_photon_gather(const PhotonMapDev* __restrict__ pms, int npm, u
 // Final-gather workgroups: sixteen wavefronts, one workgroup per compute unit (the occupancy the
 // registers allow), keeping the first nodes of the first segment's scene in LDS (kFgTopNodes for
 // the lane-per-lumel kernel: 88 KB, with the stacks 152 KB). Heightfield final gather: 289 ms with
-// four-wavefront workgroups and 184 nodes, 282 ms with sixteen and 700.
+// four-wavefront workgroups and 184 nodes, 282 ms with sixteen and 700; later, against 254 ms for
+// sixteen wavefronts and 700 nodes, twelve (768 threads, 860 nodes) measured 294.7 ms and eight (512
+// threads, 990 nodes) 383.2 ms: more cached nodes do not make up for fewer wavefronts.
 constexpr int kFgBlock = 1024;
 constexpr uint32_t kFgTopNodes = 700;
 // A closest-hit round's node phase stops once no more than this many lanes are still searching
