@@ -105,7 +105,9 @@ __device__ __forceinline__ void cswap(float& ka, uint32_t& va, float& kb, uint32
 }
 
 // Entries of a lane's traversal stack kept in LDS; deeper entries (rare) spill to a private
-// per-lane array. Blocks allocate kLdsStack words of LDS per thread.
+// per-lane array. Blocks allocate kLdsStack words of LDS per thread. 12 entries (the final gather
+// caching 828 top nodes instead of 700) measured city AO 206.5 ms and heightfield final gather
+// 252.1 ms against 205.0 and 254.2 ms at 16; 8 entries (956 nodes) 210.9 and 254.7 ms.
 constexpr int kLdsStack = 16;
 
 // Number of set bits of `mask` below this lane (a wavefront-local prefix count).
