@@ -12,7 +12,7 @@ from .relight import AmbientOcclusionSettings
 
 DEFAULT_GRID = {
     "ao_kernel": [RL_AO_KERNEL_REFILL],
-    "refill_min": [8, 16, 24, 32],
+    "refill_min": [8, 16, 20, 24, 32],
     "ao_chunk_lumels": [4, 16],
     "blocks_per_cu": [8, 16],
 }

@@ -111,7 +111,7 @@ struct rl_relight {
     unsigned long long* d_queue = nullptr;  // persistent-kernel work counter
     uint32_t max_blocks = 4096;             // persistent grid: blocks_per_cu blocks per compute unit
     int cus = 256;
-    rl_tuning tuning{RL_AO_KERNEL_REFILL, 16u, 8u, 16u, RL_FG_KERNEL_AUTO, RL_DIRECT_KERNEL_AUTO, 1, 1};
+    rl_tuning tuning{RL_AO_KERNEL_REFILL, 20u, 8u, 16u, RL_FG_KERNEL_AUTO, RL_DIRECT_KERNEL_AUTO, 1, 1};
     rl_stats stats{};
     uint32_t next_lm_seed = 0;
     // Event pairs around each kernel launch of the current call: (slot, start, end).
