@@ -180,7 +180,8 @@ def main():
     if not args.no_autotune:
         from relight_amd.autotune import autotune
         t0 = time.time()
-        tuning, tune_results = autotune(relight, scene, lms, spec.ao)
+        # Probe passes at the timed pass's own sample count (the autotuner's default probes at 64).
+        tuning, tune_results = autotune(relight, scene, lms, spec.ao, probe_samples=int(spec.ao.samples))
         log(f"autotune ({time.time()-t0:.1f} s): {tuning}; "
             + ", ".join(f"{r[1]:.1f}" for r in tune_results))
     # Check pass (also the first warmup): one batched pass from zero, kept for the parity check.
